@@ -1,0 +1,1117 @@
+/*
+ * CPU restatement of the reference 3D fractional 4-step LES time step -- TEST INFRASTRUCTURE.
+ *
+ * This file is the oracle: a plain-C, single-threaded restatement of the reference solver in
+ * LES/v2.3_LES-Reynolds stresses/ (cited below as RLES/), with one MPI rank (the reference's
+ * decomposition collapsed), in the reference's loop order and expression association, so that
+ * it reproduces the reference's arithmetic.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it, and only as the checker / the timed CPU baseline;
+ * the product (fvm_amd) never links or calls it.
+ *
+ * Pinning: every routine here is checked against the flang build of the reference itself
+ * (oracle/ref/build_ref.sh + oracle/ref/harness3d.f90, golden dumps under tests/golden/) and
+ * against closed forms (tests/test_oracle3d.py).  See DESIGN.md for the per-function status.
+ *
+ * Layout: every cell array is (NI+1)x(NJ+1)x(NK+1) doubles, Fortran order (I fastest), holding
+ * indices 0..NI, 0..NJ, 0..NK exactly like the reference's (IS:IE,JS:JE,KS:KE) arrays with one
+ * rank.  Node arrays XG/YG/ZG use the same shape and hold nodes 1..NI, 1..NJ, 1..NK.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+typedef struct orc3d {
+  int ni, nj, nk;
+  long sj, sk, n;
+  double re, dtime, ref_a, pi;
+  /* geometry */
+  double *xg, *yg, *zg, *x, *y, *z;
+  double *djac, *xix, *xiy, *xiz, *etx, *ety, *etz, *ztx, *zty, *ztz;
+  double *q11, *q12, *q13, *q22, *q23, *q33;
+  double *de, *dw, *dn, *ds, *dt, *db;
+  double *theta, *area1, *area2;
+  /* fields */
+  double *u, *v, *w, *p, *un, *vn, *wn;
+  double *u1, *u2, *u3, *ue, *uw, *vnf, *vs, *wt, *wb;
+  double *nlx, *nly, *nlz, *nlx1, *nly1, *nlz1;
+  double *difx, *dify, *difz, *s_u, *s_v, *s_w;
+  double *ap, *ae, *aw, *an, *as, *at, *ab, *su, *dd;
+  /* LES */
+  double *filt, *lss[6], *mag, *uh, *vh, *wh, *magh, *dumin, *dumout, *lij, *mij, *lm, *mm;
+  double *nut, *sgsx, *sgsy, *sgsz;
+  double c_avg;
+  /* scalars of the last step */
+  double uin, resip, pref, cfl, cpb, cd, cl, cdp, cdv, clp, clv;
+  int niter;
+} orc3d;
+
+#define IX(o, i, j, k) ((long)(i) + (o)->sj * (long)(j) + (o)->sk * (long)(k))
+
+static double *alloc0(long n) { return (double *)calloc((size_t)n, sizeof(double)); }
+
+orc3d *orc3d_create(int ni, int nj, int nk, double re, double dtime, double ref_a) {
+  orc3d *o = (orc3d *)calloc(1, sizeof(orc3d));
+  o->ni = ni; o->nj = nj; o->nk = nk;
+  o->sj = ni + 1; o->sk = (long)(ni + 1) * (nj + 1); o->n = o->sk * (nk + 1);
+  o->re = re; o->dtime = dtime; o->ref_a = ref_a; o->pi = 4.0 * atan(1.0);
+  double **all[] = {&o->xg, &o->yg, &o->zg, &o->x, &o->y, &o->z, &o->djac, &o->xix, &o->xiy,
+                    &o->xiz, &o->etx, &o->ety, &o->etz, &o->ztx, &o->zty, &o->ztz, &o->q11,
+                    &o->q12, &o->q13, &o->q22, &o->q23, &o->q33, &o->de, &o->dw, &o->dn,
+                    &o->ds, &o->dt, &o->db, &o->theta, &o->area1, &o->area2, &o->u, &o->v,
+                    &o->w, &o->p, &o->un, &o->vn, &o->wn, &o->u1, &o->u2, &o->u3, &o->ue,
+                    &o->uw, &o->vnf, &o->vs, &o->wt, &o->wb, &o->nlx, &o->nly, &o->nlz,
+                    &o->nlx1, &o->nly1, &o->nlz1, &o->difx, &o->dify, &o->difz, &o->s_u,
+                    &o->s_v, &o->s_w, &o->ap, &o->ae, &o->aw, &o->an, &o->as, &o->at, &o->ab,
+                    &o->su, &o->dd, &o->filt, &o->lss[0], &o->lss[1], &o->lss[2], &o->lss[3],
+                    &o->lss[4], &o->lss[5], &o->mag, &o->uh, &o->vh, &o->wh, &o->magh,
+                    &o->dumin, &o->dumout, &o->lij, &o->mij, &o->lm, &o->mm, &o->nut,
+                    &o->sgsx, &o->sgsy, &o->sgsz};
+  for (size_t a = 0; a < sizeof(all) / sizeof(all[0]); ++a) *all[a] = alloc0(o->n);
+  return o;
+}
+
+void orc3d_destroy(orc3d *o) {
+  if (!o) return;
+  double **p = &o->xg;
+  /* every double* member from xg to sgsz is an allocation */
+  double **end = &o->sgsz;
+  for (; p <= end; ++p) free(*p);
+  free(o);
+}
+
+/* Named access for the Python test harness. */
+double *orc3d_array(orc3d *o, const char *name) {
+#define N(s, f) if (!strcmp(name, s)) return o->f;
+  N("xg", xg) N("yg", yg) N("zg", zg) N("x", x) N("y", y) N("z", z) N("djac", djac)
+  N("xix", xix) N("xiy", xiy) N("xiz", xiz) N("etx", etx) N("ety", ety) N("etz", etz)
+  N("ztx", ztx) N("zty", zty) N("ztz", ztz) N("q11", q11) N("q12", q12) N("q13", q13)
+  N("q22", q22) N("q23", q23) N("q33", q33) N("de", de) N("dw", dw) N("dn", dn) N("ds", ds)
+  N("dt", dt) N("db", db) N("theta", theta) N("area1", area1) N("area2", area2) N("u", u)
+  N("v", v) N("w", w) N("p", p) N("un", un) N("vn", vn) N("wn", wn) N("ue", ue) N("uw", uw)
+  N("vnf", vnf) N("vs", vs) N("wt", wt) N("wb", wb) N("nlx", nlx) N("nly", nly) N("nlz", nlz)
+  N("nlx1", nlx1) N("nly1", nly1) N("nlz1", nlz1) N("difx", difx) N("dify", dify)
+  N("difz", difz) N("s_u", s_u) N("s_v", s_v) N("s_w", s_w) N("su", su) N("dd", dd)
+  N("ap", ap) N("filt", filt) N("mag", mag) N("uh", uh) N("vh", vh) N("wh", wh)
+  N("magh", magh) N("lm", lm) N("mm", mm) N("nut", nut) N("sgsx", sgsx) N("sgsy", sgsy)
+  N("sgsz", sgsz) N("u1", u1) N("u2", u2) N("u3", u3)
+#undef N
+  return NULL;
+}
+
+double orc3d_scalar(orc3d *o, const char *name) {
+  if (!strcmp(name, "resip")) return o->resip;
+  if (!strcmp(name, "niter")) return (double)o->niter;
+  if (!strcmp(name, "pref")) return o->pref;
+  if (!strcmp(name, "cfl")) return o->cfl;
+  if (!strcmp(name, "cpb")) return o->cpb;
+  if (!strcmp(name, "cd")) return o->cd;
+  if (!strcmp(name, "cl")) return o->cl;
+  if (!strcmp(name, "cdp")) return o->cdp;
+  if (!strcmp(name, "cdv")) return o->cdv;
+  if (!strcmp(name, "clp")) return o->clp;
+  if (!strcmp(name, "clv")) return o->clv;
+  if (!strcmp(name, "c_avg")) return o->c_avg;
+  return NAN;
+}
+
+/* MPI_X with one rank: periodic J ghosts (branch cut) for K=0..NK, then periodic K ghosts for
+ * J=0..NJ, whole I rows (RLES/mpi_x.f90:26-53). */
+void orc3d_mpi_x(orc3d *o, double *a) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  for (int k = 0; k <= nk; ++k)
+    for (int i = 0; i <= ni; ++i) {
+      a[IX(o, i, 0, k)] = a[IX(o, i, nj - 1, k)];
+      a[IX(o, i, nj, k)] = a[IX(o, i, 1, k)];
+    }
+  for (int j = 0; j <= nj; ++j)
+    for (int i = 0; i <= ni; ++i) {
+      a[IX(o, i, j, 0)] = a[IX(o, i, j, nk - 1)];
+      a[IX(o, i, j, nk)] = a[IX(o, i, j, 1)];
+    }
+}
+
+/* ----------------------------------------------------------------------------------------- */
+/* GRID: cell centres and ghost centres (RLES/grid.f90:69-346)                                 */
+/* ----------------------------------------------------------------------------------------- */
+static void centres(orc3d *o, const double *g, double *c) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+#define G(i, j, k) g[IX(o, i, j, k)]
+#define C(i, j, k) c[IX(o, i, j, k)]
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i)
+        C(i, j, k) = 0.125 * (G(i + 1, j + 1, k) + G(i + 1, j, k) + G(i, j + 1, k) + G(i, j, k) +
+                              G(i + 1, j + 1, k + 1) + G(i + 1, j, k + 1) + G(i, j + 1, k + 1) +
+                              G(i, j, k + 1));
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j) {
+      int i = 0;
+      C(i, j, k) = 0.25 * (G(i + 1, j + 1, k) + G(i + 1, j, k) + G(i + 1, j + 1, k + 1) +
+                           G(i + 1, j, k + 1));
+      C(i, j, k) = C(i, j, k) - (C(i + 1, j, k) - C(i, j, k));
+      i = ni;
+      C(i, j, k) = 0.25 * (G(i, j + 1, k) + G(i, j, k) + G(i, j + 1, k + 1) + G(i, j, k + 1));
+      C(i, j, k) = C(i, j, k) + (C(i, j, k) - C(i - 1, j, k));
+    }
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int i = 1; i <= ni - 1; ++i) {
+      int j = 0;
+      C(i, j, k) = 0.25 * (G(i + 1, j + 1, k + 1) + G(i + 1, j + 1, k) + G(i, j + 1, k + 1) +
+                           G(i, j + 1, k));
+      C(i, j, k) = C(i, j, k) - (C(i, j + 1, k) - C(i, j, k));
+      j = nj;
+      C(i, j, k) = 0.25 * (G(i + 1, j, k + 1) + G(i + 1, j, k) + G(i, j, k + 1) + G(i, j, k));
+      C(i, j, k) = C(i, j, k) - (C(i, j - 1, k) - C(i, j, k));
+    }
+  for (int j = 1; j <= nj - 1; ++j)
+    for (int i = 1; i <= ni - 1; ++i) {
+      int k = 0;
+      C(i, j, k) = 0.25 * (G(i, j, k + 1) + G(i + 1, j, k + 1) + G(i + 1, j + 1, k + 1) +
+                           G(i, j + 1, k + 1));
+      C(i, j, k) = C(i, j, k) - (C(i, j, k + 1) - C(i, j, k));
+      k = nk;
+      C(i, j, k) = 0.25 * (G(i, j, k) + G(i + 1, j, k) + G(i + 1, j + 1, k) + G(i, j + 1, k));
+      C(i, j, k) = C(i, j, k) + (C(i, j, k) - C(i, j, k - 1));
+    }
+  orc3d_mpi_x(o, c);
+#undef G
+#undef C
+}
+
+/* Z-directional periodic cell-centre fix-up (RLES/grid.f90:275-346). */
+static void zfix(orc3d *o) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+#define G(i, j, k) o->zg[IX(o, i, j, k)]
+#define Z(i, j, k) o->z[IX(o, i, j, k)]
+  for (int pass = 0; pass < 2; ++pass) {
+    int k = pass == 0 ? 0 : nk;
+    int kn = pass == 0 ? 1 : nk - 1;
+    for (int j = 0; j <= nj; ++j)
+      for (int i = 0; i <= ni; ++i) {
+        o->x[IX(o, i, j, k)] = o->x[IX(o, i, j, kn)];
+        o->y[IX(o, i, j, k)] = o->y[IX(o, i, j, kn)];
+      }
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        if (pass == 0) {
+          Z(i, j, k) = 0.25 * (G(i, j, k + 1) + G(i + 1, j, k + 1) + G(i + 1, j + 1, k + 1) +
+                               G(i, j + 1, k + 1));
+          Z(i, j, k) = Z(i, j, k) - (Z(i, j, k + 1) - Z(i, j, k));
+        } else {
+          Z(i, j, k) = 0.25 * (G(i, j, k) + G(i + 1, j, k) + G(i + 1, j + 1, k) + G(i, j + 1, k));
+          Z(i, j, k) = Z(i, j, k) + (Z(i, j, k) - Z(i, j, k - 1));
+        }
+      }
+    for (int j = 1; j <= nj - 1; ++j) {
+      Z(0, j, k) = Z(1, j, k);
+      Z(ni, j, k) = Z(ni - 1, j, k);
+    }
+    for (int i = 1; i <= ni - 1; ++i) {
+      Z(i, 0, k) = Z(i, 1, k);
+      Z(i, nj, k) = Z(i, nj - 1, k);
+    }
+    Z(0, 0, k) = Z(1, 1, k);
+    Z(ni, 0, k) = Z(ni - 1, 1, k);
+    Z(ni, nj, k) = Z(ni - 1, nj - 1, k);
+    Z(0, nj, k) = Z(1, nj - 1, k);
+  }
+#undef G
+#undef Z
+}
+
+/* ----------------------------------------------------------------------------------------- */
+/* METRIC (RLES/metric.f90:32-518).  The edge/corner one-sided formulas of the reference are  */
+/* overwritten by its MPI_X of the derivative arrays (metric.f90:400-410); the derivatives     */
+/* that survive are central inside, one-sided in I on the I=0 / I=NI planes (interior J,K).   */
+/* ----------------------------------------------------------------------------------------- */
+static void deriv(orc3d *o, const double *c, double *dxi, double *det, double *dzt) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+#define C(i, j, k) c[IX(o, i, j, k)]
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        dxi[IX(o, i, j, k)] = 0.5 * (C(i + 1, j, k) - C(i - 1, j, k));
+        det[IX(o, i, j, k)] = 0.5 * (C(i, j + 1, k) - C(i, j - 1, k));
+        dzt[IX(o, i, j, k)] = 0.5 * (C(i, j, k + 1) - C(i, j, k - 1));
+      }
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j) {
+      int i = 0;
+      dxi[IX(o, i, j, k)] = (-3.0 * C(i, j, k) + 4.0 * C(i + 1, j, k) - C(i + 2, j, k)) / 2.0;
+      det[IX(o, i, j, k)] = (C(i, j + 1, k) - C(i, j - 1, k)) / 2.0;
+      dzt[IX(o, i, j, k)] = (C(i, j, k + 1) - C(i, j, k - 1)) / 2.0;
+      i = ni;
+      dxi[IX(o, i, j, k)] = (3.0 * C(i, j, k) - 4.0 * C(i - 1, j, k) + C(i - 2, j, k)) / 2.0;
+      det[IX(o, i, j, k)] = (C(i, j + 1, k) - C(i, j - 1, k)) / 2.0;
+      dzt[IX(o, i, j, k)] = (C(i, j, k + 1) - C(i, j, k - 1)) / 2.0;
+    }
+  orc3d_mpi_x(o, dxi);
+  orc3d_mpi_x(o, det);
+  orc3d_mpi_x(o, dzt);
+#undef C
+}
+
+void orc3d_setup(orc3d *o, const double *xg, const double *yg, const double *zg) {
+  long n = o->n;
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  memcpy(o->xg, xg, n * sizeof(double));
+  memcpy(o->yg, yg, n * sizeof(double));
+  memcpy(o->zg, zg, n * sizeof(double));
+  centres(o, o->xg, o->x);
+  centres(o, o->yg, o->y);
+  centres(o, o->zg, o->z);
+  zfix(o);
+
+  double *xxi = alloc0(n), *xet = alloc0(n), *xzt = alloc0(n);
+  double *yxi = alloc0(n), *yet = alloc0(n), *yzt = alloc0(n);
+  double *zxi = alloc0(n), *zet = alloc0(n), *zzt = alloc0(n);
+  deriv(o, o->x, xxi, xet, xzt);
+  deriv(o, o->y, yxi, yet, yzt);
+  deriv(o, o->z, zxi, zet, zzt);
+  for (long c = 0; c < n; ++c) {
+    o->djac[c] = xxi[c] * (yet[c] * zzt[c] - yzt[c] * zet[c]) -
+                 yxi[c] * (xet[c] * zzt[c] - zet[c] * xzt[c]) +
+                 zxi[c] * (xet[c] * yzt[c] - xzt[c] * yet[c]);
+  }
+  for (long c = 0; c < n; ++c) {
+    double dj = o->djac[c];
+    o->xix[c] = (yet[c] * zzt[c] - yzt[c] * zet[c]) / dj;
+    o->xiy[c] = (zet[c] * xzt[c] - zzt[c] * xet[c]) / dj;
+    o->xiz[c] = (xet[c] * yzt[c] - xzt[c] * yet[c]) / dj;
+    o->etx[c] = (yzt[c] * zxi[c] - yxi[c] * zzt[c]) / dj;
+    o->ety[c] = (zzt[c] * xxi[c] - zxi[c] * xzt[c]) / dj;
+    o->etz[c] = (xzt[c] * yxi[c] - xxi[c] * yzt[c]) / dj;
+    o->ztx[c] = (yxi[c] * zet[c] - yet[c] * zxi[c]) / dj;
+    o->zty[c] = (zxi[c] * xet[c] - zet[c] * xxi[c]) / dj;
+    o->ztz[c] = (xxi[c] * yet[c] - xet[c] * yxi[c]) / dj;
+  }
+  free(xxi); free(xet); free(xzt); free(yxi); free(yet); free(yzt);
+  free(zxi); free(zet); free(zzt);
+  for (long c = 0; c < n; ++c) {
+    double a = o->xix[c], b = o->xiy[c], d = o->xiz[c];
+    double e = o->etx[c], f = o->ety[c], g = o->etz[c];
+    double h = o->ztx[c], q = o->zty[c], r = o->ztz[c];
+    o->q11[c] = a * a + b * b + d * d;
+    o->q12[c] = a * e + b * f + d * g;
+    o->q13[c] = h * a + q * b + r * d;
+    o->q22[c] = e * e + f * f + g * g;
+    o->q23[c] = e * h + f * q + g * r;
+    o->q33[c] = h * h + q * q + r * r;
+  }
+  double *J = o->djac;
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        o->de[c] = 0.5 * (J[c] * o->q11[c] + J[c + 1] * o->q11[c + 1]);
+        o->dw[c] = 0.5 * (J[c] * o->q11[c] + J[c - 1] * o->q11[c - 1]);
+        o->dn[c] = 0.5 * (J[c] * o->q22[c] + J[c + o->sj] * o->q22[c + o->sj]);
+        o->ds[c] = 0.5 * (J[c] * o->q22[c] + J[c - o->sj] * o->q22[c - o->sj]);
+        o->dt[c] = 0.5 * (J[c] * o->q33[c] + J[c + o->sk] * o->q33[c + o->sk]);
+        o->db[c] = 0.5 * (J[c] * o->q33[c] + J[c - o->sk] * o->q33[c - o->sk]);
+      }
+  /* THETA (RLES/init.f90:210-237) */
+  double pi = o->pi;
+  for (long c = 0; c < n; ++c) {
+    double X = o->x[c], Y = o->y[c];
+    double rad = sqrt(X * X + Y * Y);
+    double th = atan(Y / X);
+    double as = asin(Y / rad);
+    if (th >= 0.0 && as > 0.0) th = th;
+    else if (th < 0.0 && as > 0.0) th = pi + th;
+    else if (th >= 0.0 && as <= 0.0) th = pi + th;
+    else if (th < 0.0 && as < 0.0) th = 2.0 * pi + th;
+    o->theta[c] = th;
+  }
+  /* wall triangle areas (RLES/init.f90:240-259) */
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j) {
+      int i = 1;
+#define XG(i, j, k) o->xg[IX(o, i, j, k)]
+#define YG(i, j, k) o->yg[IX(o, i, j, k)]
+#define ZG(i, j, k) o->zg[IX(o, i, j, k)]
+#define SQ(a) ((a) * (a))
+      double l1 = sqrt(SQ(XG(i, j + 1, k) - XG(i, j, k)) + SQ(YG(i, j + 1, k) - YG(i, j, k)) +
+                       SQ(ZG(i, j + 1, k) - ZG(i, j, k)));
+      double l2 = sqrt(SQ(XG(i, j, k + 1) - XG(i, j, k)) + SQ(YG(i, j, k + 1) - YG(i, j, k)) +
+                       SQ(ZG(i, j, k + 1) - ZG(i, j, k)));
+      double l3 = sqrt(SQ(XG(i, j + 1, k) - XG(i, j, k + 1)) +
+                       SQ(YG(i, j + 1, k) - YG(i, j, k + 1)) +
+                       SQ(ZG(i, j + 1, k) - ZG(i, j, k + 1)));
+      double th1 = acos((l3 * l3 - l2 * l2 - l1 * l1) / (-2.0 * l1 * l2));
+      o->area1[IX(o, i, j, k)] = 0.5 * l1 * l2 * sin(th1);
+      l1 = sqrt(SQ(XG(i, j + 1, k + 1) - XG(i, j + 1, k)) +
+                SQ(YG(i, j + 1, k + 1) - YG(i, j + 1, k)) +
+                SQ(ZG(i, j + 1, k + 1) - ZG(i, j + 1, k)));
+      l2 = sqrt(SQ(XG(i, j + 1, k + 1) - XG(i, j, k + 1)) +
+                SQ(YG(i, j + 1, k + 1) - YG(i, j, k + 1)) +
+                SQ(ZG(i, j + 1, k + 1) - ZG(i, j, k + 1)));
+      th1 = acos((l3 * l3 - l2 * l2 - l1 * l1) / (-2.0 * l1 * l2));
+      o->area2[IX(o, i, j, k)] = 0.5 * l1 * l2 * sin(th1);
+    }
+  /* FILT_WID = DJAC**(1/3) (RLES/stress_tensor_dy.f90:24) */
+  for (long c = 0; c < n; ++c) o->filt[c] = pow(o->djac[c], 1.0 / 3.0);
+}
+
+void orc3d_set_state(orc3d *o, const double *u, const double *v, const double *w,
+                     const double *p, const double *nlx, const double *nly, const double *nlz) {
+  size_t b = o->n * sizeof(double);
+  memcpy(o->u, u, b); memcpy(o->v, v, b); memcpy(o->w, w, b); memcpy(o->p, p, b);
+  memcpy(o->nlx, nlx, b); memcpy(o->nly, nly, b); memcpy(o->nlz, nlz, b);
+}
+
+/* ----------------------------------------------------------------------------------------- */
+/* CONTRA_VEL (RLES/contra_vel.f90:13-56)                                                     */
+/* ----------------------------------------------------------------------------------------- */
+void orc3d_contra_vel(orc3d *o) {
+  long n = o->n;
+  for (long c = 0; c < n; ++c) {
+    o->u1[c] = o->xix[c] * o->u[c] + o->xiy[c] * o->v[c] + o->xiz[c] * o->w[c];
+    o->u2[c] = o->etx[c] * o->u[c] + o->ety[c] * o->v[c] + o->etz[c] * o->w[c];
+    o->u3[c] = o->ztx[c] * o->u[c] + o->zty[c] * o->v[c] + o->ztz[c] * o->w[c];
+  }
+  orc3d_mpi_x(o, o->u1);
+  orc3d_mpi_x(o, o->u2);
+  orc3d_mpi_x(o, o->u3);
+  double *J = o->djac;
+  long sj = o->sj, sk = o->sk;
+  for (int k = 1; k <= o->nk - 1; ++k)
+    for (int j = 1; j <= o->nj - 1; ++j)
+      for (int i = 1; i <= o->ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        o->ue[c] = 0.5 * (J[c] * o->u1[c] + J[c + 1] * o->u1[c + 1]);
+        o->uw[c] = 0.5 * (J[c] * o->u1[c] + J[c - 1] * o->u1[c - 1]);
+        o->vnf[c] = 0.5 * (J[c] * o->u2[c] + J[c + sj] * o->u2[c + sj]);
+        o->vs[c] = 0.5 * (J[c] * o->u2[c] + J[c - sj] * o->u2[c - sj]);
+        o->wt[c] = 0.5 * (J[c] * o->u3[c] + J[c + sk] * o->u3[c + sk]);
+        o->wb[c] = 0.5 * (J[c] * o->u3[c] + J[c - sk] * o->u3[c - sk]);
+      }
+  for (int k = 1; k <= o->nk - 1; ++k)
+    for (int j = 1; j <= o->nj - 1; ++j) o->uw[IX(o, 1, j, k)] = 0.0;
+}
+
+/* NL(IN,OUT) (RLES/nl.f90:23-31) */
+void orc3d_nl(orc3d *o, const double *in, double *out) {
+  long sj = o->sj, sk = o->sk;
+  for (int k = 1; k <= o->nk - 1; ++k)
+    for (int j = 1; j <= o->nj - 1; ++j)
+      for (int i = 1; i <= o->ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        double t = o->ue[c] * (in[c + 1] + in[c]);
+        t = t - o->uw[c] * (in[c - 1] + in[c]);
+        t = t + o->vnf[c] * (in[c + sj] + in[c]);
+        t = t - o->vs[c] * (in[c - sj] + in[c]);
+        t = t + o->wt[c] * (in[c + sk] + in[c]);
+        t = t - o->wb[c] * (in[c - sk] + in[c]);
+        out[c] = 0.5 * t;
+      }
+}
+
+/* DIF_DIAGONAL(INN,OUT) (RLES/dif_diagonal.f90:13-32): explicit cross-derivative terms. */
+void orc3d_dif_diagonal(orc3d *o, const double *f, double *out) {
+  long sj = o->sj, sk = o->sk;
+  const double *J = o->djac, *q12 = o->q12, *q13 = o->q13, *q23 = o->q23;
+  for (int k = 1; k <= o->nk - 1; ++k)
+    for (int j = 1; j <= o->nj - 1; ++j)
+      for (int i = 1; i <= o->ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        long e = c + 1, w = c - 1, nn = c + sj, s = c - sj, t = c + sk, b = c - sk;
+        double r = q12[e] * J[e] * (f[e + sj] - f[e - sj]);
+        r = r - q12[w] * J[w] * (f[w + sj] - f[w - sj]);
+        r = r + q13[e] * J[e] * (f[e + sk] - f[e - sk]);
+        r = r - q13[w] * J[w] * (f[w + sk] - f[w - sk]);
+        r = r + q12[nn] * J[nn] * (f[nn + 1] - f[nn - 1]);
+        r = r - q12[s] * J[s] * (f[s + 1] - f[s - 1]);
+        r = r + q23[nn] * J[nn] * (f[nn + sk] - f[nn - sk]);
+        r = r - q23[s] * J[s] * (f[s + sk] - f[s - sk]);
+        r = r + q13[t] * J[t] * (f[t + 1] - f[t - 1]);
+        r = r - q13[b] * J[b] * (f[b + 1] - f[b - 1]);
+        r = r + q23[t] * J[t] * (f[t + sj] - f[t - sj]);
+        r = r - q23[b] * J[b] * (f[b + sj] - f[b - sj]);
+        out[c] = 0.25 * r;
+      }
+}
+
+/* DIF(IN,OUT) (RLES/dif.f90:13-23) */
+void orc3d_dif(orc3d *o, const double *in, double *out) {
+  orc3d_dif_diagonal(o, in, o->dd);
+  long sj = o->sj, sk = o->sk;
+  for (int k = 1; k <= o->nk - 1; ++k)
+    for (int j = 1; j <= o->nj - 1; ++j)
+      for (int i = 1; i <= o->ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        double t = o->de[c] * (in[c + 1] - in[c]);
+        t = t + o->dw[c] * (in[c - 1] - in[c]);
+        t = t + o->dn[c] * (in[c + sj] - in[c]);
+        t = t + o->ds[c] * (in[c - sj] - in[c]);
+        t = t + o->dt[c] * (in[c + sk] - in[c]);
+        t = t + o->db[c] * (in[c - sk] - in[c]);
+        t = t + o->dd[c];
+        out[c] = 1.0 * t;
+      }
+}
+
+/* P_SOURCE(IN) (RLES/p_source.f90:14-42): S = -J grad p on J-weighted metrics. */
+void orc3d_p_source(orc3d *o, const double *in) {
+  long sj = o->sj, sk = o->sk;
+  const double *J = o->djac;
+  const double *m[3][3] = {{o->xix, o->etx, o->ztx}, {o->xiy, o->ety, o->zty},
+                           {o->xiz, o->etz, o->ztz}};
+  double *out[3] = {o->s_u, o->s_v, o->s_w};
+  for (int d = 0; d < 3; ++d)
+    for (int k = 1; k <= o->nk - 1; ++k)
+      for (int j = 1; j <= o->nj - 1; ++j)
+        for (int i = 1; i <= o->ni - 1; ++i) {
+          long c = IX(o, i, j, k);
+          const double *a = m[d][0], *b = m[d][1], *g = m[d][2];
+          double t = J[c + 1] * a[c + 1] * in[c + 1];
+          t = t - J[c - 1] * a[c - 1] * in[c - 1];
+          t = t + J[c + sj] * b[c + sj] * in[c + sj];
+          t = t - J[c - sj] * b[c - sj] * in[c - sj];
+          t = t + J[c + sk] * g[c + sk] * in[c + sk];
+          t = t - J[c - sk] * g[c - sk] * in[c - sk];
+          out[d][c] = -(0.5 * t);
+        }
+}
+
+/* ----------------------------------------------------------------------------------------- */
+/* BCU(A,B) (RLES/bcu.f90:15-48): wall no-slip and far-field inflow / convective outflow,     */
+/* folded into AP/SU/AW/AE of the current stage, ghost values, then MPI_X.                   */
+/* ----------------------------------------------------------------------------------------- */
+void orc3d_bcu(orc3d *o, double *a, const double *b) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  long sj = o->sj, sk = o->sk;
+  double uf = 0.0;
+  for (int k = 0; k <= nk; ++k)
+    for (int j = 0; j <= nj; ++j) {
+      long c1 = IX(o, 1, j, k);
+      o->ap[c1] = o->ap[c1] + o->aw[c1];
+      o->su[c1] = o->su[c1] + 2.0 * o->aw[c1] * uf;
+      o->aw[c1] = 0.0;
+      a[c1 - 1] = -a[c1] + 2.0 * uf;
+    }
+  double pi = o->pi;
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j) {
+      long c = IX(o, ni, j, k), m = c - 1;
+      double th = o->theta[c];
+      if (th >= 0.5 * pi && th <= 1.5 * pi) {
+        uf = o->uin;
+      } else {
+        double t1 = 0.5 * (o->xix[c] + o->xix[m]) * (b[c] - b[m]);
+        double t2 = 0.5 * (o->etx[c] + o->etx[m]) * 0.25 *
+                    (b[c + sj] + b[m + sj] - b[c - sj] - b[m - sj]);
+        double t3 = 0.5 * (o->ztx[c] + o->ztx[m]) * 0.25 *
+                    (b[c + sk] + b[m + sk] - b[c - sk] - b[m - sk]);
+        uf = 0.5 * (b[c] + b[m]) - o->dtime * 1.0 * (t1 + t2 + t3);
+      }
+      o->ap[m] = o->ap[m] + o->ae[m];
+      o->su[m] = o->su[m] + 2.0 * o->ae[m] * uf;
+      o->ae[m] = 0.0;
+      a[c] = -a[m] + 2.0 * uf;
+    }
+  orc3d_mpi_x(o, a);
+}
+
+/* Lines solved by the Thomas algorithm with the SAVE'd B(0)=0 start (RLES/iter_u.f90:86-101). */
+static double *g_A, *g_B, *g_D;
+static void ensure_tmp(int n) {
+  static int cap = 0;
+  if (n + 2 > cap) {
+    cap = n + 2;
+    g_A = (double *)realloc(g_A, cap * sizeof(double));
+    g_B = (double *)realloc(g_B, cap * sizeof(double));
+    g_D = (double *)realloc(g_D, cap * sizeof(double));
+  }
+  g_B[0] = 0.0;
+}
+
+/* ITER_U(IN,INM,INA,INB,INC,IND) (RLES/iter_u.f90:1-256): 3 AF stages, each an in-place
+ * lexicographic line Gauss-Seidel sweep (I lines, then J lines, then K lines). */
+void orc3d_iter_u(orc3d *o, double *in, const double *inm, const double *ina, const double *inb,
+                  const double *inc, const double *ind) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  long sj = o->sj, sk = o->sk;
+  const double *J = o->djac;
+  const double *ms[3][3] = {{o->xix, o->etx, o->ztx}, {o->xiy, o->ety, o->zty},
+                            {o->xiz, o->etz, o->ztz}};
+  double re = o->re, dtime = o->dtime;
+  int nmax = ni > nj ? ni : nj;
+  if (nk > nmax) nmax = nk;
+  ensure_tmp(nmax);
+  for (int s = 0; s < 3; ++s) {
+    const double *X = ms[s][0], *E = ms[s][1], *Z = ms[s][2];
+    for (int k = 1; k <= nk - 1; ++k)
+      for (int j = 1; j <= nj - 1; ++j)
+        for (int i = 1; i <= ni - 1; ++i) {
+          long c = IX(o, i, j, k);
+          o->ae[c] = 0.5 * (J[c] * (X[c] * X[c]) + J[c + 1] * (X[c + 1] * X[c + 1]));
+          o->aw[c] = 0.5 * (J[c] * (X[c] * X[c]) + J[c - 1] * (X[c - 1] * X[c - 1]));
+          o->an[c] = 0.5 * (J[c] * (E[c] * E[c]) + J[c + sj] * (E[c + sj] * E[c + sj]));
+          o->as[c] = 0.5 * (J[c] * (E[c] * E[c]) + J[c - sj] * (E[c - sj] * E[c - sj]));
+          o->at[c] = 0.5 * (J[c] * (Z[c] * Z[c]) + J[c + sk] * (Z[c + sk] * Z[c + sk]));
+          o->ab[c] = 0.5 * (J[c] * (Z[c] * Z[c]) + J[c - sk] * (Z[c - sk] * Z[c - sk]));
+          o->ap[c] = 2.0 * re * J[c] / dtime + o->ae[c] + o->aw[c] + o->an[c] + o->as[c] +
+                     o->at[c] + o->ab[c];
+        }
+    for (int k = 1; k <= nk - 1; ++k)
+      for (int j = 1; j <= nj - 1; ++j)
+        for (int i = 1; i <= ni - 1; ++i) {
+          long c = IX(o, i, j, k);
+          long e = c + 1, w = c - 1, nn = c + sj, ss = c - sj, t = c + sk, b = c - sk;
+          const double *f = in;
+          double r = J[e] * X[e] * E[e] * (f[e + sj] - f[e - sj]);
+          r = r - J[w] * X[w] * E[w] * (f[w + sj] - f[w - sj]);
+          r = r + J[e] * X[e] * Z[e] * (f[e + sk] - f[e - sk]);
+          r = r - J[w] * X[w] * Z[w] * (f[w + sk] - f[w - sk]);
+          r = r + J[nn] * E[nn] * X[nn] * (f[nn + 1] - f[nn - 1]);
+          r = r - J[ss] * E[ss] * X[ss] * (f[ss + 1] - f[ss - 1]);
+          r = r + J[nn] * E[nn] * Z[nn] * (f[nn + sk] - f[nn - sk]);
+          r = r - J[ss] * E[ss] * Z[ss] * (f[ss + sk] - f[ss - sk]);
+          r = r + J[t] * Z[t] * X[t] * (f[t + 1] - f[t - 1]);
+          r = r - J[b] * Z[b] * X[b] * (f[b + 1] - f[b - 1]);
+          r = r + J[t] * Z[t] * E[t] * (f[t + sj] - f[t - sj]);
+          r = r - J[b] * Z[b] * E[b] * (f[b + sj] - f[b - sj]);
+          o->dd[c] = 0.25 * r;
+        }
+    for (int k = 1; k <= nk - 1; ++k)
+      for (int j = 1; j <= nj - 1; ++j)
+        for (int i = 1; i <= ni - 1; ++i) {
+          long c = IX(o, i, j, k);
+          if (s == 0) {
+            double t = 2.0 * re * J[c] / dtime * inm[c];
+            t = t - 3.0 * re * ina[c];
+            t = t + 1.0 * re * inb[c];
+            t = t + inc[c];
+            t = t + 2.0 * re * ind[c];
+            t = t + o->dd[c];
+            o->su[c] = t;
+          } else {
+            o->su[c] = 2.0 * re * J[c] / dtime * in[c] + o->dd[c];
+          }
+        }
+    orc3d_bcu(o, in, inm);
+    double *A = g_A, *B = g_B, *D = g_D;
+    if (s == 0) {
+      for (int k = 1; k <= nk - 1; ++k)
+        for (int j = 1; j <= nj - 1; ++j) {
+          for (int i = 1; i <= ni - 1; ++i) {
+            long c = IX(o, i, j, k);
+            A[i] = -o->aw[c];
+            D[i] = o->ap[c] - A[i] * B[i - 1];
+            B[i] = -o->ae[c] / D[i];
+            double C = o->an[c] * in[c + sj] + o->as[c] * in[c - sj] + o->at[c] * in[c + sk] +
+                       o->ab[c] * in[c - sk] + o->su[c];
+            in[c] = (C - A[i] * in[c - 1]) / D[i];
+          }
+          for (int i = ni - 1; i >= 1; --i) {
+            long c = IX(o, i, j, k);
+            in[c] = in[c] - B[i] * in[c + 1];
+          }
+        }
+    } else if (s == 1) {
+      for (int k = 1; k <= nk - 1; ++k)
+        for (int i = 1; i <= ni - 1; ++i) {
+          for (int j = 1; j <= nj - 1; ++j) {
+            long c = IX(o, i, j, k);
+            A[j] = -o->as[c];
+            D[j] = o->ap[c] - A[j] * B[j - 1];
+            B[j] = -o->an[c] / D[j];
+            double C = o->ae[c] * in[c + 1] + o->aw[c] * in[c - 1] + o->at[c] * in[c + sk] +
+                       o->ab[c] * in[c - sk] + o->su[c];
+            in[c] = (C - A[j] * in[c - sj]) / D[j];
+          }
+          for (int j = nj - 1; j >= 1; --j) {
+            long c = IX(o, i, j, k);
+            in[c] = in[c] - B[j] * in[c + sj];
+          }
+        }
+    } else {
+      for (int j = 1; j <= nj - 1; ++j)
+        for (int i = 1; i <= ni - 1; ++i) {
+          for (int k = 1; k <= nk - 1; ++k) {
+            long c = IX(o, i, j, k);
+            A[k] = -o->ab[c];
+            D[k] = o->ap[c] - A[k] * B[k - 1];
+            B[k] = -o->at[c] / D[k];
+            double C = o->ae[c] * in[c + 1] + o->aw[c] * in[c - 1] + o->an[c] * in[c + sj] +
+                       o->as[c] * in[c - sj] + o->su[c];
+            in[c] = (C - A[k] * in[c - sk]) / D[k];
+          }
+          for (int k = nk - 1; k >= 1; --k) {
+            long c = IX(o, i, j, k);
+            in[c] = in[c] - B[k] * in[c + sk];
+          }
+        }
+    }
+  }
+}
+
+/* BCP (RLES/bcp.f90:15-36) */
+void orc3d_bcp(orc3d *o) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  double *P = o->p;
+  for (int k = 0; k <= nk; ++k)
+    for (int j = 1; j <= nj - 1; ++j) {
+      long c = IX(o, 1, j, k);
+      o->ap[c] = o->ap[c] - o->aw[c];
+      o->aw[c] = 0.0;
+      P[c - 1] = P[c];
+    }
+  for (int k = 0; k <= nk; ++k)
+    for (int j = 1; j <= nj - 1; ++j) {
+      long c = IX(o, ni, j, k);
+      o->ae[c - 1] = 0.0;
+      P[c] = 3.0 * P[c - 1] - 3.0 * P[c - 2] + P[c - 3];
+    }
+  orc3d_mpi_x(o, P);
+}
+
+/* LISOLV (RLES/lisolv.f90:15-30): lexicographic radial-line Gauss-Seidel sweep. */
+void orc3d_lisolv(orc3d *o) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  long sj = o->sj, sk = o->sk;
+  double *P = o->p;
+  ensure_tmp(ni);
+  double *A = g_A, *B = g_B, *D = g_D;
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j) {
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        A[i] = -o->aw[c];
+        D[i] = o->ap[c] - A[i] * B[i - 1];
+        B[i] = -o->ae[c] / D[i];
+        double C = o->an[c] * P[c + sj] + o->as[c] * P[c - sj] + o->at[c] * P[c + sk] +
+                   o->ab[c] * P[c - sk] + o->su[c];
+        P[c] = (C - A[i] * P[c - 1]) / D[i];
+      }
+      for (int i = ni - 1; i >= 1; --i) {
+        long c = IX(o, i, j, k);
+        P[c] = P[c] - B[i] * P[c + 1];
+      }
+    }
+}
+
+/* ITER_P (RLES/iter_p.f90:12-63) */
+void orc3d_iter_p(orc3d *o, int nswpp, double resimax) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  long sj = o->sj, sk = o->sk;
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        o->ap[c] = -o->de[c] - o->dw[c] - o->dn[c] - o->ds[c] - o->dt[c] - o->db[c];
+        o->ae[c] = -o->de[c];
+        o->aw[c] = -o->dw[c];
+        o->an[c] = -o->dn[c];
+        o->as[c] = -o->ds[c];
+        o->at[c] = -o->dt[c];
+        o->ab[c] = -o->db[c];
+      }
+  double dtime = o->dtime;
+  o->niter = 0;
+  for (;;) {
+    o->niter += 1;
+    orc3d_dif_diagonal(o, o->p, o->dd);
+    for (int k = 1; k <= nk - 1; ++k)
+      for (int j = 1; j <= nj - 1; ++j)
+        for (int i = 1; i <= ni - 1; ++i) {
+          long c = IX(o, i, j, k);
+          double dv = o->ue[c] - o->uw[c] + o->vnf[c] - o->vs[c] + o->wt[c] - o->wb[c];
+          o->su[c] = 1.0 / dtime * dv - o->dd[c];
+        }
+    orc3d_bcp(o);
+    orc3d_lisolv(o);
+    double res = 0.0, dum = 0.0;
+    double *P = o->p;
+    for (int k = 1; k <= nk - 1; ++k)
+      for (int j = 1; j <= nj - 1; ++j)
+        for (int i = 1; i <= ni - 1; ++i) {
+          long c = IX(o, i, j, k);
+          double r = o->ae[c] * P[c + 1] + o->aw[c] * P[c - 1] + o->an[c] * P[c + sj] +
+                     o->as[c] * P[c - sj] + o->at[c] * P[c + sk] + o->ab[c] * P[c - sk] +
+                     o->su[c] - o->ap[c] * P[c];
+          res = res + fabs(r);
+          dum = dum + fabs(o->ap[c] * P[c]);
+        }
+    o->resip = res / dum;
+    if (!(o->resip > resimax && o->niter < nswpp)) break;
+  }
+}
+
+/* TEST_FILTER(IN,OUT) (RLES/test_filter.f90:13-30): (1,2,1)^3/64 on index space. */
+void orc3d_test_filter(orc3d *o, const double *in, double *out) {
+  long sj = o->sj, sk = o->sk;
+  for (int k = 1; k <= o->nk - 1; ++k)
+    for (int j = 1; j <= o->nj - 1; ++j)
+      for (int i = 1; i <= o->ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        const double *f = in + c;
+        /* the reference's term order: K-1 plane rows J-1, J, J+1; K plane; K+1 plane */
+        static const int wk[3] = {1, 2, 1};
+        double t = 0.0;
+        int first = 1;
+        for (int dk = -1; dk <= 1; ++dk)
+          for (int dj = -1; dj <= 1; ++dj)
+            for (int di = -1; di <= 1; ++di) {
+              double wgt = (double)(wk[di + 1] * wk[dj + 1] * wk[dk + 1]);
+              double term = wgt * f[di + sj * dj + sk * dk];
+              if (wgt == 1.0) term = f[di + sj * dj + sk * dk];
+              if (first) { t = term; first = 0; } else t = t + term;
+            }
+        out[c] = t / 64.0;
+      }
+}
+
+/* strain components at interior cells (RLES/stress_tensor_dy.f90:27-76), the I=0 / I=NI
+ * one-sided variants (:96-197), MPI_X (:200-214), and |S| (:218-221). */
+static void strain(orc3d *o) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  long sj = o->sj, sk = o->sk;
+  const double *U = o->u, *V = o->v, *W = o->w;
+  double *L11 = o->lss[0], *L12 = o->lss[1], *L13 = o->lss[2], *L22 = o->lss[3],
+         *L23 = o->lss[4], *L33 = o->lss[5];
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+#define DI(F) (F[c + 1] - F[c - 1])
+#define DJ(F) (F[c + sj] - F[c - sj])
+#define DK(F) (F[c + sk] - F[c - sk])
+        double t;
+        t = o->xix[c] * DI(U) + o->etx[c] * DJ(U) + o->ztx[c] * DK(U);
+        L11[c] = 0.5 * t;
+        t = o->xiy[c] * DI(U) + o->ety[c] * DJ(U) + o->zty[c] * DK(U) + o->xix[c] * DI(V) +
+            o->etx[c] * DJ(V) + o->ztx[c] * DK(V);
+        L12[c] = 0.25 * t;
+        t = o->xiz[c] * DI(U) + o->etz[c] * DJ(U) + o->ztz[c] * DK(U) + o->xix[c] * DI(W) +
+            o->etx[c] * DJ(W) + o->ztx[c] * DK(W);
+        L13[c] = 0.25 * t;
+        t = o->xiy[c] * DI(V) + o->ety[c] * DJ(V) + o->zty[c] * DK(V);
+        L22[c] = 0.5 * t;
+        t = o->xiz[c] * DI(V) + o->etz[c] * DJ(V) + o->ztz[c] * DK(V) + o->xiy[c] * DI(W) +
+            o->ety[c] * DJ(W) + o->zty[c] * DK(W);
+        L23[c] = 0.25 * t;
+        t = o->xiz[c] * DI(W) + o->etz[c] * DJ(W) + o->ztz[c] * DK(W);
+        L33[c] = 0.5 * t;
+#undef DI
+      }
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int side = 0; side < 2; ++side) {
+        int i = side == 0 ? 0 : ni;
+        long c = IX(o, i, j, k);
+        /* one-sided I difference: U(I+1)-U(I) at I=0, U(I)-U(I-1) at I=NI */
+#define DI(F) (side == 0 ? (F[c + 1] - F[c]) : (F[c] - F[c - 1]))
+        L11[c] = 0.5 * (2.0 * o->xix[c] * DI(U) + o->etx[c] * DJ(U) + o->ztx[c] * DK(U));
+        L12[c] = 0.25 * (2.0 * o->xiy[c] * DI(U) + o->ety[c] * DJ(U) + o->zty[c] * DK(U) +
+                         2.0 * o->xix[c] * DI(V) + o->etx[c] * DJ(V) + o->ztx[c] * DK(V));
+        L13[c] = 0.25 * (2.0 * o->xiz[c] * DI(U) + o->etz[c] * DJ(U) + o->ztz[c] * DK(U) +
+                         2.0 * o->xix[c] * DI(W) + o->etx[c] * DJ(W) + o->ztx[c] * DK(W));
+        L22[c] = 0.5 * (2.0 * o->xiy[c] * DI(V) + o->ety[c] * DJ(V) + o->zty[c] * DK(V));
+        L23[c] = 0.25 * (2.0 * o->xiz[c] * DI(V) + o->etz[c] * DJ(V) + o->ztz[c] * DK(V) +
+                         2.0 * o->xiy[c] * DI(W) + o->ety[c] * DJ(W) + o->zty[c] * DK(W));
+        L33[c] = 0.5 * (2.0 * o->xiz[c] * DI(W) + o->etz[c] * DJ(W) + o->ztz[c] * DK(W));
+#undef DI
+#undef DJ
+#undef DK
+      }
+  for (int q = 0; q < 6; ++q) orc3d_mpi_x(o, o->lss[q]);
+  for (long c = 0; c < o->n; ++c) {
+    double a = L11[c], b = L12[c], d = L13[c], e = L22[c], f = L23[c], g = L33[c];
+    /* order L11 L12 L13 L21 L22 L23 L31 L32 L33 with L21=L12, L31=L13, L32=L23 */
+    double s = a * a + b * b + d * d + b * b + e * e + f * f + d * d + f * f + g * g;
+    o->mag[c] = sqrt(2.0 * s);
+  }
+}
+
+/* STRESS_TENSOR_DY velocity part (RLES/stress_tensor_dy.f90:24-286): dynamic Smagorinsky with
+ * the Germano identity, Lilly least squares and the reference's global conditional average. */
+void orc3d_sgs(orc3d *o) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  long sj = o->sj, sk = o->sk, n = o->n;
+  strain(o);
+  orc3d_test_filter(o, o->u, o->uh);
+  orc3d_test_filter(o, o->v, o->vh);
+  orc3d_test_filter(o, o->w, o->wh);
+  double save_uin = o->uin;
+  o->uin = 1.0; orc3d_bcu(o, o->uh, o->un);
+  o->uin = 0.0; orc3d_bcu(o, o->vh, o->vn);
+  orc3d_bcu(o, o->wh, o->wn);
+  o->uin = save_uin;
+  const double *UH = o->uh, *VH = o->vh, *WH = o->wh;
+  double *sh[6];
+  for (int q = 0; q < 6; ++q) sh[q] = alloc0(n);
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+#define DI(F) (F[c + 1] - F[c - 1])
+#define DJ(F) (F[c + sj] - F[c - sj])
+#define DK(F) (F[c + sk] - F[c - sk])
+        double s11 = 0.5 * (o->xix[c] * DI(UH) + o->etx[c] * DJ(UH) + o->ztx[c] * DK(UH));
+        double s12 = 0.25 * (o->xiy[c] * DI(UH) + o->ety[c] * DJ(UH) + o->zty[c] * DK(UH) +
+                             o->xix[c] * DI(VH) + o->etx[c] * DJ(VH) + o->ztx[c] * DK(VH));
+        double s13 = 0.25 * (o->xiz[c] * DI(UH) + o->etz[c] * DJ(UH) + o->ztz[c] * DK(UH) +
+                             o->xix[c] * DI(WH) + o->etx[c] * DJ(WH) + o->ztx[c] * DK(WH));
+        double s22 = 0.5 * (o->xiy[c] * DI(VH) + o->ety[c] * DJ(VH) + o->zty[c] * DK(VH));
+        double s23 = 0.25 * (o->xiz[c] * DI(VH) + o->etz[c] * DJ(VH) + o->ztz[c] * DK(VH) +
+                             o->xiy[c] * DI(WH) + o->ety[c] * DJ(WH) + o->zty[c] * DK(WH));
+        double s33 = 0.5 * (o->xiz[c] * DI(WH) + o->etz[c] * DJ(WH) + o->ztz[c] * DK(WH));
+#undef DI
+#undef DJ
+#undef DK
+        double d = 2.0 * (s11 * s11 + s12 * s12 + s13 * s13 + s12 * s12 + s22 * s22 +
+                          s23 * s23 + s13 * s13 + s23 * s23 + s33 * s33);
+        o->magh[c] = sqrt(d);
+        sh[0][c] = s11; sh[1][c] = s12; sh[2][c] = s13;
+        sh[3][c] = s22; sh[4][c] = s23; sh[5][c] = s33;
+      }
+  /* component order of the reference: 11, 22, 33, 12, 13, 23 */
+  static const int qord[6] = {0, 3, 5, 1, 2, 4};
+  const double *ua[6] = {o->u, o->v, o->w, o->u, o->u, o->v};
+  const double *ub[6] = {o->u, o->v, o->w, o->v, o->w, o->w};
+  const double *ha[6] = {UH, VH, WH, UH, UH, VH};
+  const double *hb[6] = {UH, VH, WH, VH, WH, WH};
+  for (int m = 0; m < 6; ++m) {
+    int q = qord[m];
+    for (long c = 0; c < n; ++c) o->dumin[c] = ua[m][c] * ub[m][c];
+    orc3d_test_filter(o, o->dumin, o->dumout);
+    for (int k = 1; k <= nk - 1; ++k)
+      for (int j = 1; j <= nj - 1; ++j)
+        for (int i = 1; i <= ni - 1; ++i) {
+          long c = IX(o, i, j, k);
+          o->lij[c] = m < 3 ? -o->dumout[c] + ha[m][c] * ha[m][c]
+                            : -o->dumout[c] + ha[m][c] * hb[m][c];
+          double mh = o->magh[c];
+          o->mij[c] = m < 3 ? 4.0 * mh * 0.5 * (2.0 * sh[q][c])
+                            : 4.0 * mh * 0.25 * (4.0 * sh[q][c]);
+        }
+    for (long c = 0; c < n; ++c) o->dumin[c] = o->mag[c] * o->lss[q][c];
+    orc3d_test_filter(o, o->dumin, o->dumout);
+    for (int k = 1; k <= nk - 1; ++k)
+      for (int j = 1; j <= nj - 1; ++j)
+        for (int i = 1; i <= ni - 1; ++i) {
+          long c = IX(o, i, j, k);
+          double fw = o->filt[c];
+          double M = fw * fw * (o->mij[c] - o->dumout[c]);
+          double L = o->lij[c];
+          if (m == 0) {
+            o->lm[c] = L * M;
+            o->mm[c] = M * M;
+          } else if (m < 3) {
+            o->lm[c] = o->lm[c] + L * M;
+            o->mm[c] = o->mm[c] + M * M;
+          } else {
+            o->lm[c] = o->lm[c] + 2.0 * L * M;
+            o->mm[c] = o->mm[c] + 2.0 * M * M;
+          }
+        }
+  }
+  for (int q = 0; q < 6; ++q) free(sh[q]);
+  double lm_avg = 0.0, mm_avg = 0.0, inc = 0.0;
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        double cs = 0.5 * o->lm[c] / o->mm[c];
+        if (cs >= 0.0) {
+          lm_avg = lm_avg + o->lm[c];
+          mm_avg = mm_avg + o->mm[c];
+          inc = inc + 1.0;
+        }
+      }
+  double tlm = lm_avg / inc, tmm = mm_avg / inc;
+  double cavg = 0.5 * tlm / tmm;
+  if (tmm < 1e-6) cavg = 0.0;
+  o->c_avg = cavg;
+  for (long c = 0; c < n; ++c) o->nut[c] = cavg * (o->filt[c] * o->filt[c]) * o->mag[c];
+  const double *NU = o->nut;
+  const double *L[3][3] = {{o->lss[0], o->lss[1], o->lss[2]},
+                           {o->lss[1], o->lss[3], o->lss[4]},
+                           {o->lss[2], o->lss[4], o->lss[5]}};
+  double *out[3] = {o->sgsx, o->sgsy, o->sgsz};
+  const double *mx[3][3] = {{o->xix, o->etx, o->ztx}, {o->xiy, o->ety, o->zty},
+                            {o->xiz, o->etz, o->ztz}};
+  for (int d = 0; d < 3; ++d)
+    for (int k = 1; k <= nk - 1; ++k)
+      for (int j = 1; j <= nj - 1; ++j)
+        for (int i = 1; i <= ni - 1; ++i) {
+          long c = IX(o, i, j, k);
+          double t = 0.0;
+          int first = 1;
+          for (int cmp = 0; cmp < 3; ++cmp) {
+            const double *S = L[d][cmp];
+            long off[3] = {1, sj, sk};
+            for (int dir = 0; dir < 3; ++dir) {
+              long of = off[dir];
+              double inner = -(2.0 * NU[c + of] * S[c + of]) + 2.0 * NU[c - of] * S[c - of];
+              double term = mx[cmp][dir][c] * inner;
+              if (first) { t = term; first = 0; } else t = t + term;
+            }
+          }
+          out[d][c] = 0.5 * t;
+        }
+}
+
+/* CALCCOEF (RLES/calccoef.f90:11-201): CFL, base pressure and force coefficients. */
+void orc3d_calccoef(orc3d *o) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  (void)ni;
+  long sj = o->sj, sk = o->sk;
+  double cfl = 0.0;
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= o->ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        double a = fabs(o->xix[c] * o->u[c] + o->xiy[c] * o->v[c] + o->xiz[c] * o->w[c]);
+        double b = fabs(o->etx[c] * o->u[c] + o->ety[c] * o->v[c] + o->etz[c] * o->w[c]);
+        double d = fabs(o->ztx[c] * o->u[c] + o->zty[c] * o->v[c] + o->ztz[c] * o->w[c]);
+        double l = o->dtime * (a + b + d);
+        cfl = cfl > l ? cfl : l;
+      }
+  o->cfl = cfl;
+  /* base-point pressure: PG at I=1, J=1 (average of the 8 surrounding cells) */
+  double dumb = 0.0;
+  for (int k = 1; k <= nk - 1; ++k) {
+    long c = IX(o, 1, 1, k);
+    const double *P = o->p;
+    double pg = 0.125 * (P[c] + P[c - sj] + P[c - 1] + P[c - 1 - sj] + P[c + sk] +
+                         P[c - sj + sk] + P[c - 1 + sk] + P[c - 1 - sj + sk]);
+    dumb = dumb + pg;
+  }
+  dumb = dumb / (double)(nk - 1);
+  o->cpb = 2.0 * (0.0 - dumb);
+  double pd = 0.0, ply = 0.0, td = 0.0, tly = 0.0;
+  for (int tri = 0; tri < 2; ++tri)
+    for (int k = 1; k <= nk - 1; ++k)
+      for (int j = 1; j <= nj - 1; ++j) {
+        int i = 1;
+#define XG(i, j, k) o->xg[IX(o, i, j, k)]
+#define YG(i, j, k) o->yg[IX(o, i, j, k)]
+#define ZG(i, j, k) o->zg[IX(o, i, j, k)]
+        double A, B, Cc, D;
+        if (tri == 0) {
+          A = YG(i, j, k) * (ZG(i, j + 1, k) - ZG(i, j, k + 1)) +
+              YG(i, j + 1, k) * (ZG(i, j, k + 1) - ZG(i, j, k)) +
+              YG(i, j, k + 1) * (ZG(i, j, k) - ZG(i, j + 1, k));
+          B = ZG(i, j, k) * (XG(i, j + 1, k) - XG(i, j, k + 1)) +
+              ZG(i, j + 1, k) * (XG(i, j, k + 1) - XG(i, j, k)) +
+              ZG(i, j, k + 1) * (XG(i, j, k) - XG(i, j + 1, k));
+          Cc = XG(i, j, k) * (YG(i, j + 1, k) - YG(i, j, k + 1)) +
+               XG(i, j + 1, k) * (YG(i, j, k + 1) - YG(i, j, k)) +
+               XG(i, j, k + 1) * (YG(i, j, k) - YG(i, j + 1, k));
+          D = XG(i, j, k) * (YG(i, j + 1, k) * ZG(i, j, k + 1) - YG(i, j, k + 1) * ZG(i, j + 1, k)) +
+              XG(i, j + 1, k) * (YG(i, j, k + 1) * ZG(i, j, k) - YG(i, j, k) * ZG(i, j, k + 1)) +
+              XG(i, j, k + 1) * (YG(i, j, k) * ZG(i, j + 1, k) - YG(i, j + 1, k) * ZG(i, j, k));
+        } else {
+          A = YG(i, j + 1, k + 1) * (ZG(i, j, k + 1) - ZG(i, j, k)) +
+              YG(i, j, k + 1) * (ZG(i, j, k) - ZG(i, j + 1, k + 1)) +
+              YG(i, j, k) * (ZG(i, j + 1, k + 1) - ZG(i, j, k + 1));
+          B = ZG(i, j + 1, k + 1) * (XG(i, j, k + 1) - XG(i, j, k)) +
+              ZG(i, j, k + 1) * (XG(i, j, k) - XG(i, j + 1, k + 1)) +
+              ZG(i, j, k) * (XG(i, j + 1, k + 1) - XG(i, j, k + 1));
+          Cc = XG(i, j + 1, k + 1) * (YG(i, j, k + 1) - YG(i, j, k)) +
+               XG(i, j, k + 1) * (YG(i, j, k) - YG(i, j + 1, k + 1)) +
+               XG(i, j, k) * (YG(i, j + 1, k + 1) - YG(i, j, k + 1));
+          D = XG(i, j + 1, k + 1) * (YG(i, j, k + 1) * ZG(i, j, k) - YG(i, j, k) * ZG(i, j, k + 1)) +
+              XG(i, j, k + 1) * (YG(i, j, k) * ZG(i, j + 1, k + 1) - YG(i, j + 1, k + 1) * ZG(i, j, k)) +
+              XG(i, j, k) * (YG(i, j + 1, k + 1) * ZG(i, j, k + 1) - YG(i, j, k + 1) * ZG(i, j + 1, k + 1));
+        }
+#undef XG
+#undef YG
+#undef ZG
+        D = -1.0 * D;
+        double dum = sqrt(A * A + B * B + Cc * Cc);
+        long c = IX(o, i, j, k);
+        double dn = fabs(A * o->x[c] + B * o->y[c] + Cc * o->z[c] + D) / dum;
+        double U = o->u[c], V = o->v[c], W = o->w[c];
+        double t1 = V * Cc / dum - W * B / dum, t2 = U * Cc / dum - W * A / dum,
+               t3 = U * B / dum - V * A / dum;
+        double dudn = sqrt(t1 * t1 + t2 * t2 + t3 * t3);
+        dudn = dudn / dn;
+        double ar = tri == 0 ? o->area1[c] : o->area2[c];
+        double um = sqrt(U * U + V * V + W * W);
+        pd = pd - 2.0 * o->p[c] * ar * A / dum;
+        ply = ply - 2.0 * o->p[c] * ar * B / dum;
+        td = td + (2.0 / o->re) * dudn * ar * U / um;
+        tly = tly + (2.0 / o->re) * dudn * ar * V / um;
+      }
+  o->cdp = pd / o->ref_a;
+  o->clp = ply / o->ref_a;
+  o->cdv = td / o->ref_a;
+  o->clv = tly / o->ref_a;
+  o->cd = o->cdp + o->cdv;
+  o->cl = o->clp + o->clv;
+}
+
+/* One time step of the reference time loop (RLES/main.f90:39-150, thermal scalar and file
+ * output excluded). */
+void orc3d_step(orc3d *o, int nswpp, double resimax, int use_les) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  long n = o->n;
+  size_t b = n * sizeof(double);
+  memcpy(o->un, o->u, b); memcpy(o->vn, o->v, b); memcpy(o->wn, o->w, b);
+  orc3d_contra_vel(o);
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        o->nlx1[c] = o->nlx[c]; o->nly1[c] = o->nly[c]; o->nlz1[c] = o->nlz[c];
+      }
+  orc3d_nl(o, o->un, o->nlx);
+  orc3d_nl(o, o->vn, o->nly);
+  orc3d_nl(o, o->wn, o->nlz);
+  if (use_les) {
+    orc3d_sgs(o);
+    for (int k = 1; k <= nk - 1; ++k)
+      for (int j = 1; j <= nj - 1; ++j)
+        for (int i = 1; i <= ni - 1; ++i) {
+          long c = IX(o, i, j, k);
+          o->nlx[c] = o->nlx[c] + o->sgsx[c] * o->djac[c];
+          o->nly[c] = o->nly[c] + o->sgsy[c] * o->djac[c];
+          o->nlz[c] = o->nlz[c] + o->sgsz[c] * o->djac[c];
+        }
+  }
+  orc3d_dif(o, o->un, o->difx);
+  orc3d_dif(o, o->vn, o->dify);
+  orc3d_dif(o, o->wn, o->difz);
+  orc3d_p_source(o, o->p);
+  o->uin = 1.0;
+  orc3d_iter_u(o, o->u, o->un, o->nlx, o->nlx1, o->difx, o->s_u);
+  o->uin = 0.0;
+  orc3d_iter_u(o, o->v, o->vn, o->nly, o->nly1, o->dify, o->s_v);
+  orc3d_iter_u(o, o->w, o->wn, o->nlz, o->nlz1, o->difz, o->s_w);
+  double dt = o->dtime;
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        o->u[c] = o->u[c] - dt * o->s_u[c] / o->djac[c];
+        o->v[c] = o->v[c] - dt * o->s_v[c] / o->djac[c];
+        o->w[c] = o->w[c] - dt * o->s_w[c] / o->djac[c];
+      }
+  o->uin = 1.0; orc3d_bcu(o, o->u, o->un);
+  o->uin = 0.0; orc3d_bcu(o, o->v, o->vn); orc3d_bcu(o, o->w, o->wn);
+  orc3d_contra_vel(o);
+  orc3d_iter_p(o, nswpp, resimax);
+  orc3d_bcp(o);
+  orc3d_p_source(o, o->p);
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        o->u[c] = o->u[c] + dt * o->s_u[c] / o->djac[c];
+        o->v[c] = o->v[c] + dt * o->s_v[c] / o->djac[c];
+        o->w[c] = o->w[c] + dt * o->s_w[c] / o->djac[c];
+      }
+  double dumb = 0.0;
+  for (int k = 1; k <= nk - 1; ++k) dumb = dumb + o->p[IX(o, ni, nj / 4, k)];
+  dumb = dumb / (double)(nk - 1);
+  o->pref = dumb;
+  for (long c = 0; c < n; ++c) o->p[c] = o->p[c] - dumb / 1.0;
+  o->uin = 1.0; orc3d_bcu(o, o->u, o->un);
+  o->uin = 0.0; orc3d_bcu(o, o->v, o->vn); orc3d_bcu(o, o->w, o->wn);
+  orc3d_calccoef(o);
+}

@@ -1,0 +1,91 @@
+"""ctypes wrapper of the C restatement of the reference 3D solver (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this module; the
+product package fvm_amd never imports it.  Arrays are (NI+1, NJ+1, NK+1) float64 in Fortran
+order, ghosts included, exactly as the reference holds them with one MPI rank.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle3d.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (run make -C oracle)")
+        L = ctypes.CDLL(path)
+        vp, ci, cd, cs = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_char_p
+        L.orc3d_create.restype = vp
+        L.orc3d_create.argtypes = [ci, ci, ci, cd, cd, cd]
+        L.orc3d_destroy.argtypes = [vp]
+        L.orc3d_array.restype = ctypes.POINTER(ctypes.c_double)
+        L.orc3d_array.argtypes = [vp, cs]
+        L.orc3d_scalar.restype = cd
+        L.orc3d_scalar.argtypes = [vp, cs]
+        L.orc3d_setup.argtypes = [vp, vp, vp, vp]
+        L.orc3d_set_state.argtypes = [vp] * 8
+        L.orc3d_step.argtypes = [vp, ci, cd, ci]
+        for f in ("orc3d_contra_vel", "orc3d_sgs", "orc3d_bcp", "orc3d_lisolv", "orc3d_calccoef"):
+            getattr(L, f).argtypes = [vp]
+        L.orc3d_iter_p.argtypes = [vp, ci, cd]
+        L.orc3d_nl.argtypes = [vp, vp, vp]
+        L.orc3d_dif.argtypes = [vp, vp, vp]
+        L.orc3d_dif_diagonal.argtypes = [vp, vp, vp]
+        L.orc3d_p_source.argtypes = [vp, vp]
+        L.orc3d_test_filter.argtypes = [vp, vp, vp]
+        L.orc3d_bcu.argtypes = [vp, vp, vp]
+        L.orc3d_mpi_x.argtypes = [vp, vp]
+        L.orc3d_iter_u.argtypes = [vp] * 7
+        _LIB = L
+    return _LIB
+
+
+def _full(nodes, ni, nj, nk):
+    """Node array (NI, NJ, NK) -> oracle node layout (NI+1, NJ+1, NK+1) holding nodes 1..N."""
+    a = np.zeros((ni + 1, nj + 1, nk + 1), order="F")
+    a[1:, 1:, 1:] = nodes
+    return a
+
+
+class Oracle3D:
+    """The reference 3D time step on the CPU, one rank, reference loop order."""
+
+    def __init__(self, ni, nj, nk, re, dtime, ref_a, xg, yg, zg):
+        self.L = lib()
+        self.ni, self.nj, self.nk = ni, nj, nk
+        self.shape = (ni + 1, nj + 1, nk + 1)
+        self.h = ctypes.c_void_p(self.L.orc3d_create(ni, nj, nk, re, dtime, ref_a))
+        nodes = [_full(np.asarray(a, dtype=np.float64), ni, nj, nk) for a in (xg, yg, zg)]
+        self.L.orc3d_setup(self.h, *[a.ctypes.data_as(ctypes.c_void_p) for a in nodes])
+
+    def __del__(self):
+        try:
+            self.L.orc3d_destroy(self.h)
+        except Exception:
+            pass
+
+    def array(self, name: str) -> np.ndarray:
+        """Live view of a named oracle array (Fortran order, ghosts included)."""
+        p = self.L.orc3d_array(self.h, name.encode())
+        if not p:
+            raise KeyError(name)
+        n = int(np.prod(self.shape))
+        return np.ctypeslib.as_array(p, shape=(n,)).reshape(self.shape, order="F")
+
+    def scalar(self, name: str) -> float:
+        return float(self.L.orc3d_scalar(self.h, name.encode()))
+
+    def set_state(self, u, v, w, p, nlx, nly, nlz):
+        arrs = [np.asfortranarray(a, dtype=np.float64) for a in (u, v, w, p, nlx, nly, nlz)]
+        self.L.orc3d_set_state(self.h, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs])
+
+    def step(self, nswpp=20, resimax=1e-4, use_les=True):
+        self.L.orc3d_step(self.h, int(nswpp), float(resimax), 1 if use_les else 0)

@@ -1,0 +1,91 @@
+"""Write the golden fixtures of the 3D reference solver (runs in the build container only).
+
+Each fixture is produced by the reference solver itself: the LES/v2.3_LES-Reynolds stresses
+sources compiled with flang by oracle/ref/build_ref.sh and driven step by step by the harness
+oracle/ref/harness3d.f90.  The inputs (grid nodes, seeded restart state) come from
+fvm_amd.grid; the outputs are the harness dumps after each sub-stage.  The fixture stores the
+inputs and the outputs as float64 arrays in an .npz (data only).
+
+usage: python tests/golden/make_golden.py            (writes tests/golden/*.npz)
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+from fvm_amd.grid import (OGrid, seeded_ic, write_grid3d, write_restart3d,  # noqa: E402
+                          read_restart3d, grid_filename)
+
+CASES = {
+    # 12 x 16 x 8 cells, reference Poisson settings (20 sweeps, 1e-4), LES on, 3 steps
+    "rles_13x17x9_les_nswpp20": dict(ni=13, nj=17, nk=9, h1=0.05, r_out=5.0, sigma=0.05,
+                                      re=100.0, dt=5e-3, nswpp=20, resimax="1.0D-4", les=1,
+                                      steps=3, keep_steps=(1, 3)),
+    # 8 x 20 x 4 cells, Poisson run to convergence (fixed point), LES on, 2 steps
+    "rles_9x21x5_les_converged": dict(ni=9, nj=21, nk=5, h1=0.08, r_out=4.0, sigma=0.05,
+                                      re=200.0, dt=4e-3, nswpp=200000, resimax="1.0D-13",
+                                      les=1, steps=2, keep_steps=(1, 2)),
+    # (no USE_LES=0 fixture: the reference's laminar mode writes YPLUS in CALCCOEF
+    # (calccoef.f90:121), an array INIT allocates only when USE_LES=1 (init.f90:79-81))
+}
+
+STEP_TAGS = ("ue", "vn", "wt", "nl0x", "nlx", "nly", "nlz", "difx", "dify", "difz", "su0",
+             "sgsx", "sgsy", "sgsz", "nut", "ustar", "vstar", "wstar", "uhat", "p", "u", "v",
+             "w", "pref")
+SETUP_TAGS = ("x", "y", "z", "djac", "xix", "xiy", "xiz", "etx", "ety", "etz", "ztx", "zty",
+              "ztz", "q12", "q13", "q23", "de", "dn", "dt", "theta")
+
+
+def build(label, c):
+    exe = os.path.join(REPO, "oracle", "_ref", label, "harness3d")
+    if not os.path.exists(exe):
+        subprocess.run([os.path.join(REPO, "oracle", "ref", "build_ref.sh"), label, str(c["ni"]),
+                        str(c["nj"]), str(c["nk"]), f"{c['re']}D0", f"{c['dt']}D0".replace("e", "D"),
+                        str(c["nswpp"]), c["resimax"], str(c["les"]), "3.14159265358979D0"],
+                       check=True)
+    return exe
+
+
+def run_case(label, c):
+    exe = build(label, c)
+    ni, nj, nk = c["ni"], c["nj"], c["nk"]
+    g = OGrid(ni, nj, nk, c["h1"], c["r_out"])
+    xg, yg, zg = g.nodes()
+    ic = seeded_ic(ni, nj, nk, c["sigma"])
+    out = {"ni": ni, "nj": nj, "nk": nk, "re": c["re"], "dt": c["dt"], "ref_a": 3.14159265358979,
+           "nswpp": c["nswpp"], "resimax": float(c["resimax"].replace("D", "e")),
+           "use_les": c["les"], "xg": xg, "yg": yg, "zg": zg}
+    shape = (ni + 1, nj + 1, nk + 1)
+    with tempfile.TemporaryDirectory() as d:
+        write_grid3d(os.path.join(d, grid_filename(ni, nj, nk)), xg, yg, zg)
+        write_restart3d(os.path.join(d, "RESTART000.DAT"), ic, c["re"], c["re"] * 0.71, c["dt"], 0.0)
+        # the state the reference reads: the E24.16 records parsed back (16 digits)
+        rs = read_restart3d(os.path.join(d, "RESTART000.DAT"), ni, nj, nk)
+        for k in ("u", "v", "w", "p", "nlx", "nly", "nlz"):
+            out["in_" + k] = rs[k]
+        subprocess.run([exe, str(c["steps"])], cwd=d, check=True, stdout=subprocess.DEVNULL)
+        rd = lambda t: np.fromfile(os.path.join(d, t + ".bin")).reshape(shape, order="F")
+        for t in SETUP_TAGS:
+            out["setup_" + t] = rd("setup_" + t)
+        for s in c["keep_steps"]:
+            for t in STEP_TAGS:
+                p = os.path.join(d, f"s{s:03d}_{t}.bin")
+                if os.path.exists(p):
+                    out[f"s{s}_{t}"] = rd(f"s{s:03d}_{t}")
+        rows = [l.split() for l in open(os.path.join(d, "scalars.txt")) if l.startswith("step")]
+        # columns: step, niter, resip, pref, cfl, cpb, cd, cl, cd_p, cd_v
+        out["scalars"] = np.array([[float(x) for x in r[1:]] for r in rows])
+    np.savez_compressed(os.path.join(HERE, label + ".npz"), **out)
+    print("wrote", label, "steps", c["steps"], "niter", out["scalars"][:, 1])
+
+
+if __name__ == "__main__":
+    for label, c in CASES.items():
+        run_case(label, c)
