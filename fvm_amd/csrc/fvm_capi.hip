@@ -1,0 +1,354 @@
+// C ABI of the solver: context life cycle, restart I/O, the time-step sequence and kernel
+// timing.  See include/fvm_capi.h for the contract.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "fvm_ctx.h"
+
+ProfScope::ProfScope(FvmCtx* ctx, const char* nm) : c(ctx), name(nm) {
+  if (!c->prof_on) return;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
+  (void)hipEventRecord(a, c->stream);
+}
+
+ProfScope::~ProfScope() {
+  if (!c->prof_on || !a) return;
+  (void)hipEventRecord(b, c->stream);
+  c->ev_pending.push_back({name, {a, b}});
+}
+
+static void prof_flush(FvmCtx* c) {
+  if (c->ev_pending.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev_pending) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e.second.first, e.second.second);
+    ProfAcc& acc = c->prof[e.first];
+    acc.ms += ms;
+    acc.calls += 1;
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+  c->ev_pending.clear();
+}
+
+static int dalloc(FvmCtx* c, double** p, long n) {
+  if (hipMalloc((void**)p, n * sizeof(double)) != hipSuccess) return FVM_ERR_HIP;
+  c->allocs.push_back(*p);
+  if (hipMemsetAsync(*p, 0, n * sizeof(double), c->stream) != hipSuccess) return FVM_ERR_HIP;
+  return FVM_OK;
+}
+
+static double* field_ptr(FvmCtx* c, int f) {
+  switch (f) {
+    case FVM_U: return c->u;
+    case FVM_V: return c->v;
+    case FVM_W: return c->w;
+    case FVM_P: return c->p;
+    case FVM_NLX: return c->nl[0];
+    case FVM_NLY: return c->nl[1];
+    case FVM_NLZ: return c->nl[2];
+    case FVM_DIFX: return c->dif[0];
+    case FVM_DIFY: return c->dif[1];
+    case FVM_DIFZ: return c->dif[2];
+    case FVM_SU: return c->s[0];
+    case FVM_SV: return c->s[1];
+    case FVM_SW: return c->s[2];
+    case FVM_DJAC: return c->djac;
+    case FVM_XIX: return c->m[0];
+    case FVM_ETX: return c->m[1];
+    case FVM_ZTX: return c->m[2];
+    case FVM_XIY: return c->m[3];
+    case FVM_ETY: return c->m[4];
+    case FVM_ZTY: return c->m[5];
+    case FVM_XIZ: return c->m[6];
+    case FVM_ETZ: return c->m[7];
+    case FVM_ZTZ: return c->m[8];
+    case FVM_X: return c->x;
+    case FVM_Y: return c->y;
+    case FVM_Z: return c->z;
+    case FVM_THETA: return c->theta;
+    case FVM_NUT: return c->nut;
+    case FVM_SGSX: return c->sgs[0];
+    case FVM_SGSY: return c->sgs[1];
+    case FVM_SGSZ: return c->sgs[2];
+    case FVM_UN: return c->un;
+    default: return nullptr;
+  }
+}
+
+extern "C" {
+
+int fvm_create(const fvm_params* prm, const double* xg, const double* yg, const double* zg,
+               fvm_ctx** out) {
+  if (!prm || !xg || !yg || !zg || !out) return FVM_ERR_ARG;
+  if (prm->ni < 4 || prm->nj < 4 || prm->nk < 3) return FVM_ERR_ARG;
+  if (prm->ni - 1 > FVM_MAX_LINE || prm->nj - 1 > FVM_MAX_LINE || prm->nk - 1 > FVM_MAX_LINE)
+    return FVM_ERR_SIZE;
+  if (!(prm->re > 0) || !(prm->dtime > 0) || prm->nswpp < 1) return FVM_ERR_ARG;
+  FvmCtx* c = new FvmCtx();
+  c->prm = *prm;
+  if (c->prm.check_every < 1) c->prm.check_every = 1;
+  c->d.ni = prm->ni; c->d.nj = prm->nj; c->d.nk = prm->nk;
+  c->d.sj = prm->ni + 1;
+  c->d.sk = (long)(prm->ni + 1) * (prm->nj + 1);
+  c->d.n = c->d.sk * (prm->nk + 1);
+  c->pi = 4.0 * atan(1.0);
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1) {
+    fprintf(stderr, "fvm_amd: no HIP device\n");
+    delete c;
+    return FVM_ERR_HIP;
+  }
+  (void)hipGetDevice(&c->dev);
+  if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return FVM_ERR_HIP; }
+  const long n = c->d.n;
+  int r = 0;
+  double** cells[] = {&c->xg, &c->yg, &c->zg, &c->x, &c->y, &c->z, &c->djac, &c->jq11, &c->jq22,
+                      &c->jq33, &c->jq12, &c->jq13, &c->jq23, &c->theta, &c->filt2, &c->ap0,
+                      &c->pdI, &c->pdJ, &c->pdK, &c->u, &c->v, &c->w, &c->p, &c->un, &c->vn,
+                      &c->wn, &c->rhs, &c->psu, &c->mag, &c->nut};
+  for (double** q : cells) r |= dalloc(c, q, n);
+  for (int a = 0; a < 9; ++a) r |= dalloc(c, &c->m[a], n);
+  for (int s = 0; s < 3; ++s)
+    for (int q = 0; q < 3; ++q) r |= dalloc(c, &c->fa[s][q], n);
+  for (int q = 0; q < 3; ++q) {
+    r |= dalloc(c, &c->nl[q], n);
+    r |= dalloc(c, &c->nl1[q], n);
+    r |= dalloc(c, &c->dif[q], n);
+    r |= dalloc(c, &c->s[q], n);
+    r |= dalloc(c, &c->su[q], n);
+    r |= dalloc(c, &c->ju[q], n);
+    r |= dalloc(c, &c->uh[q], n);
+    r |= dalloc(c, &c->sgs[q], n);
+  }
+  for (int q = 0; q < 6; ++q) r |= dalloc(c, &c->lss[q], n);
+  r |= dalloc(c, &c->ufar, 3L * (prm->nj + 1) * (prm->nk + 1));
+  r |= dalloc(c, &c->wall, 12L * (prm->nj - 1) * (prm->nk - 1));
+  long nblk = (long)((prm->ni - 1 + 63) / 64) * ((prm->nj - 1 + 3) / 4) * (prm->nk - 1);
+  c->nred = (int)std::max(nblk * 2, 4096L);
+  r |= dalloc(c, &c->red, c->nred);
+  r |= dalloc(c, &c->scal, 64);
+  double *p32 = nullptr, *su32 = nullptr;
+  r |= dalloc(c, &p32, (n + 1) / 2);
+  r |= dalloc(c, &su32, (n + 1) / 2);
+  c->p32 = (float*)p32;
+  c->su32 = (float*)su32;
+  if (r) { fvm_destroy(c); return FVM_ERR_HIP; }
+  if (hipHostMalloc((void**)&c->hscal, 64 * sizeof(double)) != hipSuccess) {
+    fvm_destroy(c);
+    return FVM_ERR_HIP;
+  }
+  // device pointer tables
+  double** tabs[5];
+  for (int t = 0; t < 5; ++t) {
+    if (hipMalloc((void**)&tabs[t], 9 * sizeof(double*)) != hipSuccess) { fvm_destroy(c); return FVM_ERR_HIP; }
+    c->allocs.push_back(tabs[t]);
+  }
+  c->dm = tabs[0];
+  c->dlss = tabs[1];
+  for (int s = 0; s < 3; ++s) c->dfa[s] = tabs[2 + s];
+  (void)hipMemcpy(c->dm, c->m, 9 * sizeof(double*), hipMemcpyHostToDevice);
+  (void)hipMemcpy(c->dlss, c->lss, 6 * sizeof(double*), hipMemcpyHostToDevice);
+  for (int s = 0; s < 3; ++s) (void)hipMemcpy(c->dfa[s], c->fa[s], 3 * sizeof(double*), hipMemcpyHostToDevice);
+  // node coordinates: NI*NJ*NK host records -> cell layout, nodes 1..N
+  const int ni = prm->ni, nj = prm->nj, nk = prm->nk;
+  std::vector<double> h(n, 0.0);
+  const double* src[3] = {xg, yg, zg};
+  double* dst[3] = {c->xg, c->yg, c->zg};
+  for (int a = 0; a < 3; ++a) {
+    for (int k = 0; k < nk; ++k)
+      for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni; ++i)
+          h[cidx(c->d, i + 1, j + 1, k + 1)] = src[a][i + (long)ni * (j + (long)nj * k)];
+    if (hipMemcpy(dst[a], h.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      fvm_destroy(c);
+      return FVM_ERR_HIP;
+    }
+  }
+  r = fvm_geom_setup(c);
+  if (r) { fvm_destroy(c); return r; }
+  *out = c;
+  return FVM_OK;
+}
+
+int fvm_destroy(fvm_ctx* c) {
+  if (!c) return FVM_ERR_ARG;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev_pending) {
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return FVM_OK;
+}
+
+int fvm_set_field(fvm_ctx* c, int field, const double* host) {
+  if (!c || !host) return FVM_ERR_ARG;
+  double* dptr = field_ptr(c, field);
+  if (!dptr) return FVM_ERR_ARG;
+  HIP_CHECK(hipMemcpyAsync(dptr, host, c->d.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (field >= FVM_NLX && field <= FVM_NLZ)
+    HIP_CHECK(hipMemcpyAsync(c->nl1[field - FVM_NLX], host, c->d.n * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return FVM_OK;
+}
+
+int fvm_get_field(fvm_ctx* c, int field, double* host) {
+  if (!c || !host) return FVM_ERR_ARG;
+  double* dptr = field_ptr(c, field);
+  if (!dptr) return FVM_ERR_ARG;
+  HIP_CHECK(hipMemcpyAsync(host, dptr, c->d.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return FVM_OK;
+}
+
+int fvm_begin_step(fvm_ctx* c) {
+  if (!c) return FVM_ERR_ARG;
+  size_t b = c->d.n * sizeof(double);
+  HIP_CHECK(hipMemcpyAsync(c->un, c->u, b, hipMemcpyDeviceToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->vn, c->v, b, hipMemcpyDeviceToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->wn, c->w, b, hipMemcpyDeviceToDevice, c->stream));
+  // far-field boundary values depend only on U_N: one evaluation serves every BCU of the step
+  return fvm_k_ufar(c);
+}
+
+int fvm_contra_vel(fvm_ctx* c) { return c ? fvm_k_contra_vel(c) : FVM_ERR_ARG; }
+int fvm_nl(fvm_ctx* c) { return c ? fvm_k_nl(c) : FVM_ERR_ARG; }
+int fvm_dif(fvm_ctx* c) { return c ? fvm_k_dif(c) : FVM_ERR_ARG; }
+int fvm_p_source(fvm_ctx* c) { return c ? fvm_k_p_source(c) : FVM_ERR_ARG; }
+int fvm_iter_u(fvm_ctx* c) { return c ? fvm_k_iter_u(c) : FVM_ERR_ARG; }
+
+int fvm_stress_tensor_dy(fvm_ctx* c, double* c_avg) {
+  if (!c) return FVM_ERR_ARG;
+  int r = fvm_k_sgs(c);
+  if (r) return r;
+  if (c_avg) {
+    HIP_CHECK(hipMemcpyAsync(c_avg, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
+  return FVM_OK;
+}
+
+int fvm_correct(fvm_ctx* c, double sign) {
+  if (!c) return FVM_ERR_ARG;
+  int r = fvm_k_correct(c, sign);
+  if (r) return r;
+  double* f[3] = {c->u, c->v, c->w};
+  return fvm_k_bcu_ghosts(c, f, 3);
+}
+
+int fvm_iter_p(fvm_ctx* c, double* resip, int* niter) {
+  if (!c) return FVM_ERR_ARG;
+  double res = 0.0;
+  int it = 0;
+  int r = fvm_k_iter_p(c, &res, &it);
+  if (resip) *resip = res;
+  if (niter) *niter = it;
+  return r;
+}
+
+int fvm_pressure_reference(fvm_ctx* c, double* pref) {
+  return c ? fvm_k_pref(c, pref) : FVM_ERR_ARG;
+}
+
+int fvm_calccoef(fvm_ctx* c, fvm_step_info* info) {
+  if (!c) return FVM_ERR_ARG;
+  double o[8];
+  int r = fvm_k_calccoef(c, o);
+  if (r) return r;
+  if (info) {
+    info->cfl = o[0]; info->cpb = o[1]; info->cd = o[2]; info->cl = o[3];
+    info->cd_p = o[4]; info->cd_v = o[5]; info->cl_p = o[6]; info->cl_v = o[7];
+  }
+  return FVM_OK;
+}
+
+int fvm_step(fvm_ctx* c, fvm_step_info* info) {
+  if (!c) return FVM_ERR_ARG;
+  fvm_step_info st{};
+  int r;
+#define CK(x) do { r = (x); if (r && r != FVM_ERR_NOCONV) return r; } while (0)
+  CK(fvm_begin_step(c));                          // main.f90:42-45
+  CK(fvm_k_contra_vel(c));                        // :48
+  CK(fvm_k_nl(c));                                // :51-57
+  if (c->prm.use_les) {                           // :59-69
+    CK(fvm_k_sgs(c));
+  }
+  CK(fvm_k_dif(c));                               // :72-75
+  CK(fvm_k_p_source(c));                          // :78
+  CK(fvm_k_iter_u(c));                            // :82-88
+  CK(fvm_correct(c, -1.0));                       // :90-98
+  CK(fvm_k_contra_vel(c));                        // :101
+  double res = 0.0;
+  int it = 0;
+  int rp = fvm_k_iter_p(c, &res, &it);            // :103-105 (BCP included)
+  if (rp && rp != FVM_ERR_NOCONV) return rp;
+  CK(fvm_k_p_source(c));                          // :108
+  CK(fvm_k_correct(c, +1.0));                     // :109-111
+  CK(fvm_k_pref(c, nullptr));                     // :125-135
+  double* f[3] = {c->u, c->v, c->w};
+  CK(fvm_k_bcu_ghosts(c, f, 3));                  // :138-142
+  CK(fvm_k_calccoef(c, nullptr));                 // :150
+#undef CK
+  HIP_CHECK(hipMemcpyAsync(c->hscal, c->scal, 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  st.c_avg = c->prm.use_les ? c->hscal[0] : 0.0;
+  st.pref = c->hscal[2];
+  st.cfl = c->hscal[8]; st.cpb = c->hscal[9]; st.cd = c->hscal[10]; st.cl = c->hscal[11];
+  st.cd_p = c->hscal[12]; st.cd_v = c->hscal[13]; st.cl_p = c->hscal[14]; st.cl_v = c->hscal[15];
+  st.resip = res;
+  st.niter = it;
+  st.nouter = c->last.nouter;
+  c->last = st;
+  if (info) *info = st;
+  prof_flush(c);
+  if (!(st.cd == st.cd) || !(st.cfl == st.cfl)) return FVM_ERR_NAN;
+  return rp;
+}
+
+int fvm_profile_enable(fvm_ctx* c, int on) {
+  if (!c) return FVM_ERR_ARG;
+  prof_flush(c);
+  c->prof_on = on != 0;
+  return FVM_OK;
+}
+
+int fvm_profile_reset(fvm_ctx* c) {
+  if (!c) return FVM_ERR_ARG;
+  prof_flush(c);
+  c->prof.clear();
+  return FVM_OK;
+}
+
+int fvm_profile_get(fvm_ctx* c, char* names, double* ms, long* calls, int cap) {
+  if (!c) return -1;
+  prof_flush(c);
+  int q = 0;
+  for (auto& kv : c->prof) {
+    if (q >= cap) break;
+    if (names) {
+      strncpy(names + 48 * q, kv.first.c_str(), 47);
+      names[48 * q + 47] = 0;
+    }
+    if (ms) ms[q] = kv.second.ms;
+    if (calls) calls[q] = kv.second.calls;
+    ++q;
+  }
+  return q;
+}
+
+int fvm_sync(fvm_ctx* c) {
+  if (!c) return FVM_ERR_ARG;
+  prof_flush(c);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return FVM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+// Shared definitions of the MI355X fractional-step solver: grid dimensions, HBM layout,
+// launch helpers and the in-wavefront parallel-cyclic-reduction line solver.
+//
+// HBM layout: every cell field is one FP64 array of (NI+1)(NJ+1)(NK+1) values, I fastest,
+// holding indices 0..NI, 0..NJ, 0..NK exactly as the reference's (IS:IE,JS:JE,KS:KE) arrays with
+// one MPI rank (I = radial, J = circumferential, K = span; index 0 and N are ghosts).  The
+// column-major order makes the reference's restart records a straight copy.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#define FVM_OK 0
+#define FVM_ERR_ARG 1
+#define FVM_ERR_HIP 2
+#define FVM_ERR_NAN 3
+#define FVM_ERR_NOCONV 4
+#define FVM_ERR_SIZE 5
+
+struct Dims {
+  int ni, nj, nk;     // node counts NI, NJ, NK
+  long sj, sk, n;     // strides of J and K, number of cells incl. ghosts
+};
+
+__host__ __device__ inline long cidx(const Dims& d, int i, int j, int k) {
+  return (long)i + d.sj * (long)j + d.sk * (long)k;
+}
+
+// Maximum line length handled by one 64-lane wavefront (8 rows per lane).
+#define FVM_MAX_LINE 512
+
+#define HIP_CHECK(x)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) {                                                            \
+      fprintf(stderr, "fvm_amd: HIP error %s at %s:%d\n", hipGetErrorString(e_),       \
+              __FILE__, __LINE__);                                                     \
+      return FVM_ERR_HIP;                                                              \
+    }                                                                                  \
+  } while (0)
+
+// Parallel cyclic reduction of one tridiagonal system by one 64-thread workgroup.
+// Rows r = 0..n-1 live in LDS: a[r] x[r-1] + b[r] x[r] + c[r] x[r+1] = d[r] with a[0] = 0 and
+// c[n-1] = 0.  Lane t owns rows t, t+64, ... (R rows).  On return d[r] holds x[r].
+// Each step of stride s eliminates the couplings to rows r-s and r+s; after ceil(log2 n)
+// steps every row is decoupled.  Stable for the diagonally dominant systems of the AF and
+// Poisson line solves.
+template <int R, typename T>
+__device__ inline void pcr_solve(T* a, T* b, T* c, T* d, int n) {
+  const int t = threadIdx.x;
+  for (int s = 1; s < n; s <<= 1) {
+    T na[R], nb[R], nc[R], nd[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int r = t + 64 * q;
+      na[q] = nb[q] = nc[q] = nd[q] = T(0);
+      if (r < n) {
+        T ar = a[r], br = b[r], cr = c[r], dr = d[r];
+        T xa = T(0), xc = T(0), xd = T(0), ya = T(0), yc = T(0), yd = T(0);
+        T alpha = T(0), gamma = T(0);
+        if (r >= s) {
+          alpha = -ar / b[r - s];
+          xa = a[r - s]; xc = c[r - s]; xd = d[r - s];
+        }
+        if (r + s < n) {
+          gamma = -cr / b[r + s];
+          ya = a[r + s]; yc = c[r + s]; yd = d[r + s];
+        }
+        na[q] = alpha * xa;
+        nc[q] = gamma * yc;
+        nb[q] = br + alpha * xc + gamma * ya;
+        nd[q] = dr + alpha * xd + gamma * yd;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int r = t + 64 * q;
+      if (r < n) { a[r] = na[q]; b[r] = nb[q]; c[r] = nc[q]; d[r] = nd[q]; }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int r = t + 64 * q;
+    if (r < n) d[r] = d[r] / b[r];
+  }
+  __syncthreads();
+}
+
+// Dispatch on rows per lane.
+template <typename T>
+__device__ inline void pcr_solve_any(T* a, T* b, T* c, T* d, int n) {
+  const int R = (n + 63) >> 6;
+  switch (R) {
+    case 1: pcr_solve<1>(a, b, c, d, n); break;
+    case 2: pcr_solve<2>(a, b, c, d, n); break;
+    case 3: pcr_solve<3>(a, b, c, d, n); break;
+    case 4: pcr_solve<4>(a, b, c, d, n); break;
+    case 5: pcr_solve<5>(a, b, c, d, n); break;
+    case 6: pcr_solve<6>(a, b, c, d, n); break;
+    case 7: pcr_solve<7>(a, b, c, d, n); break;
+    default: pcr_solve<8>(a, b, c, d, n); break;
+  }
+}
+
+// Periodic ghost copy of one cell array: J ghosts (branch cut) and K ghosts (span) take the
+// value of the periodic partner for whole I rows (LES/v2.3_LES-Reynolds stresses/mpi_x.f90:26-53
+// with one rank); only the ghost surface is enumerated.
+template <typename T>
+__global__ void k_ghost_fill_t(Dims d, T* __restrict__ a) {
+  // periodic J then K ghosts of every I row: a(i,j,k) = a(i, wrap j, wrap k)
+  const long nJ = (long)(d.ni + 1) * 2 * (d.nk + 1);
+  const long nK = (long)(d.ni + 1) * (d.nj + 1) * 2;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nJ + nK;
+       t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    if (t < nJ) {
+      i = (int)(t % (d.ni + 1));
+      long r = t / (d.ni + 1);
+      j = (r % 2) == 0 ? 0 : d.nj;
+      k = (int)(r / 2);
+      if (k == 0 || k == d.nk) continue;  // corners come from the K pass
+    } else {
+      long q = t - nJ;
+      i = (int)(q % (d.ni + 1));
+      long r = q / (d.ni + 1);
+      j = (int)(r % (d.nj + 1));
+      k = (r / (d.nj + 1)) == 0 ? 0 : d.nk;
+    }
+    int js = j == 0 ? d.nj - 1 : (j == d.nj ? 1 : j);
+    int ks = k == 0 ? d.nk - 1 : (k == d.nk ? 1 : k);
+    a[cidx(d, i, j, k)] = a[cidx(d, i, js, ks)];
+  }
+}
+
+// 1D launch over n items.
+inline dim3 grid1(long n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
+
+inline long ghost_count(const Dims& d) {
+  return (long)(d.ni + 1) * 2 * (d.nk + 1) + (long)(d.ni + 1) * (d.nj + 1) * 2;
+}

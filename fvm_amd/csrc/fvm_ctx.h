@@ -1,0 +1,102 @@
+// Device context of the solver: every field, metric and scratch array lives in HBM for the
+// whole run (allocated once at fvm_create, freed at fvm_destroy), all work is ordered on one
+// HIP stream owned by the context.
+#pragma once
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fvm_common.h"
+#include "../../include/fvm_capi.h"
+
+struct ProfAcc {
+  double ms = 0.0;
+  long calls = 0;
+};
+
+struct FvmCtx {
+  fvm_params prm;
+  Dims d;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  double pi = 0.0;
+
+  // geometry (node arrays use the cell layout and hold nodes 1..NI, 1..NJ, 1..NK)
+  double *xg = nullptr, *yg = nullptr, *zg = nullptr, *x = nullptr, *y = nullptr, *z = nullptr;
+  double* djac = nullptr;
+  double* m[9] = {};  // metric grouped per Cartesian component s: (xi_s, eta_s, zeta_s), s=x,y,z
+  double** dm = nullptr;    // the same nine pointers as a device-resident table
+  double *jq11 = nullptr, *jq22 = nullptr, *jq33 = nullptr;  // J^-1 Q_ii
+  double *jq12 = nullptr, *jq13 = nullptr, *jq23 = nullptr;  // J^-1 Q_ij (cross terms)
+  double* theta = nullptr;
+  double* filt2 = nullptr;  // test-filter width squared, (J^-1)^(2/3)
+  double* ap0 = nullptr;    // 2 Re J^-1 / dt
+  double* fa[3][3] = {};    // AF face coefficients per stage s and direction: I, J, K faces
+  double** dfa[3] = {};     // device tables of fa[s]
+  double *pdI = nullptr, *pdJ = nullptr, *pdK = nullptr;  // Poisson face coefficients D_E, D_N, D_T
+  double* wall = nullptr;   // per wall triangle: A, B, C, |n|, d_n, area (6 x 2 x (NJ-1)(NK-1))
+
+  // fields
+  double *u = nullptr, *v = nullptr, *w = nullptr, *p = nullptr;
+  double *un = nullptr, *vn = nullptr, *wn = nullptr;
+  double* nl[3] = {};
+  double* nl1[3] = {};
+  double* dif[3] = {};
+  double* s[3] = {};
+  double* su[3] = {};
+  double* ju[3] = {};   // J^-1 x contravariant velocity (volume-flux base)
+  double* ufar = nullptr;  // far-field boundary value per component, (NJ+1)(NK+1) each
+  double* rhs = nullptr;   // Poisson right-hand side div/dt
+  double* psu = nullptr;   // Poisson explicit source rhs - cross terms
+  float* p32 = nullptr;    // FP32 correction of the mixed-precision Poisson solve
+  float* su32 = nullptr;   // FP32 defect
+  // LES
+  double* lss[6] = {};
+  double** dlss = nullptr;  // device table of lss
+  double* mag = nullptr;
+  double* uh[3] = {};
+  double* nut = nullptr;
+  double* sgs[3] = {};
+  // reductions
+  double* red = nullptr;     // partial sums
+  double* scal = nullptr;    // device scalars (c_avg, residual sums, pref, ...)
+  double* hscal = nullptr;   // pinned host mirror of scal
+
+  int nred = 0;
+  // last-step diagnostics
+  fvm_step_info last{};
+  // profiling
+  bool prof_on = false;
+  std::map<std::string, ProfAcc> prof;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  size_t ev_next = 0;
+  std::vector<void*> allocs;
+};
+
+// Kernel timing: wraps one launch in a pair of events when profiling is on.
+struct ProfScope {
+  FvmCtx* c;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(FvmCtx* ctx, const char* nm);
+  ~ProfScope();
+};
+
+// stage launchers (return FVM_OK or an error code)
+int fvm_geom_setup(FvmCtx* c);
+int fvm_ghost_fill(FvmCtx* c, double* a);
+int fvm_ghost_fill_n(FvmCtx* c, double** a, int cnt);
+int fvm_k_contra_vel(FvmCtx* c);
+int fvm_k_nl(FvmCtx* c);
+int fvm_k_dif(FvmCtx* c);
+int fvm_k_p_source(FvmCtx* c);
+int fvm_k_ufar(FvmCtx* c);
+int fvm_k_bcu_ghosts(FvmCtx* c, double** f, int cnt);
+int fvm_k_correct(FvmCtx* c, double sign);
+int fvm_k_pref(FvmCtx* c, double* pref_host);
+int fvm_k_calccoef(FvmCtx* c, double* out8);
+int fvm_k_sgs(FvmCtx* c);
+int fvm_k_iter_u(FvmCtx* c);
+int fvm_k_iter_p(FvmCtx* c, double* resip, int* niter);
+int fvm_k_bcp(FvmCtx* c);

@@ -1,0 +1,444 @@
+// Explicit stencils of the time step on interior cells, boundary conditions, projection,
+// pressure reference and force coefficients.
+//
+// Every kernel evaluates the reference's expressions in the reference's association order
+// (FP contraction is off for this library), so its FP64 results equal the reference's bit for
+// bit where the operation sequence is the same.
+#include "fvm_ctx.h"
+
+namespace {
+
+// interior launch: x -> i (64 wide), y -> j (4), z -> k
+inline dim3 interior_grid(const Dims& d, int nz = 1) {
+  return dim3((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1) * nz);
+}
+const dim3 kBlk(64, 4, 1);
+
+#define INTERIOR_IJK(d)                                   \
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;        \
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;         \
+  const int k = 1 + (int)(blockIdx.z % (d.nk - 1));       \
+  if (i > d.ni - 1 || j > d.nj - 1) return;               \
+  const long c = cidx(d, i, j, k);
+
+// Contravariant velocity times J^-1 on i = 0..NI, interior j,k (RLES/contra_vel.f90:13-23);
+// the face volume fluxes are 0.5 (JU(c) + JU(c+1)) (contra_vel.f90:37-53).
+__global__ void k_contra(Dims d, const double* const* __restrict__ m, const double* __restrict__ djac,
+                         const double* __restrict__ u, const double* __restrict__ v,
+                         const double* __restrict__ w, double* ju0, double* ju1, double* ju2) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ni || j > d.nj - 1) return;
+  const long c = cidx(d, i, j, k);
+  double U = u[c], V = v[c], W = w[c], J = djac[c];
+  double u1 = m[0][c] * U + m[3][c] * V + m[6][c] * W;
+  double u2 = m[1][c] * U + m[4][c] * V + m[7][c] * W;
+  double u3 = m[2][c] * U + m[5][c] * V + m[8][c] * W;
+  ju0[c] = J * u1;
+  ju1[c] = J * u2;
+  ju2[c] = J * u3;
+}
+
+struct Faces {
+  double ue, uw, vn, vs, wt, wb;
+};
+
+__device__ inline Faces faces(const Dims& d, int i, long c, const double* __restrict__ ju0,
+                              const double* __restrict__ ju1, const double* __restrict__ ju2) {
+  Faces f;
+  f.ue = 0.5 * (ju0[c] + ju0[c + 1]);
+  f.uw = i == 1 ? 0.0 : 0.5 * (ju0[c] + ju0[c - 1]);  // UW(1) = 0 (contra_vel.f90:56)
+  f.vn = 0.5 * (ju1[c] + ju1[c + d.sj]);
+  f.vs = 0.5 * (ju1[c] + ju1[c - d.sj]);
+  f.wt = 0.5 * (ju2[c] + ju2[c + d.sk]);
+  f.wb = 0.5 * (ju2[c] + ju2[c - d.sk]);
+  return f;
+}
+
+// NL(IN,OUT) for the three velocity components (RLES/nl.f90:23-31)
+__global__ void k_nl(Dims d, const double* __restrict__ ju0, const double* __restrict__ ju1,
+                     const double* __restrict__ ju2, const double* __restrict__ un,
+                     const double* __restrict__ vn, const double* __restrict__ wn, double* nlx,
+                     double* nly, double* nlz) {
+  INTERIOR_IJK(d);
+  Faces F = faces(d, i, c, ju0, ju1, ju2);
+  const double* in[3] = {un, vn, wn};
+  double* out[3] = {nlx, nly, nlz};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double* f = in[q];
+    double fc = f[c];
+    double t = F.ue * (f[c + 1] + fc);
+    t = t - F.uw * (f[c - 1] + fc);
+    t = t + F.vn * (f[c + d.sj] + fc);
+    t = t - F.vs * (f[c - d.sj] + fc);
+    t = t + F.wt * (f[c + d.sk] + fc);
+    t = t - F.wb * (f[c - d.sk] + fc);
+    out[q][c] = 0.5 * t;
+  }
+}
+
+// explicit cross-derivative (non-orthogonal) diffusion terms DIF_DIAGONAL
+// (RLES/dif_diagonal.f90:13-32); jqAB = J^-1 Q_AB
+__device__ inline double cross_terms(const Dims& d, long c, const double* __restrict__ f,
+                                     const double* __restrict__ jq12,
+                                     const double* __restrict__ jq13,
+                                     const double* __restrict__ jq23) {
+  const long sj = d.sj, sk = d.sk;
+  const long e = c + 1, w = c - 1, n = c + sj, s = c - sj, t = c + sk, b = c - sk;
+  double r = jq12[e] * (f[e + sj] - f[e - sj]);
+  r = r - jq12[w] * (f[w + sj] - f[w - sj]);
+  r = r + jq13[e] * (f[e + sk] - f[e - sk]);
+  r = r - jq13[w] * (f[w + sk] - f[w - sk]);
+  r = r + jq12[n] * (f[n + 1] - f[n - 1]);
+  r = r - jq12[s] * (f[s + 1] - f[s - 1]);
+  r = r + jq23[n] * (f[n + sk] - f[n - sk]);
+  r = r - jq23[s] * (f[s + sk] - f[s - sk]);
+  r = r + jq13[t] * (f[t + 1] - f[t - 1]);
+  r = r - jq13[b] * (f[b + 1] - f[b - 1]);
+  r = r + jq23[t] * (f[t + sj] - f[t - sj]);
+  r = r - jq23[b] * (f[b + sj] - f[b - sj]);
+  return 0.25 * r;
+}
+
+// DIF(IN,OUT) for the three components (RLES/dif.f90:13-23)
+__global__ void k_dif(Dims d, const double* __restrict__ pdI, const double* __restrict__ pdJ,
+                      const double* __restrict__ pdK, const double* __restrict__ jq12,
+                      const double* __restrict__ jq13, const double* __restrict__ jq23,
+                      const double* __restrict__ un, const double* __restrict__ vn,
+                      const double* __restrict__ wn, double* dx, double* dy, double* dz) {
+  INTERIOR_IJK(d);
+  const double de = pdI[c], dw = pdI[c - 1], dn = pdJ[c], ds = pdJ[c - d.sj], dt = pdK[c],
+               db = pdK[c - d.sk];
+  const double* in[3] = {un, vn, wn};
+  double* out[3] = {dx, dy, dz};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double* f = in[q];
+    double fc = f[c];
+    double t = de * (f[c + 1] - fc);
+    t = t + dw * (f[c - 1] - fc);
+    t = t + dn * (f[c + d.sj] - fc);
+    t = t + ds * (f[c - d.sj] - fc);
+    t = t + dt * (f[c + d.sk] - fc);
+    t = t + db * (f[c - d.sk] - fc);
+    t = t + cross_terms(d, c, f, jq12, jq13, jq23);
+    out[q][c] = 1.0 * t;
+  }
+}
+
+// P_SOURCE(IN): S = -0.5 sum_dir [(J xi_s p)_+ - (J xi_s p)_-] (RLES/p_source.f90:14-42)
+__global__ void k_psource(Dims d, const double* const* __restrict__ m,
+                          const double* __restrict__ djac, const double* __restrict__ p,
+                          double* s0, double* s1, double* s2) {
+  INTERIOR_IJK(d);
+  double* out[3] = {s0, s1, s2};
+  const long sj = d.sj, sk = d.sk;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double *a = m[3 * q], *b = m[3 * q + 1], *g = m[3 * q + 2];
+    double t = djac[c + 1] * a[c + 1] * p[c + 1];
+    t = t - djac[c - 1] * a[c - 1] * p[c - 1];
+    t = t + djac[c + sj] * b[c + sj] * p[c + sj];
+    t = t - djac[c - sj] * b[c - sj] * p[c - sj];
+    t = t + djac[c + sk] * g[c + sk] * p[c + sk];
+    t = t - djac[c - sk] * g[c - sk] * p[c - sk];
+    out[q][c] = -(0.5 * t);
+  }
+}
+
+// Far-field boundary value U_F per (j,k) and component: inflow Dirichlet UIN where the ghost
+// centre's angle lies in [pi/2, 3pi/2], else the convective outflow condition advanced with
+// the step-start velocity B = U_N (RLES/bcu.f90:27-37).  Constant within a step.
+__global__ void k_ufar(Dims d, const double* const* __restrict__ m, const double* __restrict__ theta,
+                       const double* __restrict__ un, const double* __restrict__ vn,
+                       const double* __restrict__ wn, double dtime, double pi, double* ufar) {
+  const long nf = (long)(d.nj + 1) * (d.nk + 1);
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < 3 * nf;
+       t += (long)gridDim.x * blockDim.x) {
+    int comp = (int)(t / nf);
+    long r = t - comp * nf;
+    int j = (int)(r % (d.nj + 1)), k = (int)(r / (d.nj + 1));
+    if (j < 1 || j > d.nj - 1 || k < 1 || k > d.nk - 1) { ufar[t] = 0.0; continue; }
+    const double* b = comp == 0 ? un : (comp == 1 ? vn : wn);
+    const double uin = comp == 0 ? 1.0 : 0.0;
+    long c = cidx(d, d.ni, j, k), mm = c - 1;
+    double th = theta[c], uf;
+    if (th >= 0.5 * pi && th <= 1.5 * pi) {
+      uf = uin;
+    } else {
+      const double *xix = m[0], *etx = m[1], *ztx = m[2];
+      const long sj = d.sj, sk = d.sk;
+      double t1 = 0.5 * (xix[c] + xix[mm]) * (b[c] - b[mm]);
+      double t2 = 0.5 * (etx[c] + etx[mm]) * 0.25 * (b[c + sj] + b[mm + sj] - b[c - sj] - b[mm - sj]);
+      double t3 = 0.5 * (ztx[c] + ztx[mm]) * 0.25 * (b[c + sk] + b[mm + sk] - b[c - sk] - b[mm - sk]);
+      uf = 0.5 * (b[c] + b[mm]) - dtime * 1.0 * (t1 + t2 + t3);
+    }
+    ufar[t] = uf;
+  }
+}
+
+// BCU ghost values (RLES/bcu.f90:21-22, 41-42): wall no-slip mirror, far-field mirror about
+// U_F; the periodic J/K ghosts follow (fvm_ghost_fill).
+__global__ void k_bcu_ghosts(Dims d, double* f, const double* __restrict__ uf) {
+  const long nf = (long)(d.nj - 1) * (d.nk - 1);
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nf;
+       t += (long)gridDim.x * blockDim.x) {
+    int j = 1 + (int)(t % (d.nj - 1)), k = 1 + (int)(t / (d.nj - 1));
+    long c0 = cidx(d, 0, j, k), cn = cidx(d, d.ni, j, k);
+    f[c0] = -f[c0 + 1] + 2.0 * 0.0;
+    f[cn] = -f[cn - 1] + 2.0 * uf[j + (long)(d.nj + 1) * k];
+  }
+}
+
+// U = U + sign DTIME S / J^-1 (RLES/main.f90:90-92 with sign -1, 109-111 with +1)
+__global__ void k_correct(Dims d, double sign, double dtime, const double* __restrict__ djac,
+                          double* u, double* v, double* w, const double* __restrict__ s0,
+                          const double* __restrict__ s1, const double* __restrict__ s2) {
+  INTERIOR_IJK(d);
+  double J = djac[c];
+  if (sign < 0) {
+    u[c] = u[c] - dtime * s0[c] / J;
+    v[c] = v[c] - dtime * s1[c] / J;
+    w[c] = w[c] - dtime * s2[c] / J;
+  } else {
+    u[c] = u[c] + dtime * s0[c] / J;
+    v[c] = v[c] + dtime * s1[c] / J;
+    w[c] = w[c] + dtime * s2[c] / J;
+  }
+}
+
+// span mean of p at (IE, NJ/4, K), summed in the reference's order by one lane
+// (RLES/main.f90:125-131)
+__global__ void k_pref(Dims d, const double* __restrict__ p, double* out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 1; k <= d.nk - 1; ++k) s = s + p[cidx(d, d.ni, d.nj / 4, k)];
+    out[0] = s / (double)(d.nk - 1);
+  }
+}
+
+__global__ void k_sub_pref(Dims d, double* p, const double* __restrict__ pref) {
+  const double r = pref[0] / 1.0;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < d.n;
+       q += (long)gridDim.x * blockDim.x)
+    p[q] = p[q] - r;
+}
+
+// ---------------------------------------------------------------------------------------
+// CALCCOEF (RLES/calccoef.f90:11-201)
+// ---------------------------------------------------------------------------------------
+__device__ inline double block_reduce_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = sh[t] + sh[t + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ inline double block_reduce_max(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// local CFL number max over the interior, per block partial maxima
+__global__ void k_cfl(Dims d, const double* const* __restrict__ m, const double* __restrict__ u,
+                      const double* __restrict__ v, const double* __restrict__ w, double dtime,
+                      double* part) {
+  __shared__ double sh[256];
+  const long nint = (long)(d.ni - 1) * (d.nj - 1) * (d.nk - 1);
+  double mx = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nint;
+       t += (long)gridDim.x * blockDim.x) {
+    int i = 1 + (int)(t % (d.ni - 1));
+    long r = t / (d.ni - 1);
+    int j = 1 + (int)(r % (d.nj - 1)), k = 1 + (int)(r / (d.nj - 1));
+    long c = cidx(d, i, j, k);
+    double U = u[c], V = v[c], W = w[c];
+    double a = fabs(m[0][c] * U + m[3][c] * V + m[6][c] * W);
+    double b = fabs(m[1][c] * U + m[4][c] * V + m[7][c] * W);
+    double e = fabs(m[2][c] * U + m[5][c] * V + m[8][c] * W);
+    double l = dtime * (a + b + e);
+    mx = mx > l ? mx : l;
+  }
+  double r = block_reduce_max(mx, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// base pressure and surface forces: one block of 256 lanes, fixed-order tree per quantity
+__global__ void k_forces(Dims d, const double* __restrict__ wall, const double* __restrict__ p,
+                         const double* __restrict__ u, const double* __restrict__ v,
+                         const double* __restrict__ w, double re, double ref_a,
+                         const double* __restrict__ cflpart, int ncfl, double* out) {
+  __shared__ double sh[256];
+  const long nw = (long)(d.nj - 1) * (d.nk - 1);
+  double pd = 0.0, ply = 0.0, td = 0.0, tly = 0.0;
+  for (long t = threadIdx.x; t < 2 * nw; t += blockDim.x) {
+    int tri = (int)(t / nw);
+    long r = t - tri * nw;
+    int j = 1 + (int)(r % (d.nj - 1)), k = 1 + (int)(r / (d.nj - 1));
+    long c = cidx(d, 1, j, k);
+    const double* o = wall + 6 * t;
+    double A = o[0], B = o[1], C = o[2], dum = o[3], dn = o[4], ar = o[5];
+    double U = u[c], V = v[c], W = w[c];
+    double t1 = V * C / dum - W * B / dum, t2 = U * C / dum - W * A / dum,
+           t3 = U * B / dum - V * A / dum;
+    double dudn = sqrt(t1 * t1 + t2 * t2 + t3 * t3);
+    dudn = dudn / dn;
+    double um = sqrt(U * U + V * V + W * W);
+    pd = pd - 2.0 * p[c] * ar * A / dum;
+    ply = ply - 2.0 * p[c] * ar * B / dum;
+    td = td + (2.0 / re) * dudn * ar * U / um;
+    tly = tly + (2.0 / re) * dudn * ar * V / um;
+  }
+  pd = block_reduce_sum(pd, sh);
+  ply = block_reduce_sum(ply, sh);
+  td = block_reduce_sum(td, sh);
+  tly = block_reduce_sum(tly, sh);
+  double cmax = 0.0;
+  for (int q = threadIdx.x; q < ncfl; q += blockDim.x) cmax = fmax(cmax, cflpart[q]);
+  cmax = block_reduce_max(cmax, sh);
+  if (threadIdx.x == 0) {
+    // base pressure: 8-cell average at the first wall node line, span mean (calccoef.f90:39-74)
+    double dumb = 0.0;
+    for (int k = 1; k <= d.nk - 1; ++k) {
+      long c = cidx(d, 1, 1, k);
+      const long sj = d.sj, sk = d.sk;
+      double pg = 0.125 * (p[c] + p[c - sj] + p[c - 1] + p[c - 1 - sj] + p[c + sk] +
+                           p[c - sj + sk] + p[c - 1 + sk] + p[c - 1 - sj + sk]);
+      dumb = dumb + pg;
+    }
+    dumb = dumb / (double)(d.nk - 1);
+    double cdp = pd / ref_a, clp = ply / ref_a, cdv = td / ref_a, clv = tly / ref_a;
+    out[0] = cmax;
+    out[1] = 2.0 * (0.0 - dumb);
+    out[2] = cdp + cdv;
+    out[3] = clp + clv;
+    out[4] = cdp;
+    out[5] = cdv;
+    out[6] = clp;
+    out[7] = clv;
+  }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------------------
+// launchers
+// ----------------------------------------------------------------------------------------
+int fvm_k_contra_vel(FvmCtx* c) {
+  const Dims d = c->d;
+  double** m = c->dm;
+  {
+    ProfScope ps(c, "contra_vel");
+    dim3 g((d.ni + 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
+    hipLaunchKernelGGL(k_contra, g, kBlk, 0, c->stream, d, (const double* const*)m, c->djac,
+                       c->u, c->v, c->w, c->ju[0], c->ju[1], c->ju[2]);
+    HIP_CHECK(hipGetLastError());
+  }
+  return fvm_ghost_fill_n(c, c->ju, 3);
+}
+
+int fvm_k_nl(FvmCtx* c) {
+  for (int q = 0; q < 3; ++q) std::swap(c->nl[q], c->nl1[q]);
+  ProfScope ps(c, "nl");
+  hipLaunchKernelGGL(k_nl, interior_grid(c->d), kBlk, 0, c->stream, c->d, c->ju[0], c->ju[1],
+                     c->ju[2], c->un, c->vn, c->wn, c->nl[0], c->nl[1], c->nl[2]);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_k_dif(FvmCtx* c) {
+  ProfScope ps(c, "dif");
+  hipLaunchKernelGGL(k_dif, interior_grid(c->d), kBlk, 0, c->stream, c->d, c->pdI, c->pdJ,
+                     c->pdK, c->jq12, c->jq13, c->jq23, c->un, c->vn, c->wn, c->dif[0],
+                     c->dif[1], c->dif[2]);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_k_p_source(FvmCtx* c) {
+  double** m = c->dm;
+  ProfScope ps(c, "p_source");
+  hipLaunchKernelGGL(k_psource, interior_grid(c->d), kBlk, 0, c->stream, c->d,
+                     (const double* const*)m, c->djac, c->p, c->s[0], c->s[1], c->s[2]);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_k_ufar(FvmCtx* c) {
+  double** m = c->dm;
+  long nf = 3L * (c->d.nj + 1) * (c->d.nk + 1);
+  hipLaunchKernelGGL(k_ufar, grid1(nf, 256), dim3(256), 0, c->stream, c->d,
+                     (const double* const*)m, c->theta, c->un, c->vn, c->wn, c->prm.dtime,
+                     c->pi, c->ufar);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_k_bcu_ghosts(FvmCtx* c, double** f, int cnt) {
+  long nf = (long)(c->d.nj - 1) * (c->d.nk - 1);
+  long stride = (long)(c->d.nj + 1) * (c->d.nk + 1);
+  ProfScope ps(c, "bcu");
+  for (int q = 0; q < cnt; ++q) {
+    hipLaunchKernelGGL(k_bcu_ghosts, grid1(nf, 256), dim3(256), 0, c->stream, c->d, f[q],
+                       c->ufar + q * stride);
+    HIP_CHECK(hipGetLastError());
+    int r = fvm_ghost_fill(c, f[q]);
+    if (r) return r;
+  }
+  return FVM_OK;
+}
+
+int fvm_k_correct(FvmCtx* c, double sign) {
+  ProfScope ps(c, "correct");
+  hipLaunchKernelGGL(k_correct, interior_grid(c->d), kBlk, 0, c->stream, c->d, sign,
+                     c->prm.dtime, c->djac, c->u, c->v, c->w, c->s[0], c->s[1], c->s[2]);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_k_pref(FvmCtx* c, double* pref_host) {
+  ProfScope ps(c, "pref");
+  hipLaunchKernelGGL(k_pref, dim3(1), dim3(64), 0, c->stream, c->d, c->p, c->scal + 2);
+  hipLaunchKernelGGL(k_sub_pref, grid1(c->d.n, 256), dim3(256), 0, c->stream, c->d, c->p,
+                     c->scal + 2);
+  HIP_CHECK(hipGetLastError());
+  if (pref_host) {
+    HIP_CHECK(hipMemcpyAsync(pref_host, c->scal + 2, sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
+  return FVM_OK;
+}
+
+int fvm_k_calccoef(FvmCtx* c, double* out8) {
+  double** m = c->dm;
+  ProfScope ps(c, "calccoef");
+  const int ncfl = 512;
+  hipLaunchKernelGGL(k_cfl, dim3(ncfl), dim3(256), 0, c->stream, c->d, (const double* const*)m,
+                     c->u, c->v, c->w, c->prm.dtime, c->red);
+  hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, c->stream, c->d, c->wall, c->p, c->u,
+                     c->v, c->w, c->prm.re, c->prm.ref_a, c->red, ncfl, c->scal + 8);
+  HIP_CHECK(hipGetLastError());
+  if (out8) {
+    HIP_CHECK(hipMemcpyAsync(out8, c->scal + 8, 8 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
+  return FVM_OK;
+}

@@ -1,0 +1,308 @@
+// Dynamic Smagorinsky subgrid-scale model on the device
+// (LES/v2.3_LES-Reynolds stresses/stress_tensor_dy.f90:24-286 velocity part, test_filter.f90).
+//
+// Passes: (1) strain tensor S_ij and |S| on i = 0..NI (one-sided at the I boundaries), periodic
+// ghosts; (2) test-filtered velocity (1,2,1)^3/64 with the velocity boundary ghosts;
+// (3) per cell the Germano identity L_ij, the Lilly tensor M_ij and the contractions LM, MM,
+// with every filtered product recomputed from its 27 neighbours instead of stored;
+// (4) the reference's global conditional average C = <LM>/<2MM> over cells with LM/MM >= 0 as
+// a deterministic two-level FP64 reduction; (5) eddy viscosity and the divergence of the SGS
+// stress added to the convective term, NL += J^-1 SGS (RLES/main.f90:61-66).
+#include "fvm_ctx.h"
+
+namespace {
+
+const dim3 kBlk(64, 4, 1);
+
+__global__ void k_strain(Dims d, const double* const* __restrict__ m, const double* __restrict__ U,
+                         const double* __restrict__ V, const double* __restrict__ W,
+                         double* const* __restrict__ L, double* mag) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ni || j > d.nj - 1) return;
+  const long c = cidx(d, i, j, k);
+  const long sj = d.sj, sk = d.sk;
+  const double xix = m[0][c], etx = m[1][c], ztx = m[2][c];
+  const double xiy = m[3][c], ety = m[4][c], zty = m[5][c];
+  const double xiz = m[6][c], etz = m[7][c], ztz = m[8][c];
+#define DJ(F) (F[c + sj] - F[c - sj])
+#define DK(F) (F[c + sk] - F[c - sk])
+  double l11, l12, l13, l22, l23, l33;
+  if (i > 0 && i < d.ni) {
+#define DI(F) (F[c + 1] - F[c - 1])
+    double t;
+    t = xix * DI(U) + etx * DJ(U) + ztx * DK(U);
+    l11 = 0.5 * t;
+    t = xiy * DI(U) + ety * DJ(U) + zty * DK(U) + xix * DI(V) + etx * DJ(V) + ztx * DK(V);
+    l12 = 0.25 * t;
+    t = xiz * DI(U) + etz * DJ(U) + ztz * DK(U) + xix * DI(W) + etx * DJ(W) + ztx * DK(W);
+    l13 = 0.25 * t;
+    t = xiy * DI(V) + ety * DJ(V) + zty * DK(V);
+    l22 = 0.5 * t;
+    t = xiz * DI(V) + etz * DJ(V) + ztz * DK(V) + xiy * DI(W) + ety * DJ(W) + zty * DK(W);
+    l23 = 0.25 * t;
+    t = xiz * DI(W) + etz * DJ(W) + ztz * DK(W);
+    l33 = 0.5 * t;
+#undef DI
+  } else {
+    // one-sided radial difference on the wall / far-field ghost planes
+    const bool lo = (i == 0);
+#define DI(F) (lo ? (F[c + 1] - F[c]) : (F[c] - F[c - 1]))
+    l11 = 0.5 * (2.0 * xix * DI(U) + etx * DJ(U) + ztx * DK(U));
+    l12 = 0.25 * (2.0 * xiy * DI(U) + ety * DJ(U) + zty * DK(U) + 2.0 * xix * DI(V) +
+                  etx * DJ(V) + ztx * DK(V));
+    l13 = 0.25 * (2.0 * xiz * DI(U) + etz * DJ(U) + ztz * DK(U) + 2.0 * xix * DI(W) +
+                  etx * DJ(W) + ztx * DK(W));
+    l22 = 0.5 * (2.0 * xiy * DI(V) + ety * DJ(V) + zty * DK(V));
+    l23 = 0.25 * (2.0 * xiz * DI(V) + etz * DJ(V) + ztz * DK(V) + 2.0 * xiy * DI(W) +
+                  ety * DJ(W) + zty * DK(W));
+    l33 = 0.5 * (2.0 * xiz * DI(W) + etz * DJ(W) + ztz * DK(W));
+#undef DI
+  }
+#undef DJ
+#undef DK
+  L[0][c] = l11; L[1][c] = l12; L[2][c] = l13; L[3][c] = l22; L[4][c] = l23; L[5][c] = l33;
+  double s = l11 * l11 + l12 * l12 + l13 * l13 + l12 * l12 + l22 * l22 + l23 * l23 +
+             l13 * l13 + l23 * l23 + l33 * l33;
+  mag[c] = sqrt(2.0 * s);
+}
+
+// (1,2,1)^3/64 index-space filter in the reference's term order (test_filter.f90:13-30):
+// K-1 plane, K plane, K+1 plane; rows J-1, J, J+1; columns I-1, I, I+1.
+template <typename F>
+__device__ inline double filter27(const Dims& d, long c, F f) {
+  double t = 0.0;
+  bool first = true;
+#pragma unroll
+  for (int dk = -1; dk <= 1; ++dk)
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int di = -1; di <= 1; ++di) {
+        const int wgt = (di == 0 ? 2 : 1) * (dj == 0 ? 2 : 1) * (dk == 0 ? 2 : 1);
+        double v = f(c + di + d.sj * dj + d.sk * dk);
+        double term = wgt == 1 ? v : (double)wgt * v;
+        t = first ? term : t + term;
+        first = false;
+      }
+  return t / 64.0;
+}
+
+__global__ void k_filter_vel(Dims d, const double* __restrict__ U, const double* __restrict__ V,
+                             const double* __restrict__ W, double* uh, double* vh, double* wh) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ni - 1 || j > d.nj - 1) return;
+  const long c = cidx(d, i, j, k);
+  uh[c] = filter27(d, c, [&](long q) { return U[q]; });
+  vh[c] = filter27(d, c, [&](long q) { return V[q]; });
+  wh[c] = filter27(d, c, [&](long q) { return W[q]; });
+}
+
+// Germano/Lilly contractions LM and MM per interior cell (stress_tensor_dy.f90:236-483)
+__global__ void k_lm(Dims d, const double* const* __restrict__ m, const double* __restrict__ U,
+                     const double* __restrict__ V, const double* __restrict__ W,
+                     const double* __restrict__ UH, const double* __restrict__ VH,
+                     const double* __restrict__ WH, const double* const* __restrict__ L,
+                     const double* __restrict__ mag, const double* __restrict__ filt2,
+                     double* lm_out, double* mm_out) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ni - 1 || j > d.nj - 1) return;
+  const long c = cidx(d, i, j, k);
+  const long sj = d.sj, sk = d.sk;
+  const double xix = m[0][c], etx = m[1][c], ztx = m[2][c];
+  const double xiy = m[3][c], ety = m[4][c], zty = m[5][c];
+  const double xiz = m[6][c], etz = m[7][c], ztz = m[8][c];
+#define DI(F) (F[c + 1] - F[c - 1])
+#define DJ(F) (F[c + sj] - F[c - sj])
+#define DK(F) (F[c + sk] - F[c - sk])
+  // raw (unscaled) test-filtered strain sums; S11 = 0.5 r11, S12 = 0.25 r12, ...
+  const double r11 = xix * DI(UH) + etx * DJ(UH) + ztx * DK(UH);
+  const double r12 = xiy * DI(UH) + ety * DJ(UH) + zty * DK(UH) + xix * DI(VH) + etx * DJ(VH) +
+                     ztx * DK(VH);
+  const double r13 = xiz * DI(UH) + etz * DJ(UH) + ztz * DK(UH) + xix * DI(WH) + etx * DJ(WH) +
+                     ztx * DK(WH);
+  const double r22 = xiy * DI(VH) + ety * DJ(VH) + zty * DK(VH);
+  const double r23 = xiz * DI(VH) + etz * DJ(VH) + ztz * DK(VH) + xiy * DI(WH) + ety * DJ(WH) +
+                     zty * DK(WH);
+  const double r33 = xiz * DI(WH) + etz * DJ(WH) + ztz * DK(WH);
+#undef DI
+#undef DJ
+#undef DK
+  const double s11 = 0.5 * r11, s12 = 0.25 * r12, s13 = 0.25 * r13, s22 = 0.5 * r22,
+               s23 = 0.25 * r23, s33 = 0.5 * r33;
+  const double dum = 2.0 * (s11 * s11 + s12 * s12 + s13 * s13 + s12 * s12 + s22 * s22 +
+                            s23 * s23 + s13 * s13 + s23 * s23 + s33 * s33);
+  const double mh = sqrt(dum);
+  const double fw2 = filt2[c];
+  const double uh = UH[c], vh = VH[c], wh = WH[c];
+  double lm = 0.0, mm = 0.0;
+  // reference component order: 11, 22, 33, 12, 13, 23
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const double* A = q == 0 ? U : q == 1 ? V : q == 2 ? W : q == 3 ? U : q == 4 ? U : V;
+    const double* B = q == 0 ? U : q == 1 ? V : q == 2 ? W : q == 3 ? V : q == 4 ? W : W;
+    const double ha = q == 0 ? uh : q == 1 ? vh : q == 2 ? wh : q == 3 ? uh : q == 4 ? uh : vh;
+    const double hb = q == 0 ? uh : q == 1 ? vh : q == 2 ? wh : q == 3 ? vh : q == 4 ? wh : wh;
+    const int lq = q == 0 ? 0 : q == 1 ? 3 : q == 2 ? 5 : q == 3 ? 1 : q == 4 ? 2 : 4;
+    const double raw = q == 0 ? r11 : q == 1 ? r22 : q == 2 ? r33 : q == 3 ? r12 : q == 4 ? r13 : r23;
+    const double* Lq = L[lq];
+    double fuu = filter27(d, c, [&](long p) { return A[p] * B[p]; });
+    double Lij = -fuu + ha * hb;
+    double mt = q < 3 ? 4.0 * mh * 0.5 * raw : 4.0 * mh * 0.25 * raw;
+    double fms = filter27(d, c, [&](long p) { return mag[p] * Lq[p]; });
+    double M = fw2 * (mt - fms);
+    if (q == 0) {
+      lm = Lij * M;
+      mm = M * M;
+    } else if (q < 3) {
+      lm = lm + Lij * M;
+      mm = mm + M * M;
+    } else {
+      lm = lm + 2.0 * Lij * M;
+      mm = mm + 2.0 * M * M;
+    }
+  }
+  lm_out[c] = lm;
+  mm_out[c] = mm;
+}
+
+// conditional sums over cells with 0.5 LM/MM >= 0 (stress_tensor_dy.f90:486-507):
+// per-block partial (sum LM, sum MM, count)
+__global__ void k_lm_reduce(Dims d, const double* __restrict__ lm, const double* __restrict__ mm,
+                            double* part) {
+  __shared__ double s0[256], s1[256], s2[256];
+  const long nint = (long)(d.ni - 1) * (d.nj - 1) * (d.nk - 1);
+  double a = 0.0, b = 0.0, n = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nint;
+       t += (long)gridDim.x * blockDim.x) {
+    int i = 1 + (int)(t % (d.ni - 1));
+    long r = t / (d.ni - 1);
+    int j = 1 + (int)(r % (d.nj - 1)), k = 1 + (int)(r / (d.nj - 1));
+    long c = cidx(d, i, j, k);
+    double cs = 0.5 * lm[c] / mm[c];
+    if (cs >= 0.0) {
+      a = a + lm[c];
+      b = b + mm[c];
+      n = n + 1.0;
+    }
+  }
+  const int t = threadIdx.x;
+  s0[t] = a; s1[t] = b; s2[t] = n;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) { s0[t] += s0[t + s]; s1[t] += s1[t + s]; s2[t] += s2[t + s]; }
+    __syncthreads();
+  }
+  if (t == 0) { part[3 * blockIdx.x] = s0[0]; part[3 * blockIdx.x + 1] = s1[0]; part[3 * blockIdx.x + 2] = s2[0]; }
+}
+
+// C = 0.5 <LM>/<MM>, zero when <MM> < 1e-6 (stress_tensor_dy.f90:509-518)
+__global__ void k_lm_final(const double* __restrict__ part, int nb, double* out) {
+  __shared__ double s0[256], s1[256], s2[256];
+  const int t = threadIdx.x;
+  double a = 0.0, b = 0.0, n = 0.0;
+  for (int q = t; q < nb; q += blockDim.x) { a += part[3 * q]; b += part[3 * q + 1]; n += part[3 * q + 2]; }
+  s0[t] = a; s1[t] = b; s2[t] = n;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) { s0[t] += s0[t + s]; s1[t] += s1[t + s]; s2[t] += s2[t + s]; }
+    __syncthreads();
+  }
+  if (t == 0) {
+    double tlm = s0[0] / s2[0], tmm = s1[0] / s2[0];
+    double cavg = 0.5 * tlm / tmm;
+    if (tmm < 1e-6) cavg = 0.0;
+    out[0] = cavg;
+  }
+}
+
+// eddy viscosity nu_t = C Delta^2 |S| on every cell (stress_tensor_dy.f90:527)
+__global__ void k_nut(Dims d, const double* __restrict__ cavg, const double* __restrict__ filt2,
+                      const double* __restrict__ mag, double* nut) {
+  const double C = cavg[0];
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < d.n;
+       q += (long)gridDim.x * blockDim.x)
+    nut[q] = C * filt2[q] * mag[q];
+}
+
+// divergence of the SGS stress -2 nu_t S_ij (stress_tensor_dy.f90:541-583), added to NL
+// with the inverse Jacobian (RLES/main.f90:61-66)
+__global__ void k_sgs_div(Dims d, const double* const* __restrict__ m,
+                          const double* __restrict__ nut, const double* const* __restrict__ L,
+                          const double* __restrict__ djac, double* nlx, double* nly, double* nlz,
+                          double* sx, double* sy, double* sz) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ni - 1 || j > d.nj - 1) return;
+  const long c = cidx(d, i, j, k);
+  const long off[3] = {1, d.sj, d.sk};
+  // Lt[row][col]: symmetric strain components stored as 11,12,13,22,23,33
+  const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  double* nl[3] = {nlx, nly, nlz};
+  double* so[3] = {sx, sy, sz};
+  const double J = djac[c];
+#pragma unroll
+  for (int row = 0; row < 3; ++row) {
+    double t = 0.0;
+    bool first = true;
+#pragma unroll
+    for (int col = 0; col < 3; ++col) {
+      const double* S = L[idx[row][col]];
+#pragma unroll
+      for (int dir = 0; dir < 3; ++dir) {
+        const long of = off[dir];
+        double inner = -(2.0 * nut[c + of] * S[c + of]) + 2.0 * nut[c - of] * S[c - of];
+        double term = m[3 * col + dir][c] * inner;
+        t = first ? term : t + term;
+        first = false;
+      }
+    }
+    double sg = 0.5 * t;
+    so[row][c] = sg;
+    nl[row][c] = nl[row][c] + sg * J;
+  }
+}
+
+}  // namespace
+
+int fvm_k_sgs(FvmCtx* c) {
+  const Dims d = c->d;
+  double** m = c->dm;
+  ProfScope ps(c, "sgs");
+  hipStream_t st = c->stream;
+  {
+    dim3 g((d.ni + 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
+    hipLaunchKernelGGL(k_strain, g, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
+                       (double* const*)c->dlss, c->mag);
+    HIP_CHECK(hipGetLastError());
+  }
+  double* gl[7] = {c->lss[0], c->lss[1], c->lss[2], c->lss[3], c->lss[4], c->lss[5], c->mag};
+  int r = fvm_ghost_fill_n(c, gl, 7);
+  if (r) return r;
+  dim3 gi((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
+  hipLaunchKernelGGL(k_filter_vel, gi, kBlk, 0, st, d, c->u, c->v, c->w, c->uh[0], c->uh[1],
+                     c->uh[2]);
+  HIP_CHECK(hipGetLastError());
+  r = fvm_k_bcu_ghosts(c, c->uh, 3);
+  if (r) return r;
+  hipLaunchKernelGGL(k_lm, gi, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
+                     c->uh[0], c->uh[1], c->uh[2], (const double* const*)c->dlss, c->mag,
+                     c->filt2, c->su[0], c->su[1]);
+  HIP_CHECK(hipGetLastError());
+  const int nb = 512;
+  hipLaunchKernelGGL(k_lm_reduce, dim3(nb), dim3(256), 0, st, d, c->su[0], c->su[1], c->red);
+  hipLaunchKernelGGL(k_lm_final, dim3(1), dim3(256), 0, st, c->red, nb, c->scal + 0);
+  hipLaunchKernelGGL(k_nut, grid1(d.n, 256), dim3(256), 0, st, d, c->scal + 0, c->filt2, c->mag,
+                     c->nut);
+  hipLaunchKernelGGL(k_sgs_div, gi, kBlk, 0, st, d, (const double* const*)m, c->nut,
+                     (const double* const*)c->dlss, c->djac, c->nl[0], c->nl[1], c->nl[2],
+                     c->sgs[0], c->sgs[1], c->sgs[2]);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}

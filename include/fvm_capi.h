@@ -1,0 +1,107 @@
+/*
+ * fvm_capi.h -- C ABI of the MI355X fractional 4-step incompressible Navier-Stokes solver.
+ *
+ * The reference (Johann426/FVM, LES/v2.3_LES-Reynolds stresses/, cited as RLES/) keeps all
+ * state in the Fortran module COMDAT_SHARED and advances it with implicit-interface external
+ * subroutines called from MAIN (RLES/main.f90:35-161).  This header is the drop-in boundary a
+ * Fortran driver binds with ISO_C_BINDING (fvm_amd/fortran/fvm_capi_mod.f90): the module state
+ * becomes an opaque device context, each subroutine becomes one entry point that acts on the
+ * device-resident state.  Host arrays cross the boundary only at set-up and restart I/O.
+ *
+ * Conventions: plain pointers and sizes, no torch or HIP types.  Cell arrays are FP64,
+ * (NI+1)(NJ+1)(NK+1) values, I fastest, indices 0..NI x 0..NJ x 0..NK incl. ghosts (the
+ * reference's (IS:IE,JS:JE,KS:KE) arrays with one rank).  Node arrays are NI*NJ*NK values,
+ * I fastest (the records of the reference's grid file).  Every function returns an int status:
+ * 0 ok, 1 bad argument, 2 HIP runtime error, 3 non-finite value, 4 Poisson iteration hit its
+ * sweep cap, 5 grid too large for a line kernel.  The reference has no status returns
+ * (failures surface as NaN); the codes are an addition.  All calls on one context are ordered
+ * on one HIP stream; only scalars are returned synchronously.
+ */
+#ifndef FVM_CAPI_H
+#define FVM_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct FvmCtx fvm_ctx;
+
+/* Run constants.  RLES/comdat_param.f90:6-48 holds them as compile-time PARAMETERs. */
+typedef struct fvm_params {
+  int ni, nj, nk;       /* node counts NI (radial), NJ (circumferential), NK (span)      */
+  double re;            /* Reynolds number RE                                            */
+  double dtime;         /* time step DTIME                                               */
+  double ref_a;         /* reference area REF_A of the force coefficients               */
+  int use_les;          /* USE_LES: dynamic Smagorinsky model on (1) / off (0)           */
+  int nswpp;            /* NSWPP: Poisson sweep cap                                       */
+  double resimax;       /* RESIMAX: scaled-L1 Poisson residual tolerance                 */
+  int poisson_mode;     /* 0: FP64 zebra line Gauss-Seidel; 1: FP32 zebra sweeps inside an
+                           FP64 defect-correction loop                                  */
+  int check_every;      /* sweeps between residual evaluations (>=1)                     */
+} fvm_params;
+
+/* Per-step diagnostics (RLES/calccoef.f90:11-201 scalars and the ITER_P counters). */
+typedef struct fvm_step_info {
+  double resip;         /* final scaled-L1 residual of the pressure equation (FP64)     */
+  int niter;            /* Poisson line sweeps executed                                  */
+  int nouter;           /* defect-correction passes (mixed mode), else residual checks  */
+  double pref;          /* span-mean far-field pressure subtracted from p                */
+  double c_avg;         /* dynamic Smagorinsky coefficient C = <LM>/<2MM>                */
+  double cfl, cpb, cd, cl, cd_p, cd_v, cl_p, cl_v;
+} fvm_step_info;
+
+/* Field selectors for fvm_set_field / fvm_get_field. */
+enum {
+  FVM_U = 0, FVM_V = 1, FVM_W = 2, FVM_P = 3, FVM_NLX = 4, FVM_NLY = 5, FVM_NLZ = 6,
+  FVM_DIFX = 7, FVM_DIFY = 8, FVM_DIFZ = 9, FVM_SU = 10, FVM_SV = 11, FVM_SW = 12,
+  FVM_DJAC = 13, FVM_XIX = 14, FVM_ETX = 15, FVM_ZTX = 16, FVM_XIY = 17, FVM_ETY = 18,
+  FVM_ZTY = 19, FVM_XIZ = 20, FVM_ETZ = 21, FVM_ZTZ = 22, FVM_X = 23, FVM_Y = 24, FVM_Z = 25,
+  FVM_THETA = 26, FVM_NUT = 27, FVM_SGSX = 28, FVM_SGSY = 29, FVM_SGSZ = 30, FVM_UN = 31,
+  FVM_NFIELDS = 32
+};
+
+/* GRID + METRIC + the geometric part of INIT (RLES/grid.f90:1-367, RLES/metric.f90:1-538,
+ * RLES/init.f90:210-259), on the device.  xg/yg/zg: node coordinates (NI*NJ*NK, I fastest). */
+int fvm_create(const fvm_params* prm, const double* xg, const double* yg, const double* zg,
+               fvm_ctx** out);
+int fvm_destroy(fvm_ctx* ctx);
+
+/* Restart read / write of the module arrays (RLES/init.f90:183-208, RLES/writedata.f90:193-216):
+ * copies one full cell array host <-> device.  Setting NLX..NLZ sets both the current and the
+ * previous-step convective term (the restart holds only the current one). */
+int fvm_set_field(fvm_ctx* ctx, int field, const double* host);
+int fvm_get_field(fvm_ctx* ctx, int field, double* host);
+
+/* One time step of MAIN (RLES/main.f90:39-150): U_N copy, CONTRA_VEL, NL, STRESS_TENSOR_DY,
+ * DIF, P_SOURCE, ITER_U x3, predictor correction, BCU x3, CONTRA_VEL, ITER_P, BCP, P_SOURCE,
+ * projection, pressure reference, BCU x3, CALCCOEF.  The thermal scalar is not advanced (it is
+ * passive).  info may be NULL. */
+int fvm_step(fvm_ctx* ctx, fvm_step_info* info);
+
+/* The subroutines of the time step, for drivers that keep MAIN's call sequence. */
+int fvm_begin_step(fvm_ctx* ctx);                 /* U_N = U (RLES/main.f90:42-45)          */
+int fvm_contra_vel(fvm_ctx* ctx);                 /* CONTRA_VEL (RLES/contra_vel.f90:1-58)   */
+int fvm_nl(fvm_ctx* ctx);                         /* NL_NM1 = NL; NL(U_N..W_N) (nl.f90:1-33) */
+int fvm_stress_tensor_dy(fvm_ctx* ctx, double* c_avg); /* SGS + NL += J SGS (main.f90:59-69)
+                                                     (RLES/stress_tensor_dy.f90:1-586)      */
+int fvm_dif(fvm_ctx* ctx);                        /* DIF(U_N..W_N) (RLES/dif.f90:1-25)       */
+int fvm_p_source(fvm_ctx* ctx);                   /* P_SOURCE(P) (RLES/p_source.f90:1-44)    */
+int fvm_iter_u(fvm_ctx* ctx);                     /* ITER_U for U, V, W (iter_u.f90:1-256)   */
+int fvm_correct(fvm_ctx* ctx, double sign);       /* U += sign*DTIME*S/J, then BCU x3
+                                                     (RLES/main.f90:90-98, 108-111, 137-142) */
+int fvm_iter_p(fvm_ctx* ctx, double* resip, int* niter); /* ITER_P + BCP (iter_p.f90:1-66,
+                                                     lisolv.f90:1-57, bcp.f90:1-38)         */
+int fvm_pressure_reference(fvm_ctx* ctx, double* pref); /* RLES/main.f90:125-135          */
+int fvm_calccoef(fvm_ctx* ctx, fvm_step_info* info);    /* CALCCOEF (calccoef.f90:1-201)  */
+
+/* Kernel timing (HIP events on the context's stream).  fvm_profile_get fills up to cap
+ * entries of names (each <= 47 chars) / total ms / call counts and returns the count. */
+int fvm_profile_enable(fvm_ctx* ctx, int on);
+int fvm_profile_get(fvm_ctx* ctx, char* names, double* ms, long* calls, int cap);
+int fvm_profile_reset(fvm_ctx* ctx);
+int fvm_sync(fvm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
