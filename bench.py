@@ -25,33 +25,38 @@ sys.path.insert(0, HERE)
 HBM_PEAK = 8.0e12  # bytes/s, MI355X
 
 
-def build_solver(w, mode, resimax, nswpp, check_every):
+def build_solver(w, mode, resimax, nswpp, check_every, af_mode=0):
     from fvm_amd.grid import make_case
     from fvm_amd.solver import Solver3D
     (xg, yg, zg), ic = make_case(**w)
     s = Solver3D(w["ni"], w["nj"], w["nk"], w["re"], w["dt"], xg, yg, zg, ref_a=math.pi,
                  use_les=True, nswpp=nswpp, resimax=resimax, poisson_mode=mode,
-                 check_every=check_every)
+                 check_every=check_every, af_mode=af_mode)
     s.set_state(*(ic[k] for k in ("u", "v", "w", "p", "nlx", "nly", "nlz")))
     return s, ic
 
 
-def time_mode(w, mode, steps, warmup, resimax, nswpp, check_every, log):
-    s, _ = build_solver(w, mode, resimax, nswpp, check_every)
+def time_mode(w, mode, steps, warmup, resimax, nswpp, check_every, log, af_mode=0):
+    s, _ = build_solver(w, mode, resimax, nswpp, check_every, af_mode)
     for q in range(warmup):
         info = s.step()
-        log(f"warmup step {q + 1}/{warmup} mode {mode}: sweeps {info['niter']} "
-            f"resip {info['resip']:.2e} cd {info['cd']:.4f}")
+        log(f"warmup step {q + 1}/{warmup} mode {mode}: poisson passes {info['niter']} "
+            f"resip {info['resip']:.2e} af passes {info['af_iters']} cfl {info['cfl']:.3f} "
+            f"cd {info['cd']:.4f}")
     s.sync()
     infos = []
     t0 = time.perf_counter()
     for q in range(steps):
-        infos.append(s.step())
+        try:
+            infos.append(s.step())
+        except Exception as e:
+            log(f"step {q + 1} mode {mode} failed: {e}")
+            raise
     s.sync()
     t1 = time.perf_counter()
     for q, info in enumerate(infos):
-        log(f"timed step {q + 1}/{steps} mode {mode}: sweeps {info['niter']} "
-            f"resip {info['resip']:.2e}")
+        log(f"timed step {q + 1}/{steps} mode {mode}: poisson passes {info['niter']} "
+            f"resip {info['resip']:.2e} af passes {info['af_iters']} cfl {info['cfl']:.3f}")
     return s, (t1 - t0), infos
 
 
@@ -62,10 +67,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="les3d_re3900")
     ap.add_argument("--resimax", type=float, default=1e-8)
-    ap.add_argument("--nswpp", type=int, default=20000)
+    ap.add_argument("--nswpp", type=int, default=50)
     ap.add_argument("--check-every", type=int, default=8)
     ap.add_argument("--modes", default="mixed,fp64")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true", help="per-kernel HIP-event times")
+    ap.add_argument("--af-mode", type=int, default=0)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the solver holds the "
@@ -77,12 +84,24 @@ def main():
     cells = (w["ni"] - 1) * (w["nj"] - 1) * (w["nk"] - 1)
     results = {}
     for mname in a.modes.split(","):
-        mode = 1 if mname == "mixed" else 0
+        mode = {"mixed": 3, "fp64": 2, "zebra64": 0, "zebra32": 1}[mname]
         s, secs, infos = time_mode(w, mode, a.steps, a.warmup, a.resimax, a.nswpp,
-                                   a.check_every, log)
+                                   a.check_every, log, a.af_mode)
         results[mname] = dict(secs=secs, infos=infos, solver=s)
         log(f"{mname}: {secs / a.steps * 1e3:.2f} ms/step "
             f"{cells * a.steps / secs / 1e6:.1f} Mcell-steps/s")
+    if a.profile:
+        for mname, rr in results.items():
+            s = rr["solver"]
+            s.profile(True)
+            s.profile_reset()
+            info = s.step()
+            prof = s.profile_get()
+            s.profile(False)
+            tot = sum(v[0] for v in prof.values())
+            log(f"profile {mname}: one step, {info['niter']} outer, kernel total {tot:.2f} ms")
+            for k, (ms, calls) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+                log(f"  {k:16s} {ms:9.3f} ms  {calls:6d} calls  {ms / max(calls, 1) * 1e3:9.1f} us/call")
     head = "mixed" if "mixed" in results else list(results)[0]
     r = results[head]
     value = cells * a.steps / r["secs"] / 1e6

@@ -18,6 +18,7 @@ FIELDS = {
     "difz": 9, "s_u": 10, "s_v": 11, "s_w": 12, "djac": 13, "xix": 14, "etx": 15, "ztx": 16,
     "xiy": 17, "ety": 18, "zty": 19, "xiz": 20, "etz": 21, "ztz": 22, "x": 23, "y": 24, "z": 25,
     "theta": 26, "nut": 27, "sgsx": 28, "sgsy": 29, "sgsz": 30, "un": 31,
+    "vn": 32, "wn": 33,
 }
 
 STATUS = {0: "ok", 1: "bad argument", 2: "HIP runtime error", 3: "non-finite value",
@@ -34,7 +35,8 @@ class Params(ctypes.Structure):
     _fields_ = [("ni", ctypes.c_int), ("nj", ctypes.c_int), ("nk", ctypes.c_int),
                 ("re", ctypes.c_double), ("dtime", ctypes.c_double), ("ref_a", ctypes.c_double),
                 ("use_les", ctypes.c_int), ("nswpp", ctypes.c_int), ("resimax", ctypes.c_double),
-                ("poisson_mode", ctypes.c_int), ("check_every", ctypes.c_int)]
+                ("poisson_mode", ctypes.c_int), ("check_every", ctypes.c_int),
+                ("af_mode", ctypes.c_int), ("af_tol", ctypes.c_double)]
 
 
 class StepInfo(ctypes.Structure):
@@ -42,10 +44,12 @@ class StepInfo(ctypes.Structure):
                 ("pref", ctypes.c_double), ("c_avg", ctypes.c_double), ("cfl", ctypes.c_double),
                 ("cpb", ctypes.c_double), ("cd", ctypes.c_double), ("cl", ctypes.c_double),
                 ("cd_p", ctypes.c_double), ("cd_v", ctypes.c_double), ("cl_p", ctypes.c_double),
-                ("cl_v", ctypes.c_double)]
+                ("cl_v", ctypes.c_double), ("af_iters", ctypes.c_int * 3)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["af_iters"] = list(self.af_iters)
+        return d
 
 
 EXPORTS = ("fvm_create", "fvm_destroy", "fvm_set_field", "fvm_get_field", "fvm_step",

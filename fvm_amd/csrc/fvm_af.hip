@@ -141,23 +141,243 @@ __global__ void __launch_bounds__(64) k_af_sweep(Dims d, AFArgs A, const double*
   for (int r = threadIdx.x; r < n; r += 64) f[base + (long)(r + 1) * st] = ld[r];
 }
 
+
+// Chunked iteration of the stage-s line Gauss-Seidel sweep.  The sweep is the block lower
+// triangular system (D - L) x = U x_old + b over lines: D = line tridiagonals, L = couplings to
+// the two "earlier" neighbours (a-1, b-1), U = couplings to the "later" ones (a+1, b+1).  A
+// workgroup owns C consecutive lines along a at one b and sweeps them in the reference order
+// (the a-1 value of each line is the line just solved, kept in registers); the first line's
+// a-1 value and every b-1 value come from the previous iterate.  The iteration is nilpotent
+// (exact after at most #chunks + nb passes) and contractive (|| D^-1 L || < 1 by the diagonal
+// dominance AP = 2 Re J^-1/dt + sum A), so it converges to the exact lexicographic sweep; the
+// host stops it once the largest change is below the tolerance.  Threads own rows p of the line
+// (blockDim >= n) and load their row of all C lines up front: for the strided J and K lines the
+// C values of one row are contiguous (the chunk runs along I).
+template <int S, int C>
+__global__ void __launch_bounds__(512) k_af_chunk(Dims d, AFArgs A, const double* const* fa,
+                                                  const double* __restrict__ ap0,
+                                                  const double* const* prevs,
+                                                  double* const* nexts, int nchunk, long nwork,
+                                                  double* part) {
+  __shared__ double la[FVM_MAX_LINE], lb[FVM_MAX_LINE], lc[FVM_MAX_LINE], ld[FVM_MAX_LINE];
+  __shared__ double r0[FVM_MAX_LINE], r1[FVM_MAX_LINE];
+  // XCD-aware mapping: block ids are dealt round-robin over the 8 XCDs; consecutive work items
+  // (neighbouring chunks sharing cache lines) are placed on the same XCD.
+  const long G = (long)gridDim.x;
+  const long lin = blockIdx.x;
+  const long w = (lin % 8) * (G / 8) + lin / 8;
+  const int t = threadIdx.x;
+  double dmax = 0.0, xmax = 0.0;
+  if (w < nwork) {
+    const int na = S == 0 ? d.nj - 1 : d.ni - 1;
+    const int nb = S == 2 ? d.nj - 1 : d.nk - 1;
+    const int n = S == 0 ? d.ni - 1 : (S == 1 ? d.nj - 1 : d.nk - 1);
+    const int ca = (int)(w % nchunk);
+    const long rest = w / nchunk;
+    const int b = 1 + (int)(rest % nb);
+    const int comp = (int)(rest / nb);
+    const long sa = S == 0 ? d.sj : 1;
+    const long sb = S == 2 ? d.sj : d.sk;
+    const long sp = S == 0 ? 1 : (S == 1 ? d.sj : d.sk);
+    const double* fo = A.f[comp];  // stage input, frozen during the sweep
+    const double* pv = prevs[comp];
+    double* nx = nexts[comp];
+    const double* su = A.su[comp];
+    const long nf = (long)(d.nj + 1) * (d.nk + 1);
+    const int p = t + 1;
+    const bool row = p <= n;
+    const int a0 = 1 + ca * C;
+    double AP[C], LO[C], UP[C], T1[C], AM[C], T3[C], T4[C], SU[C], GE[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      AP[q] = 1.0; LO[q] = UP[q] = T1[q] = AM[q] = T3[q] = T4[q] = SU[q] = GE[q] = 0.0;
+      const int a = a0 + q;
+      if (a > na || !row) continue;
+      const long c = sa * a + sb * b + sp * p;
+      // coefficients in the reference's order AE, AW, AN, AS, AT, AB (iter_u.f90:23-47)
+      double ae = fa[0][c], aw = fa[0][c - 1], an = fa[1][c], as = fa[1][c - d.sj],
+             at = fa[2][c], ab = fa[2][c - d.sk];
+      double ap = ap0[c] + ae + aw + an + as + at + ab;
+      double s = su[c];
+      const int i = S == 0 ? p : a;
+      if (i == 1) { ap = ap + aw; s = s + 2.0 * aw * 0.0; aw = 0.0; }  // bcu.f90:18-20
+      if (i == d.ni - 1) {                                            // bcu.f90:38-40
+        const int jj = S == 0 ? a : (S == 1 ? p : b);
+        const int kk = S == 0 ? b : (S == 1 ? b : p);
+        const double uf = A.ufar[comp * nf + jj + (long)(d.nj + 1) * kk];
+        ap = ap + ae;
+        s = s + 2.0 * ae * uf;
+        ae = 0.0;
+      }
+      double cap, cam, cbp, cbm, lo, up;
+      if (S == 0) { cap = an; cam = as; cbp = at; cbm = ab; lo = -aw; up = -ae; }
+      else if (S == 1) { cap = ae; cam = aw; cbp = at; cbm = ab; lo = -as; up = -an; }
+      else { cap = ae; cam = aw; cbp = an; cbm = as; lo = -ab; up = -at; }
+      T1[q] = cap * fo[c + sa];
+      AM[q] = cam;
+      T3[q] = cbp * fo[c + sb];
+      T4[q] = cbm * (b > 1 ? pv[c - sb] : fo[c - sb]);
+      SU[q] = s;
+      if (p == 1) { GE[q] = lo * fo[c - sp]; lo = 0.0; }  // line-end Dirichlet data
+      if (p == n) { GE[q] = up * fo[c + sp]; up = 0.0; }
+      AP[q] = ap; LO[q] = lo; UP[q] = up;
+    }
+    double xe = 0.0;
+    if (row) {
+      const long c0 = sa * a0 + sb * b + sp * p;
+      xe = a0 > 1 ? pv[c0 - sa] : fo[c0 - sa];
+    }
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const int a = a0 + q;
+      if (a > na) break;  // uniform over the block
+      if (row) {
+        // C = A_a+ x(a+1) + A_a- x(a-1) + A_b+ x(b+1) + A_b- x(b-1) + SU, minus the end term
+        double rhs = T1[q] + AM[q] * xe;
+        rhs = rhs + T3[q];
+        rhs = rhs + T4[q];
+        rhs = rhs + SU[q];
+        rhs = rhs - GE[q];
+        la[t] = LO[q]; lb[t] = AP[q]; lc[t] = UP[q]; ld[t] = rhs;
+      }
+      __syncthreads();
+      pcr_solve_rows(la, lb, lc, ld, n);
+      if (row) {
+        const long c = sa * a + sb * b + sp * p;
+        const double x = ld[t];
+        dmax = fmax(dmax, fabs(x - pv[c]));
+        xmax = fmax(xmax, fabs(x));
+        nx[c] = x;
+        xe = x;
+      }
+      __syncthreads();
+    }
+  }
+  r0[t] = dmax;
+  r1[t] = xmax;
+  __syncthreads();
+  for (int st = 256; st > 0; st >>= 1) {
+    if (t < st && t + st < (int)blockDim.x) {
+      r0[t] = fmax(r0[t], r0[t + st]);
+      r1[t] = fmax(r1[t], r1[t + st]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) { part[2 * lin] = r0[0]; part[2 * lin + 1] = r1[0]; }
+}
+
+__global__ void k_max_final(const double* __restrict__ part, long nb, double* out) {
+  __shared__ double s0[256], s1[256];
+  const int t = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  for (long q = t; q < nb; q += blockDim.x) { a = fmax(a, part[2 * q]); b = fmax(b, part[2 * q + 1]); }
+  s0[t] = a; s1[t] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) { s0[t] = fmax(s0[t], s0[t + s]); s1[t] = fmax(s1[t], s1[t + s]); }
+    __syncthreads();
+  }
+  if (t == 0) { out[0] = s0[0]; out[1] = s1[0]; }
+}
+
+// copy every ghost cell (I, J and K boundary planes) of src into dst
+__global__ void k_copy_ghosts(Dims d, const double* __restrict__ src, double* dst) {
+  const long nI = 2L * (d.nj + 1) * (d.nk + 1);
+  const long nJ = 2L * (d.ni + 1) * (d.nk + 1);
+  const long nK = 2L * (d.ni + 1) * (d.nj + 1);
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nI + nJ + nK;
+       t += (long)gridDim.x * blockDim.x) {
+    long c;
+    if (t < nI) {
+      long r = t / 2;
+      int i = (t % 2) ? d.ni : 0;
+      c = cidx(d, i, (int)(r % (d.nj + 1)), (int)(r / (d.nj + 1)));
+    } else if (t < nI + nJ) {
+      long q = t - nI, r = q / 2;
+      int j = (q % 2) ? d.nj : 0;
+      c = cidx(d, (int)(r % (d.ni + 1)), j, (int)(r / (d.ni + 1)));
+    } else {
+      long q = t - nI - nJ, r = q / 2;
+      int k = (q % 2) ? d.nk : 0;
+      c = cidx(d, (int)(r % (d.ni + 1)), (int)(r / (d.ni + 1)), k);
+    }
+    dst[c] = src[c];
+  }
+}
+
 }  // namespace
+
+template <int S, int C>
+static int chunk_stage(FvmCtx* c, AFArgs& A, int* iters) {
+  const Dims d = c->d;
+  hipStream_t st = c->stream;
+  const int na = S == 0 ? d.nj - 1 : d.ni - 1;
+  const int nb = S == 2 ? d.nj - 1 : d.nk - 1;
+  const int n = S == 0 ? d.ni - 1 : (S == 1 ? d.nj - 1 : d.nk - 1);
+  const int nchunk = (na + C - 1) / C;
+  const long nwork = (long)nchunk * nb * 3;
+  const long G = (nwork + 7) / 8 * 8;
+  if (2 * G > c->nred) return FVM_ERR_SIZE;
+  const int threads = (n + 63) / 64 * 64;
+  double** fld[3] = {&c->u, &c->v, &c->w};
+  double* cur[3] = {c->u, c->v, c->w};
+  double* prev[3] = {cur[0], cur[1], cur[2]};
+  double* next[3];
+  int slot = 0;
+  int it = 0;
+  const int cap = nchunk + nb + 1;
+  const double tol = c->prm.af_tol > 0 ? c->prm.af_tol : 1e-14;
+  for (;;) {
+    for (int q = 0; q < 3; ++q) next[q] = c->afb[q][slot];
+    HIP_CHECK(hipMemcpyAsync(c->afptr, prev, 3 * sizeof(double*), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(c->afptr + 3, next, 3 * sizeof(double*), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL((k_af_chunk<S, C>), dim3((unsigned)G), dim3(threads), 0, st, d, A,
+                       (const double* const*)c->dfa[S], c->ap0, (const double* const*)c->afptr,
+                       (double* const*)(c->afptr + 3), nchunk, nwork, c->red);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, st, c->red, G, c->scal + 20);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(c->hscal + 20, c->scal + 20, 2 * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    ++it;
+    const double dm = c->hscal[20], xm = c->hscal[21];
+    if (!(dm == dm) || !(xm == xm)) {
+      fprintf(stderr, "fvm_amd: non-finite AF stage %d iterate (pass %d)\n", S, it);
+      return FVM_ERR_NAN;
+    }
+    for (int q = 0; q < 3; ++q) prev[q] = next[q];
+    slot ^= 1;
+    if ((it >= 2 && dm <= tol * xm) || it >= cap) break;
+  }
+  // the sweep leaves the ghost values of the stage input untouched: carry them over, then
+  // the converged iterate becomes the field
+  for (int q = 0; q < 3; ++q) {
+    long ng = 2L * ((d.nj + 1) * (d.nk + 1) + (d.ni + 1) * (d.nk + 1) + (d.ni + 1) * (d.nj + 1));
+    hipLaunchKernelGGL(k_copy_ghosts, grid1(ng, 256), dim3(256), 0, st, d, cur[q], prev[q]);
+    HIP_CHECK(hipGetLastError());
+    for (int sl = 0; sl < 2; ++sl)
+      if (c->afb[q][sl] == prev[q]) c->afb[q][sl] = cur[q];
+    *fld[q] = prev[q];
+  }
+  *iters = it;
+  return FVM_OK;
+}
 
 int fvm_k_iter_u(FvmCtx* c) {
   const Dims d = c->d;
   hipStream_t st = c->stream;
-  AFArgs A;
-  double* fld[3] = {c->u, c->v, c->w};
-  const double* fn[3] = {c->un, c->vn, c->wn};
-  for (int q = 0; q < 3; ++q) {
-    A.f[q] = fld[q]; A.fn[q] = fn[q]; A.nl[q] = c->nl[q]; A.nl1[q] = c->nl1[q];
-    A.dif[q] = c->dif[q]; A.s[q] = c->s[q]; A.su[q] = c->su[q];
-  }
-  A.ufar = c->ufar;
   if (d.ni - 1 > FVM_MAX_LINE || d.nj - 1 > FVM_MAX_LINE || d.nk - 1 > FVM_MAX_LINE)
     return FVM_ERR_SIZE;
   const dim3 gi((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, 3 * (d.nk - 1));
   for (int s = 0; s < 3; ++s) {
+    AFArgs A;
+    double* fld[3] = {c->u, c->v, c->w};
+    const double* fn[3] = {c->un, c->vn, c->wn};
+    for (int q = 0; q < 3; ++q) {
+      A.f[q] = fld[q]; A.fn[q] = fn[q]; A.nl[q] = c->nl[q]; A.nl1[q] = c->nl1[q];
+      A.dif[q] = c->dif[q]; A.s[q] = c->s[q]; A.su[q] = c->su[q];
+    }
+    A.ufar = c->ufar;
     {
       ProfScope ps(c, "af_rhs");
       if (s == 0) hipLaunchKernelGGL(k_af_rhs<0>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re);
@@ -168,19 +388,30 @@ int fvm_k_iter_u(FvmCtx* c) {
     // boundary ghosts of the stage input (BCU, iter_u.f90:83,160,237)
     int r = fvm_k_bcu_ghosts(c, fld, 3);
     if (r) return r;
-    const int na = s == 0 ? d.nj - 1 : d.ni - 1;
-    const int nb = s == 2 ? d.nj - 1 : d.nk - 1;
-    const double* const* fa = (const double* const*)c->dfa[s];
     ProfScope ps(c, s == 0 ? "af_sweep_i" : (s == 1 ? "af_sweep_j" : "af_sweep_k"));
-    for (int h = 0; h <= na + nb - 2; ++h) {
-      int lo = h - (nb - 1) > 0 ? h - (nb - 1) : 0;
-      int hi = h < na - 1 ? h : na - 1;
-      dim3 g(hi - lo + 1, 3);
-      if (s == 0) hipLaunchKernelGGL(k_af_sweep<0>, g, dim3(64), 0, st, d, A, fa, c->ap0, h);
-      if (s == 1) hipLaunchKernelGGL(k_af_sweep<1>, g, dim3(64), 0, st, d, A, fa, c->ap0, h);
-      if (s == 2) hipLaunchKernelGGL(k_af_sweep<2>, g, dim3(64), 0, st, d, A, fa, c->ap0, h);
+    if (c->prm.af_mode == 1) {
+      // anti-diagonal wavefront: one launch per anti-diagonal of the line plane
+      const int na = s == 0 ? d.nj - 1 : d.ni - 1;
+      const int nb = s == 2 ? d.nj - 1 : d.nk - 1;
+      const double* const* fa = (const double* const*)c->dfa[s];
+      for (int h = 0; h <= na + nb - 2; ++h) {
+        int lo = h - (nb - 1) > 0 ? h - (nb - 1) : 0;
+        int hi = h < na - 1 ? h : na - 1;
+        dim3 g(hi - lo + 1, 3);
+        if (s == 0) hipLaunchKernelGGL(k_af_sweep<0>, g, dim3(64), 0, st, d, A, fa, c->ap0, h);
+        if (s == 1) hipLaunchKernelGGL(k_af_sweep<1>, g, dim3(64), 0, st, d, A, fa, c->ap0, h);
+        if (s == 2) hipLaunchKernelGGL(k_af_sweep<2>, g, dim3(64), 0, st, d, A, fa, c->ap0, h);
+      }
+      HIP_CHECK(hipGetLastError());
+      c->af_iters[s] = na + nb - 1;
+    } else {
+      int it = 0;
+      if (s == 0) r = chunk_stage<0, 8>(c, A, &it);
+      if (s == 1) r = chunk_stage<1, 8>(c, A, &it);
+      if (s == 2) r = chunk_stage<2, 8>(c, A, &it);
+      if (r) return r;
+      c->af_iters[s] = it;
     }
-    HIP_CHECK(hipGetLastError());
   }
   return FVM_OK;
 }

@@ -75,6 +75,8 @@ static double* field_ptr(FvmCtx* c, int f) {
     case FVM_SGSY: return c->sgs[1];
     case FVM_SGSZ: return c->sgs[2];
     case FVM_UN: return c->un;
+    case FVM_VN: return c->vn;
+    case FVM_WN: return c->wn;
     default: return nullptr;
   }
 }
@@ -123,12 +125,16 @@ int fvm_create(const fvm_params* prm, const double* xg, const double* yg, const 
     r |= dalloc(c, &c->ju[q], n);
     r |= dalloc(c, &c->uh[q], n);
     r |= dalloc(c, &c->sgs[q], n);
+    r |= dalloc(c, &c->afb[q][0], n);
+    r |= dalloc(c, &c->afb[q][1], n);
   }
   for (int q = 0; q < 6; ++q) r |= dalloc(c, &c->lss[q], n);
   r |= dalloc(c, &c->ufar, 3L * (prm->nj + 1) * (prm->nk + 1));
   r |= dalloc(c, &c->wall, 12L * (prm->nj - 1) * (prm->nk - 1));
   long nblk = (long)((prm->ni - 1 + 63) / 64) * ((prm->nj - 1 + 3) / 4) * (prm->nk - 1);
-  c->nred = (int)std::max(nblk * 2, 4096L);
+  // AF chunk iteration: one partial pair per workgroup, chunks of 8 lines
+  long nafw = 3L * ((std::max(prm->ni, prm->nj) + 7) / 8 + 1) * std::max(prm->nj, prm->nk) + 8;
+  c->nred = (int)std::max(std::max(nblk * 2, 2 * nafw), 4096L);
   r |= dalloc(c, &c->red, c->nred);
   r |= dalloc(c, &c->scal, 64);
   double *p32 = nullptr, *su32 = nullptr;
@@ -142,14 +148,15 @@ int fvm_create(const fvm_params* prm, const double* xg, const double* yg, const 
     return FVM_ERR_HIP;
   }
   // device pointer tables
-  double** tabs[5];
-  for (int t = 0; t < 5; ++t) {
+  double** tabs[6];
+  for (int t = 0; t < 6; ++t) {
     if (hipMalloc((void**)&tabs[t], 9 * sizeof(double*)) != hipSuccess) { fvm_destroy(c); return FVM_ERR_HIP; }
     c->allocs.push_back(tabs[t]);
   }
   c->dm = tabs[0];
   c->dlss = tabs[1];
   for (int s = 0; s < 3; ++s) c->dfa[s] = tabs[2 + s];
+  c->afptr = tabs[5];
   (void)hipMemcpy(c->dm, c->m, 9 * sizeof(double*), hipMemcpyHostToDevice);
   (void)hipMemcpy(c->dlss, c->lss, 6 * sizeof(double*), hipMemcpyHostToDevice);
   for (int s = 0; s < 3; ++s) (void)hipMemcpy(c->dfa[s], c->fa[s], 3 * sizeof(double*), hipMemcpyHostToDevice);
@@ -170,6 +177,8 @@ int fvm_create(const fvm_params* prm, const double* xg, const double* yg, const 
   }
   r = fvm_geom_setup(c);
   if (r) { fvm_destroy(c); return r; }
+  r = fvm_sep_setup(c);
+  if (r) { fvm_destroy(c); return r; }
   *out = c;
   return FVM_OK;
 }
@@ -177,6 +186,7 @@ int fvm_create(const fvm_params* prm, const double* xg, const double* yg, const 
 int fvm_destroy(fvm_ctx* c) {
   if (!c) return FVM_ERR_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  fvm_sep_destroy(c);
   for (auto& e : c->ev_pending) {
     (void)hipEventDestroy(e.second.first);
     (void)hipEventDestroy(e.second.second);
@@ -306,10 +316,14 @@ int fvm_step(fvm_ctx* c, fvm_step_info* info) {
   st.resip = res;
   st.niter = it;
   st.nouter = c->last.nouter;
+  for (int q = 0; q < 3; ++q) st.af_iters[q] = c->af_iters[q];
   c->last = st;
   if (info) *info = st;
   prof_flush(c);
-  if (!(st.cd == st.cd) || !(st.cfl == st.cfl)) return FVM_ERR_NAN;
+  if (!(st.cd == st.cd) || !(st.cfl == st.cfl)) {
+    fprintf(stderr, "fvm_amd: non-finite force coefficients / CFL after the step\n");
+    return FVM_ERR_NAN;
+  }
   return rp;
 }
 

@@ -140,3 +140,29 @@ inline dim3 grid1(long n, int block) { return dim3((unsigned)((n + block - 1) / 
 inline long ghost_count(const Dims& d) {
   return (long)(d.ni + 1) * 2 * (d.nk + 1) + (long)(d.ni + 1) * (d.nj + 1) * 2;
 }
+
+// Parallel cyclic reduction with one row per thread (row = threadIdx.x < n, blockDim >= n);
+// same recurrence as pcr_solve.  On return d[r] holds x[r].
+template <typename T>
+__device__ inline void pcr_solve_rows(T* a, T* b, T* c, T* d, int n) {
+  const int r = threadIdx.x;
+  for (int s = 1; s < n; s <<= 1) {
+    T na = T(0), nb = T(0), nc = T(0), nd = T(0);
+    if (r < n) {
+      T ar = a[r], br = b[r], cr = c[r], dr = d[r];
+      T alpha = T(0), gamma = T(0), xa = T(0), xc = T(0), xd = T(0), ya = T(0), yc = T(0),
+        yd = T(0);
+      if (r >= s) { alpha = -ar / b[r - s]; xa = a[r - s]; xc = c[r - s]; xd = d[r - s]; }
+      if (r + s < n) { gamma = -cr / b[r + s]; ya = a[r + s]; yc = c[r + s]; yd = d[r + s]; }
+      na = alpha * xa;
+      nc = gamma * yc;
+      nb = br + alpha * xc + gamma * ya;
+      nd = dr + alpha * xd + gamma * yd;
+    }
+    __syncthreads();
+    if (r < n) { a[r] = na; b[r] = nb; c[r] = nc; d[r] = nd; }
+    __syncthreads();
+  }
+  if (r < n) d[r] = d[r] / b[r];
+  __syncthreads();
+}

@@ -57,12 +57,20 @@ struct FvmCtx {
   double* uh[3] = {};
   double* nut = nullptr;
   double* sgs[3] = {};
+  // AF chunked iteration: two iterate buffers per component, device table of prev/next
+  double* afb[3][2] = {};
+  double** afptr = nullptr;
+  int af_iters[3] = {};
   // reductions
   double* red = nullptr;     // partial sums
   double* scal = nullptr;    // device scalars (c_avg, residual sums, pref, ...)
   double* hscal = nullptr;   // pinned host mirror of scal
 
   int nred = 0;
+  // separable approximate inverse of the pressure operator (fvm_poisson_spec.hip)
+  void* sep = nullptr;
+  double sep_dev = 0.0;     // largest relative deviation of a face coefficient from its J,K mean
+  long sep_fallbacks = 0;   // FP32 passes handed over to the FP64 inverse
   // last-step diagnostics
   fvm_step_info last{};
   // profiling
@@ -100,3 +108,6 @@ int fvm_k_sgs(FvmCtx* c);
 int fvm_k_iter_u(FvmCtx* c);
 int fvm_k_iter_p(FvmCtx* c, double* resip, int* niter);
 int fvm_k_bcp(FvmCtx* c);
+int fvm_sep_setup(FvmCtx* c);
+void fvm_sep_destroy(FvmCtx* c);
+int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter);

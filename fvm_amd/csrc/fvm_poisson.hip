@@ -275,6 +275,11 @@ int fvm_k_iter_p(FvmCtx* c, double* resip, int* niter) {
                        c->prm.dtime, c->rhs);
     HIP_CHECK(hipGetLastError());
   }
+  if (c->prm.poisson_mode >= 2) {
+    int rs = fvm_k_iter_p_sep(c, c->prm.poisson_mode == 3, resip, niter);
+    if (rs && rs != FVM_ERR_NOCONV) return rs;
+    return rs;
+  }
   const int every = c->prm.check_every > 0 ? c->prm.check_every : 1;
   const int cap = c->prm.nswpp;
   const double tol = c->prm.resimax;

@@ -17,13 +17,15 @@ class Solver3D:
     """3D fractional 4-step solver on one GPU (device-resident state)."""
 
     def __init__(self, ni, nj, nk, re, dtime, xg, yg, zg, ref_a=math.pi, use_les=True,
-                 nswpp=20, resimax=1e-4, poisson_mode=0, check_every=1):
+                 nswpp=20, resimax=1e-4, poisson_mode=0, check_every=1, af_mode=0,
+                 af_tol=1e-14):
         self.L = capi.load()
         self.ni, self.nj, self.nk = ni, nj, nk
         self.shape = (ni + 1, nj + 1, nk + 1)
         self.prm = capi.Params(ni, nj, nk, float(re), float(dtime), float(ref_a),
                                1 if use_les else 0, int(nswpp), float(resimax),
-                               int(poisson_mode), int(check_every))
+                               int(poisson_mode), int(check_every), int(af_mode),
+                               float(af_tol))
         arrs = [np.asfortranarray(a, dtype=np.float64).reshape((ni, nj, nk), order="F")
                 for a in (xg, yg, zg)]
         flat = [np.ascontiguousarray(a.ravel(order="F")) for a in arrs]

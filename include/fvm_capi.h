@@ -35,9 +35,17 @@ typedef struct fvm_params {
   int use_les;          /* USE_LES: dynamic Smagorinsky model on (1) / off (0)           */
   int nswpp;            /* NSWPP: Poisson sweep cap                                       */
   double resimax;       /* RESIMAX: scaled-L1 Poisson residual tolerance                 */
-  int poisson_mode;     /* 0: FP64 zebra line Gauss-Seidel; 1: FP32 zebra sweeps inside an
-                           FP64 defect-correction loop                                  */
+  int poisson_mode;     /* 0: FP64 zebra radial-line Gauss-Seidel relaxation;
+                           1: FP32 zebra sweeps inside FP64 defect correction;
+                           2: FP64 defect correction with the FP64 separable inverse
+                              (J/K Fourier modes x radial line solves);
+                           3: the same with the separable inverse in FP32 (mixed form)  */
   int check_every;      /* sweeps between residual evaluations (>=1)                     */
+  int af_mode;          /* AF line sweeps in the reference's lexicographic order, computed
+                           0: by chunked block iteration to convergence (default);
+                           1: by anti-diagonal wavefront launches                          */
+  double af_tol;        /* chunked iteration: stop when max change <= af_tol * max |u|
+                           (<= 0: 1e-14)                                                 */
 } fvm_params;
 
 /* Per-step diagnostics (RLES/calccoef.f90:11-201 scalars and the ITER_P counters). */
@@ -48,6 +56,7 @@ typedef struct fvm_step_info {
   double pref;          /* span-mean far-field pressure subtracted from p                */
   double c_avg;         /* dynamic Smagorinsky coefficient C = <LM>/<2MM>                */
   double cfl, cpb, cd, cl, cd_p, cd_v, cl_p, cl_v;
+  int af_iters[3];      /* passes of the three AF stage sweeps                          */
 } fvm_step_info;
 
 /* Field selectors for fvm_set_field / fvm_get_field. */
@@ -57,7 +66,7 @@ enum {
   FVM_DJAC = 13, FVM_XIX = 14, FVM_ETX = 15, FVM_ZTX = 16, FVM_XIY = 17, FVM_ETY = 18,
   FVM_ZTY = 19, FVM_XIZ = 20, FVM_ETZ = 21, FVM_ZTZ = 22, FVM_X = 23, FVM_Y = 24, FVM_Z = 25,
   FVM_THETA = 26, FVM_NUT = 27, FVM_SGSX = 28, FVM_SGSY = 29, FVM_SGSZ = 30, FVM_UN = 31,
-  FVM_NFIELDS = 32
+  FVM_VN = 32, FVM_WN = 33, FVM_NFIELDS = 34
 };
 
 /* GRID + METRIC + the geometric part of INIT (RLES/grid.f90:1-367, RLES/metric.f90:1-538,

@@ -100,3 +100,18 @@ def test_mixed_precision_reaches_same_fixed_point():
     for c in ("u", "v", "w"):
         assert rel(s.get(c), g[f"s1_{c}"]) < 1e-10, c
     assert rel(s.get("p"), g["s1_pref"]) < 1e-10
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_separable_defect_correction_same_fixed_point(mode):
+    """Defect correction with the separable approximate inverse (FP64: mode 2, FP32: mode 3)
+    converges to the reference's fixed point."""
+    g = load_golden("rles_9x21x5_les_converged")
+    s = make_gpu(g, nswpp=50, resimax=1e-13, poisson_mode=mode)
+    for st in (1, 2):
+        info = s.step()
+        assert info["status"] == 0 and info["resip"] <= 1e-13, info
+        assert info["niter"] <= (4 if mode == 2 else 8), info["niter"]
+        for c in ("u", "v", "w"):
+            assert rel(s.get(c), g[f"s{st}_{c}"]) < 1e-10, (st, c)
+        assert rel(s.get("p"), g[f"s{st}_pref"]) < 1e-10, (st, "p")
