@@ -5,8 +5,15 @@ O-grid (NI=193, NJ=257, NK=65 nodes), radial stretching from 0.001 D to 25 D, sp
 dt = 0.002, seeded N(0, 0.05) start.  A step is one full fractional 4-step time step (explicit
 terms, dynamic SGS model, AF momentum, pressure Poisson solved to the scaled-L1 residual
 RESIP <= 1e-8, projection, forces).  The metric is interior cells x steps / wall seconds / 1e6
-for the mixed-precision form (FP32 Poisson sweeps in FP64 defect correction) and the plain FP64
-form at equal residual; inputs are resident in HBM before the timed region.
+for the mixed-precision form (FP32 approximate inverse inside FP64 defect correction) and the
+plain FP64 form at equal residual; inputs are resident in HBM before the timed region.
+
+The impulsive cold start at dt = 0.002 diverges in the reference scheme on this grid (the
+wall-adjacent radial Courant number of the start field is 2), so the untimed start-up runs
+--startup-steps steps at dt/10 before switching to dt = 0.002 (the reference reassigns DTIME at
+run time the same way).  Per-kernel durations are HIP events on the solver's stream over the
+timed steps; the dominant kernel's algorithmic bytes per launch over its mean duration gives
+the roofline fraction against the 8 TB/s HBM peak.
 """
 from __future__ import annotations
 
@@ -22,105 +29,190 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-HBM_PEAK = 8.0e12  # bytes/s, MI355X
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# Algorithmic bytes per interior cell of one launch: each distinct cell-array operand of the
+# kernel's dataflow read once, each output written once, FP64 = 8 B (FP32 = 4 B); ghosts and
+# O(surface) work excluded.  'units' is how many cell sweeps one launch performs.
+KERNEL_BYTES = {
+    # one pass of an AF stage over the 3 components: {phi_in, SU, J, 3 stage metric
+    # products} -> phi_out = 7 doubles per cell and component
+    "k_af_chunk_i": (56, 3), "k_af_chunk_j": (56, 3), "k_af_chunk_k": (56, 3),
+    # strain rates: {u,v,w, xi 9} -> {S 6, |S|}
+    "k_strain": (8 * 19, 1),
+    # 27-point test filter of u, v, w
+    "k_filter_vel": (8 * 6, 1),
+    # Germano identity terms: {u,v,w, u^ 3, xi 9, S 6, |S|, Delta^2} -> {LM, MM}
+    "k_lm": (8 * 25, 1),
+    # SGS divergence: {xi 9, nu_t, S 6, J, NL 3} -> {NL 3, SGS 3}
+    "k_sgs_div": (8 * 26, 1),
+}
 
 
-def build_solver(w, mode, resimax, nswpp, check_every, af_mode=0):
+def cells_of(w):
+    return (w["ni"] - 1) * (w["nj"] - 1) * (w["nk"] - 1)
+
+
+def build_solver(w, mode, resimax, nswpp, check_every, dt, af_mode=0):
     from fvm_amd.grid import make_case
     from fvm_amd.solver import Solver3D
     (xg, yg, zg), ic = make_case(**w)
-    s = Solver3D(w["ni"], w["nj"], w["nk"], w["re"], w["dt"], xg, yg, zg, ref_a=math.pi,
+    s = Solver3D(w["ni"], w["nj"], w["nk"], w["re"], dt, xg, yg, zg, ref_a=math.pi,
                  use_les=True, nswpp=nswpp, resimax=resimax, poisson_mode=mode,
                  check_every=check_every, af_mode=af_mode)
     s.set_state(*(ic[k] for k in ("u", "v", "w", "p", "nlx", "nly", "nlz")))
-    return s, ic
+    return s
 
 
-def time_mode(w, mode, steps, warmup, resimax, nswpp, check_every, log, af_mode=0):
-    s, _ = build_solver(w, mode, resimax, nswpp, check_every, af_mode)
-    for q in range(warmup):
+def run_mode(w, mode, a, log):
+    dt = w["dt"]
+    s = build_solver(w, mode, a.resimax, a.nswpp, a.check_every, dt * a.startup_factor,
+                     a.af_mode)
+    for q in range(a.startup_steps):
         info = s.step()
-        log(f"warmup step {q + 1}/{warmup} mode {mode}: poisson passes {info['niter']} "
+        if q == 0 or q + 1 == a.startup_steps:
+            log(f"start-up step {q + 1}/{a.startup_steps} (dt {dt * a.startup_factor:g}): "
+                f"poisson passes {info['niter']} cfl {info['cfl']:.3f} cd {info['cd']:.3f}")
+    s.set_dtime(dt)
+    for q in range(a.warmup):
+        info = s.step()
+        log(f"warmup step {q + 1}/{a.warmup} mode {mode}: poisson passes {info['niter']} "
             f"resip {info['resip']:.2e} af passes {info['af_iters']} cfl {info['cfl']:.3f} "
             f"cd {info['cd']:.4f}")
+    s.profile(True)
+    s.profile_reset()
     s.sync()
     infos = []
     t0 = time.perf_counter()
-    for q in range(steps):
-        try:
-            infos.append(s.step())
-        except Exception as e:
-            log(f"step {q + 1} mode {mode} failed: {e}")
-            raise
+    for q in range(a.steps):
+        infos.append(s.step())
     s.sync()
     t1 = time.perf_counter()
+    prof = s.profile_get()
+    s.profile(False)
     for q, info in enumerate(infos):
-        log(f"timed step {q + 1}/{steps} mode {mode}: poisson passes {info['niter']} "
-            f"resip {info['resip']:.2e} af passes {info['af_iters']} cfl {info['cfl']:.3f}")
-    return s, (t1 - t0), infos
+        log(f"timed step {q + 1}/{a.steps} mode {mode}: poisson passes {info['niter']} "
+            f"resip {info['resip']:.2e} af passes {info['af_iters']} cfl {info['cfl']:.3f} "
+            f"cd {info['cd']:.4f}")
+    return dict(secs=t1 - t0, infos=infos, prof=prof)
+
+
+def roofline(prof, ncell):
+    rows = []
+    for name, (b, units) in KERNEL_BYTES.items():
+        if name not in prof or prof[name][1] == 0:
+            continue
+        ms, calls = prof[name]
+        avg_s = ms / calls * 1e-3
+        nbytes = b * units * ncell
+        rows.append(dict(kernel=name, calls=calls, avg_us=avg_s * 1e6, total_ms=ms,
+                         bytes_per_launch=nbytes, achieved_gbs=nbytes / avg_s / 1e9,
+                         frac=nbytes / avg_s / 1e9 / HBM_PEAK_GBS))
+    rows.sort(key=lambda r: -r["total_ms"])
+    return rows
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a
+    copy-kernel calibration (profiles/traffic.json), or None when not measured."""
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("bytes_per_launch")
+
+
+def cpu_baseline(w, log):
+    """The oracle (C restatement of the reference's step, one thread) on one C1 time step with
+    the reference's own sweep cap NSWPP = 20, after one untimed step."""
+    from fvm_amd.grid import make_case
+    from oracle.oracle3d import Oracle3D
+    (xg, yg, zg), ic = make_case(**w)
+    o = Oracle3D(w["ni"], w["nj"], w["nk"], w["re"], w["dt"] * 0.1, math.pi, xg, yg, zg)
+    o.set_state(*(ic[k] for k in ("u", "v", "w", "p", "nlx", "nly", "nlz")))
+    o.step(1, 1e-8, True)
+    t0 = time.perf_counter()
+    o.step(20, 1e-8, True)
+    t1 = time.perf_counter()
+    log(f"cpu oracle: one step with 20 Poisson sweeps in {t1 - t0:.2f} s")
+    return dict(value=cells_of(w) / (t1 - t0) / 1e6, unit="Mcell-steps/s", cores=1, kind="port",
+                sample="one 192x256x64 time step (SGS, AF, 20 lexicographic Poisson line "
+                       "sweeps = the reference's NSWPP, RESIP not driven to 1e-8), oracle C "
+                       f"restatement on 1 host core, {t1 - t0:.2f} s")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="les3d_re3900")
     ap.add_argument("--resimax", type=float, default=1e-8)
     ap.add_argument("--nswpp", type=int, default=50)
     ap.add_argument("--check-every", type=int, default=8)
     ap.add_argument("--modes", default="mixed,fp64")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile", action="store_true", help="per-kernel HIP-event times")
+    ap.add_argument("--startup-steps", type=int, default=20)
+    ap.add_argument("--startup-factor", type=float, default=0.1)
     ap.add_argument("--af-mode", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true", help="print every kernel scope")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the solver holds the "
-              "whole domain on one MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the solver holds "
+              "the whole domain on one MI355X (the span-slab sharding is a design note only)",
+              file=sys.stderr)
         sys.exit(2)
     log = lambda m: print(m, file=sys.stderr, flush=True)
     from fvm_amd.grid import WORKLOADS
     w = WORKLOADS[a.workload]
-    cells = (w["ni"] - 1) * (w["nj"] - 1) * (w["nk"] - 1)
+    ncell = cells_of(w)
     results = {}
     for mname in a.modes.split(","):
         mode = {"mixed": 3, "fp64": 2, "zebra64": 0, "zebra32": 1}[mname]
-        s, secs, infos = time_mode(w, mode, a.steps, a.warmup, a.resimax, a.nswpp,
-                                   a.check_every, log, a.af_mode)
-        results[mname] = dict(secs=secs, infos=infos, solver=s)
-        log(f"{mname}: {secs / a.steps * 1e3:.2f} ms/step "
-            f"{cells * a.steps / secs / 1e6:.1f} Mcell-steps/s")
-    if a.profile:
-        for mname, rr in results.items():
-            s = rr["solver"]
-            s.profile(True)
-            s.profile_reset()
-            info = s.step()
-            prof = s.profile_get()
-            s.profile(False)
-            tot = sum(v[0] for v in prof.values())
-            log(f"profile {mname}: one step, {info['niter']} outer, kernel total {tot:.2f} ms")
-            for k, (ms, calls) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
-                log(f"  {k:16s} {ms:9.3f} ms  {calls:6d} calls  {ms / max(calls, 1) * 1e3:9.1f} us/call")
+        r = run_mode(w, mode, a, log)
+        results[mname] = r
+        log(f"{mname}: {r['secs'] / a.steps * 1e3:.2f} ms/step "
+            f"{ncell * a.steps / r['secs'] / 1e6:.1f} Mcell-steps/s")
+        if a.profile:
+            tot = sum(v[0] for k, v in r["prof"].items() if not k.startswith("k_"))
+            log(f"profile {mname}: {a.steps} steps, scope total {tot:.2f} ms")
+            for k, (ms, calls) in sorted(r["prof"].items(), key=lambda kv: -kv[1][0]):
+                log(f"  {k:16s} {ms:9.3f} ms  {calls:6d} calls  "
+                    f"{ms / max(calls, 1) * 1e3:9.1f} us/call")
     head = "mixed" if "mixed" in results else list(results)[0]
     r = results[head]
-    value = cells * a.steps / r["secs"] / 1e6
+    value = ncell * a.steps / r["secs"] / 1e6
+    rows = roofline(r["prof"], ncell)
+    roof = None
+    if rows:
+        top = rows[0]
+        roof = {"bound": "hbm", "achieved": top["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": top["frac"], "traffic": traffic_for(top["kernel"]),
+                "kernel": top["kernel"], "bytes_per_launch": top["bytes_per_launch"],
+                "avg_us": top["avg_us"], "launches": top["calls"]}
     out = {
         "metric": "Mcell-steps/s, 3D LES Re3900, mixed vs FP64 at equal div residual; % HBM peak",
         "value": value, "unit": "Mcell-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": r["secs"] / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64" if head == "mixed" else "f64", "data": "synthetic",
+        "dtype": "f64+f32" if head == "mixed" else "f64", "data": "synthetic",
         "config": {"workload": a.workload, "cells": [w["nj"] - 1, w["ni"] - 1, w["nk"] - 1],
                    "re": w["re"], "dt": w["dt"], "resimax": a.resimax,
-                   "poisson": "mixed" if head == "mixed" else "fp64"},
+                   "poisson": "mixed" if head == "mixed" else "fp64",
+                   "startup": f"{a.startup_steps} untimed steps at dt={w['dt'] * a.startup_factor:g}"},
+        "roofline": roof,
     }
     for mname, rr in results.items():
-        out[f"{mname}_mcell_steps_per_s"] = cells * a.steps / rr["secs"] / 1e6
-        out[f"{mname}_sweeps_per_step"] = float(np.mean([i["niter"] for i in rr["infos"]]))
+        out[f"{mname}_mcell_steps_per_s"] = ncell * a.steps / rr["secs"] / 1e6
+        out[f"{mname}_poisson_passes_per_step"] = float(np.mean([i["niter"] for i in rr["infos"]]))
         out[f"{mname}_resip_max"] = float(max(i["resip"] for i in rr["infos"]))
     if "mixed" in results and "fp64" in results:
         out["mixed_over_fp64"] = results["fp64"]["secs"] / results["mixed"]["secs"]
+    out["kernels"] = [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in row.items()}
+                      for row in rows]
+    out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(w, log)
     print(json.dumps(out), flush=True)
 
 

@@ -56,7 +56,7 @@ EXPORTS = ("fvm_create", "fvm_destroy", "fvm_set_field", "fvm_get_field", "fvm_s
            "fvm_begin_step", "fvm_contra_vel", "fvm_nl", "fvm_stress_tensor_dy", "fvm_dif",
            "fvm_p_source", "fvm_iter_u", "fvm_correct", "fvm_iter_p", "fvm_pressure_reference",
            "fvm_calccoef", "fvm_profile_enable", "fvm_profile_get", "fvm_profile_reset",
-           "fvm_sync")
+           "fvm_sync", "fvm_set_dtime")
 
 _LIB = None
 
@@ -82,6 +82,7 @@ def load(path: str = LIB_PATH):
         getattr(L, f).argtypes = [vp]
     L.fvm_stress_tensor_dy.argtypes = [vp, pd]
     L.fvm_correct.argtypes = [vp, cd]
+    L.fvm_set_dtime.argtypes = [vp, cd]
     L.fvm_iter_p.argtypes = [vp, pd, ctypes.POINTER(ci)]
     L.fvm_pressure_reference.argtypes = [vp, pd]
     L.fvm_calccoef.argtypes = [vp, ctypes.POINTER(StepInfo)]

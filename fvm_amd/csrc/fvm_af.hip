@@ -331,9 +331,12 @@ static int chunk_stage(FvmCtx* c, AFArgs& A, int* iters) {
     for (int q = 0; q < 3; ++q) next[q] = c->afb[q][slot];
     HIP_CHECK(hipMemcpyAsync(c->afptr, prev, 3 * sizeof(double*), hipMemcpyHostToDevice, st));
     HIP_CHECK(hipMemcpyAsync(c->afptr + 3, next, 3 * sizeof(double*), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL((k_af_chunk<S, C>), dim3((unsigned)G), dim3(threads), 0, st, d, A,
-                       (const double* const*)c->dfa[S], c->ap0, (const double* const*)c->afptr,
-                       (double* const*)(c->afptr + 3), nchunk, nwork, c->red);
+    {
+      ProfScope pk(c, S == 0 ? "k_af_chunk_i" : (S == 1 ? "k_af_chunk_j" : "k_af_chunk_k"));
+      hipLaunchKernelGGL((k_af_chunk<S, C>), dim3((unsigned)G), dim3(threads), 0, st, d, A,
+                         (const double* const*)c->dfa[S], c->ap0, (const double* const*)c->afptr,
+                         (double* const*)(c->afptr + 3), nchunk, nwork, c->red);
+    }
     hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, st, c->red, G, c->scal + 20);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(c->hscal + 20, c->scal + 20, 2 * sizeof(double),

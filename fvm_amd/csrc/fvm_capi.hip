@@ -229,6 +229,11 @@ int fvm_begin_step(fvm_ctx* c) {
   return fvm_k_ufar(c);
 }
 
+int fvm_set_dtime(fvm_ctx* c, double dtime) {
+  if (!c || !(dtime > 0)) return FVM_ERR_ARG;
+  return fvm_k_set_dtime(c, dtime);
+}
+
 int fvm_contra_vel(fvm_ctx* c) { return c ? fvm_k_contra_vel(c) : FVM_ERR_ARG; }
 int fvm_nl(fvm_ctx* c) { return c ? fvm_k_nl(c) : FVM_ERR_ARG; }
 int fvm_dif(fvm_ctx* c) { return c ? fvm_k_dif(c) : FVM_ERR_ARG; }

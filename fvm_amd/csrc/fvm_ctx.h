@@ -93,6 +93,7 @@ struct ProfScope {
 
 // stage launchers (return FVM_OK or an error code)
 int fvm_geom_setup(FvmCtx* c);
+int fvm_k_set_dtime(FvmCtx* c, double dtime);
 int fvm_ghost_fill(FvmCtx* c, double* a);
 int fvm_ghost_fill_n(FvmCtx* c, double** a, int cnt);
 int fvm_k_contra_vel(FvmCtx* c);

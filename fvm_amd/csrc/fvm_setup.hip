@@ -216,6 +216,14 @@ __global__ void k_metric(Dims d, const double* const* __restrict__ D, double* dj
   }
 }
 
+// 2 Re J^-1/dt alone, after a change of the time step
+__global__ void k_set_ap0(long n, const double* __restrict__ djac, double re, double dtime,
+                          double* ap0) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n;
+       q += (long)gridDim.x * blockDim.x)
+    ap0[q] = 2.0 * re * djac[q] / dtime;
+}
+
 // constant per-cell coefficients: 2 Re J^-1/dt, squared test-filter width (J^-1)^(2/3),
 // circumferential angle of the cell centre (init.f90:210-237)
 __global__ void k_cell_consts(Dims d, const double* __restrict__ djac, const double* x,
@@ -408,5 +416,13 @@ int fvm_geom_setup(FvmCtx* c) {
   for (int a = 0; a < 9; ++a) HIP_CHECK(hipFree(D[a]));
   HIP_CHECK(hipFree(dD));
   HIP_CHECK(hipFree(dM));
+  return FVM_OK;
+}
+
+int fvm_k_set_dtime(FvmCtx* c, double dtime) {
+  c->prm.dtime = dtime;
+  hipLaunchKernelGGL(k_set_ap0, grid1(c->d.n, 256), dim3(256), 0, c->stream, c->d.n, c->djac,
+                     c->prm.re, dtime, c->ap0);
+  HIP_CHECK(hipGetLastError());
   return FVM_OK;
 }

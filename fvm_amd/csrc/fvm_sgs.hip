@@ -277,6 +277,7 @@ int fvm_k_sgs(FvmCtx* c) {
   ProfScope ps(c, "sgs");
   hipStream_t st = c->stream;
   {
+    ProfScope pk(c, "k_strain");
     dim3 g((d.ni + 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
     hipLaunchKernelGGL(k_strain, g, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
                        (double* const*)c->dlss, c->mag);
@@ -286,23 +287,32 @@ int fvm_k_sgs(FvmCtx* c) {
   int r = fvm_ghost_fill_n(c, gl, 7);
   if (r) return r;
   dim3 gi((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
-  hipLaunchKernelGGL(k_filter_vel, gi, kBlk, 0, st, d, c->u, c->v, c->w, c->uh[0], c->uh[1],
-                     c->uh[2]);
-  HIP_CHECK(hipGetLastError());
+  {
+    ProfScope pk(c, "k_filter_vel");
+    hipLaunchKernelGGL(k_filter_vel, gi, kBlk, 0, st, d, c->u, c->v, c->w, c->uh[0], c->uh[1],
+                       c->uh[2]);
+    HIP_CHECK(hipGetLastError());
+  }
   r = fvm_k_bcu_ghosts(c, c->uh, 3);
   if (r) return r;
-  hipLaunchKernelGGL(k_lm, gi, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
-                     c->uh[0], c->uh[1], c->uh[2], (const double* const*)c->dlss, c->mag,
-                     c->filt2, c->su[0], c->su[1]);
-  HIP_CHECK(hipGetLastError());
+  {
+    ProfScope pk(c, "k_lm");
+    hipLaunchKernelGGL(k_lm, gi, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
+                       c->uh[0], c->uh[1], c->uh[2], (const double* const*)c->dlss, c->mag,
+                       c->filt2, c->su[0], c->su[1]);
+    HIP_CHECK(hipGetLastError());
+  }
   const int nb = 512;
   hipLaunchKernelGGL(k_lm_reduce, dim3(nb), dim3(256), 0, st, d, c->su[0], c->su[1], c->red);
   hipLaunchKernelGGL(k_lm_final, dim3(1), dim3(256), 0, st, c->red, nb, c->scal + 0);
   hipLaunchKernelGGL(k_nut, grid1(d.n, 256), dim3(256), 0, st, d, c->scal + 0, c->filt2, c->mag,
                      c->nut);
-  hipLaunchKernelGGL(k_sgs_div, gi, kBlk, 0, st, d, (const double* const*)m, c->nut,
-                     (const double* const*)c->dlss, c->djac, c->nl[0], c->nl[1], c->nl[2],
-                     c->sgs[0], c->sgs[1], c->sgs[2]);
-  HIP_CHECK(hipGetLastError());
+  {
+    ProfScope pk(c, "k_sgs_div");
+    hipLaunchKernelGGL(k_sgs_div, gi, kBlk, 0, st, d, (const double* const*)m, c->nut,
+                       (const double* const*)c->dlss, c->djac, c->nl[0], c->nl[1], c->nl[2],
+                       c->sgs[0], c->sgs[1], c->sgs[2]);
+    HIP_CHECK(hipGetLastError());
+  }
   return FVM_OK;
 }

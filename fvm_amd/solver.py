@@ -74,6 +74,11 @@ class Solver3D:
     def sync(self):
         capi.check(self.L.fvm_sync(self.h), "fvm_sync")
 
+    def set_dtime(self, dtime: float):
+        """Change DTIME between steps (the reference reassigns it at run time)."""
+        capi.check(self.L.fvm_set_dtime(self.h, float(dtime)), "fvm_set_dtime")
+        self.dtime = float(dtime)
+
     # the subroutines of the step (reference names) ------------------------------------
     def begin_step(self): capi.check(self.L.fvm_begin_step(self.h), "begin_step")
     def contra_vel(self): capi.check(self.L.fvm_contra_vel(self.h), "contra_vel")

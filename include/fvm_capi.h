@@ -87,6 +87,12 @@ int fvm_get_field(fvm_ctx* ctx, int field, double* host);
  * passive).  info may be NULL. */
 int fvm_step(fvm_ctx* ctx, fvm_step_info* info);
 
+/* Change DTIME between steps.  DTIME is a run-time module variable of the reference, reassigned
+ * by POST_PROCESS (RLES/post_process.f90:59) and by the CFL control of CALCCOEF
+ * (RLES/calccoef.f90:27-33, commented out there); the dt-dependent coefficient 2 Re J^-1/DTIME
+ * is recomputed.  The convective history NL is kept, as on a restart with a new DTIME. */
+int fvm_set_dtime(fvm_ctx* ctx, double dtime);
+
 /* The subroutines of the time step, for drivers that keep MAIN's call sequence. */
 int fvm_begin_step(fvm_ctx* ctx);                 /* U_N = U (RLES/main.f90:42-45)          */
 int fvm_contra_vel(fvm_ctx* ctx);                 /* CONTRA_VEL (RLES/contra_vel.f90:1-58)   */

@@ -10,7 +10,7 @@
  *
  * Pinning: every routine here is checked against the flang build of the reference itself
  * (oracle/ref/build_ref.sh + oracle/ref/harness3d.f90, golden dumps under tests/golden/) and
- * against closed forms (tests/test_oracle3d.py).  See DESIGN.md for the per-function status.
+ * against closed forms (tests/test_oracle3d.py).
  *
  * Layout: every cell array is (NI+1)x(NJ+1)x(NK+1) doubles, Fortran order (I fastest), holding
  * indices 0..NI, 0..NJ, 0..NK exactly like the reference's (IS:IE,JS:JE,KS:KE) arrays with one

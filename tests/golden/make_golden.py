@@ -32,6 +32,12 @@ CASES = {
     "rles_9x21x5_les_converged": dict(ni=9, nj=21, nk=5, h1=0.08, r_out=4.0, sigma=0.05,
                                       re=200.0, dt=4e-3, nswpp=200000, resimax="1.0D-13",
                                       les=1, steps=2, keep_steps=(1, 2)),
+    # 32 x 64 x 8 cells with the 192 x 256 x 64 LES case's wall spacing (0.001 D), outer
+    # radius, Re, dt and seeded start, Poisson to 1e-8: the impulsive cold start diverges in
+    # the reference scheme (only the scalar history and the step-1 velocity are kept)
+    "rles_33x65x9_coldstart_dt2e-3": dict(ni=33, nj=65, nk=9, h1=0.001, r_out=25.0, sigma=0.05,
+                                          re=3900.0, dt=2e-3, nswpp=200000, resimax="1.0D-8",
+                                          les=1, steps=3, keep_steps=(1,), keep_tags=("u",)),
     # (no USE_LES=0 fixture: the reference's laminar mode writes YPLUS in CALCCOEF
     # (calccoef.f90:121), an array INIT allocates only when USE_LES=1 (init.f90:79-81))
 }
@@ -72,10 +78,11 @@ def run_case(label, c):
             out["in_" + k] = rs[k]
         subprocess.run([exe, str(c["steps"])], cwd=d, check=True, stdout=subprocess.DEVNULL)
         rd = lambda t: np.fromfile(os.path.join(d, t + ".bin")).reshape(shape, order="F")
-        for t in SETUP_TAGS:
+        setup_tags = SETUP_TAGS if "keep_tags" not in c else ()
+        for t in setup_tags:
             out["setup_" + t] = rd("setup_" + t)
         for s in c["keep_steps"]:
-            for t in STEP_TAGS:
+            for t in c.get("keep_tags", STEP_TAGS):
                 p = os.path.join(d, f"s{s:03d}_{t}.bin")
                 if os.path.exists(p):
                     out[f"s{s}_{t}"] = rd(f"s{s:03d}_{t}")
@@ -87,5 +94,7 @@ def run_case(label, c):
 
 
 if __name__ == "__main__":
+    only = sys.argv[1:]
     for label, c in CASES.items():
-        run_case(label, c)
+        if not only or label in only:
+            run_case(label, c)

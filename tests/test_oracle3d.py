@@ -146,3 +146,19 @@ def test_poisson_fixed_point_residual():
     o = make(g)
     o.step(int(g["nswpp"]), float(g["resimax"]), True)
     assert o.scalar("resip") <= 1e-13
+
+
+def test_coldstart_step_bitexact_and_reference_divergence():
+    """Wall spacing 0.001 D, dt 0.002, Re 3900, impulsive seeded start (the 192x256x64 LES
+    case's parameters on a 32x64x8 grid): step 1 of the oracle equals the reference's bit for
+    bit, and the reference's own CFL history grows by about 10x per step -- the impulsive start
+    at this dt diverges in the reference scheme itself."""
+    g = load_golden("rles_33x65x9_coldstart_dt2e-3")
+    o = make(g)
+    o.step(int(g["nswpp"]), float(g["resimax"]), bool(g["use_les"]))
+    row = g["scalars"][0]
+    assert o.scalar("niter") == row[1] and o.scalar("resip") == row[2]
+    assert o.scalar("cfl") == row[4] and o.scalar("cd") == row[6]
+    assert np.array_equal(o.array("u"), g["s1_u"])
+    cfl = g["scalars"][:, 4]
+    assert cfl[0] > 10 and cfl[1] > 5 * cfl[0] and cfl[2] > 50 * cfl[1]
