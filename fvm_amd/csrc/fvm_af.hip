@@ -16,6 +16,9 @@
 // each launch.
 #include "fvm_ctx.h"
 
+#include <algorithm>
+#include <vector>
+
 namespace {
 
 const dim3 kBlk(64, 4, 1);
@@ -305,6 +308,93 @@ __global__ void k_copy_ghosts(Dims d, const double* __restrict__ src, double* ds
   }
 }
 
+// Largest |coupling| of stage S over the interior: part[2*block] along the sweep direction a,
+// part[2*block+1] across it (b).  Exact zeros (extruded grid: xi_z = eta_z = zeta_x = zeta_y = 0)
+// make the lines of a stage independent, and the sweep is then exact in one pass.
+template <int S>
+__global__ void k_af_couplings(Dims d, const double* const* __restrict__ fa, double* part) {
+  __shared__ double s0[256], s1[256];
+  const long nint = (long)(d.ni - 1) * (d.nj - 1) * (d.nk - 1);
+  double ma = 0.0, mb = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nint;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = 1 + (int)(t % (d.ni - 1));
+    const long r = t / (d.ni - 1);
+    const int j = 1 + (int)(r % (d.nj - 1)), k = 1 + (int)(r / (d.nj - 1));
+    const long c = cidx(d, i, j, k);
+    const double ae = fa[0][c], aw = fa[0][c - 1], an = fa[1][c], as = fa[1][c - d.sj],
+                 at = fa[2][c], ab = fa[2][c - d.sk];
+    const double ca = S == 0 ? fmax(fabs(an), fabs(as)) : fmax(fabs(ae), fabs(aw));
+    const double cb = S == 2 ? fmax(fabs(an), fabs(as)) : fmax(fabs(at), fabs(ab));
+    ma = fmax(ma, ca);
+    mb = fmax(mb, cb);
+  }
+  const int t = threadIdx.x;
+  s0[t] = ma; s1[t] = mb;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st) { s0[t] = fmax(s0[t], s0[t + st]); s1[t] = fmax(s1[t], s1[t + st]); }
+    __syncthreads();
+  }
+  if (t == 0) { part[2 * blockIdx.x] = s0[0]; part[2 * blockIdx.x + 1] = s1[0]; }
+}
+
+// Factors of the reference's Thomas recursion along a K-line (iter_u.f90:240-255):
+// A = -AB, D = AP - A B(K-1), B = -AT/D with B(0) = 0 (the SAVE'd scratch is never written at
+// index 0).  AP carries the wall / far-field folds of BCU.  One thread per (I, J) line.
+__global__ void k_af_factor_k(Dims d, const double* const* __restrict__ fa,
+                              const double* __restrict__ ap0, double* ta, double* td,
+                              double* tb) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  if (i > d.ni - 1 || j > d.nj - 1) return;
+  double bprev = 0.0;
+  for (int k = 1; k <= d.nk - 1; ++k) {
+    const long c = cidx(d, i, j, k);
+    const double ae = fa[0][c], aw = fa[0][c - 1], an = fa[1][c], as = fa[1][c - d.sj],
+                 at = fa[2][c], ab = fa[2][c - d.sk];
+    double ap = ap0[c] + ae + aw + an + as + at + ab;
+    if (i == 1) ap = ap + aw;          // bcu.f90:18
+    if (i == d.ni - 1) ap = ap + ae;   // bcu.f90:38
+    const double A = -ab;
+    const double D = ap - A * bprev;
+    const double B = -at / D;
+    ta[c] = A; td[c] = D; tb[c] = B;
+    bprev = B;
+  }
+}
+
+// The K-line sweep when its lines are independent: one thread per line and component, lanes
+// along I (coalesced), the reference's forward elimination and back substitution in its
+// operation order, in place.  With the I and J couplings identically zero the lagged-neighbour
+// sum C of the reference is exactly SU (the BCU additions to SU multiply the same zeros).
+__global__ void __launch_bounds__(256) k_af_thomas_k(Dims d, AFArgs A,
+                                                     const double* __restrict__ ta,
+                                                     const double* __restrict__ td,
+                                                     const double* __restrict__ tb) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int comp = blockIdx.z;
+  if (i > d.ni - 1 || j > d.nj - 1) return;
+  double* f = A.f[comp];
+  const double* su = A.su[comp];
+  const int n = d.nk - 1;
+  const long c0 = cidx(d, i, j, 0);
+  double x = f[c0];  // K ghost of the stage input
+  for (int k = 1; k <= n; ++k) {
+    const long c = c0 + (long)k * d.sk;
+    x = (su[c] - ta[c] * x) / td[c];
+    f[c] = x;
+  }
+  double xn = f[c0 + (long)(n + 1) * d.sk];
+  for (int k = n; k >= 1; --k) {
+    const long c = c0 + (long)k * d.sk;
+    x = f[c] - tb[c] * xn;
+    f[c] = x;
+    xn = x;
+  }
+}
+
 }  // namespace
 
 template <int S, int C>
@@ -392,7 +482,15 @@ int fvm_k_iter_u(FvmCtx* c) {
     int r = fvm_k_bcu_ghosts(c, fld, 3);
     if (r) return r;
     ProfScope ps(c, s == 0 ? "af_sweep_i" : (s == 1 ? "af_sweep_j" : "af_sweep_k"));
-    if (c->prm.af_mode == 1) {
+    if (s == 2 && c->prm.af_mode == 0 && c->tk_a && c->af_zero_a[2] && c->af_zero_b[2]) {
+      // independent K-lines: the exact sweep in one pass
+      ProfScope pk(c, "k_af_thomas_k");
+      const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, 3);
+      hipLaunchKernelGGL(k_af_thomas_k, g, dim3(64, 4, 1), 0, st, d, A, c->tk_a, c->tk_d,
+                         c->tk_b);
+      HIP_CHECK(hipGetLastError());
+      c->af_iters[s] = 1;
+    } else if (c->prm.af_mode == 1) {
       // anti-diagonal wavefront: one launch per anti-diagonal of the line plane
       const int na = s == 0 ? d.nj - 1 : d.ni - 1;
       const int nb = s == 2 ? d.nj - 1 : d.nk - 1;
@@ -415,6 +513,42 @@ int fvm_k_iter_u(FvmCtx* c) {
       if (r) return r;
       c->af_iters[s] = it;
     }
+  }
+  return FVM_OK;
+}
+
+// Which stage couplings vanish identically, and the K-line Thomas factors when that stage's
+// lines are independent.  Called after the geometry set-up and after every change of dt.
+int fvm_af_setup(FvmCtx* c) {
+  const Dims d = c->d;
+  hipStream_t st = c->stream;
+  const int nb = 256;
+  std::vector<double> h(2 * nb);
+  for (int s = 0; s < 3; ++s) {
+    if (s == 0) hipLaunchKernelGGL(k_af_couplings<0>, dim3(nb), dim3(256), 0, st, d, (const double* const*)c->dfa[0], c->red);
+    if (s == 1) hipLaunchKernelGGL(k_af_couplings<1>, dim3(nb), dim3(256), 0, st, d, (const double* const*)c->dfa[1], c->red);
+    if (s == 2) hipLaunchKernelGGL(k_af_couplings<2>, dim3(nb), dim3(256), 0, st, d, (const double* const*)c->dfa[2], c->red);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(h.data(), c->red, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    double ma = 0.0, mb = 0.0;
+    for (int q = 0; q < nb; ++q) { ma = std::max(ma, h[2 * q]); mb = std::max(mb, h[2 * q + 1]); }
+    c->af_zero_a[s] = (ma == 0.0);
+    c->af_zero_b[s] = (mb == 0.0);
+  }
+  if (c->af_zero_a[2] && c->af_zero_b[2]) {
+    if (!c->tk_a) {
+      double** arr[3] = {&c->tk_a, &c->tk_d, &c->tk_b};
+      for (double** q : arr) {
+        HIP_CHECK(hipMalloc((void**)q, d.n * sizeof(double)));
+        c->allocs.push_back(*q);
+      }
+    }
+    const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, 1);
+    hipLaunchKernelGGL(k_af_factor_k, g, dim3(64, 4, 1), 0, st, d,
+                       (const double* const*)c->dfa[2], c->ap0, c->tk_a, c->tk_d, c->tk_b);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(st));
   }
   return FVM_OK;
 }

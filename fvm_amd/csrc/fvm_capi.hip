@@ -177,6 +177,8 @@ int fvm_create(const fvm_params* prm, const double* xg, const double* yg, const 
   }
   r = fvm_geom_setup(c);
   if (r) { fvm_destroy(c); return r; }
+  r = fvm_af_setup(c);
+  if (r) { fvm_destroy(c); return r; }
   r = fvm_sep_setup(c);
   if (r) { fvm_destroy(c); return r; }
   *out = c;

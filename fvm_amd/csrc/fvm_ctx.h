@@ -61,6 +61,10 @@ struct FvmCtx {
   double* afb[3][2] = {};
   double** afptr = nullptr;
   int af_iters[3] = {};
+  // stage couplings that vanish identically on this grid (a: along the sweep, b: across it)
+  bool af_zero_a[3] = {}, af_zero_b[3] = {};
+  // Thomas factors A, D, B of the K-line stage when its lines are independent
+  double *tk_a = nullptr, *tk_d = nullptr, *tk_b = nullptr;
   // reductions
   double* red = nullptr;     // partial sums
   double* scal = nullptr;    // device scalars (c_avg, residual sums, pref, ...)
@@ -94,6 +98,7 @@ struct ProfScope {
 // stage launchers (return FVM_OK or an error code)
 int fvm_geom_setup(FvmCtx* c);
 int fvm_k_set_dtime(FvmCtx* c, double dtime);
+int fvm_af_setup(FvmCtx* c);
 int fvm_ghost_fill(FvmCtx* c, double* a);
 int fvm_ghost_fill_n(FvmCtx* c, double** a, int cnt);
 int fvm_k_contra_vel(FvmCtx* c);

@@ -424,5 +424,5 @@ int fvm_k_set_dtime(FvmCtx* c, double dtime) {
   hipLaunchKernelGGL(k_set_ap0, grid1(c->d.n, 256), dim3(256), 0, c->stream, c->d.n, c->djac,
                      c->prm.re, dtime, c->ap0);
   HIP_CHECK(hipGetLastError());
-  return FVM_OK;
+  return fvm_af_setup(c);
 }

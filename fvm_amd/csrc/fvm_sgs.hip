@@ -101,74 +101,135 @@ __global__ void k_filter_vel(Dims d, const double* __restrict__ U, const double*
   wh[c] = filter27(d, c, [&](long q) { return W[q]; });
 }
 
-// Germano/Lilly contractions LM and MM per interior cell (stress_tensor_dy.f90:236-483)
-__global__ void k_lm(Dims d, const double* const* __restrict__ m, const double* __restrict__ U,
-                     const double* __restrict__ V, const double* __restrict__ W,
-                     const double* __restrict__ UH, const double* __restrict__ VH,
-                     const double* __restrict__ WH, const double* const* __restrict__ L,
-                     const double* __restrict__ mag, const double* __restrict__ filt2,
-                     double* lm_out, double* mm_out) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (i > d.ni - 1 || j > d.nj - 1) return;
-  const long c = cidx(d, i, j, k);
-  const long sj = d.sj, sk = d.sk;
-  const double xix = m[0][c], etx = m[1][c], ztx = m[2][c];
-  const double xiy = m[3][c], ety = m[4][c], zty = m[5][c];
-  const double xiz = m[6][c], etz = m[7][c], ztz = m[8][c];
+// Germano/Lilly contractions LM and MM per interior cell (stress_tensor_dy.f90:236-483).
+// The twelve test-filtered products (u_i u_j and |S| S_ij) are never stored: a workgroup owns a
+// 64 x 4 (I x J) column of cells over LM_KC consecutive K planes, stages each plane of the ten
+// factor fields (with a one-cell halo) in LDS, forms the in-plane (1,2,1)x(1,2,1) sums of the
+// products once per plane and combines three consecutive planes with (1,2,1) weights.  The sum
+// is the reference's 27-point filter with its terms grouped by plane (rounding-level
+// differences only).
+constexpr int LM_KC = 8;
+constexpr int LM_W = 66, LM_H = 6;  // tile with halo
+
+__global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restrict__ m,
+                                            const double* __restrict__ U,
+                                            const double* __restrict__ V,
+                                            const double* __restrict__ W,
+                                            const double* __restrict__ UH,
+                                            const double* __restrict__ VH,
+                                            const double* __restrict__ WH,
+                                            const double* const* __restrict__ L,
+                                            const double* __restrict__ mag,
+                                            const double* __restrict__ filt2, double* lm_out,
+                                            double* mm_out) {
+  __shared__ double sh[10][LM_W * LM_H];
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+  const int i = 1 + blockIdx.x * 64 + tx;
+  const int j = 1 + blockIdx.y * 4 + ty;
+  const int k0 = 1 + blockIdx.z * LM_KC;
+  const int k1 = min(k0 + LM_KC - 1, d.nk - 1);
+  const bool act = i <= d.ni - 1 && j <= d.nj - 1;
+  const double* L0 = L[0]; const double* L1 = L[1]; const double* L2 = L[2];
+  const double* L3 = L[3]; const double* L4 = L[4]; const double* L5 = L[5];
+  // rolling in-plane sums of the products for planes k-1, k, k+1; product order:
+  // uu vv ww uv uw vw, |S|S11 |S|S22 |S|S33 |S|S12 |S|S13 |S|S23
+  double Pa[12], Pb[12], Pc[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) Pa[q] = Pb[q] = Pc[q] = 0.0;
+  for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
+    __syncthreads();
+    for (int q = tid; q < LM_W * LM_H; q += 256) {
+      const int gi = min(blockIdx.x * 64 + q % LM_W, d.ni);
+      const int gj = min(blockIdx.y * 4 + q / LM_W, d.nj);
+      const long c = cidx(d, gi, gj, kp);
+      sh[0][q] = U[c]; sh[1][q] = V[c]; sh[2][q] = W[c]; sh[3][q] = mag[c];
+      sh[4][q] = L0[c]; sh[5][q] = L1[c]; sh[6][q] = L2[c];
+      sh[7][q] = L3[c]; sh[8][q] = L4[c]; sh[9][q] = L5[c];
+    }
+    __syncthreads();
+    double Pn[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) Pn[q] = 0.0;
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int di = -1; di <= 1; ++di) {
+        const double wgt = (double)((di == 0 ? 2 : 1) * (dj == 0 ? 2 : 1));
+        const int q = (ty + 1 + dj) * LM_W + tx + 1 + di;
+        const double u = sh[0][q], v = sh[1][q], w = sh[2][q], mg = sh[3][q];
+        Pn[0] += wgt * (u * u);
+        Pn[1] += wgt * (v * v);
+        Pn[2] += wgt * (w * w);
+        Pn[3] += wgt * (u * v);
+        Pn[4] += wgt * (u * w);
+        Pn[5] += wgt * (v * w);
+        Pn[6] += wgt * (mg * sh[4][q]);
+        Pn[7] += wgt * (mg * sh[7][q]);
+        Pn[8] += wgt * (mg * sh[9][q]);
+        Pn[9] += wgt * (mg * sh[5][q]);
+        Pn[10] += wgt * (mg * sh[6][q]);
+        Pn[11] += wgt * (mg * sh[8][q]);
+      }
+#pragma unroll
+    for (int q = 0; q < 12; ++q) { Pa[q] = Pb[q]; Pb[q] = Pc[q]; Pc[q] = Pn[q]; }
+    if (kp < k0 + 1 || !act) continue;
+    // finish cell (i, j, kp - 1)
+    const int k = kp - 1;
+    const long c = cidx(d, i, j, k);
+    const long sj = d.sj, sk = d.sk;
+    const double xix = m[0][c], etx = m[1][c], ztx = m[2][c];
+    const double xiy = m[3][c], ety = m[4][c], zty = m[5][c];
+    const double xiz = m[6][c], etz = m[7][c], ztz = m[8][c];
 #define DI(F) (F[c + 1] - F[c - 1])
 #define DJ(F) (F[c + sj] - F[c - sj])
 #define DK(F) (F[c + sk] - F[c - sk])
-  // raw (unscaled) test-filtered strain sums; S11 = 0.5 r11, S12 = 0.25 r12, ...
-  const double r11 = xix * DI(UH) + etx * DJ(UH) + ztx * DK(UH);
-  const double r12 = xiy * DI(UH) + ety * DJ(UH) + zty * DK(UH) + xix * DI(VH) + etx * DJ(VH) +
-                     ztx * DK(VH);
-  const double r13 = xiz * DI(UH) + etz * DJ(UH) + ztz * DK(UH) + xix * DI(WH) + etx * DJ(WH) +
-                     ztx * DK(WH);
-  const double r22 = xiy * DI(VH) + ety * DJ(VH) + zty * DK(VH);
-  const double r23 = xiz * DI(VH) + etz * DJ(VH) + ztz * DK(VH) + xiy * DI(WH) + ety * DJ(WH) +
-                     zty * DK(WH);
-  const double r33 = xiz * DI(WH) + etz * DJ(WH) + ztz * DK(WH);
+    // raw (unscaled) test-filtered strain sums; S11 = 0.5 r11, S12 = 0.25 r12, ...
+    const double r11 = xix * DI(UH) + etx * DJ(UH) + ztx * DK(UH);
+    const double r12 = xiy * DI(UH) + ety * DJ(UH) + zty * DK(UH) + xix * DI(VH) +
+                       etx * DJ(VH) + ztx * DK(VH);
+    const double r13 = xiz * DI(UH) + etz * DJ(UH) + ztz * DK(UH) + xix * DI(WH) +
+                       etx * DJ(WH) + ztx * DK(WH);
+    const double r22 = xiy * DI(VH) + ety * DJ(VH) + zty * DK(VH);
+    const double r23 = xiz * DI(VH) + etz * DJ(VH) + ztz * DK(VH) + xiy * DI(WH) +
+                       ety * DJ(WH) + zty * DK(WH);
+    const double r33 = xiz * DI(WH) + etz * DJ(WH) + ztz * DK(WH);
 #undef DI
 #undef DJ
 #undef DK
-  const double s11 = 0.5 * r11, s12 = 0.25 * r12, s13 = 0.25 * r13, s22 = 0.5 * r22,
-               s23 = 0.25 * r23, s33 = 0.5 * r33;
-  const double dum = 2.0 * (s11 * s11 + s12 * s12 + s13 * s13 + s12 * s12 + s22 * s22 +
-                            s23 * s23 + s13 * s13 + s23 * s23 + s33 * s33);
-  const double mh = sqrt(dum);
-  const double fw2 = filt2[c];
-  const double uh = UH[c], vh = VH[c], wh = WH[c];
-  double lm = 0.0, mm = 0.0;
-  // reference component order: 11, 22, 33, 12, 13, 23
+    const double s11 = 0.5 * r11, s12 = 0.25 * r12, s13 = 0.25 * r13, s22 = 0.5 * r22,
+                 s23 = 0.25 * r23, s33 = 0.5 * r33;
+    const double dum = 2.0 * (s11 * s11 + s12 * s12 + s13 * s13 + s12 * s12 + s22 * s22 +
+                              s23 * s23 + s13 * s13 + s23 * s23 + s33 * s33);
+    const double mh = sqrt(dum);
+    const double fw2 = filt2[c];
+    const double uh = UH[c], vh = VH[c], wh = WH[c];
+    double F[12];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    const double* A = q == 0 ? U : q == 1 ? V : q == 2 ? W : q == 3 ? U : q == 4 ? U : V;
-    const double* B = q == 0 ? U : q == 1 ? V : q == 2 ? W : q == 3 ? V : q == 4 ? W : W;
-    const double ha = q == 0 ? uh : q == 1 ? vh : q == 2 ? wh : q == 3 ? uh : q == 4 ? uh : vh;
-    const double hb = q == 0 ? uh : q == 1 ? vh : q == 2 ? wh : q == 3 ? vh : q == 4 ? wh : wh;
-    const int lq = q == 0 ? 0 : q == 1 ? 3 : q == 2 ? 5 : q == 3 ? 1 : q == 4 ? 2 : 4;
-    const double raw = q == 0 ? r11 : q == 1 ? r22 : q == 2 ? r33 : q == 3 ? r12 : q == 4 ? r13 : r23;
-    const double* Lq = L[lq];
-    double fuu = filter27(d, c, [&](long p) { return A[p] * B[p]; });
-    double Lij = -fuu + ha * hb;
-    double mt = q < 3 ? 4.0 * mh * 0.5 * raw : 4.0 * mh * 0.25 * raw;
-    double fms = filter27(d, c, [&](long p) { return mag[p] * Lq[p]; });
-    double M = fw2 * (mt - fms);
-    if (q == 0) {
-      lm = Lij * M;
-      mm = M * M;
-    } else if (q < 3) {
-      lm = lm + Lij * M;
-      mm = mm + M * M;
-    } else {
-      lm = lm + 2.0 * Lij * M;
-      mm = mm + 2.0 * M * M;
+    for (int q = 0; q < 12; ++q) F[q] = (Pa[q] + 2.0 * Pb[q] + Pc[q]) / 64.0;
+    // reference component order: 11, 22, 33, 12, 13, 23
+    const double hA[6] = {uh, vh, wh, uh, uh, vh};
+    const double hB[6] = {uh, vh, wh, vh, wh, wh};
+    const double raw[6] = {r11, r22, r33, r12, r13, r23};
+    double lm = 0.0, mm = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const double Lij = -F[q] + hA[q] * hB[q];
+      const double mt = q < 3 ? 4.0 * mh * 0.5 * raw[q] : 4.0 * mh * 0.25 * raw[q];
+      const double M = fw2 * (mt - F[6 + q]);
+      if (q == 0) {
+        lm = Lij * M;
+        mm = M * M;
+      } else if (q < 3) {
+        lm = lm + Lij * M;
+        mm = mm + M * M;
+      } else {
+        lm = lm + 2.0 * Lij * M;
+        mm = mm + 2.0 * M * M;
+      }
     }
+    lm_out[c] = lm;
+    mm_out[c] = mm;
   }
-  lm_out[c] = lm;
-  mm_out[c] = mm;
 }
 
 // conditional sums over cells with 0.5 LM/MM >= 0 (stress_tensor_dy.f90:486-507):
@@ -297,7 +358,8 @@ int fvm_k_sgs(FvmCtx* c) {
   if (r) return r;
   {
     ProfScope pk(c, "k_lm");
-    hipLaunchKernelGGL(k_lm, gi, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
+    const dim3 gl((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + LM_KC - 1) / LM_KC);
+    hipLaunchKernelGGL(k_lm, gl, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
                        c->uh[0], c->uh[1], c->uh[2], (const double* const*)c->dlss, c->mag,
                        c->filt2, c->su[0], c->su[1]);
     HIP_CHECK(hipGetLastError());
