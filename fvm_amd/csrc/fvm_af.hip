@@ -23,17 +23,6 @@ namespace {
 
 const dim3 kBlk(64, 4, 1);
 
-struct AFArgs {
-  double* f[3];           // stage input / output field per component
-  const double* fn[3];    // step-start field U_N per component
-  const double* nl[3];
-  const double* nl1[3];
-  const double* dif[3];
-  const double* s[3];
-  double* su[3];
-  const double* ufar;     // far-field boundary values, (NJ+1)(NK+1) per component
-};
-
 // right-hand side of stage s (iter_u.f90:49-81, 129-158, 206-235)
 template <int S>
 __global__ void k_af_rhs(Dims d, AFArgs A, const double* const* __restrict__ m,
@@ -312,10 +301,11 @@ __global__ void k_copy_ghosts(Dims d, const double* __restrict__ src, double* ds
 // part[2*block+1] across it (b).  Exact zeros (extruded grid: xi_z = eta_z = zeta_x = zeta_y = 0)
 // make the lines of a stage independent, and the sweep is then exact in one pass.
 template <int S>
-__global__ void k_af_couplings(Dims d, const double* const* __restrict__ fa, double* part) {
-  __shared__ double s0[256], s1[256];
+__global__ void k_af_couplings(Dims d, const double* const* __restrict__ fa,
+                               const double* __restrict__ ap0, double* part) {
+  __shared__ double s0[256], s1[256], s2[256];
   const long nint = (long)(d.ni - 1) * (d.nj - 1) * (d.nk - 1);
-  double ma = 0.0, mb = 0.0;
+  double ma = 0.0, mb = 0.0, rb = 0.0;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nint;
        t += (long)gridDim.x * blockDim.x) {
     const int i = 1 + (int)(t % (d.ni - 1));
@@ -326,17 +316,25 @@ __global__ void k_af_couplings(Dims d, const double* const* __restrict__ fa, dou
                  at = fa[2][c], ab = fa[2][c - d.sk];
     const double ca = S == 0 ? fmax(fabs(an), fabs(as)) : fmax(fabs(ae), fabs(aw));
     const double cb = S == 2 ? fmax(fabs(an), fabs(as)) : fmax(fabs(at), fabs(ab));
+    const double sb = S == 2 ? fabs(an) + fabs(as) : fabs(at) + fabs(ab);
     ma = fmax(ma, ca);
     mb = fmax(mb, cb);
+    rb = fmax(rb, sb / ap0[c]);
   }
   const int t = threadIdx.x;
-  s0[t] = ma; s1[t] = mb;
+  s0[t] = ma; s1[t] = mb; s2[t] = rb;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if (t < st) { s0[t] = fmax(s0[t], s0[t + st]); s1[t] = fmax(s1[t], s1[t + st]); }
+    if (t < st) {
+      s0[t] = fmax(s0[t], s0[t + st]);
+      s1[t] = fmax(s1[t], s1[t + st]);
+      s2[t] = fmax(s2[t], s2[t + st]);
+    }
     __syncthreads();
   }
-  if (t == 0) { part[2 * blockIdx.x] = s0[0]; part[2 * blockIdx.x + 1] = s1[0]; }
+  if (t == 0) {
+    part[3 * blockIdx.x] = s0[0]; part[3 * blockIdx.x + 1] = s1[0]; part[3 * blockIdx.x + 2] = s2[0];
+  }
 }
 
 // Factors of the reference's Thomas recursion along a K-line (iter_u.f90:240-255):
@@ -490,6 +488,11 @@ int fvm_k_iter_u(FvmCtx* c) {
                          c->tk_b);
       HIP_CHECK(hipGetLastError());
       c->af_iters[s] = 1;
+    } else if (s < 2 && c->prm.af_mode == 0 && c->af_scan[s]) {
+      int it = 0;
+      r = fvm_af_scan_stage(c, s, A, &it);
+      if (r) return r;
+      c->af_iters[s] = it;
     } else if (c->prm.af_mode == 1) {
       // anti-diagonal wavefront: one launch per anti-diagonal of the line plane
       const int na = s == 0 ? d.nj - 1 : d.ni - 1;
@@ -523,18 +526,21 @@ int fvm_af_setup(FvmCtx* c) {
   const Dims d = c->d;
   hipStream_t st = c->stream;
   const int nb = 256;
-  std::vector<double> h(2 * nb);
+  std::vector<double> h(3 * nb);
   for (int s = 0; s < 3; ++s) {
-    if (s == 0) hipLaunchKernelGGL(k_af_couplings<0>, dim3(nb), dim3(256), 0, st, d, (const double* const*)c->dfa[0], c->red);
-    if (s == 1) hipLaunchKernelGGL(k_af_couplings<1>, dim3(nb), dim3(256), 0, st, d, (const double* const*)c->dfa[1], c->red);
-    if (s == 2) hipLaunchKernelGGL(k_af_couplings<2>, dim3(nb), dim3(256), 0, st, d, (const double* const*)c->dfa[2], c->red);
+    if (s == 0) hipLaunchKernelGGL(k_af_couplings<0>, dim3(nb), dim3(256), 0, st, d, (const double* const*)c->dfa[0], c->ap0, c->red);
+    if (s == 1) hipLaunchKernelGGL(k_af_couplings<1>, dim3(nb), dim3(256), 0, st, d, (const double* const*)c->dfa[1], c->ap0, c->red);
+    if (s == 2) hipLaunchKernelGGL(k_af_couplings<2>, dim3(nb), dim3(256), 0, st, d, (const double* const*)c->dfa[2], c->ap0, c->red);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(h.data(), c->red, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(h.data(), c->red, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    double ma = 0.0, mb = 0.0;
-    for (int q = 0; q < nb; ++q) { ma = std::max(ma, h[2 * q]); mb = std::max(mb, h[2 * q + 1]); }
+    double ma = 0.0, mb = 0.0, rb = 0.0;
+    for (int q = 0; q < nb; ++q) {
+      ma = std::max(ma, h[3 * q]); mb = std::max(mb, h[3 * q + 1]); rb = std::max(rb, h[3 * q + 2]);
+    }
     c->af_zero_a[s] = (ma == 0.0);
     c->af_zero_b[s] = (mb == 0.0);
+    c->af_b_ratio[s] = rb;
   }
   if (c->af_zero_a[2] && c->af_zero_b[2]) {
     if (!c->tk_a) {
@@ -550,5 +556,5 @@ int fvm_af_setup(FvmCtx* c) {
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(st));
   }
-  return FVM_OK;
+  return fvm_af_scan_setup(c);
 }

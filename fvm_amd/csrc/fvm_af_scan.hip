@@ -1,0 +1,457 @@
+// Line-scan form of the I and J stages of ITER_U (LES/v2.3_LES-Reynolds stresses/
+// iter_u.f90:86-101 for I-lines, 163-178 for J-lines) on grids whose K couplings of these
+// stages are negligible: on the extruded O-grid zeta_x and zeta_y are rounding residues of the
+// metric (~1e-12), so AT, AB ~ 1e-24 J^-1 against a diagonal AP >= 2 Re J^-1/dt.  The form is
+// enabled when (|AT| + |AB|) / AP <= 1e-15 everywhere; those terms are then taken from the
+// stage input instead of the current sweep, a change below FP64 resolution.
+//
+// Each K plane is then an independent line Gauss-Seidel sweep: lines a = 1..na in order, each a
+// tridiagonal solve along its rows p = 1..n whose right-hand side takes the line a-1 just solved
+// and the stage input at line a+1 (the reference's lexicographic order within the plane).
+//
+// * The tridiagonal matrices depend only on the geometry, Re and dt: the reference's Thomas
+//   factors A = -(lower coefficient), D = AP - A B(p-1), B = -(upper coefficient)/D
+//   (B(0) = 0) are computed once per dt and stored as beta = -A/D, 1/D, B, together with the
+//   coupling to line a-1 (BCU's wall / far-field folds included, bcu.f90:15-44).
+// * A line solve is the forward recurrence y(p) = alpha(p) + beta(p) y(p-1),
+//   alpha = (Q + cam x_{a-1}) / D, and the back substitution x(p) = y(p) - B(p) x(p+1); both are
+//   first-order affine recurrences, evaluated by one 64-lane wavefront as an affine-map scan
+//   (each lane composes its block of rows, then a log2(64)-step shuffle scan, no LDS barriers).
+//   Q = SU' + (coupling to line a+1) x_in(a+1) is formed once per stage.
+// * A wavefront owns C consecutive lines of one plane and sweeps them in order; chunks are
+//   coupled through the last line of the preceding chunk from the previous pass (block Jacobi
+//   across chunks).  The passes are nilpotent (exact after nchunk passes) and contractive by the
+//   diagonal dominance of AP; the host stops once the chunk boundary lines no longer change
+//   (max change <= af_tol max |x|).  The result equals the reference's sweep up to rounding.
+// * Rows are contiguous for every load: I-lines are contiguous in the natural layout; for
+//   J-lines the line factors, Q and the iterate live in a line-major copy (J fastest), and the
+//   converged iterate is transposed back into the field.
+#include "fvm_ctx.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+constexpr int SC_C = 8;   // lines per chunk
+constexpr int SC_R = 8;   // max rows per lane (n <= 512)
+
+struct ScanGeo {
+  int n, na, nb;          // rows per line, lines per plane, planes
+  long ls_a, ls_b;        // line-major strides of a and b (row stride 1)
+  long ns_p, ns_a, ns_b;  // natural strides of p, a, b
+};
+
+__host__ __device__ inline ScanGeo scan_geo(const Dims& d, int S) {
+  ScanGeo g;
+  if (S == 0) {  // I-lines: p = i, a = j, b = k; line-major == natural
+    g.n = d.ni - 1; g.na = d.nj - 1; g.nb = d.nk - 1;
+    g.ls_a = d.sj; g.ls_b = d.sk;
+    g.ns_p = 1; g.ns_a = d.sj; g.ns_b = d.sk;
+  } else {       // J-lines: p = j, a = i, b = k; line-major index j + (NJ+1)(i + (NI+1) k)
+    g.n = d.nj - 1; g.na = d.ni - 1; g.nb = d.nk - 1;
+    g.ls_a = d.nj + 1; g.ls_b = (long)(d.nj + 1) * (d.ni + 1);
+    g.ns_p = d.sj; g.ns_a = 1; g.ns_b = d.sk;
+  }
+  return g;
+}
+
+// Line factors of stage S, one thread per line (a, b), rows in order (setup only).
+template <int S>
+__global__ void k_scan_factor(Dims d, const double* const* __restrict__ fa,
+                              const double* __restrict__ ap0, double* beta, double* invd,
+                              double* bb, double* cam) {
+  const ScanGeo g = scan_geo(d, S);
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)g.na * g.nb) return;
+  const int a = 1 + (int)(t % g.na), b = 1 + (int)(t / g.na);
+  double bprev = 0.0;
+  for (int p = 1; p <= g.n; ++p) {
+    const long c = g.ns_p * p + g.ns_a * a + g.ns_b * b;
+    const long l = (long)p + g.ls_a * a + g.ls_b * b;
+    const int i = S == 0 ? p : a;
+    double ae = fa[0][c], aw = fa[0][c - 1], an = fa[1][c], as = fa[1][c - d.sj],
+           at = fa[2][c], ab = fa[2][c - d.sk];
+    double ap = ap0[c] + ae + aw + an + as + at + ab;
+    if (i == 1) { ap = ap + aw; aw = 0.0; }          // bcu.f90:18-20
+    if (i == d.ni - 1) { ap = ap + ae; ae = 0.0; }   // bcu.f90:38-40
+    const double lo = S == 0 ? aw : as;   // reference A(p) = -lo
+    const double up = S == 0 ? ae : an;   // reference B(p) = -up / D(p)
+    const double A = -lo;
+    const double D = ap - A * bprev;
+    const double B = -up / D;
+    beta[l] = -A / D;
+    invd[l] = 1.0 / D;
+    bb[l] = B;
+    cam[l] = S == 0 ? as : aw;
+    bprev = B;
+  }
+}
+
+// Q = SU' + cap x_in(a+1) in line-major order; SU' carries BCU's far-field addition
+// 2 AE U_F at I = NI-1 (bcu.f90:38-40; the wall addition multiplies U_F = 0).
+// Stage 0: one thread per cell.  Stage 1: 32 x 32 (I x J) tiles transposed through LDS.
+__device__ inline double scan_q_value(const Dims& d, int S, const AFArgs& A,
+                                      const double* const* fa, int comp, int i, int j, int k) {
+  const long c = cidx(d, i, j, k);
+  const long nf = (long)(d.nj + 1) * (d.nk + 1);
+  double s = A.su[comp][c];
+  double ae = fa[0][c];
+  if (i == 1) s = s + 2.0 * fa[0][c - 1] * 0.0;
+  if (i == d.ni - 1) {
+    s = s + 2.0 * ae * A.ufar[comp * nf + j + (long)(d.nj + 1) * k];
+    ae = 0.0;
+  }
+  const double* f = A.f[comp];
+  double qv = S == 0 ? s + fa[1][c] * f[c + d.sj] : s + ae * f[c + 1];
+  // K couplings (below 1e-15 of the diagonal when this form is enabled) from the stage input
+  qv = qv + fa[2][c] * f[c + d.sk];
+  qv = qv + fa[2][c - d.sk] * f[c - d.sk];
+  return qv;
+}
+
+__global__ void k_scan_q0(Dims d, AFArgs A, const double* const* __restrict__ fa,
+                          double* const* q) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + (int)(blockIdx.z % (d.nk - 1));
+  const int comp = (int)(blockIdx.z / (d.nk - 1));
+  if (i > d.ni - 1 || j > d.nj - 1) return;
+  q[comp][cidx(d, i, j, k)] = scan_q_value(d, 0, A, fa, comp, i, j, k);
+}
+
+__global__ void __launch_bounds__(256) k_scan_q1(Dims d, AFArgs A,
+                                                 const double* const* __restrict__ fa,
+                                                 double* const* q) {
+  __shared__ double tile[32][33];
+  const int i0 = 1 + blockIdx.x * 32, j0 = 1 + blockIdx.y * 32;
+  const int k = 1 + (int)(blockIdx.z % (d.nk - 1));
+  const int comp = (int)(blockIdx.z / (d.nk - 1));
+  const int t = threadIdx.x;
+  for (int r = 0; r < 4; ++r) {
+    const int li = t % 32, lj = t / 32 + 8 * r;
+    const int i = i0 + li, j = j0 + lj;
+    if (i <= d.ni - 1 && j <= d.nj - 1) tile[lj][li] = scan_q_value(d, 1, A, fa, comp, i, j, k);
+  }
+  __syncthreads();
+  const long lsa = d.nj + 1, lsb = (long)(d.nj + 1) * (d.ni + 1);
+  for (int r = 0; r < 4; ++r) {
+    const int lj = t % 32, li = t / 32 + 8 * r;
+    const int i = i0 + li, j = j0 + lj;
+    if (i <= d.ni - 1 && j <= d.nj - 1) q[comp][j + lsa * i + lsb * k] = tile[lj][li];
+  }
+}
+
+// converged J-line iterate (line-major) back into the field interior
+__global__ void __launch_bounds__(256) k_scan_unpack1(Dims d, const double* const* xo,
+                                                      double* const* f) {
+  __shared__ double tile[32][33];
+  const int i0 = 1 + blockIdx.x * 32, j0 = 1 + blockIdx.y * 32;
+  const int k = 1 + (int)(blockIdx.z % (d.nk - 1));
+  const int comp = (int)(blockIdx.z / (d.nk - 1));
+  const int t = threadIdx.x;
+  const long lsa = d.nj + 1, lsb = (long)(d.nj + 1) * (d.ni + 1);
+  for (int r = 0; r < 4; ++r) {
+    const int lj = t % 32, li = t / 32 + 8 * r;
+    const int i = i0 + li, j = j0 + lj;
+    if (i <= d.ni - 1 && j <= d.nj - 1) tile[lj][li] = xo[comp][j + lsa * i + lsb * k];
+  }
+  __syncthreads();
+  for (int r = 0; r < 4; ++r) {
+    const int li = t % 32, lj = t / 32 + 8 * r;
+    const int i = i0 + li, j = j0 + lj;
+    if (i <= d.ni - 1 && j <= d.nj - 1) f[comp][cidx(d, i, j, k)] = tile[lj][li];
+  }
+}
+
+// boundary lines of the first pass: the stage input at the last line of every chunk
+// (bnd[((comp nb + b-1) nchunk + ca) n + p-1])
+template <int S>
+__global__ void k_scan_seed(Dims d, AFArgs A, int nchunk, double* bnd) {
+  const ScanGeo g = scan_geo(d, S);
+  const long tot = 3L * g.nb * nchunk * g.n;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot;
+       t += (long)gridDim.x * blockDim.x) {
+    const int p = 1 + (int)(t % g.n);
+    long r = t / g.n;
+    const int ca = (int)(r % nchunk);
+    r /= nchunk;
+    const int b = 1 + (int)(r % g.nb);
+    const int comp = (int)(r / g.nb);
+    const int a = min((ca + 1) * SC_C, g.na);
+    bnd[t] = A.f[comp][g.ns_p * p + g.ns_a * a + g.ns_b * b];
+  }
+}
+
+__device__ inline double shfl_up_d(double v, int o) { return __shfl_up(v, o, 64); }
+__device__ inline double shfl_down_d(double v, int o) { return __shfl_down(v, o, 64); }
+
+// One pass: wavefront w owns chunk ca of plane b and sweeps its lines in order, all three
+// components.  Lane l owns rows l*R+1 .. l*R+R.  Writes the iterate (line-major), the chunk's
+// last line to bnd_out, and per wavefront the largest change of that line between passes and
+// the largest |x| (part[2w], part[2w+1]).
+template <int S, int R>
+__global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double* __restrict__ beta,
+                                                  const double* __restrict__ invd,
+                                                  const double* __restrict__ bb,
+                                                  const double* __restrict__ cam,
+                                                  const double* const* q, double* const* xo,
+                                                  const double* __restrict__ bnd_in,
+                                                  double* bnd_out, int nchunk, double* part) {
+  const ScanGeo g = scan_geo(d, S);
+  const long G = gridDim.x;
+  const long lin = blockIdx.x;
+  const long w = (lin % 8) * (G / 8) + lin / 8;  // XCD-aware: neighbouring chunks share an XCD
+  const int l = threadIdx.x;
+  double dmax = 0.0, xmax = 0.0;
+  if (w < (long)nchunk * g.nb) {
+    const int ca = (int)(w % nchunk);
+    const int b = 1 + (int)(w / nchunk);
+    const int a0 = 1 + ca * SC_C;
+    const int p0 = l * R + 1;
+    double xp[3][R];
+    // line a0-1: the stage input's ghost line for the first chunk, else the preceding
+    // chunk's last line of the previous pass
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp)
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int p = p0 + r;
+        double v = 0.0;
+        if (p <= g.n) {
+          if (ca == 0)
+            v = A.f[comp][g.ns_p * p + g.ns_a * (a0 - 1) + g.ns_b * b];
+          else
+            v = bnd_in[(((long)comp * g.nb + (b - 1)) * nchunk + (ca - 1)) * g.n + (p - 1)];
+        }
+        xp[comp][r] = v;
+      }
+    for (int qq = 0; qq < SC_C; ++qq) {
+      const int a = a0 + qq;
+      if (a > g.na) break;  // uniform over the wavefront
+      const long lb = g.ls_a * a + g.ls_b * b;
+      double be[R], iv[R], bv[R], cm[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int p = p0 + r;
+        if (p <= g.n) {
+          be[r] = beta[lb + p]; iv[r] = invd[lb + p]; bv[r] = bb[lb + p]; cm[r] = cam[lb + p];
+        } else {
+          be[r] = 0.0; iv[r] = 0.0; bv[r] = 0.0; cm[r] = 0.0;
+        }
+      }
+      const long nb0 = g.ns_a * a + g.ns_b * b;  // natural base of the line
+#pragma unroll
+      for (int comp = 0; comp < 3; ++comp) {
+        const double* qc = q[comp];
+        // forward: y(p) = al(p) + be(p) y(p-1)
+        double al[R];
+        double Bt = 1.0, At = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int p = p0 + r;
+          if (p <= g.n) {
+            al[r] = (qc[lb + p] + cm[r] * xp[comp][r]) * iv[r];
+            At = al[r] + be[r] * At;
+            Bt = be[r] * Bt;
+          } else {
+            al[r] = 0.0;
+          }
+        }
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const double Ao = shfl_up_d(At, o), Bo = shfl_up_d(Bt, o);
+          if (l >= o) { At = At + Bt * Ao; Bt = Bt * Bo; }
+        }
+        double Ae = shfl_up_d(At, 1), Be = shfl_up_d(Bt, 1);
+        if (l == 0) { Ae = 0.0; Be = 1.0; }
+        const double y0 = A.f[comp][nb0];                        // row 0 ghost
+        double yin = Ae + Be * y0;
+        double y[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          y[r] = al[r] + be[r] * yin;
+          if (p0 + r <= g.n) yin = y[r];
+        }
+        // backward: x(p) = y(p) - B(p) x(p+1)
+        double Ct = 1.0, Dt = 0.0;
+#pragma unroll
+        for (int r = R - 1; r >= 0; --r) {
+          if (p0 + r <= g.n) {
+            Dt = y[r] - bv[r] * Dt;
+            Ct = -bv[r] * Ct;
+          }
+        }
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const double Do = shfl_down_d(Dt, o), Co = shfl_down_d(Ct, o);
+          if (l + o < 64) { Dt = Dt + Ct * Do; Ct = Ct * Co; }
+        }
+        double De = shfl_down_d(Dt, 1), Ce = shfl_down_d(Ct, 1);
+        if (l == 63) { De = 0.0; Ce = 1.0; }
+        const double xn = A.f[comp][nb0 + g.ns_p * (g.n + 1)];  // row n+1 ghost
+        double xin = De + Ce * xn;
+        double* xc = xo[comp];
+#pragma unroll
+        for (int r = R - 1; r >= 0; --r) {
+          const int p = p0 + r;
+          if (p <= g.n) {
+            const double x = y[r] - bv[r] * xin;
+            xin = x;
+            xc[lb + p] = x;
+            xp[comp][r] = x;
+          }
+        }
+      }
+    }
+    // chunk boundary line: change since the previous pass
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp)
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int p = p0 + r;
+        if (p <= g.n) {
+          const long o = (((long)comp * g.nb + (b - 1)) * nchunk + ca) * g.n + (p - 1);
+          const double x = xp[comp][r];
+          dmax = fmax(dmax, fabs(x - bnd_in[o]));
+          xmax = fmax(xmax, fabs(x));
+          bnd_out[o] = x;
+        }
+      }
+  }
+  // wavefront max
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    dmax = fmax(dmax, shfl_down_d(dmax, o));
+    xmax = fmax(xmax, shfl_down_d(xmax, o));
+  }
+  if (l == 0) { part[2 * lin] = dmax; part[2 * lin + 1] = xmax; }
+}
+
+__global__ void k_scan_max(const double* __restrict__ part, long nb, double* out) {
+  __shared__ double s0[256], s1[256];
+  const int t = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  for (long q = t; q < nb; q += blockDim.x) { a = fmax(a, part[2 * q]); b = fmax(b, part[2 * q + 1]); }
+  s0[t] = a; s1[t] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) { s0[t] = fmax(s0[t], s0[t + s]); s1[t] = fmax(s1[t], s1[t + s]); }
+    __syncthreads();
+  }
+  if (t == 0) { out[0] = s0[0]; out[1] = s1[0]; }
+}
+
+template <int S, int R>
+int scan_stage(FvmCtx* c, const AFArgs& A, int* iters) {
+  const Dims d = c->d;
+  hipStream_t st = c->stream;
+  const ScanGeo g = scan_geo(d, S);
+  const int nchunk = (g.na + SC_C - 1) / SC_C;
+  const long nwork = (long)nchunk * g.nb;
+  const long G = (nwork + 7) / 8 * 8;
+  if (2 * G > c->nred) return FVM_ERR_SIZE;
+  double* q[3] = {c->afb[0][1], c->afb[1][1], c->afb[2][1]};
+  double* xo[3];
+  for (int k = 0; k < 3; ++k) xo[k] = S == 0 ? A.f[k] : c->afb[k][0];
+  // device pointer table: q at afptr[0..2], iterate at [3..5], field at [6..8]
+  double* tab[9] = {q[0], q[1], q[2], xo[0], xo[1], xo[2], A.f[0], A.f[1], A.f[2]};
+  HIP_CHECK(hipMemcpyAsync(c->afptr, tab, sizeof(tab), hipMemcpyHostToDevice, st));
+  double* const* dq = c->afptr;
+  double* const* dxo = c->afptr + 3;
+  {
+    ProfScope pk(c, "af_scan_q");
+    if (S == 0)
+      hipLaunchKernelGGL(k_scan_q0, dim3((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, 3 * (d.nk - 1)),
+                         dim3(64, 4, 1), 0, st, d, A, (const double* const*)c->dfa[0], dq);
+    else
+      hipLaunchKernelGGL(k_scan_q1, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, 3 * (d.nk - 1)),
+                         dim3(256), 0, st, d, A, (const double* const*)c->dfa[1], dq);
+    hipLaunchKernelGGL(k_scan_seed<S>, grid1(3L * g.nb * nchunk * g.n, 256), dim3(256), 0, st, d,
+                       A, nchunk, c->bnd[0]);
+    HIP_CHECK(hipGetLastError());
+  }
+  const double tol = c->prm.af_tol > 0 ? c->prm.af_tol : 1e-14;
+  const int cap = nchunk + 1;
+  int slot = 0, it = 0;
+  double* const* lf = c->lf[S];
+  for (;;) {
+    {
+      ProfScope pk(c, S == 0 ? "k_scan_pass_i" : "k_scan_pass_j");
+      hipLaunchKernelGGL((k_scan_pass<S, R>), dim3((unsigned)G), dim3(64), 0, st, d, A, lf[0],
+                         lf[1], lf[2], lf[3], (const double* const*)dq, dxo, c->bnd[slot],
+                         c->bnd[slot ^ 1], nchunk, c->red);
+    }
+    hipLaunchKernelGGL(k_scan_max, dim3(1), dim3(256), 0, st, c->red, G, c->scal + 20);
+    HIP_CHECK(hipGetLastError());
+    slot ^= 1;
+    ++it;
+    if (it < 2) continue;  // the first pass never suffices on its own
+    HIP_CHECK(hipMemcpyAsync(c->hscal + 20, c->scal + 20, 2 * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    const double dm = c->hscal[20], xm = c->hscal[21];
+    if (!(dm == dm) || !(xm == xm)) {
+      fprintf(stderr, "fvm_amd: non-finite AF stage %d iterate (pass %d)\n", S, it);
+      return FVM_ERR_NAN;
+    }
+    if (dm <= tol * xm || it >= cap) break;
+  }
+  if (S == 1) {
+    ProfScope pk(c, "af_scan_unpack");
+    hipLaunchKernelGGL(k_scan_unpack1, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, 3 * (d.nk - 1)),
+                       dim3(256), 0, st, d, (const double* const*)dxo, c->afptr + 6);
+    HIP_CHECK(hipGetLastError());
+  }
+  *iters = it;
+  return FVM_OK;
+}
+
+}  // namespace
+
+// Enables the scan form of stage s (0: I-lines, 1: J-lines) when its K couplings vanish and the
+// lines fit the wavefront (n <= 64 * SC_R), and computes the line factors.
+int fvm_af_scan_setup(FvmCtx* c) {
+  const Dims d = c->d;
+  hipStream_t st = c->stream;
+  long nbnd = 0;
+  for (int s = 0; s < 2; ++s) {
+    const ScanGeo g = scan_geo(d, s);
+    c->af_scan[s] = c->af_b_ratio[s] <= 1e-15 && g.n <= 64 * SC_R;
+    if (!c->af_scan[s]) continue;
+    const int nchunk = (g.na + SC_C - 1) / SC_C;
+    nbnd = std::max(nbnd, 3L * g.nb * nchunk * g.n);
+    if (!c->lf[s][0]) {
+      for (int q = 0; q < 4; ++q) {
+        HIP_CHECK(hipMalloc((void**)&c->lf[s][q], d.n * sizeof(double)));
+        c->allocs.push_back(c->lf[s][q]);
+      }
+    }
+    const long nl = (long)g.na * g.nb;
+    if (s == 0)
+      hipLaunchKernelGGL(k_scan_factor<0>, grid1(nl, 64), dim3(64), 0, st, d,
+                         (const double* const*)c->dfa[0], c->ap0, c->lf[0][0], c->lf[0][1],
+                         c->lf[0][2], c->lf[0][3]);
+    else
+      hipLaunchKernelGGL(k_scan_factor<1>, grid1(nl, 64), dim3(64), 0, st, d,
+                         (const double* const*)c->dfa[1], c->ap0, c->lf[1][0], c->lf[1][1],
+                         c->lf[1][2], c->lf[1][3]);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (nbnd > c->nbnd) {
+    for (int q = 0; q < 2; ++q) {
+      HIP_CHECK(hipMalloc((void**)&c->bnd[q], nbnd * sizeof(double)));
+      c->allocs.push_back(c->bnd[q]);
+    }
+    c->nbnd = nbnd;
+  }
+  HIP_CHECK(hipStreamSynchronize(st));
+  return FVM_OK;
+}
+
+int fvm_af_scan_stage(FvmCtx* c, int s, const AFArgs& A, int* iters) {
+  const ScanGeo g = scan_geo(c->d, s);
+  const bool small = g.n <= 64 * 4;
+  if (s == 0) return small ? scan_stage<0, 4>(c, A, iters) : scan_stage<0, SC_R>(c, A, iters);
+  return small ? scan_stage<1, 4>(c, A, iters) : scan_stage<1, SC_R>(c, A, iters);
+}

@@ -63,8 +63,15 @@ struct FvmCtx {
   int af_iters[3] = {};
   // stage couplings that vanish identically on this grid (a: along the sweep, b: across it)
   bool af_zero_a[3] = {}, af_zero_b[3] = {};
+  double af_b_ratio[3] = {};  // max over cells of (|b couplings|) / (2 Re J^-1/dt)
   // Thomas factors A, D, B of the K-line stage when its lines are independent
   double *tk_a = nullptr, *tk_d = nullptr, *tk_b = nullptr;
+  // line-scan form of the I and J stages when the K couplings vanish: per stage the line
+  // factors (beta = -A/D, 1/D, B, coupling to the previous line) and the chunk boundary lines
+  bool af_scan[2] = {};
+  double* lf[2][4] = {};
+  double* bnd[2] = {};
+  long nbnd = 0;
   // reductions
   double* red = nullptr;     // partial sums
   double* scal = nullptr;    // device scalars (c_avg, residual sums, pref, ...)
@@ -87,6 +94,18 @@ struct FvmCtx {
 };
 
 // Kernel timing: wraps one launch in a pair of events when profiling is on.
+// arguments of the AF stage kernels
+struct AFArgs {
+  double* f[3];           // stage input / output field per component
+  const double* fn[3];    // step-start field U_N per component
+  const double* nl[3];
+  const double* nl1[3];
+  const double* dif[3];
+  const double* s[3];
+  double* su[3];
+  const double* ufar;     // far-field boundary values, (NJ+1)(NK+1) per component
+};
+
 struct ProfScope {
   FvmCtx* c;
   const char* name;
@@ -99,6 +118,8 @@ struct ProfScope {
 int fvm_geom_setup(FvmCtx* c);
 int fvm_k_set_dtime(FvmCtx* c, double dtime);
 int fvm_af_setup(FvmCtx* c);
+int fvm_af_scan_setup(FvmCtx* c);
+int fvm_af_scan_stage(FvmCtx* c, int s, const AFArgs& A, int* iters);
 int fvm_ghost_fill(FvmCtx* c, double* a);
 int fvm_ghost_fill_n(FvmCtx* c, double** a, int cnt);
 int fvm_k_contra_vel(FvmCtx* c);
