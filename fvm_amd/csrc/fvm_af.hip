@@ -23,15 +23,15 @@ namespace {
 
 const dim3 kBlk(64, 4, 1);
 
-// right-hand side of stage s (iter_u.f90:49-81, 129-158, 206-235)
+// right-hand side of stage s (iter_u.f90:49-81, 129-158, 206-235); one thread per cell and all
+// three components, so the metric products of the cross terms are loaded once
 template <int S>
 __global__ void k_af_rhs(Dims d, AFArgs A, const double* const* __restrict__ m,
                          const double* __restrict__ djac, const double* __restrict__ ap0,
                          double re) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + (int)(blockIdx.z % (d.nk - 1));
-  const int comp = (int)(blockIdx.z / (d.nk - 1));
+  const int k = 1 + blockIdx.z;
   if (i > d.ni - 1 || j > d.nj - 1) return;
   const long c = cidx(d, i, j, k);
   const long sj = d.sj, sk = d.sk;
@@ -39,33 +39,44 @@ __global__ void k_af_rhs(Dims d, AFArgs A, const double* const* __restrict__ m,
   const double* E = m[3 * S + 1];
   const double* Z = m[3 * S + 2];
   const double* J = djac;
-  const double* f = A.f[comp];
   const long e = c + 1, w = c - 1, nn = c + sj, ss = c - sj, t = c + sk, b = c - sk;
-  double r = J[e] * X[e] * E[e] * (f[e + sj] - f[e - sj]);
-  r = r - J[w] * X[w] * E[w] * (f[w + sj] - f[w - sj]);
-  r = r + J[e] * X[e] * Z[e] * (f[e + sk] - f[e - sk]);
-  r = r - J[w] * X[w] * Z[w] * (f[w + sk] - f[w - sk]);
-  r = r + J[nn] * E[nn] * X[nn] * (f[nn + 1] - f[nn - 1]);
-  r = r - J[ss] * E[ss] * X[ss] * (f[ss + 1] - f[ss - 1]);
-  r = r + J[nn] * E[nn] * Z[nn] * (f[nn + sk] - f[nn - sk]);
-  r = r - J[ss] * E[ss] * Z[ss] * (f[ss + sk] - f[ss - sk]);
-  r = r + J[t] * Z[t] * X[t] * (f[t + 1] - f[t - 1]);
-  r = r - J[b] * Z[b] * X[b] * (f[b + 1] - f[b - 1]);
-  r = r + J[t] * Z[t] * E[t] * (f[t + sj] - f[t - sj]);
-  r = r - J[b] * Z[b] * E[b] * (f[b + sj] - f[b - sj]);
-  const double dd = 0.25 * r;
-  double su;
-  if (S == 0) {
-    su = ap0[c] * A.fn[comp][c];
-    su = su - 3.0 * re * A.nl[comp][c];
-    su = su + 1.0 * re * A.nl1[comp][c];
-    su = su + A.dif[comp][c];
-    su = su + 2.0 * re * A.s[comp][c];
-    su = su + dd;
-  } else {
-    su = ap0[c] * f[c] + dd;
+  // metric products in the reference's association (J X) E etc.
+  const double pe_xe = J[e] * X[e] * E[e], pw_xe = J[w] * X[w] * E[w];
+  const double pe_xz = J[e] * X[e] * Z[e], pw_xz = J[w] * X[w] * Z[w];
+  const double pn_ex = J[nn] * E[nn] * X[nn], ps_ex = J[ss] * E[ss] * X[ss];
+  const double pn_ez = J[nn] * E[nn] * Z[nn], ps_ez = J[ss] * E[ss] * Z[ss];
+  const double pt_zx = J[t] * Z[t] * X[t], pb_zx = J[b] * Z[b] * X[b];
+  const double pt_ze = J[t] * Z[t] * E[t], pb_ze = J[b] * Z[b] * E[b];
+  const double a0 = ap0[c];
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    const double* f = A.f[comp];
+    double r = pe_xe * (f[e + sj] - f[e - sj]);
+    r = r - pw_xe * (f[w + sj] - f[w - sj]);
+    r = r + pe_xz * (f[e + sk] - f[e - sk]);
+    r = r - pw_xz * (f[w + sk] - f[w - sk]);
+    r = r + pn_ex * (f[nn + 1] - f[nn - 1]);
+    r = r - ps_ex * (f[ss + 1] - f[ss - 1]);
+    r = r + pn_ez * (f[nn + sk] - f[nn - sk]);
+    r = r - ps_ez * (f[ss + sk] - f[ss - sk]);
+    r = r + pt_zx * (f[t + 1] - f[t - 1]);
+    r = r - pb_zx * (f[b + 1] - f[b - 1]);
+    r = r + pt_ze * (f[t + sj] - f[t - sj]);
+    r = r - pb_ze * (f[b + sj] - f[b - sj]);
+    const double dd = 0.25 * r;
+    double su;
+    if (S == 0) {
+      su = a0 * A.fn[comp][c];
+      su = su - 3.0 * re * A.nl[comp][c];
+      su = su + 1.0 * re * A.nl1[comp][c];
+      su = su + A.dif[comp][c];
+      su = su + 2.0 * re * A.s[comp][c];
+      su = su + dd;
+    } else {
+      su = a0 * f[c] + dd;
+    }
+    A.su[comp][c] = su;
   }
-  A.su[comp][c] = su;
 }
 
 // Stage-s line sweep over the lines of anti-diagonal h of the line plane.
@@ -459,7 +470,7 @@ int fvm_k_iter_u(FvmCtx* c) {
   hipStream_t st = c->stream;
   if (d.ni - 1 > FVM_MAX_LINE || d.nj - 1 > FVM_MAX_LINE || d.nk - 1 > FVM_MAX_LINE)
     return FVM_ERR_SIZE;
-  const dim3 gi((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, 3 * (d.nk - 1));
+  const dim3 gi((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
   for (int s = 0; s < 3; ++s) {
     AFArgs A;
     double* fld[3] = {c->u, c->v, c->w};

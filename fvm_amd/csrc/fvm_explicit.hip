@@ -279,15 +279,16 @@ __global__ void k_cfl(Dims d, const double* const* __restrict__ m, const double*
   if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
-// base pressure and surface forces: one block of 256 lanes, fixed-order tree per quantity
-__global__ void k_forces(Dims d, const double* __restrict__ wall, const double* __restrict__ p,
-                         const double* __restrict__ u, const double* __restrict__ v,
-                         const double* __restrict__ w, double re, double ref_a,
-                         const double* __restrict__ cflpart, int ncfl, double* out) {
+// surface-force partial sums over the wall triangles (calccoef.f90:76-183): one element per
+// lane, a fixed-order tree per block; part[4 b + q] for pressure drag / lift, viscous drag / lift
+__global__ void k_forces_part(Dims d, const double* __restrict__ wall, const double* __restrict__ p,
+                              const double* __restrict__ u, const double* __restrict__ v,
+                              const double* __restrict__ w, double re, double* part) {
   __shared__ double sh[256];
   const long nw = (long)(d.nj - 1) * (d.nk - 1);
   double pd = 0.0, ply = 0.0, td = 0.0, tly = 0.0;
-  for (long t = threadIdx.x; t < 2 * nw; t += blockDim.x) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t < 2 * nw) {
     int tri = (int)(t / nw);
     long r = t - tri * nw;
     int j = 1 + (int)(r % (d.nj - 1)), k = 1 + (int)(r / (d.nj - 1));
@@ -300,15 +301,33 @@ __global__ void k_forces(Dims d, const double* __restrict__ wall, const double* 
     double dudn = sqrt(t1 * t1 + t2 * t2 + t3 * t3);
     dudn = dudn / dn;
     double um = sqrt(U * U + V * V + W * W);
-    pd = pd - 2.0 * p[c] * ar * A / dum;
-    ply = ply - 2.0 * p[c] * ar * B / dum;
-    td = td + (2.0 / re) * dudn * ar * U / um;
-    tly = tly + (2.0 / re) * dudn * ar * V / um;
+    pd = -2.0 * p[c] * ar * A / dum;
+    ply = -2.0 * p[c] * ar * B / dum;
+    td = (2.0 / re) * dudn * ar * U / um;
+    tly = (2.0 / re) * dudn * ar * V / um;
   }
   pd = block_reduce_sum(pd, sh);
   ply = block_reduce_sum(ply, sh);
   td = block_reduce_sum(td, sh);
   tly = block_reduce_sum(tly, sh);
+  if (threadIdx.x == 0) {
+    part[4 * blockIdx.x] = pd; part[4 * blockIdx.x + 1] = ply;
+    part[4 * blockIdx.x + 2] = td; part[4 * blockIdx.x + 3] = tly;
+  }
+}
+
+// force coefficients, base pressure and CFL maximum (calccoef.f90:11-183), one block
+__global__ void k_forces_final(Dims d, const double* __restrict__ p,
+                               const double* __restrict__ fpart, int nfb, double ref_a,
+                               const double* __restrict__ cflpart, int ncfl, double* out) {
+  __shared__ double sh[256];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nfb; b += blockDim.x)
+    for (int q = 0; q < 4; ++q) acc[q] = acc[q] + fpart[4 * b + q];
+  const double pd = block_reduce_sum(acc[0], sh);
+  const double ply = block_reduce_sum(acc[1], sh);
+  const double td = block_reduce_sum(acc[2], sh);
+  const double tly = block_reduce_sum(acc[3], sh);
   double cmax = 0.0;
   for (int q = threadIdx.x; q < ncfl; q += blockDim.x) cmax = fmax(cmax, cflpart[q]);
   cmax = block_reduce_max(cmax, sh);
@@ -432,8 +451,13 @@ int fvm_k_calccoef(FvmCtx* c, double* out8) {
   const int ncfl = 512;
   hipLaunchKernelGGL(k_cfl, dim3(ncfl), dim3(256), 0, c->stream, c->d, (const double* const*)m,
                      c->u, c->v, c->w, c->prm.dtime, c->red);
-  hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, c->stream, c->d, c->wall, c->p, c->u,
-                     c->v, c->w, c->prm.re, c->prm.ref_a, c->red, ncfl, c->scal + 8);
+  const long nw2 = 2L * (c->d.nj - 1) * (c->d.nk - 1);
+  const int nfb = (int)((nw2 + 255) / 256);
+  if (ncfl + 4L * nfb > c->nred) return FVM_ERR_SIZE;
+  hipLaunchKernelGGL(k_forces_part, dim3(nfb), dim3(256), 0, c->stream, c->d, c->wall, c->p,
+                     c->u, c->v, c->w, c->prm.re, c->red + ncfl);
+  hipLaunchKernelGGL(k_forces_final, dim3(1), dim3(256), 0, c->stream, c->d, c->p, c->red + ncfl,
+                     nfb, c->prm.ref_a, c->red, ncfl, c->scal + 8);
   HIP_CHECK(hipGetLastError());
   if (out8) {
     HIP_CHECK(hipMemcpyAsync(out8, c->scal + 8, 8 * sizeof(double), hipMemcpyDeviceToHost,
