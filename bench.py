@@ -35,8 +35,13 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # kernel's dataflow read once, each output written once, FP64 = 8 B (FP32 = 4 B); ghosts and
 # O(surface) work excluded.  'units' is how many cell sweeps one launch performs.
 KERNEL_BYTES = {
-    # one pass of an AF stage over the 3 components: {phi_in, SU, J, 3 stage metric
-    # products} -> phi_out = 7 doubles per cell and component
+    # one pass of the line-scan AF stage: line factors {beta, 1/D, B, c_(a-1)} shared by the
+    # three components, {Q} -> {x} per component: 32 + 3 x 16 bytes
+    "k_scan_pass_i": (80, 1), "k_scan_pass_j": (80, 1),
+    # exact K-line sweep: Thomas factors {A, D, B} shared, {SU} -> {x} per component
+    "k_af_thomas_k": (72, 1),
+    # chunked fallback pass over the 3 components: {phi_in, SU, J, 3 stage metric products}
+    # -> phi_out = 7 doubles per cell and component
     "k_af_chunk_i": (56, 3), "k_af_chunk_j": (56, 3), "k_af_chunk_k": (56, 3),
     # strain rates: {u,v,w, xi 9} -> {S 6, |S|}
     "k_strain": (8 * 19, 1),

@@ -25,10 +25,14 @@ const dim3 kBlk(64, 4, 1);
 
 // right-hand side of stage s (iter_u.f90:49-81, 129-158, 206-235); one thread per cell and all
 // three components, so the metric products of the cross terms are loaded once
+// With qform set (line-scan form of stages 0 and 1) the kernel stores instead
+// Q = SU' + c_(a+1) x_in(a+1) + AT x_in(K+1) + AB x_in(K-1): SU' carries BCU's far-field
+// addition 2 AE U_F at I = NI-1 (bcu.f90:38-40; the wall addition multiplies U_F = 0), c_(a+1)
+// is AN (stage 0) or the folded AE (stage 1).
 template <int S>
 __global__ void k_af_rhs(Dims d, AFArgs A, const double* const* __restrict__ m,
                          const double* __restrict__ djac, const double* __restrict__ ap0,
-                         double re) {
+                         double re, const double* const* __restrict__ fa, int qform) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -74,6 +78,18 @@ __global__ void k_af_rhs(Dims d, AFArgs A, const double* const* __restrict__ m,
       su = su + dd;
     } else {
       su = a0 * f[c] + dd;
+    }
+    if (S < 2 && qform) {
+      double ae = fa[0][c];
+      if (i == 1) su = su + 2.0 * fa[0][c - 1] * 0.0;
+      if (i == d.ni - 1) {
+        const long nf = (long)(d.nj + 1) * (d.nk + 1);
+        su = su + 2.0 * ae * A.ufar[comp * nf + j + (long)(d.nj + 1) * k];
+        ae = 0.0;
+      }
+      su = S == 0 ? su + fa[1][c] * f[nn] : su + ae * f[e];
+      su = su + fa[2][c] * f[t];
+      su = su + fa[2][b] * f[b];
     }
     A.su[comp][c] = su;
   }
@@ -308,6 +324,43 @@ __global__ void k_copy_ghosts(Dims d, const double* __restrict__ src, double* ds
   }
 }
 
+// The same sweep with the forward results held in registers (lines of at most NMAX rows): one
+// read of SU and the factors, one write of the result.
+template <int NMAX>
+__global__ void __launch_bounds__(256) k_af_thomas_k_reg(Dims d, AFArgs A,
+                                                         const double* __restrict__ ta,
+                                                         const double* __restrict__ td,
+                                                         const double* __restrict__ tb) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int comp = blockIdx.z;
+  if (i > d.ni - 1 || j > d.nj - 1) return;
+  double* f = A.f[comp];
+  const double* su = A.su[comp];
+  const int n = d.nk - 1;
+  const long c0 = cidx(d, i, j, 0);
+  double xs[NMAX];
+  double x = f[c0];  // K ghost of the stage input
+#pragma unroll
+  for (int k = 1; k <= NMAX; ++k) {
+    if (k <= n) {
+      const long c = c0 + (long)k * d.sk;
+      x = (su[c] - ta[c] * x) / td[c];
+      xs[k - 1] = x;
+    }
+  }
+  double xn = f[c0 + (long)(n + 1) * d.sk];
+#pragma unroll
+  for (int k = NMAX; k >= 1; --k) {
+    if (k <= n) {
+      const long c = c0 + (long)k * d.sk;
+      x = xs[k - 1] - tb[c] * xn;
+      f[c] = x;
+      xn = x;
+    }
+  }
+}
+
 // Largest |coupling| of stage S over the interior: part[2*block] along the sweep direction a,
 // part[2*block+1] across it (b).  Exact zeros (extruded grid: xi_z = eta_z = zeta_x = zeta_y = 0)
 // make the lines of a stage independent, and the sweep is then exact in one pass.
@@ -482,9 +535,11 @@ int fvm_k_iter_u(FvmCtx* c) {
     A.ufar = c->ufar;
     {
       ProfScope ps(c, "af_rhs");
-      if (s == 0) hipLaunchKernelGGL(k_af_rhs<0>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re);
-      if (s == 1) hipLaunchKernelGGL(k_af_rhs<1>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re);
-      if (s == 2) hipLaunchKernelGGL(k_af_rhs<2>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re);
+      const int qf = (s < 2 && c->prm.af_mode == 0 && c->af_scan[s]) ? 1 : 0;
+      const double* const* fas = (const double* const*)c->dfa[s];
+      if (s == 0) hipLaunchKernelGGL(k_af_rhs<0>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf);
+      if (s == 1) hipLaunchKernelGGL(k_af_rhs<1>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf);
+      if (s == 2) hipLaunchKernelGGL(k_af_rhs<2>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf);
       HIP_CHECK(hipGetLastError());
     }
     // boundary ghosts of the stage input (BCU, iter_u.f90:83,160,237)
@@ -495,8 +550,12 @@ int fvm_k_iter_u(FvmCtx* c) {
       // independent K-lines: the exact sweep in one pass
       ProfScope pk(c, "k_af_thomas_k");
       const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, 3);
-      hipLaunchKernelGGL(k_af_thomas_k, g, dim3(64, 4, 1), 0, st, d, A, c->tk_a, c->tk_d,
-                         c->tk_b);
+      if (d.nk - 1 <= 64)
+        hipLaunchKernelGGL(k_af_thomas_k_reg<64>, g, dim3(64, 4, 1), 0, st, d, A, c->tk_a,
+                           c->tk_d, c->tk_b);
+      else
+        hipLaunchKernelGGL(k_af_thomas_k, g, dim3(64, 4, 1), 0, st, d, A, c->tk_a, c->tk_d,
+                           c->tk_b);
       HIP_CHECK(hipGetLastError());
       c->af_iters[s] = 1;
     } else if (s < 2 && c->prm.af_mode == 0 && c->af_scan[s]) {

@@ -18,7 +18,7 @@
 //   first-order affine recurrences, evaluated by one 64-lane wavefront as an affine-map scan
 //   (each lane composes its block of rows, then a log2(64)-step shuffle scan, no LDS barriers).
 //   Q = SU' + (coupling to line a+1) x_in(a+1) is formed once per stage.
-// * A wavefront owns C consecutive lines of one plane and sweeps them in order; chunks are
+// * A wavefront owns SC_C consecutive lines of one plane and sweeps them in order; chunks are
 //   coupled through the last line of the preceding chunk from the previous pass (block Jacobi
 //   across chunks).  The passes are nilpotent (exact after nchunk passes) and contractive by the
 //   diagonal dominance of AP; the host stops once the chunk boundary lines no longer change
@@ -33,7 +33,7 @@
 
 namespace {
 
-constexpr int SC_C = 8;   // lines per chunk
+constexpr int SC_C = 16;  // lines per chunk
 constexpr int SC_R = 8;   // max rows per lane (n <= 512)
 
 struct ScanGeo {
@@ -88,41 +88,9 @@ __global__ void k_scan_factor(Dims d, const double* const* __restrict__ fa,
   }
 }
 
-// Q = SU' + cap x_in(a+1) in line-major order; SU' carries BCU's far-field addition
-// 2 AE U_F at I = NI-1 (bcu.f90:38-40; the wall addition multiplies U_F = 0).
-// Stage 0: one thread per cell.  Stage 1: 32 x 32 (I x J) tiles transposed through LDS.
-__device__ inline double scan_q_value(const Dims& d, int S, const AFArgs& A,
-                                      const double* const* fa, int comp, int i, int j, int k) {
-  const long c = cidx(d, i, j, k);
-  const long nf = (long)(d.nj + 1) * (d.nk + 1);
-  double s = A.su[comp][c];
-  double ae = fa[0][c];
-  if (i == 1) s = s + 2.0 * fa[0][c - 1] * 0.0;
-  if (i == d.ni - 1) {
-    s = s + 2.0 * ae * A.ufar[comp * nf + j + (long)(d.nj + 1) * k];
-    ae = 0.0;
-  }
-  const double* f = A.f[comp];
-  double qv = S == 0 ? s + fa[1][c] * f[c + d.sj] : s + ae * f[c + 1];
-  // K couplings (below 1e-15 of the diagonal when this form is enabled) from the stage input
-  qv = qv + fa[2][c] * f[c + d.sk];
-  qv = qv + fa[2][c - d.sk] * f[c - d.sk];
-  return qv;
-}
-
-__global__ void k_scan_q0(Dims d, AFArgs A, const double* const* __restrict__ fa,
-                          double* const* q) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + (int)(blockIdx.z % (d.nk - 1));
-  const int comp = (int)(blockIdx.z / (d.nk - 1));
-  if (i > d.ni - 1 || j > d.nj - 1) return;
-  q[comp][cidx(d, i, j, k)] = scan_q_value(d, 0, A, fa, comp, i, j, k);
-}
-
-__global__ void __launch_bounds__(256) k_scan_q1(Dims d, AFArgs A,
-                                                 const double* const* __restrict__ fa,
-                                                 double* const* q) {
+// Q of the J stage (formed by the right-hand-side kernel in the natural layout) into the
+// line-major layout, 32 x 32 (I x J) tiles transposed through LDS.
+__global__ void __launch_bounds__(256) k_scan_q1v(Dims d, PtrSet8 qn, double* const* q) {
   __shared__ double tile[32][33];
   const int i0 = 1 + blockIdx.x * 32, j0 = 1 + blockIdx.y * 32;
   const int k = 1 + (int)(blockIdx.z % (d.nk - 1));
@@ -131,7 +99,7 @@ __global__ void __launch_bounds__(256) k_scan_q1(Dims d, AFArgs A,
   for (int r = 0; r < 4; ++r) {
     const int li = t % 32, lj = t / 32 + 8 * r;
     const int i = i0 + li, j = j0 + lj;
-    if (i <= d.ni - 1 && j <= d.nj - 1) tile[lj][li] = scan_q_value(d, 1, A, fa, comp, i, j, k);
+    if (i <= d.ni - 1 && j <= d.nj - 1) tile[lj][li] = qn.p[comp][cidx(d, i, j, k)];
   }
   __syncthreads();
   const long lsa = d.nj + 1, lsb = (long)(d.nj + 1) * (d.ni + 1);
@@ -186,10 +154,12 @@ __global__ void k_scan_seed(Dims d, AFArgs A, int nchunk, double* bnd) {
 __device__ inline double shfl_up_d(double v, int o) { return __shfl_up(v, o, 64); }
 __device__ inline double shfl_down_d(double v, int o) { return __shfl_down(v, o, 64); }
 
-// One pass: wavefront w owns chunk ca of plane b and sweeps its lines in order, all three
-// components.  Lane l owns rows l*R+1 .. l*R+R.  Writes the iterate (line-major), the chunk's
-// last line to bnd_out, and per wavefront the largest change of that line between passes and
-// the largest |x| (part[2w], part[2w+1]).
+// One pass: wavefront w owns chunk ca of plane b for one component and sweeps its lines in
+// order (the component is the fastest index of w, so the three wavefronts sharing the line
+// factors run side by side on one XCD).  Lane l owns rows l*R+1 .. l*R+R.  The loads of line
+// a+1 are issued before line a is solved.  Writes the iterate (line-major), the chunk's last line
+// to bnd_out, and per wavefront the largest change of that line between passes and the largest
+// |x| (part[2w], part[2w+1]).
 template <int S, int R>
 __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double* __restrict__ beta,
                                                   const double* __restrict__ invd,
@@ -201,123 +171,123 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
   const ScanGeo g = scan_geo(d, S);
   const long G = gridDim.x;
   const long lin = blockIdx.x;
-  const long w = (lin % 8) * (G / 8) + lin / 8;  // XCD-aware: neighbouring chunks share an XCD
+  const long w = (lin % 8) * (G / 8) + lin / 8;  // XCD-aware: neighbouring items share an XCD
   const int l = threadIdx.x;
   double dmax = 0.0, xmax = 0.0;
-  if (w < (long)nchunk * g.nb) {
-    const int ca = (int)(w % nchunk);
-    const int b = 1 + (int)(w / nchunk);
+  if (w < 3L * nchunk * g.nb) {
+    const int comp = (int)(w % 3);
+    const long wr = w / 3;
+    const int ca = (int)(wr % nchunk);
+    const int b = 1 + (int)(wr / nchunk);
     const int a0 = 1 + ca * SC_C;
+    const int a1 = min(a0 + SC_C - 1, g.na);
     const int p0 = l * R + 1;
-    double xp[3][R];
+    const double* qc = q[comp];
+    const double* fc = A.f[comp];
+    double* xc = xo[comp];
+    double xp[R];
     // line a0-1: the stage input's ghost line for the first chunk, else the preceding
     // chunk's last line of the previous pass
 #pragma unroll
-    for (int comp = 0; comp < 3; ++comp)
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int p = p0 + r;
-        double v = 0.0;
-        if (p <= g.n) {
-          if (ca == 0)
-            v = A.f[comp][g.ns_p * p + g.ns_a * (a0 - 1) + g.ns_b * b];
-          else
-            v = bnd_in[(((long)comp * g.nb + (b - 1)) * nchunk + (ca - 1)) * g.n + (p - 1)];
-        }
-        xp[comp][r] = v;
+    for (int r = 0; r < R; ++r) {
+      const int p = p0 + r;
+      double v = 0.0;
+      if (p <= g.n) {
+        if (ca == 0)
+          v = fc[g.ns_p * p + g.ns_a * (a0 - 1) + g.ns_b * b];
+        else
+          v = bnd_in[(((long)comp * g.nb + (b - 1)) * nchunk + (ca - 1)) * g.n + (p - 1)];
       }
-    for (int qq = 0; qq < SC_C; ++qq) {
-      const int a = a0 + qq;
-      if (a > g.na) break;  // uniform over the wavefront
+      xp[r] = v;
+    }
+    double be[R], iv[R], bv[R], cm[R], qv[R];
+    auto load = [&](int a) {
       const long lb = g.ls_a * a + g.ls_b * b;
-      double be[R], iv[R], bv[R], cm[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int p = p0 + r;
         if (p <= g.n) {
-          be[r] = beta[lb + p]; iv[r] = invd[lb + p]; bv[r] = bb[lb + p]; cm[r] = cam[lb + p];
+          be[r] = beta[lb + p]; iv[r] = invd[lb + p]; bv[r] = bb[lb + p];
+          cm[r] = cam[lb + p]; qv[r] = qc[lb + p];
         } else {
-          be[r] = 0.0; iv[r] = 0.0; bv[r] = 0.0; cm[r] = 0.0;
+          be[r] = 0.0; iv[r] = 0.0; bv[r] = 0.0; cm[r] = 0.0; qv[r] = 0.0;
         }
       }
+    };
+    load(a0);
+    for (int a = a0; a <= a1; ++a) {
+      const long lb = g.ls_a * a + g.ls_b * b;
       const long nb0 = g.ns_a * a + g.ns_b * b;  // natural base of the line
+      const double y0 = fc[nb0];                 // row 0 ghost
+      const double xn = fc[nb0 + g.ns_p * (g.n + 1)];  // row n+1 ghost
+      // forward: y(p) = al(p) + be(p) y(p-1)
+      double al[R], bt[R], bw[R];
+      double Bt = 1.0, At = 0.0;
 #pragma unroll
-      for (int comp = 0; comp < 3; ++comp) {
-        const double* qc = q[comp];
-        // forward: y(p) = al(p) + be(p) y(p-1)
-        double al[R];
-        double Bt = 1.0, At = 0.0;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const int p = p0 + r;
-          if (p <= g.n) {
-            al[r] = (qc[lb + p] + cm[r] * xp[comp][r]) * iv[r];
-            At = al[r] + be[r] * At;
-            Bt = be[r] * Bt;
-          } else {
-            al[r] = 0.0;
-          }
+      for (int r = 0; r < R; ++r) {
+        al[r] = (qv[r] + cm[r] * xp[r]) * iv[r];
+        bt[r] = be[r];
+        bw[r] = bv[r];
+        if (p0 + r <= g.n) {
+          At = al[r] + bt[r] * At;
+          Bt = bt[r] * Bt;
         }
+      }
+      if (a < a1) load(a + 1);  // next line's operands in flight during the scans
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-          const double Ao = shfl_up_d(At, o), Bo = shfl_up_d(Bt, o);
-          if (l >= o) { At = At + Bt * Ao; Bt = Bt * Bo; }
+      for (int o = 1; o < 64; o <<= 1) {
+        const double Ao = shfl_up_d(At, o), Bo = shfl_up_d(Bt, o);
+        if (l >= o) { At = At + Bt * Ao; Bt = Bt * Bo; }
+      }
+      double Ae = shfl_up_d(At, 1), Be = shfl_up_d(Bt, 1);
+      if (l == 0) { Ae = 0.0; Be = 1.0; }
+      double yin = Ae + Be * y0;
+      double y[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        y[r] = al[r] + bt[r] * yin;
+        if (p0 + r <= g.n) yin = y[r];
+      }
+      // backward: x(p) = y(p) - B(p) x(p+1)
+      double Ct = 1.0, Dt = 0.0;
+#pragma unroll
+      for (int r = R - 1; r >= 0; --r) {
+        if (p0 + r <= g.n) {
+          Dt = y[r] - bw[r] * Dt;
+          Ct = -bw[r] * Ct;
         }
-        double Ae = shfl_up_d(At, 1), Be = shfl_up_d(Bt, 1);
-        if (l == 0) { Ae = 0.0; Be = 1.0; }
-        const double y0 = A.f[comp][nb0];                        // row 0 ghost
-        double yin = Ae + Be * y0;
-        double y[R];
+      }
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          y[r] = al[r] + be[r] * yin;
-          if (p0 + r <= g.n) yin = y[r];
-        }
-        // backward: x(p) = y(p) - B(p) x(p+1)
-        double Ct = 1.0, Dt = 0.0;
+      for (int o = 1; o < 64; o <<= 1) {
+        const double Do = shfl_down_d(Dt, o), Co = shfl_down_d(Ct, o);
+        if (l + o < 64) { Dt = Dt + Ct * Do; Ct = Ct * Co; }
+      }
+      double De = shfl_down_d(Dt, 1), Ce = shfl_down_d(Ct, 1);
+      if (l == 63) { De = 0.0; Ce = 1.0; }
+      double xin = De + Ce * xn;
 #pragma unroll
-        for (int r = R - 1; r >= 0; --r) {
-          if (p0 + r <= g.n) {
-            Dt = y[r] - bv[r] * Dt;
-            Ct = -bv[r] * Ct;
-          }
-        }
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-          const double Do = shfl_down_d(Dt, o), Co = shfl_down_d(Ct, o);
-          if (l + o < 64) { Dt = Dt + Ct * Do; Ct = Ct * Co; }
-        }
-        double De = shfl_down_d(Dt, 1), Ce = shfl_down_d(Ct, 1);
-        if (l == 63) { De = 0.0; Ce = 1.0; }
-        const double xn = A.f[comp][nb0 + g.ns_p * (g.n + 1)];  // row n+1 ghost
-        double xin = De + Ce * xn;
-        double* xc = xo[comp];
-#pragma unroll
-        for (int r = R - 1; r >= 0; --r) {
-          const int p = p0 + r;
-          if (p <= g.n) {
-            const double x = y[r] - bv[r] * xin;
-            xin = x;
-            xc[lb + p] = x;
-            xp[comp][r] = x;
-          }
+      for (int r = R - 1; r >= 0; --r) {
+        const int p = p0 + r;
+        if (p <= g.n) {
+          const double x = y[r] - bw[r] * xin;
+          xin = x;
+          xc[lb + p] = x;
+          xp[r] = x;
         }
       }
     }
     // chunk boundary line: change since the previous pass
 #pragma unroll
-    for (int comp = 0; comp < 3; ++comp)
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int p = p0 + r;
-        if (p <= g.n) {
-          const long o = (((long)comp * g.nb + (b - 1)) * nchunk + ca) * g.n + (p - 1);
-          const double x = xp[comp][r];
-          dmax = fmax(dmax, fabs(x - bnd_in[o]));
-          xmax = fmax(xmax, fabs(x));
-          bnd_out[o] = x;
-        }
+    for (int r = 0; r < R; ++r) {
+      const int p = p0 + r;
+      if (p <= g.n) {
+        const long o = (((long)comp * g.nb + (b - 1)) * nchunk + ca) * g.n + (p - 1);
+        const double x = xp[r];
+        dmax = fmax(dmax, fabs(x - bnd_in[o]));
+        xmax = fmax(xmax, fabs(x));
+        bnd_out[o] = x;
       }
+    }
   }
   // wavefront max
 #pragma unroll
@@ -348,10 +318,13 @@ int scan_stage(FvmCtx* c, const AFArgs& A, int* iters) {
   hipStream_t st = c->stream;
   const ScanGeo g = scan_geo(d, S);
   const int nchunk = (g.na + SC_C - 1) / SC_C;
-  const long nwork = (long)nchunk * g.nb;
+  const long nwork = 3L * nchunk * g.nb;
   const long G = (nwork + 7) / 8 * 8;
   if (2 * G > c->nred) return FVM_ERR_SIZE;
-  double* q[3] = {c->afb[0][1], c->afb[1][1], c->afb[2][1]};
+  // Q: the right-hand-side kernel left it in A.su (natural layout); the J stage uses a
+  // line-major copy
+  double* q[3];
+  for (int k = 0; k < 3; ++k) q[k] = S == 0 ? A.su[k] : c->afb[k][1];
   double* xo[3];
   for (int k = 0; k < 3; ++k) xo[k] = S == 0 ? A.f[k] : c->afb[k][0];
   // device pointer table: q at afptr[0..2], iterate at [3..5], field at [6..8]
@@ -361,12 +334,12 @@ int scan_stage(FvmCtx* c, const AFArgs& A, int* iters) {
   double* const* dxo = c->afptr + 3;
   {
     ProfScope pk(c, "af_scan_q");
-    if (S == 0)
-      hipLaunchKernelGGL(k_scan_q0, dim3((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, 3 * (d.nk - 1)),
-                         dim3(64, 4, 1), 0, st, d, A, (const double* const*)c->dfa[0], dq);
-    else
-      hipLaunchKernelGGL(k_scan_q1, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, 3 * (d.nk - 1)),
-                         dim3(256), 0, st, d, A, (const double* const*)c->dfa[1], dq);
+    if (S == 1) {
+      PtrSet8 qn{};
+      for (int k = 0; k < 3; ++k) qn.p[k] = A.su[k];
+      hipLaunchKernelGGL(k_scan_q1v, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, 3 * (d.nk - 1)),
+                         dim3(256), 0, st, d, qn, dq);
+    }
     hipLaunchKernelGGL(k_scan_seed<S>, grid1(3L * g.nb * nchunk * g.n, 256), dim3(256), 0, st, d,
                        A, nchunk, c->bnd[0]);
     HIP_CHECK(hipGetLastError());
@@ -451,7 +424,15 @@ int fvm_af_scan_setup(FvmCtx* c) {
 
 int fvm_af_scan_stage(FvmCtx* c, int s, const AFArgs& A, int* iters) {
   const ScanGeo g = scan_geo(c->d, s);
-  const bool small = g.n <= 64 * 4;
-  if (s == 0) return small ? scan_stage<0, 4>(c, A, iters) : scan_stage<0, SC_R>(c, A, iters);
-  return small ? scan_stage<1, 4>(c, A, iters) : scan_stage<1, SC_R>(c, A, iters);
+  const int R = (g.n + 63) / 64;
+  if (s == 0) {
+    if (R <= 3) return scan_stage<0, 3>(c, A, iters);
+    if (R <= 4) return scan_stage<0, 4>(c, A, iters);
+    if (R <= 6) return scan_stage<0, 6>(c, A, iters);
+    return scan_stage<0, SC_R>(c, A, iters);
+  }
+  if (R <= 3) return scan_stage<1, 3>(c, A, iters);
+  if (R <= 4) return scan_stage<1, 4>(c, A, iters);
+  if (R <= 6) return scan_stage<1, 6>(c, A, iters);
+  return scan_stage<1, SC_R>(c, A, iters);
 }

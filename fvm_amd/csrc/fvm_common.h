@@ -134,6 +134,39 @@ __global__ void k_ghost_fill_t(Dims d, T* __restrict__ a) {
   }
 }
 
+// up to eight arrays passed by value (no device pointer table needed)
+struct PtrSet8 {
+  double* p[8];
+};
+
+// the periodic ghost copy of several arrays in one launch (blockIdx.y selects the array); every
+// ghost reads an interior (J, K) cell only, so the arrays and the cells are independent
+__global__ inline void k_ghost_fill_multi(Dims d, PtrSet8 set) {
+  double* __restrict__ a = set.p[blockIdx.y];
+  const long nJ = (long)(d.ni + 1) * 2 * (d.nk + 1);
+  const long nK = (long)(d.ni + 1) * (d.nj + 1) * 2;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nJ + nK;
+       t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    if (t < nJ) {
+      i = (int)(t % (d.ni + 1));
+      long r = t / (d.ni + 1);
+      j = (r % 2) == 0 ? 0 : d.nj;
+      k = (int)(r / 2);
+      if (k == 0 || k == d.nk) continue;
+    } else {
+      long q = t - nJ;
+      i = (int)(q % (d.ni + 1));
+      long r = q / (d.ni + 1);
+      j = (int)(r % (d.nj + 1));
+      k = (r / (d.nj + 1)) == 0 ? 0 : d.nk;
+    }
+    int js = j == 0 ? d.nj - 1 : (j == d.nj ? 1 : j);
+    int ks = k == 0 ? d.nk - 1 : (k == d.nk ? 1 : k);
+    a[cidx(d, i, j, k)] = a[cidx(d, i, js, ks)];
+  }
+}
+
 // 1D launch over n items.
 inline dim3 grid1(long n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
 

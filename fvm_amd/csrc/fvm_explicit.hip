@@ -181,8 +181,10 @@ __global__ void k_ufar(Dims d, const double* const* __restrict__ m, const double
 
 // BCU ghost values (RLES/bcu.f90:21-22, 41-42): wall no-slip mirror, far-field mirror about
 // U_F; the periodic J/K ghosts follow (fvm_ghost_fill).
-__global__ void k_bcu_ghosts(Dims d, double* f, const double* __restrict__ uf) {
+__global__ void k_bcu_ghosts(Dims d, PtrSet8 set, const double* __restrict__ ufar) {
   const long nf = (long)(d.nj - 1) * (d.nk - 1);
+  double* __restrict__ f = set.p[blockIdx.y];
+  const double* __restrict__ uf = ufar + blockIdx.y * (long)(d.nj + 1) * (d.nk + 1);
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nf;
        t += (long)gridDim.x * blockDim.x) {
     int j = 1 + (int)(t % (d.nj - 1)), k = 1 + (int)(t / (d.nj - 1));
@@ -409,18 +411,17 @@ int fvm_k_ufar(FvmCtx* c) {
   return FVM_OK;
 }
 
+// f[q] takes the far-field values of component q (cnt <= 3)
 int fvm_k_bcu_ghosts(FvmCtx* c, double** f, int cnt) {
+  if (cnt < 1 || cnt > 3) return FVM_ERR_ARG;
   long nf = (long)(c->d.nj - 1) * (c->d.nk - 1);
-  long stride = (long)(c->d.nj + 1) * (c->d.nk + 1);
   ProfScope ps(c, "bcu");
-  for (int q = 0; q < cnt; ++q) {
-    hipLaunchKernelGGL(k_bcu_ghosts, grid1(nf, 256), dim3(256), 0, c->stream, c->d, f[q],
-                       c->ufar + q * stride);
-    HIP_CHECK(hipGetLastError());
-    int r = fvm_ghost_fill(c, f[q]);
-    if (r) return r;
-  }
-  return FVM_OK;
+  PtrSet8 set{};
+  for (int q = 0; q < cnt; ++q) set.p[q] = f[q];
+  hipLaunchKernelGGL(k_bcu_ghosts, dim3(grid1(nf, 256).x, cnt), dim3(256), 0, c->stream, c->d,
+                     set, c->ufar);
+  HIP_CHECK(hipGetLastError());
+  return fvm_ghost_fill_n(c, f, cnt);
 }
 
 int fvm_k_correct(FvmCtx* c, double sign) {

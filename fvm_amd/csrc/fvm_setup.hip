@@ -9,6 +9,8 @@
 // copies (metric.f90:400-410), so only the I-face one-sided formulas are evaluated here.
 #include "fvm_ctx.h"
 
+#include <algorithm>
+
 namespace {
 
 struct Box {
@@ -357,9 +359,13 @@ int fvm_ghost_fill(FvmCtx* c, double* a) {
 }
 
 int fvm_ghost_fill_n(FvmCtx* c, double** a, int cnt) {
-  for (int q = 0; q < cnt; ++q) {
-    int r = fvm_ghost_fill(c, a[q]);
-    if (r) return r;
+  for (int q0 = 0; q0 < cnt; q0 += 8) {
+    PtrSet8 set{};
+    const int m = std::min(8, cnt - q0);
+    for (int q = 0; q < m; ++q) set.p[q] = a[q0 + q];
+    hipLaunchKernelGGL(k_ghost_fill_multi, dim3(grid1(ghost_count(c->d), 256).x, m), dim3(256),
+                       0, c->stream, c->d, set);
+    HIP_CHECK(hipGetLastError());
   }
   return FVM_OK;
 }
