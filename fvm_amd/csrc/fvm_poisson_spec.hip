@@ -106,30 +106,109 @@ __global__ void k_res_final(const double* __restrict__ part, long nb, double* ou
   if (t == 0) { out[0] = s0[0]; out[1] = s1[0]; out[2] = s0[0] / s1[0]; }
 }
 
-// radial line of mode (mJ, mK): one 64-lane wavefront, PCR in LDS, solved in place.  The
-// line is solved in FP64 whatever the storage type T: the Neumann-Dirichlet radial operator of
-// the lowest modes is ill-conditioned (condition ~1e6 on the stretched grid), beyond FP32.
-template <typename T>
-__global__ void __launch_bounds__(64) k_mode_lines(int n, int nj1, const double* __restrict__ dE,
-                                                   const double* __restrict__ dW,
-                                                   const double* __restrict__ dN,
-                                                   const double* __restrict__ dT,
-                                                   const double* __restrict__ lamJ,
-                                                   const double* __restrict__ lamK, T* Z) {
-  __shared__ double la[FVM_MAX_LINE], lb[FVM_MAX_LINE], lc[FVM_MAX_LINE], ld[FVM_MAX_LINE];
-  const int mj = blockIdx.x, mk = blockIdx.y;
+// Radial line operator of Fourier mode (mJ, mK): lo x(r-1) + dg x(r) + up x(r+1) with
+// lo = <D_W>, up = <D_E>, dg = -(<D_E> + <D_W>) - lambda_J <D_N> - lambda_K <D_T> (the J, K
+// second differences diagonalised, lambda = 4 sin^2(pi m / N)), the wall Neumann fold at r = 0
+// and a zero ghost beyond the last row.  Its Thomas factors 1/D(r) and B(r) = up(r)/D(r),
+// D(r) = dg(r) - lo(r) B(r-1), are stored once in the layout of the transformed array
+// [r + N1 (mJ + NJ1 mK)]; one thread per line (set-up only).
+__global__ void k_mode_factor(int n, int nj1, int nk1, const double* __restrict__ dE,
+                              const double* __restrict__ dW, const double* __restrict__ dN,
+                              const double* __restrict__ dT, const double* __restrict__ lamJ,
+                              const double* __restrict__ lamK, double* invd, double* bf) {
+  const int mj = blockIdx.x * 64 + threadIdx.x, mk = blockIdx.y;
+  if (mj >= nj1) return;
   const double lj = lamJ[mj], lk = lamK[mk];
-  T* z = Z + (long)n * (mj + (long)nj1 * mk);
-  for (int r = threadIdx.x; r < n; r += 64) {
-    double lo = r == 0 ? 0.0 : dW[r];
-    double up = r == n - 1 ? 0.0 : dE[r];
+  const long base = (long)n * (mj + (long)nj1 * mk);
+  double bprev = 0.0;
+  for (int r = 0; r < n; ++r) {
+    const double lo = r == 0 ? 0.0 : dW[r];
+    const double up = r == n - 1 ? 0.0 : dE[r];
     double dg = -(dE[r] + dW[r]) - lj * dN[r] - lk * dT[r];
     if (r == 0) dg = dg + dW[r];  // wall Neumann fold
-    la[r] = lo; lb[r] = dg; lc[r] = up; ld[r] = (double)z[r];
+    const double D = dg - lo * bprev;
+    const double iv = 1.0 / D;
+    const double B = up * iv;
+    invd[base + r] = iv;
+    bf[base + r] = B;
+    bprev = B;
   }
-  __syncthreads();
-  pcr_solve_any(la, lb, lc, ld, n);
-  for (int r = threadIdx.x; r < n; r += 64) z[r] = (T)ld[r];
+}
+
+// Radial mode-line solves with the stored factors, in FP64 whatever the storage type T: one
+// 64-lane wavefront per line, lane l owning rows l*R .. l*R+R-1; forward elimination
+// y(r) = z(r)/D(r) - lo(r)/D(r) y(r-1) and back substitution x(r) = y(r) - B(r) x(r+1) as
+// affine-map scans across the lanes (shuffles, no LDS).
+template <typename T, int R>
+__global__ void __launch_bounds__(64) k_mode_scan(int n, const double* __restrict__ dW,
+                                                  const double* __restrict__ invd,
+                                                  const double* __restrict__ bf, T* Z) {
+  const long base = (long)n * blockIdx.x;
+  const int l = threadIdx.x;
+  const int r0 = l * R;
+  double al[R], be[R], bb[R];
+  double At = 0.0, Bt = 1.0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int r = r0 + q;
+    if (r < n) {
+      const double iv = invd[base + r];
+      al[q] = (double)Z[base + r] * iv;
+      be[q] = r == 0 ? 0.0 : -dW[r] * iv;
+      bb[q] = bf[base + r];
+      At = al[q] + be[q] * At;
+      Bt = be[q] * Bt;
+    } else {
+      al[q] = 0.0; be[q] = 0.0; bb[q] = 0.0;
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double Ao = __shfl_up(At, o, 64), Bo = __shfl_up(Bt, o, 64);
+    if (l >= o) { At = At + Bt * Ao; Bt = Bt * Bo; }
+  }
+  double yin = __shfl_up(At, 1, 64);  // y(-1) = 0: only the offset part survives
+  if (l == 0) yin = 0.0;
+  double y[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    y[q] = al[q] + be[q] * yin;
+    if (r0 + q < n) yin = y[q];
+  }
+  double Ct = 1.0, Dt = 0.0;
+#pragma unroll
+  for (int q = R - 1; q >= 0; --q) {
+    if (r0 + q < n) {
+      Dt = y[q] - bb[q] * Dt;
+      Ct = -bb[q] * Ct;
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double Do = __shfl_down(Dt, o, 64), Co = __shfl_down(Ct, o, 64);
+    if (l + o < 64) { Dt = Dt + Ct * Do; Ct = Ct * Co; }
+  }
+  double xin = __shfl_down(Dt, 1, 64);  // x(n) = 0
+  if (l == 63) xin = 0.0;
+#pragma unroll
+  for (int q = R - 1; q >= 0; --q) {
+    const int r = r0 + q;
+    if (r < n) {
+      const double x = y[q] - bb[q] * xin;
+      xin = x;
+      Z[base + r] = (T)x;
+    }
+  }
+}
+
+template <typename T>
+void mode_lines(int n1, int nlines, const double* dW, const double* invd, const double* bf, T* Z,
+                hipStream_t st) {
+  const int R = (n1 + 63) / 64;
+  if (R <= 3) hipLaunchKernelGGL((k_mode_scan<T, 3>), dim3(nlines), dim3(64), 0, st, n1, dW, invd, bf, Z);
+  else if (R <= 4) hipLaunchKernelGGL((k_mode_scan<T, 4>), dim3(nlines), dim3(64), 0, st, n1, dW, invd, bf, Z);
+  else if (R <= 6) hipLaunchKernelGGL((k_mode_scan<T, 6>), dim3(nlines), dim3(64), 0, st, n1, dW, invd, bf, Z);
+  else hipLaunchKernelGGL((k_mode_scan<T, 8>), dim3(nlines), dim3(64), 0, st, n1, dW, invd, bf, Z);
 }
 
 template <typename T>
@@ -269,6 +348,7 @@ struct SepData {
   double *lamJ = nullptr, *lamK = nullptr;              // line operators in FP64
   double *dE = nullptr, *dW = nullptr, *dN = nullptr, *dT = nullptr;
   T *X = nullptr, *Y = nullptr;
+  double *invd = nullptr, *bf = nullptr;  // mode-line Thomas factors (FP64)
 };
 
 template <typename T>
@@ -301,7 +381,17 @@ int sep_init(FvmCtx* c, SepData<T>& S, const std::vector<double>& FJ,
   c->allocs.push_back(S.X);
   if (hipMalloc((void**)&S.Y, ncomp * sizeof(T)) != hipSuccess) return FVM_ERR_HIP;
   c->allocs.push_back(S.Y);
-  return r;
+  if (r) return r;
+  double** fac[2] = {&S.invd, &S.bf};
+  for (double** q : fac) {
+    if (hipMalloc((void**)q, ncomp * sizeof(double)) != hipSuccess) return FVM_ERR_HIP;
+    c->allocs.push_back(*q);
+  }
+  hipLaunchKernelGGL(k_mode_factor, dim3((nj1 + 63) / 64, nk1), dim3(64), 0, c->stream, n1, nj1,
+                     nk1, S.dE, S.dW, S.dN, S.dT, S.lamJ, S.lamK, S.invd, S.bf);
+  if (hipGetLastError() != hipSuccess) return FVM_ERR_HIP;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return FVM_ERR_HIP;
+  return FVM_OK;
 }
 
 // e = M^-1 r on the compact array X (in place through Y)
@@ -320,8 +410,7 @@ int sep_apply(FvmCtx* c, SepData<T>& S, rocblas_handle h) {
   }
   {
     ProfScope ps(c, "p_mode_lines");
-    hipLaunchKernelGGL(k_mode_lines<T>, dim3(nj1, nk1), dim3(64), 0, c->stream, n1, nj1, S.dE,
-                       S.dW, S.dN, S.dT, S.lamJ, S.lamK, S.X);
+    mode_lines<T>(n1, nj1 * nk1, S.dW, S.invd, S.bf, S.X, c->stream);
     HIP_CHECK(hipGetLastError());
   }
   {
@@ -346,6 +435,7 @@ struct SepState {
 int fvm_sep_setup(FvmCtx* c) {
   const Dims d = c->d;
   const int n1 = d.ni - 1, nj1 = d.nj - 1, nk1 = d.nk - 1;
+  if (n1 > 512) return FVM_ERR_SIZE;  // mode lines: at most 8 rows per lane of a wavefront
   SepState* S = new SepState();
   c->sep = S;
   if (rocblas_create_handle(&S->h) != rocblas_status_success) return FVM_ERR_HIP;
