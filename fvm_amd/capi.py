@@ -56,7 +56,7 @@ EXPORTS = ("fvm_create", "fvm_destroy", "fvm_set_field", "fvm_get_field", "fvm_s
            "fvm_begin_step", "fvm_contra_vel", "fvm_nl", "fvm_stress_tensor_dy", "fvm_dif",
            "fvm_p_source", "fvm_iter_u", "fvm_correct", "fvm_iter_p", "fvm_pressure_reference",
            "fvm_calccoef", "fvm_profile_enable", "fvm_profile_get", "fvm_profile_reset",
-           "fvm_sync", "fvm_set_dtime")
+           "fvm_sync", "fvm_set_dtime", "fvm_create2d")
 
 _LIB = None
 
@@ -73,6 +73,7 @@ def load(path: str = LIB_PATH):
     vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     pd = ctypes.POINTER(ctypes.c_double)
     L.fvm_create.argtypes = [ctypes.POINTER(Params), pd, pd, pd, ctypes.POINTER(vp)]
+    L.fvm_create2d.argtypes = [ctypes.POINTER(Params), pd, pd, ctypes.POINTER(vp)]
     L.fvm_destroy.argtypes = [vp]
     L.fvm_set_field.argtypes = [vp, ci, pd]
     L.fvm_get_field.argtypes = [vp, ci, pd]

@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "fvm_ctx.h"
 
@@ -83,10 +84,52 @@ static double* field_ptr(FvmCtx* c, int f) {
 
 extern "C" {
 
+static int create_impl(const fvm_params* prm, const double* xg, const double* yg,
+                       const double* zg, int min_nk, fvm_ctx** out);
+
 int fvm_create(const fvm_params* prm, const double* xg, const double* yg, const double* zg,
                fvm_ctx** out) {
+  return create_impl(prm, xg, yg, zg, 3, out);
+}
+
+// The 2D solver (cylinder/cylinder_v3.0.3_AF_original 4step_face_nl/) on a one-cell span slab:
+// nodes at z = 0 and z = LZ2D with periodic span ghosts.  Every span term then vanishes (equal
+// neighbours, zeta_x = zeta_y = xi_z = eta_z = 0) and the span face coefficients are ~1/LZ2D^2
+// of the others, so the step reduces to the 2D scheme; the span length scales J^-1, the
+// fluxes and the operators alike.  No pressure reference is subtracted (the 2D MAIN has none)
+// and CALCCOEF follows the 2D definitions.
+static const double LZ2D = 1.0e6;
+
+int fvm_create2d(const fvm_params* prm, const double* xg, const double* yg, fvm_ctx** out) {
+  if (!prm || !xg || !yg || !out) return FVM_ERR_ARG;
+  if (prm->use_les) return FVM_ERR_ARG;
+  const int ni = prm->ni, nj = prm->nj;
+  if (ni < 4 || nj < 4) return FVM_ERR_ARG;
+  const long nn = (long)ni * nj;
+  std::vector<double> x3(2 * nn), y3(2 * nn), z3(2 * nn);
+  for (int k = 0; k < 2; ++k)
+    for (long q = 0; q < nn; ++q) {
+      x3[q + k * nn] = xg[q];
+      y3[q + k * nn] = yg[q];
+      z3[q + k * nn] = k == 0 ? 0.0 : LZ2D;
+    }
+  fvm_params p3 = *prm;
+  p3.nk = 2;
+  p3.ref_a = LZ2D;  // forces per unit span
+  fvm_ctx* c = nullptr;
+  int r = create_impl(&p3, x3.data(), y3.data(), z3.data(), 2, &c);
+  if (r) return r;
+  c->two_d = true;
+  r = fvm_wall2d_setup(c, xg, yg);
+  if (r) { fvm_destroy(c); return r; }
+  *out = c;
+  return FVM_OK;
+}
+
+static int create_impl(const fvm_params* prm, const double* xg, const double* yg,
+                       const double* zg, int min_nk, fvm_ctx** out) {
   if (!prm || !xg || !yg || !zg || !out) return FVM_ERR_ARG;
-  if (prm->ni < 4 || prm->nj < 4 || prm->nk < 3) return FVM_ERR_ARG;
+  if (prm->ni < 4 || prm->nj < 4 || prm->nk < min_nk) return FVM_ERR_ARG;
   if (prm->ni - 1 > FVM_MAX_LINE || prm->nj - 1 > FVM_MAX_LINE || prm->nk - 1 > FVM_MAX_LINE)
     return FVM_ERR_SIZE;
   if (!(prm->re > 0) || !(prm->dtime > 0) || prm->nswpp < 1) return FVM_ERR_ARG;
@@ -204,6 +247,13 @@ int fvm_set_field(fvm_ctx* c, int field, const double* host) {
   if (!c || !host) return FVM_ERR_ARG;
   double* dptr = field_ptr(c, field);
   if (!dptr) return FVM_ERR_ARG;
+  std::vector<double> slab;
+  if (c->two_d) {  // one (NI+1)(NJ+1) plane, repeated on the slab's three K planes
+    slab.resize(c->d.n);
+    for (int k = 0; k < 3; ++k)
+      memcpy(slab.data() + k * c->d.sk, host, c->d.sk * sizeof(double));
+    host = slab.data();
+  }
   HIP_CHECK(hipMemcpyAsync(dptr, host, c->d.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   if (field >= FVM_NLX && field <= FVM_NLZ)
     HIP_CHECK(hipMemcpyAsync(c->nl1[field - FVM_NLX], host, c->d.n * sizeof(double),
@@ -216,7 +266,9 @@ int fvm_get_field(fvm_ctx* c, int field, double* host) {
   if (!c || !host) return FVM_ERR_ARG;
   double* dptr = field_ptr(c, field);
   if (!dptr) return FVM_ERR_ARG;
-  HIP_CHECK(hipMemcpyAsync(host, dptr, c->d.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (c->two_d) dptr += c->d.sk;  // the slab's interior plane K = 1
+  const long cnt = c->two_d ? c->d.sk : c->d.n;
+  HIP_CHECK(hipMemcpyAsync(host, dptr, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
   return FVM_OK;
 }
@@ -272,13 +324,18 @@ int fvm_iter_p(fvm_ctx* c, double* resip, int* niter) {
 }
 
 int fvm_pressure_reference(fvm_ctx* c, double* pref) {
-  return c ? fvm_k_pref(c, pref) : FVM_ERR_ARG;
+  if (!c) return FVM_ERR_ARG;
+  if (c->two_d) {  // the 2D MAIN keeps p unshifted
+    if (pref) *pref = 0.0;
+    return FVM_OK;
+  }
+  return fvm_k_pref(c, pref);
 }
 
 int fvm_calccoef(fvm_ctx* c, fvm_step_info* info) {
   if (!c) return FVM_ERR_ARG;
   double o[8];
-  int r = fvm_k_calccoef(c, o);
+  int r = c->two_d ? fvm_k_calccoef2d(c, o) : fvm_k_calccoef(c, o);
   if (r) return r;
   if (info) {
     info->cfl = o[0]; info->cpb = o[1]; info->cd = o[2]; info->cl = o[3];
@@ -309,10 +366,13 @@ int fvm_step(fvm_ctx* c, fvm_step_info* info) {
   if (rp && rp != FVM_ERR_NOCONV) return rp;
   CK(fvm_k_p_source(c));                          // :108
   CK(fvm_k_correct(c, +1.0));                     // :109-111
-  CK(fvm_k_pref(c, nullptr));                     // :125-135
+  if (!c->two_d) CK(fvm_k_pref(c, nullptr));     // :125-135 (none in 2D)
   double* f[3] = {c->u, c->v, c->w};
   CK(fvm_k_bcu_ghosts(c, f, 3));                  // :138-142
-  CK(fvm_k_calccoef(c, nullptr));                 // :150
+  if (c->two_d)
+    CK(fvm_k_calccoef2d(c, nullptr));             // 2D CALCCOEF
+  else
+    CK(fvm_k_calccoef(c, nullptr));               // :150
 #undef CK
   HIP_CHECK(hipMemcpyAsync(c->hscal, c->scal, 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));

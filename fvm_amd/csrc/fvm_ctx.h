@@ -82,6 +82,9 @@ struct FvmCtx {
   void* sep = nullptr;
   double sep_dev = 0.0;     // largest relative deviation of a face coefficient from its J,K mean
   long sep_fallbacks = 0;   // FP32 passes handed over to the FP64 inverse
+  // 2D solver on a one-cell span slab
+  bool two_d = false;
+  double* wall2d = nullptr;  // per wall cell J: length, DN1, sin TH2, cos TH2, cos th, sin th
   // last-step diagnostics
   fvm_step_info last{};
   // profiling
@@ -131,6 +134,8 @@ int fvm_k_bcu_ghosts(FvmCtx* c, double** f, int cnt);
 int fvm_k_correct(FvmCtx* c, double sign);
 int fvm_k_pref(FvmCtx* c, double* pref_host);
 int fvm_k_calccoef(FvmCtx* c, double* out8);
+int fvm_k_calccoef2d(FvmCtx* c, double* out8);
+int fvm_wall2d_setup(FvmCtx* c, const double* xg, const double* yg);
 int fvm_k_sgs(FvmCtx* c);
 int fvm_k_iter_u(FvmCtx* c);
 int fvm_k_iter_p(FvmCtx* c, double* resip, int* niter);

@@ -6,6 +6,9 @@
 // bit where the operation sequence is the same.
 #include "fvm_ctx.h"
 
+#include <math.h>
+#include <vector>
+
 namespace {
 
 // interior launch: x -> i (64 wide), y -> j (4), z -> k
@@ -356,6 +359,46 @@ __global__ void k_forces_final(Dims d, const double* __restrict__ p,
   }
 }
 
+// CALCCOEF of the 2D solver (cylinder/cylinder_v3.0.3_AF_original 4step_face_nl/
+// calccoef.f90:11-144) on the slab's plane K = 1, one lane, the reference's summation order.
+// wall2d per J: segment length AREA, centre-to-wall distance DN1, sin/cos TH2 (direction of the
+// wall normal through the centre), cos/sin THETA of the wall cell.
+__global__ void k_calccoef2d(Dims d, const double* __restrict__ wall2d,
+                             const double* __restrict__ p, const double* __restrict__ u,
+                             const double* __restrict__ v, double re, double vel_rot,
+                             const double* __restrict__ cflpart, int ncfl, double* out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int k = 1;
+  double cmax = 0.0;
+  for (int q = 0; q < ncfl; ++q) cmax = fmax(cmax, cflpart[q]);
+  // base pressure: corner average PG(1,1) (calccoef.f90:30-44).  The reference's far-field
+  // value PG(NI,NJ/2) is not assigned by CALCCOEF (its loop covers I = 1..3); it holds what the
+  // last field dump left there (writedata.f90:20), zero before the first dump, and zero here.
+  const double dum_a = 0.0;
+  const double dum_b = 0.25 * (p[cidx(d, 1, 1, k)] + p[cidx(d, 1, 0, k)] + p[cidx(d, 0, 1, k)] +
+                               p[cidx(d, 0, 0, k)]);
+  const double cpb = 2.0 * (dum_a - dum_b);
+  double pdrag = 0.0, plift = 0.0, tdrag = 0.0, tlift = 0.0;
+  for (int j = 1; j <= d.nj - 1; ++j) {
+    const double* o = wall2d + 6 * (j - 1);
+    const double area = o[0], dn1 = o[1], s2 = o[2], c2 = o[3], ct = o[4], st = o[5];
+    const long c = cidx(d, 1, j, k);
+    const double dudn = (u[c] * s2 - v[c] * c2 - vel_rot) / dn1;
+    pdrag = pdrag - 2.0 * p[c] * area * ct;
+    plift = plift - 2.0 * p[c] * area * st;
+    tdrag = tdrag + (2.0 / re) * dudn * area * st;
+    tlift = tlift - (2.0 / re) * dudn * area * ct;
+  }
+  out[0] = cmax;
+  out[1] = cpb;
+  out[2] = pdrag + tdrag;
+  out[3] = plift + tlift;
+  out[4] = pdrag;
+  out[5] = tdrag;
+  out[6] = plift;
+  out[7] = tlift;
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------------------------------
@@ -465,5 +508,72 @@ int fvm_k_calccoef(FvmCtx* c, double* out8) {
                              c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   }
+  return FVM_OK;
+}
+
+int fvm_k_calccoef2d(FvmCtx* c, double* out8) {
+  double** m = c->dm;
+  ProfScope ps(c, "calccoef");
+  const int ncfl = 512;
+  hipLaunchKernelGGL(k_cfl, dim3(ncfl), dim3(256), 0, c->stream, c->d, (const double* const*)m,
+                     c->u, c->v, c->w, c->prm.dtime, c->red);
+  hipLaunchKernelGGL(k_calccoef2d, dim3(1), dim3(64), 0, c->stream, c->d, c->wall2d, c->p, c->u,
+                     c->v, c->prm.re, 0.0, c->red, ncfl, c->scal + 8);
+  HIP_CHECK(hipGetLastError());
+  if (out8) {
+    HIP_CHECK(hipMemcpyAsync(out8, c->scal + 8, 8 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
+  return FVM_OK;
+}
+
+// Wall-cell geometry of the 2D force integrals (calccoef.f90:46-79, init.f90:122-129), from the
+// wall nodes (xg, yg: NI*NJ, I fastest) and the device cell centres and angles, in the
+// reference's operations.
+int fvm_wall2d_setup(FvmCtx* c, const double* xg, const double* yg) {
+  const Dims d = c->d;
+  const int ni = d.ni, nj = d.nj;
+  std::vector<double> x(d.n), y(d.n), th(d.n);
+  HIP_CHECK(hipMemcpy(x.data(), c->x, d.n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(y.data(), c->y, d.n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(th.data(), c->theta, d.n * sizeof(double), hipMemcpyDeviceToHost));
+  const double pi = c->pi;
+  std::vector<double> w(6L * (nj - 1));
+  for (int j = 1; j <= nj - 1; ++j) {
+    const long cc = cidx(d, 1, j, 1);
+    const double X = x[cc], Y = y[cc];
+    const double xa = xg[(long)(j - 1) * ni], ya = yg[(long)(j - 1) * ni];  // node (1, j)
+    const double xb = xg[(long)j * ni], yb = yg[(long)j * ni];              // node (1, j+1)
+    const double l1 = pow(pow(X - xa, 2.0) + pow(Y - ya, 2.0), 0.5);
+    const double l2 = pow(pow(X - xb, 2.0) + pow(Y - yb, 2.0), 0.5);
+    const double l3 = pow(pow(xb - xa, 2.0) + pow(yb - ya, 2.0), 0.5);
+    const double th1 = acos((l1 * l1 + l2 * l2 - l3 * l3) / (2.0 * l1 * l2));
+    const double dn1 = l1 * l2 / l3 * sin(th1);
+    const double la = (yb - ya) / (xb - xa);
+    const double lb = ya - la * xa;
+    const double la_ = -1.0 / la;
+    const double lb_ = Y - la_ * X;
+    const double x1 = (lb_ - lb) / (la - la_);
+    const double y1 = la * (lb_ - lb) / (la - la_) + lb;
+    const double dn2 = pow(pow(Y - y1, 2.0) + pow(X - x1, 2.0), 0.5);
+    double th2 = atan((Y - y1) / (X - x1));
+    const double as = asin((Y - y1) / dn2);
+    if (th2 >= 0.0 && as > 0.0) {
+      // first quadrant
+    } else if (th2 < 0.0 && as > 0.0) th2 = pi + th2;
+    else if (th2 >= 0.0 && as <= 0.0) th2 = pi + th2;
+    else if (th2 < 0.0 && as < 0.0) th2 = 2.0 * pi + th2;
+    double* o = w.data() + 6 * (j - 1);
+    o[0] = l3;            // AREA(1, J) = |XG(1,J+1) - XG(1,J)|
+    o[1] = dn1;
+    o[2] = sin(th2);
+    o[3] = cos(th2);
+    o[4] = cos(th[cc]);
+    o[5] = sin(th[cc]);
+  }
+  HIP_CHECK(hipMalloc((void**)&c->wall2d, w.size() * sizeof(double)));
+  c->allocs.push_back(c->wall2d);
+  HIP_CHECK(hipMemcpy(c->wall2d, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
   return FVM_OK;
 }

@@ -1,7 +1,9 @@
-"""Host-side mirror of the reference time loop over the C ABI (RLES/main.f90:35-161).
+"""Host-side mirror of the reference time loops over the C ABI.
 
-Solver3D owns one device context; its methods keep the reference's subroutine names so a
-driver reads like the reference's MAIN.
+Solver3D (RLES/main.f90:35-161) and Solver2D (cylinder/cylinder_v3.0.3_AF_original
+4step_face_nl/Main.f90:48-110) own one device context each; their methods keep the reference's
+subroutine names so a driver reads like the reference's MAIN.  Post2D is the 2D post-processing
+phase of CALCCOEF (Strouhal number from C_L extrema, running means and rms).
 """
 from __future__ import annotations
 
@@ -128,3 +130,68 @@ class Solver3D:
             nm = names.raw[48 * q:48 * q + 48].split(b"\0", 1)[0].decode()
             out[nm] = (ms[q], calls[q])
         return out
+
+
+class Solver2D(Solver3D):
+    """2D fractional 4-step solver (one-cell span slab of the 3D kernels, see fvm_create2d).
+    Fields are (NI+1, NJ+1) arrays of the single plane, as in the 2D reference's RESTART.DAT."""
+
+    def __init__(self, ni, nj, re, dtime, xg, yg, nswpp=20, resimax=1e-4, poisson_mode=2,
+                 check_every=1, af_mode=0, af_tol=1e-14):
+        self.L = capi.load()
+        self.ni, self.nj, self.nk = ni, nj, 2
+        self.shape = (ni + 1, nj + 1)
+        self.prm = capi.Params(ni, nj, 2, float(re), float(dtime), 1.0, 0, int(nswpp),
+                               float(resimax), int(poisson_mode), int(check_every),
+                               int(af_mode), float(af_tol))
+        flat = [np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape((ni, nj), order="F")
+                                     .ravel(order="F")) for a in (xg, yg)]
+        h = ctypes.c_void_p()
+        capi.check(self.L.fvm_create2d(ctypes.byref(self.prm), *[capi._dptr(a) for a in flat],
+                                       ctypes.byref(h)), "fvm_create2d")
+        self.h = h
+
+    def set_state(self, u, v, p, nlx, nly):
+        z = np.zeros(self.shape)
+        for n, a in zip(("u", "v", "w", "p", "nlx", "nly", "nlz"), (u, v, z, p, nlx, nly, z)):
+            self.set(n, a)
+
+
+class Post2D:
+    """Post-processing phase of the 2D CALCCOEF (calccoef.f90:156-195): the Strouhal number
+    from successive sign changes of dC_L, the C_L amplitude, running means of C_D, C_L, St,
+    C_pb and the rms of C_D and C_L about their running means."""
+
+    def __init__(self, cl_last=0.0):
+        # cl_last: C_L of the step before the post phase (CALCCOEF keeps CL_NM1 every step)
+        self.toggle = 1
+        self.time1 = self.time2 = 0.0
+        self.cl_prev = cl_last
+        self.cl1 = 0.0
+        self.st = 0.0
+        self.cl_amp = 0.0
+        self.n = 0
+        self.cd_ta = self.cl_ta = self.st_ta = self.cpb_ta = self.amp_ta = 0.0
+        self.rms_cd = self.rms_cl = 0.0
+
+    def update(self, rtime, cd, cl, cpb):
+        self.n += 1
+        if float(self.toggle) * (cl - self.cl_prev) < 0.0:
+            self.toggle = -self.toggle
+            self.st = 1.0 / (rtime - self.time2)
+            self.time2 = self.time1
+            self.time1 = rtime
+            self.cl_amp = abs(cl - self.cl1) / 2.0
+            self.cl1 = cl
+        self.cl_prev = cl
+        self.cd_ta += cd
+        self.cl_ta += cl
+        self.st_ta += self.st
+        self.cpb_ta += cpb
+        self.amp_ta += self.cl_amp
+        mcd, mcl = self.cd_ta / self.n, self.cl_ta / self.n
+        self.rms_cd += (cd - mcd) ** 2
+        self.rms_cl += (cl - mcl) ** 2
+        return dict(cd_mean=mcd, cl_mean=mcl, st=self.st, st_mean=self.st_ta / self.n,
+                    cl_amp=self.cl_amp, cpb_mean=self.cpb_ta / self.n,
+                    cd_rms=math.sqrt(self.rms_cd / self.n), cl_rms=math.sqrt(self.rms_cl / self.n))

@@ -75,6 +75,15 @@ int fvm_create(const fvm_params* prm, const double* xg, const double* yg, const 
                fvm_ctx** out);
 int fvm_destroy(fvm_ctx* ctx);
 
+/* The 2D solver (cylinder/cylinder_v3.0.3_AF_original 4step_face_nl/: GRID grid.f90, METRIC
+ * metric.f90, INIT init.f90, MAIN Main.f90:48-110) on a one-cell span slab of the same kernels.
+ * xg/yg: NI*NJ node coordinates of the 2D grid file (I fastest); prm->nk and prm->ref_a are
+ * ignored (forces per unit span), prm->use_les must be 0 and the wall does not rotate
+ * (VEL_ROT = 0).  Field arrays of a 2D context are one (NI+1)(NJ+1) plane; fvm_step skips the
+ * pressure reference and fvm_calccoef returns the 2D force coefficients
+ * (calccoef.f90:11-144). */
+int fvm_create2d(const fvm_params* prm, const double* xg, const double* yg, fvm_ctx** out);
+
 /* Restart read / write of the module arrays (RLES/init.f90:183-208, RLES/writedata.f90:193-216):
  * copies one full cell array host <-> device.  Setting NLX..NLZ sets both the current and the
  * previous-step convective term (the restart holds only the current one). */

@@ -44,6 +44,7 @@ typedef struct orc3d {
   /* scalars of the last step */
   double uin, resip, pref, cfl, cpb, cd, cl, cdp, cdv, clp, clv;
   int niter;
+  int no_pref; /* 1: skip the span-mean pressure reference (the 2D solver has none) */
 } orc3d;
 
 #define IX(o, i, j, k) ((long)(i) + (o)->sj * (long)(j) + (o)->sk * (long)(k))
@@ -98,6 +99,8 @@ double *orc3d_array(orc3d *o, const char *name) {
 #undef N
   return NULL;
 }
+
+void orc3d_set_no_pref(orc3d *o, int on) { o->no_pref = on; }
 
 double orc3d_scalar(orc3d *o, const char *name) {
   if (!strcmp(name, "resip")) return o->resip;
@@ -1106,11 +1109,13 @@ void orc3d_step(orc3d *o, int nswpp, double resimax, int use_les) {
         o->v[c] = o->v[c] + dt * o->s_v[c] / o->djac[c];
         o->w[c] = o->w[c] + dt * o->s_w[c] / o->djac[c];
       }
-  double dumb = 0.0;
-  for (int k = 1; k <= nk - 1; ++k) dumb = dumb + o->p[IX(o, ni, nj / 4, k)];
-  dumb = dumb / (double)(nk - 1);
-  o->pref = dumb;
-  for (long c = 0; c < n; ++c) o->p[c] = o->p[c] - dumb / 1.0;
+  if (!o->no_pref) {
+    double dumb = 0.0;
+    for (int k = 1; k <= nk - 1; ++k) dumb = dumb + o->p[IX(o, ni, nj / 4, k)];
+    dumb = dumb / (double)(nk - 1);
+    o->pref = dumb;
+    for (long c = 0; c < n; ++c) o->p[c] = o->p[c] - dumb / 1.0;
+  }
   o->uin = 1.0; orc3d_bcu(o, o->u, o->un);
   o->uin = 0.0; orc3d_bcu(o, o->v, o->vn); orc3d_bcu(o, o->w, o->wn);
   orc3d_calccoef(o);
