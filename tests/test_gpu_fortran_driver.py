@@ -48,3 +48,35 @@ def test_fortran_driver_two_steps(tmp_path):
         assert abs(rows[st, 2] - ref[6]) <= 1e-9 * abs(ref[6])      # CD
         assert abs(rows[st, 3] - ref[7]) <= 1e-9 * max(abs(ref[7]), 1.0)  # CL
         assert abs(rows[st, 9] - ref[4]) <= 1e-10 * ref[4]           # CFL
+
+
+EXE2D = os.path.join(REPO, "fvm_amd", "bin", "fvm_r2d")
+
+
+@pytest.mark.skipif(not os.path.exists(EXE2D), reason="Fortran driver not built (flang absent)")
+def test_fortran_2d_driver_three_steps(tmp_path):
+    """The 2D driver reads the 2D reference's NNNxNNN.dat and list-directed RESTART.DAT and
+    reproduces its three converged steps."""
+    from fvm_amd.grid import grid_filename, read_restart2d, write_grid2d, write_restart2d
+    g = load_golden("r2d_17x33_re100_converged")
+    ni, nj = int(g["ni"]), int(g["nj"])
+    write_grid2d(str(tmp_path / grid_filename(ni, nj)), g["xg"][:, :, None], g["yg"][:, :, None])
+    z = np.zeros((ni + 1, nj + 1, 1))
+    f = {k: g["in_" + k][:, :, None] for k in ("u", "v", "p", "nlx", "nly")}
+    f.update(adv=z, t=z)
+    write_restart2d(str(tmp_path / "RESTART.DAT"), f, float(g["re"]), 0.71, float(g["dt"]), 0.0)
+    (tmp_path / "fvm_r2d.nml").write_text(
+        f"&FVM_RUN NI={ni}, NJ={nj}, RESIMAX=1.0D-15, NSWPP=50, POISSON_MODE=2, NSTEP=3 /\n")
+    print("fortran 2D driver: r2d_17x33, 3 steps", flush=True)
+    out = subprocess.run([EXE2D], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    r = read_restart2d(str(tmp_path / "RESTART.DAT"), ni, nj)
+    for c in ("u", "v", "p"):
+        assert rel(r[c][:, :, 0], g[f"s3_{c}"]) < 1e-10, c
+    body = (tmp_path / "NU_CD_CL.PLT").read_text().split("\n", 1)[1]
+    rows = np.array(body.split(), dtype=np.float64).reshape(-1, 10)
+    assert rows.shape == (3, 10)
+    for st in range(3):
+        ref = g["scalars"][st]  # step niter resip cfl cpb cd cl cd_p cd_v cl_p cl_v
+        assert abs(rows[st, 2] - ref[5]) <= 1e-9 * abs(ref[5])             # CD
+        assert abs(rows[st, 3] - ref[6]) <= 1e-9 * max(abs(ref[6]), 1.0)   # CL
