@@ -29,15 +29,24 @@ const dim3 kBlk(64, 4, 1);
 // Q = SU' + c_(a+1) x_in(a+1) + AT x_in(K+1) + AB x_in(K-1): SU' carries BCU's far-field
 // addition 2 AE U_F at I = NI-1 (bcu.f90:38-40; the wall addition multiplies U_F = 0), c_(a+1)
 // is AN (stage 0) or the folded AE (stage 1).
+// With ddzero set (stage 2 on a grid whose xi_z and eta_z vanish identically around every cell)
+// each of the twelve cross terms is an exact zero and SU = 2 Re J^-1/dt phi exactly.
 template <int S>
 __global__ void k_af_rhs(Dims d, AFArgs A, const double* const* __restrict__ m,
                          const double* __restrict__ djac, const double* __restrict__ ap0,
-                         double re, const double* const* __restrict__ fa, int qform) {
+                         double re, const double* const* __restrict__ fa, int qform,
+                         int ddzero) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
   if (i > d.ni - 1 || j > d.nj - 1) return;
   const long c = cidx(d, i, j, k);
+  if (S == 2 && ddzero) {
+    const double a0 = ap0[c];
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) A.su[comp][c] = a0 * A.f[comp][c];
+    return;
+  }
   const long sj = d.sj, sk = d.sk;
   const double* X = m[3 * S];
   const double* E = m[3 * S + 1];
@@ -401,6 +410,31 @@ __global__ void k_af_couplings(Dims d, const double* const* __restrict__ fa,
   }
 }
 
+// Largest |xi_z|, |eta_z| over every interior cell and its six neighbours (part[block])
+__global__ void k_zmetric_max(Dims d, const double* __restrict__ xiz,
+                              const double* __restrict__ etz, double* part) {
+  __shared__ double s0[256];
+  const long nint = (long)(d.ni - 1) * (d.nj - 1) * (d.nk - 1);
+  double mx = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nint;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = 1 + (int)(t % (d.ni - 1));
+    const long r = t / (d.ni - 1);
+    const int j = 1 + (int)(r % (d.nj - 1)), k = 1 + (int)(r / (d.nj - 1));
+    const long c = cidx(d, i, j, k);
+    const long nb[7] = {c, c + 1, c - 1, c + d.sj, c - d.sj, c + d.sk, c - d.sk};
+    for (int q = 0; q < 7; ++q) mx = fmax(mx, fmax(fabs(xiz[nb[q]]), fabs(etz[nb[q]])));
+  }
+  const int t = threadIdx.x;
+  s0[t] = mx;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st) s0[t] = fmax(s0[t], s0[t + st]);
+    __syncthreads();
+  }
+  if (t == 0) part[blockIdx.x] = s0[0];
+}
+
 // Factors of the reference's Thomas recursion along a K-line (iter_u.f90:240-255):
 // A = -AB, D = AP - A B(K-1), B = -AT/D with B(0) = 0 (the SAVE'd scratch is never written at
 // index 0).  AP carries the wall / far-field folds of BCU.  One thread per (I, J) line.
@@ -537,9 +571,10 @@ int fvm_k_iter_u(FvmCtx* c) {
       ProfScope ps(c, "af_rhs");
       const int qf = (s < 2 && c->prm.af_mode == 0 && c->af_scan[s]) ? 1 : 0;
       const double* const* fas = (const double* const*)c->dfa[s];
-      if (s == 0) hipLaunchKernelGGL(k_af_rhs<0>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf);
-      if (s == 1) hipLaunchKernelGGL(k_af_rhs<1>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf);
-      if (s == 2) hipLaunchKernelGGL(k_af_rhs<2>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf);
+      const int dz = c->dd2_zero ? 1 : 0;
+      if (s == 0) hipLaunchKernelGGL(k_af_rhs<0>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf, 0);
+      if (s == 1) hipLaunchKernelGGL(k_af_rhs<1>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf, 0);
+      if (s == 2) hipLaunchKernelGGL(k_af_rhs<2>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf, dz);
       HIP_CHECK(hipGetLastError());
     }
     // boundary ghosts of the stage input (BCU, iter_u.f90:83,160,237)
@@ -611,6 +646,15 @@ int fvm_af_setup(FvmCtx* c) {
     c->af_zero_a[s] = (ma == 0.0);
     c->af_zero_b[s] = (mb == 0.0);
     c->af_b_ratio[s] = rb;
+  }
+  {
+    hipLaunchKernelGGL(k_zmetric_max, dim3(nb), dim3(256), 0, st, d, c->m[6], c->m[7], c->red);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(h.data(), c->red, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    double mz = 0.0;
+    for (int q = 0; q < nb; ++q) mz = std::max(mz, h[q]);
+    c->dd2_zero = (mz == 0.0);
   }
   if (c->af_zero_a[2] && c->af_zero_b[2]) {
     if (!c->tk_a) {

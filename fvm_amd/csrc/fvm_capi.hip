@@ -358,14 +358,12 @@ int fvm_step(fvm_ctx* c, fvm_step_info* info) {
   CK(fvm_k_dif(c));                               // :72-75
   CK(fvm_k_p_source(c));                          // :78
   CK(fvm_k_iter_u(c));                            // :82-88
-  CK(fvm_correct(c, -1.0));                       // :90-98
-  CK(fvm_k_contra_vel(c));                        // :101
+  CK(fvm_k_correct_bcu_contra(c));                // :90-101 (correction, BCU, CONTRA_VEL)
   double res = 0.0;
   int it = 0;
   int rp = fvm_k_iter_p(c, &res, &it);            // :103-105 (BCP included)
   if (rp && rp != FVM_ERR_NOCONV) return rp;
-  CK(fvm_k_p_source(c));                          // :108
-  CK(fvm_k_correct(c, +1.0));                     // :109-111
+  CK(fvm_k_psource_correct(c, +1.0));             // :108-111 (P_SOURCE, projection)
   if (!c->two_d) CK(fvm_k_pref(c, nullptr));     // :125-135 (none in 2D)
   double* f[3] = {c->u, c->v, c->w};
   CK(fvm_k_bcu_ghosts(c, f, 3));                  // :138-142

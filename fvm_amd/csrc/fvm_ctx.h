@@ -63,7 +63,8 @@ struct FvmCtx {
   int af_iters[3] = {};
   // stage couplings that vanish identically on this grid (a: along the sweep, b: across it)
   bool af_zero_a[3] = {}, af_zero_b[3] = {};
-  double af_b_ratio[3] = {};  // max over cells of (|b couplings|) / (2 Re J^-1/dt)
+  double af_b_ratio[3] = {};
+  bool dd2_zero = false;      // xi_z = eta_z = 0 around every cell: stage-3 cross terms vanish  // max over cells of (|b couplings|) / (2 Re J^-1/dt)
   // Thomas factors A, D, B of the K-line stage when its lines are independent
   double *tk_a = nullptr, *tk_d = nullptr, *tk_b = nullptr;
   // line-scan form of the I and J stages when the K couplings vanish: per stage the line
@@ -132,6 +133,8 @@ int fvm_k_p_source(FvmCtx* c);
 int fvm_k_ufar(FvmCtx* c);
 int fvm_k_bcu_ghosts(FvmCtx* c, double** f, int cnt);
 int fvm_k_correct(FvmCtx* c, double sign);
+int fvm_k_psource_correct(FvmCtx* c, double sign);
+int fvm_k_correct_bcu_contra(FvmCtx* c);
 int fvm_k_pref(FvmCtx* c, double* pref_host);
 int fvm_k_calccoef(FvmCtx* c, double* out8);
 int fvm_k_calccoef2d(FvmCtx* c, double* out8);

@@ -151,6 +151,75 @@ __global__ void k_psource(Dims d, const double* const* __restrict__ m,
   }
 }
 
+// P_SOURCE(P) followed by U = U + sign DTIME S / J^-1 on the same cell (RLES/main.f90:108-111;
+// 90-92 with sign -1): one pass, S kept for the next step's predictor.
+__global__ void k_psource_correct(Dims d, const double* const* __restrict__ m,
+                                  const double* __restrict__ djac, const double* __restrict__ p,
+                                  double sign, double dtime, double* s0, double* s1, double* s2,
+                                  double* u, double* v, double* w) {
+  INTERIOR_IJK(d);
+  double* out[3] = {s0, s1, s2};
+  double* f[3] = {u, v, w};
+  const long sj = d.sj, sk = d.sk;
+  const double J = djac[c];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double *a = m[3 * q], *b = m[3 * q + 1], *g = m[3 * q + 2];
+    double t = djac[c + 1] * a[c + 1] * p[c + 1];
+    t = t - djac[c - 1] * a[c - 1] * p[c - 1];
+    t = t + djac[c + sj] * b[c + sj] * p[c + sj];
+    t = t - djac[c - sj] * b[c - sj] * p[c - sj];
+    t = t + djac[c + sk] * g[c + sk] * p[c + sk];
+    t = t - djac[c - sk] * g[c - sk] * p[c - sk];
+    const double sv = -(0.5 * t);
+    out[q][c] = sv;
+    f[q][c] = sign < 0 ? f[q][c] - dtime * sv / J : f[q][c] + dtime * sv / J;
+  }
+}
+
+// predictor correction U = U - DTIME S / J^-1 fused with CONTRA_VEL on the interior cells
+// (RLES/main.f90:90-92, contra_vel.f90:13-23); the I ghost planes follow after BCU
+__global__ void k_correct_contra(Dims d, const double* const* __restrict__ m,
+                                 const double* __restrict__ djac, double dtime,
+                                 const double* __restrict__ s0, const double* __restrict__ s1,
+                                 const double* __restrict__ s2, double* u, double* v, double* w,
+                                 double* ju0, double* ju1, double* ju2) {
+  INTERIOR_IJK(d);
+  const double J = djac[c];
+  const double U = u[c] - dtime * s0[c] / J;
+  const double V = v[c] - dtime * s1[c] / J;
+  const double W = w[c] - dtime * s2[c] / J;
+  u[c] = U; v[c] = V; w[c] = W;
+  const double u1 = m[0][c] * U + m[3][c] * V + m[6][c] * W;
+  const double u2 = m[1][c] * U + m[4][c] * V + m[7][c] * W;
+  const double u3 = m[2][c] * U + m[5][c] * V + m[8][c] * W;
+  ju0[c] = J * u1;
+  ju1[c] = J * u2;
+  ju2[c] = J * u3;
+}
+
+// CONTRA_VEL on the I ghost planes i = 0 and i = NI only
+__global__ void k_contra_iplanes(Dims d, const double* const* __restrict__ m,
+                                 const double* __restrict__ djac, const double* __restrict__ u,
+                                 const double* __restrict__ v, const double* __restrict__ w,
+                                 double* ju0, double* ju1, double* ju2) {
+  const long nf = (long)(d.nj - 1) * (d.nk - 1);
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < 2 * nf;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = t < nf ? 0 : d.ni;
+    const long r = t % nf;
+    const int j = 1 + (int)(r % (d.nj - 1)), k = 1 + (int)(r / (d.nj - 1));
+    const long c = cidx(d, i, j, k);
+    const double U = u[c], V = v[c], W = w[c], J = djac[c];
+    const double u1 = m[0][c] * U + m[3][c] * V + m[6][c] * W;
+    const double u2 = m[1][c] * U + m[4][c] * V + m[7][c] * W;
+    const double u3 = m[2][c] * U + m[5][c] * V + m[8][c] * W;
+    ju0[c] = J * u1;
+    ju1[c] = J * u2;
+    ju2[c] = J * u3;
+  }
+}
+
 // Far-field boundary value U_F per (j,k) and component: inflow Dirichlet UIN where the ghost
 // centre's angle lies in [pi/2, 3pi/2], else the convective outflow condition advanced with
 // the step-start velocity B = U_N (RLES/bcu.f90:27-37).  Constant within a step.
@@ -576,4 +645,34 @@ int fvm_wall2d_setup(FvmCtx* c, const double* xg, const double* yg) {
   c->allocs.push_back(c->wall2d);
   HIP_CHECK(hipMemcpy(c->wall2d, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
   return FVM_OK;
+}
+
+int fvm_k_psource_correct(FvmCtx* c, double sign) {
+  ProfScope ps(c, "psource_correct");
+  hipLaunchKernelGGL(k_psource_correct, interior_grid(c->d), kBlk, 0, c->stream, c->d,
+                     (const double* const*)c->dm, c->djac, c->p, sign, c->prm.dtime, c->s[0],
+                     c->s[1], c->s[2], c->u, c->v, c->w);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+// U = U - DTIME S / J^-1, BCU on u, v, w, CONTRA_VEL (RLES/main.f90:90-101)
+int fvm_k_correct_bcu_contra(FvmCtx* c) {
+  const Dims d = c->d;
+  {
+    ProfScope ps(c, "correct_contra");
+    hipLaunchKernelGGL(k_correct_contra, interior_grid(d), kBlk, 0, c->stream, d,
+                       (const double* const*)c->dm, c->djac, c->prm.dtime, c->s[0], c->s[1],
+                       c->s[2], c->u, c->v, c->w, c->ju[0], c->ju[1], c->ju[2]);
+    HIP_CHECK(hipGetLastError());
+  }
+  double* f[3] = {c->u, c->v, c->w};
+  int r = fvm_k_bcu_ghosts(c, f, 3);
+  if (r) return r;
+  const long nf = (long)(d.nj - 1) * (d.nk - 1);
+  hipLaunchKernelGGL(k_contra_iplanes, grid1(2 * nf, 256), dim3(256), 0, c->stream, d,
+                     (const double* const*)c->dm, c->djac, c->u, c->v, c->w, c->ju[0], c->ju[1],
+                     c->ju[2]);
+  HIP_CHECK(hipGetLastError());
+  return fvm_ghost_fill_n(c, c->ju, 3);
 }
