@@ -43,10 +43,8 @@ KERNEL_BYTES = {
     # chunked fallback pass over the 3 components: {phi_in, SU, J, 3 stage metric products}
     # -> phi_out = 7 doubles per cell and component
     "k_af_chunk_i": (56, 3), "k_af_chunk_j": (56, 3), "k_af_chunk_k": (56, 3),
-    # strain rates: {u,v,w, xi 9} -> {S 6, |S|}
-    "k_strain": (8 * 19, 1),
-    # 27-point test filter of u, v, w
-    "k_filter_vel": (8 * 6, 1),
+    # strain rates fused with the 27-point test filter: {u,v,w, xi 9} -> {S 6, |S|, u^ 3}
+    "k_strain_filter": (8 * 22, 1),
     # Germano identity terms: {u,v,w, u^ 3, xi 9, S 6, |S|, Delta^2} -> {LM, MM}
     "k_lm": (8 * 25, 1),
     # SGS divergence: {xi 9, nu_t, S 6, J, NL 3} -> {NL 3, SGS 3}

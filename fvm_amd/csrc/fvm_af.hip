@@ -36,9 +36,10 @@ __global__ void k_af_rhs(Dims d, AFArgs A, const double* const* __restrict__ m,
                          const double* __restrict__ djac, const double* __restrict__ ap0,
                          double re, const double* const* __restrict__ fa, int qform,
                          int ddzero) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const uint3 bb = xcd_block();
+  const int i = 1 + bb.x * 64 + threadIdx.x;
+  const int j = 1 + bb.y * 4 + threadIdx.y;
+  const int k = 1 + bb.z;
   if (i > d.ni - 1 || j > d.nj - 1) return;
   const long c = cidx(d, i, j, k);
   if (S == 2 && ddzero) {

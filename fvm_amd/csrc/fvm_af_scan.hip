@@ -28,6 +28,8 @@
 //   converged iterate is transposed back into the field.
 #include "fvm_ctx.h"
 
+#include <string.h>
+
 #include <algorithm>
 #include <vector>
 
@@ -327,11 +329,15 @@ int scan_stage(FvmCtx* c, const AFArgs& A, int* iters) {
   for (int k = 0; k < 3; ++k) q[k] = S == 0 ? A.su[k] : c->afb[k][1];
   double* xo[3];
   for (int k = 0; k < 3; ++k) xo[k] = S == 0 ? A.f[k] : c->afb[k][0];
-  // device pointer table: q at afptr[0..2], iterate at [3..5], field at [6..8]
+  // device pointer table of this stage: q at [0..2], iterate at [3..5], field at [6..8]; the
+  // buffers are the context's own, so it is uploaded once and again only if they change
   double* tab[9] = {q[0], q[1], q[2], xo[0], xo[1], xo[2], A.f[0], A.f[1], A.f[2]};
-  HIP_CHECK(hipMemcpyAsync(c->afptr, tab, sizeof(tab), hipMemcpyHostToDevice, st));
-  double* const* dq = c->afptr;
-  double* const* dxo = c->afptr + 3;
+  if (memcmp(tab, c->scantab_h[S], sizeof(tab)) != 0) {
+    HIP_CHECK(hipMemcpyAsync(c->scantab[S], tab, sizeof(tab), hipMemcpyHostToDevice, st));
+    memcpy(c->scantab_h[S], tab, sizeof(tab));
+  }
+  double* const* dq = c->scantab[S];
+  double* const* dxo = c->scantab[S] + 3;
   {
     ProfScope pk(c, "af_scan_q");
     if (S == 1) {
@@ -373,7 +379,7 @@ int scan_stage(FvmCtx* c, const AFArgs& A, int* iters) {
   if (S == 1) {
     ProfScope pk(c, "af_scan_unpack");
     hipLaunchKernelGGL(k_scan_unpack1, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, 3 * (d.nk - 1)),
-                       dim3(256), 0, st, d, (const double* const*)dxo, c->afptr + 6);
+                       dim3(256), 0, st, d, (const double* const*)dxo, c->scantab[S] + 6);
     HIP_CHECK(hipGetLastError());
   }
   *iters = it;

@@ -179,6 +179,8 @@ static int create_impl(const fvm_params* prm, const double* xg, const double* yg
   long nafw = 3L * ((std::max(prm->ni, prm->nj) + 7) / 8 + 1) * std::max(prm->nj, prm->nk) + 8;
   c->nred = (int)std::max(std::max(nblk * 2, 2 * nafw), 4096L);
   r |= dalloc(c, &c->red, c->nred);
+  c->ncflpart = (int)nblk;
+  r |= dalloc(c, &c->cflpart, nblk);
   r |= dalloc(c, &c->scal, 64);
   double *p32 = nullptr, *su32 = nullptr;
   r |= dalloc(c, &p32, (n + 1) / 2);
@@ -191,8 +193,8 @@ static int create_impl(const fvm_params* prm, const double* xg, const double* yg
     return FVM_ERR_HIP;
   }
   // device pointer tables
-  double** tabs[6];
-  for (int t = 0; t < 6; ++t) {
+  double** tabs[8];
+  for (int t = 0; t < 8; ++t) {
     if (hipMalloc((void**)&tabs[t], 9 * sizeof(double*)) != hipSuccess) { fvm_destroy(c); return FVM_ERR_HIP; }
     c->allocs.push_back(tabs[t]);
   }
@@ -200,6 +202,8 @@ static int create_impl(const fvm_params* prm, const double* xg, const double* yg
   c->dlss = tabs[1];
   for (int s = 0; s < 3; ++s) c->dfa[s] = tabs[2 + s];
   c->afptr = tabs[5];
+  c->scantab[0] = tabs[6];
+  c->scantab[1] = tabs[7];
   (void)hipMemcpy(c->dm, c->m, 9 * sizeof(double*), hipMemcpyHostToDevice);
   (void)hipMemcpy(c->dlss, c->lss, 6 * sizeof(double*), hipMemcpyHostToDevice);
   for (int s = 0; s < 3; ++s) (void)hipMemcpy(c->dfa[s], c->fa[s], 3 * sizeof(double*), hipMemcpyHostToDevice);
@@ -363,14 +367,14 @@ int fvm_step(fvm_ctx* c, fvm_step_info* info) {
   int it = 0;
   int rp = fvm_k_iter_p(c, &res, &it);            // :103-105 (BCP included)
   if (rp && rp != FVM_ERR_NOCONV) return rp;
-  CK(fvm_k_psource_correct(c, +1.0));             // :108-111 (P_SOURCE, projection)
+  CK(fvm_k_psource_correct(c, +1.0, true));       // :108-111 (P_SOURCE, projection; CFL maxima)
   if (!c->two_d) CK(fvm_k_pref(c, nullptr));     // :125-135 (none in 2D)
   double* f[3] = {c->u, c->v, c->w};
   CK(fvm_k_bcu_ghosts(c, f, 3));                  // :138-142
   if (c->two_d)
-    CK(fvm_k_calccoef2d(c, nullptr));             // 2D CALCCOEF
+    CK(fvm_k_calccoef2d(c, nullptr, true));       // 2D CALCCOEF
   else
-    CK(fvm_k_calccoef(c, nullptr));               // :150
+    CK(fvm_k_calccoef(c, nullptr, true));         // :150
 #undef CK
   HIP_CHECK(hipMemcpyAsync(c->hscal, c->scal, 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));

@@ -26,6 +26,19 @@ __host__ __device__ inline long cidx(const Dims& d, int i, int j, int k) {
   return (long)i + d.sj * (long)j + d.sk * (long)k;
 }
 
+// XCD-aware block order for the 3D stencil launches.  Workgroups are dealt round-robin, in flat
+// (x fastest) order, over the 8 XCDs, each with its own L2.  This bijection of the flat block id
+// hands XCD x one contiguous run of tiles in flat order (J rows, then K planes), so the J+-1 and
+// K+-1 neighbours a tile reads were mostly fetched into the same L2 by the tiles next to it.
+__device__ inline uint3 xcd_block() {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned nb = gx * gy * gridDim.z;
+  const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned x = b & 7u, slot = b >> 3, q = nb >> 3, r = nb & 7u;
+  const unsigned w = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + slot;
+  return make_uint3(w % gx, (w / gx) % gy, w / (gx * gy));
+}
+
 // Maximum line length handled by one 64-lane wavefront (8 rows per lane).
 #define FVM_MAX_LINE 512
 

@@ -75,6 +75,10 @@ struct FvmCtx {
   long nbnd = 0;
   // reductions
   double* red = nullptr;     // partial sums
+  double** scantab[2] = {nullptr, nullptr};  // device pointer tables of the I/J scan stages
+  double* scantab_h[2][9] = {};               // their last uploaded contents
+  double* cflpart = nullptr; // per-tile CFL maxima of the projection kernel (one per 64x4 tile)
+  int ncflpart = 0;
   double* scal = nullptr;    // device scalars (c_avg, residual sums, pref, ...)
   double* hscal = nullptr;   // pinned host mirror of scal
 
@@ -133,11 +137,11 @@ int fvm_k_p_source(FvmCtx* c);
 int fvm_k_ufar(FvmCtx* c);
 int fvm_k_bcu_ghosts(FvmCtx* c, double** f, int cnt);
 int fvm_k_correct(FvmCtx* c, double sign);
-int fvm_k_psource_correct(FvmCtx* c, double sign);
+int fvm_k_psource_correct(FvmCtx* c, double sign, bool with_cfl);
 int fvm_k_correct_bcu_contra(FvmCtx* c);
 int fvm_k_pref(FvmCtx* c, double* pref_host);
-int fvm_k_calccoef(FvmCtx* c, double* out8);
-int fvm_k_calccoef2d(FvmCtx* c, double* out8);
+int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready = false);
+int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready = false);
 int fvm_wall2d_setup(FvmCtx* c, const double* xg, const double* yg);
 int fvm_k_sgs(FvmCtx* c);
 int fvm_k_iter_u(FvmCtx* c);

@@ -18,9 +18,10 @@ inline dim3 interior_grid(const Dims& d, int nz = 1) {
 const dim3 kBlk(64, 4, 1);
 
 #define INTERIOR_IJK(d)                                   \
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;        \
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;         \
-  const int k = 1 + (int)(blockIdx.z % (d.nk - 1));       \
+  const uint3 bb = xcd_block();                           \
+  const int i = 1 + bb.x * 64 + threadIdx.x;              \
+  const int j = 1 + bb.y * 4 + threadIdx.y;               \
+  const int k = 1 + (int)(bb.z % (d.nk - 1));             \
   if (i > d.ni - 1 || j > d.nj - 1) return;               \
   const long c = cidx(d, i, j, k);
 
@@ -29,9 +30,10 @@ const dim3 kBlk(64, 4, 1);
 __global__ void k_contra(Dims d, const double* const* __restrict__ m, const double* __restrict__ djac,
                          const double* __restrict__ u, const double* __restrict__ v,
                          const double* __restrict__ w, double* ju0, double* ju1, double* ju2) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const uint3 bb = xcd_block();
+  const int i = bb.x * 64 + threadIdx.x;
+  const int j = 1 + bb.y * 4 + threadIdx.y;
+  const int k = 1 + bb.z;
   if (i > d.ni || j > d.nj - 1) return;
   const long c = cidx(d, i, j, k);
   double U = u[c], V = v[c], W = w[c], J = djac[c];
@@ -156,25 +158,51 @@ __global__ void k_psource(Dims d, const double* const* __restrict__ m,
 __global__ void k_psource_correct(Dims d, const double* const* __restrict__ m,
                                   const double* __restrict__ djac, const double* __restrict__ p,
                                   double sign, double dtime, double* s0, double* s1, double* s2,
-                                  double* u, double* v, double* w) {
-  INTERIOR_IJK(d);
-  double* out[3] = {s0, s1, s2};
-  double* f[3] = {u, v, w};
-  const long sj = d.sj, sk = d.sk;
-  const double J = djac[c];
+                                  double* u, double* v, double* w, double* cflpart) {
+  __shared__ double sh[256];
+  const uint3 bb = xcd_block();
+  const int i = 1 + bb.x * 64 + threadIdx.x;
+  const int j = 1 + bb.y * 4 + threadIdx.y;
+  const int k = 1 + bb.z;
+  double lcfl = 0.0;
+  if (i <= d.ni - 1 && j <= d.nj - 1) {
+    const long c = cidx(d, i, j, k);
+    double* out[3] = {s0, s1, s2};
+    double* f[3] = {u, v, w};
+    double nv[3];
+    const long sj = d.sj, sk = d.sk;
+    const double J = djac[c];
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const double *a = m[3 * q], *b = m[3 * q + 1], *g = m[3 * q + 2];
-    double t = djac[c + 1] * a[c + 1] * p[c + 1];
-    t = t - djac[c - 1] * a[c - 1] * p[c - 1];
-    t = t + djac[c + sj] * b[c + sj] * p[c + sj];
-    t = t - djac[c - sj] * b[c - sj] * p[c - sj];
-    t = t + djac[c + sk] * g[c + sk] * p[c + sk];
-    t = t - djac[c - sk] * g[c - sk] * p[c - sk];
-    const double sv = -(0.5 * t);
-    out[q][c] = sv;
-    f[q][c] = sign < 0 ? f[q][c] - dtime * sv / J : f[q][c] + dtime * sv / J;
+    for (int q = 0; q < 3; ++q) {
+      const double *a = m[3 * q], *b = m[3 * q + 1], *g = m[3 * q + 2];
+      double t = djac[c + 1] * a[c + 1] * p[c + 1];
+      t = t - djac[c - 1] * a[c - 1] * p[c - 1];
+      t = t + djac[c + sj] * b[c + sj] * p[c + sj];
+      t = t - djac[c - sj] * b[c - sj] * p[c - sj];
+      t = t + djac[c + sk] * g[c + sk] * p[c + sk];
+      t = t - djac[c - sk] * g[c - sk] * p[c - sk];
+      const double sv = -(0.5 * t);
+      out[q][c] = sv;
+      nv[q] = sign < 0 ? f[q][c] - dtime * sv / J : f[q][c] + dtime * sv / J;
+      f[q][c] = nv[q];
+    }
+    if (cflpart) {  // CALCCOEF's local CFL number of the projected velocity (k_cfl's formula)
+      const double U = nv[0], V = nv[1], W = nv[2];
+      const double ca = fabs(m[0][c] * U + m[3][c] * V + m[6][c] * W);
+      const double cb = fabs(m[1][c] * U + m[4][c] * V + m[7][c] * W);
+      const double ce = fabs(m[2][c] * U + m[5][c] * V + m[8][c] * W);
+      lcfl = dtime * (ca + cb + ce);
+    }
   }
+  if (!cflpart) return;  // uniform over the workgroup
+  const int t = threadIdx.x + 64 * threadIdx.y;
+  sh[t] = lcfl;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  if (t == 0) cflpart[bb.x + gridDim.x * (bb.y + (long)gridDim.y * bb.z)] = sh[0];
 }
 
 // predictor correction U = U - DTIME S / J^-1 fused with CONTRA_VEL on the interior cells
@@ -402,19 +430,36 @@ __global__ void k_forces_final(Dims d, const double* __restrict__ p,
   const double ply = block_reduce_sum(acc[1], sh);
   const double td = block_reduce_sum(acc[2], sh);
   const double tly = block_reduce_sum(acc[3], sh);
-  double cmax = 0.0;
-  for (int q = threadIdx.x; q < ncfl; q += blockDim.x) cmax = fmax(cmax, cflpart[q]);
+  double cm[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // eight loads in flight per lane
+  int q = threadIdx.x;
+  for (; q + 7 * (int)blockDim.x < ncfl; q += 8 * blockDim.x) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cm[u] = fmax(cm[u], cflpart[q + u * blockDim.x]);
+  }
+  for (; q < ncfl; q += blockDim.x) cm[0] = fmax(cm[0], cflpart[q]);
+  double cmax = cm[0];
+#pragma unroll
+  for (int u = 1; u < 8; ++u) cmax = fmax(cmax, cm[u]);
   cmax = block_reduce_max(cmax, sh);
-  if (threadIdx.x == 0) {
-    // base pressure: 8-cell average at the first wall node line, span mean (calccoef.f90:39-74)
-    double dumb = 0.0;
-    for (int k = 1; k <= d.nk - 1; ++k) {
-      long c = cidx(d, 1, 1, k);
-      const long sj = d.sj, sk = d.sk;
-      double pg = 0.125 * (p[c] + p[c - sj] + p[c - 1] + p[c - 1 - sj] + p[c + sk] +
-                           p[c - sj + sk] + p[c - 1 + sk] + p[c - 1 - sj + sk]);
-      dumb = dumb + pg;
+  // base pressure: 8-cell average at the first wall node line, span mean (calccoef.f90:39-74);
+  // the node averages are formed by one lane per K, the span sum in K order by lane 0
+  double dumb = 0.0;
+  const long sj = d.sj, sk = d.sk;
+  for (int k0 = 1; k0 <= d.nk - 1; k0 += blockDim.x) {
+    const int k = k0 + (int)threadIdx.x;
+    if (k <= d.nk - 1) {
+      const long c = cidx(d, 1, 1, k);
+      sh[threadIdx.x] = 0.125 * (p[c] + p[c - sj] + p[c - 1] + p[c - 1 - sj] + p[c + sk] +
+                                 p[c - sj + sk] + p[c - 1 + sk] + p[c - 1 - sj + sk]);
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int cnt = min((int)blockDim.x, d.nk - k0);
+      for (int q = 0; q < cnt; ++q) dumb = dumb + sh[q];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
     dumb = dumb / (double)(d.nk - 1);
     double cdp = pd / ref_a, clp = ply / ref_a, cdv = td / ref_a, clv = tly / ref_a;
     out[0] = cmax;
@@ -558,19 +603,23 @@ int fvm_k_pref(FvmCtx* c, double* pref_host) {
   return FVM_OK;
 }
 
-int fvm_k_calccoef(FvmCtx* c, double* out8) {
+// cfl_ready: the CFL partial maxima were left in c->cflpart by the projection of this step
+int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready) {
   double** m = c->dm;
   ProfScope ps(c, "calccoef");
-  const int ncfl = 512;
-  hipLaunchKernelGGL(k_cfl, dim3(ncfl), dim3(256), 0, c->stream, c->d, (const double* const*)m,
-                     c->u, c->v, c->w, c->prm.dtime, c->red);
+  const int ncfl = cfl_ready ? c->ncflpart : 512;
+  const double* cfl = cfl_ready ? c->cflpart : c->red;
+  const int off = cfl_ready ? 0 : ncfl;
+  if (!cfl_ready)
+    hipLaunchKernelGGL(k_cfl, dim3(ncfl), dim3(256), 0, c->stream, c->d, (const double* const*)m,
+                       c->u, c->v, c->w, c->prm.dtime, c->red);
   const long nw2 = 2L * (c->d.nj - 1) * (c->d.nk - 1);
   const int nfb = (int)((nw2 + 255) / 256);
-  if (ncfl + 4L * nfb > c->nred) return FVM_ERR_SIZE;
+  if (off + 4L * nfb > c->nred) return FVM_ERR_SIZE;
   hipLaunchKernelGGL(k_forces_part, dim3(nfb), dim3(256), 0, c->stream, c->d, c->wall, c->p,
-                     c->u, c->v, c->w, c->prm.re, c->red + ncfl);
-  hipLaunchKernelGGL(k_forces_final, dim3(1), dim3(256), 0, c->stream, c->d, c->p, c->red + ncfl,
-                     nfb, c->prm.ref_a, c->red, ncfl, c->scal + 8);
+                     c->u, c->v, c->w, c->prm.re, c->red + off);
+  hipLaunchKernelGGL(k_forces_final, dim3(1), dim3(256), 0, c->stream, c->d, c->p, c->red + off,
+                     nfb, c->prm.ref_a, cfl, ncfl, c->scal + 8);
   HIP_CHECK(hipGetLastError());
   if (out8) {
     HIP_CHECK(hipMemcpyAsync(out8, c->scal + 8, 8 * sizeof(double), hipMemcpyDeviceToHost,
@@ -580,14 +629,16 @@ int fvm_k_calccoef(FvmCtx* c, double* out8) {
   return FVM_OK;
 }
 
-int fvm_k_calccoef2d(FvmCtx* c, double* out8) {
+int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready) {
   double** m = c->dm;
   ProfScope ps(c, "calccoef");
-  const int ncfl = 512;
-  hipLaunchKernelGGL(k_cfl, dim3(ncfl), dim3(256), 0, c->stream, c->d, (const double* const*)m,
-                     c->u, c->v, c->w, c->prm.dtime, c->red);
+  const int ncfl = cfl_ready ? c->ncflpart : 512;
+  const double* cfl = cfl_ready ? c->cflpart : c->red;
+  if (!cfl_ready)
+    hipLaunchKernelGGL(k_cfl, dim3(ncfl), dim3(256), 0, c->stream, c->d, (const double* const*)m,
+                       c->u, c->v, c->w, c->prm.dtime, c->red);
   hipLaunchKernelGGL(k_calccoef2d, dim3(1), dim3(64), 0, c->stream, c->d, c->wall2d, c->p, c->u,
-                     c->v, c->prm.re, 0.0, c->red, ncfl, c->scal + 8);
+                     c->v, c->prm.re, 0.0, cfl, ncfl, c->scal + 8);
   HIP_CHECK(hipGetLastError());
   if (out8) {
     HIP_CHECK(hipMemcpyAsync(out8, c->scal + 8, 8 * sizeof(double), hipMemcpyDeviceToHost,
@@ -647,11 +698,16 @@ int fvm_wall2d_setup(FvmCtx* c, const double* xg, const double* yg) {
   return FVM_OK;
 }
 
-int fvm_k_psource_correct(FvmCtx* c, double sign) {
+// with_cfl: also leave the per-tile CFL maxima of the projected velocity in c->cflpart for the
+// CALCCOEF that ends the step (the pressure reference and BCU that come between change neither
+// the interior velocity nor the metrics)
+int fvm_k_psource_correct(FvmCtx* c, double sign, bool with_cfl) {
   ProfScope ps(c, "psource_correct");
-  hipLaunchKernelGGL(k_psource_correct, interior_grid(c->d), kBlk, 0, c->stream, c->d,
+  const dim3 g = interior_grid(c->d);
+  if (with_cfl && (long)g.x * g.y * g.z > c->ncflpart) return FVM_ERR_SIZE;
+  hipLaunchKernelGGL(k_psource_correct, g, kBlk, 0, c->stream, c->d,
                      (const double* const*)c->dm, c->djac, c->p, sign, c->prm.dtime, c->s[0],
-                     c->s[1], c->s[2], c->u, c->v, c->w);
+                     c->s[1], c->s[2], c->u, c->v, c->w, with_cfl ? c->cflpart : nullptr);
   HIP_CHECK(hipGetLastError());
   return FVM_OK;
 }

@@ -23,9 +23,10 @@ const dim3 kBlk(64, 4, 1);
 // div / dt from the face volume fluxes (iter_p.f90:31-34)
 __global__ void k_p_rhs(Dims d, const double* __restrict__ ju0, const double* __restrict__ ju1,
                         const double* __restrict__ ju2, double dtime, double* rhs) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const uint3 bb = xcd_block();
+  const int i = 1 + bb.x * 64 + threadIdx.x;
+  const int j = 1 + bb.y * 4 + threadIdx.y;
+  const int k = 1 + bb.z;
   if (i > d.ni - 1 || j > d.nj - 1) return;
   const long c = cidx(d, i, j, k);
   double ue = 0.5 * (ju0[c] + ju0[c + 1]);
@@ -84,10 +85,11 @@ __global__ void k_p_su(Dims d, const double* __restrict__ p, const double* __res
                        const double* __restrict__ pdK, const double* __restrict__ jq12,
                        const double* __restrict__ jq13, const double* __restrict__ jq23,
                        double* psu, int want_res, double* part) {
+  const uint3 bb = xcd_block();
   __shared__ double s0[256], s1[256];
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const int i = 1 + bb.x * 64 + threadIdx.x;
+  const int j = 1 + bb.y * 4 + threadIdx.y;
+  const int k = 1 + bb.z;
   const int t = threadIdx.x + 64 * threadIdx.y;
   double ra = 0.0, rb = 0.0;
   if (i <= d.ni - 1 && j <= d.nj - 1) {
@@ -110,7 +112,7 @@ __global__ void k_p_su(Dims d, const double* __restrict__ p, const double* __res
     __syncthreads();
   }
   if (t == 0) {
-    long bid = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    long bid = bb.x + (long)gridDim.x * (bb.y + (long)gridDim.y * bb.z);
     part[2 * bid] = s0[0];
     part[2 * bid + 1] = s1[0];
   }
@@ -179,9 +181,10 @@ __global__ void __launch_bounds__(64) k_p_sweep(Dims d, T* p, const T* __restric
 __global__ void k_p_defect32(Dims d, const double* __restrict__ p, const double* __restrict__ psu,
                              const double* __restrict__ pdI, const double* __restrict__ pdJ,
                              const double* __restrict__ pdK, float* r32, float* e32) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const uint3 bb = xcd_block();
+  const int i = 1 + bb.x * 64 + threadIdx.x;
+  const int j = 1 + bb.y * 4 + threadIdx.y;
+  const int k = 1 + bb.z;
   if (i > d.ni - 1 || j > d.nj - 1) return;
   const long c = cidx(d, i, j, k);
   PCoef q = pcoef(d, i, c, pdI, pdJ, pdK);
@@ -196,18 +199,20 @@ __global__ void k_p_defect32(Dims d, const double* __restrict__ p, const double*
 __global__ void k_e_su(Dims d, const float* __restrict__ e, const float* __restrict__ r32,
                        const double* __restrict__ jq12, const double* __restrict__ jq13,
                        const double* __restrict__ jq23, float* esu) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const uint3 bb = xcd_block();
+  const int i = 1 + bb.x * 64 + threadIdx.x;
+  const int j = 1 + bb.y * 4 + threadIdx.y;
+  const int k = 1 + bb.z;
   if (i > d.ni - 1 || j > d.nj - 1) return;
   const long c = cidx(d, i, j, k);
   esu[c] = (float)((double)r32[c] - cross_p(d, c, e, jq12, jq13, jq23));
 }
 
 __global__ void k_p_add(Dims d, double* p, const float* __restrict__ e) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const uint3 bb = xcd_block();
+  const int i = 1 + bb.x * 64 + threadIdx.x;
+  const int j = 1 + bb.y * 4 + threadIdx.y;
+  const int k = 1 + bb.z;
   if (i > d.ni - 1 || j > d.nj - 1) return;
   const long c = cidx(d, i, j, k);
   p[c] = p[c] + (double)e[c];

@@ -57,10 +57,11 @@ __global__ void k_residual_compact(Dims d, const double* __restrict__ p,
                                    const double* __restrict__ pdK, const double* __restrict__ jq12,
                                    const double* __restrict__ jq13, const double* __restrict__ jq23,
                                    T* __restrict__ R, double* part) {
+  const uint3 bb = xcd_block();
   __shared__ double s0[256], s1[256];
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const int i = 1 + bb.x * 64 + threadIdx.x;
+  const int j = 1 + bb.y * 4 + threadIdx.y;
+  const int k = 1 + bb.z;
   const int t = threadIdx.x + 64 * threadIdx.y;
   double ra = 0.0, rb = 0.0;
   if (i <= d.ni - 1 && j <= d.nj - 1) {
@@ -86,7 +87,7 @@ __global__ void k_residual_compact(Dims d, const double* __restrict__ p,
     __syncthreads();
   }
   if (t == 0) {
-    long bid = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    long bid = bb.x + (long)gridDim.x * (bb.y + (long)gridDim.y * bb.z);
     part[2 * bid] = s0[0];
     part[2 * bid + 1] = s1[0];
   }
@@ -96,7 +97,19 @@ __global__ void k_res_final(const double* __restrict__ part, long nb, double* ou
   __shared__ double s0[256], s1[256];
   const int t = threadIdx.x;
   double a = 0.0, b = 0.0;
-  for (long q = t; q < nb; q += blockDim.x) { a += part[2 * q]; b += part[2 * q + 1]; }
+  long q = t;
+  // eight partial pairs in flight per lane, added in the same order as one at a time
+  for (; q + 7L * blockDim.x < nb; q += 8L * blockDim.x) {
+    double x[8], y[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      x[u] = part[2 * (q + (long)u * blockDim.x)];
+      y[u] = part[2 * (q + (long)u * blockDim.x) + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) { a += x[u]; b += y[u]; }
+  }
+  for (; q < nb; q += blockDim.x) { a += part[2 * q]; b += part[2 * q + 1]; }
   s0[t] = a; s1[t] = b;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -213,9 +226,10 @@ void mode_lines(int n1, int nlines, const double* dW, const double* invd, const 
 
 template <typename T>
 __global__ void k_add_compact(Dims d, double* p, const T* __restrict__ X) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const uint3 bb = xcd_block();
+  const int i = 1 + bb.x * 64 + threadIdx.x;
+  const int j = 1 + bb.y * 4 + threadIdx.y;
+  const int k = 1 + bb.z;
   if (i > d.ni - 1 || j > d.nj - 1) return;
   const long c = cidx(d, i, j, k);
   const long ci = (long)(i - 1) + (long)(d.ni - 1) * ((j - 1) + (long)(d.nj - 1) * (k - 1));
@@ -501,15 +515,19 @@ int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
       hipLaunchKernelGGL(k_res_final, dim3(1), dim3(256), 0, c->stream, c->red, nb, c->scal + 4);
       HIP_CHECK(hipGetLastError());
     }
-    HIP_CHECK(hipMemcpyAsync(c->hscal + 4, c->scal + 4, 3 * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(hipStreamSynchronize(c->stream));
-    res = c->hscal[6];
-    if (!(res == res)) {
-      fprintf(stderr, "fvm_amd: non-finite pressure residual (outer pass %d)\n", outer);
-      return FVM_ERR_NAN;
+    // Like ITER_P's sweep loop (iter_p.f90:36-62), at least one correction pass is made before
+    // the residual is tested, so the first residual needs no round trip to the host.
+    if (outer > 0 || cap < 1) {
+      HIP_CHECK(hipMemcpyAsync(c->hscal + 4, c->scal + 4, 3 * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+      res = c->hscal[6];
+      if (!(res == res)) {
+        fprintf(stderr, "fvm_amd: non-finite pressure residual (outer pass %d)\n", outer);
+        return FVM_ERR_NAN;
+      }
+      if (res <= c->prm.resimax || outer >= cap) break;
     }
-    if (res <= c->prm.resimax || outer >= cap) break;
     if (use32 && outer > 0 && !(res < 0.5 * res_prev)) {
       // the FP32 approximate inverse stopped contracting: finish with the FP64 inverse
       use32 = false;

@@ -14,91 +14,144 @@ namespace {
 
 const dim3 kBlk(64, 4, 1);
 
-__global__ void k_strain(Dims d, const double* const* __restrict__ m, const double* __restrict__ U,
-                         const double* __restrict__ V, const double* __restrict__ W,
-                         double* const* __restrict__ L, double* mag) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (i > d.ni || j > d.nj - 1) return;
-  const long c = cidx(d, i, j, k);
-  const long sj = d.sj, sk = d.sk;
+// Strain S_ij and |S| of one cell (stress_tensor_dy.f90:27-76 central, :96-197 one-sided at
+// I = 0 / NI); the differences are passed in, so the LDS-staged and the global-memory callers
+// share one expression order.
+struct Diff3 {
+  double i, j, k;
+};
+
+__device__ inline void strain_store(const double* const* __restrict__ m, long c, bool onesided,
+                                    Diff3 u, Diff3 v, Diff3 w, double* const* __restrict__ L,
+                                    double* mag) {
   const double xix = m[0][c], etx = m[1][c], ztx = m[2][c];
   const double xiy = m[3][c], ety = m[4][c], zty = m[5][c];
   const double xiz = m[6][c], etz = m[7][c], ztz = m[8][c];
-#define DJ(F) (F[c + sj] - F[c - sj])
-#define DK(F) (F[c + sk] - F[c - sk])
   double l11, l12, l13, l22, l23, l33;
-  if (i > 0 && i < d.ni) {
-#define DI(F) (F[c + 1] - F[c - 1])
+  if (!onesided) {
     double t;
-    t = xix * DI(U) + etx * DJ(U) + ztx * DK(U);
+    t = xix * u.i + etx * u.j + ztx * u.k;
     l11 = 0.5 * t;
-    t = xiy * DI(U) + ety * DJ(U) + zty * DK(U) + xix * DI(V) + etx * DJ(V) + ztx * DK(V);
+    t = xiy * u.i + ety * u.j + zty * u.k + xix * v.i + etx * v.j + ztx * v.k;
     l12 = 0.25 * t;
-    t = xiz * DI(U) + etz * DJ(U) + ztz * DK(U) + xix * DI(W) + etx * DJ(W) + ztx * DK(W);
+    t = xiz * u.i + etz * u.j + ztz * u.k + xix * w.i + etx * w.j + ztx * w.k;
     l13 = 0.25 * t;
-    t = xiy * DI(V) + ety * DJ(V) + zty * DK(V);
+    t = xiy * v.i + ety * v.j + zty * v.k;
     l22 = 0.5 * t;
-    t = xiz * DI(V) + etz * DJ(V) + ztz * DK(V) + xiy * DI(W) + ety * DJ(W) + zty * DK(W);
+    t = xiz * v.i + etz * v.j + ztz * v.k + xiy * w.i + ety * w.j + zty * w.k;
     l23 = 0.25 * t;
-    t = xiz * DI(W) + etz * DJ(W) + ztz * DK(W);
+    t = xiz * w.i + etz * w.j + ztz * w.k;
     l33 = 0.5 * t;
-#undef DI
   } else {
-    // one-sided radial difference on the wall / far-field ghost planes
-    const bool lo = (i == 0);
-#define DI(F) (lo ? (F[c + 1] - F[c]) : (F[c] - F[c - 1]))
-    l11 = 0.5 * (2.0 * xix * DI(U) + etx * DJ(U) + ztx * DK(U));
-    l12 = 0.25 * (2.0 * xiy * DI(U) + ety * DJ(U) + zty * DK(U) + 2.0 * xix * DI(V) +
-                  etx * DJ(V) + ztx * DK(V));
-    l13 = 0.25 * (2.0 * xiz * DI(U) + etz * DJ(U) + ztz * DK(U) + 2.0 * xix * DI(W) +
-                  etx * DJ(W) + ztx * DK(W));
-    l22 = 0.5 * (2.0 * xiy * DI(V) + ety * DJ(V) + zty * DK(V));
-    l23 = 0.25 * (2.0 * xiz * DI(V) + etz * DJ(V) + ztz * DK(V) + 2.0 * xiy * DI(W) +
-                  ety * DJ(W) + zty * DK(W));
-    l33 = 0.5 * (2.0 * xiz * DI(W) + etz * DJ(W) + ztz * DK(W));
-#undef DI
+    l11 = 0.5 * (2.0 * xix * u.i + etx * u.j + ztx * u.k);
+    l12 = 0.25 * (2.0 * xiy * u.i + ety * u.j + zty * u.k + 2.0 * xix * v.i + etx * v.j +
+                  ztx * v.k);
+    l13 = 0.25 * (2.0 * xiz * u.i + etz * u.j + ztz * u.k + 2.0 * xix * w.i + etx * w.j +
+                  ztx * w.k);
+    l22 = 0.5 * (2.0 * xiy * v.i + ety * v.j + zty * v.k);
+    l23 = 0.25 * (2.0 * xiz * v.i + etz * v.j + ztz * v.k + 2.0 * xiy * w.i + ety * w.j +
+                  zty * w.k);
+    l33 = 0.5 * (2.0 * xiz * w.i + etz * w.j + ztz * w.k);
   }
-#undef DJ
-#undef DK
   L[0][c] = l11; L[1][c] = l12; L[2][c] = l13; L[3][c] = l22; L[4][c] = l23; L[5][c] = l33;
   double s = l11 * l11 + l12 * l12 + l13 * l13 + l12 * l12 + l22 * l22 + l23 * l23 +
              l13 * l13 + l23 * l23 + l33 * l33;
   mag[c] = sqrt(2.0 * s);
 }
 
-// (1,2,1)^3/64 index-space filter in the reference's term order (test_filter.f90:13-30):
-// K-1 plane, K plane, K+1 plane; rows J-1, J, J+1; columns I-1, I, I+1.
-template <typename F>
-__device__ inline double filter27(const Dims& d, long c, F f) {
-  double t = 0.0;
-  bool first = true;
-#pragma unroll
-  for (int dk = -1; dk <= 1; ++dk)
-#pragma unroll
-    for (int dj = -1; dj <= 1; ++dj)
-#pragma unroll
-      for (int di = -1; di <= 1; ++di) {
-        const int wgt = (di == 0 ? 2 : 1) * (dj == 0 ? 2 : 1) * (dk == 0 ? 2 : 1);
-        double v = f(c + di + d.sj * dj + d.sk * dk);
-        double term = wgt == 1 ? v : (double)wgt * v;
-        t = first ? term : t + term;
-        first = false;
-      }
-  return t / 64.0;
+// Strain on the I ghost planes i = 0 and i = NI (one-sided radial difference), interior j, k.
+__global__ void k_strain_iplanes(Dims d, const double* const* __restrict__ m,
+                                 const double* __restrict__ U, const double* __restrict__ V,
+                                 const double* __restrict__ W, double* const* __restrict__ L,
+                                 double* mag) {
+  const long nl = (long)(d.nj - 1) * (d.nk - 1);
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= 2 * nl) return;
+  const bool lo = t < nl;
+  const long r = lo ? t : t - nl;
+  const int j = 1 + (int)(r % (d.nj - 1)), k = 1 + (int)(r / (d.nj - 1));
+  const long c = cidx(d, lo ? 0 : d.ni, j, k);
+  const long sj = d.sj, sk = d.sk;
+  auto diff = [&](const double* F) {
+    Diff3 q;
+    q.i = lo ? (F[c + 1] - F[c]) : (F[c] - F[c - 1]);
+    q.j = F[c + sj] - F[c - sj];
+    q.k = F[c + sk] - F[c - sk];
+    return q;
+  };
+  strain_store(m, c, true, diff(U), diff(V), diff(W), L, mag);
 }
 
-__global__ void k_filter_vel(Dims d, const double* __restrict__ U, const double* __restrict__ V,
-                             const double* __restrict__ W, double* uh, double* vh, double* wh) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (i > d.ni - 1 || j > d.nj - 1) return;
-  const long c = cidx(d, i, j, k);
-  uh[c] = filter27(d, c, [&](long q) { return U[q]; });
-  vh[c] = filter27(d, c, [&](long q) { return V[q]; });
-  wh[c] = filter27(d, c, [&](long q) { return W[q]; });
+// Strain on the interior cells fused with the (1,2,1)^3/64 test filter of u, v, w
+// (test_filter.f90:13-30).  A workgroup owns a 64 x 4 (I x J) column of cells over SF_KC
+// consecutive K planes and marches up K with the last three planes of u, v, w (one-cell halo)
+// in LDS, so every velocity value is fetched from HBM once per column.  The filter sums its 27
+// terms in the reference's order: K-1 plane, K plane, K+1 plane; rows J-1, J, J+1; columns
+// I-1, I, I+1.
+constexpr int SF_KC = 8;
+constexpr int SF_W = 66, SF_H = 6, SF_P = SF_W * SF_H;
+
+__global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* const* __restrict__ m,
+                                                       const double* __restrict__ U,
+                                                       const double* __restrict__ V,
+                                                       const double* __restrict__ W,
+                                                       double* const* __restrict__ L, double* mag,
+                                                       double* uh, double* vh, double* wh) {
+  const uint3 bb = xcd_block();
+  __shared__ double sh[3][3][SF_P];  // [component][plane slot][tile cell]
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+  const int i = 1 + bb.x * 64 + tx;
+  const int j = 1 + bb.y * 4 + ty;
+  const int k0 = 1 + bb.z * SF_KC;
+  const int k1 = min(k0 + SF_KC - 1, d.nk - 1);
+  const bool act = i <= d.ni - 1 && j <= d.nj - 1;
+  const int q0 = (ty + 1) * SF_W + tx + 1;
+  double* out[3] = {uh, vh, wh};
+  for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
+    const int slot = (kp - (k0 - 1)) % 3;
+    __syncthreads();  // plane kp - 3, held in this slot, is no longer read
+    for (int q = tid; q < SF_P; q += 256) {
+      const int gi = min(bb.x * 64 + q % SF_W, d.ni);
+      const int gj = min(bb.y * 4 + q / SF_W, d.nj);
+      const long c = cidx(d, gi, gj, kp);
+      sh[0][slot][q] = U[c];
+      sh[1][slot][q] = V[c];
+      sh[2][slot][q] = W[c];
+    }
+    __syncthreads();
+    if (kp < k0 + 1 || !act) continue;
+    // cell (i, j, kp - 1): planes k-1, k, k+1 sit in slots slot+1, slot+2, slot (mod 3)
+    const int k = kp - 1;
+    const int sm = (slot + 1) % 3, s0 = (slot + 2) % 3, sp = slot;
+    const long c = cidx(d, i, j, k);
+    Diff3 g[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      g[f].i = sh[f][s0][q0 + 1] - sh[f][s0][q0 - 1];
+      g[f].j = sh[f][s0][q0 + SF_W] - sh[f][s0][q0 - SF_W];
+      g[f].k = sh[f][sp][q0] - sh[f][sm][q0];
+    }
+    strain_store(m, c, false, g[0], g[1], g[2], L, mag);
+    const int pl[3] = {sm, s0, sp};
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      double t = 0.0;
+      bool first = true;
+#pragma unroll
+      for (int dk = -1; dk <= 1; ++dk)
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+          for (int di = -1; di <= 1; ++di) {
+            const int wgt = (di == 0 ? 2 : 1) * (dj == 0 ? 2 : 1) * (dk == 0 ? 2 : 1);
+            const double v = sh[f][pl[dk + 1]][q0 + dj * SF_W + di];
+            const double term = wgt == 1 ? v : (double)wgt * v;
+            t = first ? term : t + term;
+            first = false;
+          }
+      out[f][c] = t / 64.0;
+    }
+  }
 }
 
 // Germano/Lilly contractions LM and MM per interior cell (stress_tensor_dy.f90:236-483).
@@ -122,11 +175,12 @@ __global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restr
                                             const double* __restrict__ mag,
                                             const double* __restrict__ filt2, double* lm_out,
                                             double* mm_out) {
+  const uint3 bb = xcd_block();
   __shared__ double sh[10][LM_W * LM_H];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
-  const int i = 1 + blockIdx.x * 64 + tx;
-  const int j = 1 + blockIdx.y * 4 + ty;
-  const int k0 = 1 + blockIdx.z * LM_KC;
+  const int i = 1 + bb.x * 64 + tx;
+  const int j = 1 + bb.y * 4 + ty;
+  const int k0 = 1 + bb.z * LM_KC;
   const int k1 = min(k0 + LM_KC - 1, d.nk - 1);
   const bool act = i <= d.ni - 1 && j <= d.nj - 1;
   const double* L0 = L[0]; const double* L1 = L[1]; const double* L2 = L[2];
@@ -139,8 +193,8 @@ __global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restr
   for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
     __syncthreads();
     for (int q = tid; q < LM_W * LM_H; q += 256) {
-      const int gi = min(blockIdx.x * 64 + q % LM_W, d.ni);
-      const int gj = min(blockIdx.y * 4 + q / LM_W, d.nj);
+      const int gi = min(bb.x * 64 + q % LM_W, d.ni);
+      const int gj = min(bb.y * 4 + q / LM_W, d.nj);
       const long c = cidx(d, gi, gj, kp);
       sh[0][q] = U[c]; sh[1][q] = V[c]; sh[2][q] = W[c]; sh[3][q] = mag[c];
       sh[4][q] = L0[c]; sh[5][q] = L1[c]; sh[6][q] = L2[c];
@@ -292,41 +346,88 @@ __global__ void k_nut(Dims d, const double* __restrict__ cavg, const double* __r
 }
 
 // divergence of the SGS stress -2 nu_t S_ij (stress_tensor_dy.f90:541-583), added to NL
-// with the inverse Jacobian (RLES/main.f90:61-66)
-__global__ void k_sgs_div(Dims d, const double* const* __restrict__ m,
-                          const double* __restrict__ nut, const double* const* __restrict__ L,
-                          const double* __restrict__ djac, double* nlx, double* nly, double* nlz,
-                          double* sx, double* sy, double* sz) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (i > d.ni - 1 || j > d.nj - 1) return;
-  const long c = cidx(d, i, j, k);
-  const long off[3] = {1, d.sj, d.sk};
-  // Lt[row][col]: symmetric strain components stored as 11,12,13,22,23,33
-  const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+// with the inverse Jacobian (RLES/main.f90:61-66).  A workgroup owns a 64 x 4 (I x J) column
+// of cells over SD_KC consecutive K planes and marches up K: the six products
+// P_q = (2 nu_t) S_q of each plane are formed once, with a one-cell halo, in one of two LDS
+// slots (the I and J differences of plane k and the own-column value of plane k+1), and the
+// own-column products of plane k-1 are kept in registers.  Differences, products and sums are
+// the reference's, in its order.
+constexpr int SD_KC = 8;
+
+__global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __restrict__ m,
+                                                 const double* __restrict__ nut,
+                                                 const double* const* __restrict__ L,
+                                                 const double* __restrict__ djac, double* nlx,
+                                                 double* nly, double* nlz, double* sx, double* sy,
+                                                 double* sz) {
+  const uint3 bb = xcd_block();
+  __shared__ double sh[2][6][SF_P];  // [plane slot][product][tile cell]
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+  const int i = 1 + bb.x * 64 + tx;
+  const int j = 1 + bb.y * 4 + ty;
+  const int k0 = 1 + bb.z * SD_KC;
+  const int k1 = min(k0 + SD_KC - 1, d.nk - 1);
+  const bool act = i <= d.ni - 1 && j <= d.nj - 1;
+  const int e0 = (ty + 1) * SF_W + tx + 1;
+  const double* S0 = L[0]; const double* S1 = L[1]; const double* S2 = L[2];
+  const double* S3 = L[3]; const double* S4 = L[4]; const double* S5 = L[5];
   double* nl[3] = {nlx, nly, nlz};
   double* so[3] = {sx, sy, sz};
-  const double J = djac[c];
+  // symmetric strain components stored as 11,12,13,22,23,33: row/col -> product index
+  const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  double Pm[6];  // own-column products of plane k-1
 #pragma unroll
-  for (int row = 0; row < 3; ++row) {
-    double t = 0.0;
-    bool first = true;
-#pragma unroll
-    for (int col = 0; col < 3; ++col) {
-      const double* S = L[idx[row][col]];
-#pragma unroll
-      for (int dir = 0; dir < 3; ++dir) {
-        const long of = off[dir];
-        double inner = -(2.0 * nut[c + of] * S[c + of]) + 2.0 * nut[c - of] * S[c - of];
-        double term = m[3 * col + dir][c] * inner;
-        t = first ? term : t + term;
-        first = false;
-      }
+  for (int q = 0; q < 6; ++q) Pm[q] = 0.0;
+  for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
+    const int slot = (kp - (k0 - 1)) & 1;
+    __syncthreads();  // plane kp - 2, held in this slot, is no longer read
+    for (int e = tid; e < SF_P; e += 256) {
+      const int gi = min(bb.x * 64 + e % SF_W, d.ni);
+      const int gj = min(bb.y * 4 + e / SF_W, d.nj);
+      const long c = cidx(d, gi, gj, kp);
+      const double tn = 2.0 * nut[c];
+      sh[slot][0][e] = tn * S0[c];
+      sh[slot][1][e] = tn * S1[c];
+      sh[slot][2][e] = tn * S2[c];
+      sh[slot][3][e] = tn * S3[c];
+      sh[slot][4][e] = tn * S4[c];
+      sh[slot][5][e] = tn * S5[c];
     }
-    double sg = 0.5 * t;
-    so[row][c] = sg;
-    nl[row][c] = nl[row][c] + sg * J;
+    __syncthreads();
+    if (kp == k0 - 1) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) Pm[q] = sh[slot][q][e0];
+      continue;
+    }
+    if (kp == k0 || !act) continue;
+    // cell (i, j, kp - 1): plane k in slot ^ 1, plane k + 1 in slot
+    const int sc = slot ^ 1;
+    const int k = kp - 1;
+    const long c = cidx(d, i, j, k);
+    const double J = djac[c];
+#pragma unroll
+    for (int row = 0; row < 3; ++row) {
+      double t = 0.0;
+#pragma unroll
+      for (int col = 0; col < 3; ++col) {
+        const int q = idx[row][col];
+#pragma unroll
+        for (int dir = 0; dir < 3; ++dir) {
+          double pp, pn;
+          if (dir == 0) { pp = sh[sc][q][e0 + 1]; pn = sh[sc][q][e0 - 1]; }
+          else if (dir == 1) { pp = sh[sc][q][e0 + SF_W]; pn = sh[sc][q][e0 - SF_W]; }
+          else { pp = sh[slot][q][e0]; pn = Pm[q]; }
+          const double inner = -pp + pn;
+          const double term = m[3 * col + dir][c] * inner;
+          t = (col == 0 && dir == 0) ? term : t + term;
+        }
+      }
+      const double sg = 0.5 * t;
+      so[row][c] = sg;
+      nl[row][c] = nl[row][c] + sg * J;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Pm[q] = sh[sc][q][e0];
   }
 }
 
@@ -338,22 +439,19 @@ int fvm_k_sgs(FvmCtx* c) {
   ProfScope ps(c, "sgs");
   hipStream_t st = c->stream;
   {
-    ProfScope pk(c, "k_strain");
-    dim3 g((d.ni + 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
-    hipLaunchKernelGGL(k_strain, g, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
-                       (double* const*)c->dlss, c->mag);
+    ProfScope pk(c, "k_strain_filter");
+    const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + SF_KC - 1) / SF_KC);
+    hipLaunchKernelGGL(k_strain_filter, g, kBlk, 0, st, d, (const double* const*)m, c->u, c->v,
+                       c->w, (double* const*)c->dlss, c->mag, c->uh[0], c->uh[1], c->uh[2]);
+    const long nb = 2L * (d.nj - 1) * (d.nk - 1);
+    hipLaunchKernelGGL(k_strain_iplanes, grid1(nb, 256), dim3(256), 0, st, d,
+                       (const double* const*)m, c->u, c->v, c->w, (double* const*)c->dlss,
+                       c->mag);
     HIP_CHECK(hipGetLastError());
   }
   double* gl[7] = {c->lss[0], c->lss[1], c->lss[2], c->lss[3], c->lss[4], c->lss[5], c->mag};
   int r = fvm_ghost_fill_n(c, gl, 7);
   if (r) return r;
-  dim3 gi((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
-  {
-    ProfScope pk(c, "k_filter_vel");
-    hipLaunchKernelGGL(k_filter_vel, gi, kBlk, 0, st, d, c->u, c->v, c->w, c->uh[0], c->uh[1],
-                       c->uh[2]);
-    HIP_CHECK(hipGetLastError());
-  }
   r = fvm_k_bcu_ghosts(c, c->uh, 3);
   if (r) return r;
   {
@@ -371,7 +469,8 @@ int fvm_k_sgs(FvmCtx* c) {
                      c->nut);
   {
     ProfScope pk(c, "k_sgs_div");
-    hipLaunchKernelGGL(k_sgs_div, gi, kBlk, 0, st, d, (const double* const*)m, c->nut,
+    const dim3 gs((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + SD_KC - 1) / SD_KC);
+    hipLaunchKernelGGL(k_sgs_div, gs, kBlk, 0, st, d, (const double* const*)m, c->nut,
                        (const double* const*)c->dlss, c->djac, c->nl[0], c->nl[1], c->nl[2],
                        c->sgs[0], c->sgs[1], c->sgs[2]);
     HIP_CHECK(hipGetLastError());
