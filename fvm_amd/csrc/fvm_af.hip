@@ -105,6 +105,130 @@ __global__ void k_af_rhs(Dims d, AFArgs A, const double* const* __restrict__ m,
   }
 }
 
+// Metric products of the cross terms of stage s, once per cell including ghosts, in the
+// association of k_af_rhs: (J X) E, (J X) Z, (J E) X, (J E) Z, (J Z) X, (J Z) E with X, E, Z the
+// stage's xi_s, eta_s, zeta_s.
+__global__ void k_af_products(Dims d, const double* __restrict__ J, const double* __restrict__ X,
+                              const double* __restrict__ E, const double* __restrict__ Z,
+                              double* const* __restrict__ P) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < d.n;
+       q += (long)gridDim.x * blockDim.x) {
+    const double j = J[q], x = X[q], e = E[q], z = Z[q];
+    P[0][q] = j * x * e;
+    P[1][q] = j * x * z;
+    P[2][q] = j * e * x;
+    P[3][q] = j * e * z;
+    P[4][q] = j * z * x;
+    P[5][q] = j * z * e;
+  }
+}
+
+// k_af_rhs for stages 0 and 1, K-marching: a workgroup owns a 64 x 4 (I x J) column of cells
+// over RK_KC consecutive K planes and keeps the last three planes of the three stage inputs
+// (one-cell halo) in LDS, so the twelve edge neighbours of the cross terms come from LDS and
+// every input value is fetched from HBM once per column.  The metric products come from the
+// per-stage arrays of k_af_products (equal, bit for bit, to the products k_af_rhs forms); those
+// at K+-1 and AT at K-1 ride along in registers.  Expressions and their order are k_af_rhs's.
+constexpr int RK_KC = 8;
+
+template <int S>
+__global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
+                                                   const double* const* __restrict__ P,
+                                                   const double* __restrict__ ap0, double re,
+                                                   const double* const* __restrict__ fa,
+                                                   int qform) {
+  const uint3 bb = xcd_block();
+  __shared__ double sh[3][3][SF_P];  // [component][plane slot][tile cell]
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+  const int i = 1 + bb.x * 64 + tx;
+  const int j = 1 + bb.y * 4 + ty;
+  const int k0 = 1 + bb.z * RK_KC;
+  const int k1 = min(k0 + RK_KC - 1, d.nk - 1);
+  const bool act = i <= d.ni - 1 && j <= d.nj - 1;
+  const int q0 = (ty + 1) * SF_W + tx + 1;
+  const long sj = d.sj, sk = d.sk;
+  const double *PXE = P[0], *PXZ = P[1], *PEX = P[2], *PEZ = P[3], *PZX = P[4], *PZE = P[5];
+  const double *fE = fa[0], *fN = fa[1], *fT = fa[2];
+  // own-column values of planes kp-2, kp-1, kp: (J Z) X, (J Z) E and AT
+  double zx[3] = {0.0, 0.0, 0.0}, ze[3] = {0.0, 0.0, 0.0}, at[3] = {0.0, 0.0, 0.0};
+  for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
+    const int slot = (kp - (k0 - 1)) % 3;
+    __syncthreads();  // plane kp - 3, held in this slot, is no longer read
+    for (int q = tid; q < SF_P; q += 256) {
+      const int gi = min(bb.x * 64 + q % SF_W, d.ni);
+      const int gj = min(bb.y * 4 + q / SF_W, d.nj);
+      const long c = cidx(d, gi, gj, kp);
+      sh[0][slot][q] = A.f[0][c];
+      sh[1][slot][q] = A.f[1][c];
+      sh[2][slot][q] = A.f[2][c];
+    }
+    zx[0] = zx[1]; zx[1] = zx[2];
+    ze[0] = ze[1]; ze[1] = ze[2];
+    at[0] = at[1]; at[1] = at[2];
+    if (act) {
+      const long cp = cidx(d, i, j, kp);
+      zx[2] = PZX[cp];
+      ze[2] = PZE[cp];
+      at[2] = (S < 2 && qform) ? fT[cp] : 0.0;
+    }
+    __syncthreads();
+    if (kp < k0 + 1 || !act) continue;
+    // cell (i, j, kp - 1): planes k-1, k, k+1 sit in slots slot+1, slot+2, slot (mod 3)
+    const int k = kp - 1;
+    const int pl[3] = {(slot + 1) % 3, (slot + 2) % 3, slot};
+    const long c = cidx(d, i, j, k);
+    const double pe_xe = PXE[c + 1], pw_xe = PXE[c - 1];
+    const double pe_xz = PXZ[c + 1], pw_xz = PXZ[c - 1];
+    const double pn_ex = PEX[c + sj], ps_ex = PEX[c - sj];
+    const double pn_ez = PEZ[c + sj], ps_ez = PEZ[c - sj];
+    const double pt_zx = zx[2], pb_zx = zx[0];
+    const double pt_ze = ze[2], pb_ze = ze[0];
+    const double a0 = ap0[c];
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+#define F(dk, dj, di) sh[comp][pl[(dk) + 1]][q0 + (dj) * SF_W + (di)]
+      double r = pe_xe * (F(0, 1, 1) - F(0, -1, 1));
+      r = r - pw_xe * (F(0, 1, -1) - F(0, -1, -1));
+      r = r + pe_xz * (F(1, 0, 1) - F(-1, 0, 1));
+      r = r - pw_xz * (F(1, 0, -1) - F(-1, 0, -1));
+      r = r + pn_ex * (F(0, 1, 1) - F(0, 1, -1));
+      r = r - ps_ex * (F(0, -1, 1) - F(0, -1, -1));
+      r = r + pn_ez * (F(1, 1, 0) - F(-1, 1, 0));
+      r = r - ps_ez * (F(1, -1, 0) - F(-1, -1, 0));
+      r = r + pt_zx * (F(1, 0, 1) - F(1, 0, -1));
+      r = r - pb_zx * (F(-1, 0, 1) - F(-1, 0, -1));
+      r = r + pt_ze * (F(1, 1, 0) - F(1, -1, 0));
+      r = r - pb_ze * (F(-1, 1, 0) - F(-1, -1, 0));
+      const double dd = 0.25 * r;
+      double su;
+      if (S == 0) {
+        su = a0 * A.fn[comp][c];
+        su = su - 3.0 * re * A.nl[comp][c];
+        su = su + 1.0 * re * A.nl1[comp][c];
+        su = su + A.dif[comp][c];
+        su = su + 2.0 * re * A.s[comp][c];
+        su = su + dd;
+      } else {
+        su = a0 * F(0, 0, 0) + dd;
+      }
+      if (qform) {
+        double ae = fE[c];
+        if (i == 1) su = su + 2.0 * fE[c - 1] * 0.0;
+        if (i == d.ni - 1) {
+          const long nf = (long)(d.nj + 1) * (d.nk + 1);
+          su = su + 2.0 * ae * A.ufar[comp * nf + j + (long)(d.nj + 1) * k];
+          ae = 0.0;
+        }
+        su = S == 0 ? su + fN[c] * F(0, 1, 0) : su + ae * F(0, 0, 1);
+        su = su + at[1] * F(1, 0, 0);
+        su = su + at[0] * F(-1, 0, 0);
+      }
+#undef F
+      A.su[comp][c] = su;
+    }
+  }
+}
+
 // Stage-s line sweep over the lines of anti-diagonal h of the line plane.
 // Line plane: stage 0 lines (j,k), inner a=j, outer b=k; stage 1 lines (i,k), inner a=i,
 // outer b=k; stage 2 lines (i,j), inner a=i, outer b=j.
@@ -573,8 +697,9 @@ int fvm_k_iter_u(FvmCtx* c) {
       const int qf = (s < 2 && c->prm.af_mode == 0 && c->af_scan[s]) ? 1 : 0;
       const double* const* fas = (const double* const*)c->dfa[s];
       const int dz = c->dd2_zero ? 1 : 0;
-      if (s == 0) hipLaunchKernelGGL(k_af_rhs<0>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf, 0);
-      if (s == 1) hipLaunchKernelGGL(k_af_rhs<1>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf, 0);
+      const dim3 gk((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + RK_KC - 1) / RK_KC);
+      if (s == 0) hipLaunchKernelGGL(k_af_rhs_km<0>, gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[0], c->ap0, c->prm.re, fas, qf);
+      if (s == 1) hipLaunchKernelGGL(k_af_rhs_km<1>, gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[1], c->ap0, c->prm.re, fas, qf);
       if (s == 2) hipLaunchKernelGGL(k_af_rhs<2>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf, dz);
       HIP_CHECK(hipGetLastError());
     }
@@ -632,6 +757,21 @@ int fvm_af_setup(FvmCtx* c) {
   const Dims d = c->d;
   hipStream_t st = c->stream;
   const int nb = 256;
+  if (!c->afp[0][0]) {  // metric products of the stage 0 and 1 cross terms (geometry only)
+    for (int s = 0; s < 2; ++s) {
+      for (int q = 0; q < 6; ++q) {
+        HIP_CHECK(hipMalloc((void**)&c->afp[s][q], d.n * sizeof(double)));
+        c->allocs.push_back(c->afp[s][q]);
+      }
+      HIP_CHECK(hipMalloc((void**)&c->dafp[s], 6 * sizeof(double*)));
+      c->allocs.push_back(c->dafp[s]);
+      HIP_CHECK(hipMemcpy(c->dafp[s], c->afp[s], 6 * sizeof(double*), hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_af_products, grid1(d.n, 256), dim3(256), 0, st, d, c->djac,
+                         c->m[3 * s], c->m[3 * s + 1], c->m[3 * s + 2],
+                         (double* const*)c->dafp[s]);
+      HIP_CHECK(hipGetLastError());
+    }
+  }
   std::vector<double> h(3 * nb);
   for (int s = 0; s < 3; ++s) {
     if (s == 0) hipLaunchKernelGGL(k_af_couplings<0>, dim3(nb), dim3(256), 0, st, d, (const double* const*)c->dfa[0], c->ap0, c->red);

@@ -39,6 +39,10 @@ __device__ inline uint3 xcd_block() {
   return make_uint3(w % gx, (w / gx) % gy, w / (gx * gy));
 }
 
+// A 64 x 4 (I x J) tile of the K-marching stencil kernels with its one-cell halo, as staged in
+// LDS: SF_W x SF_H cells, row-major.
+constexpr int SF_W = 66, SF_H = 6, SF_P = SF_W * SF_H;
+
 // Maximum line length handled by one 64-lane wavefront (8 rows per lane).
 #define FVM_MAX_LINE 512
 

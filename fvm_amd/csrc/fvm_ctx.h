@@ -77,6 +77,8 @@ struct FvmCtx {
   double* red = nullptr;     // partial sums
   double** scantab[2] = {nullptr, nullptr};  // device pointer tables of the I/J scan stages
   double* scantab_h[2][9] = {};               // their last uploaded contents
+  double* afp[2][6] = {};                     // metric products of the stage 0/1 cross terms
+  double** dafp[2] = {nullptr, nullptr};      // their device pointer tables
   double* cflpart = nullptr; // per-tile CFL maxima of the projection kernel (one per 64x4 tile)
   int ncflpart = 0;
   double* scal = nullptr;    // device scalars (c_avg, residual sums, pref, ...)

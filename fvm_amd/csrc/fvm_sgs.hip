@@ -89,7 +89,6 @@ __global__ void k_strain_iplanes(Dims d, const double* const* __restrict__ m,
 // terms in the reference's order: K-1 plane, K plane, K+1 plane; rows J-1, J, J+1; columns
 // I-1, I, I+1.
 constexpr int SF_KC = 8;
-constexpr int SF_W = 66, SF_H = 6, SF_P = SF_W * SF_H;
 
 __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* const* __restrict__ m,
                                                        const double* __restrict__ U,
