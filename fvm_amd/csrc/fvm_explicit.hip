@@ -84,125 +84,174 @@ __global__ void k_nl(Dims d, const double* __restrict__ ju0, const double* __res
   }
 }
 
-// explicit cross-derivative (non-orthogonal) diffusion terms DIF_DIAGONAL
-// (RLES/dif_diagonal.f90:13-32); jqAB = J^-1 Q_AB
-__device__ inline double cross_terms(const Dims& d, long c, const double* __restrict__ f,
-                                     const double* __restrict__ jq12,
-                                     const double* __restrict__ jq13,
-                                     const double* __restrict__ jq23) {
-  const long sj = d.sj, sk = d.sk;
-  const long e = c + 1, w = c - 1, n = c + sj, s = c - sj, t = c + sk, b = c - sk;
-  double r = jq12[e] * (f[e + sj] - f[e - sj]);
-  r = r - jq12[w] * (f[w + sj] - f[w - sj]);
-  r = r + jq13[e] * (f[e + sk] - f[e - sk]);
-  r = r - jq13[w] * (f[w + sk] - f[w - sk]);
-  r = r + jq12[n] * (f[n + 1] - f[n - 1]);
-  r = r - jq12[s] * (f[s + 1] - f[s - 1]);
-  r = r + jq23[n] * (f[n + sk] - f[n - sk]);
-  r = r - jq23[s] * (f[s + sk] - f[s - sk]);
-  r = r + jq13[t] * (f[t + 1] - f[t - 1]);
-  r = r - jq13[b] * (f[b + 1] - f[b - 1]);
-  r = r + jq23[t] * (f[t + sj] - f[t - sj]);
-  r = r - jq23[b] * (f[b + sj] - f[b - sj]);
-  return 0.25 * r;
-}
+// DIF(IN,OUT) for the three components (RLES/dif.f90:13-23): the orthogonal part
+// sum D_f (phi_nb - phi_P) plus the cross-derivative terms DIF_DIAGONAL
+// (RLES/dif_diagonal.f90:13-32, jqAB = J^-1 Q_AB), in the reference's order.  A workgroup owns a
+// 64 x 4 (I x J) column of cells over DF_KC consecutive planes and marches up K with the last
+// three planes of u_n, v_n, w_n (one-cell halo) in LDS, so the seven-point and twelve-edge
+// stencils are read from HBM once per column; J^-1 Q13, J^-1 Q23 and D_T of the planes K+-1
+// ride along in registers.
+constexpr int DF_KC = 8;
 
-// DIF(IN,OUT) for the three components (RLES/dif.f90:13-23)
-__global__ void k_dif(Dims d, const double* __restrict__ pdI, const double* __restrict__ pdJ,
-                      const double* __restrict__ pdK, const double* __restrict__ jq12,
-                      const double* __restrict__ jq13, const double* __restrict__ jq23,
-                      const double* __restrict__ un, const double* __restrict__ vn,
-                      const double* __restrict__ wn, double* dx, double* dy, double* dz) {
-  INTERIOR_IJK(d);
-  const double de = pdI[c], dw = pdI[c - 1], dn = pdJ[c], ds = pdJ[c - d.sj], dt = pdK[c],
-               db = pdK[c - d.sk];
-  const double* in[3] = {un, vn, wn};
+__global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict__ pdI,
+                                                const double* __restrict__ pdJ,
+                                                const double* __restrict__ pdK,
+                                                const double* __restrict__ jq12,
+                                                const double* __restrict__ jq13,
+                                                const double* __restrict__ jq23,
+                                                const double* __restrict__ un,
+                                                const double* __restrict__ vn,
+                                                const double* __restrict__ wn, double* dx,
+                                                double* dy, double* dz) {
+  const uint3 bb = xcd_block();
+  __shared__ double sh[3][3][SF_P];  // [component][plane slot][tile cell]
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+  const int i = 1 + bb.x * 64 + tx;
+  const int j = 1 + bb.y * 4 + ty;
+  const int k0 = 1 + bb.z * DF_KC;
+  const int k1 = min(k0 + DF_KC - 1, d.nk - 1);
+  const bool act = i <= d.ni - 1 && j <= d.nj - 1;
+  const int q0 = (ty + 1) * SF_W + tx + 1;
+  const long sj = d.sj;
   double* out[3] = {dx, dy, dz};
+  double j13[3] = {0.0, 0.0, 0.0}, j23[3] = {0.0, 0.0, 0.0}, pk[3] = {0.0, 0.0, 0.0};
+  for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
+    const int slot = (kp - (k0 - 1)) % 3;
+    __syncthreads();  // plane kp - 3, held in this slot, is no longer read
+    for (int q = tid; q < SF_P; q += 256) {
+      const int gi = min(bb.x * 64 + q % SF_W, d.ni);
+      const int gj = min(bb.y * 4 + q / SF_W, d.nj);
+      const long c = cidx(d, gi, gj, kp);
+      sh[0][slot][q] = un[c];
+      sh[1][slot][q] = vn[c];
+      sh[2][slot][q] = wn[c];
+    }
+    j13[0] = j13[1]; j13[1] = j13[2];
+    j23[0] = j23[1]; j23[1] = j23[2];
+    pk[0] = pk[1]; pk[1] = pk[2];
+    if (act) {
+      const long cp = cidx(d, i, j, kp);
+      j13[2] = jq13[cp];
+      j23[2] = jq23[cp];
+      pk[2] = pdK[cp];
+    }
+    __syncthreads();
+    if (kp < k0 + 1 || !act) continue;
+    const int k = kp - 1;
+    const int pl[3] = {(slot + 1) % 3, (slot + 2) % 3, slot};
+    const long c = cidx(d, i, j, k);
+    const double de = pdI[c], dw = pdI[c - 1], dn = pdJ[c], ds = pdJ[c - sj], dt = pk[1],
+                 db = pk[0];
+    const double e12 = jq12[c + 1], w12 = jq12[c - 1], e13 = jq13[c + 1], w13 = jq13[c - 1];
+    const double n12 = jq12[c + sj], s12 = jq12[c - sj], n23 = jq23[c + sj], s23 = jq23[c - sj];
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const double* f = in[q];
-    double fc = f[c];
-    double t = de * (f[c + 1] - fc);
-    t = t + dw * (f[c - 1] - fc);
-    t = t + dn * (f[c + d.sj] - fc);
-    t = t + ds * (f[c - d.sj] - fc);
-    t = t + dt * (f[c + d.sk] - fc);
-    t = t + db * (f[c - d.sk] - fc);
-    t = t + cross_terms(d, c, f, jq12, jq13, jq23);
-    out[q][c] = 1.0 * t;
+    for (int comp = 0; comp < 3; ++comp) {
+#define F(dk, dj, di) sh[comp][pl[(dk) + 1]][q0 + (dj) * SF_W + (di)]
+      const double fc = F(0, 0, 0);
+      double t = de * (F(0, 0, 1) - fc);
+      t = t + dw * (F(0, 0, -1) - fc);
+      t = t + dn * (F(0, 1, 0) - fc);
+      t = t + ds * (F(0, -1, 0) - fc);
+      t = t + dt * (F(1, 0, 0) - fc);
+      t = t + db * (F(-1, 0, 0) - fc);
+      double r = e12 * (F(0, 1, 1) - F(0, -1, 1));
+      r = r - w12 * (F(0, 1, -1) - F(0, -1, -1));
+      r = r + e13 * (F(1, 0, 1) - F(-1, 0, 1));
+      r = r - w13 * (F(1, 0, -1) - F(-1, 0, -1));
+      r = r + n12 * (F(0, 1, 1) - F(0, 1, -1));
+      r = r - s12 * (F(0, -1, 1) - F(0, -1, -1));
+      r = r + n23 * (F(1, 1, 0) - F(-1, 1, 0));
+      r = r - s23 * (F(1, -1, 0) - F(-1, -1, 0));
+      r = r + j13[2] * (F(1, 0, 1) - F(1, 0, -1));
+      r = r - j13[0] * (F(-1, 0, 1) - F(-1, 0, -1));
+      r = r + j23[2] * (F(1, 1, 0) - F(1, -1, 0));
+      r = r - j23[0] * (F(-1, 1, 0) - F(-1, -1, 0));
+#undef F
+      t = t + 0.25 * r;
+      out[comp][c] = 1.0 * t;
+    }
   }
 }
 
-// P_SOURCE(IN): S = -0.5 sum_dir [(J xi_s p)_+ - (J xi_s p)_-] (RLES/p_source.f90:14-42)
-__global__ void k_psource(Dims d, const double* const* __restrict__ m,
-                          const double* __restrict__ djac, const double* __restrict__ p,
-                          double* s0, double* s1, double* s2) {
-  INTERIOR_IJK(d);
-  double* out[3] = {s0, s1, s2};
-  const long sj = d.sj, sk = d.sk;
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const double *a = m[3 * q], *b = m[3 * q + 1], *g = m[3 * q + 2];
-    double t = djac[c + 1] * a[c + 1] * p[c + 1];
-    t = t - djac[c - 1] * a[c - 1] * p[c - 1];
-    t = t + djac[c + sj] * b[c + sj] * p[c + sj];
-    t = t - djac[c - sj] * b[c - sj] * p[c - sj];
-    t = t + djac[c + sk] * g[c + sk] * p[c + sk];
-    t = t - djac[c - sk] * g[c - sk] * p[c - sk];
-    out[q][c] = -(0.5 * t);
-  }
-}
+// P_SOURCE(IN): S = -0.5 sum_dir [(J xi_s p)_+ - (J xi_s p)_-] (RLES/p_source.f90:14-42).
+// With CORR set, followed on the same cell by U = U + sign DTIME S / J^-1 (RLES/main.f90:108-111;
+// 90-92 with sign -1), S kept for the next step's predictor, and, when cflpart is given, the
+// local CFL number of the corrected velocity (CALCCOEF, k_cfl's formula) reduced to one maximum
+// per 64 x 4 x 1 tile.  A workgroup owns a 64 x 4 (I x J) column of cells over PS_KC
+// consecutive planes and marches up K; the products (J zeta_s p) of the planes K+-1 ride along
+// in registers, so only the I and J neighbours are fetched per cell.
+constexpr int PS_KC = 8;
 
-// P_SOURCE(P) followed by U = U + sign DTIME S / J^-1 on the same cell (RLES/main.f90:108-111;
-// 90-92 with sign -1): one pass, S kept for the next step's predictor.
-__global__ void k_psource_correct(Dims d, const double* const* __restrict__ m,
-                                  const double* __restrict__ djac, const double* __restrict__ p,
-                                  double sign, double dtime, double* s0, double* s1, double* s2,
-                                  double* u, double* v, double* w, double* cflpart) {
+template <int CORR>
+__global__ void __launch_bounds__(256) k_psource_km(Dims d, const double* const* __restrict__ m,
+                                                    const double* __restrict__ djac,
+                                                    const double* __restrict__ p, double sign,
+                                                    double dtime, double* s0, double* s1,
+                                                    double* s2, double* u, double* v, double* w,
+                                                    double* cflpart) {
   __shared__ double sh[256];
   const uint3 bb = xcd_block();
-  const int i = 1 + bb.x * 64 + threadIdx.x;
-  const int j = 1 + bb.y * 4 + threadIdx.y;
-  const int k = 1 + bb.z;
-  double lcfl = 0.0;
-  if (i <= d.ni - 1 && j <= d.nj - 1) {
-    const long c = cidx(d, i, j, k);
-    double* out[3] = {s0, s1, s2};
-    double* f[3] = {u, v, w};
-    double nv[3];
-    const long sj = d.sj, sk = d.sk;
-    const double J = djac[c];
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+  const int i = 1 + bb.x * 64 + tx;
+  const int j = 1 + bb.y * 4 + ty;
+  const int k0 = 1 + bb.z * PS_KC;
+  const int k1 = min(k0 + PS_KC - 1, d.nk - 1);
+  const bool act = i <= d.ni - 1 && j <= d.nj - 1;
+  const long sj = d.sj;
+  double* out[3] = {s0, s1, s2};
+  double* f[3] = {u, v, w};
+  // (J zeta_s p) of the own column, planes kp-2, kp-1, kp, per component s
+  double zp[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const double *a = m[3 * q], *b = m[3 * q + 1], *g = m[3 * q + 2];
-      double t = djac[c + 1] * a[c + 1] * p[c + 1];
-      t = t - djac[c - 1] * a[c - 1] * p[c - 1];
-      t = t + djac[c + sj] * b[c + sj] * p[c + sj];
-      t = t - djac[c - sj] * b[c - sj] * p[c - sj];
-      t = t + djac[c + sk] * g[c + sk] * p[c + sk];
-      t = t - djac[c - sk] * g[c - sk] * p[c - sk];
-      const double sv = -(0.5 * t);
-      out[q][c] = sv;
-      nv[q] = sign < 0 ? f[q][c] - dtime * sv / J : f[q][c] + dtime * sv / J;
-      f[q][c] = nv[q];
+    for (int q = 0; q < 3; ++q) { zp[q][0] = zp[q][1]; zp[q][1] = zp[q][2]; }
+    if (act) {
+      const long cp = cidx(d, i, j, kp);
+      const double jc = djac[cp], pc = p[cp];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) zp[q][2] = jc * m[3 * q + 2][cp] * pc;
     }
-    if (cflpart) {  // CALCCOEF's local CFL number of the projected velocity (k_cfl's formula)
-      const double U = nv[0], V = nv[1], W = nv[2];
-      const double ca = fabs(m[0][c] * U + m[3][c] * V + m[6][c] * W);
-      const double cb = fabs(m[1][c] * U + m[4][c] * V + m[7][c] * W);
-      const double ce = fabs(m[2][c] * U + m[5][c] * V + m[8][c] * W);
-      lcfl = dtime * (ca + cb + ce);
+    if (kp < k0 + 1) continue;  // uniform over the workgroup
+    const int k = kp - 1;
+    double lcfl = 0.0;
+    if (act) {
+      const long c = cidx(d, i, j, k);
+      double nv[3];
+      const double J = djac[c];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double *a = m[3 * q], *b = m[3 * q + 1];
+        double t = djac[c + 1] * a[c + 1] * p[c + 1];
+        t = t - djac[c - 1] * a[c - 1] * p[c - 1];
+        t = t + djac[c + sj] * b[c + sj] * p[c + sj];
+        t = t - djac[c - sj] * b[c - sj] * p[c - sj];
+        t = t + zp[q][2];
+        t = t - zp[q][0];
+        const double sv = -(0.5 * t);
+        out[q][c] = sv;
+        if (CORR) {
+          nv[q] = sign < 0 ? f[q][c] - dtime * sv / J : f[q][c] + dtime * sv / J;
+          f[q][c] = nv[q];
+        }
+      }
+      if (CORR && cflpart) {
+        const double U = nv[0], V = nv[1], W = nv[2];
+        const double ca = fabs(m[0][c] * U + m[3][c] * V + m[6][c] * W);
+        const double cb = fabs(m[1][c] * U + m[4][c] * V + m[7][c] * W);
+        const double ce = fabs(m[2][c] * U + m[5][c] * V + m[8][c] * W);
+        lcfl = dtime * (ca + cb + ce);
+      }
     }
-  }
-  if (!cflpart) return;  // uniform over the workgroup
-  const int t = threadIdx.x + 64 * threadIdx.y;
-  sh[t] = lcfl;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+    if (!(CORR && cflpart)) continue;  // uniform over the workgroup
+    sh[tid] = lcfl;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) sh[tid] = fmax(sh[tid], sh[tid + s]);
+      __syncthreads();
+    }
+    if (tid == 0) cflpart[bb.x + gridDim.x * (bb.y + (long)gridDim.y * (k - 1))] = sh[0];
     __syncthreads();
   }
-  if (t == 0) cflpart[bb.x + gridDim.x * (bb.y + (long)gridDim.y * bb.z)] = sh[0];
 }
 
 // predictor correction U = U - DTIME S / J^-1 fused with CONTRA_VEL on the interior cells
@@ -542,7 +591,9 @@ int fvm_k_nl(FvmCtx* c) {
 
 int fvm_k_dif(FvmCtx* c) {
   ProfScope ps(c, "dif");
-  hipLaunchKernelGGL(k_dif, interior_grid(c->d), kBlk, 0, c->stream, c->d, c->pdI, c->pdJ,
+  const Dims d = c->d;
+  const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + DF_KC - 1) / DF_KC);
+  hipLaunchKernelGGL(k_dif_km, g, kBlk, 0, c->stream, c->d, c->pdI, c->pdJ,
                      c->pdK, c->jq12, c->jq13, c->jq23, c->un, c->vn, c->wn, c->dif[0],
                      c->dif[1], c->dif[2]);
   HIP_CHECK(hipGetLastError());
@@ -552,8 +603,11 @@ int fvm_k_dif(FvmCtx* c) {
 int fvm_k_p_source(FvmCtx* c) {
   double** m = c->dm;
   ProfScope ps(c, "p_source");
-  hipLaunchKernelGGL(k_psource, interior_grid(c->d), kBlk, 0, c->stream, c->d,
-                     (const double* const*)m, c->djac, c->p, c->s[0], c->s[1], c->s[2]);
+  const Dims d = c->d;
+  const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + PS_KC - 1) / PS_KC);
+  hipLaunchKernelGGL(k_psource_km<0>, g, kBlk, 0, c->stream, c->d, (const double* const*)m,
+                     c->djac, c->p, 0.0, 0.0, c->s[0], c->s[1], c->s[2], nullptr, nullptr,
+                     nullptr, nullptr);
   HIP_CHECK(hipGetLastError());
   return FVM_OK;
 }
@@ -703,11 +757,13 @@ int fvm_wall2d_setup(FvmCtx* c, const double* xg, const double* yg) {
 // the interior velocity nor the metrics)
 int fvm_k_psource_correct(FvmCtx* c, double sign, bool with_cfl) {
   ProfScope ps(c, "psource_correct");
-  const dim3 g = interior_grid(c->d);
-  if (with_cfl && (long)g.x * g.y * g.z > c->ncflpart) return FVM_ERR_SIZE;
-  hipLaunchKernelGGL(k_psource_correct, g, kBlk, 0, c->stream, c->d,
-                     (const double* const*)c->dm, c->djac, c->p, sign, c->prm.dtime, c->s[0],
-                     c->s[1], c->s[2], c->u, c->v, c->w, with_cfl ? c->cflpart : nullptr);
+  const Dims d = c->d;
+  const dim3 gt = interior_grid(d);  // CFL maxima: one per 64 x 4 x 1 tile
+  if (with_cfl && (long)gt.x * gt.y * gt.z > c->ncflpart) return FVM_ERR_SIZE;
+  const dim3 g(gt.x, gt.y, (d.nk - 1 + PS_KC - 1) / PS_KC);
+  hipLaunchKernelGGL(k_psource_km<1>, g, kBlk, 0, c->stream, c->d, (const double* const*)c->dm,
+                     c->djac, c->p, sign, c->prm.dtime, c->s[0], c->s[1], c->s[2], c->u, c->v,
+                     c->w, with_cfl ? c->cflpart : nullptr);
   HIP_CHECK(hipGetLastError());
   return FVM_OK;
 }
