@@ -153,6 +153,23 @@ def seeded_ic(ni: int, nj: int, nk: int, sigma: float, seed: int = SEED, three_d
     return out
 
 
+def seeded_temperature(ni: int, nj: int, nk: int, sigma: float, seed: int = SEED) -> np.ndarray:
+    """Temperature start field: N(0, sigma) on the interior cells (stream 3 of the seed), wall
+    ghost mirrored about T_WALL = 1 (T(0) = 2 - T(1)), far-field ghost about TIN = 0, periodic
+    copies across the branch cut and the span."""
+    t = np.zeros((ni + 1, nj + 1, nk + 1), order="F")
+    interior = (slice(1, ni), slice(1, nj), slice(1, nk))
+    t[interior] = normal_stream(3, (ni - 1) * (nj - 1) * (nk - 1), sigma, seed).reshape(
+        t[interior].shape, order="F")
+    t[0] = 2.0 - t[1]
+    t[ni] = -t[ni - 1]
+    t[:, 0] = t[:, nj - 1]
+    t[:, nj] = t[:, 1]
+    t[:, :, 0] = t[:, :, nk - 1]
+    t[:, :, nk] = t[:, :, 1]
+    return t
+
+
 # --------------------------------------------------------------------------------------
 # Restart files
 # --------------------------------------------------------------------------------------

@@ -40,11 +40,14 @@ typedef struct orc3d {
   /* LES */
   double *filt, *lss[6], *mag, *uh, *vh, *wh, *magh, *dumin, *dumout, *lij, *mij, *lm, *mm;
   double *nut, *sgsx, *sgsy, *sgsz;
-  double c_avg;
+  /* passive temperature (ITER_T, BCT, SGS_T) */
+  double *t, *tn, *adv, *adv1, *dift, *sgst, *th, *dtdx, *dtdy, *dtdz, *kappa;
+  double c_avg, c_t, pe, nu, sum_area;
   /* scalars of the last step */
   double uin, resip, pref, cfl, cpb, cd, cl, cdp, cdv, clp, clv;
   int niter;
   int no_pref; /* 1: skip the span-mean pressure reference (the 2D solver has none) */
+  int thermal; /* 1: advance the passive temperature as well (ITER_T, BCT, SGS_T) */
 } orc3d;
 
 #define IX(o, i, j, k) ((long)(i) + (o)->sj * (long)(j) + (o)->sk * (long)(k))
@@ -56,6 +59,7 @@ orc3d *orc3d_create(int ni, int nj, int nk, double re, double dtime, double ref_
   o->ni = ni; o->nj = nj; o->nk = nk;
   o->sj = ni + 1; o->sk = (long)(ni + 1) * (nj + 1); o->n = o->sk * (nk + 1);
   o->re = re; o->dtime = dtime; o->ref_a = ref_a; o->pi = 4.0 * atan(1.0);
+  o->pe = 2769.0; /* the reference's Peclet number (comdat_param.f90:48) */
   double **all[] = {&o->xg, &o->yg, &o->zg, &o->x, &o->y, &o->z, &o->djac, &o->xix, &o->xiy,
                     &o->xiz, &o->etx, &o->ety, &o->etz, &o->ztx, &o->zty, &o->ztz, &o->q11,
                     &o->q12, &o->q13, &o->q22, &o->q23, &o->q33, &o->de, &o->dw, &o->dn,
@@ -67,7 +71,8 @@ orc3d *orc3d_create(int ni, int nj, int nk, double re, double dtime, double ref_
                     &o->su, &o->dd, &o->filt, &o->lss[0], &o->lss[1], &o->lss[2], &o->lss[3],
                     &o->lss[4], &o->lss[5], &o->mag, &o->uh, &o->vh, &o->wh, &o->magh,
                     &o->dumin, &o->dumout, &o->lij, &o->mij, &o->lm, &o->mm, &o->nut,
-                    &o->sgsx, &o->sgsy, &o->sgsz};
+                    &o->sgsx, &o->sgsy, &o->sgsz, &o->t, &o->tn, &o->adv, &o->adv1,
+                    &o->dift, &o->sgst, &o->th, &o->dtdx, &o->dtdy, &o->dtdz, &o->kappa};
   for (size_t a = 0; a < sizeof(all) / sizeof(all[0]); ++a) *all[a] = alloc0(o->n);
   return o;
 }
@@ -75,8 +80,8 @@ orc3d *orc3d_create(int ni, int nj, int nk, double re, double dtime, double ref_
 void orc3d_destroy(orc3d *o) {
   if (!o) return;
   double **p = &o->xg;
-  /* every double* member from xg to sgsz is an allocation */
-  double **end = &o->sgsz;
+  /* every double* member from xg to kappa is an allocation */
+  double **end = &o->kappa;
   for (; p <= end; ++p) free(*p);
   free(o);
 }
@@ -95,12 +100,20 @@ double *orc3d_array(orc3d *o, const char *name) {
   N("difz", difz) N("s_u", s_u) N("s_v", s_v) N("s_w", s_w) N("su", su) N("dd", dd)
   N("ap", ap) N("filt", filt) N("mag", mag) N("uh", uh) N("vh", vh) N("wh", wh)
   N("magh", magh) N("lm", lm) N("mm", mm) N("nut", nut) N("sgsx", sgsx) N("sgsy", sgsy)
-  N("sgsz", sgsz) N("u1", u1) N("u2", u2) N("u3", u3)
+  N("sgsz", sgsz) N("u1", u1) N("u2", u2) N("u3", u3) N("t", t) N("tn", tn) N("adv", adv)
+  N("adv1", adv1) N("dift", dift) N("sgst", sgst) N("that", th) N("dtdx", dtdx)
+  N("dtdy", dtdy) N("dtdz", dtdz) N("kappa", kappa)
 #undef N
   return NULL;
 }
 
 void orc3d_set_no_pref(orc3d *o, int on) { o->no_pref = on; }
+void orc3d_set_thermal(orc3d *o, int on) { o->thermal = on; }
+void orc3d_set_temperature(orc3d *o, const double *t, const double *adv) {
+  memcpy(o->t, t, o->n * sizeof(double));
+  memcpy(o->adv, adv, o->n * sizeof(double));
+}
+void orc3d_set_pe(orc3d *o, double pe) { o->pe = pe; }
 
 double orc3d_scalar(orc3d *o, const char *name) {
   if (!strcmp(name, "resip")) return o->resip;
@@ -115,6 +128,9 @@ double orc3d_scalar(orc3d *o, const char *name) {
   if (!strcmp(name, "clp")) return o->clp;
   if (!strcmp(name, "clv")) return o->clv;
   if (!strcmp(name, "c_avg")) return o->c_avg;
+  if (!strcmp(name, "c_t")) return o->c_t;
+  if (!strcmp(name, "nu")) return o->nu;
+  if (!strcmp(name, "sum_area")) return o->sum_area;
   return NAN;
 }
 
@@ -328,6 +344,7 @@ void orc3d_setup(orc3d *o, const double *xg, const double *yg, const double *zg)
     o->theta[c] = th;
   }
   /* wall triangle areas (RLES/init.f90:240-259) */
+  double sum_area = 0.0;
   for (int k = 1; k <= nk - 1; ++k)
     for (int j = 1; j <= nj - 1; ++j) {
       int i = 1;
@@ -352,7 +369,9 @@ void orc3d_setup(orc3d *o, const double *xg, const double *yg, const double *zg)
                 SQ(ZG(i, j + 1, k + 1) - ZG(i, j, k + 1)));
       th1 = acos((l3 * l3 - l2 * l2 - l1 * l1) / (-2.0 * l1 * l2));
       o->area2[IX(o, i, j, k)] = 0.5 * l1 * l2 * sin(th1);
+      sum_area = sum_area + o->area1[IX(o, i, j, k)] + o->area2[IX(o, i, j, k)];
     }
+  o->sum_area = sum_area; /* SUM_AREA (init.f90:240-261) */
   /* FILT_WID = DJAC**(1/3) (RLES/stress_tensor_dy.f90:24) */
   for (long c = 0; c < n; ++c) o->filt[c] = pow(o->djac[c], 1.0 / 3.0);
 }
@@ -482,10 +501,12 @@ void orc3d_p_source(orc3d *o, const double *in) {
 /* BCU(A,B) (RLES/bcu.f90:15-48): wall no-slip and far-field inflow / convective outflow,     */
 /* folded into AP/SU/AW/AE of the current stage, ghost values, then MPI_X.                   */
 /* ----------------------------------------------------------------------------------------- */
-void orc3d_bcu(orc3d *o, double *a, const double *b) {
+/* BCU(A,B) (RLES/bcu.f90:15-48) and BCT(A,B) (RLES/bct.f90:10-49): the same folds with the
+ * wall value (U_F = 0 / T_WALL = 1) and the inflow value (UIN / TIN = 0). */
+static void bc_fold(orc3d *o, double *a, const double *b, double wall, double inflow) {
   int ni = o->ni, nj = o->nj, nk = o->nk;
   long sj = o->sj, sk = o->sk;
-  double uf = 0.0;
+  double uf = wall;
   for (int k = 0; k <= nk; ++k)
     for (int j = 0; j <= nj; ++j) {
       long c1 = IX(o, 1, j, k);
@@ -500,7 +521,7 @@ void orc3d_bcu(orc3d *o, double *a, const double *b) {
       long c = IX(o, ni, j, k), m = c - 1;
       double th = o->theta[c];
       if (th >= 0.5 * pi && th <= 1.5 * pi) {
-        uf = o->uin;
+        uf = inflow;
       } else {
         double t1 = 0.5 * (o->xix[c] + o->xix[m]) * (b[c] - b[m]);
         double t2 = 0.5 * (o->etx[c] + o->etx[m]) * 0.25 *
@@ -517,6 +538,9 @@ void orc3d_bcu(orc3d *o, double *a, const double *b) {
   orc3d_mpi_x(o, a);
 }
 
+void orc3d_bcu(orc3d *o, double *a, const double *b) { bc_fold(o, a, b, 0.0, o->uin); }
+void orc3d_bct(orc3d *o, double *a, const double *b) { bc_fold(o, a, b, 1.0, 0.0); }
+
 /* Lines solved by the Thomas algorithm with the SAVE'd B(0)=0 start (RLES/iter_u.f90:86-101). */
 static double *g_A, *g_B, *g_D;
 static void ensure_tmp(int n) {
@@ -531,15 +555,18 @@ static void ensure_tmp(int n) {
 }
 
 /* ITER_U(IN,INM,INA,INB,INC,IND) (RLES/iter_u.f90:1-256): 3 AF stages, each an in-place
- * lexicographic line Gauss-Seidel sweep (I lines, then J lines, then K lines). */
-void orc3d_iter_u(orc3d *o, double *in, const double *inm, const double *ina, const double *inb,
-                  const double *inc, const double *ind) {
+ * lexicographic line Gauss-Seidel sweep (I lines, then J lines, then K lines).  With
+ * thermal set it is ITER_T (RLES/iter_t.f90:1-247): the same stages with PE for RE, the
+ * stage-1 source 2 PE J/dt T - 3 PE ADV + PE ADV_NM1 + DIFT + DD (no pressure term; ind
+ * unused) and BCT for BCU. */
+static void iter_af(orc3d *o, double *in, const double *inm, const double *ina,
+                    const double *inb, const double *inc, const double *ind, int thermal) {
   int ni = o->ni, nj = o->nj, nk = o->nk;
   long sj = o->sj, sk = o->sk;
   const double *J = o->djac;
   const double *ms[3][3] = {{o->xix, o->etx, o->ztx}, {o->xiy, o->ety, o->zty},
                             {o->xiz, o->etz, o->ztz}};
-  double re = o->re, dtime = o->dtime;
+  double re = thermal ? o->pe : o->re, dtime = o->dtime;
   int nmax = ni > nj ? ni : nj;
   if (nk > nmax) nmax = nk;
   ensure_tmp(nmax);
@@ -582,7 +609,14 @@ void orc3d_iter_u(orc3d *o, double *in, const double *inm, const double *ina, co
       for (int j = 1; j <= nj - 1; ++j)
         for (int i = 1; i <= ni - 1; ++i) {
           long c = IX(o, i, j, k);
-          if (s == 0) {
+          if (s == 0 && thermal) {
+            double t = 2.0 * re * J[c] / dtime * in[c];
+            t = t - 3.0 * re * ina[c];
+            t = t + 1.0 * re * inb[c];
+            t = t + inc[c];
+            t = t + o->dd[c];
+            o->su[c] = t;
+          } else if (s == 0) {
             double t = 2.0 * re * J[c] / dtime * inm[c];
             t = t - 3.0 * re * ina[c];
             t = t + 1.0 * re * inb[c];
@@ -594,7 +628,10 @@ void orc3d_iter_u(orc3d *o, double *in, const double *inm, const double *ina, co
             o->su[c] = 2.0 * re * J[c] / dtime * in[c] + o->dd[c];
           }
         }
-    orc3d_bcu(o, in, inm);
+    if (thermal)
+      orc3d_bct(o, in, inm);
+    else
+      orc3d_bcu(o, in, inm);
     double *A = g_A, *B = g_B, *D = g_D;
     if (s == 0) {
       for (int k = 1; k <= nk - 1; ++k)
@@ -650,6 +687,14 @@ void orc3d_iter_u(orc3d *o, double *in, const double *inm, const double *ina, co
     }
   }
 }
+
+void orc3d_iter_u(orc3d *o, double *in, const double *inm, const double *ina, const double *inb,
+                  const double *inc, const double *ind) {
+  iter_af(o, in, inm, ina, inb, inc, ind, 0);
+}
+
+/* ITER_T (RLES/iter_t.f90:1-247) on T with T_N, ADV, ADV_NM1, DIFT */
+void orc3d_iter_t(orc3d *o) { iter_af(o, o->t, o->tn, o->adv, o->adv1, o->dift, NULL, 1); }
 
 /* BCP (RLES/bcp.f90:15-36) */
 void orc3d_bcp(orc3d *o) {
@@ -798,6 +843,14 @@ static void strain(orc3d *o) {
         L23[c] = 0.25 * t;
         t = o->xiz[c] * DI(W) + o->etz[c] * DJ(W) + o->ztz[c] * DK(W);
         L33[c] = 0.5 * t;
+        /* temperature gradient (stress_tensor_dy.f90:78-94) */
+        const double *T = o->t;
+        t = o->xix[c] * DI(T) + o->etx[c] * DJ(T) + o->ztx[c] * DK(T);
+        o->dtdx[c] = 0.5 * t;
+        t = o->xiy[c] * DI(T) + o->ety[c] * DJ(T) + o->zty[c] * DK(T);
+        o->dtdy[c] = 0.5 * t;
+        t = o->xiz[c] * DI(T) + o->etz[c] * DJ(T) + o->ztz[c] * DK(T);
+        o->dtdz[c] = 0.5 * t;
 #undef DI
       }
   for (int k = 1; k <= nk - 1; ++k)
@@ -816,17 +869,108 @@ static void strain(orc3d *o) {
         L23[c] = 0.25 * (2.0 * o->xiz[c] * DI(V) + o->etz[c] * DJ(V) + o->ztz[c] * DK(V) +
                          2.0 * o->xiy[c] * DI(W) + o->ety[c] * DJ(W) + o->zty[c] * DK(W));
         L33[c] = 0.5 * (2.0 * o->xiz[c] * DI(W) + o->etz[c] * DJ(W) + o->ztz[c] * DK(W));
+        const double *T = o->t;  /* (stress_tensor_dy.f90:138-146, 188-196) */
+        o->dtdx[c] = 0.5 * (2.0 * o->xix[c] * DI(T) + o->etx[c] * DJ(T) + o->ztx[c] * DK(T));
+        o->dtdy[c] = 0.5 * (2.0 * o->xiy[c] * DI(T) + o->ety[c] * DJ(T) + o->zty[c] * DK(T));
+        o->dtdz[c] = 0.5 * (2.0 * o->xiz[c] * DI(T) + o->etz[c] * DJ(T) + o->ztz[c] * DK(T));
 #undef DI
 #undef DJ
 #undef DK
       }
   for (int q = 0; q < 6; ++q) orc3d_mpi_x(o, o->lss[q]);
+  orc3d_mpi_x(o, o->dtdx);
+  orc3d_mpi_x(o, o->dtdy);
+  orc3d_mpi_x(o, o->dtdz);
   for (long c = 0; c < o->n; ++c) {
     double a = L11[c], b = L12[c], d = L13[c], e = L22[c], f = L23[c], g = L33[c];
     /* order L11 L12 L13 L21 L22 L23 L31 L32 L33 with L21=L12, L31=L13, L32=L23 */
     double s = a * a + b * b + d * d + b * b + e * e + f * f + d * d + f * f + g * g;
     o->mag[c] = sqrt(2.0 * s);
   }
+}
+
+/* STRESS_TENSOR_DY temperature part (RLES/stress_tensor_dy.f90:223-233, 589-738): the test-filtered
+ * temperature with BCT, the Germano identity for the heat flux (P_i = -(u_i T)^ + u_i^ T^,
+ * R_i = Delta^2 (4 |S^| dT^/dx_i - (|S| dT/dx_i)^)), the global conditional average
+ * C_t = <PR>/<2RR> over cells with PR/RR >= 0, the eddy conductivity kappa = C_t Delta^2 |S|
+ * and SGS_T = the divergence of the subgrid heat flux.  Uses |S^| (magh) of the velocity part. */
+static void sgs_thermal(orc3d *o) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  long sj = o->sj, sk = o->sk, n = o->n;
+  orc3d_test_filter(o, o->t, o->th);
+  orc3d_bct(o, o->th, o->tn);
+  const double *TH = o->th;
+  const double *ui[3] = {o->u, o->v, o->w};
+  const double *hi[3] = {o->uh, o->vh, o->wh};
+  const double *dt[3] = {o->dtdx, o->dtdy, o->dtdz};
+  const double *mx[3][3] = {{o->xix, o->etx, o->ztx}, {o->xiy, o->ety, o->zty},
+                            {o->xiz, o->etz, o->ztz}};
+  for (int m = 0; m < 3; ++m) {
+    for (long c = 0; c < n; ++c) o->dumin[c] = ui[m][c] * o->t[c];
+    orc3d_test_filter(o, o->dumin, o->dumout);
+    for (int k = 1; k <= nk - 1; ++k)
+      for (int j = 1; j <= nj - 1; ++j)
+        for (int i = 1; i <= ni - 1; ++i) {
+          long c = IX(o, i, j, k);
+          o->lij[c] = -o->dumout[c] + hi[m][c] * TH[c];
+        }
+    for (long c = 0; c < n; ++c) o->dumin[c] = o->mag[c] * dt[m][c];
+    orc3d_test_filter(o, o->dumin, o->dumout);
+    for (int k = 1; k <= nk - 1; ++k)
+      for (int j = 1; j <= nj - 1; ++j)
+        for (int i = 1; i <= ni - 1; ++i) {
+          long c = IX(o, i, j, k);
+          double g = mx[m][0][c] * (TH[c + 1] - TH[c - 1]) + mx[m][1][c] * (TH[c + sj] - TH[c - sj]) +
+                     mx[m][2][c] * (TH[c + sk] - TH[c - sk]);
+          double M = 4.0 * o->magh[c] * 0.5 * g;
+          double fw = o->filt[c];
+          M = fw * fw * (M - o->dumout[c]);
+          double L = o->lij[c];
+          if (m == 0) {
+            o->lm[c] = L * M;
+            o->mm[c] = M * M;
+          } else {
+            o->lm[c] = o->lm[c] + L * M;
+            o->mm[c] = o->mm[c] + M * M;
+          }
+        }
+  }
+  double lm_avg = 0.0, mm_avg = 0.0, inc = 0.0;
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        double cs = 0.5 * o->lm[c] / o->mm[c];
+        if (cs >= 0.0) {
+          lm_avg = lm_avg + o->lm[c];
+          mm_avg = mm_avg + o->mm[c];
+          inc = inc + 1.0;
+        }
+      }
+  double tlm = lm_avg / inc, tmm = mm_avg / inc;
+  double ct = 0.5 * tlm / tmm;
+  if (tmm < 1e-6) ct = 0.0;
+  o->c_t = ct;
+  for (long c = 0; c < n; ++c) o->kappa[c] = ct * (o->filt[c] * o->filt[c]) * o->mag[c];
+  const double *KA = o->kappa;
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        double t = 0.0;
+        int first = 1;
+        for (int cmp = 0; cmp < 3; ++cmp) {
+          const double *D = dt[cmp];
+          long off[3] = {1, sj, sk};
+          for (int dir = 0; dir < 3; ++dir) {
+            long of = off[dir];
+            double inner = -(2.0 * KA[c + of] * D[c + of]) + 2.0 * KA[c - of] * D[c - of];
+            double term = mx[cmp][dir][c] * inner;
+            if (first) { t = term; first = 0; } else t = t + term;
+          }
+        }
+        o->sgst[c] = 0.5 * t;
+      }
 }
 
 /* STRESS_TENSOR_DY velocity part (RLES/stress_tensor_dy.f90:24-286): dynamic Smagorinsky with
@@ -956,9 +1100,8 @@ void orc3d_sgs(orc3d *o) {
           }
           out[d][c] = 0.5 * t;
         }
+  if (o->thermal) sgs_thermal(o);
 }
-
-/* CALCCOEF (RLES/calccoef.f90:11-201): CFL, base pressure and force coefficients. */
 void orc3d_calccoef(orc3d *o) {
   int ni = o->ni, nj = o->nj, nk = o->nk;
   (void)ni;
@@ -986,7 +1129,7 @@ void orc3d_calccoef(orc3d *o) {
   }
   dumb = dumb / (double)(nk - 1);
   o->cpb = 2.0 * (0.0 - dumb);
-  double pd = 0.0, ply = 0.0, td = 0.0, tly = 0.0;
+  double pd = 0.0, ply = 0.0, td = 0.0, tly = 0.0, sum_nu = 0.0;
   for (int tri = 0; tri < 2; ++tri)
     for (int k = 1; k <= nk - 1; ++k)
       for (int j = 1; j <= nj - 1; ++j) {
@@ -1040,7 +1183,10 @@ void orc3d_calccoef(orc3d *o) {
         ply = ply - 2.0 * o->p[c] * ar * B / dum;
         td = td + (2.0 / o->re) * dudn * ar * U / um;
         tly = tly + (2.0 / o->re) * dudn * ar * V / um;
+        double dtdn = (o->t[c] - 1.0) / dn; /* (T - T_WALL) / D_N (calccoef.f90:117-119) */
+        sum_nu = sum_nu + dtdn * ar;
       }
+  o->nu = -sum_nu / o->sum_area;
   o->cdp = pd / o->ref_a;
   o->clp = ply / o->ref_a;
   o->cdv = td / o->ref_a;
@@ -1049,23 +1195,26 @@ void orc3d_calccoef(orc3d *o) {
   o->cl = o->clp + o->clv;
 }
 
-/* One time step of the reference time loop (RLES/main.f90:39-150, thermal scalar and file
- * output excluded). */
+/* One time step of the reference time loop (RLES/main.f90:39-150, file output excluded; the
+ * passive temperature when thermal is set). */
 void orc3d_step(orc3d *o, int nswpp, double resimax, int use_les) {
   int ni = o->ni, nj = o->nj, nk = o->nk;
   long n = o->n;
   size_t b = n * sizeof(double);
   memcpy(o->un, o->u, b); memcpy(o->vn, o->v, b); memcpy(o->wn, o->w, b);
+  if (o->thermal) memcpy(o->tn, o->t, b);
   orc3d_contra_vel(o);
   for (int k = 1; k <= nk - 1; ++k)
     for (int j = 1; j <= nj - 1; ++j)
       for (int i = 1; i <= ni - 1; ++i) {
         long c = IX(o, i, j, k);
         o->nlx1[c] = o->nlx[c]; o->nly1[c] = o->nly[c]; o->nlz1[c] = o->nlz[c];
+        o->adv1[c] = o->adv[c];
       }
   orc3d_nl(o, o->un, o->nlx);
   orc3d_nl(o, o->vn, o->nly);
   orc3d_nl(o, o->wn, o->nlz);
+  if (o->thermal) orc3d_nl(o, o->tn, o->adv);
   if (use_les) {
     orc3d_sgs(o);
     for (int k = 1; k <= nk - 1; ++k)
@@ -1075,11 +1224,13 @@ void orc3d_step(orc3d *o, int nswpp, double resimax, int use_les) {
           o->nlx[c] = o->nlx[c] + o->sgsx[c] * o->djac[c];
           o->nly[c] = o->nly[c] + o->sgsy[c] * o->djac[c];
           o->nlz[c] = o->nlz[c] + o->sgsz[c] * o->djac[c];
+          if (o->thermal) o->adv[c] = o->adv[c] + o->sgst[c] * o->djac[c];
         }
   }
   orc3d_dif(o, o->un, o->difx);
   orc3d_dif(o, o->vn, o->dify);
   orc3d_dif(o, o->wn, o->difz);
+  if (o->thermal) orc3d_dif(o, o->tn, o->dift);
   orc3d_p_source(o, o->p);
   o->uin = 1.0;
   orc3d_iter_u(o, o->u, o->un, o->nlx, o->nlx1, o->difx, o->s_u);
@@ -1118,5 +1269,9 @@ void orc3d_step(orc3d *o, int nswpp, double resimax, int use_les) {
   }
   o->uin = 1.0; orc3d_bcu(o, o->u, o->un);
   o->uin = 0.0; orc3d_bcu(o, o->v, o->vn); orc3d_bcu(o, o->w, o->wn);
+  if (o->thermal) {  /* main.f90:145-147 */
+    orc3d_iter_t(o);
+    orc3d_bct(o, o->t, o->tn);
+  }
   orc3d_calccoef(o);
 }

@@ -45,6 +45,11 @@ def lib():
         L.orc3d_bcu.argtypes = [vp, vp, vp]
         L.orc3d_mpi_x.argtypes = [vp, vp]
         L.orc3d_iter_u.argtypes = [vp] * 7
+        L.orc3d_set_thermal.argtypes = [vp, ci]
+        L.orc3d_set_pe.argtypes = [vp, cd]
+        L.orc3d_set_temperature.argtypes = [vp, vp, vp]
+        L.orc3d_iter_t.argtypes = [vp]
+        L.orc3d_bct.argtypes = [vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -87,6 +92,13 @@ class Oracle3D:
     def set_state(self, u, v, w, p, nlx, nly, nlz):
         arrs = [np.asfortranarray(a, dtype=np.float64) for a in (u, v, w, p, nlx, nly, nlz)]
         self.L.orc3d_set_state(self.h, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs])
+
+    def set_temperature(self, t, adv, pe=2769.0):
+        """Passive temperature T and its convective history ADV; enables ITER_T/BCT/SGS_T."""
+        arrs = [np.asfortranarray(a, dtype=np.float64) for a in (t, adv)]
+        self.L.orc3d_set_temperature(self.h, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs])
+        self.L.orc3d_set_pe(self.h, float(pe))
+        self.L.orc3d_set_thermal(self.h, 1)
 
     def set_no_pref(self, on=True):
         self.L.orc3d_set_no_pref(self.h, 1 if on else 0)

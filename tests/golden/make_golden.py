@@ -20,7 +20,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, REPO)
-from fvm_amd.grid import (OGrid, seeded_ic, write_grid2d, write_grid3d,  # noqa: E402
+from fvm_amd.grid import (OGrid, seeded_ic, seeded_temperature,  # noqa: E402
+                          write_grid2d, write_grid3d,
                           write_restart2d, write_restart3d, read_restart2d, read_restart3d,
                           grid_filename)
 
@@ -39,6 +40,14 @@ CASES = {
     "rles_33x65x9_coldstart_dt2e-3": dict(ni=33, nj=65, nk=9, h1=0.001, r_out=25.0, sigma=0.05,
                                           re=3900.0, dt=2e-3, nswpp=200000, resimax="1.0D-8",
                                           les=1, steps=3, keep_steps=(1,), keep_tags=("u",)),
+    # the 8 x 20 x 4 converged case with a seeded temperature field: the passive scalar
+    # (NL, SGS_T, DIF, ITER_T, BCT, Nusselt number) at the reference's Peclet number 2769
+    "rles_9x21x5_les_thermal": dict(ni=9, nj=21, nk=5, h1=0.08, r_out=4.0, sigma=0.05,
+                                    re=200.0, dt=4e-3, nswpp=200000, resimax="1.0D-13",
+                                    les=1, steps=2, keep_steps=(1, 2), t_sigma=0.05,
+                                    keep_tags=("u", "v", "w", "pref", "nut", "adv0", "adv",
+                                               "sgst", "kappa", "that", "dtdx", "dtdy", "dtdz",
+                                               "dift", "t")),
     # (no USE_LES=0 fixture: the reference's laminar mode writes YPLUS in CALCCOEF
     # (calccoef.f90:121), an array INIT allocates only when USE_LES=1 (init.f90:79-81))
 }
@@ -66,6 +75,8 @@ def run_case(label, c):
     g = OGrid(ni, nj, nk, c["h1"], c["r_out"])
     xg, yg, zg = g.nodes()
     ic = seeded_ic(ni, nj, nk, c["sigma"])
+    if c.get("t_sigma"):
+        ic["t"] = seeded_temperature(ni, nj, nk, c["t_sigma"])
     out = {"ni": ni, "nj": nj, "nk": nk, "re": c["re"], "dt": c["dt"], "ref_a": 3.14159265358979,
            "nswpp": c["nswpp"], "resimax": float(c["resimax"].replace("D", "e")),
            "use_les": c["les"], "xg": xg, "yg": yg, "zg": zg}
@@ -77,9 +88,11 @@ def run_case(label, c):
         rs = read_restart3d(os.path.join(d, "RESTART000.DAT"), ni, nj, nk)
         for k in ("u", "v", "w", "p", "nlx", "nly", "nlz"):
             out["in_" + k] = rs[k]
+        if c.get("t_sigma"):
+            out["in_t"], out["in_adv"], out["pe"] = rs["t"], rs["adv"], 2769.0
         subprocess.run([exe, str(c["steps"])], cwd=d, check=True, stdout=subprocess.DEVNULL)
         rd = lambda t: np.fromfile(os.path.join(d, t + ".bin")).reshape(shape, order="F")
-        setup_tags = SETUP_TAGS if "keep_tags" not in c else ()
+        setup_tags = SETUP_TAGS if ("keep_tags" not in c or c.get("t_sigma")) else ()
         for t in setup_tags:
             out["setup_" + t] = rd("setup_" + t)
         for s in c["keep_steps"]:
@@ -88,8 +101,14 @@ def run_case(label, c):
                 if os.path.exists(p):
                     out[f"s{s}_{t}"] = rd(f"s{s:03d}_{t}")
         rows = [l.split() for l in open(os.path.join(d, "scalars.txt")) if l.startswith("step")]
-        # columns: step, niter, resip, pref, cfl, cpb, cd, cl, cd_p, cd_v
+        # columns: step, niter, resip, pref, cfl, cpb, cd, cl, cd_p, cd_v[, nu]
         out["scalars"] = np.array([[float(x) for x in r[1:]] for r in rows])
+        if c.get("t_sigma"):
+            # sqrt of the dynamic coefficients per step, as the reference writes them
+            # (stress_tensor_dy.f90:519-522, 707-711; list-directed RTIME, DSQRT(C_AVG))
+            for f, key in (("CS.PLT", "cs_sqrt"), ("CT.PLT", "ct_sqrt")):
+                vals = [l.split() for l in open(os.path.join(d, f)) if l.strip()]
+                out[key] = np.array([float(v[1].replace("D", "e")) for v in vals])
     np.savez_compressed(os.path.join(HERE, label + ".npz"), **out)
     print("wrote", label, "steps", c["steps"], "niter", out["scalars"][:, 1])
 
@@ -142,6 +161,12 @@ def run_case2d(label, c):
         rows = [l.split() for l in open(os.path.join(d, "scalars.txt")) if l.startswith("step")]
         # columns: step, niter, resip, cfl, cpb, cd, cl, cd_p, cd_v, cl_p, cl_v
         out["scalars"] = np.array([[float(x) for x in r[1:]] for r in rows])
+        if c.get("t_sigma"):
+            # sqrt of the dynamic coefficients per step, as the reference writes them
+            # (stress_tensor_dy.f90:519-522, 707-711; list-directed RTIME, DSQRT(C_AVG))
+            for f, key in (("CS.PLT", "cs_sqrt"), ("CT.PLT", "ct_sqrt")):
+                vals = [l.split() for l in open(os.path.join(d, f)) if l.strip()]
+                out[key] = np.array([float(v[1].replace("D", "e")) for v in vals])
     np.savez_compressed(os.path.join(HERE, label + ".npz"), **out)
     print("wrote", label, "steps", c["steps"], "niter", out["scalars"][:, 1])
 

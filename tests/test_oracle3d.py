@@ -162,3 +162,47 @@ def test_coldstart_step_bitexact_and_reference_divergence():
     assert np.array_equal(o.array("u"), g["s1_u"])
     cfl = g["scalars"][:, 4]
     assert cfl[0] > 10 and cfl[1] > 5 * cfl[0] and cfl[2] > 50 * cfl[1]
+
+
+def test_thermal_steps_bitexact():
+    """The passive temperature (NL, SGS_T, DIF, ITER_T, BCT, Nusselt number) of two converged
+    steps equals the reference's dumps bit for bit, and the velocity is unchanged by it."""
+    g = load_golden("rles_9x21x5_les_thermal")
+    o = make(g)
+    o.set_temperature(g["in_t"], g["in_adv"], float(g["pe"]))
+    ni, nj, nk = int(g["ni"]), int(g["nj"]), int(g["nk"])
+    inner = (slice(1, ni), slice(1, nj), slice(1, nk))
+    for s in (1, 2):
+        o.step(int(g["nswpp"]), float(g["resimax"]), True)
+        for t in ("adv", "sgst", "dift"):
+            assert np.array_equal(o.array(t)[inner], g[f"s{s}_{t}"][inner]), (s, t)
+        for t in ("t", "kappa", "that", "dtdx", "dtdy", "dtdz"):
+            assert np.array_equal(o.array(t), g[f"s{s}_{t}"]), (s, t)
+        for t in ("u", "v", "w", "nut"):
+            assert np.array_equal(o.array(t), g[f"s{s}_{t}"]), (s, t)
+        row = g["scalars"][s - 1]  # ... cd_v, nu
+        assert o.scalar("nu") == row[10]
+        assert np.sqrt(o.scalar("c_t")) == g["ct_sqrt"][s - 1]
+        assert np.sqrt(o.scalar("c_avg")) == g["cs_sqrt"][s - 1]
+
+
+def test_thermal_closed_forms():
+    """Without the dynamic model the temperature step is an affine map of T (the velocity is
+    independent of T): the step of the mean of two temperature fields is the mean of their
+    steps.  A cold fluid (T = TIN = 0) against the hot wall (T_WALL = 1) gives Nu > 0."""
+    g = load_golden("rles_9x21x5_les_converged")
+    rng = np.random.default_rng(7)
+    out = []
+    o = make(g)
+    ta, tb = (rng.normal(0.0, 0.1, o.shape) for _ in range(2))
+    for t0 in (ta, tb, 0.5 * (ta + tb)):
+        o = make(g)
+        o.set_temperature(np.asfortranarray(t0), np.zeros(o.shape, order="F"))
+        o.step(int(g["nswpp"]), float(g["resimax"]), False)
+        out.append(o.array("t").copy())
+    mid = 0.5 * (out[0] + out[1])
+    assert np.abs(out[2] - mid).max() <= 1e-13 * np.abs(mid).max()
+    o2 = make(g)
+    o2.set_temperature(np.zeros(o2.shape, order="F"), np.zeros(o2.shape, order="F"))
+    o2.step(int(g["nswpp"]), float(g["resimax"]), True)
+    assert o2.scalar("nu") > 0.0
