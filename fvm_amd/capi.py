@@ -18,7 +18,8 @@ FIELDS = {
     "difz": 9, "s_u": 10, "s_v": 11, "s_w": 12, "djac": 13, "xix": 14, "etx": 15, "ztx": 16,
     "xiy": 17, "ety": 18, "zty": 19, "xiz": 20, "etz": 21, "ztz": 22, "x": 23, "y": 24, "z": 25,
     "theta": 26, "nut": 27, "sgsx": 28, "sgsy": 29, "sgsz": 30, "un": 31,
-    "vn": 32, "wn": 33,
+    "vn": 32, "wn": 33, "t": 34, "adv": 35, "dift": 36, "sgst": 37, "kappa": 38, "that": 39,
+    "dtdx": 40, "dtdy": 41, "dtdz": 42, "tn": 43,
 }
 
 STATUS = {0: "ok", 1: "bad argument", 2: "HIP runtime error", 3: "non-finite value",
@@ -44,7 +45,8 @@ class StepInfo(ctypes.Structure):
                 ("pref", ctypes.c_double), ("c_avg", ctypes.c_double), ("cfl", ctypes.c_double),
                 ("cpb", ctypes.c_double), ("cd", ctypes.c_double), ("cl", ctypes.c_double),
                 ("cd_p", ctypes.c_double), ("cd_v", ctypes.c_double), ("cl_p", ctypes.c_double),
-                ("cl_v", ctypes.c_double), ("af_iters", ctypes.c_int * 3)]
+                ("cl_v", ctypes.c_double), ("af_iters", ctypes.c_int * 3),
+                ("nu", ctypes.c_double), ("c_t", ctypes.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -56,7 +58,7 @@ EXPORTS = ("fvm_create", "fvm_destroy", "fvm_set_field", "fvm_get_field", "fvm_s
            "fvm_begin_step", "fvm_contra_vel", "fvm_nl", "fvm_stress_tensor_dy", "fvm_dif",
            "fvm_p_source", "fvm_iter_u", "fvm_correct", "fvm_iter_p", "fvm_pressure_reference",
            "fvm_calccoef", "fvm_profile_enable", "fvm_profile_get", "fvm_profile_reset",
-           "fvm_sync", "fvm_set_dtime", "fvm_create2d")
+           "fvm_sync", "fvm_set_dtime", "fvm_create2d", "fvm_set_thermal", "fvm_iter_t")
 
 _LIB = None
 
@@ -84,6 +86,8 @@ def load(path: str = LIB_PATH):
     L.fvm_stress_tensor_dy.argtypes = [vp, pd]
     L.fvm_correct.argtypes = [vp, cd]
     L.fvm_set_dtime.argtypes = [vp, cd]
+    L.fvm_set_thermal.argtypes = [vp, cd]
+    L.fvm_iter_t.argtypes = [vp]
     L.fvm_iter_p.argtypes = [vp, pd, ctypes.POINTER(ci)]
     L.fvm_pressure_reference.argtypes = [vp, pd]
     L.fvm_calccoef.argtypes = [vp, ctypes.POINTER(StepInfo)]

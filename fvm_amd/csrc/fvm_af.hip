@@ -45,7 +45,7 @@ __global__ void k_af_rhs(Dims d, AFArgs A, const double* const* __restrict__ m,
   if (S == 2 && ddzero) {
     const double a0 = ap0[c];
 #pragma unroll
-    for (int comp = 0; comp < 3; ++comp) A.su[comp][c] = a0 * A.f[comp][c];
+    for (int comp = 0; comp < A.nc; ++comp) A.su[comp][c] = a0 * A.f[comp][c];
     return;
   }
   const long sj = d.sj, sk = d.sk;
@@ -62,8 +62,7 @@ __global__ void k_af_rhs(Dims d, AFArgs A, const double* const* __restrict__ m,
   const double pt_zx = J[t] * Z[t] * X[t], pb_zx = J[b] * Z[b] * X[b];
   const double pt_ze = J[t] * Z[t] * E[t], pb_ze = J[b] * Z[b] * E[b];
   const double a0 = ap0[c];
-#pragma unroll
-  for (int comp = 0; comp < 3; ++comp) {
+  for (int comp = 0; comp < A.nc; ++comp) {
     const double* f = A.f[comp];
     double r = pe_xe * (f[e + sj] - f[e - sj]);
     r = r - pw_xe * (f[w + sj] - f[w - sj]);
@@ -124,19 +123,23 @@ __global__ void k_af_products(Dims d, const double* __restrict__ J, const double
 }
 
 // k_af_rhs for stages 0 and 1, K-marching: a workgroup owns a 64 x 4 (I x J) column of cells
-// over RK_KC consecutive K planes and keeps the last three planes of the three stage inputs
-// (one-cell halo) in LDS, so the twelve edge neighbours of the cross terms come from LDS and
-// every input value is fetched from HBM once per column.  The metric products come from the
-// per-stage arrays of k_af_products (equal, bit for bit, to the products k_af_rhs forms); those
-// at K+-1 and AT at K-1 ride along in registers.  Expressions and their order are k_af_rhs's.
+// over RK_KC consecutive K planes and keeps the last three planes of the stage inputs (one-cell
+// halo) in LDS, so the twelve edge neighbours of the cross terms come from LDS and every input
+// value is fetched from HBM once per column.  The metric products come from the per-stage
+// arrays of k_af_products (equal, bit for bit, to the products k_af_rhs forms); those at K+-1
+// and AT at K-1 ride along in registers.  Expressions and their order are k_af_rhs's.
+// TH selects ITER_T (RLES/iter_t.f90:49-55): re is PE, the stage-1 source is
+// 2 PE J^-1/dt T - 3 PE ADV + PE ADV_NM1 + DIFT + DD (A.nl = ADV, A.nl1 = ADV_NM1,
+// A.dif = DIFT) and BCT's wall fold adds 2 AW T_WALL (bct.f90:16-19).
 constexpr int RK_KC = 8;
 
-template <int S>
+template <int S, bool TH>
 __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
                                                    const double* const* __restrict__ P,
                                                    const double* __restrict__ ap0, double re,
                                                    const double* const* __restrict__ fa,
                                                    int qform) {
+  constexpr double kTWall = 1.0;  // T_WALL (comdat_param.f90:40)
   const uint3 bb = xcd_block();
   __shared__ double sh[3][3][SF_P];  // [component][plane slot][tile cell]
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
@@ -158,9 +161,7 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
       const int gi = min(bb.x * 64 + q % SF_W, d.ni);
       const int gj = min(bb.y * 4 + q / SF_W, d.nj);
       const long c = cidx(d, gi, gj, kp);
-      sh[0][slot][q] = A.f[0][c];
-      sh[1][slot][q] = A.f[1][c];
-      sh[2][slot][q] = A.f[2][c];
+      for (int comp = 0; comp < A.nc; ++comp) sh[comp][slot][q] = A.f[comp][c];
     }
     zx[0] = zx[1]; zx[1] = zx[2];
     ze[0] = ze[1]; ze[1] = ze[2];
@@ -184,8 +185,7 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
     const double pt_zx = zx[2], pb_zx = zx[0];
     const double pt_ze = ze[2], pb_ze = ze[0];
     const double a0 = ap0[c];
-#pragma unroll
-    for (int comp = 0; comp < 3; ++comp) {
+    for (int comp = 0; comp < A.nc; ++comp) {
 #define F(dk, dj, di) sh[comp][pl[(dk) + 1]][q0 + (dj) * SF_W + (di)]
       double r = pe_xe * (F(0, 1, 1) - F(0, -1, 1));
       r = r - pw_xe * (F(0, 1, -1) - F(0, -1, -1));
@@ -201,7 +201,13 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
       r = r - pb_ze * (F(-1, 1, 0) - F(-1, -1, 0));
       const double dd = 0.25 * r;
       double su;
-      if (S == 0) {
+      if (S == 0 && TH) {
+        su = a0 * F(0, 0, 0);
+        su = su - 3.0 * re * A.nl[comp][c];
+        su = su + 1.0 * re * A.nl1[comp][c];
+        su = su + A.dif[comp][c];
+        su = su + dd;
+      } else if (S == 0) {
         su = a0 * A.fn[comp][c];
         su = su - 3.0 * re * A.nl[comp][c];
         su = su + 1.0 * re * A.nl1[comp][c];
@@ -213,7 +219,7 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
       }
       if (qform) {
         double ae = fE[c];
-        if (i == 1) su = su + 2.0 * fE[c - 1] * 0.0;
+        if (i == 1) su = su + 2.0 * fE[c - 1] * (TH ? kTWall : 0.0);
         if (i == d.ni - 1) {
           const long nf = (long)(d.nj + 1) * (d.nk + 1);
           su = su + 2.0 * ae * A.ufar[comp * nf + j + (long)(d.nj + 1) * k];
@@ -698,8 +704,8 @@ int fvm_k_iter_u(FvmCtx* c) {
       const double* const* fas = (const double* const*)c->dfa[s];
       const int dz = c->dd2_zero ? 1 : 0;
       const dim3 gk((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + RK_KC - 1) / RK_KC);
-      if (s == 0) hipLaunchKernelGGL(k_af_rhs_km<0>, gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[0], c->ap0, c->prm.re, fas, qf);
-      if (s == 1) hipLaunchKernelGGL(k_af_rhs_km<1>, gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[1], c->ap0, c->prm.re, fas, qf);
+      if (s == 0) hipLaunchKernelGGL((k_af_rhs_km<0, false>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[0], c->ap0, c->prm.re, fas, qf);
+      if (s == 1) hipLaunchKernelGGL((k_af_rhs_km<1, false>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[1], c->ap0, c->prm.re, fas, qf);
       if (s == 2) hipLaunchKernelGGL(k_af_rhs<2>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf, dz);
       HIP_CHECK(hipGetLastError());
     }
@@ -710,7 +716,7 @@ int fvm_k_iter_u(FvmCtx* c) {
     if (s == 2 && c->prm.af_mode == 0 && c->tk_a && c->af_zero_a[2] && c->af_zero_b[2]) {
       // independent K-lines: the exact sweep in one pass
       ProfScope pk(c, "k_af_thomas_k");
-      const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, 3);
+      const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, A.nc);
       if (d.nk - 1 <= 64)
         hipLaunchKernelGGL(k_af_thomas_k_reg<64>, g, dim3(64, 4, 1), 0, st, d, A, c->tk_a,
                            c->tk_d, c->tk_b);
@@ -808,8 +814,67 @@ int fvm_af_setup(FvmCtx* c) {
     const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, 1);
     hipLaunchKernelGGL(k_af_factor_k, g, dim3(64, 4, 1), 0, st, d,
                        (const double* const*)c->dfa[2], c->ap0, c->tk_a, c->tk_d, c->tk_b);
+    if (c->thermal) {  // the temperature's K-line factors (2 Pe J^-1/dt on the diagonal)
+      if (!c->tk_at) {
+        double** arr[3] = {&c->tk_at, &c->tk_dt, &c->tk_bt};
+        for (double** q : arr) {
+          HIP_CHECK(hipMalloc((void**)q, d.n * sizeof(double)));
+          c->allocs.push_back(*q);
+        }
+      }
+      hipLaunchKernelGGL(k_af_factor_k, g, dim3(64, 4, 1), 0, st, d,
+                         (const double* const*)c->dfa[2], c->ap0t, c->tk_at, c->tk_dt,
+                         c->tk_bt);
+    }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(st));
   }
   return fvm_af_scan_setup(c);
+}
+
+// ITER_T (RLES/iter_t.f90:1-247) followed by BCT(T, T_N) (main.f90:145-147): the three AF stages
+// of the momentum step on the temperature alone, with 2 Pe J^-1/dt for 2 Re J^-1/dt and BCT's
+// folds.  Needs the exact K-line stage and the line-scan form of the I and J stages (checked
+// by fvm_thermal_setup).
+int fvm_k_iter_t(FvmCtx* c) {
+  const Dims d = c->d;
+  hipStream_t st = c->stream;
+  if (!c->thermal) return FVM_ERR_ARG;
+  AFArgs A;
+  A.nc = 1;
+  for (int q = 0; q < 3; ++q) {
+    A.f[q] = c->t; A.fn[q] = c->tn; A.nl[q] = c->adv; A.nl1[q] = c->adv1;
+    A.dif[q] = c->dift; A.s[q] = nullptr; A.su[q] = c->su[0];
+  }
+  A.ufar = c->tfar;
+  const dim3 gi((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
+  const dim3 gk((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + RK_KC - 1) / RK_KC);
+  for (int s = 0; s < 3; ++s) {
+    {
+      ProfScope ps(c, "t_rhs");
+      const double* const* fas = (const double* const*)c->dfa[s];
+      if (s == 0) hipLaunchKernelGGL((k_af_rhs_km<0, true>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[0], c->ap0t, c->pe, fas, 1);
+      if (s == 1) hipLaunchKernelGGL((k_af_rhs_km<1, true>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[1], c->ap0t, c->pe, fas, 1);
+      if (s == 2) hipLaunchKernelGGL(k_af_rhs<2>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0t, c->pe, fas, 0, c->dd2_zero ? 1 : 0);
+      HIP_CHECK(hipGetLastError());
+    }
+    int r = fvm_k_bct_ghosts(c, c->t);  // iter_t.f90:84,150,226
+    if (r) return r;
+    ProfScope ps(c, "t_sweep");
+    if (s < 2) {
+      int it = 0;
+      r = fvm_af_scan_stage(c, s, A, &it, true);
+      if (r) return r;
+    } else {
+      const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, 1);
+      if (d.nk - 1 <= 64)
+        hipLaunchKernelGGL(k_af_thomas_k_reg<64>, g, dim3(64, 4, 1), 0, st, d, A, c->tk_at,
+                           c->tk_dt, c->tk_bt);
+      else
+        hipLaunchKernelGGL(k_af_thomas_k, g, dim3(64, 4, 1), 0, st, d, A, c->tk_at, c->tk_dt,
+                           c->tk_bt);
+      HIP_CHECK(hipGetLastError());
+    }
+  }
+  return fvm_k_bct_ghosts(c, c->t);  // BCT(T, T_N), main.f90:147
 }

@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(256) k_scan_unpack1(Dims d, const double* cons
 template <int S>
 __global__ void k_scan_seed(Dims d, AFArgs A, int nchunk, double* bnd) {
   const ScanGeo g = scan_geo(d, S);
-  const long tot = 3L * g.nb * nchunk * g.n;
+  const long tot = (long)A.nc * g.nb * nchunk * g.n;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot;
        t += (long)gridDim.x * blockDim.x) {
     const int p = 1 + (int)(t % g.n);
@@ -176,9 +176,9 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
   const long w = (lin % 8) * (G / 8) + lin / 8;  // XCD-aware: neighbouring items share an XCD
   const int l = threadIdx.x;
   double dmax = 0.0, xmax = 0.0;
-  if (w < 3L * nchunk * g.nb) {
-    const int comp = (int)(w % 3);
-    const long wr = w / 3;
+  if (w < (long)A.nc * nchunk * g.nb) {
+    const int comp = (int)(w % A.nc);
+    const long wr = w / A.nc;
     const int ca = (int)(wr % nchunk);
     const int b = 1 + (int)(wr / nchunk);
     const int a0 = 1 + ca * SC_C;
@@ -315,45 +315,51 @@ __global__ void k_scan_max(const double* __restrict__ part, long nb, double* out
 }
 
 template <int S, int R>
-int scan_stage(FvmCtx* c, const AFArgs& A, int* iters) {
+int scan_stage(FvmCtx* c, const AFArgs& A, int* iters, bool temp) {
   const Dims d = c->d;
   hipStream_t st = c->stream;
   const ScanGeo g = scan_geo(d, S);
   const int nchunk = (g.na + SC_C - 1) / SC_C;
-  const long nwork = 3L * nchunk * g.nb;
+  const int nc = A.nc;
+  const long nwork = (long)nc * nchunk * g.nb;
   const long G = (nwork + 7) / 8 * 8;
   if (2 * G > c->nred) return FVM_ERR_SIZE;
   // Q: the right-hand-side kernel left it in A.su (natural layout); the J stage uses a
   // line-major copy
-  double* q[3];
-  for (int k = 0; k < 3; ++k) q[k] = S == 0 ? A.su[k] : c->afb[k][1];
-  double* xo[3];
-  for (int k = 0; k < 3; ++k) xo[k] = S == 0 ? A.f[k] : c->afb[k][0];
+  double* q[3] = {};
+  for (int k = 0; k < nc; ++k) q[k] = S == 0 ? A.su[k] : c->afb[k][1];
+  double* xo[3] = {};
+  for (int k = 0; k < nc; ++k) xo[k] = S == 0 ? A.f[k] : c->afb[k][0];
+  double* fo[3] = {};
+  for (int k = 0; k < nc; ++k) fo[k] = A.f[k];
   // device pointer table of this stage: q at [0..2], iterate at [3..5], field at [6..8]; the
   // buffers are the context's own, so it is uploaded once and again only if they change
-  double* tab[9] = {q[0], q[1], q[2], xo[0], xo[1], xo[2], A.f[0], A.f[1], A.f[2]};
-  if (memcmp(tab, c->scantab_h[S], sizeof(tab)) != 0) {
-    HIP_CHECK(hipMemcpyAsync(c->scantab[S], tab, sizeof(tab), hipMemcpyHostToDevice, st));
-    memcpy(c->scantab_h[S], tab, sizeof(tab));
+  // (the temperature has a table of its own)
+  double* tab[9] = {q[0], q[1], q[2], xo[0], xo[1], xo[2], fo[0], fo[1], fo[2]};
+  double** dtab = temp ? c->scantab_t[S] : c->scantab[S];
+  double** htab = temp ? c->scantab_th[S] : c->scantab_h[S];
+  if (memcmp(tab, htab, sizeof(tab)) != 0) {
+    HIP_CHECK(hipMemcpyAsync(dtab, tab, sizeof(tab), hipMemcpyHostToDevice, st));
+    memcpy(htab, tab, sizeof(tab));
   }
-  double* const* dq = c->scantab[S];
-  double* const* dxo = c->scantab[S] + 3;
+  double* const* dq = dtab;
+  double* const* dxo = dtab + 3;
   {
     ProfScope pk(c, "af_scan_q");
     if (S == 1) {
       PtrSet8 qn{};
-      for (int k = 0; k < 3; ++k) qn.p[k] = A.su[k];
-      hipLaunchKernelGGL(k_scan_q1v, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, 3 * (d.nk - 1)),
+      for (int k = 0; k < nc; ++k) qn.p[k] = A.su[k];
+      hipLaunchKernelGGL(k_scan_q1v, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, nc * (d.nk - 1)),
                          dim3(256), 0, st, d, qn, dq);
     }
-    hipLaunchKernelGGL(k_scan_seed<S>, grid1(3L * g.nb * nchunk * g.n, 256), dim3(256), 0, st, d,
-                       A, nchunk, c->bnd[0]);
+    hipLaunchKernelGGL(k_scan_seed<S>, grid1((long)nc * g.nb * nchunk * g.n, 256), dim3(256), 0,
+                       st, d, A, nchunk, c->bnd[0]);
     HIP_CHECK(hipGetLastError());
   }
   const double tol = c->prm.af_tol > 0 ? c->prm.af_tol : 1e-14;
   const int cap = nchunk + 1;
   int slot = 0, it = 0;
-  double* const* lf = c->lf[S];
+  double* const* lf = temp ? c->lft[S] : c->lf[S];
   for (;;) {
     {
       ProfScope pk(c, S == 0 ? "k_scan_pass_i" : "k_scan_pass_j");
@@ -378,8 +384,8 @@ int scan_stage(FvmCtx* c, const AFArgs& A, int* iters) {
   }
   if (S == 1) {
     ProfScope pk(c, "af_scan_unpack");
-    hipLaunchKernelGGL(k_scan_unpack1, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, 3 * (d.nk - 1)),
-                       dim3(256), 0, st, d, (const double* const*)dxo, c->scantab[S] + 6);
+    hipLaunchKernelGGL(k_scan_unpack1, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, nc * (d.nk - 1)),
+                       dim3(256), 0, st, d, (const double* const*)dxo, dtab + 6);
     HIP_CHECK(hipGetLastError());
   }
   *iters = it;
@@ -400,22 +406,24 @@ int fvm_af_scan_setup(FvmCtx* c) {
     if (!c->af_scan[s]) continue;
     const int nchunk = (g.na + SC_C - 1) / SC_C;
     nbnd = std::max(nbnd, 3L * g.nb * nchunk * g.n);
-    if (!c->lf[s][0]) {
-      for (int q = 0; q < 4; ++q) {
-        HIP_CHECK(hipMalloc((void**)&c->lf[s][q], d.n * sizeof(double)));
-        c->allocs.push_back(c->lf[s][q]);
-      }
-    }
     const long nl = (long)g.na * g.nb;
-    if (s == 0)
-      hipLaunchKernelGGL(k_scan_factor<0>, grid1(nl, 64), dim3(64), 0, st, d,
-                         (const double* const*)c->dfa[0], c->ap0, c->lf[0][0], c->lf[0][1],
-                         c->lf[0][2], c->lf[0][3]);
-    else
-      hipLaunchKernelGGL(k_scan_factor<1>, grid1(nl, 64), dim3(64), 0, st, d,
-                         (const double* const*)c->dfa[1], c->ap0, c->lf[1][0], c->lf[1][1],
-                         c->lf[1][2], c->lf[1][3]);
-    HIP_CHECK(hipGetLastError());
+    for (int set = 0; set < (c->thermal ? 2 : 1); ++set) {  // set 1: the temperature's factors
+      double** L = set ? c->lft[s] : c->lf[s];
+      const double* a0 = set ? c->ap0t : c->ap0;
+      if (!L[0]) {
+        for (int q = 0; q < 4; ++q) {
+          HIP_CHECK(hipMalloc((void**)&L[q], d.n * sizeof(double)));
+          c->allocs.push_back(L[q]);
+        }
+      }
+      if (s == 0)
+        hipLaunchKernelGGL(k_scan_factor<0>, grid1(nl, 64), dim3(64), 0, st, d,
+                           (const double* const*)c->dfa[0], a0, L[0], L[1], L[2], L[3]);
+      else
+        hipLaunchKernelGGL(k_scan_factor<1>, grid1(nl, 64), dim3(64), 0, st, d,
+                           (const double* const*)c->dfa[1], a0, L[0], L[1], L[2], L[3]);
+      HIP_CHECK(hipGetLastError());
+    }
   }
   if (nbnd > c->nbnd) {
     for (int q = 0; q < 2; ++q) {
@@ -428,17 +436,18 @@ int fvm_af_scan_setup(FvmCtx* c) {
   return FVM_OK;
 }
 
-int fvm_af_scan_stage(FvmCtx* c, int s, const AFArgs& A, int* iters) {
+int fvm_af_scan_stage(FvmCtx* c, int s, const AFArgs& A, int* iters, bool temp) {
   const ScanGeo g = scan_geo(c->d, s);
   const int R = (g.n + 63) / 64;
+  if (A.nc < 1 || A.nc > 3 || (temp && !c->lft[s][0])) return FVM_ERR_ARG;
   if (s == 0) {
-    if (R <= 3) return scan_stage<0, 3>(c, A, iters);
-    if (R <= 4) return scan_stage<0, 4>(c, A, iters);
-    if (R <= 6) return scan_stage<0, 6>(c, A, iters);
-    return scan_stage<0, SC_R>(c, A, iters);
+    if (R <= 3) return scan_stage<0, 3>(c, A, iters, temp);
+    if (R <= 4) return scan_stage<0, 4>(c, A, iters, temp);
+    if (R <= 6) return scan_stage<0, 6>(c, A, iters, temp);
+    return scan_stage<0, SC_R>(c, A, iters, temp);
   }
-  if (R <= 3) return scan_stage<1, 3>(c, A, iters);
-  if (R <= 4) return scan_stage<1, 4>(c, A, iters);
-  if (R <= 6) return scan_stage<1, 6>(c, A, iters);
-  return scan_stage<1, SC_R>(c, A, iters);
+  if (R <= 3) return scan_stage<1, 3>(c, A, iters, temp);
+  if (R <= 4) return scan_stage<1, 4>(c, A, iters, temp);
+  if (R <= 6) return scan_stage<1, 6>(c, A, iters, temp);
+  return scan_stage<1, SC_R>(c, A, iters, temp);
 }

@@ -42,6 +42,46 @@ static int dalloc(FvmCtx* c, double** p, long n) {
   return FVM_OK;
 }
 
+// Passive temperature: arrays (once), 2 Pe J^-1/dt, SUM_AREA of the wall triangles in the
+// reference's order (init.f90:240-259) and the temperature's line factors.
+int fvm_thermal_setup(FvmCtx* c) {
+  const Dims d = c->d;
+  const long n = d.n;
+  if (!c->t) {
+    int r = 0;
+    double** arr[] = {&c->t, &c->tn, &c->adv, &c->adv1, &c->dift, &c->sgst, &c->that,
+                      &c->dtd[0], &c->dtd[1], &c->dtd[2], &c->kappa, &c->magh, &c->ap0t};
+    for (double** q : arr) r |= dalloc(c, q, n);
+    r |= dalloc(c, &c->tfar, (long)(d.nj + 1) * (d.nk + 1));
+    for (int s = 0; s < 2; ++s) {
+      if (hipMalloc((void**)&c->scantab_t[s], 9 * sizeof(double*)) != hipSuccess) r = FVM_ERR_HIP;
+      else c->allocs.push_back(c->scantab_t[s]);
+    }
+    if (r) return FVM_ERR_HIP;
+    const long nw = (long)(d.nj - 1) * (d.nk - 1);
+    std::vector<double> w(12 * nw);
+    HIP_CHECK(hipMemcpy(w.data(), c->wall, w.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double sa = 0.0;
+    for (long t = 0; t < nw; ++t) sa = sa + w[6 * t + 5] + w[6 * (nw + t) + 5];
+    c->sum_area = sa;
+  }
+  c->thermal = true;
+  int r = fvm_k_set_dtime(c, c->prm.dtime);  // ap0t and every factor set
+  if (r) return r;
+  // the temperature uses the exact K-line stage and the line-scan form of the I and J stages
+  const double ratio = c->prm.re / c->pe;
+  const bool ok = c->prm.af_mode == 0 && c->tk_at && c->af_zero_a[2] && c->af_zero_b[2] &&
+                  c->af_scan[0] && c->af_scan[1] && c->af_b_ratio[0] * ratio <= 1e-15 &&
+                  c->af_b_ratio[1] * ratio <= 1e-15 && c->lft[0][0] && c->lft[1][0];
+  if (!ok) {
+    fprintf(stderr, "fvm_amd: the passive temperature needs an extruded grid (independent K "
+                    "lines, negligible K couplings of the I and J stages) and af_mode 0\n");
+    c->thermal = false;
+    return FVM_ERR_ARG;
+  }
+  return FVM_OK;
+}
+
 static double* field_ptr(FvmCtx* c, int f) {
   switch (f) {
     case FVM_U: return c->u;
@@ -78,6 +118,16 @@ static double* field_ptr(FvmCtx* c, int f) {
     case FVM_UN: return c->un;
     case FVM_VN: return c->vn;
     case FVM_WN: return c->wn;
+    case FVM_T: return c->t;
+    case FVM_ADV: return c->adv;
+    case FVM_DIFT: return c->dift;
+    case FVM_SGST: return c->sgst;
+    case FVM_KAPPA: return c->kappa;
+    case FVM_THAT: return c->that;
+    case FVM_DTDX: return c->dtd[0];
+    case FVM_DTDY: return c->dtd[1];
+    case FVM_DTDZ: return c->dtd[2];
+    case FVM_TN: return c->tn;
     default: return nullptr;
   }
 }
@@ -262,6 +312,9 @@ int fvm_set_field(fvm_ctx* c, int field, const double* host) {
   if (field >= FVM_NLX && field <= FVM_NLZ)
     HIP_CHECK(hipMemcpyAsync(c->nl1[field - FVM_NLX], host, c->d.n * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));
+  if (field == FVM_ADV)
+    HIP_CHECK(hipMemcpyAsync(c->adv1, host, c->d.n * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
   return FVM_OK;
 }
@@ -283,9 +336,22 @@ int fvm_begin_step(fvm_ctx* c) {
   HIP_CHECK(hipMemcpyAsync(c->un, c->u, b, hipMemcpyDeviceToDevice, c->stream));
   HIP_CHECK(hipMemcpyAsync(c->vn, c->v, b, hipMemcpyDeviceToDevice, c->stream));
   HIP_CHECK(hipMemcpyAsync(c->wn, c->w, b, hipMemcpyDeviceToDevice, c->stream));
+  if (c->thermal) {  // T_N = T; BCT's far-field values depend only on T_N
+    HIP_CHECK(hipMemcpyAsync(c->tn, c->t, b, hipMemcpyDeviceToDevice, c->stream));
+    int r = fvm_k_tfar(c);
+    if (r) return r;
+  }
   // far-field boundary values depend only on U_N: one evaluation serves every BCU of the step
   return fvm_k_ufar(c);
 }
+
+int fvm_set_thermal(fvm_ctx* c, double pe) {
+  if (!c || c->two_d || !(pe > 0)) return FVM_ERR_ARG;
+  c->pe = pe;
+  return fvm_thermal_setup(c);
+}
+
+int fvm_iter_t(fvm_ctx* c) { return c ? fvm_k_iter_t(c) : FVM_ERR_ARG; }
 
 int fvm_set_dtime(fvm_ctx* c, double dtime) {
   if (!c || !(dtime > 0)) return FVM_ERR_ARG;
@@ -344,6 +410,15 @@ int fvm_calccoef(fvm_ctx* c, fvm_step_info* info) {
   if (info) {
     info->cfl = o[0]; info->cpb = o[1]; info->cd = o[2]; info->cl = o[3];
     info->cd_p = o[4]; info->cd_v = o[5]; info->cl_p = o[6]; info->cl_v = o[7];
+    // the dynamic coefficients of this step's STRESS_TENSOR_DY and the Nusselt number
+    double sc[2] = {0.0, 0.0};
+    HIP_CHECK(hipMemcpy(sc, c->scal, 2 * sizeof(double), hipMemcpyDeviceToHost));
+    info->c_avg = c->prm.use_les ? sc[0] : 0.0;
+    info->c_t = (c->thermal && c->prm.use_les) ? sc[1] : 0.0;
+    info->nu = 0.0;
+    if (c->thermal && !c->two_d) {
+      HIP_CHECK(hipMemcpy(&info->nu, c->scal + 16, sizeof(double), hipMemcpyDeviceToHost));
+    }
   }
   return FVM_OK;
 }
@@ -371,14 +446,17 @@ int fvm_step(fvm_ctx* c, fvm_step_info* info) {
   if (!c->two_d) CK(fvm_k_pref(c, nullptr));     // :125-135 (none in 2D)
   double* f[3] = {c->u, c->v, c->w};
   CK(fvm_k_bcu_ghosts(c, f, 3));                  // :138-142
+  if (c->thermal) CK(fvm_k_iter_t(c));            // :145-147 (ITER_T, BCT)
   if (c->two_d)
     CK(fvm_k_calccoef2d(c, nullptr, true));       // 2D CALCCOEF
   else
     CK(fvm_k_calccoef(c, nullptr, true));         // :150
 #undef CK
-  HIP_CHECK(hipMemcpyAsync(c->hscal, c->scal, 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->hscal, c->scal, 17 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
   st.c_avg = c->prm.use_les ? c->hscal[0] : 0.0;
+  st.c_t = (c->thermal && c->prm.use_les) ? c->hscal[1] : 0.0;
+  st.nu = c->thermal ? c->hscal[16] : 0.0;
   st.pref = c->hscal[2];
   st.cfl = c->hscal[8]; st.cpb = c->hscal[9]; st.cd = c->hscal[10]; st.cl = c->hscal[11];
   st.cd_p = c->hscal[12]; st.cd_v = c->hscal[13]; st.cl_p = c->hscal[14]; st.cl_v = c->hscal[15];

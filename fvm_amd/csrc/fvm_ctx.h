@@ -92,6 +92,19 @@ struct FvmCtx {
   // 2D solver on a one-cell span slab
   bool two_d = false;
   double* wall2d = nullptr;  // per wall cell J: length, DN1, sin TH2, cos TH2, cos th, sin th
+  // passive temperature (ITER_T, BCT, SGS_T; RLES/iter_t.f90, bct.f90,
+  // stress_tensor_dy.f90:589-738), allocated by fvm_set_thermal
+  bool thermal = false;
+  double pe = 0.0;
+  double *t = nullptr, *tn = nullptr, *adv = nullptr, *adv1 = nullptr, *dift = nullptr;
+  double *sgst = nullptr, *that = nullptr, *dtd[3] = {}, *kappa = nullptr, *magh = nullptr;
+  double *tfar = nullptr;   // far-field boundary value of T, (NJ+1)(NK+1)
+  double *ap0t = nullptr;   // 2 Pe J^-1 / dt
+  double *tk_at = nullptr, *tk_dt = nullptr, *tk_bt = nullptr;  // K-line factors with ap0t
+  double* lft[2][4] = {};   // I/J line-scan factors with ap0t
+  double** scantab_t[2] = {nullptr, nullptr};
+  double* scantab_th[2][9] = {};
+  double sum_area = 0.0;    // SUM_AREA of the wall triangles (init.f90:240-261)
   // last-step diagnostics
   fvm_step_info last{};
   // profiling
@@ -106,6 +119,7 @@ struct FvmCtx {
 // Kernel timing: wraps one launch in a pair of events when profiling is on.
 // arguments of the AF stage kernels
 struct AFArgs {
+  int nc = 3;             // components: 3 (u, v, w) or 1 (T)
   double* f[3];           // stage input / output field per component
   const double* fn[3];    // step-start field U_N per component
   const double* nl[3];
@@ -129,7 +143,7 @@ int fvm_geom_setup(FvmCtx* c);
 int fvm_k_set_dtime(FvmCtx* c, double dtime);
 int fvm_af_setup(FvmCtx* c);
 int fvm_af_scan_setup(FvmCtx* c);
-int fvm_af_scan_stage(FvmCtx* c, int s, const AFArgs& A, int* iters);
+int fvm_af_scan_stage(FvmCtx* c, int s, const AFArgs& A, int* iters, bool temp = false);
 int fvm_ghost_fill(FvmCtx* c, double* a);
 int fvm_ghost_fill_n(FvmCtx* c, double** a, int cnt);
 int fvm_k_contra_vel(FvmCtx* c);
@@ -138,6 +152,10 @@ int fvm_k_dif(FvmCtx* c);
 int fvm_k_p_source(FvmCtx* c);
 int fvm_k_ufar(FvmCtx* c);
 int fvm_k_bcu_ghosts(FvmCtx* c, double** f, int cnt);
+int fvm_k_bct_ghosts(FvmCtx* c, double* f);
+int fvm_k_tfar(FvmCtx* c);
+int fvm_k_iter_t(FvmCtx* c);
+int fvm_thermal_setup(FvmCtx* c);
 int fvm_k_correct(FvmCtx* c, double sign);
 int fvm_k_psource_correct(FvmCtx* c, double sign, bool with_cfl);
 int fvm_k_correct_bcu_contra(FvmCtx* c);

@@ -61,18 +61,21 @@ __device__ inline Faces faces(const Dims& d, int i, long c, const double* __rest
   return f;
 }
 
-// NL(IN,OUT) for the three velocity components (RLES/nl.f90:23-31)
+// fields in and out of the per-component stencils: u_n, v_n, w_n (and T_N when the passive
+// temperature is on)
+struct CompSet {
+  const double* in[4];
+  double* out[4];
+};
+
+// NL(IN,OUT) for the velocity components and, with nc = 4, the temperature (NL(T_N,ADV),
+// RLES/main.f90:58) on the same face fluxes (RLES/nl.f90:23-31)
 __global__ void k_nl(Dims d, const double* __restrict__ ju0, const double* __restrict__ ju1,
-                     const double* __restrict__ ju2, const double* __restrict__ un,
-                     const double* __restrict__ vn, const double* __restrict__ wn, double* nlx,
-                     double* nly, double* nlz) {
+                     const double* __restrict__ ju2, CompSet cs, int nc) {
   INTERIOR_IJK(d);
   Faces F = faces(d, i, c, ju0, ju1, ju2);
-  const double* in[3] = {un, vn, wn};
-  double* out[3] = {nlx, nly, nlz};
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const double* f = in[q];
+  for (int q = 0; q < nc; ++q) {
+    const double* f = cs.in[q];
     double fc = f[c];
     double t = F.ue * (f[c + 1] + fc);
     t = t - F.uw * (f[c - 1] + fc);
@@ -80,7 +83,7 @@ __global__ void k_nl(Dims d, const double* __restrict__ ju0, const double* __res
     t = t - F.vs * (f[c - d.sj] + fc);
     t = t + F.wt * (f[c + d.sk] + fc);
     t = t - F.wb * (f[c - d.sk] + fc);
-    out[q][c] = 0.5 * t;
+    cs.out[q][c] = 0.5 * t;
   }
 }
 
@@ -93,18 +96,15 @@ __global__ void k_nl(Dims d, const double* __restrict__ ju0, const double* __res
 // ride along in registers.
 constexpr int DF_KC = 8;
 
+template <int NC>
 __global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict__ pdI,
                                                 const double* __restrict__ pdJ,
                                                 const double* __restrict__ pdK,
                                                 const double* __restrict__ jq12,
                                                 const double* __restrict__ jq13,
-                                                const double* __restrict__ jq23,
-                                                const double* __restrict__ un,
-                                                const double* __restrict__ vn,
-                                                const double* __restrict__ wn, double* dx,
-                                                double* dy, double* dz) {
+                                                const double* __restrict__ jq23, CompSet cs) {
   const uint3 bb = xcd_block();
-  __shared__ double sh[3][3][SF_P];  // [component][plane slot][tile cell]
+  __shared__ double sh[NC][3][SF_P];  // [component][plane slot][tile cell]
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
   const int i = 1 + bb.x * 64 + tx;
   const int j = 1 + bb.y * 4 + ty;
@@ -113,7 +113,6 @@ __global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict
   const bool act = i <= d.ni - 1 && j <= d.nj - 1;
   const int q0 = (ty + 1) * SF_W + tx + 1;
   const long sj = d.sj;
-  double* out[3] = {dx, dy, dz};
   double j13[3] = {0.0, 0.0, 0.0}, j23[3] = {0.0, 0.0, 0.0}, pk[3] = {0.0, 0.0, 0.0};
   for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
     const int slot = (kp - (k0 - 1)) % 3;
@@ -122,9 +121,8 @@ __global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict
       const int gi = min(bb.x * 64 + q % SF_W, d.ni);
       const int gj = min(bb.y * 4 + q / SF_W, d.nj);
       const long c = cidx(d, gi, gj, kp);
-      sh[0][slot][q] = un[c];
-      sh[1][slot][q] = vn[c];
-      sh[2][slot][q] = wn[c];
+#pragma unroll
+      for (int comp = 0; comp < NC; ++comp) sh[comp][slot][q] = cs.in[comp][c];
     }
     j13[0] = j13[1]; j13[1] = j13[2];
     j23[0] = j23[1]; j23[1] = j23[2];
@@ -145,7 +143,7 @@ __global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict
     const double e12 = jq12[c + 1], w12 = jq12[c - 1], e13 = jq13[c + 1], w13 = jq13[c - 1];
     const double n12 = jq12[c + sj], s12 = jq12[c - sj], n23 = jq23[c + sj], s23 = jq23[c - sj];
 #pragma unroll
-    for (int comp = 0; comp < 3; ++comp) {
+    for (int comp = 0; comp < NC; ++comp) {
 #define F(dk, dj, di) sh[comp][pl[(dk) + 1]][q0 + (dj) * SF_W + (di)]
       const double fc = F(0, 0, 0);
       double t = de * (F(0, 0, 1) - fc);
@@ -168,7 +166,7 @@ __global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict
       r = r - j23[0] * (F(-1, 1, 0) - F(-1, -1, 0));
 #undef F
       t = t + 0.25 * r;
-      out[comp][c] = 1.0 * t;
+      cs.out[comp][c] = 1.0 * t;
     }
   }
 }
@@ -300,18 +298,24 @@ __global__ void k_contra_iplanes(Dims d, const double* const* __restrict__ m,
 // Far-field boundary value U_F per (j,k) and component: inflow Dirichlet UIN where the ghost
 // centre's angle lies in [pi/2, 3pi/2], else the convective outflow condition advanced with
 // the step-start velocity B = U_N (RLES/bcu.f90:27-37).  Constant within a step.
+// (nc components of b with inflow values uin[]: u, v, w with UIN = 1, 0, 0, or the
+// temperature with TIN = 0, bct.f90:31-37)
+struct FarSet {
+  const double* b[3];
+  double uin[3];
+};
+
 __global__ void k_ufar(Dims d, const double* const* __restrict__ m, const double* __restrict__ theta,
-                       const double* __restrict__ un, const double* __restrict__ vn,
-                       const double* __restrict__ wn, double dtime, double pi, double* ufar) {
+                       FarSet fs, int nc, double dtime, double pi, double* ufar) {
   const long nf = (long)(d.nj + 1) * (d.nk + 1);
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < 3 * nf;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nc * nf;
        t += (long)gridDim.x * blockDim.x) {
     int comp = (int)(t / nf);
     long r = t - comp * nf;
     int j = (int)(r % (d.nj + 1)), k = (int)(r / (d.nj + 1));
     if (j < 1 || j > d.nj - 1 || k < 1 || k > d.nk - 1) { ufar[t] = 0.0; continue; }
-    const double* b = comp == 0 ? un : (comp == 1 ? vn : wn);
-    const double uin = comp == 0 ? 1.0 : 0.0;
+    const double* b = fs.b[comp];
+    const double uin = fs.uin[comp];
     long c = cidx(d, d.ni, j, k), mm = c - 1;
     double th = theta[c], uf;
     if (th >= 0.5 * pi && th <= 1.5 * pi) {
@@ -328,9 +332,10 @@ __global__ void k_ufar(Dims d, const double* const* __restrict__ m, const double
   }
 }
 
-// BCU ghost values (RLES/bcu.f90:21-22, 41-42): wall no-slip mirror, far-field mirror about
-// U_F; the periodic J/K ghosts follow (fvm_ghost_fill).
-__global__ void k_bcu_ghosts(Dims d, PtrSet8 set, const double* __restrict__ ufar) {
+// BCU ghost values (RLES/bcu.f90:21-22, 41-42): wall mirror about U_F = 0 (no slip), far-field
+// mirror about the far-field value; BCT's (bct.f90:22, 45) with the wall value T_WALL; the
+// periodic J/K ghosts follow (fvm_ghost_fill).
+__global__ void k_bcu_ghosts(Dims d, PtrSet8 set, const double* __restrict__ ufar, double wall) {
   const long nf = (long)(d.nj - 1) * (d.nk - 1);
   double* __restrict__ f = set.p[blockIdx.y];
   const double* __restrict__ uf = ufar + blockIdx.y * (long)(d.nj + 1) * (d.nk + 1);
@@ -338,7 +343,7 @@ __global__ void k_bcu_ghosts(Dims d, PtrSet8 set, const double* __restrict__ ufa
        t += (long)gridDim.x * blockDim.x) {
     int j = 1 + (int)(t % (d.nj - 1)), k = 1 + (int)(t / (d.nj - 1));
     long c0 = cidx(d, 0, j, k), cn = cidx(d, d.ni, j, k);
-    f[c0] = -f[c0 + 1] + 2.0 * 0.0;
+    f[c0] = -f[c0 + 1] + 2.0 * wall;
     f[cn] = -f[cn - 1] + 2.0 * uf[j + (long)(d.nj + 1) * k];
   }
 }
@@ -431,13 +436,16 @@ __global__ void k_cfl(Dims d, const double* const* __restrict__ m, const double*
 }
 
 // surface-force partial sums over the wall triangles (calccoef.f90:76-183): one element per
-// lane, a fixed-order tree per block; part[4 b + q] for pressure drag / lift, viscous drag / lift
+// lane, a fixed-order tree per block; part[5 b + q] for pressure drag / lift, viscous drag /
+// lift and the wall heat flux sum (T - T_WALL)/D_N AREA of the Nusselt number (calccoef.f90:117-119,
+// 155-157; zero without the temperature, T = nullptr)
 __global__ void k_forces_part(Dims d, const double* __restrict__ wall, const double* __restrict__ p,
                               const double* __restrict__ u, const double* __restrict__ v,
-                              const double* __restrict__ w, double re, double* part) {
+                              const double* __restrict__ w, const double* __restrict__ T,
+                              double re, double* part) {
   __shared__ double sh[256];
   const long nw = (long)(d.nj - 1) * (d.nk - 1);
-  double pd = 0.0, ply = 0.0, td = 0.0, tly = 0.0;
+  double pd = 0.0, ply = 0.0, td = 0.0, tly = 0.0, snu = 0.0;
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t < 2 * nw) {
     int tri = (int)(t / nw);
@@ -456,29 +464,38 @@ __global__ void k_forces_part(Dims d, const double* __restrict__ wall, const dou
     ply = -2.0 * p[c] * ar * B / dum;
     td = (2.0 / re) * dudn * ar * U / um;
     tly = (2.0 / re) * dudn * ar * V / um;
+    if (T) {
+      const double dtdn = (T[c] - 1.0) / dn;  // T_WALL = 1
+      snu = dtdn * ar;
+    }
   }
   pd = block_reduce_sum(pd, sh);
   ply = block_reduce_sum(ply, sh);
   td = block_reduce_sum(td, sh);
   tly = block_reduce_sum(tly, sh);
+  snu = block_reduce_sum(snu, sh);
   if (threadIdx.x == 0) {
-    part[4 * blockIdx.x] = pd; part[4 * blockIdx.x + 1] = ply;
-    part[4 * blockIdx.x + 2] = td; part[4 * blockIdx.x + 3] = tly;
+    part[5 * blockIdx.x] = pd; part[5 * blockIdx.x + 1] = ply;
+    part[5 * blockIdx.x + 2] = td; part[5 * blockIdx.x + 3] = tly;
+    part[5 * blockIdx.x + 4] = snu;
   }
 }
 
-// force coefficients, base pressure and CFL maximum (calccoef.f90:11-183), one block
+// force coefficients, base pressure, CFL maximum and Nusselt number NU = -sum / SUM_AREA
+// (calccoef.f90:11-184), one block; out[0..7] as fvm_step_info, out[8] = NU
 __global__ void k_forces_final(Dims d, const double* __restrict__ p,
                                const double* __restrict__ fpart, int nfb, double ref_a,
-                               const double* __restrict__ cflpart, int ncfl, double* out) {
+                               double sum_area, const double* __restrict__ cflpart, int ncfl,
+                               double* out) {
   __shared__ double sh[256];
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int b = threadIdx.x; b < nfb; b += blockDim.x)
-    for (int q = 0; q < 4; ++q) acc[q] = acc[q] + fpart[4 * b + q];
+    for (int q = 0; q < 5; ++q) acc[q] = acc[q] + fpart[5 * b + q];
   const double pd = block_reduce_sum(acc[0], sh);
   const double ply = block_reduce_sum(acc[1], sh);
   const double td = block_reduce_sum(acc[2], sh);
   const double tly = block_reduce_sum(acc[3], sh);
+  const double snu = block_reduce_sum(acc[4], sh);
   double cm[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // eight loads in flight per lane
   int q = threadIdx.x;
   for (; q + 7 * (int)blockDim.x < ncfl; q += 8 * blockDim.x) {
@@ -519,6 +536,7 @@ __global__ void k_forces_final(Dims d, const double* __restrict__ p,
     out[5] = cdv;
     out[6] = clp;
     out[7] = clv;
+    out[8] = sum_area > 0.0 ? -snu / sum_area : 0.0;
   }
 }
 
@@ -582,9 +600,11 @@ int fvm_k_contra_vel(FvmCtx* c) {
 
 int fvm_k_nl(FvmCtx* c) {
   for (int q = 0; q < 3; ++q) std::swap(c->nl[q], c->nl1[q]);
+  if (c->thermal) std::swap(c->adv, c->adv1);
   ProfScope ps(c, "nl");
+  const CompSet cs = {{c->un, c->vn, c->wn, c->tn}, {c->nl[0], c->nl[1], c->nl[2], c->adv}};
   hipLaunchKernelGGL(k_nl, interior_grid(c->d), kBlk, 0, c->stream, c->d, c->ju[0], c->ju[1],
-                     c->ju[2], c->un, c->vn, c->wn, c->nl[0], c->nl[1], c->nl[2]);
+                     c->ju[2], cs, c->thermal ? 4 : 3);
   HIP_CHECK(hipGetLastError());
   return FVM_OK;
 }
@@ -593,9 +613,13 @@ int fvm_k_dif(FvmCtx* c) {
   ProfScope ps(c, "dif");
   const Dims d = c->d;
   const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + DF_KC - 1) / DF_KC);
-  hipLaunchKernelGGL(k_dif_km, g, kBlk, 0, c->stream, c->d, c->pdI, c->pdJ,
-                     c->pdK, c->jq12, c->jq13, c->jq23, c->un, c->vn, c->wn, c->dif[0],
-                     c->dif[1], c->dif[2]);
+  const CompSet cs = {{c->un, c->vn, c->wn, c->tn}, {c->dif[0], c->dif[1], c->dif[2], c->dift}};
+  if (c->thermal)  // DIF(T_N, DIFT), RLES/main.f90:75
+    hipLaunchKernelGGL(k_dif_km<4>, g, kBlk, 0, c->stream, c->d, c->pdI, c->pdJ, c->pdK,
+                       c->jq12, c->jq13, c->jq23, cs);
+  else
+    hipLaunchKernelGGL(k_dif_km<3>, g, kBlk, 0, c->stream, c->d, c->pdI, c->pdJ, c->pdK,
+                       c->jq12, c->jq13, c->jq23, cs);
   HIP_CHECK(hipGetLastError());
   return FVM_OK;
 }
@@ -615,11 +639,33 @@ int fvm_k_p_source(FvmCtx* c) {
 int fvm_k_ufar(FvmCtx* c) {
   double** m = c->dm;
   long nf = 3L * (c->d.nj + 1) * (c->d.nk + 1);
+  const FarSet fs = {{c->un, c->vn, c->wn}, {1.0, 0.0, 0.0}};
   hipLaunchKernelGGL(k_ufar, grid1(nf, 256), dim3(256), 0, c->stream, c->d,
-                     (const double* const*)m, c->theta, c->un, c->vn, c->wn, c->prm.dtime,
-                     c->pi, c->ufar);
+                     (const double* const*)m, c->theta, fs, 3, c->prm.dtime, c->pi, c->ufar);
   HIP_CHECK(hipGetLastError());
   return FVM_OK;
+}
+
+// far-field values of T for BCT from T_N (bct.f90:28-37)
+int fvm_k_tfar(FvmCtx* c) {
+  long nf = (long)(c->d.nj + 1) * (c->d.nk + 1);
+  const FarSet fs = {{c->tn, nullptr, nullptr}, {0.0, 0.0, 0.0}};
+  hipLaunchKernelGGL(k_ufar, grid1(nf, 256), dim3(256), 0, c->stream, c->d,
+                     (const double* const*)c->dm, c->theta, fs, 1, c->prm.dtime, c->pi, c->tfar);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+// BCT ghosts of one temperature array (wall T_WALL = 1, far field from tfar) and MPI_X
+int fvm_k_bct_ghosts(FvmCtx* c, double* f) {
+  long nf = (long)(c->d.nj - 1) * (c->d.nk - 1);
+  ProfScope ps(c, "bct");
+  PtrSet8 set{};
+  set.p[0] = f;
+  hipLaunchKernelGGL(k_bcu_ghosts, dim3(grid1(nf, 256).x, 1), dim3(256), 0, c->stream, c->d, set,
+                     c->tfar, 1.0);
+  HIP_CHECK(hipGetLastError());
+  return fvm_ghost_fill(c, f);
 }
 
 // f[q] takes the far-field values of component q (cnt <= 3)
@@ -630,7 +676,7 @@ int fvm_k_bcu_ghosts(FvmCtx* c, double** f, int cnt) {
   PtrSet8 set{};
   for (int q = 0; q < cnt; ++q) set.p[q] = f[q];
   hipLaunchKernelGGL(k_bcu_ghosts, dim3(grid1(nf, 256).x, cnt), dim3(256), 0, c->stream, c->d,
-                     set, c->ufar);
+                     set, c->ufar, 0.0);
   HIP_CHECK(hipGetLastError());
   return fvm_ghost_fill_n(c, f, cnt);
 }
@@ -669,11 +715,11 @@ int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready) {
                        c->u, c->v, c->w, c->prm.dtime, c->red);
   const long nw2 = 2L * (c->d.nj - 1) * (c->d.nk - 1);
   const int nfb = (int)((nw2 + 255) / 256);
-  if (off + 4L * nfb > c->nred) return FVM_ERR_SIZE;
+  if (off + 5L * nfb > c->nred) return FVM_ERR_SIZE;
   hipLaunchKernelGGL(k_forces_part, dim3(nfb), dim3(256), 0, c->stream, c->d, c->wall, c->p,
-                     c->u, c->v, c->w, c->prm.re, c->red + off);
+                     c->u, c->v, c->w, c->thermal ? c->t : nullptr, c->prm.re, c->red + off);
   hipLaunchKernelGGL(k_forces_final, dim3(1), dim3(256), 0, c->stream, c->d, c->p, c->red + off,
-                     nfb, c->prm.ref_a, cfl, ncfl, c->scal + 8);
+                     nfb, c->prm.ref_a, c->thermal ? c->sum_area : 0.0, cfl, ncfl, c->scal + 8);
   HIP_CHECK(hipGetLastError());
   if (out8) {
     HIP_CHECK(hipMemcpyAsync(out8, c->scal + 8, 8 * sizeof(double), hipMemcpyDeviceToHost,

@@ -429,6 +429,9 @@ int fvm_k_set_dtime(FvmCtx* c, double dtime) {
   c->prm.dtime = dtime;
   hipLaunchKernelGGL(k_set_ap0, grid1(c->d.n, 256), dim3(256), 0, c->stream, c->d.n, c->djac,
                      c->prm.re, dtime, c->ap0);
+  if (c->thermal)  // 2 PE DJAC/DTIME of ITER_T (iter_t.f90:31)
+    hipLaunchKernelGGL(k_set_ap0, grid1(c->d.n, 256), dim3(256), 0, c->stream, c->d.n, c->djac,
+                       c->pe, dtime, c->ap0t);
   HIP_CHECK(hipGetLastError());
   return fvm_af_setup(c);
 }

@@ -63,7 +63,8 @@ __device__ inline void strain_store(const double* const* __restrict__ m, long c,
 __global__ void k_strain_iplanes(Dims d, const double* const* __restrict__ m,
                                  const double* __restrict__ U, const double* __restrict__ V,
                                  const double* __restrict__ W, double* const* __restrict__ L,
-                                 double* mag) {
+                                 double* mag, const double* __restrict__ T, double* dtx,
+                                 double* dty, double* dtz) {
   const long nl = (long)(d.nj - 1) * (d.nk - 1);
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= 2 * nl) return;
@@ -80,6 +81,12 @@ __global__ void k_strain_iplanes(Dims d, const double* const* __restrict__ m,
     return q;
   };
   strain_store(m, c, true, diff(U), diff(V), diff(W), L, mag);
+  if (T) {  // one-sided temperature gradient (stress_tensor_dy.f90:138-146, 188-196)
+    const Diff3 g = diff(T);
+    dtx[c] = 0.5 * (2.0 * m[0][c] * g.i + m[1][c] * g.j + m[2][c] * g.k);
+    dty[c] = 0.5 * (2.0 * m[3][c] * g.i + m[4][c] * g.j + m[5][c] * g.k);
+    dtz[c] = 0.5 * (2.0 * m[6][c] * g.i + m[7][c] * g.j + m[8][c] * g.k);
+  }
 }
 
 // Strain on the interior cells fused with the (1,2,1)^3/64 test filter of u, v, w
@@ -88,16 +95,22 @@ __global__ void k_strain_iplanes(Dims d, const double* const* __restrict__ m,
 // in LDS, so every velocity value is fetched from HBM once per column.  The filter sums its 27
 // terms in the reference's order: K-1 plane, K plane, K+1 plane; rows J-1, J, J+1; columns
 // I-1, I, I+1.
+// With NF = 4 the temperature rides along: its gradient dT/dx_i = 0.5 (xi_i dT/dxi +
+// eta_i dT/deta + zeta_i dT/dzeta) (stress_tensor_dy.f90:78-94) and its test filter T^.
 constexpr int SF_KC = 8;
 
+struct SfOut {
+  const double* in[4];  // u, v, w, T
+  double* hat[4];       // u^, v^, w^, T^
+  double* dt[3];        // dT/dx, dT/dy, dT/dz
+};
+
+template <int NF>
 __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* const* __restrict__ m,
-                                                       const double* __restrict__ U,
-                                                       const double* __restrict__ V,
-                                                       const double* __restrict__ W,
-                                                       double* const* __restrict__ L, double* mag,
-                                                       double* uh, double* vh, double* wh) {
+                                                       SfOut io, double* const* __restrict__ L,
+                                                       double* mag) {
   const uint3 bb = xcd_block();
-  __shared__ double sh[3][3][SF_P];  // [component][plane slot][tile cell]
+  __shared__ double sh[NF][3][SF_P];  // [field][plane slot][tile cell]
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
   const int i = 1 + bb.x * 64 + tx;
   const int j = 1 + bb.y * 4 + ty;
@@ -105,7 +118,6 @@ __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* con
   const int k1 = min(k0 + SF_KC - 1, d.nk - 1);
   const bool act = i <= d.ni - 1 && j <= d.nj - 1;
   const int q0 = (ty + 1) * SF_W + tx + 1;
-  double* out[3] = {uh, vh, wh};
   for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
     const int slot = (kp - (k0 - 1)) % 3;
     __syncthreads();  // plane kp - 3, held in this slot, is no longer read
@@ -113,9 +125,8 @@ __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* con
       const int gi = min(bb.x * 64 + q % SF_W, d.ni);
       const int gj = min(bb.y * 4 + q / SF_W, d.nj);
       const long c = cidx(d, gi, gj, kp);
-      sh[0][slot][q] = U[c];
-      sh[1][slot][q] = V[c];
-      sh[2][slot][q] = W[c];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) sh[f][slot][q] = io.in[f][c];
     }
     __syncthreads();
     if (kp < k0 + 1 || !act) continue;
@@ -131,9 +142,20 @@ __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* con
       g[f].k = sh[f][sp][q0] - sh[f][sm][q0];
     }
     strain_store(m, c, false, g[0], g[1], g[2], L, mag);
+    if (NF == 4) {
+      const double ti = sh[3][s0][q0 + 1] - sh[3][s0][q0 - 1];
+      const double tj = sh[3][s0][q0 + SF_W] - sh[3][s0][q0 - SF_W];
+      const double tk = sh[3][sp][q0] - sh[3][sm][q0];
+      double t = m[0][c] * ti + m[1][c] * tj + m[2][c] * tk;
+      io.dt[0][c] = 0.5 * t;
+      t = m[3][c] * ti + m[4][c] * tj + m[5][c] * tk;
+      io.dt[1][c] = 0.5 * t;
+      t = m[6][c] * ti + m[7][c] * tj + m[8][c] * tk;
+      io.dt[2][c] = 0.5 * t;
+    }
     const int pl[3] = {sm, s0, sp};
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
+    for (int f = 0; f < NF; ++f) {
       double t = 0.0;
       bool first = true;
 #pragma unroll
@@ -148,7 +170,7 @@ __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* con
             t = first ? term : t + term;
             first = false;
           }
-      out[f][c] = t / 64.0;
+      io.hat[f][c] = t / 64.0;
     }
   }
 }
@@ -161,6 +183,7 @@ __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* con
 // is the reference's 27-point filter with its terms grouped by plane (rounding-level
 // differences only).
 constexpr int LM_KC = 8;
+constexpr int SD_KC = 8;  // planes per workgroup of the K-marching SGS divergences
 constexpr int LM_W = 66, LM_H = 6;  // tile with halo
 
 __global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restrict__ m,
@@ -173,7 +196,7 @@ __global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restr
                                             const double* const* __restrict__ L,
                                             const double* __restrict__ mag,
                                             const double* __restrict__ filt2, double* lm_out,
-                                            double* mm_out) {
+                                            double* mm_out, double* magh_out) {
   const uint3 bb = xcd_block();
   __shared__ double sh[10][LM_W * LM_H];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
@@ -254,6 +277,7 @@ __global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restr
     const double dum = 2.0 * (s11 * s11 + s12 * s12 + s13 * s13 + s12 * s12 + s22 * s22 +
                               s23 * s23 + s13 * s13 + s23 * s23 + s33 * s33);
     const double mh = sqrt(dum);
+    if (magh_out) magh_out[c] = mh;
     const double fw2 = filt2[c];
     const double uh = UH[c], vh = VH[c], wh = WH[c];
     double F[12];
@@ -282,6 +306,163 @@ __global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restr
     }
     lm_out[c] = lm;
     mm_out[c] = mm;
+  }
+}
+
+// Germano identity of the subgrid heat flux per interior cell (stress_tensor_dy.f90:589-690):
+// P_i = -(u_i T)^ + u_i^ T^, R_i = Delta^2 (4 |S^| dT^/dx_i - (|S| dT/dx_i)^), with
+// dT^/dx_i = 0.5 (xi_i (T^(I+1) - T^(I-1)) + eta_i (..J..) + zeta_i (..K..)), and the sums
+// PR = P_1 R_1 + P_2 R_2 + P_3 R_3, RR = R_1^2 + R_2^2 + R_3^2.  Same structure as k_lm: the six
+// filtered products are never stored; each plane of u, v, w, T, |S|, dT/dx_i is staged in LDS
+// (one-cell halo), its in-plane (1,2,1)x(1,2,1) sums are formed once, and three consecutive
+// planes are combined with (1,2,1) weights.
+__global__ void __launch_bounds__(256) k_lm_t(Dims d, const double* const* __restrict__ m,
+                                              const double* __restrict__ U,
+                                              const double* __restrict__ V,
+                                              const double* __restrict__ W,
+                                              const double* __restrict__ T,
+                                              const double* __restrict__ UH,
+                                              const double* __restrict__ VH,
+                                              const double* __restrict__ WH,
+                                              const double* __restrict__ TH,
+                                              const double* __restrict__ mag,
+                                              const double* __restrict__ DX,
+                                              const double* __restrict__ DY,
+                                              const double* __restrict__ DZ,
+                                              const double* __restrict__ magh,
+                                              const double* __restrict__ filt2, double* lm_out,
+                                              double* mm_out) {
+  const uint3 bb = xcd_block();
+  __shared__ double sh[8][LM_W * LM_H];
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+  const int i = 1 + bb.x * 64 + tx;
+  const int j = 1 + bb.y * 4 + ty;
+  const int k0 = 1 + bb.z * LM_KC;
+  const int k1 = min(k0 + LM_KC - 1, d.nk - 1);
+  const bool act = i <= d.ni - 1 && j <= d.nj - 1;
+  // in-plane sums of uT vT wT |S|dT/dx |S|dT/dy |S|dT/dz for planes k-1, k, k+1
+  double Pa[6], Pb[6], Pc[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) Pa[q] = Pb[q] = Pc[q] = 0.0;
+  for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
+    __syncthreads();
+    for (int q = tid; q < LM_W * LM_H; q += 256) {
+      const int gi = min(bb.x * 64 + q % LM_W, d.ni);
+      const int gj = min(bb.y * 4 + q / LM_W, d.nj);
+      const long c = cidx(d, gi, gj, kp);
+      sh[0][q] = U[c]; sh[1][q] = V[c]; sh[2][q] = W[c]; sh[3][q] = T[c];
+      sh[4][q] = mag[c]; sh[5][q] = DX[c]; sh[6][q] = DY[c]; sh[7][q] = DZ[c];
+    }
+    __syncthreads();
+    double Pn[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Pn[q] = 0.0;
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int di = -1; di <= 1; ++di) {
+        const double wgt = (double)((di == 0 ? 2 : 1) * (dj == 0 ? 2 : 1));
+        const int q = (ty + 1 + dj) * LM_W + tx + 1 + di;
+        const double t = sh[3][q], mg = sh[4][q];
+        Pn[0] += wgt * (sh[0][q] * t);
+        Pn[1] += wgt * (sh[1][q] * t);
+        Pn[2] += wgt * (sh[2][q] * t);
+        Pn[3] += wgt * (mg * sh[5][q]);
+        Pn[4] += wgt * (mg * sh[6][q]);
+        Pn[5] += wgt * (mg * sh[7][q]);
+      }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) { Pa[q] = Pb[q]; Pb[q] = Pc[q]; Pc[q] = Pn[q]; }
+    if (kp < k0 + 1 || !act) continue;
+    const int k = kp - 1;
+    const long c = cidx(d, i, j, k);
+    const long sj = d.sj, sk = d.sk;
+    const double gi_ = TH[c + 1] - TH[c - 1], gj_ = TH[c + sj] - TH[c - sj],
+                 gk_ = TH[c + sk] - TH[c - sk];
+    const double th = TH[c], mh = magh[c], fw2 = filt2[c];
+    const double hat[3] = {UH[c], VH[c], WH[c]};
+    double lm = 0.0, mm = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double Fp = (Pa[q] + 2.0 * Pb[q] + Pc[q]) / 64.0;
+      const double Fr = (Pa[3 + q] + 2.0 * Pb[3 + q] + Pc[3 + q]) / 64.0;
+      const double Lq = -Fp + hat[q] * th;
+      const double g = m[3 * q][c] * gi_ + m[3 * q + 1][c] * gj_ + m[3 * q + 2][c] * gk_;
+      const double M = fw2 * (4.0 * mh * 0.5 * g - Fr);
+      if (q == 0) {
+        lm = Lq * M;
+        mm = M * M;
+      } else {
+        lm = lm + Lq * M;
+        mm = mm + M * M;
+      }
+    }
+    lm_out[c] = lm;
+    mm_out[c] = mm;
+  }
+}
+
+// divergence of the subgrid heat flux -2 kappa dT/dx_i (stress_tensor_dy.f90:719-736), added to
+// ADV with the inverse Jacobian (RLES/main.f90:67-68).  K-marching like k_sgs_div with the three
+// products (2 kappa) dT/dx_i.
+__global__ void __launch_bounds__(256) k_sgs_t_div(Dims d, const double* const* __restrict__ m,
+                                                   const double* __restrict__ kappa,
+                                                   const double* __restrict__ DX,
+                                                   const double* __restrict__ DY,
+                                                   const double* __restrict__ DZ,
+                                                   const double* __restrict__ djac, double* adv,
+                                                   double* sgst) {
+  const uint3 bb = xcd_block();
+  __shared__ double sh[2][3][SF_P];  // [plane slot][component][tile cell]
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+  const int i = 1 + bb.x * 64 + tx;
+  const int j = 1 + bb.y * 4 + ty;
+  const int k0 = 1 + bb.z * SD_KC;
+  const int k1 = min(k0 + SD_KC - 1, d.nk - 1);
+  const bool act = i <= d.ni - 1 && j <= d.nj - 1;
+  const int e0 = (ty + 1) * SF_W + tx + 1;
+  double Pm[3] = {0.0, 0.0, 0.0};  // own-column products of plane k-1
+  for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
+    const int slot = (kp - (k0 - 1)) & 1;
+    __syncthreads();  // plane kp - 2, held in this slot, is no longer read
+    for (int e = tid; e < SF_P; e += 256) {
+      const int gi = min(bb.x * 64 + e % SF_W, d.ni);
+      const int gj = min(bb.y * 4 + e / SF_W, d.nj);
+      const long c = cidx(d, gi, gj, kp);
+      const double tk = 2.0 * kappa[c];
+      sh[slot][0][e] = tk * DX[c];
+      sh[slot][1][e] = tk * DY[c];
+      sh[slot][2][e] = tk * DZ[c];
+    }
+    __syncthreads();
+    if (kp == k0 - 1) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) Pm[q] = sh[slot][q][e0];
+      continue;
+    }
+    if (kp == k0 || !act) continue;
+    const int sc = slot ^ 1;
+    const int k = kp - 1;
+    const long c = cidx(d, i, j, k);
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+#pragma unroll
+      for (int dir = 0; dir < 3; ++dir) {
+        double pp, pn;
+        if (dir == 0) { pp = sh[sc][q][e0 + 1]; pn = sh[sc][q][e0 - 1]; }
+        else if (dir == 1) { pp = sh[sc][q][e0 + SF_W]; pn = sh[sc][q][e0 - SF_W]; }
+        else { pp = sh[slot][q][e0]; pn = Pm[q]; }
+        const double inner = -pp + pn;
+        const double term = m[3 * q + dir][c] * inner;
+        t = (q == 0 && dir == 0) ? term : t + term;
+      }
+    }
+    const double sg = 0.5 * t;
+    sgst[c] = sg;
+    adv[c] = adv[c] + sg * djac[c];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) Pm[q] = sh[sc][q][e0];
   }
 }
 
@@ -351,8 +532,6 @@ __global__ void k_nut(Dims d, const double* __restrict__ cavg, const double* __r
 // slots (the I and J differences of plane k and the own-column value of plane k+1), and the
 // own-column products of plane k-1 are kept in registers.  Differences, products and sums are
 // the reference's, in its order.
-constexpr int SD_KC = 8;
-
 __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __restrict__ m,
                                                  const double* __restrict__ nut,
                                                  const double* const* __restrict__ L,
@@ -437,20 +616,32 @@ int fvm_k_sgs(FvmCtx* c) {
   double** m = c->dm;
   ProfScope ps(c, "sgs");
   hipStream_t st = c->stream;
+  const bool th = c->thermal;
   {
     ProfScope pk(c, "k_strain_filter");
     const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + SF_KC - 1) / SF_KC);
-    hipLaunchKernelGGL(k_strain_filter, g, kBlk, 0, st, d, (const double* const*)m, c->u, c->v,
-                       c->w, (double* const*)c->dlss, c->mag, c->uh[0], c->uh[1], c->uh[2]);
+    const SfOut io = {{c->u, c->v, c->w, c->t}, {c->uh[0], c->uh[1], c->uh[2], c->that},
+                      {c->dtd[0], c->dtd[1], c->dtd[2]}};
+    if (th)
+      hipLaunchKernelGGL(k_strain_filter<4>, g, kBlk, 0, st, d, (const double* const*)m, io,
+                         (double* const*)c->dlss, c->mag);
+    else
+      hipLaunchKernelGGL(k_strain_filter<3>, g, kBlk, 0, st, d, (const double* const*)m, io,
+                         (double* const*)c->dlss, c->mag);
     const long nb = 2L * (d.nj - 1) * (d.nk - 1);
     hipLaunchKernelGGL(k_strain_iplanes, grid1(nb, 256), dim3(256), 0, st, d,
                        (const double* const*)m, c->u, c->v, c->w, (double* const*)c->dlss,
-                       c->mag);
+                       c->mag, th ? c->t : nullptr, c->dtd[0], c->dtd[1], c->dtd[2]);
     HIP_CHECK(hipGetLastError());
   }
-  double* gl[7] = {c->lss[0], c->lss[1], c->lss[2], c->lss[3], c->lss[4], c->lss[5], c->mag};
-  int r = fvm_ghost_fill_n(c, gl, 7);
+  double* gl[10] = {c->lss[0], c->lss[1], c->lss[2], c->lss[3], c->lss[4], c->lss[5], c->mag,
+                    c->dtd[0], c->dtd[1], c->dtd[2]};
+  int r = fvm_ghost_fill_n(c, gl, th ? 10 : 7);
   if (r) return r;
+  if (th) {  // BCT(T_HAT, T_N) (stress_tensor_dy.f90:233)
+    r = fvm_k_bct_ghosts(c, c->that);
+    if (r) return r;
+  }
   r = fvm_k_bcu_ghosts(c, c->uh, 3);
   if (r) return r;
   {
@@ -458,7 +649,7 @@ int fvm_k_sgs(FvmCtx* c) {
     const dim3 gl((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + LM_KC - 1) / LM_KC);
     hipLaunchKernelGGL(k_lm, gl, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
                        c->uh[0], c->uh[1], c->uh[2], (const double* const*)c->dlss, c->mag,
-                       c->filt2, c->su[0], c->su[1]);
+                       c->filt2, c->su[0], c->su[1], th ? c->magh : nullptr);
     HIP_CHECK(hipGetLastError());
   }
   const int nb = 512;
@@ -472,6 +663,21 @@ int fvm_k_sgs(FvmCtx* c) {
     hipLaunchKernelGGL(k_sgs_div, gs, kBlk, 0, st, d, (const double* const*)m, c->nut,
                        (const double* const*)c->dlss, c->djac, c->nl[0], c->nl[1], c->nl[2],
                        c->sgs[0], c->sgs[1], c->sgs[2]);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (th) {  // the temperature's dynamic model (stress_tensor_dy.f90:589-738), main.f90:67-68
+    ProfScope pk(c, "sgs_t");
+    const dim3 gl((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + LM_KC - 1) / LM_KC);
+    hipLaunchKernelGGL(k_lm_t, gl, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
+                       c->t, c->uh[0], c->uh[1], c->uh[2], c->that, c->mag, c->dtd[0],
+                       c->dtd[1], c->dtd[2], c->magh, c->filt2, c->su[0], c->su[1]);
+    hipLaunchKernelGGL(k_lm_reduce, dim3(nb), dim3(256), 0, st, d, c->su[0], c->su[1], c->red);
+    hipLaunchKernelGGL(k_lm_final, dim3(1), dim3(256), 0, st, c->red, nb, c->scal + 1);
+    hipLaunchKernelGGL(k_nut, grid1(d.n, 256), dim3(256), 0, st, d, c->scal + 1, c->filt2,
+                       c->mag, c->kappa);
+    const dim3 gs((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + SD_KC - 1) / SD_KC);
+    hipLaunchKernelGGL(k_sgs_t_div, gs, kBlk, 0, st, d, (const double* const*)m, c->kappa,
+                       c->dtd[0], c->dtd[1], c->dtd[2], c->djac, c->adv, c->sgst);
     HIP_CHECK(hipGetLastError());
   }
   return FVM_OK;

@@ -63,6 +63,13 @@ class Solver3D:
         for n, a in zip(("u", "v", "w", "p", "nlx", "nly", "nlz"), (u, v, w, p, nlx, nly, nlz)):
             self.set(n, a)
 
+    def set_temperature(self, t, adv, pe=2769.0):
+        """Switch on the passive temperature (ITER_T, BCT, SGS_T, Nusselt number) with Peclet
+        number pe and start it from T and its convective term ADV."""
+        capi.check(self.L.fvm_set_thermal(self.h, float(pe)), "fvm_set_thermal")
+        self.set("t", t)
+        self.set("adv", adv)
+
     # the time step ----------------------------------------------------------------------
     def step(self):
         info = capi.StepInfo()
@@ -88,6 +95,7 @@ class Solver3D:
     def dif(self): capi.check(self.L.fvm_dif(self.h), "dif")
     def p_source(self): capi.check(self.L.fvm_p_source(self.h), "p_source")
     def iter_u(self): capi.check(self.L.fvm_iter_u(self.h), "iter_u")
+    def iter_t(self): capi.check(self.L.fvm_iter_t(self.h), "iter_t")
     def correct(self, sign): capi.check(self.L.fvm_correct(self.h, float(sign)), "correct")
 
     def stress_tensor_dy(self):

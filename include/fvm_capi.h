@@ -57,6 +57,10 @@ typedef struct fvm_step_info {
   double c_avg;         /* dynamic Smagorinsky coefficient C = <LM>/<2MM>                */
   double cfl, cpb, cd, cl, cd_p, cd_v, cl_p, cl_v;
   int af_iters[3];      /* passes of the three AF stage sweeps                          */
+  double nu;            /* Nusselt number NU (calccoef.f90:117-184); 0 without the
+                           passive temperature                                           */
+  double c_t;           /* dynamic coefficient of the subgrid heat flux
+                           (stress_tensor_dy.f90:686-711); 0 without the temperature     */
 } fvm_step_info;
 
 /* Field selectors for fvm_set_field / fvm_get_field. */
@@ -66,7 +70,11 @@ enum {
   FVM_DJAC = 13, FVM_XIX = 14, FVM_ETX = 15, FVM_ZTX = 16, FVM_XIY = 17, FVM_ETY = 18,
   FVM_ZTY = 19, FVM_XIZ = 20, FVM_ETZ = 21, FVM_ZTZ = 22, FVM_X = 23, FVM_Y = 24, FVM_Z = 25,
   FVM_THETA = 26, FVM_NUT = 27, FVM_SGSX = 28, FVM_SGSY = 29, FVM_SGSZ = 30, FVM_UN = 31,
-  FVM_VN = 32, FVM_WN = 33, FVM_NFIELDS = 34
+  FVM_VN = 32, FVM_WN = 33,
+  /* passive temperature (after fvm_set_thermal): T, its convective term ADV (setting it sets
+     ADV_NM1 too), DIFT, SGS_T, the eddy conductivity, the test-filtered T^, dT/dx_i, T_N */
+  FVM_T = 34, FVM_ADV = 35, FVM_DIFT = 36, FVM_SGST = 37, FVM_KAPPA = 38, FVM_THAT = 39,
+  FVM_DTDX = 40, FVM_DTDY = 41, FVM_DTDZ = 42, FVM_TN = 43, FVM_NFIELDS = 44
 };
 
 /* GRID + METRIC + the geometric part of INIT (RLES/grid.f90:1-367, RLES/metric.f90:1-538,
@@ -92,9 +100,18 @@ int fvm_get_field(fvm_ctx* ctx, int field, double* host);
 
 /* One time step of MAIN (RLES/main.f90:39-150): U_N copy, CONTRA_VEL, NL, STRESS_TENSOR_DY,
  * DIF, P_SOURCE, ITER_U x3, predictor correction, BCU x3, CONTRA_VEL, ITER_P, BCP, P_SOURCE,
- * projection, pressure reference, BCU x3, CALCCOEF.  The thermal scalar is not advanced (it is
- * passive).  info may be NULL. */
+ * projection, pressure reference, BCU x3, [ITER_T, BCT,] CALCCOEF.  The passive temperature is
+ * advanced only after fvm_set_thermal.  info may be NULL. */
 int fvm_step(fvm_ctx* ctx, fvm_step_info* info);
+
+/* Switch on the passive temperature T of the 3D solver: T_N copy, NL(T_N, ADV), the subgrid
+ * heat flux of STRESS_TENSOR_DY (stress_tensor_dy.f90:589-738), DIF(T_N, DIFT), ITER_T + BCT
+ * (RLES/iter_t.f90:1-247, bct.f90:1-49; main.f90:41-147) and the Nusselt number of CALCCOEF,
+ * with T_WALL = 1 and TIN = 0 (comdat_param.f90:39-40) and Peclet number pe (the reference's
+ * PE, comdat_param.f90:48).  T and ADV start at zero; set them with fvm_set_field.  Needs the
+ * exact K-line stage and the line-scan form of the I and J stages (extruded grids);
+ * FVM_ERR_ARG otherwise and for 2D contexts. */
+int fvm_set_thermal(fvm_ctx* ctx, double pe);
 
 /* Change DTIME between steps.  DTIME is a run-time module variable of the reference, reassigned
  * by POST_PROCESS (RLES/post_process.f90:59) and by the CFL control of CALCCOEF
@@ -103,7 +120,7 @@ int fvm_step(fvm_ctx* ctx, fvm_step_info* info);
 int fvm_set_dtime(fvm_ctx* ctx, double dtime);
 
 /* The subroutines of the time step, for drivers that keep MAIN's call sequence. */
-int fvm_begin_step(fvm_ctx* ctx);                 /* U_N = U (RLES/main.f90:42-45)          */
+int fvm_begin_step(fvm_ctx* ctx);                 /* U_N = U, T_N = T (main.f90:41-45)      */
 int fvm_contra_vel(fvm_ctx* ctx);                 /* CONTRA_VEL (RLES/contra_vel.f90:1-58)   */
 int fvm_nl(fvm_ctx* ctx);                         /* NL_NM1 = NL; NL(U_N..W_N) (nl.f90:1-33) */
 int fvm_stress_tensor_dy(fvm_ctx* ctx, double* c_avg); /* SGS + NL += J SGS (main.f90:59-69)
@@ -111,6 +128,8 @@ int fvm_stress_tensor_dy(fvm_ctx* ctx, double* c_avg); /* SGS + NL += J SGS (mai
 int fvm_dif(fvm_ctx* ctx);                        /* DIF(U_N..W_N) (RLES/dif.f90:1-25)       */
 int fvm_p_source(fvm_ctx* ctx);                   /* P_SOURCE(P) (RLES/p_source.f90:1-44)    */
 int fvm_iter_u(fvm_ctx* ctx);                     /* ITER_U for U, V, W (iter_u.f90:1-256)   */
+int fvm_iter_t(fvm_ctx* ctx);                     /* ITER_T + BCT(T,T_N) (iter_t.f90:1-247,
+                                                     bct.f90:1-49, main.f90:145-147)        */
 int fvm_correct(fvm_ctx* ctx, double sign);       /* U += sign*DTIME*S/J, then BCU x3
                                                      (RLES/main.f90:90-98, 108-111, 137-142) */
 int fvm_iter_p(fvm_ctx* ctx, double* resip, int* niter); /* ITER_P + BCP (iter_p.f90:1-66,

@@ -1,6 +1,7 @@
 """The Fortran driver (fvm_amd/bin/fvm_rles) runs the reference's own input files -- the grid
-file NNNxNNNxNNN.dat and RESTART000.DAT -- through the C ABI and writes the reference's restart
-and NU_CD_CL.PLT; its result after two converged steps matches the reference's golden dump."""
+file NNNxNNNxNNN.dat and RESTART000.DAT -- through the C ABI and writes the reference's restart,
+NU_CD_CL.PLT, CS.PLT and CT.PLT; its result after two converged steps, the passive temperature
+included, matches the reference's golden dump."""
 import os
 import subprocess
 
@@ -21,17 +22,15 @@ def rel(a, b):
 @pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built (flang absent)")
 def test_fortran_driver_two_steps(tmp_path):
     from fvm_amd.grid import grid_filename, read_restart3d, write_grid3d, write_restart3d
-    g = load_golden("rles_9x21x5_les_converged")
+    g = load_golden("rles_9x21x5_les_thermal")
     ni, nj, nk = int(g["ni"]), int(g["nj"]), int(g["nk"])
     write_grid3d(str(tmp_path / grid_filename(ni, nj, nk)), g["xg"], g["yg"], g["zg"])
-    z = np.zeros_like(g["in_u"])
-    f = {k: g["in_" + k] for k in ("u", "v", "w", "p", "nlx", "nly", "nlz")}
-    f.update(adv=z, t=z)
+    f = {k: g["in_" + k] for k in ("u", "v", "w", "p", "nlx", "nly", "nlz", "t", "adv")}
     write_restart3d(str(tmp_path / "RESTART000.DAT"), f, float(g["re"]), 0.0, float(g["dt"]), 0.0)
     (tmp_path / "fvm_rles.nml").write_text(
         f"&FVM_RUN NI={ni}, NJ={nj}, NK={nk}, RE={float(g['re'])!r}, DTIME={float(g['dt'])!r},\n"
         f" RESIMAX=1.0D-13, NSWPP=50, USE_LES=1, REF_A={float(g['ref_a'])!r},"
-        f" POISSON_MODE=2, NSTEP=2 /\n")
+        f" POISSON_MODE=2, NSTEP=2, THERMAL=1, PE={float(g['pe'])!r} /\n")
     print("fortran driver: rles_9x21x5, 2 steps", flush=True)
     out = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
@@ -39,6 +38,7 @@ def test_fortran_driver_two_steps(tmp_path):
     for c in ("u", "v", "w"):
         assert rel(r[c], g[f"s2_{c}"]) < 1e-10, c
     assert rel(r["p"], g["s2_pref"]) < 1e-10
+    assert rel(r["t"], g["s2_t"]) < 1e-10
     assert r["niter"] == 2 and abs(r["rtime"] - 2 * float(g["dt"])) < 1e-15
     body = (tmp_path / "NU_CD_CL.PLT").read_text().split("\n", 1)[1]
     rows = np.array(body.split(), dtype=np.float64).reshape(-1, 10)
@@ -48,6 +48,11 @@ def test_fortran_driver_two_steps(tmp_path):
         assert abs(rows[st, 2] - ref[6]) <= 1e-9 * abs(ref[6])      # CD
         assert abs(rows[st, 3] - ref[7]) <= 1e-9 * max(abs(ref[7]), 1.0)  # CL
         assert abs(rows[st, 9] - ref[4]) <= 1e-10 * ref[4]           # CFL
+        assert abs(rows[st, 1] - ref[10]) <= 1e-9 * abs(ref[10])     # NU
+    for fname, key in (("CS.PLT", "cs_sqrt"), ("CT.PLT", "ct_sqrt")):
+        body = (tmp_path / fname).read_text().split("\n", 1)[1]
+        vals = np.array(body.split(), dtype=np.float64).reshape(-1, 2)
+        assert np.abs(vals[:, 1] - g[key]).max() <= 1e-10 * np.abs(g[key]).max(), fname
 
 
 EXE2D = os.path.join(REPO, "fvm_amd", "bin", "fvm_r2d")
