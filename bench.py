@@ -56,21 +56,24 @@ def cells_of(w):
     return (w["ni"] - 1) * (w["nj"] - 1) * (w["nk"] - 1)
 
 
-def build_solver(w, mode, resimax, nswpp, check_every, dt, af_mode=0):
-    from fvm_amd.grid import make_case
+def build_solver(w, mode, resimax, nswpp, check_every, dt, af_mode=0, thermal=False):
+    from fvm_amd.grid import make_case, seeded_temperature
     from fvm_amd.solver import Solver3D
     (xg, yg, zg), ic = make_case(**w)
     s = Solver3D(w["ni"], w["nj"], w["nk"], w["re"], dt, xg, yg, zg, ref_a=math.pi,
                  use_les=True, nswpp=nswpp, resimax=resimax, poisson_mode=mode,
                  check_every=check_every, af_mode=af_mode)
     s.set_state(*(ic[k] for k in ("u", "v", "w", "p", "nlx", "nly", "nlz")))
+    if thermal:  # the passive temperature at the reference's Peclet number
+        t = seeded_temperature(w["ni"], w["nj"], w["nk"], 0.05)
+        s.set_temperature(t, 0.0 * t, pe=2769.0)
     return s
 
 
 def run_mode(w, mode, a, log):
     dt = w["dt"]
     s = build_solver(w, mode, a.resimax, a.nswpp, a.check_every, dt * a.startup_factor,
-                     a.af_mode)
+                     a.af_mode, a.thermal)
     for q in range(a.startup_steps):
         info = s.step()
         if q == 0 or q + 1 == a.startup_steps:
@@ -166,6 +169,8 @@ def main():
     ap.add_argument("--af-mode", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="print every kernel scope")
+    ap.add_argument("--thermal", action="store_true",
+                    help="advance the passive temperature too (not part of the headline step)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the solver holds "
@@ -209,7 +214,8 @@ def main():
         "config": {"workload": a.workload, "cells": [w["nj"] - 1, w["ni"] - 1, w["nk"] - 1],
                    "re": w["re"], "dt": w["dt"], "resimax": a.resimax,
                    "poisson": "mixed" if head == "mixed" else "fp64",
-                   "startup": f"{a.startup_steps} untimed steps at dt={w['dt'] * a.startup_factor:g}"},
+                   "startup": f"{a.startup_steps} untimed steps at dt={w['dt'] * a.startup_factor:g}",
+                   "passive_temperature": bool(a.thermal)},
         "roofline": roof,
     }
     for mname, rr in results.items():

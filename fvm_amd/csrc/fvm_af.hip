@@ -140,8 +140,9 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
                                                    const double* const* __restrict__ fa,
                                                    int qform) {
   constexpr double kTWall = 1.0;  // T_WALL (comdat_param.f90:40)
+  constexpr int NC = TH ? 1 : 3;  // components: T, or u, v, w
   const uint3 bb = xcd_block();
-  __shared__ double sh[3][3][SF_P];  // [component][plane slot][tile cell]
+  __shared__ double sh[NC][3][SF_P];  // [component][plane slot][tile cell]
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
   const int i = 1 + bb.x * 64 + tx;
   const int j = 1 + bb.y * 4 + ty;
@@ -161,7 +162,8 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
       const int gi = min(bb.x * 64 + q % SF_W, d.ni);
       const int gj = min(bb.y * 4 + q / SF_W, d.nj);
       const long c = cidx(d, gi, gj, kp);
-      for (int comp = 0; comp < A.nc; ++comp) sh[comp][slot][q] = A.f[comp][c];
+#pragma unroll
+      for (int comp = 0; comp < NC; ++comp) sh[comp][slot][q] = A.f[comp][c];
     }
     zx[0] = zx[1]; zx[1] = zx[2];
     ze[0] = ze[1]; ze[1] = ze[2];
@@ -185,7 +187,8 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
     const double pt_zx = zx[2], pb_zx = zx[0];
     const double pt_ze = ze[2], pb_ze = ze[0];
     const double a0 = ap0[c];
-    for (int comp = 0; comp < A.nc; ++comp) {
+#pragma unroll
+    for (int comp = 0; comp < NC; ++comp) {
 #define F(dk, dj, di) sh[comp][pl[(dk) + 1]][q0 + (dj) * SF_W + (di)]
       double r = pe_xe * (F(0, 1, 1) - F(0, -1, 1));
       r = r - pw_xe * (F(0, 1, -1) - F(0, -1, -1));
