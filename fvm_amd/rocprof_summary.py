@@ -5,9 +5,12 @@ python -m fvm_amd.rocprof_summary trace DB [N]
 python -m fvm_amd.rocprof_summary pmc DB
     per-kernel mean of every collected counter per dispatch
 python -m fvm_amd.rocprof_summary traffic FETCH_DB WRITE_DB COPY_FETCH_DB COPY_WRITE_DB COPY_MIB
+        [CELLS [NAME=ROCPROF_NAME ...]]
     HBM bytes per dispatch of every kernel: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
     scaled so that the plain copy kernel of COPY_MIB MiB reads and writes exactly that many
-    bytes; prints JSON {"fetch_scale", "write_scale", "kernels": {short name: {...}}}
+    bytes; prints JSON {"fetch_scale", "write_scale", "all_kernels": {short name: {...}}}.
+    With CELLS every entry also gets bytes_per_cell, and each NAME=ROCPROF_NAME pair adds
+    kernels[NAME] (the name bench.py times the kernel under) for that rocprof kernel.
 """
 from __future__ import annotations
 
@@ -66,7 +69,8 @@ def copy_kib(db: str, counter: str) -> float:
     return vals[0]
 
 
-def traffic(fdb: str, wdb: str, cfdb: str, cwdb: str, mib: float):
+def traffic(fdb: str, wdb: str, cfdb: str, cwdb: str, mib: float, cells: int = 0,
+            aliases=()):
     kib = mib * 1024.0
     fs = kib / copy_kib(cfdb, "FETCH_SIZE")
     ws = kib / copy_kib(cwdb, "WRITE_SIZE")
@@ -79,8 +83,18 @@ def traffic(fdb: str, wdb: str, cfdb: str, cwdb: str, mib: float):
             continue
         ks[k] = {"fetch_bytes": fb * 1024.0 * fs, "write_bytes": wb * 1024.0 * ws,
                  "bytes_per_launch": (fb * fs + wb * ws) * 1024.0}
-    print(json.dumps({"fetch_scale": fs, "write_scale": ws, "copy_mib": mib, "kernels": ks},
-                     indent=1))
+        if cells:
+            ks[k]["bytes_per_cell"] = ks[k]["bytes_per_launch"] / cells
+    out = {"fetch_scale": fs, "write_scale": ws, "copy_mib": mib}
+    named = {}
+    for a in aliases:
+        name, rp = a.split("=", 1)
+        if rp in ks:
+            named[name] = dict(ks[rp], rocprof_name=rp)
+    if named:
+        out["kernels"] = named
+    out["all_kernels"] = ks
+    print(json.dumps(out, indent=1))
 
 
 if __name__ == "__main__":
@@ -90,6 +104,7 @@ if __name__ == "__main__":
     elif cmd == "pmc":
         pmc(sys.argv[2])
     elif cmd == "traffic":
-        traffic(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5], float(sys.argv[6]))
+        traffic(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5], float(sys.argv[6]),
+                int(sys.argv[7]) if len(sys.argv) > 7 else 0, sys.argv[8:])
     else:
         raise SystemExit(__doc__)
