@@ -707,8 +707,14 @@ int fvm_k_iter_u(FvmCtx* c) {
       const double* const* fas = (const double* const*)c->dfa[s];
       const int dz = c->dd2_zero ? 1 : 0;
       const dim3 gk((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + RK_KC - 1) / RK_KC);
-      if (s == 0) hipLaunchKernelGGL((k_af_rhs_km<0, false>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[0], c->ap0, c->prm.re, fas, qf);
-      if (s == 1) hipLaunchKernelGGL((k_af_rhs_km<1, false>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[1], c->ap0, c->prm.re, fas, qf);
+      if (s == 0) {
+        ProfScope pk(c, "k_af_rhs_i");
+        hipLaunchKernelGGL((k_af_rhs_km<0, false>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[0], c->ap0, c->prm.re, fas, qf);
+      }
+      if (s == 1) {
+        ProfScope pk(c, "k_af_rhs_j");
+        hipLaunchKernelGGL((k_af_rhs_km<1, false>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[1], c->ap0, c->prm.re, fas, qf);
+      }
       if (s == 2) hipLaunchKernelGGL(k_af_rhs<2>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf, dz);
       HIP_CHECK(hipGetLastError());
     }

@@ -332,13 +332,10 @@ int fvm_get_field(fvm_ctx* c, int field, double* host) {
 
 int fvm_begin_step(fvm_ctx* c) {
   if (!c) return FVM_ERR_ARG;
-  size_t b = c->d.n * sizeof(double);
-  HIP_CHECK(hipMemcpyAsync(c->un, c->u, b, hipMemcpyDeviceToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->vn, c->v, b, hipMemcpyDeviceToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->wn, c->w, b, hipMemcpyDeviceToDevice, c->stream));
-  if (c->thermal) {  // T_N = T; BCT's far-field values depend only on T_N
-    HIP_CHECK(hipMemcpyAsync(c->tn, c->t, b, hipMemcpyDeviceToDevice, c->stream));
-    int r = fvm_k_tfar(c);
+  int r = fvm_k_step_copy(c);  // U_N = U ... (T_N = T)
+  if (r) return r;
+  if (c->thermal) {  // BCT's far-field values depend only on T_N
+    r = fvm_k_tfar(c);
     if (r) return r;
   }
   // far-field boundary values depend only on U_N: one evaluation serves every BCU of the step

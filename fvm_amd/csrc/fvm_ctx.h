@@ -154,6 +154,7 @@ int fvm_k_ufar(FvmCtx* c);
 int fvm_k_bcu_ghosts(FvmCtx* c, double** f, int cnt);
 int fvm_k_bct_ghosts(FvmCtx* c, double* f);
 int fvm_k_tfar(FvmCtx* c);
+int fvm_k_step_copy(FvmCtx* c);
 int fvm_k_iter_t(FvmCtx* c);
 int fvm_thermal_setup(FvmCtx* c);
 int fvm_k_correct(FvmCtx* c, double sign);

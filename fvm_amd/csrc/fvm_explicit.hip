@@ -332,19 +332,61 @@ __global__ void k_ufar(Dims d, const double* const* __restrict__ m, const double
   }
 }
 
-// BCU ghost values (RLES/bcu.f90:21-22, 41-42): wall mirror about U_F = 0 (no slip), far-field
-// mirror about the far-field value; BCT's (bct.f90:22, 45) with the wall value T_WALL; the
-// periodic J/K ghosts follow (fvm_ghost_fill).
-__global__ void k_bcu_ghosts(Dims d, PtrSet8 set, const double* __restrict__ ufar, double wall) {
-  const long nf = (long)(d.nj - 1) * (d.nk - 1);
-  double* __restrict__ f = set.p[blockIdx.y];
+// dst[q] = src[q] for whole cell arrays, ghosts included (U_N = U, ..., T_N = T), one launch;
+// two doubles per lane per step of the grid-stride loop
+__global__ void k_copy_multi(long n, PtrSet8 dst, PtrSet8 src) {
+  const double2* __restrict__ s = reinterpret_cast<const double2*>(src.p[blockIdx.y]);
+  double2* __restrict__ t = reinterpret_cast<double2*>(dst.p[blockIdx.y]);
+  const long n2 = n / 2;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n2;
+       q += (long)gridDim.x * blockDim.x)
+    t[q] = s[q];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1))
+    dst.p[blockIdx.y][n - 1] = src.p[blockIdx.y][n - 1];
+}
+
+// BCU / BCT ghosts and the periodic MPI_X copy in one pass over the ghost surface.  Every ghost
+// is a function of one interior cell: with (js, ks) the periodic partner of (j, k) (the cell
+// itself when j, k are interior), the wall ghost is -a(1, js, ks) + 2 wall, the far-field ghost
+// -a(NI-1, js, ks) + 2 U_F(js, ks) and any other ghost a(i, js, ks) -- exactly the values the
+// reference's BCU (wall and far-field ghosts of the interior rows) followed by MPI_X leaves.
+__global__ void k_bc_ghosts(Dims d, PtrSet8 set, const double* __restrict__ ufar, double wall) {
+  double* __restrict__ a = set.p[blockIdx.y];
   const double* __restrict__ uf = ufar + blockIdx.y * (long)(d.nj + 1) * (d.nk + 1);
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nf;
+  const long nI = 2L * (d.nj - 1) * (d.nk - 1);          // I ghost planes, interior j, k
+  const long nJ = (long)(d.ni + 1) * 2 * (d.nk - 1);      // J ghost rows, interior k
+  const long nK = (long)(d.ni + 1) * (d.nj + 1) * 2;      // K ghost planes
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nI + nJ + nK;
        t += (long)gridDim.x * blockDim.x) {
-    int j = 1 + (int)(t % (d.nj - 1)), k = 1 + (int)(t / (d.nj - 1));
-    long c0 = cidx(d, 0, j, k), cn = cidx(d, d.ni, j, k);
-    f[c0] = -f[c0 + 1] + 2.0 * wall;
-    f[cn] = -f[cn - 1] + 2.0 * uf[j + (long)(d.nj + 1) * k];
+    int i, j, k;
+    if (t < nI) {
+      const long q = t % (nI / 2);
+      i = t < nI / 2 ? 0 : d.ni;
+      j = 1 + (int)(q % (d.nj - 1));
+      k = 1 + (int)(q / (d.nj - 1));
+    } else if (t < nI + nJ) {
+      const long q = t - nI;
+      i = (int)(q % (d.ni + 1));
+      const long r = q / (d.ni + 1);
+      j = (r % 2) == 0 ? 0 : d.nj;
+      k = 1 + (int)(r / 2);
+    } else {
+      const long q = t - nI - nJ;
+      i = (int)(q % (d.ni + 1));
+      const long r = q / (d.ni + 1);
+      j = (int)(r % (d.nj + 1));
+      k = (r / (d.nj + 1)) == 0 ? 0 : d.nk;
+    }
+    const int js = j == 0 ? d.nj - 1 : (j == d.nj ? 1 : j);
+    const int ks = k == 0 ? d.nk - 1 : (k == d.nk ? 1 : k);
+    double v;
+    if (i == 0)
+      v = -a[cidx(d, 1, js, ks)] + 2.0 * wall;
+    else if (i == d.ni)
+      v = -a[cidx(d, d.ni - 1, js, ks)] + 2.0 * uf[js + (long)(d.nj + 1) * ks];
+    else
+      v = a[cidx(d, i, js, ks)];
+    a[cidx(d, i, j, k)] = v;
   }
 }
 
@@ -646,6 +688,23 @@ int fvm_k_ufar(FvmCtx* c) {
   return FVM_OK;
 }
 
+// U_N = U, V_N = V, W_N = W (and T_N = T) at the start of the step (RLES/main.f90:41-44)
+int fvm_k_step_copy(FvmCtx* c) {
+  PtrSet8 dst{}, src{};
+  dst.p[0] = c->un; dst.p[1] = c->vn; dst.p[2] = c->wn; dst.p[3] = c->tn;
+  src.p[0] = c->u; src.p[1] = c->v; src.p[2] = c->w; src.p[3] = c->t;
+  const int cnt = c->thermal ? 4 : 3;
+  hipLaunchKernelGGL(k_copy_multi, dim3(2048, cnt), dim3(256), 0, c->stream, c->d.n, dst, src);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+// cells of the ghost surface enumerated by k_bc_ghosts
+inline long bc_ghost_count(const Dims& d) {
+  return 2L * (d.nj - 1) * (d.nk - 1) + (long)(d.ni + 1) * 2 * (d.nk - 1) +
+         (long)(d.ni + 1) * (d.nj + 1) * 2;
+}
+
 // far-field values of T for BCT from T_N (bct.f90:28-37)
 int fvm_k_tfar(FvmCtx* c) {
   long nf = (long)(c->d.nj + 1) * (c->d.nk + 1);
@@ -658,27 +717,25 @@ int fvm_k_tfar(FvmCtx* c) {
 
 // BCT ghosts of one temperature array (wall T_WALL = 1, far field from tfar) and MPI_X
 int fvm_k_bct_ghosts(FvmCtx* c, double* f) {
-  long nf = (long)(c->d.nj - 1) * (c->d.nk - 1);
   ProfScope ps(c, "bct");
   PtrSet8 set{};
   set.p[0] = f;
-  hipLaunchKernelGGL(k_bcu_ghosts, dim3(grid1(nf, 256).x, 1), dim3(256), 0, c->stream, c->d, set,
-                     c->tfar, 1.0);
+  hipLaunchKernelGGL(k_bc_ghosts, dim3(grid1(bc_ghost_count(c->d), 256).x, 1), dim3(256), 0,
+                     c->stream, c->d, set, c->tfar, 1.0);
   HIP_CHECK(hipGetLastError());
-  return fvm_ghost_fill(c, f);
+  return FVM_OK;
 }
 
 // f[q] takes the far-field values of component q (cnt <= 3)
 int fvm_k_bcu_ghosts(FvmCtx* c, double** f, int cnt) {
   if (cnt < 1 || cnt > 3) return FVM_ERR_ARG;
-  long nf = (long)(c->d.nj - 1) * (c->d.nk - 1);
   ProfScope ps(c, "bcu");
   PtrSet8 set{};
   for (int q = 0; q < cnt; ++q) set.p[q] = f[q];
-  hipLaunchKernelGGL(k_bcu_ghosts, dim3(grid1(nf, 256).x, cnt), dim3(256), 0, c->stream, c->d,
-                     set, c->ufar, 0.0);
+  hipLaunchKernelGGL(k_bc_ghosts, dim3(grid1(bc_ghost_count(c->d), 256).x, cnt), dim3(256), 0,
+                     c->stream, c->d, set, c->ufar, 0.0);
   HIP_CHECK(hipGetLastError());
-  return fvm_ghost_fill_n(c, f, cnt);
+  return FVM_OK;
 }
 
 int fvm_k_correct(FvmCtx* c, double sign) {

@@ -506,6 +506,7 @@ int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
     if (r) return r;
     {
       ProfScope ps(c, "p_residual");
+      ProfScope pk(c, "k_residual");
       if (use32)
         hipLaunchKernelGGL(k_residual_compact<float>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
                            c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d32.X, c->red);
