@@ -42,6 +42,8 @@ typedef struct orc3d {
   double *nut, *sgsx, *sgsy, *sgsz;
   /* passive temperature (ITER_T, BCT, SGS_T) */
   double *t, *tn, *adv, *adv1, *dift, *sgst, *th, *dtdx, *dtdy, *dtdz, *kappa;
+  /* field output (WRITEDATA): cell |omega|, lambda_2; node averages at IX(I,J,K), I=1..NI */
+  double *vorm, *lam2, *pg, *ug, *vg, *wg, *tg, *vorg, *lamg;
   double c_avg, c_t, pe, nu, sum_area;
   /* scalars of the last step */
   double uin, resip, pref, cfl, cpb, cd, cl, cdp, cdv, clp, clv;
@@ -72,7 +74,9 @@ orc3d *orc3d_create(int ni, int nj, int nk, double re, double dtime, double ref_
                     &o->lss[4], &o->lss[5], &o->mag, &o->uh, &o->vh, &o->wh, &o->magh,
                     &o->dumin, &o->dumout, &o->lij, &o->mij, &o->lm, &o->mm, &o->nut,
                     &o->sgsx, &o->sgsy, &o->sgsz, &o->t, &o->tn, &o->adv, &o->adv1,
-                    &o->dift, &o->sgst, &o->th, &o->dtdx, &o->dtdy, &o->dtdz, &o->kappa};
+                    &o->dift, &o->sgst, &o->th, &o->dtdx, &o->dtdy, &o->dtdz, &o->kappa,
+                    &o->vorm, &o->lam2, &o->pg, &o->ug, &o->vg, &o->wg, &o->tg, &o->vorg,
+                    &o->lamg};
   for (size_t a = 0; a < sizeof(all) / sizeof(all[0]); ++a) *all[a] = alloc0(o->n);
   return o;
 }
@@ -80,8 +84,8 @@ orc3d *orc3d_create(int ni, int nj, int nk, double re, double dtime, double ref_
 void orc3d_destroy(orc3d *o) {
   if (!o) return;
   double **p = &o->xg;
-  /* every double* member from xg to kappa is an allocation */
-  double **end = &o->kappa;
+  /* every double* member from xg to lamg is an allocation */
+  double **end = &o->lamg;
   for (; p <= end; ++p) free(*p);
   free(o);
 }
@@ -102,7 +106,8 @@ double *orc3d_array(orc3d *o, const char *name) {
   N("magh", magh) N("lm", lm) N("mm", mm) N("nut", nut) N("sgsx", sgsx) N("sgsy", sgsy)
   N("sgsz", sgsz) N("u1", u1) N("u2", u2) N("u3", u3) N("t", t) N("tn", tn) N("adv", adv)
   N("adv1", adv1) N("dift", dift) N("sgst", sgst) N("that", th) N("dtdx", dtdx)
-  N("dtdy", dtdy) N("dtdz", dtdz) N("kappa", kappa)
+  N("dtdy", dtdy) N("dtdz", dtdz) N("kappa", kappa) N("vorm", vorm) N("lamda2", lam2)
+  N("pg", pg) N("ug", ug) N("vg", vg) N("wg", wg) N("tg", tg) N("vorg", vorg) N("lamdag", lamg)
 #undef N
   return NULL;
 }
@@ -1274,4 +1279,124 @@ void orc3d_step(orc3d *o, int nswpp, double resimax, int use_les) {
     orc3d_bct(o, o->t, o->tn);
   }
   orc3d_calccoef(o);
+}
+
+/* ----------------------------------------------------------------------------------------- */
+/* Field output of WRITEDATA (RLES/writedata.f90:8-189, calc_vor.f90:1-141, find_root.f90:1-27) */
+/* ----------------------------------------------------------------------------------------- */
+
+/* FIND_ROOT(A,B,C,D,ROOT): the three real roots of A x^3 + B x^2 + C x + D by the
+ * trigonometric form (find_root.f90:10-25). */
+static void find_root(double A, double B, double C, double D, double root[3]) {
+  double q = (3.0 * A * C - B * B) / (9.0 * (A * A));
+  double r = (9.0 * A * B * C - 27.0 * (A * A) * D - 2.0 * (B * B * B)) / (54.0 * (A * A * A));
+  double rho = sqrt(-(q * q * q));
+  double phi = acos(r / rho);
+  double b3 = -(B / (3.0 * A));
+  double cr = pow(rho, 1.0 / 3.0);
+  double c = cos(phi / 3.0), sn = sin(phi / 3.0), s3 = sqrt(3.0);
+  root[0] = b3 + 2.0 * cr * c;
+  root[1] = b3 - cr * c + s3 * cr * sn;
+  root[2] = b3 - cr * c - s3 * cr * sn;
+}
+
+/* CALC_VOR's |omega| on the interior cells + MPI_X, lambda_2 of S^2 + Omega^2 with the
+ * three-point extrapolation to the six ghost faces + MPI_X, and the 8-cell nodal averages of
+ * p, u, v, w, T, |omega| and lambda_2 (nodes I = 1..NI, J = 1..NJ, K = 1..NK). */
+void orc3d_writedata_fields(orc3d *o) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  long sj = o->sj, sk = o->sk;
+  const double *U = o->u, *V = o->v, *W = o->w;
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+#define DI(F) (F[c + 1] - F[c - 1])
+#define DJ(F) (F[c + sj] - F[c - sj])
+#define DK(F) (F[c + sk] - F[c - sk])
+        double vz = o->xix[c] * 0.5 * DI(V) + o->etx[c] * 0.5 * DJ(V) + o->ztx[c] * 0.5 * DK(V) -
+                    o->xiy[c] * 0.5 * DI(U) - o->ety[c] * 0.5 * DJ(U) - o->zty[c] * 0.5 * DK(U);
+        double vy = o->xiz[c] * 0.5 * DI(U) + o->etz[c] * 0.5 * DJ(U) + o->ztz[c] * 0.5 * DK(U) -
+                    o->xix[c] * 0.5 * DI(W) - o->etx[c] * 0.5 * DJ(W) - o->ztx[c] * 0.5 * DK(W);
+        double vx = o->xiy[c] * 0.5 * DI(W) + o->ety[c] * 0.5 * DJ(W) + o->zty[c] * 0.5 * DK(W) -
+                    o->xiz[c] * 0.5 * DI(V) - o->etz[c] * 0.5 * DJ(V) - o->ztz[c] * 0.5 * DK(V);
+        o->vorm[c] = sqrt(vx * vx + vy * vy + vz * vz);
+        /* velocity gradient (writedata.f90:36-50) */
+        double g[3][3]; /* g[a][b] = d u_a / d x_b */
+        const double *F3[3] = {U, V, W};
+        const double *mx[3][3] = {{o->xix, o->etx, o->ztx}, {o->xiy, o->ety, o->zty},
+                                  {o->xiz, o->etz, o->ztz}};
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            const double *F = F3[a];
+            g[a][b] = 0.5 * (mx[b][0][c] * DI(F) + mx[b][1][c] * DJ(F) + mx[b][2][c] * DK(F));
+          }
+#undef DI
+#undef DJ
+#undef DK
+        /* S1(m,n) = 0.5 (d u_m/dx_n + d u_n/dx_m), O1 with the minus sign (writedata.f90:53-75) */
+        double S1[3][3], O1[3][3], A2[3][3];
+        for (int m = 0; m < 3; ++m)
+          for (int nn = 0; nn < 3; ++nn) {
+            S1[m][nn] = 0.5 * (g[m][nn] + g[nn][m]);
+            O1[m][nn] = 0.5 * (g[m][nn] - g[nn][m]);
+          }
+        for (int m = 0; m < 3; ++m)
+          for (int nn = 0; nn < 3; ++nn) {
+            double s2 = S1[m][0] * S1[0][nn] + S1[m][1] * S1[1][nn] + S1[m][2] * S1[2][nn];
+            double o2 = O1[m][0] * O1[0][nn] + O1[m][1] * O1[1][nn] + O1[m][2] * O1[2][nn];
+            A2[m][nn] = s2 + o2;
+          }
+        double da = -1.0;
+        double db = A2[0][0] + A2[1][1] + A2[2][2];
+        double dc = -(A2[0][0] * A2[1][1]) - A2[1][1] * A2[2][2] - A2[2][2] * A2[0][0] +
+                    A2[0][1] * A2[1][0] + A2[1][2] * A2[2][1] + A2[2][0] * A2[0][2];
+        double dd = A2[0][0] * A2[1][1] * A2[2][2] + A2[0][1] * A2[1][2] * A2[2][0] +
+                    A2[0][2] * A2[1][0] * A2[2][1] - A2[0][0] * A2[1][2] * A2[2][1] -
+                    A2[1][1] * A2[0][2] * A2[2][0] - A2[2][2] * A2[0][1] * A2[1][0];
+        double lam[3];
+        find_root(da, db, dc, dd, lam);
+        double l1 = fmax(fmax(lam[0], lam[1]), lam[2]), l3 = fmin(fmin(lam[0], lam[1]), lam[2]);
+        for (int q = 0; q < 3; ++q)
+          if (lam[q] < l1 && lam[q] > l3) o->lam2[c] = lam[q];
+      }
+  orc3d_mpi_x(o, o->vorm);
+  /* extrapolation to the ghost faces, in the reference's statement order (writedata.f90:106-111) */
+  double *L = o->lam2;
+  for (int k = 0; k <= nk; ++k)
+    for (int j = 0; j <= nj; ++j) {
+      L[IX(o, 0, j, k)] = 3.0 * L[IX(o, 1, j, k)] - 3.0 * L[IX(o, 2, j, k)] + L[IX(o, 3, j, k)];
+    }
+  for (int k = 0; k <= nk; ++k)
+    for (int j = 0; j <= nj; ++j) {
+      L[IX(o, ni, j, k)] = 3.0 * L[IX(o, ni - 1, j, k)] - 3.0 * L[IX(o, ni - 2, j, k)] +
+                           L[IX(o, ni - 3, j, k)];
+    }
+  for (int k = 0; k <= nk; ++k)
+    for (int i = 0; i <= ni; ++i)
+      L[IX(o, i, 0, k)] = 3.0 * L[IX(o, i, 1, k)] - 3.0 * L[IX(o, i, 2, k)] + L[IX(o, i, 3, k)];
+  for (int k = 0; k <= nk; ++k)
+    for (int i = 0; i <= ni; ++i)
+      L[IX(o, i, nj, k)] = 3.0 * L[IX(o, i, nj - 1, k)] - 3.0 * L[IX(o, i, nj - 2, k)] +
+                           L[IX(o, i, nj - 3, k)];
+  for (int j = 0; j <= nj; ++j)
+    for (int i = 0; i <= ni; ++i)
+      L[IX(o, i, j, 0)] = 3.0 * L[IX(o, i, j, 1)] - 3.0 * L[IX(o, i, j, 2)] + L[IX(o, i, j, 3)];
+  for (int j = 0; j <= nj; ++j)
+    for (int i = 0; i <= ni; ++i)
+      L[IX(o, i, j, nk)] = 3.0 * L[IX(o, i, j, nk - 1)] - 3.0 * L[IX(o, i, j, nk - 2)] +
+                           L[IX(o, i, j, nk - 3)];
+  orc3d_mpi_x(o, L);
+  /* nodal averages (writedata.f90:115-133) */
+  const double *src[7] = {o->p, o->u, o->v, o->w, o->t, o->vorm, o->lam2};
+  double *dst[7] = {o->pg, o->ug, o->vg, o->wg, o->tg, o->vorg, o->lamg};
+  for (int f = 0; f < 7; ++f)
+    for (int k = 1; k <= nk; ++k)
+      for (int j = 1; j <= nj; ++j)
+        for (int i = 1; i <= ni; ++i) {
+          long c = IX(o, i, j, k);
+          const double *P = src[f];
+          dst[f][c] = 0.125 * (P[c] + P[c - sj] + P[c - 1] + P[c - 1 - sj] + P[c - sk] +
+                               P[c - sj - sk] + P[c - 1 - sk] + P[c - 1 - sj - sk]);
+        }
 }

@@ -50,6 +50,7 @@ def lib():
         L.orc3d_set_temperature.argtypes = [vp, vp, vp]
         L.orc3d_iter_t.argtypes = [vp]
         L.orc3d_bct.argtypes = [vp, vp, vp]
+        L.orc3d_writedata_fields.argtypes = [vp]
         _LIB = L
     return _LIB
 
@@ -99,6 +100,10 @@ class Oracle3D:
         self.L.orc3d_set_temperature(self.h, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs])
         self.L.orc3d_set_pe(self.h, float(pe))
         self.L.orc3d_set_thermal(self.h, 1)
+
+    def writedata_fields(self):
+        """|omega|, lambda_2 and the nodal averages of WRITEDATA for the current state."""
+        self.L.orc3d_writedata_fields(self.h)
 
     def set_no_pref(self, on=True):
         self.L.orc3d_set_no_pref(self.h, 1 if on else 0)

@@ -145,6 +145,21 @@ PROGRAM HARNESS3D
          CD, CL, CD_P, CD_V, NU
   ENDDO
   CLOSE(61)
+
+  ! field output of the last step (WRITEDATA, writedata.f90:8-189: CALC_VOR, lambda_2 by
+  ! FIND_ROOT, nodal averages, Tecplot files).  VOR_M's I ghost planes are never assigned by the
+  ! reference (calc_vor.f90:100-104 covers IS1:IE1 only) and LAMDA2 keeps its value where no
+  ! root is strictly between the other two: both start from zero here.
+  VOR_M = 0.D0
+  LAMDA2 = 0.D0
+  CONTOUR = 'YPLUS.PLT'
+  NITER_W = 1
+  NITER_R = HUGE(1)
+  CALL WRITEDATA
+  CALL DUMP('out_lamda2', LAMDA2); CALL DUMP('out_vorm', VOR_M)
+  CALL DUMPN('out_pg', PG); CALL DUMPN('out_ug', UG); CALL DUMPN('out_vg', VG)
+  CALL DUMPN('out_wg', WG); CALL DUMPN('out_tg', TG); CALL DUMPN('out_lamdag', LAMDAG)
+  CALL DUMP('out_dum3d', DUM_3D)
   CALL MPI_FINALIZE(ERR)
 
 CONTAINS
@@ -156,6 +171,14 @@ CONTAINS
     WRITE(62) A
     CLOSE(62)
   END SUBROUTINE DUMP
+
+  SUBROUTINE DUMPN(TAG, A)   ! node arrays (IS1:IE, JS1:JE, KS1:KE)
+    CHARACTER(*), INTENT(IN) :: TAG
+    DOUBLE PRECISION, INTENT(IN) :: A(IS1:IE,JS1:JE,KS1:KE)
+    OPEN(UNIT=62, FILE=TAG//'.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(62) A
+    CLOSE(62)
+  END SUBROUTINE DUMPN
 
   SUBROUTINE DUMPS(ISTEP, TAG, A)
     INTEGER, INTENT(IN) :: ISTEP

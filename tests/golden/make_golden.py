@@ -104,6 +104,13 @@ def run_case(label, c):
         # columns: step, niter, resip, pref, cfl, cpb, cd, cl, cd_p, cd_v[, nu]
         out["scalars"] = np.array([[float(x) for x in r[1:]] for r in rows])
         if c.get("t_sigma"):
+            # field output of the last step (WRITEDATA): lambda_2, |omega| on the cells, the
+            # nodal averages on nodes 1..N
+            for t in ("out_lamda2", "out_vorm", "out_dum3d"):
+                out[t] = rd(t)
+            for t in ("out_pg", "out_ug", "out_vg", "out_wg", "out_tg", "out_lamdag"):
+                out[t] = np.fromfile(os.path.join(d, t + ".bin")).reshape((ni, nj, nk),
+                                                                          order="F")
             # sqrt of the dynamic coefficients per step, as the reference writes them
             # (stress_tensor_dy.f90:519-522, 707-711; list-directed RTIME, DSQRT(C_AVG))
             for f, key in (("CS.PLT", "cs_sqrt"), ("CT.PLT", "ct_sqrt")):
@@ -162,6 +169,13 @@ def run_case2d(label, c):
         # columns: step, niter, resip, cfl, cpb, cd, cl, cd_p, cd_v, cl_p, cl_v
         out["scalars"] = np.array([[float(x) for x in r[1:]] for r in rows])
         if c.get("t_sigma"):
+            # field output of the last step (WRITEDATA): lambda_2, |omega| on the cells, the
+            # nodal averages on nodes 1..N
+            for t in ("out_lamda2", "out_vorm", "out_dum3d"):
+                out[t] = rd(t)
+            for t in ("out_pg", "out_ug", "out_vg", "out_wg", "out_tg", "out_lamdag"):
+                out[t] = np.fromfile(os.path.join(d, t + ".bin")).reshape((ni, nj, nk),
+                                                                          order="F")
             # sqrt of the dynamic coefficients per step, as the reference writes them
             # (stress_tensor_dy.f90:519-522, 707-711; list-directed RTIME, DSQRT(C_AVG))
             for f, key in (("CS.PLT", "cs_sqrt"), ("CT.PLT", "ct_sqrt")):

@@ -206,3 +206,43 @@ def test_thermal_closed_forms():
     o2.set_temperature(np.zeros(o2.shape, order="F"), np.zeros(o2.shape, order="F"))
     o2.step(int(g["nswpp"]), float(g["resimax"]), True)
     assert o2.scalar("nu") > 0.0
+
+
+def test_field_output_bitexact():
+    """WRITEDATA's field output after the two thermal steps: |omega| (CALC_VOR), lambda_2 of
+    S^2 + Omega^2 by FIND_ROOT with the ghost-face extrapolation, and the nodal averages of p,
+    u, v, w, T, |omega|, lambda_2, equal to the reference's arrays bit for bit."""
+    g = load_golden("rles_9x21x5_les_thermal")
+    o = make(g)
+    o.set_temperature(g["in_t"], g["in_adv"], float(g["pe"]))
+    for _ in (1, 2):
+        o.step(int(g["nswpp"]), float(g["resimax"]), True)
+    o.writedata_fields()
+    assert np.array_equal(o.array("lamda2"), g["out_lamda2"])
+    assert np.array_equal(o.array("vorm"), g["out_vorm"])
+    for a, t in (("pg", "out_pg"), ("ug", "out_ug"), ("vg", "out_vg"), ("wg", "out_wg"),
+                 ("tg", "out_tg"), ("lamdag", "out_lamdag")):
+        assert np.array_equal(o.array(a)[1:, 1:, 1:], g[t]), a
+    assert np.array_equal(o.array("vorg")[1:, 1:, 1:], g["out_dum3d"][1:, 1:, 1:])
+
+
+def test_lambda2_closed_forms():
+    """lambda_2 and |omega| of a linear velocity field u = A x: the discrete gradient is A to
+    rounding on the interior, so lambda_2 is the middle eigenvalue of S^2 + Omega^2
+    (S, Omega the symmetric and antisymmetric parts of A) and |omega| the norm of the curl."""
+    g = load_golden("rles_9x21x5_les_converged")
+    o = make(g)
+    x, y, zc = o.array("x"), o.array("y"), o.array("z")
+    A = np.array([[0.3, 0.8, 0.1], [-0.5, -0.2, 0.4], [0.2, -0.7, -0.1]])
+    vel = [np.asfortranarray(A[a, 0] * x + A[a, 1] * y + A[a, 2] * zc) for a in range(3)]
+    z = np.zeros(o.shape, order="F")
+    o.set_state(vel[0], vel[1], vel[2], z, z, z, z)
+    o.set_temperature(z, z)
+    o.writedata_fields()
+    S, Om = 0.5 * (A + A.T), 0.5 * (A - A.T)
+    lam = np.sort(np.linalg.eigvalsh(S @ S + Om @ Om))
+    curl = np.array([A[2, 1] - A[1, 2], A[0, 2] - A[2, 0], A[1, 0] - A[0, 1]])
+    ni, nj, nk = int(g["ni"]), int(g["nj"]), int(g["nk"])
+    I = (slice(2, ni - 1), slice(1, nj), slice(1, nk))
+    assert np.abs(o.array("lamda2")[I] - lam[1]).max() < 1e-9
+    assert np.abs(o.array("vorm")[I] - np.linalg.norm(curl)).max() < 1e-9
