@@ -19,7 +19,8 @@ FIELDS = {
     "xiy": 17, "ety": 18, "zty": 19, "xiz": 20, "etz": 21, "ztz": 22, "x": 23, "y": 24, "z": 25,
     "theta": 26, "nut": 27, "sgsx": 28, "sgsy": 29, "sgsz": 30, "un": 31,
     "vn": 32, "wn": 33, "t": 34, "adv": 35, "dift": 36, "sgst": 37, "kappa": 38, "that": 39,
-    "dtdx": 40, "dtdy": 41, "dtdz": 42, "tn": 43,
+    "dtdx": 40, "dtdy": 41, "dtdz": 42, "tn": 43, "vorm": 44, "lamda2": 45, "pg": 46,
+    "ug": 47, "vg": 48, "wg": 49, "tg": 50, "vorg": 51, "lamdag": 52,
 }
 
 STATUS = {0: "ok", 1: "bad argument", 2: "HIP runtime error", 3: "non-finite value",
@@ -58,7 +59,8 @@ EXPORTS = ("fvm_create", "fvm_destroy", "fvm_set_field", "fvm_get_field", "fvm_s
            "fvm_begin_step", "fvm_contra_vel", "fvm_nl", "fvm_stress_tensor_dy", "fvm_dif",
            "fvm_p_source", "fvm_iter_u", "fvm_correct", "fvm_iter_p", "fvm_pressure_reference",
            "fvm_calccoef", "fvm_profile_enable", "fvm_profile_get", "fvm_profile_reset",
-           "fvm_sync", "fvm_set_dtime", "fvm_create2d", "fvm_set_thermal", "fvm_iter_t")
+           "fvm_sync", "fvm_set_dtime", "fvm_create2d", "fvm_set_thermal", "fvm_iter_t",
+           "fvm_writedata_fields")
 
 _LIB = None
 
@@ -88,6 +90,7 @@ def load(path: str = LIB_PATH):
     L.fvm_set_dtime.argtypes = [vp, cd]
     L.fvm_set_thermal.argtypes = [vp, cd]
     L.fvm_iter_t.argtypes = [vp]
+    L.fvm_writedata_fields.argtypes = [vp]
     L.fvm_iter_p.argtypes = [vp, pd, ctypes.POINTER(ci)]
     L.fvm_pressure_reference.argtypes = [vp, pd]
     L.fvm_calccoef.argtypes = [vp, ctypes.POINTER(StepInfo)]

@@ -128,6 +128,10 @@ static double* field_ptr(FvmCtx* c, int f) {
     case FVM_DTDY: return c->dtd[1];
     case FVM_DTDZ: return c->dtd[2];
     case FVM_TN: return c->tn;
+    case FVM_VORM: return c->vorm;
+    case FVM_LAMDA2: return c->lam2;
+    case FVM_PG: case FVM_UG: case FVM_VG: case FVM_WG: case FVM_TG: case FVM_VORG:
+    case FVM_LAMDAG: return c->nodal[f - FVM_PG];
     default: return nullptr;
   }
 }
@@ -349,6 +353,8 @@ int fvm_set_thermal(fvm_ctx* c, double pe) {
 }
 
 int fvm_iter_t(fvm_ctx* c) { return c ? fvm_k_iter_t(c) : FVM_ERR_ARG; }
+
+int fvm_writedata_fields(fvm_ctx* c) { return c ? fvm_k_writedata_fields(c) : FVM_ERR_ARG; }
 
 int fvm_set_dtime(fvm_ctx* c, double dtime) {
   if (!c || !(dtime > 0)) return FVM_ERR_ARG;

@@ -105,6 +105,10 @@ struct FvmCtx {
   double** scantab_t[2] = {nullptr, nullptr};
   double* scantab_th[2][9] = {};
   double sum_area = 0.0;    // SUM_AREA of the wall triangles (init.f90:240-261)
+  // field output of WRITEDATA (allocated on first use): |omega|, lambda_2 on the cells and the
+  // nodal averages of p, u, v, w, T, |omega|, lambda_2 at the node's cell index
+  double *vorm = nullptr, *lam2 = nullptr;
+  double* nodal[7] = {};
   // last-step diagnostics
   fvm_step_info last{};
   // profiling
@@ -157,6 +161,7 @@ int fvm_k_tfar(FvmCtx* c);
 int fvm_k_step_copy(FvmCtx* c);
 int fvm_k_iter_t(FvmCtx* c);
 int fvm_thermal_setup(FvmCtx* c);
+int fvm_k_writedata_fields(FvmCtx* c);
 int fvm_k_correct(FvmCtx* c, double sign);
 int fvm_k_psource_correct(FvmCtx* c, double sign, bool with_cfl);
 int fvm_k_correct_bcu_contra(FvmCtx* c);

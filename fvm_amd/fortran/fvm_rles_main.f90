@@ -4,8 +4,10 @@
 ! subroutine of the step is one entry point of the C ABI acting on the device-resident state.
 ! Inputs are the reference's own files: the grid file NNNxNNNxNNN.dat (grid.f90:46-67) and
 ! RESTART000.DAT (init.f90:183-208).  Outputs: NU_CD_CL.PLT (calccoef.f90:186-187), the console
-! lines of calccoef.f90:190-198, CS.PLT / CT.PLT (stress_tensor_dy.f90:519-522, 707-711) and
-! RESTART000.DAT every NITER_R steps and at the end (writedata.f90:193-216).  Run constants
+! lines of calccoef.f90:190-198, CS.PLT / CT.PLT (stress_tensor_dy.f90:519-522, 707-711), the
+! nodal Tecplot fields PVT_nnn_000.PLT (x, y, z, p, u, v, w, T) and VOR_nnn_000.PLT (x, y, z,
+! |omega|, lambda_2) every NITER_W steps (writedata.f90:8-189) and RESTART000.DAT every NITER_R
+! steps and at the end (writedata.f90:193-216).  Run constants
 ! come from the namelist file fvm_rles.nml (the reference compiles them in,
 ! comdat_param.f90:6-48).  The passive temperature is advanced as in the reference
 ! (THERMAL = 1, Peclet number PE) or carried through the restart unchanged with NU = 0
@@ -16,11 +18,11 @@ PROGRAM FVM_RLES
   IMPLICIT NONE
 
   INTEGER :: NI = 0, NJ = 0, NK = 0, NSWPP = 50, USE_LES = 1, POISSON_MODE = 2
-  INTEGER :: NITER_R = HUGE(1), NSTEP = HUGE(1), AF_MODE = 0, THERMAL = 1
+  INTEGER :: NITER_R = HUGE(1), NSTEP = HUGE(1), AF_MODE = 0, THERMAL = 1, NITER_W = HUGE(1)
   DOUBLE PRECISION :: RE = 3900.D0, DTIME = 2.D-3, RESIMAX = 1.D-8, REF_A = -1.D0
-  DOUBLE PRECISION :: RTIME_MAX = HUGE(1.D0), PE = 2769.D0
+  DOUBLE PRECISION :: RTIME_MAX = HUGE(1.D0), PE = 2769.D0, PR = 0.71D0
   NAMELIST /FVM_RUN/ NI, NJ, NK, RE, DTIME, RESIMAX, NSWPP, USE_LES, REF_A, POISSON_MODE, &
-       RTIME_MAX, NITER_R, NSTEP, AF_MODE, THERMAL, PE
+       RTIME_MAX, NITER_R, NSTEP, AF_MODE, THERMAL, PE, PR, NITER_W
 
   TYPE(FVM_PARAMS) :: PRM
   TYPE(FVM_STEP_INFO) :: INFO
@@ -28,7 +30,7 @@ PROGRAM FVM_RLES
   DOUBLE PRECISION, ALLOCATABLE :: XG(:), YG(:), ZG(:)
   DOUBLE PRECISION, ALLOCATABLE :: U(:), V(:), W(:), P(:), T(:), NLX(:), NLY(:), NLZ(:), ADV(:)
   DOUBLE PRECISION :: RE_F, PE_F, DT_F, RTIME, RESIP, C_AVG, PREF, NU
-  INTEGER :: NITER_TIME, NITER, ISTEP, NNODE, NCELL, Q, IU, IOS
+  INTEGER :: NITER_TIME, NITER, ISTEP, NNODE, NCELL, Q, IU, IOS, NUM_WRITE = 0
   CHARACTER(LEN=256) :: NMLFILE
   CHARACTER(LEN=15) :: GRIDFILE
   LOGICAL :: EXISTS
@@ -142,6 +144,7 @@ PROGRAM FVM_RLES
     WRITE (*, 103) INFO%CD, INFO%CD_P, INFO%CD_V
     WRITE (*, 104) INFO%CL, INFO%CL_P, INFO%CL_V, INFO%CPB
     WRITE (*, 105) NITER, RESIP, C_AVG, PREF
+    IF (MOD(NITER_TIME, NITER_W) .EQ. 0) CALL WRITE_FIELDS()              ! main.f90:154
     IF (MOD(NITER_TIME, NITER_R) .EQ. 0) CALL WRITE_RESTART()
   END DO
   CALL WRITE_RESTART()
@@ -159,6 +162,58 @@ PROGRAM FVM_RLES
 129 FORMAT(TR1,E24.16,TR1,E24.16,TR1,E24.16,TR1,E24.16,TR1,E24.16)
 
 CONTAINS
+
+  ! nodal Tecplot fields (writedata.f90:115-189): PVT_nnn_000.PLT and VOR_nnn_000.PLT, nodes
+  ! I = 1..NI, J = 1..NJ, K = 1..NK (one rank: the rank field of the name is 000)
+  SUBROUTINE WRITE_FIELDS()
+    INTEGER :: JU, I, J, K, C, N
+    DOUBLE PRECISION, ALLOCATABLE :: F(:,:)
+    CHARACTER(LEN=15) :: FNAME
+    INTEGER(C_INT) :: SEL(7)
+    SEL = (/ FVM_PG, FVM_UG, FVM_VG, FVM_WG, FVM_TG, FVM_VORG, FVM_LAMDAG /)
+    ALLOCATE (F(NCELL, 7))
+    CALL FVM_CHECK(FVM_WRITEDATA_FIELDS(CTX), 'WRITEDATA fields')
+    DO N = 1, 7
+      CALL FVM_CHECK(FVM_GET_FIELD(CTX, SEL(N), F(:, N)), 'fvm_get_field')
+    END DO
+    NUM_WRITE = NUM_WRITE + 1
+    FNAME = 'PVT_000_000.PLT'
+    WRITE (FNAME(5:7), '(I3.3)') NUM_WRITE
+    OPEN (NEWUNIT=JU, FILE=FNAME, STATUS='REPLACE')
+    WRITE (JU, *) 'TITLE="', 'RE', RE, 'PR', PR, '"'
+    WRITE (JU, *) 'variables=x,y,z,p,u,v,w,T'
+    WRITE (JU, 156) NI, NJ, NK, RTIME
+    DO K = 1, NK
+      DO J = 1, NJ
+        DO I = 1, NI
+          C = 1 + I + (NI + 1) * (J + (NJ + 1) * K)
+          Q = I + NI * ((J - 1) + NJ * (K - 1))
+          WRITE (JU, 161) XG(Q), YG(Q), ZG(Q), F(C, 1), F(C, 2), F(C, 3), F(C, 4), F(C, 5)
+        END DO
+      END DO
+    END DO
+    CLOSE (JU)
+    FNAME = 'VOR_000_000.PLT'
+    WRITE (FNAME(5:7), '(I3.3)') NUM_WRITE
+    OPEN (NEWUNIT=JU, FILE=FNAME, STATUS='REPLACE')
+    WRITE (JU, *) 'TITLE="', 'RE', RE, 'PR', PR, '"'
+    WRITE (JU, *) 'variables=x,y,z,vorM,lamda'
+    WRITE (JU, 156) NI, NJ, NK, RTIME
+    DO K = 1, NK
+      DO J = 1, NJ
+        DO I = 1, NI
+          C = 1 + I + (NI + 1) * (J + (NJ + 1) * K)
+          Q = I + NI * ((J - 1) + NJ * (K - 1))
+          WRITE (JU, 183) XG(Q), YG(Q), ZG(Q), F(C, 6), F(C, 7)
+        END DO
+      END DO
+    END DO
+    CLOSE (JU)
+    DEALLOCATE (F)
+156 FORMAT(TR1,'ZONE T="WHOLE", I=',I6,', J=',I6,', K=',I6,', SOLUTIONTIME=',F10.4)
+161 FORMAT(TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9)
+183 FORMAT(TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9)
+  END SUBROUTINE WRITE_FIELDS
 
   ! RESTART000.DAT in the reference's layout (writedata.f90:193-216)
   SUBROUTINE WRITE_RESTART()

@@ -96,6 +96,8 @@ class Solver3D:
     def p_source(self): capi.check(self.L.fvm_p_source(self.h), "p_source")
     def iter_u(self): capi.check(self.L.fvm_iter_u(self.h), "iter_u")
     def iter_t(self): capi.check(self.L.fvm_iter_t(self.h), "iter_t")
+    def writedata_fields(self):
+        capi.check(self.L.fvm_writedata_fields(self.h), "writedata_fields")
     def correct(self, sign): capi.check(self.L.fvm_correct(self.h, float(sign)), "correct")
 
     def stress_tensor_dy(self):

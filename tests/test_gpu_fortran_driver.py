@@ -30,7 +30,7 @@ def test_fortran_driver_two_steps(tmp_path):
     (tmp_path / "fvm_rles.nml").write_text(
         f"&FVM_RUN NI={ni}, NJ={nj}, NK={nk}, RE={float(g['re'])!r}, DTIME={float(g['dt'])!r},\n"
         f" RESIMAX=1.0D-13, NSWPP=50, USE_LES=1, REF_A={float(g['ref_a'])!r},"
-        f" POISSON_MODE=2, NSTEP=2, THERMAL=1, PE={float(g['pe'])!r} /\n")
+        f" POISSON_MODE=2, NSTEP=2, THERMAL=1, PE={float(g['pe'])!r}, NITER_W=2 /\n")
     print("fortran driver: rles_9x21x5, 2 steps", flush=True)
     out = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
@@ -49,6 +49,18 @@ def test_fortran_driver_two_steps(tmp_path):
         assert abs(rows[st, 3] - ref[7]) <= 1e-9 * max(abs(ref[7]), 1.0)  # CL
         assert abs(rows[st, 9] - ref[4]) <= 1e-10 * ref[4]           # CFL
         assert abs(rows[st, 1] - ref[10]) <= 1e-9 * abs(ref[10])     # NU
+    # nodal Tecplot fields of the last step (D16.9 records: 10 significant digits)
+    nn = ni * nj * nk
+    for fname, cols in (("PVT_001_000.PLT", ("out_pg", "out_ug", "out_vg", "out_wg", "out_tg")),
+                        ("VOR_001_000.PLT", ("out_dum3d", "out_lamdag"))):
+        lines = (tmp_path / fname).read_text().splitlines()
+        assert lines[2].split(",")[1].split("=")[1].strip() == str(ni)
+        vals = np.array(" ".join(lines[3:]).replace("D", "e").split(), dtype=np.float64)
+        vals = vals.reshape(nn, 3 + len(cols))
+        for q, key in enumerate(cols):
+            ref = g[key][1:, 1:, 1:] if key == "out_dum3d" else g[key]
+            ref = ref.ravel(order="F")
+            assert np.abs(vals[:, 3 + q] - ref).max() <= 1e-8 * np.abs(ref).max(), (fname, key)
     for fname, key in (("CS.PLT", "cs_sqrt"), ("CT.PLT", "ct_sqrt")):
         body = (tmp_path / fname).read_text().split("\n", 1)[1]
         vals = np.array(body.split(), dtype=np.float64).reshape(-1, 2)
