@@ -21,6 +21,7 @@ FIELDS = {
     "vn": 32, "wn": 33, "t": 34, "adv": 35, "dift": 36, "sgst": 37, "kappa": 38, "that": 39,
     "dtdx": 40, "dtdy": 41, "dtdz": 42, "tn": 43, "vorm": 44, "lamda2": 45, "pg": 46,
     "ug": 47, "vg": 48, "wg": 49, "tg": 50, "vorg": 51, "lamdag": 52,
+    "yplus": 53,
 }
 
 STATUS = {0: "ok", 1: "bad argument", 2: "HIP runtime error", 3: "non-finite value",

@@ -132,6 +132,7 @@ static double* field_ptr(FvmCtx* c, int f) {
     case FVM_LAMDA2: return c->lam2;
     case FVM_PG: case FVM_UG: case FVM_VG: case FVM_WG: case FVM_TG: case FVM_VORG:
     case FVM_LAMDAG: return c->nodal[f - FVM_PG];
+    case FVM_YPLUS: return c->yplus;
     default: return nullptr;
   }
 }

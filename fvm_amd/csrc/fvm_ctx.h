@@ -109,6 +109,7 @@ struct FvmCtx {
   // nodal averages of p, u, v, w, T, |omega|, lambda_2 at the node's cell index
   double *vorm = nullptr, *lam2 = nullptr;
   double* nodal[7] = {};
+  double* yplus = nullptr;  // wall y+ of CALCCOEF on the I = 1 cells (zero elsewhere)
   // last-step diagnostics
   fvm_step_info last{};
   // profiling

@@ -2,8 +2,9 @@
 // calc_vor.f90:1-141, find_root.f90:1-27): the vorticity magnitude of CALC_VOR, the lambda_2
 // vortex criterion (the middle eigenvalue of S^2 + Omega^2 from the trigonometric roots of its
 // characteristic cubic) and the 8-cell nodal averages of p, u, v, w, T, |omega| and lambda_2
-// that the reference writes to its Tecplot files.  Every expression follows the reference's
-// association order; the transcendental functions are the device's.
+// that the reference writes to its Tecplot files, and the wall y+ of CALCCOEF that it writes to
+// its YPLUS contour.  Every expression follows the reference's association order; the
+// transcendental functions are the device's.
 #include "fvm_ctx.h"
 
 namespace {
@@ -130,6 +131,34 @@ __global__ void k_nodal(Dims d, NodalSet ns) {
   }
 }
 
+// YPLUS of CALCCOEF on the wall cells (calccoef.f90:90-160): per triangle, the wall shear rate
+// |u x n|/d_n of the wall cell's velocity, weighted by the triangle's share of the cell's wall
+// area; the first triangle assigns, the second adds.  One thread per wall cell (J, K); the wall
+// table holds A, B, C, |n|, d_n, area per triangle, the second triangles after the first.
+__global__ void k_yplus(Dims d, const double* __restrict__ wall, const double* __restrict__ u,
+                        const double* __restrict__ v, const double* __restrict__ w, double re,
+                        double* yplus) {
+  const long nw = (long)(d.nj - 1) * (d.nk - 1);
+  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (r >= nw) return;
+  const int j = 1 + (int)(r % (d.nj - 1)), k = 1 + (int)(r / (d.nj - 1));
+  const long c = cidx(d, 1, j, k);
+  const double U = u[c], V = v[c], W = w[c];
+  const double a1 = wall[6 * r + 5], a2 = wall[6 * (nw + r) + 5];
+  double yp = 0.0;
+  for (int tri = 0; tri < 2; ++tri) {
+    const double* o = wall + 6 * (tri * nw + r);
+    const double A = o[0], B = o[1], C = o[2], dum = o[3], dn = o[4], ar = o[5];
+    const double t1 = V * C / dum - W * B / dum, t2 = U * C / dum - W * A / dum,
+                 t3 = U * B / dum - V * A / dum;
+    double dudn = sqrt(t1 * t1 + t2 * t2 + t3 * t3);
+    dudn = dudn / dn;
+    const double y = sqrt(re * fabs(dudn)) * dn * ar / (a1 + a2);
+    yp = tri == 0 ? y : yp + y;
+  }
+  yplus[c] = yp;
+}
+
 }  // namespace
 
 int fvm_k_writedata_fields(FvmCtx* c) {
@@ -138,7 +167,7 @@ int fvm_k_writedata_fields(FvmCtx* c) {
   if (d.ni < 4 || c->two_d) return FVM_ERR_ARG;  // three-point extrapolation needs I = 1..3
   if (!c->vorm) {
     double** arr[] = {&c->vorm, &c->lam2, &c->nodal[0], &c->nodal[1], &c->nodal[2],
-                      &c->nodal[3], &c->nodal[4], &c->nodal[5], &c->nodal[6]};
+                      &c->nodal[3], &c->nodal[4], &c->nodal[5], &c->nodal[6], &c->yplus};
     for (double** q : arr) {
       HIP_CHECK(hipMalloc((void**)q, d.n * sizeof(double)));
       c->allocs.push_back(*q);
@@ -160,6 +189,8 @@ int fvm_k_writedata_fields(FvmCtx* c) {
                   c->nodal[5], c->nodal[6]}};
   const long nn = (long)d.ni * d.nj * d.nk;
   hipLaunchKernelGGL(k_nodal, dim3(grid1(nn, 256).x, 7), dim3(256), 0, st, d, ns);
+  hipLaunchKernelGGL(k_yplus, grid1(nf, 256), dim3(256), 0, st, d, c->wall, c->u, c->v, c->w,
+                     c->prm.re, c->yplus);
   HIP_CHECK(hipGetLastError());
   return FVM_OK;
 }

@@ -163,20 +163,37 @@ PROGRAM FVM_RLES
 
 CONTAINS
 
-  ! nodal Tecplot fields (writedata.f90:115-189): PVT_nnn_000.PLT and VOR_nnn_000.PLT, nodes
-  ! I = 1..NI, J = 1..NJ, K = 1..NK (one rank: the rank field of the name is 000)
+  ! Tecplot output of WRITEDATA (writedata.f90:8-189), one rank (the rank field of each name
+  ! is 000): the wall y+ on the wall cells J = 1..NJ-1, K = 1..NK-1 to Ypl_nnn_000.PLT (the
+  ! reference composes this name but opens its CONTOUR variable, writedata.f90:14-20), and the
+  ! nodal fields on the nodes I = 1..NI, J = 1..NJ, K = 1..NK to PVT_nnn_000.PLT and
+  ! VOR_nnn_000.PLT
   SUBROUTINE WRITE_FIELDS()
     INTEGER :: JU, I, J, K, C, N
     DOUBLE PRECISION, ALLOCATABLE :: F(:,:)
     CHARACTER(LEN=15) :: FNAME
-    INTEGER(C_INT) :: SEL(7)
-    SEL = (/ FVM_PG, FVM_UG, FVM_VG, FVM_WG, FVM_TG, FVM_VORG, FVM_LAMDAG /)
-    ALLOCATE (F(NCELL, 7))
+    INTEGER(C_INT) :: SEL(11)
+    SEL = (/ FVM_PG, FVM_UG, FVM_VG, FVM_WG, FVM_TG, FVM_VORG, FVM_LAMDAG, FVM_X, FVM_Y, &
+             FVM_Z, FVM_YPLUS /)
+    ALLOCATE (F(NCELL, 11))
     CALL FVM_CHECK(FVM_WRITEDATA_FIELDS(CTX), 'WRITEDATA fields')
-    DO N = 1, 7
+    DO N = 1, 11
       CALL FVM_CHECK(FVM_GET_FIELD(CTX, SEL(N), F(:, N)), 'fvm_get_field')
     END DO
     NUM_WRITE = NUM_WRITE + 1
+    FNAME = 'Ypl_000_000.PLT'
+    WRITE (FNAME(5:7), '(I3.3)') NUM_WRITE
+    OPEN (NEWUNIT=JU, FILE=FNAME, STATUS='REPLACE')
+    WRITE (JU, *) 'TITLE="YPLUS"'
+    WRITE (JU, *) 'variables="x","y","z","Yplus"'
+    WRITE (JU, 132) 1, NJ - 1, NK - 1, RTIME
+    DO K = 1, NK - 1
+      DO J = 1, NJ - 1
+        C = 2 + (NI + 1) * (J + (NJ + 1) * K)
+        WRITE (JU, 137) F(C, 8), F(C, 9), F(C, 10), F(C, 11)
+      END DO
+    END DO
+    CLOSE (JU)
     FNAME = 'PVT_000_000.PLT'
     WRITE (FNAME(5:7), '(I3.3)') NUM_WRITE
     OPEN (NEWUNIT=JU, FILE=FNAME, STATUS='REPLACE')
@@ -210,6 +227,8 @@ CONTAINS
     END DO
     CLOSE (JU)
     DEALLOCATE (F)
+132 FORMAT(TR1,'ZONE T="POST", I=',I6,', j=',I6,', k=',I6,', SOLUTIONTIME=',F10.4)
+137 FORMAT(TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9)
 156 FORMAT(TR1,'ZONE T="WHOLE", I=',I6,', J=',I6,', K=',I6,', SOLUTIONTIME=',F10.4)
 161 FORMAT(TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9)
 183 FORMAT(TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9)

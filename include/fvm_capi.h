@@ -77,9 +77,10 @@ enum {
   FVM_DTDX = 40, FVM_DTDY = 41, FVM_DTDZ = 42, FVM_TN = 43,
   /* field output (after fvm_writedata_fields): |omega| and lambda_2 on the cells, and the
      nodal averages PG, UG, VG, WG, TG, |omega| and lambda_2 held at the node's cell index
-     (nodes I = 1..NI, J = 1..NJ, K = 1..NK of the cell-array layout) */
+     (nodes I = 1..NI, J = 1..NJ, K = 1..NK of the cell-array layout); the wall y+ of CALCCOEF
+     on the I = 1 cells (zero elsewhere) */
   FVM_VORM = 44, FVM_LAMDA2 = 45, FVM_PG = 46, FVM_UG = 47, FVM_VG = 48, FVM_WG = 49,
-  FVM_TG = 50, FVM_VORG = 51, FVM_LAMDAG = 52, FVM_NFIELDS = 53
+  FVM_TG = 50, FVM_VORG = 51, FVM_LAMDAG = 52, FVM_YPLUS = 53, FVM_NFIELDS = 54
 };
 
 /* GRID + METRIC + the geometric part of INIT (RLES/grid.f90:1-367, RLES/metric.f90:1-538,
@@ -121,8 +122,9 @@ int fvm_set_thermal(fvm_ctx* ctx, double pe);
 /* Field output of WRITEDATA (RLES/writedata.f90:8-133, calc_vor.f90:1-141, find_root.f90:1-27):
  * the vorticity magnitude of CALC_VOR, lambda_2 (middle eigenvalue of S^2 + Omega^2, kept where
  * no root lies strictly between the other two, extrapolated to the wall and far-field ghost
- * planes, periodic ghosts) and the nodal averages of p, u, v, w, T, |omega| and lambda_2, for
- * the current state; read them with fvm_get_field.  3D contexts with NI >= 4. */
+ * planes, periodic ghosts), the nodal averages of p, u, v, w, T, |omega| and lambda_2 and the
+ * wall y+ that CALCCOEF computes and WRITEDATA writes (calccoef.f90:121, 159; writedata.f90:
+ * 14-32), for the current state; read them with fvm_get_field.  3D contexts with NI >= 4. */
 int fvm_writedata_fields(fvm_ctx* ctx);
 
 /* Change DTIME between steps.  DTIME is a run-time module variable of the reference, reassigned

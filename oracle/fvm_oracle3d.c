@@ -44,6 +44,7 @@ typedef struct orc3d {
   double *t, *tn, *adv, *adv1, *dift, *sgst, *th, *dtdx, *dtdy, *dtdz, *kappa;
   /* field output (WRITEDATA): cell |omega|, lambda_2; node averages at IX(I,J,K), I=1..NI */
   double *vorm, *lam2, *pg, *ug, *vg, *wg, *tg, *vorg, *lamg;
+  double *yplus; /* wall-cell y+ of CALCCOEF on the I = 1 plane (calccoef.f90:121, 159) */
   double c_avg, c_t, pe, nu, sum_area;
   /* scalars of the last step */
   double uin, resip, pref, cfl, cpb, cd, cl, cdp, cdv, clp, clv;
@@ -75,7 +76,7 @@ orc3d *orc3d_create(int ni, int nj, int nk, double re, double dtime, double ref_
                     &o->dumin, &o->dumout, &o->lij, &o->mij, &o->lm, &o->mm, &o->nut,
                     &o->sgsx, &o->sgsy, &o->sgsz, &o->t, &o->tn, &o->adv, &o->adv1,
                     &o->dift, &o->sgst, &o->th, &o->dtdx, &o->dtdy, &o->dtdz, &o->kappa,
-                    &o->vorm, &o->lam2, &o->pg, &o->ug, &o->vg, &o->wg, &o->tg, &o->vorg,
+                    &o->vorm, &o->lam2, &o->pg, &o->ug, &o->vg, &o->wg, &o->tg, &o->vorg, &o->yplus,
                     &o->lamg};
   for (size_t a = 0; a < sizeof(all) / sizeof(all[0]); ++a) *all[a] = alloc0(o->n);
   return o;
@@ -84,8 +85,8 @@ orc3d *orc3d_create(int ni, int nj, int nk, double re, double dtime, double ref_
 void orc3d_destroy(orc3d *o) {
   if (!o) return;
   double **p = &o->xg;
-  /* every double* member from xg to lamg is an allocation */
-  double **end = &o->lamg;
+  /* every double* member from xg to yplus is an allocation */
+  double **end = &o->yplus;
   for (; p <= end; ++p) free(*p);
   free(o);
 }
@@ -107,7 +108,7 @@ double *orc3d_array(orc3d *o, const char *name) {
   N("sgsz", sgsz) N("u1", u1) N("u2", u2) N("u3", u3) N("t", t) N("tn", tn) N("adv", adv)
   N("adv1", adv1) N("dift", dift) N("sgst", sgst) N("that", th) N("dtdx", dtdx)
   N("dtdy", dtdy) N("dtdz", dtdz) N("kappa", kappa) N("vorm", vorm) N("lamda2", lam2)
-  N("pg", pg) N("ug", ug) N("vg", vg) N("wg", wg) N("tg", tg) N("vorg", vorg) N("lamdag", lamg)
+  N("pg", pg) N("ug", ug) N("vg", vg) N("wg", wg) N("tg", tg) N("vorg", vorg) N("lamdag", lamg) N("yplus", yplus)
 #undef N
   return NULL;
 }
@@ -1190,6 +1191,9 @@ void orc3d_calccoef(orc3d *o) {
         tly = tly + (2.0 / o->re) * dudn * ar * V / um;
         double dtdn = (o->t[c] - 1.0) / dn; /* (T - T_WALL) / D_N (calccoef.f90:117-119) */
         sum_nu = sum_nu + dtdn * ar;
+        /* YPLUS: the first triangle assigns, the second adds (calccoef.f90:121, 159) */
+        double yp = sqrt(o->re * fabs(dudn)) * dn * ar / (o->area1[c] + o->area2[c]);
+        o->yplus[c] = tri == 0 ? yp : o->yplus[c] + yp;
       }
   o->nu = -sum_nu / o->sum_area;
   o->cdp = pd / o->ref_a;

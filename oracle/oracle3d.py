@@ -105,6 +105,10 @@ class Oracle3D:
         """|omega|, lambda_2 and the nodal averages of WRITEDATA for the current state."""
         self.L.orc3d_writedata_fields(self.h)
 
+    def calccoef(self):
+        """CALCCOEF on the current state: force coefficients, CFL, NU and the wall y+."""
+        self.L.orc3d_calccoef(self.h)
+
     def set_no_pref(self, on=True):
         self.L.orc3d_set_no_pref(self.h, 1 if on else 0)
 

@@ -160,6 +160,10 @@ PROGRAM HARNESS3D
   CALL DUMPN('out_pg', PG); CALL DUMPN('out_ug', UG); CALL DUMPN('out_vg', VG)
   CALL DUMPN('out_wg', WG); CALL DUMPN('out_tg', TG); CALL DUMPN('out_lamdag', LAMDAG)
   CALL DUMP('out_dum3d', DUM_3D)
+  ! YPLUS of the last CALCCOEF: the wall plane only, YPLUS(IS1, JS1:JE1, KS1:KE1) (init.f90:81)
+  OPEN(UNIT=62, FILE='out_yplus.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+  WRITE(62) YPLUS
+  CLOSE(62)
   CALL MPI_FINALIZE(ERR)
 
 CONTAINS

@@ -111,6 +111,9 @@ def run_case(label, c):
             for t in ("out_pg", "out_ug", "out_vg", "out_wg", "out_tg", "out_lamdag"):
                 out[t] = np.fromfile(os.path.join(d, t + ".bin")).reshape((ni, nj, nk),
                                                                           order="F")
+            # YPLUS of the last CALCCOEF on the wall cells (J, K = 1..N-1)
+            out["out_yplus"] = np.fromfile(os.path.join(d, "out_yplus.bin")).reshape(
+                (nj - 1, nk - 1), order="F")
             # sqrt of the dynamic coefficients per step, as the reference writes them
             # (stress_tensor_dy.f90:519-522, 707-711; list-directed RTIME, DSQRT(C_AVG))
             for f, key in (("CS.PLT", "cs_sqrt"), ("CT.PLT", "ct_sqrt")):
@@ -168,19 +171,6 @@ def run_case2d(label, c):
         rows = [l.split() for l in open(os.path.join(d, "scalars.txt")) if l.startswith("step")]
         # columns: step, niter, resip, cfl, cpb, cd, cl, cd_p, cd_v, cl_p, cl_v
         out["scalars"] = np.array([[float(x) for x in r[1:]] for r in rows])
-        if c.get("t_sigma"):
-            # field output of the last step (WRITEDATA): lambda_2, |omega| on the cells, the
-            # nodal averages on nodes 1..N
-            for t in ("out_lamda2", "out_vorm", "out_dum3d"):
-                out[t] = rd(t)
-            for t in ("out_pg", "out_ug", "out_vg", "out_wg", "out_tg", "out_lamdag"):
-                out[t] = np.fromfile(os.path.join(d, t + ".bin")).reshape((ni, nj, nk),
-                                                                          order="F")
-            # sqrt of the dynamic coefficients per step, as the reference writes them
-            # (stress_tensor_dy.f90:519-522, 707-711; list-directed RTIME, DSQRT(C_AVG))
-            for f, key in (("CS.PLT", "cs_sqrt"), ("CT.PLT", "ct_sqrt")):
-                vals = [l.split() for l in open(os.path.join(d, f)) if l.strip()]
-                out[key] = np.array([float(v[1].replace("D", "e")) for v in vals])
     np.savez_compressed(os.path.join(HERE, label + ".npz"), **out)
     print("wrote", label, "steps", c["steps"], "niter", out["scalars"][:, 1])
 

@@ -61,6 +61,15 @@ def test_fortran_driver_two_steps(tmp_path):
             ref = g[key][1:, 1:, 1:] if key == "out_dum3d" else g[key]
             ref = ref.ravel(order="F")
             assert np.abs(vals[:, 3 + q] - ref).max() <= 1e-8 * np.abs(ref).max(), (fname, key)
+    # wall y+ contour of the last step on the wall cells, with their centres
+    lines = (tmp_path / "Ypl_001_000.PLT").read_text().splitlines()
+    assert [int(f.split("=")[1]) for f in lines[2].split(",")[1:4]] == [1, nj - 1, nk - 1]
+    vals = np.array(" ".join(lines[3:]).replace("D", "e").split(), dtype=np.float64)
+    vals = vals.reshape((nj - 1) * (nk - 1), 4)
+    for q, ref in enumerate((g["setup_x"][1, 1:nj, 1:nk], g["setup_y"][1, 1:nj, 1:nk],
+                             g["setup_z"][1, 1:nj, 1:nk], g["out_yplus"])):
+        ref = ref.ravel(order="F")
+        assert np.abs(vals[:, q] - ref).max() <= 1e-8 * np.abs(ref).max(), ("Ypl", q)
     for fname, key in (("CS.PLT", "cs_sqrt"), ("CT.PLT", "ct_sqrt")):
         body = (tmp_path / fname).read_text().split("\n", 1)[1]
         vals = np.array(body.split(), dtype=np.float64).reshape(-1, 2)

@@ -1,7 +1,7 @@
 """GPU parity of the field output of WRITEDATA (|omega| of CALC_VOR, lambda_2 by FIND_ROOT with
 its ghost-face extrapolation, nodal averages) on the reference's own state after two thermal
-steps.  |omega| and the nodal averages use only +, -, *, sqrt in the reference's order
-(bit-exact); lambda_2 goes through acos, cos, sin and pow, whose device implementations differ
+steps, and the wall y+ of CALCCOEF.  |omega| and the nodal averages use only +, -, *, sqrt in
+the reference's order (bit-exact); lambda_2 goes through acos, cos, sin and pow, whose device implementations differ
 from the host's in the last bits (1e-12 relative to the field maximum)."""
 import numpy as np
 import pytest
@@ -31,4 +31,7 @@ def test_writedata_fields_match_reference():
     assert np.array_equal(s.get("vorg")[1:, 1:, 1:], g["out_dum3d"][1:, 1:, 1:])
     lg = s.get("lamdag")[1:, 1:, 1:]
     assert np.abs(lg - g["out_lamdag"]).max() <= 1e-12 * np.abs(g["out_lamdag"]).max()
+    # wall y+ of the last CALCCOEF: +, *, /, sqrt of the wall table in the reference's order
+    yp = s.get("yplus")[1, 1:nj, 1:nk]
+    assert np.abs(yp - g["out_yplus"]).max() <= 1e-14 * np.abs(g["out_yplus"]).max()
     s.close()

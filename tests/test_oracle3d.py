@@ -224,6 +224,47 @@ def test_field_output_bitexact():
                  ("tg", "out_tg"), ("lamdag", "out_lamdag")):
         assert np.array_equal(o.array(a)[1:, 1:, 1:], g[t]), a
     assert np.array_equal(o.array("vorg")[1:, 1:, 1:], g["out_dum3d"][1:, 1:, 1:])
+    nj, nk = int(g["nj"]), int(g["nk"])
+    assert np.array_equal(o.array("yplus")[1, 1:nj, 1:nk], g["out_yplus"])
+
+
+def test_yplus_geometry():
+    """The wall y+ of CALCCOEF from vector geometry: for each wall triangle of the cell's wall
+    face, n the unit normal of its plane, d_n the distance of the cell centre from that plane
+    and u_t = u - (u.n) n the tangential velocity, y+ is the area-weighted sum over the two
+    triangles of sqrt(Re |u_t| / d_n) d_n."""
+    g = load_golden("rles_9x21x5_les_thermal")
+    o = make(g)
+    o.set_state(g["s2_u"], g["s2_v"], g["s2_w"], g["s2_pref"], g["in_nlx"], g["in_nly"],
+                g["in_nlz"])
+    o.calccoef()
+    nj, nk, re = int(g["nj"]), int(g["nk"]), float(g["re"])
+    P = np.stack([o.array(a)[1] for a in ("xg", "yg", "zg")], axis=-1)   # wall nodes (j, k)
+    C = np.stack([o.array(a)[1] for a in ("x", "y", "z")], axis=-1)      # wall-cell centres
+    U = np.stack([o.array(a)[1] for a in ("u", "v", "w")], axis=-1)
+    J, K = slice(1, nj), slice(1, nk)
+    J1, K1 = slice(2, nj + 1), slice(2, nk + 1)
+    tris = ((P[J, K], P[J1, K], P[J, K1]), (P[J1, K1], P[J, K1], P[J, K]))
+    area, parts, dns = 0.0, [], []
+    for a, b, c in tris:
+        n = np.cross(b - a, c - a)
+        ar = 0.5 * np.linalg.norm(n, axis=-1)
+        n = n / np.linalg.norm(n, axis=-1)[..., None]
+        dn = np.abs(np.sum((C[J, K] - a) * n, axis=-1))
+        ut = U[J, K] - np.sum(U[J, K] * n, axis=-1)[..., None] * n
+        parts.append((np.sqrt(re * np.linalg.norm(ut, axis=-1) / dn) * dn, ar))
+        dns.append((dn, ar))
+        area = area + ar
+    want = sum(y * ar for y, ar in parts) / area
+    got = o.array("yplus")[1, J, K]
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    # a slip velocity along the span (the cylinder's axis) gives sqrt(Re |w| d_n) per triangle
+    z = np.zeros(o.shape, order="F")
+    o.set_state(z, z, np.full(o.shape, 0.7, order="F"), z, z, z, z)
+    o.calccoef()
+    want = sum(np.sqrt(re * 0.7 * dn) * ar for dn, ar in dns) / area
+    got = o.array("yplus")[1, J, K]
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
 
 
 def test_lambda2_closed_forms():
