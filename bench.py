@@ -55,8 +55,9 @@ KERNEL_BYTES = {
     "dif": (8 * 12, 1),
     # P_SOURCE + projection: {p, J, xi 9, u, v, w} -> {S 3, u, v, w}
     "psource_correct": (8 * 20, 1),
-    # Germano identity terms: {u,v,w, u^ 3, xi 9, S 6, |S|, Delta^2} -> {LM, MM}
-    "k_lm": (8 * 25, 1),
+    # Germano identity terms: {u,v,w, u^ 3, xi 9, S 6, Delta^2} -> {LM, MM} (|S| is formed
+    # from S as it is staged)
+    "k_lm": (8 * 24, 1),
     # SGS divergence: {xi 9, nu_t, S 6, J, NL 3} -> {NL 3, SGS 3}
     "k_sgs_div": (8 * 26, 1),
 }

@@ -162,6 +162,15 @@ __device__ inline double shfl_down_d(double v, int o) { return __shfl_down(v, o,
 // a+1 are issued before line a is solved.  Writes the iterate (line-major), the chunk's last line
 // to bnd_out, and per wavefront the largest change of that line between passes and the largest
 // |x| (part[2w], part[2w+1]).
+//
+// Early stop (passes after the first, early != 0): a pass only changes a chunk through its
+// incoming line a0-1, and a change d of line a moves line a+1 by at most rho d, with
+// rho = max_p |cam| / (D - |lower| - |upper|) < 1 by the diagonal dominance of AP (the 2 Re J^-1/dt
+// term).  So once the change of a whole line from the previous pass is <= tol times the largest
+// |x| the wavefront has produced (never more than the global max |x| of the host's test), the
+// remaining lines of the chunk would change by less than that too and keep their values from
+// the previous pass; the chunk's last line is then unchanged (bnd_out = bnd_in).  Where rho is
+// small (rho ~ dt |grad a|^2 / 2 Re), a pass after the first re-solves a few lines per chunk.
 template <int S, int R>
 __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double* __restrict__ beta,
                                                   const double* __restrict__ invd,
@@ -169,7 +178,8 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
                                                   const double* __restrict__ cam,
                                                   const double* const* q, double* const* xo,
                                                   const double* __restrict__ bnd_in,
-                                                  double* bnd_out, int nchunk, double* part) {
+                                                  double* bnd_out, int nchunk, double* part,
+                                                  int early, double tol) {
   const ScanGeo g = scan_geo(d, S);
   const long G = gridDim.x;
   const long lin = blockIdx.x;
@@ -202,7 +212,7 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
       }
       xp[r] = v;
     }
-    double be[R], iv[R], bv[R], cm[R], qv[R];
+    double be[R], iv[R], bv[R], cm[R], qv[R], xv[R];
     auto load = [&](int a) {
       const long lb = g.ls_a * a + g.ls_b * b;
 #pragma unroll
@@ -211,25 +221,29 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
         if (p <= g.n) {
           be[r] = beta[lb + p]; iv[r] = invd[lb + p]; bv[r] = bb[lb + p];
           cm[r] = cam[lb + p]; qv[r] = qc[lb + p];
+          xv[r] = early ? xc[lb + p] : 0.0;  // the previous pass's value of the line
         } else {
-          be[r] = 0.0; iv[r] = 0.0; bv[r] = 0.0; cm[r] = 0.0; qv[r] = 0.0;
+          be[r] = 0.0; iv[r] = 0.0; bv[r] = 0.0; cm[r] = 0.0; qv[r] = 0.0; xv[r] = 0.0;
         }
       }
     };
     load(a0);
+    bool stopped = false;
+    double xrun = 0.0;  // largest |x| of the lines this wavefront solved in this pass
     for (int a = a0; a <= a1; ++a) {
       const long lb = g.ls_a * a + g.ls_b * b;
       const long nb0 = g.ns_a * a + g.ns_b * b;  // natural base of the line
       const double y0 = fc[nb0];                 // row 0 ghost
       const double xn = fc[nb0 + g.ns_p * (g.n + 1)];  // row n+1 ghost
       // forward: y(p) = al(p) + be(p) y(p-1)
-      double al[R], bt[R], bw[R];
+      double al[R], bt[R], bw[R], xo[R];
       double Bt = 1.0, At = 0.0;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         al[r] = (qv[r] + cm[r] * xp[r]) * iv[r];
         bt[r] = be[r];
         bw[r] = bv[r];
+        xo[r] = xv[r];  // line a's previous value (load(a + 1) below refills xv)
         if (p0 + r <= g.n) {
           At = al[r] + bt[r] * At;
           Bt = bt[r] * Bt;
@@ -267,15 +281,33 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
       double De = shfl_down_d(Dt, 1), Ce = shfl_down_d(Ct, 1);
       if (l == 63) { De = 0.0; Ce = 1.0; }
       double xin = De + Ce * xn;
+      double dl = 0.0;
 #pragma unroll
       for (int r = R - 1; r >= 0; --r) {
         const int p = p0 + r;
         if (p <= g.n) {
           const double x = y[r] - bw[r] * xin;
           xin = x;
+          dl = fmax(dl, fabs(x - xo[r]));
+          xrun = fmax(xrun, fabs(x));
           xc[lb + p] = x;
           xp[r] = x;
         }
+      }
+      if (early && a < a1) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          dl = fmax(dl, __shfl_xor(dl, o, 64));
+          xrun = fmax(xrun, __shfl_xor(xrun, o, 64));
+        }
+        if (dl <= tol * xrun) { stopped = true; break; }  // uniform over the wavefront
+      }
+    }
+    if (stopped) {  // the chunk's last line keeps the previous pass's value
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int p = p0 + r;
+        if (p <= g.n) xp[r] = bnd_in[(((long)comp * g.nb + (b - 1)) * nchunk + ca) * g.n + (p - 1)];
       }
     }
     // chunk boundary line: change since the previous pass
@@ -365,7 +397,7 @@ int scan_stage(FvmCtx* c, const AFArgs& A, int* iters, bool temp) {
       ProfScope pk(c, S == 0 ? "k_scan_pass_i" : "k_scan_pass_j");
       hipLaunchKernelGGL((k_scan_pass<S, R>), dim3((unsigned)G), dim3(64), 0, st, d, A, lf[0],
                          lf[1], lf[2], lf[3], (const double* const*)dq, dxo, c->bnd[slot],
-                         c->bnd[slot ^ 1], nchunk, c->red);
+                         c->bnd[slot ^ 1], nchunk, c->red, it > 0 ? 1 : 0, tol);
     }
     hipLaunchKernelGGL(k_scan_max, dim3(1), dim3(256), 0, st, c->red, G, c->scal + 20);
     HIP_CHECK(hipGetLastError());

@@ -178,7 +178,8 @@ __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* con
 // Germano/Lilly contractions LM and MM per interior cell (stress_tensor_dy.f90:236-483).
 // The twelve test-filtered products (u_i u_j and |S| S_ij) are never stored: a workgroup owns a
 // 64 x 4 (I x J) column of cells over LM_KC consecutive K planes, stages each plane of the ten
-// factor fields (with a one-cell halo) in LDS, forms the in-plane (1,2,1)x(1,2,1) sums of the
+// factor fields (with a one-cell halo) in LDS (|S| formed from the six S_ij as it is staged, so
+// only nine fields are read), forms the in-plane (1,2,1)x(1,2,1) sums of the
 // products once per plane and combines three consecutive planes with (1,2,1) weights.  The sum
 // is the reference's 27-point filter with its terms grouped by plane (rounding-level
 // differences only).
@@ -194,7 +195,6 @@ __global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restr
                                             const double* __restrict__ VH,
                                             const double* __restrict__ WH,
                                             const double* const* __restrict__ L,
-                                            const double* __restrict__ mag,
                                             const double* __restrict__ filt2, double* lm_out,
                                             double* mm_out, double* magh_out) {
   const uint3 bb = xcd_block();
@@ -218,9 +218,14 @@ __global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restr
       const int gi = min(bb.x * 64 + q % LM_W, d.ni);
       const int gj = min(bb.y * 4 + q / LM_W, d.nj);
       const long c = cidx(d, gi, gj, kp);
-      sh[0][q] = U[c]; sh[1][q] = V[c]; sh[2][q] = W[c]; sh[3][q] = mag[c];
-      sh[4][q] = L0[c]; sh[5][q] = L1[c]; sh[6][q] = L2[c];
-      sh[7][q] = L3[c]; sh[8][q] = L4[c]; sh[9][q] = L5[c];
+      const double l11 = L0[c], l12 = L1[c], l13 = L2[c], l22 = L3[c], l23 = L4[c], l33 = L5[c];
+      sh[0][q] = U[c]; sh[1][q] = V[c]; sh[2][q] = W[c];
+      sh[4][q] = l11; sh[5][q] = l12; sh[6][q] = l13;
+      sh[7][q] = l22; sh[8][q] = l23; sh[9][q] = l33;
+      // |S| from the stored strain rates in strain_store's order: the value it wrote to |S|
+      const double s2 = l11 * l11 + l12 * l12 + l13 * l13 + l12 * l12 + l22 * l22 + l23 * l23 +
+                        l13 * l13 + l23 * l23 + l33 * l33;
+      sh[3][q] = sqrt(2.0 * s2);
     }
     __syncthreads();
     double Pn[12];
@@ -648,8 +653,8 @@ int fvm_k_sgs(FvmCtx* c) {
     ProfScope pk(c, "k_lm");
     const dim3 gl((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + LM_KC - 1) / LM_KC);
     hipLaunchKernelGGL(k_lm, gl, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
-                       c->uh[0], c->uh[1], c->uh[2], (const double* const*)c->dlss, c->mag,
-                       c->filt2, c->su[0], c->su[1], th ? c->magh : nullptr);
+                       c->uh[0], c->uh[1], c->uh[2], (const double* const*)c->dlss, c->filt2,
+                       c->su[0], c->su[1], th ? c->magh : nullptr);
     HIP_CHECK(hipGetLastError());
   }
   const int nb = 512;
