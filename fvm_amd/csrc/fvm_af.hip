@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
                                                    const double* const* __restrict__ P,
                                                    const double* __restrict__ ap0, double re,
                                                    const double* const* __restrict__ fa,
-                                                   int qform) {
+                                                   int qform, int fn_alias) {
   constexpr double kTWall = 1.0;  // T_WALL (comdat_param.f90:40)
   constexpr int NC = TH ? 1 : 3;  // components: T, or u, v, w
   const uint3 bb = xcd_block();
@@ -211,7 +211,8 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
         su = su + A.dif[comp][c];
         su = su + dd;
       } else if (S == 0) {
-        su = a0 * A.fn[comp][c];
+        // U_N: equal to the staged stage input when nothing has written u since the step start
+        su = a0 * (fn_alias ? F(0, 0, 0) : A.fn[comp][c]);
         su = su - 3.0 * re * A.nl[comp][c];
         su = su + 1.0 * re * A.nl1[comp][c];
         su = su + A.dif[comp][c];
@@ -659,9 +660,8 @@ static int chunk_stage(FvmCtx* c, AFArgs& A, int* iters) {
     }
     hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, st, c->red, G, c->scal + 20);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(c->hscal + 20, c->scal + 20, 2 * sizeof(double),
-                             hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
+    const int rf = fvm_fetch(c, c->scal + 20, 2, c->hscal + 20);
+    if (rf) return rf;
     ++it;
     const double dm = c->hscal[20], xm = c->hscal[21];
     if (!(dm == dm) || !(xm == xm)) {
@@ -689,6 +689,8 @@ static int chunk_stage(FvmCtx* c, AFArgs& A, int* iters) {
 int fvm_k_iter_u(FvmCtx* c) {
   const Dims d = c->d;
   hipStream_t st = c->stream;
+  const int fn_alias = c->u_is_un ? 1 : 0;
+  c->u_is_un = false;  // the stages write u, v, w
   if (d.ni - 1 > FVM_MAX_LINE || d.nj - 1 > FVM_MAX_LINE || d.nk - 1 > FVM_MAX_LINE)
     return FVM_ERR_SIZE;
   const dim3 gi((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
@@ -709,11 +711,11 @@ int fvm_k_iter_u(FvmCtx* c) {
       const dim3 gk((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + RK_KC - 1) / RK_KC);
       if (s == 0) {
         ProfScope pk(c, "k_af_rhs_i");
-        hipLaunchKernelGGL((k_af_rhs_km<0, false>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[0], c->ap0, c->prm.re, fas, qf);
+        hipLaunchKernelGGL((k_af_rhs_km<0, false>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[0], c->ap0, c->prm.re, fas, qf, fn_alias);
       }
       if (s == 1) {
         ProfScope pk(c, "k_af_rhs_j");
-        hipLaunchKernelGGL((k_af_rhs_km<1, false>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[1], c->ap0, c->prm.re, fas, qf);
+        hipLaunchKernelGGL((k_af_rhs_km<1, false>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[1], c->ap0, c->prm.re, fas, qf, 0);
       }
       if (s == 2) hipLaunchKernelGGL(k_af_rhs<2>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf, dz);
       HIP_CHECK(hipGetLastError());
@@ -862,8 +864,8 @@ int fvm_k_iter_t(FvmCtx* c) {
     {
       ProfScope ps(c, "t_rhs");
       const double* const* fas = (const double* const*)c->dfa[s];
-      if (s == 0) hipLaunchKernelGGL((k_af_rhs_km<0, true>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[0], c->ap0t, c->pe, fas, 1);
-      if (s == 1) hipLaunchKernelGGL((k_af_rhs_km<1, true>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[1], c->ap0t, c->pe, fas, 1);
+      if (s == 0) hipLaunchKernelGGL((k_af_rhs_km<0, true>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[0], c->ap0t, c->pe, fas, 1, 0);
+      if (s == 1) hipLaunchKernelGGL((k_af_rhs_km<1, true>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[1], c->ap0t, c->pe, fas, 1, 0);
       if (s == 2) hipLaunchKernelGGL(k_af_rhs<2>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0t, c->pe, fas, 0, c->dd2_zero ? 1 : 0);
       HIP_CHECK(hipGetLastError());
     }

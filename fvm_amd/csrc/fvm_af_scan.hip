@@ -404,9 +404,8 @@ int scan_stage(FvmCtx* c, const AFArgs& A, int* iters, bool temp) {
     slot ^= 1;
     ++it;
     if (it < 2) continue;  // the first pass never suffices on its own
-    HIP_CHECK(hipMemcpyAsync(c->hscal + 20, c->scal + 20, 2 * sizeof(double),
-                             hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
+    const int rf = fvm_fetch(c, c->scal + 20, 2, c->hscal + 20);
+    if (rf) return rf;
     const double dm = c->hscal[20], xm = c->hscal[21];
     if (!(dm == dm) || !(xm == xm)) {
       fprintf(stderr, "fvm_amd: non-finite AF stage %d iterate (pass %d)\n", S, it);

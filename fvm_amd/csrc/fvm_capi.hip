@@ -4,9 +4,50 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "fvm_ctx.h"
+
+// Copies n <= 48 device scalars into the host mailbox, then (after a system-scope fence) the
+// sequence number s: a host that sees s also sees the scalars.  One thread; vector stores.
+__global__ void k_publish(const double* __restrict__ src, int n, volatile double* dst,
+                          volatile unsigned long long* seq, unsigned long long s) {
+  if (threadIdx.x != 0) return;
+  for (int q = 0; q < n; ++q) dst[q] = src[q];
+  __threadfence_system();
+  *seq = s;
+}
+
+// The n device scalars at src, in stream order, into dst on the host.  Enqueues k_publish and
+// spins on the mailbox's sequence number; the stream keeps running (no synchronisation), so
+// the device idles only for the host's reaction and the next launch.  A stream error or a
+// drained stream without the expected number ends the wait with FVM_ERR_HIP.
+int fvm_fetch(FvmCtx* c, const double* src, int n, double* dst) {
+  if (n < 1 || n > 48) return FVM_ERR_ARG;
+  const unsigned long long s = ++c->seq;
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, c->stream, src, n, c->hmail, c->hseq, s);
+  HIP_CHECK(hipGetLastError());
+  volatile unsigned long long* f = c->hseq;
+  for (long it = 1; *f < s; ++it) {
+    if ((it & 1023) == 0) {
+      const hipError_t e = hipStreamQuery(c->stream);
+      if (e == hipSuccess) {
+        if (*f >= s) break;
+        fprintf(stderr, "fvm_amd: stream drained without publishing scalar set %llu\n", s);
+        return FVM_ERR_HIP;
+      }
+      if (e != hipErrorNotReady) {
+        fprintf(stderr, "fvm_amd: HIP error %s while waiting for scalars\n", hipGetErrorString(e));
+        return FVM_ERR_HIP;
+      }
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  const volatile double* m = c->hmail;
+  for (int q = 0; q < n; ++q) dst[q] = m[q];
+  return FVM_OK;
+}
 
 ProfScope::ProfScope(FvmCtx* ctx, const char* nm) : c(ctx), name(nm) {
   if (!c->prof_on) return;
@@ -247,6 +288,13 @@ static int create_impl(const fvm_params* prm, const double* xg, const double* yg
     fvm_destroy(c);
     return FVM_ERR_HIP;
   }
+  if (hipHostMalloc((void**)&c->hmail, 64 * sizeof(double), hipHostMallocCoherent) !=
+      hipSuccess) {
+    fvm_destroy(c);
+    return FVM_ERR_HIP;
+  }
+  c->hseq = reinterpret_cast<unsigned long long*>(c->hmail + 56);
+  *c->hseq = 0;
   // device pointer tables
   double** tabs[8];
   for (int t = 0; t < 8; ++t) {
@@ -297,6 +345,7 @@ int fvm_destroy(fvm_ctx* c) {
   }
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->hmail) (void)hipHostFree(c->hmail);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return FVM_OK;
@@ -304,8 +353,11 @@ int fvm_destroy(fvm_ctx* c) {
 
 int fvm_set_field(fvm_ctx* c, int field, const double* host) {
   if (!c || !host) return FVM_ERR_ARG;
+  c->u_is_un = false;
   double* dptr = field_ptr(c, field);
   if (!dptr) return FVM_ERR_ARG;
+  int rs = fvm_k_sgs_out(c);  // form SGS_x,y,z before any of their inputs can change
+  if (rs) return rs;
   std::vector<double> slab;
   if (c->two_d) {  // one (NI+1)(NJ+1) plane, repeated on the slab's three K planes
     slab.resize(c->d.n);
@@ -328,6 +380,10 @@ int fvm_get_field(fvm_ctx* c, int field, double* host) {
   if (!c || !host) return FVM_ERR_ARG;
   double* dptr = field_ptr(c, field);
   if (!dptr) return FVM_ERR_ARG;
+  if (field >= FVM_SGSX && field <= FVM_SGSZ) {
+    const int rs = fvm_k_sgs_out(c);
+    if (rs) return rs;
+  }
   if (c->two_d) dptr += c->d.sk;  // the slab's interior plane K = 1
   const long cnt = c->two_d ? c->d.sk : c->d.n;
   HIP_CHECK(hipMemcpyAsync(host, dptr, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -381,6 +437,7 @@ int fvm_stress_tensor_dy(fvm_ctx* c, double* c_avg) {
 
 int fvm_correct(fvm_ctx* c, double sign) {
   if (!c) return FVM_ERR_ARG;
+  c->u_is_un = false;
   int r = fvm_k_correct(c, sign);
   if (r) return r;
   double* f[3] = {c->u, c->v, c->w};
@@ -436,7 +493,7 @@ int fvm_step(fvm_ctx* c, fvm_step_info* info) {
   CK(fvm_k_contra_vel(c));                        // :48
   CK(fvm_k_nl(c));                                // :51-57
   if (c->prm.use_les) {                           // :59-69
-    CK(fvm_k_sgs(c));
+    CK(fvm_k_sgs(c, false));  // SGS_x,y,z formed on first read (fvm_get_field)
   }
   CK(fvm_k_dif(c));                               // :72-75
   CK(fvm_k_p_source(c));                          // :78
@@ -456,8 +513,8 @@ int fvm_step(fvm_ctx* c, fvm_step_info* info) {
   else
     CK(fvm_k_calccoef(c, nullptr, true));         // :150
 #undef CK
-  HIP_CHECK(hipMemcpyAsync(c->hscal, c->scal, 17 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(hipStreamSynchronize(c->stream));
+  r = fvm_fetch(c, c->scal, 17, c->hscal);
+  if (r) return r;
   st.c_avg = c->prm.use_les ? c->hscal[0] : 0.0;
   st.c_t = (c->thermal && c->prm.use_les) ? c->hscal[1] : 0.0;
   st.nu = c->thermal ? c->hscal[16] : 0.0;

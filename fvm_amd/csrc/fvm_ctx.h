@@ -39,6 +39,9 @@ struct FvmCtx {
   // fields
   double *u = nullptr, *v = nullptr, *w = nullptr, *p = nullptr;
   double *un = nullptr, *vn = nullptr, *wn = nullptr;
+  // u, v, w still equal U_N, V_N, W_N bit for bit (set by the step-start copy, cleared by every
+  // entry point that may write them): the first AF stage then takes U_N from its staged input
+  bool u_is_un = false;
   double* nl[3] = {};
   double* nl1[3] = {};
   double* dif[3] = {};
@@ -57,6 +60,7 @@ struct FvmCtx {
   double* uh[3] = {};
   double* nut = nullptr;
   double* sgs[3] = {};
+  bool sgs_stale = false;   // sgs[] not yet formed from the last SGS pass's nu_t and S_ij
   // AF chunked iteration: two iterate buffers per component, device table of prev/next
   double* afb[3][2] = {};
   double** afptr = nullptr;
@@ -83,6 +87,12 @@ struct FvmCtx {
   int ncflpart = 0;
   double* scal = nullptr;    // device scalars (c_avg, residual sums, pref, ...)
   double* hscal = nullptr;   // pinned host mirror of scal
+  // host mailbox: fine-grained pinned memory that k_publish writes scalars and then a sequence
+  // number into, so the host reads a convergence test's inputs without a copy kernel or a
+  // stream synchronisation (fvm_fetch)
+  double* hmail = nullptr;
+  unsigned long long* hseq = nullptr;
+  unsigned long long seq = 0;
 
   int nred = 0;
   // separable approximate inverse of the pressure operator (fvm_poisson_spec.hip)
@@ -144,6 +154,7 @@ struct ProfScope {
 };
 
 // stage launchers (return FVM_OK or an error code)
+int fvm_fetch(FvmCtx* c, const double* src, int n, double* dst);
 int fvm_geom_setup(FvmCtx* c);
 int fvm_k_set_dtime(FvmCtx* c, double dtime);
 int fvm_af_setup(FvmCtx* c);
@@ -170,7 +181,8 @@ int fvm_k_pref(FvmCtx* c, double* pref_host);
 int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready = false);
 int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready = false);
 int fvm_wall2d_setup(FvmCtx* c, const double* xg, const double* yg);
-int fvm_k_sgs(FvmCtx* c);
+int fvm_k_sgs(FvmCtx* c, bool sgs_out = true);
+int fvm_k_sgs_out(FvmCtx* c);
 int fvm_k_iter_u(FvmCtx* c);
 int fvm_k_iter_p(FvmCtx* c, double* resip, int* niter);
 int fvm_k_bcp(FvmCtx* c);

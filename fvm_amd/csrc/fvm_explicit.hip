@@ -696,6 +696,7 @@ int fvm_k_step_copy(FvmCtx* c) {
   const int cnt = c->thermal ? 4 : 3;
   hipLaunchKernelGGL(k_copy_multi, dim3(2048, cnt), dim3(256), 0, c->stream, c->d.n, dst, src);
   HIP_CHECK(hipGetLastError());
+  c->u_is_un = true;
   return FVM_OK;
 }
 

@@ -250,9 +250,8 @@ static int su_and_residual(FvmCtx* c, bool want, double* res) {
   if (want) {
     hipLaunchKernelGGL(k_p_res_final, dim3(1), dim3(256), 0, c->stream, c->red, nb, c->scal + 4);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(c->hscal + 4, c->scal + 4, 3 * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(hipStreamSynchronize(c->stream));
+    const int rf = fvm_fetch(c, c->scal + 4, 3, c->hscal + 4);
+    if (rf) return rf;
     *res = c->hscal[6];
   }
   return FVM_OK;

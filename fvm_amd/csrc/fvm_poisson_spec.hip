@@ -519,9 +519,8 @@ int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
     // Like ITER_P's sweep loop (iter_p.f90:36-62), at least one correction pass is made before
     // the residual is tested, so the first residual needs no round trip to the host.
     if (outer > 0 || cap < 1) {
-      HIP_CHECK(hipMemcpyAsync(c->hscal + 4, c->scal + 4, 3 * sizeof(double),
-                               hipMemcpyDeviceToHost, c->stream));
-      HIP_CHECK(hipStreamSynchronize(c->stream));
+      const int rf = fvm_fetch(c, c->scal + 4, 3, c->hscal + 4);
+      if (rf) return rf;
       res = c->hscal[6];
       if (!(res == res)) {
         fprintf(stderr, "fvm_amd: non-finite pressure residual (outer pass %d)\n", outer);

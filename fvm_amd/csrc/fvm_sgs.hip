@@ -4,9 +4,10 @@
 // Passes: (1) strain tensor S_ij and |S| on i = 0..NI (one-sided at the I boundaries), periodic
 // ghosts; (2) test-filtered velocity (1,2,1)^3/64 with the velocity boundary ghosts;
 // (3) per cell the Germano identity L_ij, the Lilly tensor M_ij and the contractions LM, MM,
-// with every filtered product recomputed from its 27 neighbours instead of stored;
-// (4) the reference's global conditional average C = <LM>/<2MM> over cells with LM/MM >= 0 as
-// a deterministic two-level FP64 reduction; (5) eddy viscosity and the divergence of the SGS
+// with every filtered product recomputed from its 27 neighbours instead of stored, summed per
+// workgroup over the cells with LM/MM >= 0; (4) the reference's global conditional average
+// C = <LM>/<2MM> from those partial sums (a deterministic two-level FP64 reduction, no per-cell
+// LM, MM arrays); (5) eddy viscosity and the divergence of the SGS
 // stress added to the convective term, NL += J^-1 SGS (RLES/main.f90:61-66).
 #include "fvm_ctx.h"
 
@@ -195,8 +196,8 @@ __global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restr
                                             const double* __restrict__ VH,
                                             const double* __restrict__ WH,
                                             const double* const* __restrict__ L,
-                                            const double* __restrict__ filt2, double* lm_out,
-                                            double* mm_out, double* magh_out) {
+                                            const double* __restrict__ filt2, double* part,
+                                            double* magh_out) {
   const uint3 bb = xcd_block();
   __shared__ double sh[10][LM_W * LM_H];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
@@ -212,6 +213,8 @@ __global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restr
   double Pa[12], Pb[12], Pc[12];
 #pragma unroll
   for (int q = 0; q < 12; ++q) Pa[q] = Pb[q] = Pc[q] = 0.0;
+  // this thread's sums of LM, MM and the cell count over the cells with 0.5 LM/MM >= 0
+  double sa = 0.0, sb = 0.0, sn = 0.0;
   for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
     __syncthreads();
     for (int q = tid; q < LM_W * LM_H; q += 256) {
@@ -309,8 +312,28 @@ __global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restr
         mm = mm + 2.0 * M * M;
       }
     }
-    lm_out[c] = lm;
-    mm_out[c] = mm;
+    if (0.5 * lm / mm >= 0.0) {  // stress_tensor_dy.f90:486-508
+      sa = sa + lm;
+      sb = sb + mm;
+      sn = sn + 1.0;
+    }
+  }
+  // workgroup partial sums of the conditional average, reduced by k_lm_final
+  __syncthreads();
+  double* red = &sh[0][0];
+  red[tid] = sa; red[256 + tid] = sb; red[512 + tid] = sn;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) {
+      red[tid] += red[tid + st];
+      red[256 + tid] += red[256 + tid + st];
+      red[512 + tid] += red[512 + tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const long bid = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    part[3 * bid] = red[0]; part[3 * bid + 1] = red[256]; part[3 * bid + 2] = red[512];
   }
 }
 
@@ -542,7 +565,7 @@ __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __
                                                  const double* const* __restrict__ L,
                                                  const double* __restrict__ djac, double* nlx,
                                                  double* nly, double* nlz, double* sx, double* sy,
-                                                 double* sz) {
+                                                 double* sz, int mode) {
   const uint3 bb = xcd_block();
   __shared__ double sh[2][6][SF_P];  // [plane slot][product][tile cell]
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
@@ -587,7 +610,7 @@ __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __
     const int sc = slot ^ 1;
     const int k = kp - 1;
     const long c = cidx(d, i, j, k);
-    const double J = djac[c];
+    const double J = (mode & 1) ? djac[c] : 0.0;
 #pragma unroll
     for (int row = 0; row < 3; ++row) {
       double t = 0.0;
@@ -606,8 +629,8 @@ __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __
         }
       }
       const double sg = 0.5 * t;
-      so[row][c] = sg;
-      nl[row][c] = nl[row][c] + sg * J;
+      if (mode & 2) so[row][c] = sg;
+      if (mode & 1) nl[row][c] = nl[row][c] + sg * J;
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) Pm[q] = sh[sc][q][e0];
@@ -616,7 +639,21 @@ __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __
 
 }  // namespace
 
-int fvm_k_sgs(FvmCtx* c) {
+// SGS_x, SGS_y, SGS_z from the step's nu_t and S_ij, which stay in place until the next SGS
+// pass: a step's NL update does not write them, the first read of the fields does
+int fvm_k_sgs_out(FvmCtx* c) {
+  if (!c->sgs_stale) return FVM_OK;
+  const Dims d = c->d;
+  const dim3 gs((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + SD_KC - 1) / SD_KC);
+  hipLaunchKernelGGL(k_sgs_div, gs, kBlk, 0, c->stream, d, (const double* const*)c->dm, c->nut,
+                     (const double* const*)c->dlss, c->djac, c->nl[0], c->nl[1], c->nl[2],
+                     c->sgs[0], c->sgs[1], c->sgs[2], 2);
+  HIP_CHECK(hipGetLastError());
+  c->sgs_stale = false;
+  return FVM_OK;
+}
+
+int fvm_k_sgs(FvmCtx* c, bool sgs_out) {
   const Dims d = c->d;
   double** m = c->dm;
   ProfScope ps(c, "sgs");
@@ -654,12 +691,12 @@ int fvm_k_sgs(FvmCtx* c) {
     const dim3 gl((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + LM_KC - 1) / LM_KC);
     hipLaunchKernelGGL(k_lm, gl, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
                        c->uh[0], c->uh[1], c->uh[2], (const double* const*)c->dlss, c->filt2,
-                       c->su[0], c->su[1], th ? c->magh : nullptr);
+                       c->red, th ? c->magh : nullptr);
     HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_lm_final, dim3(1), dim3(256), 0, st, c->red, (int)(gl.x * gl.y * gl.z),
+                       c->scal + 0);
   }
   const int nb = 512;
-  hipLaunchKernelGGL(k_lm_reduce, dim3(nb), dim3(256), 0, st, d, c->su[0], c->su[1], c->red);
-  hipLaunchKernelGGL(k_lm_final, dim3(1), dim3(256), 0, st, c->red, nb, c->scal + 0);
   hipLaunchKernelGGL(k_nut, grid1(d.n, 256), dim3(256), 0, st, d, c->scal + 0, c->filt2, c->mag,
                      c->nut);
   {
@@ -667,8 +704,9 @@ int fvm_k_sgs(FvmCtx* c) {
     const dim3 gs((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + SD_KC - 1) / SD_KC);
     hipLaunchKernelGGL(k_sgs_div, gs, kBlk, 0, st, d, (const double* const*)m, c->nut,
                        (const double* const*)c->dlss, c->djac, c->nl[0], c->nl[1], c->nl[2],
-                       c->sgs[0], c->sgs[1], c->sgs[2]);
+                       c->sgs[0], c->sgs[1], c->sgs[2], sgs_out ? 3 : 1);
     HIP_CHECK(hipGetLastError());
+    c->sgs_stale = !sgs_out;
   }
   if (th) {  // the temperature's dynamic model (stress_tensor_dy.f90:589-738), main.f90:67-68
     ProfScope pk(c, "sgs_t");
