@@ -43,8 +43,10 @@ KERNEL_BYTES = {
     # chunked fallback pass over the 3 components: {phi_in, SU, J, 3 stage metric products}
     # -> phi_out = 7 doubles per cell and component
     "k_af_chunk_i": (56, 3), "k_af_chunk_j": (56, 3), "k_af_chunk_k": (56, 3),
-    # strain rates fused with the 27-point test filter: {u,v,w, xi 9} -> {S 6, |S|, u^ 3}
-    "k_strain_filter": (8 * 22, 1),
+    # The metric sets below count 7 of the 9 metrics: xi_z and eta_z vanish identically on the
+    # extruded grid and are not read.
+    # strain rates fused with the 27-point test filter: {u,v,w, xi 7} -> {S 6, |S|, u^ 3}
+    "k_strain_filter": (8 * 20, 1),
     # AF right-hand side, stage 1: {phi 3 (= U_N at the first stage of a step), 6 metric
     # products, 2 Re J/dt, NL 3, NL_NM1 3, DIF 3, S 3, 3 face coefficients} -> SU 3; stage 2:
     # {phi 3, products 6, 2 Re J/dt, faces 3} -> SU 3
@@ -53,14 +55,14 @@ KERNEL_BYTES = {
     "k_residual": (8 * 8 + 4, 1),
     # DIF: {u_n, v_n, w_n, D_E, D_N, D_T, J^-1 Q12, Q13, Q23} -> DIF 3
     "dif": (8 * 12, 1),
-    # P_SOURCE + projection: {p, J, xi 9, u, v, w} -> {S 3, u, v, w}
-    "psource_correct": (8 * 20, 1),
-    # Germano identity terms: {u,v,w, u^ 3, xi 9, S 6, Delta^2} -> workgroup sums of LM, MM
+    # P_SOURCE + projection: {p, J, xi 7, u, v, w} -> {S 3, u, v, w}
+    "psource_correct": (8 * 18, 1),
+    # Germano identity terms: {u,v,w, u^ 3, xi 7, S 6, Delta^2} -> workgroup sums of LM, MM
     # (|S| is formed from S as it is staged)
-    "k_lm": (8 * 22, 1),
-    # SGS divergence in the step: {xi 9, nu_t, S 6, J, NL 3} -> {NL 3} (SGS_x,y,z are formed
+    "k_lm": (8 * 20, 1),
+    # SGS divergence in the step: {xi 7, nu_t, S 6, J, NL 3} -> {NL 3} (SGS_x,y,z are formed
     # when they are read)
-    "k_sgs_div": (8 * 23, 1),
+    "k_sgs_div": (8 * 21, 1),
 }
 
 

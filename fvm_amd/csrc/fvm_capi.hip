@@ -358,6 +358,7 @@ int fvm_set_field(fvm_ctx* c, int field, const double* host) {
   if (!dptr) return FVM_ERR_ARG;
   int rs = fvm_k_sgs_out(c);  // form SGS_x,y,z before any of their inputs can change
   if (rs) return rs;
+  if (field >= FVM_XIX && field <= FVM_ZTZ) c->xz_zero = false;  // re-derived only at set-up
   std::vector<double> slab;
   if (c->two_d) {  // one (NI+1)(NJ+1) plane, repeated on the slab's three K planes
     slab.resize(c->d.n);

@@ -39,6 +39,16 @@ __device__ inline uint3 xcd_block() {
   return make_uint3(w % gx, (w / gx) % gy, w / (gx * gy));
 }
 
+// Metric a of the table (grouped per Cartesian component: xi_s, eta_s, zeta_s for s = x, y, z)
+// at cell c.  With XZ0 the entries xi_z (a = 6) and eta_z (a = 7), which are exactly zero on
+// every cell of a grid extruded along z (FvmCtx::xz_zero), are the constant 0.0 and are not
+// loaded; a is a compile-time constant at every call, so the choice folds away.  The products
+// and sums keep the reference's order, so results are unchanged.
+template <bool XZ0>
+__device__ inline double mget(const double* const* __restrict__ m, int a, long c) {
+  return (XZ0 && (a == 6 || a == 7)) ? 0.0 : m[a][c];
+}
+
 // A 64 x 4 (I x J) tile of the K-marching stencil kernels with its one-cell halo, as staged in
 // LDS: SF_W x SF_H cells, row-major.
 constexpr int SF_W = 66, SF_H = 6, SF_P = SF_W * SF_H;

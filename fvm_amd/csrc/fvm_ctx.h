@@ -68,6 +68,7 @@ struct FvmCtx {
   // stage couplings that vanish identically on this grid (a: along the sweep, b: across it)
   bool af_zero_a[3] = {}, af_zero_b[3] = {};
   double af_b_ratio[3] = {};
+  bool xz_zero = false;       // xi_z = eta_z = 0 on every cell (metric kernels skip them)
   bool dd2_zero = false;      // xi_z = eta_z = 0 around every cell: stage-3 cross terms vanish  // max over cells of (|b couplings|) / (2 Re J^-1/dt)
   // Thomas factors A, D, B of the K-line stage when its lines are independent
   double *tk_a = nullptr, *tk_d = nullptr, *tk_b = nullptr;
@@ -83,8 +84,9 @@ struct FvmCtx {
   double* scantab_h[2][9] = {};               // their last uploaded contents
   double* afp[2][6] = {};                     // metric products of the stage 0/1 cross terms
   double** dafp[2] = {nullptr, nullptr};      // their device pointer tables
-  double* cflpart = nullptr; // per-tile CFL maxima of the projection kernel (one per 64x4 tile)
-  int ncflpart = 0;
+  double* cflpart = nullptr; // CFL maxima of the projection kernel (one per workgroup)
+  int ncflpart = 0;          // capacity of cflpart
+  int ncflused = 0;          // entries the last projection wrote
   double* scal = nullptr;    // device scalars (c_avg, residual sums, pref, ...)
   double* hscal = nullptr;   // pinned host mirror of scal
   // host mailbox: fine-grained pinned memory that k_publish writes scalars and then a sequence

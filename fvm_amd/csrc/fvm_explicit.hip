@@ -27,6 +27,7 @@ const dim3 kBlk(64, 4, 1);
 
 // Contravariant velocity times J^-1 on i = 0..NI, interior j,k (RLES/contra_vel.f90:13-23);
 // the face volume fluxes are 0.5 (JU(c) + JU(c+1)) (contra_vel.f90:37-53).
+template <bool XZ0>
 __global__ void k_contra(Dims d, const double* const* __restrict__ m, const double* __restrict__ djac,
                          const double* __restrict__ u, const double* __restrict__ v,
                          const double* __restrict__ w, double* ju0, double* ju1, double* ju2) {
@@ -37,8 +38,8 @@ __global__ void k_contra(Dims d, const double* const* __restrict__ m, const doub
   if (i > d.ni || j > d.nj - 1) return;
   const long c = cidx(d, i, j, k);
   double U = u[c], V = v[c], W = w[c], J = djac[c];
-  double u1 = m[0][c] * U + m[3][c] * V + m[6][c] * W;
-  double u2 = m[1][c] * U + m[4][c] * V + m[7][c] * W;
+  double u1 = m[0][c] * U + m[3][c] * V + mget<XZ0>(m, 6, c) * W;
+  double u2 = m[1][c] * U + m[4][c] * V + mget<XZ0>(m, 7, c) * W;
   double u3 = m[2][c] * U + m[5][c] * V + m[8][c] * W;
   ju0[c] = J * u1;
   ju1[c] = J * u2;
@@ -175,12 +176,12 @@ __global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict
 // With CORR set, followed on the same cell by U = U + sign DTIME S / J^-1 (RLES/main.f90:108-111;
 // 90-92 with sign -1), S kept for the next step's predictor, and, when cflpart is given, the
 // local CFL number of the corrected velocity (CALCCOEF, k_cfl's formula) reduced to one maximum
-// per 64 x 4 x 1 tile.  A workgroup owns a 64 x 4 (I x J) column of cells over PS_KC
+// per workgroup.  A workgroup owns a 64 x 4 (I x J) column of cells over PS_KC
 // consecutive planes and marches up K; the products (J zeta_s p) of the planes K+-1 ride along
 // in registers, so only the I and J neighbours are fetched per cell.
 constexpr int PS_KC = 8;
 
-template <int CORR>
+template <int CORR, bool XZ0>
 __global__ void __launch_bounds__(256) k_psource_km(Dims d, const double* const* __restrict__ m,
                                                     const double* __restrict__ djac,
                                                     const double* __restrict__ p, double sign,
@@ -200,6 +201,7 @@ __global__ void __launch_bounds__(256) k_psource_km(Dims d, const double* const*
   double* f[3] = {u, v, w};
   // (J zeta_s p) of the own column, planes kp-2, kp-1, kp, per component s
   double zp[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  double cmax = 0.0;  // this thread's largest local CFL number
   for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) { zp[q][0] = zp[q][1]; zp[q][1] = zp[q][2]; }
@@ -211,18 +213,16 @@ __global__ void __launch_bounds__(256) k_psource_km(Dims d, const double* const*
     }
     if (kp < k0 + 1) continue;  // uniform over the workgroup
     const int k = kp - 1;
-    double lcfl = 0.0;
     if (act) {
       const long c = cidx(d, i, j, k);
       double nv[3];
       const double J = djac[c];
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const double *a = m[3 * q], *b = m[3 * q + 1];
-        double t = djac[c + 1] * a[c + 1] * p[c + 1];
-        t = t - djac[c - 1] * a[c - 1] * p[c - 1];
-        t = t + djac[c + sj] * b[c + sj] * p[c + sj];
-        t = t - djac[c - sj] * b[c - sj] * p[c - sj];
+        double t = djac[c + 1] * mget<XZ0>(m, 3 * q, c + 1) * p[c + 1];
+        t = t - djac[c - 1] * mget<XZ0>(m, 3 * q, c - 1) * p[c - 1];
+        t = t + djac[c + sj] * mget<XZ0>(m, 3 * q + 1, c + sj) * p[c + sj];
+        t = t - djac[c - sj] * mget<XZ0>(m, 3 * q + 1, c - sj) * p[c - sj];
         t = t + zp[q][2];
         t = t - zp[q][0];
         const double sv = -(0.5 * t);
@@ -234,26 +234,26 @@ __global__ void __launch_bounds__(256) k_psource_km(Dims d, const double* const*
       }
       if (CORR && cflpart) {
         const double U = nv[0], V = nv[1], W = nv[2];
-        const double ca = fabs(m[0][c] * U + m[3][c] * V + m[6][c] * W);
-        const double cb = fabs(m[1][c] * U + m[4][c] * V + m[7][c] * W);
+        const double ca = fabs(m[0][c] * U + m[3][c] * V + mget<XZ0>(m, 6, c) * W);
+        const double cb = fabs(m[1][c] * U + m[4][c] * V + mget<XZ0>(m, 7, c) * W);
         const double ce = fabs(m[2][c] * U + m[5][c] * V + m[8][c] * W);
-        lcfl = dtime * (ca + cb + ce);
+        cmax = fmax(cmax, dtime * (ca + cb + ce));
       }
     }
-    if (!(CORR && cflpart)) continue;  // uniform over the workgroup
-    sh[tid] = lcfl;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) sh[tid] = fmax(sh[tid], sh[tid + s]);
-      __syncthreads();
-    }
-    if (tid == 0) cflpart[bb.x + gridDim.x * (bb.y + (long)gridDim.y * (k - 1))] = sh[0];
+  }
+  if (!(CORR && cflpart)) return;  // uniform over the workgroup
+  sh[tid] = cmax;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] = fmax(sh[tid], sh[tid + s]);
     __syncthreads();
   }
+  if (tid == 0) cflpart[blockIdx.x + gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = sh[0];
 }
 
 // predictor correction U = U - DTIME S / J^-1 fused with CONTRA_VEL on the interior cells
 // (RLES/main.f90:90-92, contra_vel.f90:13-23); the I ghost planes follow after BCU
+template <bool XZ0>
 __global__ void k_correct_contra(Dims d, const double* const* __restrict__ m,
                                  const double* __restrict__ djac, double dtime,
                                  const double* __restrict__ s0, const double* __restrict__ s1,
@@ -265,8 +265,8 @@ __global__ void k_correct_contra(Dims d, const double* const* __restrict__ m,
   const double V = v[c] - dtime * s1[c] / J;
   const double W = w[c] - dtime * s2[c] / J;
   u[c] = U; v[c] = V; w[c] = W;
-  const double u1 = m[0][c] * U + m[3][c] * V + m[6][c] * W;
-  const double u2 = m[1][c] * U + m[4][c] * V + m[7][c] * W;
+  const double u1 = m[0][c] * U + m[3][c] * V + mget<XZ0>(m, 6, c) * W;
+  const double u2 = m[1][c] * U + m[4][c] * V + mget<XZ0>(m, 7, c) * W;
   const double u3 = m[2][c] * U + m[5][c] * V + m[8][c] * W;
   ju0[c] = J * u1;
   ju1[c] = J * u2;
@@ -633,8 +633,12 @@ int fvm_k_contra_vel(FvmCtx* c) {
   {
     ProfScope ps(c, "contra_vel");
     dim3 g((d.ni + 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
-    hipLaunchKernelGGL(k_contra, g, kBlk, 0, c->stream, d, (const double* const*)m, c->djac,
-                       c->u, c->v, c->w, c->ju[0], c->ju[1], c->ju[2]);
+    if (c->xz_zero)
+      hipLaunchKernelGGL(k_contra<true>, g, kBlk, 0, c->stream, d, (const double* const*)m,
+                         c->djac, c->u, c->v, c->w, c->ju[0], c->ju[1], c->ju[2]);
+    else
+      hipLaunchKernelGGL(k_contra<false>, g, kBlk, 0, c->stream, d, (const double* const*)m,
+                         c->djac, c->u, c->v, c->w, c->ju[0], c->ju[1], c->ju[2]);
     HIP_CHECK(hipGetLastError());
   }
   return fvm_ghost_fill_n(c, c->ju, 3);
@@ -671,9 +675,14 @@ int fvm_k_p_source(FvmCtx* c) {
   ProfScope ps(c, "p_source");
   const Dims d = c->d;
   const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + PS_KC - 1) / PS_KC);
-  hipLaunchKernelGGL(k_psource_km<0>, g, kBlk, 0, c->stream, c->d, (const double* const*)m,
-                     c->djac, c->p, 0.0, 0.0, c->s[0], c->s[1], c->s[2], nullptr, nullptr,
-                     nullptr, nullptr);
+  if (c->xz_zero)
+    hipLaunchKernelGGL((k_psource_km<0, true>), g, kBlk, 0, c->stream, c->d,
+                       (const double* const*)m, c->djac, c->p, 0.0, 0.0, c->s[0], c->s[1],
+                       c->s[2], nullptr, nullptr, nullptr, nullptr);
+  else
+    hipLaunchKernelGGL((k_psource_km<0, false>), g, kBlk, 0, c->stream, c->d,
+                       (const double* const*)m, c->djac, c->p, 0.0, 0.0, c->s[0], c->s[1],
+                       c->s[2], nullptr, nullptr, nullptr, nullptr);
   HIP_CHECK(hipGetLastError());
   return FVM_OK;
 }
@@ -765,7 +774,7 @@ int fvm_k_pref(FvmCtx* c, double* pref_host) {
 int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready) {
   double** m = c->dm;
   ProfScope ps(c, "calccoef");
-  const int ncfl = cfl_ready ? c->ncflpart : 512;
+  const int ncfl = cfl_ready ? c->ncflused : 512;
   const double* cfl = cfl_ready ? c->cflpart : c->red;
   const int off = cfl_ready ? 0 : ncfl;
   if (!cfl_ready)
@@ -790,7 +799,7 @@ int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready) {
 int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready) {
   double** m = c->dm;
   ProfScope ps(c, "calccoef");
-  const int ncfl = cfl_ready ? c->ncflpart : 512;
+  const int ncfl = cfl_ready ? c->ncflused : 512;
   const double* cfl = cfl_ready ? c->cflpart : c->red;
   if (!cfl_ready)
     hipLaunchKernelGGL(k_cfl, dim3(ncfl), dim3(256), 0, c->stream, c->d, (const double* const*)m,
@@ -856,18 +865,24 @@ int fvm_wall2d_setup(FvmCtx* c, const double* xg, const double* yg) {
   return FVM_OK;
 }
 
-// with_cfl: also leave the per-tile CFL maxima of the projected velocity in c->cflpart for the
+// with_cfl: also leave the per-workgroup CFL maxima of the projected velocity in c->cflpart for the
 // CALCCOEF that ends the step (the pressure reference and BCU that come between change neither
 // the interior velocity nor the metrics)
 int fvm_k_psource_correct(FvmCtx* c, double sign, bool with_cfl) {
   ProfScope ps(c, "psource_correct");
   const Dims d = c->d;
-  const dim3 gt = interior_grid(d);  // CFL maxima: one per 64 x 4 x 1 tile
-  if (with_cfl && (long)gt.x * gt.y * gt.z > c->ncflpart) return FVM_ERR_SIZE;
-  const dim3 g(gt.x, gt.y, (d.nk - 1 + PS_KC - 1) / PS_KC);
-  hipLaunchKernelGGL(k_psource_km<1>, g, kBlk, 0, c->stream, c->d, (const double* const*)c->dm,
-                     c->djac, c->p, sign, c->prm.dtime, c->s[0], c->s[1], c->s[2], c->u, c->v,
-                     c->w, with_cfl ? c->cflpart : nullptr);
+  const dim3 gt = interior_grid(d);
+  const dim3 g(gt.x, gt.y, (d.nk - 1 + PS_KC - 1) / PS_KC);  // CFL maxima: one per workgroup
+  if (with_cfl && (long)g.x * g.y * g.z > c->ncflpart) return FVM_ERR_SIZE;
+  if (with_cfl) c->ncflused = (int)(g.x * g.y * g.z);
+  if (c->xz_zero)
+    hipLaunchKernelGGL((k_psource_km<1, true>), g, kBlk, 0, c->stream, c->d,
+                       (const double* const*)c->dm, c->djac, c->p, sign, c->prm.dtime, c->s[0],
+                       c->s[1], c->s[2], c->u, c->v, c->w, with_cfl ? c->cflpart : nullptr);
+  else
+    hipLaunchKernelGGL((k_psource_km<1, false>), g, kBlk, 0, c->stream, c->d,
+                       (const double* const*)c->dm, c->djac, c->p, sign, c->prm.dtime, c->s[0],
+                       c->s[1], c->s[2], c->u, c->v, c->w, with_cfl ? c->cflpart : nullptr);
   HIP_CHECK(hipGetLastError());
   return FVM_OK;
 }
@@ -877,9 +892,14 @@ int fvm_k_correct_bcu_contra(FvmCtx* c) {
   const Dims d = c->d;
   {
     ProfScope ps(c, "correct_contra");
-    hipLaunchKernelGGL(k_correct_contra, interior_grid(d), kBlk, 0, c->stream, d,
-                       (const double* const*)c->dm, c->djac, c->prm.dtime, c->s[0], c->s[1],
-                       c->s[2], c->u, c->v, c->w, c->ju[0], c->ju[1], c->ju[2]);
+    if (c->xz_zero)
+      hipLaunchKernelGGL(k_correct_contra<true>, interior_grid(d), kBlk, 0, c->stream, d,
+                         (const double* const*)c->dm, c->djac, c->prm.dtime, c->s[0], c->s[1],
+                         c->s[2], c->u, c->v, c->w, c->ju[0], c->ju[1], c->ju[2]);
+    else
+      hipLaunchKernelGGL(k_correct_contra<false>, interior_grid(d), kBlk, 0, c->stream, d,
+                         (const double* const*)c->dm, c->djac, c->prm.dtime, c->s[0], c->s[1],
+                         c->s[2], c->u, c->v, c->w, c->ju[0], c->ju[1], c->ju[2]);
     HIP_CHECK(hipGetLastError());
   }
   double* f[3] = {c->u, c->v, c->w};

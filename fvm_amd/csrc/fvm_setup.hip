@@ -422,6 +422,15 @@ int fvm_geom_setup(FvmCtx* c) {
   for (int a = 0; a < 9; ++a) HIP_CHECK(hipFree(D[a]));
   HIP_CHECK(hipFree(dD));
   HIP_CHECK(hipFree(dM));
+  {  // xi_z and eta_z identically zero (z independent of xi and eta: an extruded grid)
+    std::vector<double> h(d.n);
+    bool z = true;
+    for (int a = 6; a < 8 && z; ++a) {
+      HIP_CHECK(hipMemcpy(h.data(), c->m[a], d.n * sizeof(double), hipMemcpyDeviceToHost));
+      for (long q = 0; q < d.n && z; ++q) z = (h[q] == 0.0);
+    }
+    c->xz_zero = z;
+  }
   return FVM_OK;
 }
 

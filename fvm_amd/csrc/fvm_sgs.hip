@@ -22,12 +22,13 @@ struct Diff3 {
   double i, j, k;
 };
 
+template <bool XZ0>
 __device__ inline void strain_store(const double* const* __restrict__ m, long c, bool onesided,
                                     Diff3 u, Diff3 v, Diff3 w, double* const* __restrict__ L,
                                     double* mag) {
   const double xix = m[0][c], etx = m[1][c], ztx = m[2][c];
   const double xiy = m[3][c], ety = m[4][c], zty = m[5][c];
-  const double xiz = m[6][c], etz = m[7][c], ztz = m[8][c];
+  const double xiz = mget<XZ0>(m, 6, c), etz = mget<XZ0>(m, 7, c), ztz = m[8][c];
   double l11, l12, l13, l22, l23, l33;
   if (!onesided) {
     double t;
@@ -81,7 +82,7 @@ __global__ void k_strain_iplanes(Dims d, const double* const* __restrict__ m,
     q.k = F[c + sk] - F[c - sk];
     return q;
   };
-  strain_store(m, c, true, diff(U), diff(V), diff(W), L, mag);
+  strain_store<false>(m, c, true, diff(U), diff(V), diff(W), L, mag);
   if (T) {  // one-sided temperature gradient (stress_tensor_dy.f90:138-146, 188-196)
     const Diff3 g = diff(T);
     dtx[c] = 0.5 * (2.0 * m[0][c] * g.i + m[1][c] * g.j + m[2][c] * g.k);
@@ -106,7 +107,7 @@ struct SfOut {
   double* dt[3];        // dT/dx, dT/dy, dT/dz
 };
 
-template <int NF>
+template <int NF, bool XZ0>
 __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* const* __restrict__ m,
                                                        SfOut io, double* const* __restrict__ L,
                                                        double* mag) {
@@ -142,7 +143,7 @@ __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* con
       g[f].j = sh[f][s0][q0 + SF_W] - sh[f][s0][q0 - SF_W];
       g[f].k = sh[f][sp][q0] - sh[f][sm][q0];
     }
-    strain_store(m, c, false, g[0], g[1], g[2], L, mag);
+    strain_store<XZ0>(m, c, false, g[0], g[1], g[2], L, mag);
     if (NF == 4) {
       const double ti = sh[3][s0][q0 + 1] - sh[3][s0][q0 - 1];
       const double tj = sh[3][s0][q0 + SF_W] - sh[3][s0][q0 - SF_W];
@@ -151,7 +152,7 @@ __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* con
       io.dt[0][c] = 0.5 * t;
       t = m[3][c] * ti + m[4][c] * tj + m[5][c] * tk;
       io.dt[1][c] = 0.5 * t;
-      t = m[6][c] * ti + m[7][c] * tj + m[8][c] * tk;
+      t = mget<XZ0>(m, 6, c) * ti + mget<XZ0>(m, 7, c) * tj + m[8][c] * tk;
       io.dt[2][c] = 0.5 * t;
     }
     const int pl[3] = {sm, s0, sp};
@@ -188,7 +189,8 @@ constexpr int LM_KC = 8;
 constexpr int SD_KC = 8;  // planes per workgroup of the K-marching SGS divergences
 constexpr int LM_W = 66, LM_H = 6;  // tile with halo
 
-__global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restrict__ m,
+template <bool XZ0>
+__global__ void __launch_bounds__(256, 3) k_lm(Dims d, const double* const* __restrict__ m,
                                             const double* __restrict__ U,
                                             const double* __restrict__ V,
                                             const double* __restrict__ W,
@@ -263,7 +265,7 @@ __global__ void __launch_bounds__(256) k_lm(Dims d, const double* const* __restr
     const long sj = d.sj, sk = d.sk;
     const double xix = m[0][c], etx = m[1][c], ztx = m[2][c];
     const double xiy = m[3][c], ety = m[4][c], zty = m[5][c];
-    const double xiz = m[6][c], etz = m[7][c], ztz = m[8][c];
+    const double xiz = mget<XZ0>(m, 6, c), etz = mget<XZ0>(m, 7, c), ztz = m[8][c];
 #define DI(F) (F[c + 1] - F[c - 1])
 #define DJ(F) (F[c + sj] - F[c - sj])
 #define DK(F) (F[c + sk] - F[c - sk])
@@ -560,6 +562,7 @@ __global__ void k_nut(Dims d, const double* __restrict__ cavg, const double* __r
 // slots (the I and J differences of plane k and the own-column value of plane k+1), and the
 // own-column products of plane k-1 are kept in registers.  Differences, products and sums are
 // the reference's, in its order.
+template <bool XZ0>
 __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __restrict__ m,
                                                  const double* __restrict__ nut,
                                                  const double* const* __restrict__ L,
@@ -624,7 +627,7 @@ __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __
           else if (dir == 1) { pp = sh[sc][q][e0 + SF_W]; pn = sh[sc][q][e0 - SF_W]; }
           else { pp = sh[slot][q][e0]; pn = Pm[q]; }
           const double inner = -pp + pn;
-          const double term = m[3 * col + dir][c] * inner;
+          const double term = mget<XZ0>(m, 3 * col + dir, c) * inner;
           t = (col == 0 && dir == 0) ? term : t + term;
         }
       }
@@ -645,7 +648,8 @@ int fvm_k_sgs_out(FvmCtx* c) {
   if (!c->sgs_stale) return FVM_OK;
   const Dims d = c->d;
   const dim3 gs((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + SD_KC - 1) / SD_KC);
-  hipLaunchKernelGGL(k_sgs_div, gs, kBlk, 0, c->stream, d, (const double* const*)c->dm, c->nut,
+  auto kern = c->xz_zero ? k_sgs_div<true> : k_sgs_div<false>;
+  hipLaunchKernelGGL(kern, gs, kBlk, 0, c->stream, d, (const double* const*)c->dm, c->nut,
                      (const double* const*)c->dlss, c->djac, c->nl[0], c->nl[1], c->nl[2],
                      c->sgs[0], c->sgs[1], c->sgs[2], 2);
   HIP_CHECK(hipGetLastError());
@@ -664,12 +668,10 @@ int fvm_k_sgs(FvmCtx* c, bool sgs_out) {
     const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + SF_KC - 1) / SF_KC);
     const SfOut io = {{c->u, c->v, c->w, c->t}, {c->uh[0], c->uh[1], c->uh[2], c->that},
                       {c->dtd[0], c->dtd[1], c->dtd[2]}};
-    if (th)
-      hipLaunchKernelGGL(k_strain_filter<4>, g, kBlk, 0, st, d, (const double* const*)m, io,
-                         (double* const*)c->dlss, c->mag);
-    else
-      hipLaunchKernelGGL(k_strain_filter<3>, g, kBlk, 0, st, d, (const double* const*)m, io,
-                         (double* const*)c->dlss, c->mag);
+    auto kern = th ? k_strain_filter<4, false>
+                   : (c->xz_zero ? k_strain_filter<3, true> : k_strain_filter<3, false>);
+    hipLaunchKernelGGL(kern, g, kBlk, 0, st, d, (const double* const*)m, io,
+                       (double* const*)c->dlss, c->mag);
     const long nb = 2L * (d.nj - 1) * (d.nk - 1);
     hipLaunchKernelGGL(k_strain_iplanes, grid1(nb, 256), dim3(256), 0, st, d,
                        (const double* const*)m, c->u, c->v, c->w, (double* const*)c->dlss,
@@ -689,7 +691,8 @@ int fvm_k_sgs(FvmCtx* c, bool sgs_out) {
   {
     ProfScope pk(c, "k_lm");
     const dim3 gl((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + LM_KC - 1) / LM_KC);
-    hipLaunchKernelGGL(k_lm, gl, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
+    auto kern = c->xz_zero ? k_lm<true> : k_lm<false>;
+    hipLaunchKernelGGL(kern, gl, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
                        c->uh[0], c->uh[1], c->uh[2], (const double* const*)c->dlss, c->filt2,
                        c->red, th ? c->magh : nullptr);
     HIP_CHECK(hipGetLastError());
@@ -702,7 +705,8 @@ int fvm_k_sgs(FvmCtx* c, bool sgs_out) {
   {
     ProfScope pk(c, "k_sgs_div");
     const dim3 gs((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + SD_KC - 1) / SD_KC);
-    hipLaunchKernelGGL(k_sgs_div, gs, kBlk, 0, st, d, (const double* const*)m, c->nut,
+    auto kern = c->xz_zero ? k_sgs_div<true> : k_sgs_div<false>;
+    hipLaunchKernelGGL(kern, gs, kBlk, 0, st, d, (const double* const*)m, c->nut,
                        (const double* const*)c->dlss, c->djac, c->nl[0], c->nl[1], c->nl[2],
                        c->sgs[0], c->sgs[1], c->sgs[2], sgs_out ? 3 : 1);
     HIP_CHECK(hipGetLastError());
