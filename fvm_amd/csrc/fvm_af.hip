@@ -45,13 +45,13 @@ __global__ void k_af_rhs(Dims d, AFArgs A, const double* const* __restrict__ m,
   if (S == 2 && ddzero) {
     const double a0 = ap0[c];
 #pragma unroll
-    for (int comp = 0; comp < A.nc; ++comp) A.su[comp][c] = a0 * A.f[comp][c];
+    for (int comp = 0; comp < A.nc; ++comp) gpw(A.su[comp])[c] = a0 * gp(A.f[comp])[c];
     return;
   }
   const long sj = d.sj, sk = d.sk;
-  const double* X = m[3 * S];
-  const double* E = m[3 * S + 1];
-  const double* Z = m[3 * S + 2];
+  const GP X = gp(m[3 * S]);
+  const GP E = gp(m[3 * S + 1]);
+  const GP Z = gp(m[3 * S + 2]);
   const double* J = djac;
   const long e = c + 1, w = c - 1, nn = c + sj, ss = c - sj, t = c + sk, b = c - sk;
   // metric products in the reference's association (J X) E etc.
@@ -63,7 +63,7 @@ __global__ void k_af_rhs(Dims d, AFArgs A, const double* const* __restrict__ m,
   const double pt_ze = J[t] * Z[t] * E[t], pb_ze = J[b] * Z[b] * E[b];
   const double a0 = ap0[c];
   for (int comp = 0; comp < A.nc; ++comp) {
-    const double* f = A.f[comp];
+    const GP f = gp(A.f[comp]);
     double r = pe_xe * (f[e + sj] - f[e - sj]);
     r = r - pw_xe * (f[w + sj] - f[w - sj]);
     r = r + pe_xz * (f[e + sk] - f[e - sk]);
@@ -79,28 +79,28 @@ __global__ void k_af_rhs(Dims d, AFArgs A, const double* const* __restrict__ m,
     const double dd = 0.25 * r;
     double su;
     if (S == 0) {
-      su = a0 * A.fn[comp][c];
-      su = su - 3.0 * re * A.nl[comp][c];
-      su = su + 1.0 * re * A.nl1[comp][c];
-      su = su + A.dif[comp][c];
-      su = su + 2.0 * re * A.s[comp][c];
+      su = a0 * gp(A.fn[comp])[c];
+      su = su - 3.0 * re * gp(A.nl[comp])[c];
+      su = su + 1.0 * re * gp(A.nl1[comp])[c];
+      su = su + gp(A.dif[comp])[c];
+      su = su + 2.0 * re * gp(A.s[comp])[c];
       su = su + dd;
     } else {
       su = a0 * f[c] + dd;
     }
     if (S < 2 && qform) {
-      double ae = fa[0][c];
-      if (i == 1) su = su + 2.0 * fa[0][c - 1] * 0.0;
+      double ae = gp(fa[0])[c];
+      if (i == 1) su = su + 2.0 * gp(fa[0])[c - 1] * 0.0;
       if (i == d.ni - 1) {
         const long nf = (long)(d.nj + 1) * (d.nk + 1);
         su = su + 2.0 * ae * A.ufar[comp * nf + j + (long)(d.nj + 1) * k];
         ae = 0.0;
       }
-      su = S == 0 ? su + fa[1][c] * f[nn] : su + ae * f[e];
-      su = su + fa[2][c] * f[t];
-      su = su + fa[2][b] * f[b];
+      su = S == 0 ? su + gp(fa[1])[c] * f[nn] : su + ae * f[e];
+      su = su + gp(fa[2])[c] * f[t];
+      su = su + gp(fa[2])[b] * f[b];
     }
-    A.su[comp][c] = su;
+    gpw(A.su[comp])[c] = su;
   }
 }
 
@@ -151,8 +151,9 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
   const bool act = i <= d.ni - 1 && j <= d.nj - 1;
   const int q0 = (ty + 1) * SF_W + tx + 1;
   const long sj = d.sj, sk = d.sk;
-  const double *PXE = P[0], *PXZ = P[1], *PEX = P[2], *PEZ = P[3], *PZX = P[4], *PZE = P[5];
-  const double *fE = fa[0], *fN = fa[1], *fT = fa[2];
+  const GP PXE = gp(P[0]), PXZ = gp(P[1]), PEX = gp(P[2]), PEZ = gp(P[3]), PZX = gp(P[4]),
+           PZE = gp(P[5]);
+  const GP fE = gp(fa[0]), fN = gp(fa[1]), fT = gp(fa[2]);
   // own-column values of planes kp-2, kp-1, kp: (J Z) X, (J Z) E and AT
   double zx[3] = {0.0, 0.0, 0.0}, ze[3] = {0.0, 0.0, 0.0}, at[3] = {0.0, 0.0, 0.0};
   for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
@@ -163,7 +164,7 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
       const int gj = min(bb.y * 4 + q / SF_W, d.nj);
       const long c = cidx(d, gi, gj, kp);
 #pragma unroll
-      for (int comp = 0; comp < NC; ++comp) sh[comp][slot][q] = A.f[comp][c];
+      for (int comp = 0; comp < NC; ++comp) sh[comp][slot][q] = gp(A.f[comp])[c];
     }
     zx[0] = zx[1]; zx[1] = zx[2];
     ze[0] = ze[1]; ze[1] = ze[2];
@@ -206,17 +207,17 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
       double su;
       if (S == 0 && TH) {
         su = a0 * F(0, 0, 0);
-        su = su - 3.0 * re * A.nl[comp][c];
-        su = su + 1.0 * re * A.nl1[comp][c];
-        su = su + A.dif[comp][c];
+        su = su - 3.0 * re * gp(A.nl[comp])[c];
+        su = su + 1.0 * re * gp(A.nl1[comp])[c];
+        su = su + gp(A.dif[comp])[c];
         su = su + dd;
       } else if (S == 0) {
         // U_N: equal to the staged stage input when nothing has written u since the step start
-        su = a0 * (fn_alias ? F(0, 0, 0) : A.fn[comp][c]);
-        su = su - 3.0 * re * A.nl[comp][c];
-        su = su + 1.0 * re * A.nl1[comp][c];
-        su = su + A.dif[comp][c];
-        su = su + 2.0 * re * A.s[comp][c];
+        su = a0 * (fn_alias ? F(0, 0, 0) : gp(A.fn[comp])[c]);
+        su = su - 3.0 * re * gp(A.nl[comp])[c];
+        su = su + 1.0 * re * gp(A.nl1[comp])[c];
+        su = su + gp(A.dif[comp])[c];
+        su = su + 2.0 * re * gp(A.s[comp])[c];
         su = su + dd;
       } else {
         su = a0 * F(0, 0, 0) + dd;
@@ -234,7 +235,7 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
         su = su + at[0] * F(-1, 0, 0);
       }
 #undef F
-      A.su[comp][c] = su;
+      gpw(A.su[comp])[c] = su;
     }
   }
 }
@@ -358,8 +359,8 @@ __global__ void __launch_bounds__(512) k_af_chunk(Dims d, AFArgs A, const double
       if (a > na || !row) continue;
       const long c = sa * a + sb * b + sp * p;
       // coefficients in the reference's order AE, AW, AN, AS, AT, AB (iter_u.f90:23-47)
-      double ae = fa[0][c], aw = fa[0][c - 1], an = fa[1][c], as = fa[1][c - d.sj],
-             at = fa[2][c], ab = fa[2][c - d.sk];
+      double ae = gp(fa[0])[c], aw = gp(fa[0])[c - 1], an = gp(fa[1])[c], as = gp(fa[1])[c - d.sj],
+             at = gp(fa[2])[c], ab = gp(fa[2])[c - d.sk];
       double ap = ap0[c] + ae + aw + an + as + at + ab;
       double s = su[c];
       const int i = S == 0 ? p : a;
@@ -520,8 +521,8 @@ __global__ void k_af_couplings(Dims d, const double* const* __restrict__ fa,
     const long r = t / (d.ni - 1);
     const int j = 1 + (int)(r % (d.nj - 1)), k = 1 + (int)(r / (d.nj - 1));
     const long c = cidx(d, i, j, k);
-    const double ae = fa[0][c], aw = fa[0][c - 1], an = fa[1][c], as = fa[1][c - d.sj],
-                 at = fa[2][c], ab = fa[2][c - d.sk];
+    const double ae = gp(fa[0])[c], aw = gp(fa[0])[c - 1], an = gp(fa[1])[c], as = gp(fa[1])[c - d.sj],
+                 at = gp(fa[2])[c], ab = gp(fa[2])[c - d.sk];
     const double ca = S == 0 ? fmax(fabs(an), fabs(as)) : fmax(fabs(ae), fabs(aw));
     const double cb = S == 2 ? fmax(fabs(an), fabs(as)) : fmax(fabs(at), fabs(ab));
     const double sb = S == 2 ? fabs(an) + fabs(as) : fabs(at) + fabs(ab);
@@ -582,8 +583,8 @@ __global__ void k_af_factor_k(Dims d, const double* const* __restrict__ fa,
   double bprev = 0.0;
   for (int k = 1; k <= d.nk - 1; ++k) {
     const long c = cidx(d, i, j, k);
-    const double ae = fa[0][c], aw = fa[0][c - 1], an = fa[1][c], as = fa[1][c - d.sj],
-                 at = fa[2][c], ab = fa[2][c - d.sk];
+    const double ae = gp(fa[0])[c], aw = gp(fa[0])[c - 1], an = gp(fa[1])[c], as = gp(fa[1])[c - d.sj],
+                 at = gp(fa[2])[c], ab = gp(fa[2])[c - d.sk];
     double ap = ap0[c] + ae + aw + an + as + at + ab;
     if (i == 1) ap = ap + aw;          // bcu.f90:18
     if (i == d.ni - 1) ap = ap + ae;   // bcu.f90:38

@@ -98,17 +98,19 @@ __global__ void __launch_bounds__(256) k_scan_q1v(Dims d, PtrSet8 qn, double* co
   const int k = 1 + (int)(blockIdx.z % (d.nk - 1));
   const int comp = (int)(blockIdx.z / (d.nk - 1));
   const int t = threadIdx.x;
+  const GP src = gp(qn.p[comp]);
+  const GPW dst = gpw(q[comp]);
   for (int r = 0; r < 4; ++r) {
     const int li = t % 32, lj = t / 32 + 8 * r;
     const int i = i0 + li, j = j0 + lj;
-    if (i <= d.ni - 1 && j <= d.nj - 1) tile[lj][li] = qn.p[comp][cidx(d, i, j, k)];
+    if (i <= d.ni - 1 && j <= d.nj - 1) tile[lj][li] = src[cidx(d, i, j, k)];
   }
   __syncthreads();
   const long lsa = d.nj + 1, lsb = (long)(d.nj + 1) * (d.ni + 1);
   for (int r = 0; r < 4; ++r) {
     const int lj = t % 32, li = t / 32 + 8 * r;
     const int i = i0 + li, j = j0 + lj;
-    if (i <= d.ni - 1 && j <= d.nj - 1) q[comp][j + lsa * i + lsb * k] = tile[lj][li];
+    if (i <= d.ni - 1 && j <= d.nj - 1) dst[j + lsa * i + lsb * k] = tile[lj][li];
   }
 }
 
@@ -121,16 +123,18 @@ __global__ void __launch_bounds__(256) k_scan_unpack1(Dims d, const double* cons
   const int comp = (int)(blockIdx.z / (d.nk - 1));
   const int t = threadIdx.x;
   const long lsa = d.nj + 1, lsb = (long)(d.nj + 1) * (d.ni + 1);
+  const GP src = gp(xo[comp]);
+  const GPW dst = gpw(f[comp]);
   for (int r = 0; r < 4; ++r) {
     const int lj = t % 32, li = t / 32 + 8 * r;
     const int i = i0 + li, j = j0 + lj;
-    if (i <= d.ni - 1 && j <= d.nj - 1) tile[lj][li] = xo[comp][j + lsa * i + lsb * k];
+    if (i <= d.ni - 1 && j <= d.nj - 1) tile[lj][li] = src[j + lsa * i + lsb * k];
   }
   __syncthreads();
   for (int r = 0; r < 4; ++r) {
     const int li = t % 32, lj = t / 32 + 8 * r;
     const int i = i0 + li, j = j0 + lj;
-    if (i <= d.ni - 1 && j <= d.nj - 1) f[comp][cidx(d, i, j, k)] = tile[lj][li];
+    if (i <= d.ni - 1 && j <= d.nj - 1) dst[cidx(d, i, j, k)] = tile[lj][li];
   }
 }
 
@@ -194,9 +198,9 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
     const int a0 = 1 + ca * SC_C;
     const int a1 = min(a0 + SC_C - 1, g.na);
     const int p0 = l * R + 1;
-    const double* qc = q[comp];
-    const double* fc = A.f[comp];
-    double* xc = xo[comp];
+    const GP qc = gp(q[comp]);
+    const GP fc = gp(A.f[comp]);
+    const GPW xc = gpw(xo[comp]);
     double xp[R];
     // line a0-1: the stage input's ghost line for the first chunk, else the preceding
     // chunk's last line of the previous pass

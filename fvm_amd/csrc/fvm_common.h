@@ -39,6 +39,15 @@ __device__ inline uint3 xcd_block() {
   return make_uint3(w % gx, (w / gx) % gy, w / (gx * gy));
 }
 
+// Device arrays read through pointers that come from tables in device memory (the metric, strain
+// and face-coefficient tables) would be generic pointers to the compiler: their loads would be
+// flat loads that may alias LDS, so each one is ordered behind the LDS stores before it.  Cast
+// to the global address space they become global loads, free to be issued together.
+typedef const __attribute__((address_space(1))) double* GP;
+typedef __attribute__((address_space(1))) double* GPW;
+__device__ inline GP gp(const double* p) { return (GP)p; }
+__device__ inline GPW gpw(double* p) { return (GPW)p; }
+
 // Metric a of the table (grouped per Cartesian component: xi_s, eta_s, zeta_s for s = x, y, z)
 // at cell c.  With XZ0 the entries xi_z (a = 6) and eta_z (a = 7), which are exactly zero on
 // every cell of a grid extruded along z (FvmCtx::xz_zero), are the constant 0.0 and are not
@@ -46,7 +55,7 @@ __device__ inline uint3 xcd_block() {
 // and sums keep the reference's order, so results are unchanged.
 template <bool XZ0>
 __device__ inline double mget(const double* const* __restrict__ m, int a, long c) {
-  return (XZ0 && (a == 6 || a == 7)) ? 0.0 : m[a][c];
+  return (XZ0 && (a == 6 || a == 7)) ? 0.0 : gp(m[a])[c];
 }
 
 // A 64 x 4 (I x J) tile of the K-marching stencil kernels with its one-cell halo, as staged in

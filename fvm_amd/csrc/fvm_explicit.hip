@@ -38,9 +38,9 @@ __global__ void k_contra(Dims d, const double* const* __restrict__ m, const doub
   if (i > d.ni || j > d.nj - 1) return;
   const long c = cidx(d, i, j, k);
   double U = u[c], V = v[c], W = w[c], J = djac[c];
-  double u1 = m[0][c] * U + m[3][c] * V + mget<XZ0>(m, 6, c) * W;
-  double u2 = m[1][c] * U + m[4][c] * V + mget<XZ0>(m, 7, c) * W;
-  double u3 = m[2][c] * U + m[5][c] * V + m[8][c] * W;
+  double u1 = gp(m[0])[c] * U + gp(m[3])[c] * V + mget<XZ0>(m, 6, c) * W;
+  double u2 = gp(m[1])[c] * U + gp(m[4])[c] * V + mget<XZ0>(m, 7, c) * W;
+  double u3 = gp(m[2])[c] * U + gp(m[5])[c] * V + gp(m[8])[c] * W;
   ju0[c] = J * u1;
   ju1[c] = J * u2;
   ju2[c] = J * u3;
@@ -76,7 +76,7 @@ __global__ void k_nl(Dims d, const double* __restrict__ ju0, const double* __res
   INTERIOR_IJK(d);
   Faces F = faces(d, i, c, ju0, ju1, ju2);
   for (int q = 0; q < nc; ++q) {
-    const double* f = cs.in[q];
+    const GP f = gp(cs.in[q]);
     double fc = f[c];
     double t = F.ue * (f[c + 1] + fc);
     t = t - F.uw * (f[c - 1] + fc);
@@ -84,7 +84,7 @@ __global__ void k_nl(Dims d, const double* __restrict__ ju0, const double* __res
     t = t - F.vs * (f[c - d.sj] + fc);
     t = t + F.wt * (f[c + d.sk] + fc);
     t = t - F.wb * (f[c - d.sk] + fc);
-    cs.out[q][c] = 0.5 * t;
+    gpw(cs.out[q])[c] = 0.5 * t;
   }
 }
 
@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict
       const int gj = min(bb.y * 4 + q / SF_W, d.nj);
       const long c = cidx(d, gi, gj, kp);
 #pragma unroll
-      for (int comp = 0; comp < NC; ++comp) sh[comp][slot][q] = cs.in[comp][c];
+      for (int comp = 0; comp < NC; ++comp) sh[comp][slot][q] = gp(cs.in[comp])[c];
     }
     j13[0] = j13[1]; j13[1] = j13[2];
     j23[0] = j23[1]; j23[1] = j23[2];
@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict
       r = r - j23[0] * (F(-1, 1, 0) - F(-1, -1, 0));
 #undef F
       t = t + 0.25 * r;
-      cs.out[comp][c] = 1.0 * t;
+      gpw(cs.out[comp])[c] = 1.0 * t;
     }
   }
 }
@@ -197,8 +197,8 @@ __global__ void __launch_bounds__(256) k_psource_km(Dims d, const double* const*
   const int k1 = min(k0 + PS_KC - 1, d.nk - 1);
   const bool act = i <= d.ni - 1 && j <= d.nj - 1;
   const long sj = d.sj;
-  double* out[3] = {s0, s1, s2};
-  double* f[3] = {u, v, w};
+  const GPW out[3] = {gpw(s0), gpw(s1), gpw(s2)};
+  const GPW f[3] = {gpw(u), gpw(v), gpw(w)};
   // (J zeta_s p) of the own column, planes kp-2, kp-1, kp, per component s
   double zp[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
   double cmax = 0.0;  // this thread's largest local CFL number
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(256) k_psource_km(Dims d, const double* const*
       const long cp = cidx(d, i, j, kp);
       const double jc = djac[cp], pc = p[cp];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) zp[q][2] = jc * m[3 * q + 2][cp] * pc;
+      for (int q = 0; q < 3; ++q) zp[q][2] = jc * gp(m[3 * q + 2])[cp] * pc;
     }
     if (kp < k0 + 1) continue;  // uniform over the workgroup
     const int k = kp - 1;
@@ -234,9 +234,9 @@ __global__ void __launch_bounds__(256) k_psource_km(Dims d, const double* const*
       }
       if (CORR && cflpart) {
         const double U = nv[0], V = nv[1], W = nv[2];
-        const double ca = fabs(m[0][c] * U + m[3][c] * V + mget<XZ0>(m, 6, c) * W);
-        const double cb = fabs(m[1][c] * U + m[4][c] * V + mget<XZ0>(m, 7, c) * W);
-        const double ce = fabs(m[2][c] * U + m[5][c] * V + m[8][c] * W);
+        const double ca = fabs(gp(m[0])[c] * U + gp(m[3])[c] * V + mget<XZ0>(m, 6, c) * W);
+        const double cb = fabs(gp(m[1])[c] * U + gp(m[4])[c] * V + mget<XZ0>(m, 7, c) * W);
+        const double ce = fabs(gp(m[2])[c] * U + gp(m[5])[c] * V + gp(m[8])[c] * W);
         cmax = fmax(cmax, dtime * (ca + cb + ce));
       }
     }
@@ -265,9 +265,9 @@ __global__ void k_correct_contra(Dims d, const double* const* __restrict__ m,
   const double V = v[c] - dtime * s1[c] / J;
   const double W = w[c] - dtime * s2[c] / J;
   u[c] = U; v[c] = V; w[c] = W;
-  const double u1 = m[0][c] * U + m[3][c] * V + mget<XZ0>(m, 6, c) * W;
-  const double u2 = m[1][c] * U + m[4][c] * V + mget<XZ0>(m, 7, c) * W;
-  const double u3 = m[2][c] * U + m[5][c] * V + m[8][c] * W;
+  const double u1 = gp(m[0])[c] * U + gp(m[3])[c] * V + mget<XZ0>(m, 6, c) * W;
+  const double u2 = gp(m[1])[c] * U + gp(m[4])[c] * V + mget<XZ0>(m, 7, c) * W;
+  const double u3 = gp(m[2])[c] * U + gp(m[5])[c] * V + gp(m[8])[c] * W;
   ju0[c] = J * u1;
   ju1[c] = J * u2;
   ju2[c] = J * u3;
@@ -286,9 +286,9 @@ __global__ void k_contra_iplanes(Dims d, const double* const* __restrict__ m,
     const int j = 1 + (int)(r % (d.nj - 1)), k = 1 + (int)(r / (d.nj - 1));
     const long c = cidx(d, i, j, k);
     const double U = u[c], V = v[c], W = w[c], J = djac[c];
-    const double u1 = m[0][c] * U + m[3][c] * V + m[6][c] * W;
-    const double u2 = m[1][c] * U + m[4][c] * V + m[7][c] * W;
-    const double u3 = m[2][c] * U + m[5][c] * V + m[8][c] * W;
+    const double u1 = gp(m[0])[c] * U + gp(m[3])[c] * V + gp(m[6])[c] * W;
+    const double u2 = gp(m[1])[c] * U + gp(m[4])[c] * V + gp(m[7])[c] * W;
+    const double u3 = gp(m[2])[c] * U + gp(m[5])[c] * V + gp(m[8])[c] * W;
     ju0[c] = J * u1;
     ju1[c] = J * u2;
     ju2[c] = J * u3;
@@ -314,7 +314,7 @@ __global__ void k_ufar(Dims d, const double* const* __restrict__ m, const double
     long r = t - comp * nf;
     int j = (int)(r % (d.nj + 1)), k = (int)(r / (d.nj + 1));
     if (j < 1 || j > d.nj - 1 || k < 1 || k > d.nk - 1) { ufar[t] = 0.0; continue; }
-    const double* b = fs.b[comp];
+    const GP b = gp(fs.b[comp]);
     const double uin = fs.uin[comp];
     long c = cidx(d, d.ni, j, k), mm = c - 1;
     double th = theta[c], uf;
@@ -467,9 +467,9 @@ __global__ void k_cfl(Dims d, const double* const* __restrict__ m, const double*
     int j = 1 + (int)(r % (d.nj - 1)), k = 1 + (int)(r / (d.nj - 1));
     long c = cidx(d, i, j, k);
     double U = u[c], V = v[c], W = w[c];
-    double a = fabs(m[0][c] * U + m[3][c] * V + m[6][c] * W);
-    double b = fabs(m[1][c] * U + m[4][c] * V + m[7][c] * W);
-    double e = fabs(m[2][c] * U + m[5][c] * V + m[8][c] * W);
+    double a = fabs(gp(m[0])[c] * U + gp(m[3])[c] * V + gp(m[6])[c] * W);
+    double b = fabs(gp(m[1])[c] * U + gp(m[4])[c] * V + gp(m[7])[c] * W);
+    double e = fabs(gp(m[2])[c] * U + gp(m[5])[c] * V + gp(m[8])[c] * W);
     double l = dtime * (a + b + e);
     mx = mx > l ? mx : l;
   }

@@ -28,7 +28,7 @@ __device__ inline void strain_store(const double* const* __restrict__ m, long c,
                                     double* mag) {
   const double xix = m[0][c], etx = m[1][c], ztx = m[2][c];
   const double xiy = m[3][c], ety = m[4][c], zty = m[5][c];
-  const double xiz = mget<XZ0>(m, 6, c), etz = mget<XZ0>(m, 7, c), ztz = m[8][c];
+  const double xiz = XZ0 ? 0.0 : m[6][c], etz = XZ0 ? 0.0 : m[7][c], ztz = m[8][c];
   double l11, l12, l13, l22, l23, l33;
   if (!onesided) {
     double t;
@@ -85,9 +85,9 @@ __global__ void k_strain_iplanes(Dims d, const double* const* __restrict__ m,
   strain_store<false>(m, c, true, diff(U), diff(V), diff(W), L, mag);
   if (T) {  // one-sided temperature gradient (stress_tensor_dy.f90:138-146, 188-196)
     const Diff3 g = diff(T);
-    dtx[c] = 0.5 * (2.0 * m[0][c] * g.i + m[1][c] * g.j + m[2][c] * g.k);
-    dty[c] = 0.5 * (2.0 * m[3][c] * g.i + m[4][c] * g.j + m[5][c] * g.k);
-    dtz[c] = 0.5 * (2.0 * m[6][c] * g.i + m[7][c] * g.j + m[8][c] * g.k);
+    dtx[c] = 0.5 * (2.0 * gp(m[0])[c] * g.i + gp(m[1])[c] * g.j + gp(m[2])[c] * g.k);
+    dty[c] = 0.5 * (2.0 * gp(m[3])[c] * g.i + gp(m[4])[c] * g.j + gp(m[5])[c] * g.k);
+    dtz[c] = 0.5 * (2.0 * gp(m[6])[c] * g.i + gp(m[7])[c] * g.j + gp(m[8])[c] * g.k);
   }
 }
 
@@ -148,11 +148,11 @@ __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* con
       const double ti = sh[3][s0][q0 + 1] - sh[3][s0][q0 - 1];
       const double tj = sh[3][s0][q0 + SF_W] - sh[3][s0][q0 - SF_W];
       const double tk = sh[3][sp][q0] - sh[3][sm][q0];
-      double t = m[0][c] * ti + m[1][c] * tj + m[2][c] * tk;
+      double t = gp(m[0])[c] * ti + gp(m[1])[c] * tj + gp(m[2])[c] * tk;
       io.dt[0][c] = 0.5 * t;
-      t = m[3][c] * ti + m[4][c] * tj + m[5][c] * tk;
+      t = gp(m[3])[c] * ti + gp(m[4])[c] * tj + gp(m[5])[c] * tk;
       io.dt[1][c] = 0.5 * t;
-      t = mget<XZ0>(m, 6, c) * ti + mget<XZ0>(m, 7, c) * tj + m[8][c] * tk;
+      t = mget<XZ0>(m, 6, c) * ti + mget<XZ0>(m, 7, c) * tj + gp(m[8])[c] * tk;
       io.dt[2][c] = 0.5 * t;
     }
     const int pl[3] = {sm, s0, sp};
@@ -208,8 +208,8 @@ __global__ void __launch_bounds__(256, 3) k_lm(Dims d, const double* const* __re
   const int k0 = 1 + bb.z * LM_KC;
   const int k1 = min(k0 + LM_KC - 1, d.nk - 1);
   const bool act = i <= d.ni - 1 && j <= d.nj - 1;
-  const double* L0 = L[0]; const double* L1 = L[1]; const double* L2 = L[2];
-  const double* L3 = L[3]; const double* L4 = L[4]; const double* L5 = L[5];
+  const GP L0 = gp(L[0]), L1 = gp(L[1]), L2 = gp(L[2]);
+  const GP L3 = gp(L[3]), L4 = gp(L[4]), L5 = gp(L[5]);
   // rolling in-plane sums of the products for planes k-1, k, k+1; product order:
   // uu vv ww uv uw vw, |S|S11 |S|S22 |S|S33 |S|S12 |S|S13 |S|S23
   double Pa[12], Pb[12], Pc[12];
@@ -263,9 +263,9 @@ __global__ void __launch_bounds__(256, 3) k_lm(Dims d, const double* const* __re
     const int k = kp - 1;
     const long c = cidx(d, i, j, k);
     const long sj = d.sj, sk = d.sk;
-    const double xix = m[0][c], etx = m[1][c], ztx = m[2][c];
-    const double xiy = m[3][c], ety = m[4][c], zty = m[5][c];
-    const double xiz = mget<XZ0>(m, 6, c), etz = mget<XZ0>(m, 7, c), ztz = m[8][c];
+    const double xix = gp(m[0])[c], etx = gp(m[1])[c], ztx = gp(m[2])[c];
+    const double xiy = gp(m[3])[c], ety = gp(m[4])[c], zty = gp(m[5])[c];
+    const double xiz = mget<XZ0>(m, 6, c), etz = mget<XZ0>(m, 7, c), ztz = gp(m[8])[c];
 #define DI(F) (F[c + 1] - F[c - 1])
 #define DJ(F) (F[c + sj] - F[c - sj])
 #define DK(F) (F[c + sk] - F[c - sk])
@@ -417,7 +417,8 @@ __global__ void __launch_bounds__(256) k_lm_t(Dims d, const double* const* __res
       const double Fp = (Pa[q] + 2.0 * Pb[q] + Pc[q]) / 64.0;
       const double Fr = (Pa[3 + q] + 2.0 * Pb[3 + q] + Pc[3 + q]) / 64.0;
       const double Lq = -Fp + hat[q] * th;
-      const double g = m[3 * q][c] * gi_ + m[3 * q + 1][c] * gj_ + m[3 * q + 2][c] * gk_;
+      const double g =
+          gp(m[3 * q])[c] * gi_ + gp(m[3 * q + 1])[c] * gj_ + gp(m[3 * q + 2])[c] * gk_;
       const double M = fw2 * (4.0 * mh * 0.5 * g - Fr);
       if (q == 0) {
         lm = Lq * M;
@@ -484,7 +485,7 @@ __global__ void __launch_bounds__(256) k_sgs_t_div(Dims d, const double* const* 
         else if (dir == 1) { pp = sh[sc][q][e0 + SF_W]; pn = sh[sc][q][e0 - SF_W]; }
         else { pp = sh[slot][q][e0]; pn = Pm[q]; }
         const double inner = -pp + pn;
-        const double term = m[3 * q + dir][c] * inner;
+        const double term = gp(m[3 * q + dir])[c] * inner;
         t = (q == 0 && dir == 0) ? term : t + term;
       }
     }
@@ -578,8 +579,8 @@ __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __
   const int k1 = min(k0 + SD_KC - 1, d.nk - 1);
   const bool act = i <= d.ni - 1 && j <= d.nj - 1;
   const int e0 = (ty + 1) * SF_W + tx + 1;
-  const double* S0 = L[0]; const double* S1 = L[1]; const double* S2 = L[2];
-  const double* S3 = L[3]; const double* S4 = L[4]; const double* S5 = L[5];
+  const GP S0 = gp(L[0]), S1 = gp(L[1]), S2 = gp(L[2]);
+  const GP S3 = gp(L[3]), S4 = gp(L[4]), S5 = gp(L[5]);
   double* nl[3] = {nlx, nly, nlz};
   double* so[3] = {sx, sy, sz};
   // symmetric strain components stored as 11,12,13,22,23,33: row/col -> product index
