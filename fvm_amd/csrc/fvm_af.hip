@@ -188,6 +188,34 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
     const double pt_zx = zx[2], pb_zx = zx[0];
     const double pt_ze = ze[2], pb_ze = ze[0];
     const double a0 = ap0[c];
+    // every global input of the cell is loaded before the first SU store: the stores could alias
+    // the inputs as far as the compiler knows, and loads after them would wait for them
+    double vnl[NC], vnl1[NC], vdif[NC], vs[NC], vfn[NC], vuf[NC];
+#pragma unroll
+    for (int comp = 0; comp < NC; ++comp) {
+      vnl[comp] = vnl1[comp] = vdif[comp] = vs[comp] = vfn[comp] = vuf[comp] = 0.0;
+      if (S == 0) {
+        vnl[comp] = gp(A.nl[comp])[c];
+        vnl1[comp] = gp(A.nl1[comp])[c];
+        vdif[comp] = gp(A.dif[comp])[c];
+        if (!TH) {
+          vs[comp] = gp(A.s[comp])[c];
+          if (!fn_alias) vfn[comp] = gp(A.fn[comp])[c];
+        }
+      }
+    }
+    double qE = 0.0, qW = 0.0, qN = 0.0;
+    const long nf = (long)(d.nj + 1) * (d.nk + 1);
+    if (qform) {
+      qE = fE[c];
+      qW = fE[c - 1];
+      if (S == 0) qN = fN[c];
+      if (i == d.ni - 1) {
+#pragma unroll
+        for (int comp = 0; comp < NC; ++comp)
+          vuf[comp] = A.ufar[comp * nf + j + (long)(d.nj + 1) * k];
+      }
+    }
 #pragma unroll
     for (int comp = 0; comp < NC; ++comp) {
 #define F(dk, dj, di) sh[comp][pl[(dk) + 1]][q0 + (dj) * SF_W + (di)]
@@ -207,30 +235,29 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
       double su;
       if (S == 0 && TH) {
         su = a0 * F(0, 0, 0);
-        su = su - 3.0 * re * gp(A.nl[comp])[c];
-        su = su + 1.0 * re * gp(A.nl1[comp])[c];
-        su = su + gp(A.dif[comp])[c];
+        su = su - 3.0 * re * vnl[comp];
+        su = su + 1.0 * re * vnl1[comp];
+        su = su + vdif[comp];
         su = su + dd;
       } else if (S == 0) {
         // U_N: equal to the staged stage input when nothing has written u since the step start
-        su = a0 * (fn_alias ? F(0, 0, 0) : gp(A.fn[comp])[c]);
-        su = su - 3.0 * re * gp(A.nl[comp])[c];
-        su = su + 1.0 * re * gp(A.nl1[comp])[c];
-        su = su + gp(A.dif[comp])[c];
-        su = su + 2.0 * re * gp(A.s[comp])[c];
+        su = a0 * (fn_alias ? F(0, 0, 0) : vfn[comp]);
+        su = su - 3.0 * re * vnl[comp];
+        su = su + 1.0 * re * vnl1[comp];
+        su = su + vdif[comp];
+        su = su + 2.0 * re * vs[comp];
         su = su + dd;
       } else {
         su = a0 * F(0, 0, 0) + dd;
       }
       if (qform) {
-        double ae = fE[c];
-        if (i == 1) su = su + 2.0 * fE[c - 1] * (TH ? kTWall : 0.0);
+        double ae = qE;
+        if (i == 1) su = su + 2.0 * qW * (TH ? kTWall : 0.0);
         if (i == d.ni - 1) {
-          const long nf = (long)(d.nj + 1) * (d.nk + 1);
-          su = su + 2.0 * ae * A.ufar[comp * nf + j + (long)(d.nj + 1) * k];
+          su = su + 2.0 * ae * vuf[comp];
           ae = 0.0;
         }
-        su = S == 0 ? su + fN[c] * F(0, 1, 0) : su + ae * F(0, 0, 1);
+        su = S == 0 ? su + qN * F(0, 1, 0) : su + ae * F(0, 0, 1);
         su = su + at[1] * F(1, 0, 0);
         su = su + at[0] * F(-1, 0, 0);
       }
@@ -480,28 +507,71 @@ __global__ void __launch_bounds__(256) k_af_thomas_k_reg(Dims d, AFArgs A,
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int comp = blockIdx.z;
   if (i > d.ni - 1 || j > d.nj - 1) return;
-  double* f = A.f[comp];
-  const double* su = A.su[comp];
+  const GPW f = gpw(A.f[comp]);
+  const GP su = gp(A.su[comp]);
   const int n = d.nk - 1;
   const long c0 = cidx(d, i, j, 0);
+  // Rows are handled in blocks of KB: the operands of block b+1 are loaded before block b is
+  // solved (the scheduling barrier keeps them there), so one memory latency covers KB rows
+  // instead of each row waiting for its own loads.  Rows past n repeat row n and are never kept,
+  // so no load sits behind a branch on n.
+  constexpr int KB = 8, NB = NMAX / KB;
+  static_assert(NMAX % KB == 0, "row blocks");
   double xs[NMAX];
   double x = f[c0];  // K ghost of the stage input
+  double bs[KB], ba[KB], bd[KB];
 #pragma unroll
-  for (int k = 1; k <= NMAX; ++k) {
-    if (k <= n) {
-      const long c = c0 + (long)k * d.sk;
-      x = (su[c] - ta[c] * x) / td[c];
+  for (int r = 0; r < KB; ++r) {
+    const long c = c0 + (long)min(r + 1, n) * d.sk;
+    bs[r] = su[c]; ba[r] = ta[c]; bd[r] = td[c];
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    double ns[KB], na[KB], nd[KB];
+    if (b + 1 < NB) {
+#pragma unroll
+      for (int r = 0; r < KB; ++r) {
+        const long c = c0 + (long)min((b + 1) * KB + r + 1, n) * d.sk;
+        ns[r] = su[c]; na[r] = ta[c]; nd[r] = td[c];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int r = 0; r < KB; ++r) {
+      const int k = b * KB + r + 1;
+      const double y = (bs[r] - ba[r] * x) / bd[r];
+      x = k <= n ? y : x;
       xs[k - 1] = x;
+    }
+    if (b + 1 < NB) {
+#pragma unroll
+      for (int r = 0; r < KB; ++r) { bs[r] = ns[r]; ba[r] = na[r]; bd[r] = nd[r]; }
     }
   }
   double xn = f[c0 + (long)(n + 1) * d.sk];
+  double bt[KB];
 #pragma unroll
-  for (int k = NMAX; k >= 1; --k) {
-    if (k <= n) {
-      const long c = c0 + (long)k * d.sk;
-      x = xs[k - 1] - tb[c] * xn;
-      f[c] = x;
-      xn = x;
+  for (int r = 0; r < KB; ++r) bt[r] = tb[c0 + (long)min(NMAX - r, n) * d.sk];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    double nt[KB];
+    if (b + 1 < NB) {
+#pragma unroll
+      for (int r = 0; r < KB; ++r) nt[r] = tb[c0 + (long)min(NMAX - (b + 1) * KB - r, n) * d.sk];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int r = 0; r < KB; ++r) {
+      const int k = NMAX - b * KB - r;
+      const double y = xs[k - 1] - bt[r] * xn;
+      if (k <= n) {
+        f[c0 + (long)k * d.sk] = y;
+        xn = y;
+      }
+    }
+    if (b + 1 < NB) {
+#pragma unroll
+      for (int r = 0; r < KB; ++r) bt[r] = nt[r];
     }
   }
 }

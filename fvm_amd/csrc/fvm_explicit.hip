@@ -215,8 +215,9 @@ __global__ void __launch_bounds__(256) k_psource_km(Dims d, const double* const*
     const int k = kp - 1;
     if (act) {
       const long c = cidx(d, i, j, k);
-      double nv[3];
+      double nv[3], sv[3], fv[3];
       const double J = djac[c];
+      // all loads of the cell precede its stores (which might alias them for the compiler)
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         double t = djac[c + 1] * mget<XZ0>(m, 3 * q, c + 1) * p[c + 1];
@@ -225,10 +226,14 @@ __global__ void __launch_bounds__(256) k_psource_km(Dims d, const double* const*
         t = t - djac[c - sj] * mget<XZ0>(m, 3 * q + 1, c - sj) * p[c - sj];
         t = t + zp[q][2];
         t = t - zp[q][0];
-        const double sv = -(0.5 * t);
-        out[q][c] = sv;
+        sv[q] = -(0.5 * t);
+        fv[q] = CORR ? f[q][c] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        out[q][c] = sv[q];
         if (CORR) {
-          nv[q] = sign < 0 ? f[q][c] - dtime * sv / J : f[q][c] + dtime * sv / J;
+          nv[q] = sign < 0 ? fv[q] - dtime * sv[q] / J : fv[q] + dtime * sv[q] / J;
           f[q][c] = nv[q];
         }
       }

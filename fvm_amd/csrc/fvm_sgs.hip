@@ -615,6 +615,12 @@ __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __
     const int k = kp - 1;
     const long c = cidx(d, i, j, k);
     const double J = (mode & 1) ? djac[c] : 0.0;
+    // the metrics and NL of the cell are loaded before its stores (which might alias them)
+    double mv[9], nv[3];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) mv[a] = mget<XZ0>(m, a, c);
+#pragma unroll
+    for (int row = 0; row < 3; ++row) nv[row] = (mode & 1) ? nl[row][c] : 0.0;
 #pragma unroll
     for (int row = 0; row < 3; ++row) {
       double t = 0.0;
@@ -628,13 +634,13 @@ __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __
           else if (dir == 1) { pp = sh[sc][q][e0 + SF_W]; pn = sh[sc][q][e0 - SF_W]; }
           else { pp = sh[slot][q][e0]; pn = Pm[q]; }
           const double inner = -pp + pn;
-          const double term = mget<XZ0>(m, 3 * col + dir, c) * inner;
+          const double term = mv[3 * col + dir] * inner;
           t = (col == 0 && dir == 0) ? term : t + term;
         }
       }
       const double sg = 0.5 * t;
       if (mode & 2) so[row][c] = sg;
-      if (mode & 1) nl[row][c] = nl[row][c] + sg * J;
+      if (mode & 1) nl[row][c] = nv[row] + sg * J;
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) Pm[q] = sh[sc][q][e0];
