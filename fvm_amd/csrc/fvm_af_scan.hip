@@ -175,7 +175,7 @@ __device__ inline double shfl_down_d(double v, int o) { return __shfl_down(v, o,
 // remaining lines of the chunk would change by less than that too and keep their values from
 // the previous pass; the chunk's last line is then unchanged (bnd_out = bnd_in).  Where rho is
 // small (rho ~ dt |grad a|^2 / 2 Re), a pass after the first re-solves a few lines per chunk.
-template <int S, int R>
+template <int S, int R, bool EARLY>
 __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double* __restrict__ beta,
                                                   const double* __restrict__ invd,
                                                   const double* __restrict__ bb,
@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
                                                   const double* const* q, double* const* xo,
                                                   const double* __restrict__ bnd_in,
                                                   double* bnd_out, int nchunk, double* part,
-                                                  int early, double tol) {
+                                                  double tol, double* sink) {
   const ScanGeo g = scan_geo(d, S);
   const long G = gridDim.x;
   const long lin = blockIdx.x;
@@ -216,29 +216,29 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
       }
       xp[r] = v;
     }
-    double be[R], iv[R], bv[R], cm[R], qv[R], xv[R];
+    // Operands of one line.  Rows past n (the last lane's tail) read row n and are never kept,
+    // and the next line is always fetched (the last line again at the end of the chunk): with
+    // no load or store behind a branch the compiler counts the loads in flight exactly and
+    // waits only for the ones it needs, so the next line's loads overlap this line's scans.
+    double be[R], iv[R], bv[R], cm[R], qv[R], xv[R], y0n, xnn;
     auto load = [&](int a) {
       const long lb = g.ls_a * a + g.ls_b * b;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int p = p0 + r;
-        if (p <= g.n) {
-          be[r] = beta[lb + p]; iv[r] = invd[lb + p]; bv[r] = bb[lb + p];
-          cm[r] = cam[lb + p]; qv[r] = qc[lb + p];
-          xv[r] = early ? xc[lb + p] : 0.0;  // the previous pass's value of the line
-        } else {
-          be[r] = 0.0; iv[r] = 0.0; bv[r] = 0.0; cm[r] = 0.0; qv[r] = 0.0; xv[r] = 0.0;
-        }
+        const long o = lb + min(p0 + r, g.n);
+        be[r] = beta[o]; iv[r] = invd[o]; bv[r] = bb[o]; cm[r] = cam[o]; qv[r] = qc[o];
+        xv[r] = EARLY ? xc[o] : 0.0;  // the previous pass's value of the line
       }
+      const long nb0 = g.ns_a * a + g.ns_b * b;  // natural base of the line
+      y0n = fc[nb0];                             // row 0 ghost
+      xnn = fc[nb0 + g.ns_p * (g.n + 1)];        // row n+1 ghost
     };
     load(a0);
     bool stopped = false;
     double xrun = 0.0;  // largest |x| of the lines this wavefront solved in this pass
     for (int a = a0; a <= a1; ++a) {
       const long lb = g.ls_a * a + g.ls_b * b;
-      const long nb0 = g.ns_a * a + g.ns_b * b;  // natural base of the line
-      const double y0 = fc[nb0];                 // row 0 ghost
-      const double xn = fc[nb0 + g.ns_p * (g.n + 1)];  // row n+1 ghost
+      const double y0 = y0n, xn = xnn;
       // forward: y(p) = al(p) + be(p) y(p-1)
       double al[R], bt[R], bw[R], xo[R];
       double Bt = 1.0, At = 0.0;
@@ -247,58 +247,64 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
         al[r] = (qv[r] + cm[r] * xp[r]) * iv[r];
         bt[r] = be[r];
         bw[r] = bv[r];
-        xo[r] = xv[r];  // line a's previous value (load(a + 1) below refills xv)
-        if (p0 + r <= g.n) {
-          At = al[r] + bt[r] * At;
-          Bt = bt[r] * Bt;
-        }
+        xo[r] = xv[r];  // line a's previous value (load below refills xv)
+        const bool in = p0 + r <= g.n;
+        At = in ? al[r] + bt[r] * At : At;
+        Bt = in ? bt[r] * Bt : Bt;
       }
-      if (a < a1) load(a + 1);  // next line's operands in flight during the scans
+      load(min(a + 1, a1));  // next line's operands in flight during the scans
+      __builtin_amdgcn_sched_barrier(0);  // keep those loads ahead of the scans
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
         const double Ao = shfl_up_d(At, o), Bo = shfl_up_d(Bt, o);
-        if (l >= o) { At = At + Bt * Ao; Bt = Bt * Bo; }
+        const bool up = l >= o;
+        At = up ? At + Bt * Ao : At;
+        Bt = up ? Bt * Bo : Bt;
       }
       double Ae = shfl_up_d(At, 1), Be = shfl_up_d(Bt, 1);
-      if (l == 0) { Ae = 0.0; Be = 1.0; }
+      Ae = l == 0 ? 0.0 : Ae;
+      Be = l == 0 ? 1.0 : Be;
       double yin = Ae + Be * y0;
       double y[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         y[r] = al[r] + bt[r] * yin;
-        if (p0 + r <= g.n) yin = y[r];
+        yin = p0 + r <= g.n ? y[r] : yin;
       }
       // backward: x(p) = y(p) - B(p) x(p+1)
       double Ct = 1.0, Dt = 0.0;
 #pragma unroll
       for (int r = R - 1; r >= 0; --r) {
-        if (p0 + r <= g.n) {
-          Dt = y[r] - bw[r] * Dt;
-          Ct = -bw[r] * Ct;
-        }
+        const bool in = p0 + r <= g.n;
+        Dt = in ? y[r] - bw[r] * Dt : Dt;
+        Ct = in ? -bw[r] * Ct : Ct;
       }
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
         const double Do = shfl_down_d(Dt, o), Co = shfl_down_d(Ct, o);
-        if (l + o < 64) { Dt = Dt + Ct * Do; Ct = Ct * Co; }
+        const bool dn = l + o < 64;
+        Dt = dn ? Dt + Ct * Do : Dt;
+        Ct = dn ? Ct * Co : Ct;
       }
       double De = shfl_down_d(Dt, 1), Ce = shfl_down_d(Ct, 1);
-      if (l == 63) { De = 0.0; Ce = 1.0; }
+      De = l == 63 ? 0.0 : De;
+      Ce = l == 63 ? 1.0 : Ce;
       double xin = De + Ce * xn;
       double dl = 0.0;
 #pragma unroll
       for (int r = R - 1; r >= 0; --r) {
         const int p = p0 + r;
-        if (p <= g.n) {
-          const double x = y[r] - bw[r] * xin;
-          xin = x;
-          dl = fmax(dl, fabs(x - xo[r]));
-          xrun = fmax(xrun, fabs(x));
-          xc[lb + p] = x;
-          xp[r] = x;
-        }
+        const bool in = p <= g.n;
+        const double x = y[r] - bw[r] * xin;
+        xin = in ? x : xin;
+        dl = in ? fmax(dl, fabs(x - xo[r])) : dl;
+        xrun = in ? fmax(xrun, fabs(x)) : xrun;
+        // a row past n stores to this lane's slot of the sink instead of a branch
+        const GPW dst = in ? xc + (lb + p) : gpw(sink) + l;
+        *dst = x;
+        xp[r] = in ? x : xp[r];
       }
-      if (early && a < a1) {
+      if (EARLY && a < a1) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
           dl = fmax(dl, __shfl_xor(dl, o, 64));
@@ -399,9 +405,10 @@ int scan_stage(FvmCtx* c, const AFArgs& A, int* iters, bool temp) {
   for (;;) {
     {
       ProfScope pk(c, S == 0 ? "k_scan_pass_i" : "k_scan_pass_j");
-      hipLaunchKernelGGL((k_scan_pass<S, R>), dim3((unsigned)G), dim3(64), 0, st, d, A, lf[0],
-                         lf[1], lf[2], lf[3], (const double* const*)dq, dxo, c->bnd[slot],
-                         c->bnd[slot ^ 1], nchunk, c->red, it > 0 ? 1 : 0, tol);
+      auto kern = it > 0 ? k_scan_pass<S, R, true> : k_scan_pass<S, R, false>;
+      hipLaunchKernelGGL(kern, dim3((unsigned)G), dim3(64), 0, st, d, A, lf[0], lf[1], lf[2],
+                         lf[3], (const double* const*)dq, dxo, c->bnd[slot], c->bnd[slot ^ 1],
+                         nchunk, c->red, tol, c->sink);
     }
     hipLaunchKernelGGL(k_scan_max, dim3(1), dim3(256), 0, st, c->red, G, c->scal + 20);
     HIP_CHECK(hipGetLastError());

@@ -277,7 +277,8 @@ static int create_impl(const fvm_params* prm, const double* xg, const double* yg
   r |= dalloc(c, &c->red, c->nred);
   c->ncflpart = (int)nblk;
   r |= dalloc(c, &c->cflpart, nblk);
-  r |= dalloc(c, &c->scal, 64);
+  r |= dalloc(c, &c->scal, 128);
+  c->sink = c->scal + 64;
   double *p32 = nullptr, *su32 = nullptr;
   r |= dalloc(c, &p32, (n + 1) / 2);
   r |= dalloc(c, &su32, (n + 1) / 2);

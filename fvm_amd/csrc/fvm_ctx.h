@@ -88,6 +88,7 @@ struct FvmCtx {
   int ncflpart = 0;          // capacity of cflpart
   int ncflused = 0;          // entries the last projection wrote
   double* scal = nullptr;    // device scalars (c_avg, residual sums, pref, ...)
+  double* sink = nullptr;    // 64 doubles that lanes past a line's end store to (never read)
   double* hscal = nullptr;   // pinned host mirror of scal
   // host mailbox: fine-grained pinned memory that k_publish writes scalars and then a sequence
   // number into, so the host reads a convergence test's inputs without a copy kernel or a
