@@ -131,7 +131,7 @@ __global__ void k_af_products(Dims d, const double* __restrict__ J, const double
 // TH selects ITER_T (RLES/iter_t.f90:49-55): re is PE, the stage-1 source is
 // 2 PE J^-1/dt T - 3 PE ADV + PE ADV_NM1 + DIFT + DD (A.nl = ADV, A.nl1 = ADV_NM1,
 // A.dif = DIFT) and BCT's wall fold adds 2 AW T_WALL (bct.f90:16-19).
-constexpr int RK_KC = 8;
+constexpr int RK_KC = 16;
 
 template <int S, bool TH>
 __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,

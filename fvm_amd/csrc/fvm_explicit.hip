@@ -95,7 +95,7 @@ __global__ void k_nl(Dims d, const double* __restrict__ ju0, const double* __res
 // three planes of u_n, v_n, w_n (one-cell halo) in LDS, so the seven-point and twelve-edge
 // stencils are read from HBM once per column; J^-1 Q13, J^-1 Q23 and D_T of the planes K+-1
 // ride along in registers.
-constexpr int DF_KC = 8;
+constexpr int DF_KC = 4;
 
 template <int NC>
 __global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict__ pdI,
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict
 // per workgroup.  A workgroup owns a 64 x 4 (I x J) column of cells over PS_KC
 // consecutive planes and marches up K; the products (J zeta_s p) of the planes K+-1 ride along
 // in registers, so only the I and J neighbours are fetched per cell.
-constexpr int PS_KC = 8;
+constexpr int PS_KC = 16;
 
 template <int CORR, bool XZ0>
 __global__ void __launch_bounds__(256) k_psource_km(Dims d, const double* const* __restrict__ m,

@@ -99,7 +99,7 @@ __global__ void k_strain_iplanes(Dims d, const double* const* __restrict__ m,
 // I-1, I, I+1.
 // With NF = 4 the temperature rides along: its gradient dT/dx_i = 0.5 (xi_i dT/dxi +
 // eta_i dT/deta + zeta_i dT/dzeta) (stress_tensor_dy.f90:78-94) and its test filter T^.
-constexpr int SF_KC = 8;
+constexpr int SF_KC = 16;
 
 struct SfOut {
   const double* in[4];  // u, v, w, T
@@ -185,8 +185,8 @@ __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* con
 // products once per plane and combines three consecutive planes with (1,2,1) weights.  The sum
 // is the reference's 27-point filter with its terms grouped by plane (rounding-level
 // differences only).
-constexpr int LM_KC = 8;
-constexpr int SD_KC = 8;  // planes per workgroup of the K-marching SGS divergences
+constexpr int LM_KC = 16;
+constexpr int SD_KC = 16;  // planes per workgroup of the K-marching SGS divergences
 constexpr int LM_W = 66, LM_H = 6;  // tile with halo
 
 template <bool XZ0>
