@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
   if (l == 0) { part[2 * lin] = dmax; part[2 * lin + 1] = xmax; }
 }
 
-__global__ void k_scan_max(const double* __restrict__ part, long nb, double* out) {
+__global__ void k_scan_max(const double* __restrict__ part, long nb, double* out, Mailbox mb) {
   __shared__ double s0[256], s1[256];
   const int t = threadIdx.x;
   double a = 0.0, b = 0.0;
@@ -353,7 +353,11 @@ __global__ void k_scan_max(const double* __restrict__ part, long nb, double* out
     if (t < s) { s0[t] = fmax(s0[t], s0[t + s]); s1[t] = fmax(s1[t], s1[t + s]); }
     __syncthreads();
   }
-  if (t == 0) { out[0] = s0[0]; out[1] = s1[0]; }
+  if (t == 0) {
+    out[0] = s0[0]; out[1] = s1[0];
+    const double v[2] = {s0[0], s1[0]};
+    publish(mb, v, 2);
+  }
 }
 
 template <int S, int R>
@@ -410,12 +414,14 @@ int scan_stage(FvmCtx* c, const AFArgs& A, int* iters, bool temp) {
                          lf[3], (const double* const*)dq, dxo, c->bnd[slot], c->bnd[slot ^ 1],
                          nchunk, c->red, tol, c->sink);
     }
-    hipLaunchKernelGGL(k_scan_max, dim3(1), dim3(256), 0, st, c->red, G, c->scal + 20);
+    // the first pass never suffices on its own: its maxima are not read
+    const Mailbox mb = it > 0 ? fvm_mail(c) : Mailbox{nullptr, nullptr, 0};
+    hipLaunchKernelGGL(k_scan_max, dim3(1), dim3(256), 0, st, c->red, G, c->scal + 20, mb);
     HIP_CHECK(hipGetLastError());
     slot ^= 1;
     ++it;
-    if (it < 2) continue;  // the first pass never suffices on its own
-    const int rf = fvm_fetch(c, c->scal + 20, 2, c->hscal + 20);
+    if (it < 2) continue;
+    const int rf = fvm_mail_wait(c, mb, 2, c->hscal + 20);
     if (rf) return rf;
     const double dm = c->hscal[20], xm = c->hscal[21];
     if (!(dm == dm) || !(xm == xm)) {

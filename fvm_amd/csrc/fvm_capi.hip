@@ -19,22 +19,22 @@ __global__ void k_publish(const double* __restrict__ src, int n, volatile double
   *seq = s;
 }
 
-// The n device scalars at src, in stream order, into dst on the host.  Enqueues k_publish and
-// spins on the mailbox's sequence number; the stream keeps running (no synchronisation), so
-// the device idles only for the host's reaction and the next launch.  A stream error or a
-// drained stream without the expected number ends the wait with FVM_ERR_HIP.
-int fvm_fetch(FvmCtx* c, const double* src, int n, double* dst) {
-  if (n < 1 || n > 48) return FVM_ERR_ARG;
-  const unsigned long long s = ++c->seq;
-  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, c->stream, src, n, c->hmail, c->hseq, s);
-  HIP_CHECK(hipGetLastError());
+Mailbox fvm_mail(FvmCtx* c) {
+  return Mailbox{c->hmail, c->hseq, ++c->seq};
+}
+
+// Spins on the mailbox's sequence number until mb's kernel has published, then copies its n
+// scalars to dst.  The stream keeps running (no synchronisation), so the device idles only for
+// the host's reaction and the next launch.  A stream error or a drained stream without the
+// expected number ends the wait with FVM_ERR_HIP.
+int fvm_mail_wait(FvmCtx* c, const Mailbox& mb, int n, double* dst) {
   volatile unsigned long long* f = c->hseq;
-  for (long it = 1; *f < s; ++it) {
+  for (long it = 1; *f < mb.s; ++it) {
     if ((it & 1023) == 0) {
       const hipError_t e = hipStreamQuery(c->stream);
       if (e == hipSuccess) {
-        if (*f >= s) break;
-        fprintf(stderr, "fvm_amd: stream drained without publishing scalar set %llu\n", s);
+        if (*f >= mb.s) break;
+        fprintf(stderr, "fvm_amd: stream drained without publishing scalar set %llu\n", mb.s);
         return FVM_ERR_HIP;
       }
       if (e != hipErrorNotReady) {
@@ -47,6 +47,15 @@ int fvm_fetch(FvmCtx* c, const double* src, int n, double* dst) {
   const volatile double* m = c->hmail;
   for (int q = 0; q < n; ++q) dst[q] = m[q];
   return FVM_OK;
+}
+
+// The n device scalars at src, in stream order, into dst on the host (k_publish, then the wait).
+int fvm_fetch(FvmCtx* c, const double* src, int n, double* dst) {
+  if (n < 1 || n > 48) return FVM_ERR_ARG;
+  const Mailbox mb = fvm_mail(c);
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, c->stream, src, n, c->hmail, c->hseq, mb.s);
+  HIP_CHECK(hipGetLastError());
+  return fvm_mail_wait(c, mb, n, dst);
 }
 
 ProfScope::ProfScope(FvmCtx* ctx, const char* nm) : c(ctx), name(nm) {
@@ -510,12 +519,15 @@ int fvm_step(fvm_ctx* c, fvm_step_info* info) {
   double* f[3] = {c->u, c->v, c->w};
   CK(fvm_k_bcu_ghosts(c, f, 3));                  // :138-142
   if (c->thermal) CK(fvm_k_iter_t(c));            // :145-147 (ITER_T, BCT)
-  if (c->two_d)
+  if (c->two_d) {
     CK(fvm_k_calccoef2d(c, nullptr, true));       // 2D CALCCOEF
-  else
-    CK(fvm_k_calccoef(c, nullptr, true));         // :150
+    r = fvm_fetch(c, c->scal, 17, c->hscal);
+  } else {
+    const Mailbox mb = fvm_mail(c);               // the force kernel publishes scal[0..16]
+    CK(fvm_k_calccoef(c, nullptr, true, &mb));    // :150
+    r = fvm_mail_wait(c, mb, 17, c->hscal);
+  }
 #undef CK
-  r = fvm_fetch(c, c->scal, 17, c->hscal);
   if (r) return r;
   st.c_avg = c->prm.use_les ? c->hscal[0] : 0.0;
   st.c_t = (c->thermal && c->prm.use_les) ? c->hscal[1] : 0.0;

@@ -170,6 +170,23 @@ __global__ void k_ghost_fill_t(Dims d, T* __restrict__ a) {
   }
 }
 
+// Host mailbox of a reduction kernel's result: fine-grained pinned memory that lane 0 of the
+// kernel's last workgroup fills (values, then a system-scope fence, then the sequence number s),
+// so the host reads the result without a further kernel or a stream synchronisation.
+// dst == nullptr: nothing is published.
+struct Mailbox {
+  volatile double* dst;
+  volatile unsigned long long* seq;
+  unsigned long long s;
+};
+
+__device__ inline void publish(const Mailbox& mb, const double* v, int n) {
+  if (!mb.dst) return;
+  for (int q = 0; q < n; ++q) mb.dst[q] = v[q];
+  __threadfence_system();
+  *mb.seq = mb.s;
+}
+
 // up to eight arrays passed by value (no device pointer table needed)
 struct PtrSet8 {
   double* p[8];

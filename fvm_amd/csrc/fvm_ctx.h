@@ -158,6 +158,8 @@ struct ProfScope {
 
 // stage launchers (return FVM_OK or an error code)
 int fvm_fetch(FvmCtx* c, const double* src, int n, double* dst);
+Mailbox fvm_mail(FvmCtx* c);  // a fresh sequence number for a kernel that publishes itself
+int fvm_mail_wait(FvmCtx* c, const Mailbox& mb, int n, double* dst);
 int fvm_geom_setup(FvmCtx* c);
 int fvm_k_set_dtime(FvmCtx* c, double dtime);
 int fvm_af_setup(FvmCtx* c);
@@ -181,7 +183,8 @@ int fvm_k_correct(FvmCtx* c, double sign);
 int fvm_k_psource_correct(FvmCtx* c, double sign, bool with_cfl);
 int fvm_k_correct_bcu_contra(FvmCtx* c);
 int fvm_k_pref(FvmCtx* c, double* pref_host);
-int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready = false);
+int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready = false,
+                   const Mailbox* mb = nullptr);
 int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready = false);
 int fvm_wall2d_setup(FvmCtx* c, const double* xg, const double* yg);
 int fvm_k_sgs(FvmCtx* c, bool sgs_out = true);

@@ -533,7 +533,7 @@ __global__ void k_forces_part(Dims d, const double* __restrict__ wall, const dou
 __global__ void k_forces_final(Dims d, const double* __restrict__ p,
                                const double* __restrict__ fpart, int nfb, double ref_a,
                                double sum_area, const double* __restrict__ cflpart, int ncfl,
-                               double* out) {
+                               double* out, Mailbox mb) {
   __shared__ double sh[256];
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int b = threadIdx.x; b < nfb; b += blockDim.x)
@@ -584,6 +584,12 @@ __global__ void k_forces_final(Dims d, const double* __restrict__ p,
     out[6] = clp;
     out[7] = clv;
     out[8] = sum_area > 0.0 ? -snu / sum_area : 0.0;
+    if (mb.dst) {  // scal[0..16]: out is scal + 8; the rest was written by earlier kernels
+      const double* sc = out - 8;
+      double v[17];
+      for (int q = 0; q < 17; ++q) v[q] = sc[q];
+      publish(mb, v, 17);
+    }
   }
 }
 
@@ -776,7 +782,7 @@ int fvm_k_pref(FvmCtx* c, double* pref_host) {
 }
 
 // cfl_ready: the CFL partial maxima were left in c->cflpart by the projection of this step
-int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready) {
+int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready, const Mailbox* mb) {
   double** m = c->dm;
   ProfScope ps(c, "calccoef");
   const int ncfl = cfl_ready ? c->ncflused : 512;
@@ -791,7 +797,8 @@ int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready) {
   hipLaunchKernelGGL(k_forces_part, dim3(nfb), dim3(256), 0, c->stream, c->d, c->wall, c->p,
                      c->u, c->v, c->w, c->thermal ? c->t : nullptr, c->prm.re, c->red + off);
   hipLaunchKernelGGL(k_forces_final, dim3(1), dim3(256), 0, c->stream, c->d, c->p, c->red + off,
-                     nfb, c->prm.ref_a, c->thermal ? c->sum_area : 0.0, cfl, ncfl, c->scal + 8);
+                     nfb, c->prm.ref_a, c->thermal ? c->sum_area : 0.0, cfl, ncfl, c->scal + 8,
+                     mb ? *mb : Mailbox{nullptr, nullptr, 0});
   HIP_CHECK(hipGetLastError());
   if (out8) {
     HIP_CHECK(hipMemcpyAsync(out8, c->scal + 8, 8 * sizeof(double), hipMemcpyDeviceToHost,

@@ -93,7 +93,7 @@ __global__ void k_residual_compact(Dims d, const double* __restrict__ p,
   }
 }
 
-__global__ void k_res_final(const double* __restrict__ part, long nb, double* out) {
+__global__ void k_res_final(const double* __restrict__ part, long nb, double* out, Mailbox mb) {
   __shared__ double s0[256], s1[256];
   const int t = threadIdx.x;
   double a = 0.0, b = 0.0;
@@ -116,7 +116,11 @@ __global__ void k_res_final(const double* __restrict__ part, long nb, double* ou
     if (t < s) { s0[t] += s0[t + s]; s1[t] += s1[t + s]; }
     __syncthreads();
   }
-  if (t == 0) { out[0] = s0[0]; out[1] = s1[0]; out[2] = s0[0] / s1[0]; }
+  if (t == 0) {
+    out[0] = s0[0]; out[1] = s1[0]; out[2] = s0[0] / s1[0];
+    const double v[3] = {s0[0], s1[0], s0[0] / s1[0]};
+    publish(mb, v, 3);
+  }
 }
 
 // Radial line operator of Fourier mode (mJ, mK): lo x(r-1) + dg x(r) + up x(r+1) with
@@ -501,6 +505,7 @@ int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
   double res = INFINITY, res_prev = INFINITY;
   const int cap = c->prm.nswpp;
   bool use32 = mixed;
+  Mailbox mb{nullptr, nullptr, 0};
   for (;;) {
     int r = fvm_k_bcp(c);
     if (r) return r;
@@ -513,13 +518,15 @@ int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
       else
         hipLaunchKernelGGL(k_residual_compact<double>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
                            c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d64.X, c->red);
-      hipLaunchKernelGGL(k_res_final, dim3(1), dim3(256), 0, c->stream, c->red, nb, c->scal + 4);
+      mb = (outer > 0 || cap < 1) ? fvm_mail(c) : Mailbox{nullptr, nullptr, 0};
+      hipLaunchKernelGGL(k_res_final, dim3(1), dim3(256), 0, c->stream, c->red, nb, c->scal + 4,
+                         mb);
       HIP_CHECK(hipGetLastError());
     }
     // Like ITER_P's sweep loop (iter_p.f90:36-62), at least one correction pass is made before
     // the residual is tested, so the first residual needs no round trip to the host.
     if (outer > 0 || cap < 1) {
-      const int rf = fvm_fetch(c, c->scal + 4, 3, c->hscal + 4);
+      const int rf = fvm_mail_wait(c, mb, 3, c->hscal + 4);
       if (rf) return rf;
       res = c->hscal[6];
       if (!(res == res)) {
