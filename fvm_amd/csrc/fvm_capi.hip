@@ -366,7 +366,9 @@ int fvm_set_field(fvm_ctx* c, int field, const double* host) {
   c->u_is_un = false;
   double* dptr = field_ptr(c, field);
   if (!dptr) return FVM_ERR_ARG;
-  int rs = fvm_k_sgs_out(c);  // form SGS_x,y,z before any of their inputs can change
+  int rs = fvm_k_sgs_out(c);  // form SGS_x,y,z and nu_t before any of their inputs can change
+  if (rs) return rs;
+  rs = fvm_k_nut_out(c);
   if (rs) return rs;
   if (field >= FVM_XIX && field <= FVM_ZTZ) c->xz_zero = false;  // re-derived only at set-up
   std::vector<double> slab;
@@ -393,6 +395,10 @@ int fvm_get_field(fvm_ctx* c, int field, double* host) {
   if (!dptr) return FVM_ERR_ARG;
   if (field >= FVM_SGSX && field <= FVM_SGSZ) {
     const int rs = fvm_k_sgs_out(c);
+    if (rs) return rs;
+  }
+  if (field == FVM_NUT) {
+    const int rs = fvm_k_nut_out(c);
     if (rs) return rs;
   }
   if (c->two_d) dptr += c->d.sk;  // the slab's interior plane K = 1

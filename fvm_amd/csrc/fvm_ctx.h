@@ -61,6 +61,7 @@ struct FvmCtx {
   double* nut = nullptr;
   double* sgs[3] = {};
   bool sgs_stale = false;   // sgs[] not yet formed from the last SGS pass's nu_t and S_ij
+  bool nut_stale = false;   // nut not yet formed from the last SGS pass's C, Delta^2 and |S|
   // AF chunked iteration: two iterate buffers per component, device table of prev/next
   double* afb[3][2] = {};
   double** afptr = nullptr;
@@ -189,6 +190,7 @@ int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready = false);
 int fvm_wall2d_setup(FvmCtx* c, const double* xg, const double* yg);
 int fvm_k_sgs(FvmCtx* c, bool sgs_out = true);
 int fvm_k_sgs_out(FvmCtx* c);
+int fvm_k_nut_out(FvmCtx* c);
 int fvm_k_iter_u(FvmCtx* c);
 int fvm_k_iter_p(FvmCtx* c, double* resip, int* niter);
 int fvm_k_bcp(FvmCtx* c);

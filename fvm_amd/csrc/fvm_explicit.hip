@@ -415,11 +415,22 @@ __global__ void k_correct(Dims d, double sign, double dtime, const double* __res
 // span mean of p at (IE, NJ/4, K), summed in the reference's order by one lane
 // (RLES/main.f90:125-131)
 __global__ void k_pref(Dims d, const double* __restrict__ p, double* out) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double s = 0.0;
-    for (int k = 1; k <= d.nk - 1; ++k) s = s + p[cidx(d, d.ni, d.nj / 4, k)];
-    out[0] = s / (double)(d.nk - 1);
+  // the span values are fetched by the whole workgroup at once, then summed in K order by one
+  // lane (the reference's order)
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (int k0 = 1; k0 <= d.nk - 1; k0 += 256) {
+    const int k = k0 + t;
+    if (k <= d.nk - 1) sh[t] = p[cidx(d, d.ni, d.nj / 4, k)];
+    __syncthreads();
+    if (t == 0) {
+      const int cnt = min(256, d.nk - k0);
+      for (int q = 0; q < cnt; ++q) s = s + sh[q];
+    }
+    __syncthreads();
   }
+  if (t == 0) out[0] = s / (double)(d.nk - 1);
 }
 
 __global__ void k_sub_pref(Dims d, double* p, const double* __restrict__ pref) {
@@ -769,7 +780,7 @@ int fvm_k_correct(FvmCtx* c, double sign) {
 
 int fvm_k_pref(FvmCtx* c, double* pref_host) {
   ProfScope ps(c, "pref");
-  hipLaunchKernelGGL(k_pref, dim3(1), dim3(64), 0, c->stream, c->d, c->p, c->scal + 2);
+  hipLaunchKernelGGL(k_pref, dim3(1), dim3(256), 0, c->stream, c->d, c->p, c->scal + 2);
   hipLaunchKernelGGL(k_sub_pref, grid1(c->d.n, 256), dim3(256), 0, c->stream, c->d, c->p,
                      c->scal + 2);
   HIP_CHECK(hipGetLastError());

@@ -565,7 +565,8 @@ __global__ void k_nut(Dims d, const double* __restrict__ cavg, const double* __r
 // the reference's, in its order.
 template <bool XZ0>
 __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __restrict__ m,
-                                                 const double* __restrict__ nut,
+                                                 const double* __restrict__ filt2,
+                                                 const double* __restrict__ cavg,
                                                  const double* const* __restrict__ L,
                                                  const double* __restrict__ djac, double* nlx,
                                                  double* nly, double* nlz, double* sx, double* sy,
@@ -581,6 +582,7 @@ __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __
   const int e0 = (ty + 1) * SF_W + tx + 1;
   const GP S0 = gp(L[0]), S1 = gp(L[1]), S2 = gp(L[2]);
   const GP S3 = gp(L[3]), S4 = gp(L[4]), S5 = gp(L[5]);
+  const double C = cavg[0];
   double* nl[3] = {nlx, nly, nlz};
   double* so[3] = {sx, sy, sz};
   // symmetric strain components stored as 11,12,13,22,23,33: row/col -> product index
@@ -595,13 +597,18 @@ __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __
       const int gi = min(bb.x * 64 + e % SF_W, d.ni);
       const int gj = min(bb.y * 4 + e / SF_W, d.nj);
       const long c = cidx(d, gi, gj, kp);
-      const double tn = 2.0 * nut[c];
-      sh[slot][0][e] = tn * S0[c];
-      sh[slot][1][e] = tn * S1[c];
-      sh[slot][2][e] = tn * S2[c];
-      sh[slot][3][e] = tn * S3[c];
-      sh[slot][4][e] = tn * S4[c];
-      sh[slot][5][e] = tn * S5[c];
+      const double l11 = S0[c], l12 = S1[c], l13 = S2[c], l22 = S3[c], l23 = S4[c], l33 = S5[c];
+      // nu_t = C Delta^2 |S| as k_nut forms it, with |S| from the strain rates in
+      // strain_store's order (the value it stored)
+      const double s2 = l11 * l11 + l12 * l12 + l13 * l13 + l12 * l12 + l22 * l22 + l23 * l23 +
+                        l13 * l13 + l23 * l23 + l33 * l33;
+      const double tn = 2.0 * (C * filt2[c] * sqrt(2.0 * s2));
+      sh[slot][0][e] = tn * l11;
+      sh[slot][1][e] = tn * l12;
+      sh[slot][2][e] = tn * l13;
+      sh[slot][3][e] = tn * l22;
+      sh[slot][4][e] = tn * l23;
+      sh[slot][5][e] = tn * l33;
     }
     __syncthreads();
     if (kp == k0 - 1) {
@@ -651,14 +658,23 @@ __global__ void __launch_bounds__(256) k_sgs_div(Dims d, const double* const* __
 
 // SGS_x, SGS_y, SGS_z from the step's nu_t and S_ij, which stay in place until the next SGS
 // pass: a step's NL update does not write them, the first read of the fields does
+int fvm_k_nut_out(FvmCtx* c) {
+  if (!c->nut_stale) return FVM_OK;
+  hipLaunchKernelGGL(k_nut, grid1(c->d.n, 256), dim3(256), 0, c->stream, c->d, c->scal + 0,
+                     c->filt2, c->mag, c->nut);
+  HIP_CHECK(hipGetLastError());
+  c->nut_stale = false;
+  return FVM_OK;
+}
+
 int fvm_k_sgs_out(FvmCtx* c) {
   if (!c->sgs_stale) return FVM_OK;
   const Dims d = c->d;
   const dim3 gs((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + SD_KC - 1) / SD_KC);
   auto kern = c->xz_zero ? k_sgs_div<true> : k_sgs_div<false>;
-  hipLaunchKernelGGL(kern, gs, kBlk, 0, c->stream, d, (const double* const*)c->dm, c->nut,
-                     (const double* const*)c->dlss, c->djac, c->nl[0], c->nl[1], c->nl[2],
-                     c->sgs[0], c->sgs[1], c->sgs[2], 2);
+  hipLaunchKernelGGL(kern, gs, kBlk, 0, c->stream, d, (const double* const*)c->dm, c->filt2,
+                     c->scal + 0, (const double* const*)c->dlss, c->djac, c->nl[0], c->nl[1],
+                     c->nl[2], c->sgs[0], c->sgs[1], c->sgs[2], 2);
   HIP_CHECK(hipGetLastError());
   c->sgs_stale = false;
   return FVM_OK;
@@ -707,13 +723,17 @@ int fvm_k_sgs(FvmCtx* c, bool sgs_out) {
                        c->scal + 0);
   }
   const int nb = 512;
-  hipLaunchKernelGGL(k_nut, grid1(d.n, 256), dim3(256), 0, st, d, c->scal + 0, c->filt2, c->mag,
-                     c->nut);
+  // nu_t itself is formed here only for a standalone SGS call; in the step k_sgs_div forms it
+  // from C, Delta^2 and S_ij, and FVM_NUT is filled when read
+  if (sgs_out)
+    hipLaunchKernelGGL(k_nut, grid1(d.n, 256), dim3(256), 0, st, d, c->scal + 0, c->filt2,
+                       c->mag, c->nut);
+  c->nut_stale = !sgs_out;
   {
     ProfScope pk(c, "k_sgs_div");
     const dim3 gs((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + SD_KC - 1) / SD_KC);
     auto kern = c->xz_zero ? k_sgs_div<true> : k_sgs_div<false>;
-    hipLaunchKernelGGL(kern, gs, kBlk, 0, st, d, (const double* const*)m, c->nut,
+    hipLaunchKernelGGL(kern, gs, kBlk, 0, st, d, (const double* const*)m, c->filt2, c->scal + 0,
                        (const double* const*)c->dlss, c->djac, c->nl[0], c->nl[1], c->nl[2],
                        c->sgs[0], c->sgs[1], c->sgs[2], sgs_out ? 3 : 1);
     HIP_CHECK(hipGetLastError());

@@ -85,11 +85,12 @@ def test_converged_steps_match_reference():
         for c in ("u", "v", "w"):
             assert rel(s.get(c), g[f"s{st}_{c}"]) < 1e-10, (st, c)
         assert rel(s.get("p"), g[f"s{st}_pref"]) < 1e-10, (st, "p")
-        # SGS_x,y,z are formed from the step's nu_t and S_ij when first read
+        # SGS_x,y,z and nu_t are formed from the step's C, Delta^2 and S_ij when first read
         if st == 1:
             I = inner(g)
             for c in ("sgsx", "sgsy", "sgsz"):
                 assert rel(s.get(c), g[f"s1_{c}"], I) < 1e-10, c
+            assert rel(s.get("nut"), g["s1_nut"]) < 1e-10
         row = g["scalars"][st - 1]
         assert abs(info["cd"] - row[6]) <= 1e-9 * abs(row[6])
         assert abs(info["cl"] - row[7]) <= 1e-9 * max(abs(row[7]), 1.0)
