@@ -1204,9 +1204,9 @@ void orc3d_calccoef(orc3d *o) {
   o->cl = o->clp + o->clv;
 }
 
-/* One time step of the reference time loop (RLES/main.f90:39-150, file output excluded; the
- * passive temperature when thermal is set). */
-void orc3d_step(orc3d *o, int nswpp, double resimax, int use_les) {
+/* The predictor half of a time step (RLES/main.f90:39-88): the step copies, NL, SGS, DIF,
+ * P_SOURCE and the three AF solves ITER_U, leaving u*, v*, w* in u, v, w. */
+void orc3d_predictor(orc3d *o, int use_les) {
   int ni = o->ni, nj = o->nj, nk = o->nk;
   long n = o->n;
   size_t b = n * sizeof(double);
@@ -1246,6 +1246,14 @@ void orc3d_step(orc3d *o, int nswpp, double resimax, int use_les) {
   o->uin = 0.0;
   orc3d_iter_u(o, o->v, o->vn, o->nly, o->nly1, o->dify, o->s_v);
   orc3d_iter_u(o, o->w, o->wn, o->nlz, o->nlz1, o->difz, o->s_w);
+}
+
+/* One time step of the reference time loop (RLES/main.f90:39-150, file output excluded; the
+ * passive temperature when thermal is set). */
+void orc3d_step(orc3d *o, int nswpp, double resimax, int use_les) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  long n = o->n;
+  orc3d_predictor(o, use_les);
   double dt = o->dtime;
   for (int k = 1; k <= nk - 1; ++k)
     for (int j = 1; j <= nj - 1; ++j)

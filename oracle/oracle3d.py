@@ -33,6 +33,7 @@ def lib():
         L.orc3d_setup.argtypes = [vp, vp, vp, vp]
         L.orc3d_set_state.argtypes = [vp] * 8
         L.orc3d_step.argtypes = [vp, ci, cd, ci]
+        L.orc3d_predictor.argtypes = [vp, ci]
         L.orc3d_set_no_pref.argtypes = [vp, ci]
         for f in ("orc3d_contra_vel", "orc3d_sgs", "orc3d_bcp", "orc3d_lisolv", "orc3d_calccoef"):
             getattr(L, f).argtypes = [vp]
@@ -111,6 +112,10 @@ class Oracle3D:
 
     def set_no_pref(self, on=True):
         self.L.orc3d_set_no_pref(self.h, 1 if on else 0)
+
+    def predictor(self, use_les=True):
+        """The step up to and including the three ITER_U solves (u*, v*, w* left in u, v, w)."""
+        self.L.orc3d_predictor(self.h, 1 if use_les else 0)
 
     def step(self, nswpp=20, resimax=1e-4, use_les=True):
         self.L.orc3d_step(self.h, int(nswpp), float(resimax), 1 if use_les else 0)

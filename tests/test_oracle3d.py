@@ -63,6 +63,17 @@ def test_steps_bitexact(name):
         assert o.scalar("cdp") == row[8] and o.scalar("cdv") == row[9]
 
 
+def test_predictor_bitexact():
+    """The predictor half of step 1 leaves the reference's u*, v*, w* (ITER_U outputs)."""
+    g = load_golden("rles_13x17x9_les_nswpp20")
+    o = make(g)
+    o.predictor(bool(g["use_les"]))
+    ni, nj, nk = int(g["ni"]), int(g["nj"]), int(g["nk"])
+    inner = (slice(1, ni), slice(1, nj), slice(1, nk))
+    for t, a in (("ustar", "u"), ("vstar", "v"), ("wstar", "w")):
+        assert np.array_equal(o.array(a)[inner], g["s1_" + t][inner]), t
+
+
 def cartesian(ni, nj, nk, h=(0.3, 0.7, 0.45)):
     i, j, k = np.meshgrid(np.arange(ni), np.arange(nj), np.arange(nk), indexing="ij")
     return (np.asfortranarray(1.0 + h[0] * i), np.asfortranarray(2.0 + h[1] * j),
