@@ -45,6 +45,13 @@ typedef struct orc3d {
   /* field output (WRITEDATA): cell |omega|, lambda_2; node averages at IX(I,J,K), I=1..NI */
   double *vorm, *lam2, *pg, *ug, *vg, *wg, *tg, *vorg, *lamg;
   double *yplus; /* wall-cell y+ of CALCCOEF on the I = 1 plane (calccoef.f90:121, 159) */
+  /* time averages of POST_PROCESS (post_process.f90:9-69, calccoef.f90:286-307): running sums
+   * of u v w p T, of uu uv uw vv vw ww and of the SGS stresses -2 nu_t S_ij (11 12 13 22 23 33),
+   * then the node averages of the finalization at IX(I,J,K), I = 1..NI (p u v w T of the means;
+   * u v w rms; uu uv uw vv vw ww Reynolds stresses) and the wall-plane Cf, Cp, Nu */
+  double *pavg[5], *pmom[6], *psgs[6], *pnode[14], *cf_s, *cp_s, *nu_s;
+  long niter_pos;
+  int ipost;
   double c_avg, c_t, pe, nu, sum_area;
   /* scalars of the last step */
   double uin, resip, pref, cfl, cpb, cd, cl, cdp, cdv, clp, clv;
@@ -79,14 +86,15 @@ orc3d *orc3d_create(int ni, int nj, int nk, double re, double dtime, double ref_
                     &o->vorm, &o->lam2, &o->pg, &o->ug, &o->vg, &o->wg, &o->tg, &o->vorg, &o->yplus,
                     &o->lamg};
   for (size_t a = 0; a < sizeof(all) / sizeof(all[0]); ++a) *all[a] = alloc0(o->n);
+  for (double **q = &o->pavg[0]; q <= &o->nu_s; ++q) *q = alloc0(o->n);
   return o;
 }
 
 void orc3d_destroy(orc3d *o) {
   if (!o) return;
   double **p = &o->xg;
-  /* every double* member from xg to yplus is an allocation */
-  double **end = &o->yplus;
+  /* every double* member from xg to nu_s is an allocation */
+  double **end = &o->nu_s;
   for (; p <= end; ++p) free(*p);
   free(o);
 }
@@ -109,6 +117,15 @@ double *orc3d_array(orc3d *o, const char *name) {
   N("adv1", adv1) N("dift", dift) N("sgst", sgst) N("that", th) N("dtdx", dtdx)
   N("dtdy", dtdy) N("dtdz", dtdz) N("kappa", kappa) N("vorm", vorm) N("lamda2", lam2)
   N("pg", pg) N("ug", ug) N("vg", vg) N("wg", wg) N("tg", tg) N("vorg", vorg) N("lamdag", lamg) N("yplus", yplus)
+  N("sum_u", pavg[0]) N("sum_v", pavg[1]) N("sum_w", pavg[2]) N("sum_p", pavg[3])
+  N("sum_t", pavg[4]) N("sum_uu", pmom[0]) N("sum_uv", pmom[1]) N("sum_uw", pmom[2])
+  N("sum_vv", pmom[3]) N("sum_vw", pmom[4]) N("sum_ww", pmom[5]) N("sum_sgs11", psgs[0])
+  N("sum_sgs12", psgs[1]) N("sum_sgs13", psgs[2]) N("sum_sgs22", psgs[3])
+  N("sum_sgs23", psgs[4]) N("sum_sgs33", psgs[5]) N("avg_pg", pnode[0]) N("avg_ug", pnode[1])
+  N("avg_vg", pnode[2]) N("avg_wg", pnode[3]) N("avg_tg", pnode[4]) N("rms_ug", pnode[5])
+  N("rms_vg", pnode[6]) N("rms_wg", pnode[7]) N("rs_uu", pnode[8]) N("rs_uv", pnode[9])
+  N("rs_uw", pnode[10]) N("rs_vv", pnode[11]) N("rs_vw", pnode[12]) N("rs_ww", pnode[13])
+  N("cf_s", cf_s) N("cp_s", cp_s) N("nu_s", nu_s)
 #undef N
   return NULL;
 }
@@ -120,10 +137,13 @@ void orc3d_set_temperature(orc3d *o, const double *t, const double *adv) {
   memcpy(o->adv, adv, o->n * sizeof(double));
 }
 void orc3d_set_pe(orc3d *o, double pe) { o->pe = pe; }
+/* DTIME is a run-time module variable (POST_PROCESS sets it to DTIME_POST, post_process.f90:59) */
+void orc3d_set_dtime(orc3d *o, double dtime) { o->dtime = dtime; }
 
 double orc3d_scalar(orc3d *o, const char *name) {
   if (!strcmp(name, "resip")) return o->resip;
   if (!strcmp(name, "niter")) return (double)o->niter;
+  if (!strcmp(name, "niter_pos")) return (double)o->niter_pos;
   if (!strcmp(name, "pref")) return o->pref;
   if (!strcmp(name, "cfl")) return o->cfl;
   if (!strcmp(name, "cpb")) return o->cpb;
@@ -1108,6 +1128,64 @@ void orc3d_sgs(orc3d *o) {
         }
   if (o->thermal) sgs_thermal(o);
 }
+/* The accumulation of CALCCOEF while IPOST = 1 (calccoef.f90:286-307), on every cell,
+ * ghosts included: u v w p T, the products u_i u_j and the SGS stresses -2 nu_t S_ij of this
+ * step's EDDY_VISCOS and LSS_ij (zero without the LES model). */
+static void post_accumulate(orc3d *o) {
+  long n = o->n;
+  o->niter_pos = o->niter_pos + 1;
+  const double *f[5] = {o->u, o->v, o->w, o->p, o->t};
+  for (int q = 0; q < 5; ++q)
+    for (long c = 0; c < n; ++c) o->pavg[q][c] = o->pavg[q][c] + f[q][c];
+  const double *uvw[3] = {o->u, o->v, o->w};
+  const int a[6] = {0, 0, 0, 1, 1, 2}, b[6] = {0, 1, 2, 1, 2, 2};
+  for (int q = 0; q < 6; ++q)
+    for (long c = 0; c < n; ++c) o->pmom[q][c] = o->pmom[q][c] + uvw[a[q]][c] * uvw[b[q]][c];
+  for (int q = 0; q < 6; ++q)
+    for (long c = 0; c < n; ++c) o->psgs[q][c] = o->psgs[q][c] - 2.0 * o->nut[c] * o->lss[q][c];
+}
+
+/* Plane a x + b y + c z + d = 0 through one of the two triangles of wall face (1, J, K): the
+ * first (nodes (J,K), (J+1,K), (J,K+1)) or the second (nodes (J+1,K+1), (J,K+1), (J,K)) of
+ * calccoef.f90:47-60, 86-99 and post_process.f90:77-85, 103-110; d before its sign flip. */
+static void wall_plane(const orc3d *o, int tri, int j, int k, double *A, double *B,
+                       double *Cc, double *D) {
+  const int i = 1;
+#define XG(i, j, k) o->xg[IX(o, i, j, k)]
+#define YG(i, j, k) o->yg[IX(o, i, j, k)]
+#define ZG(i, j, k) o->zg[IX(o, i, j, k)]
+  if (tri == 0) {
+    *A = YG(i, j, k) * (ZG(i, j + 1, k) - ZG(i, j, k + 1)) +
+        YG(i, j + 1, k) * (ZG(i, j, k + 1) - ZG(i, j, k)) +
+        YG(i, j, k + 1) * (ZG(i, j, k) - ZG(i, j + 1, k));
+    *B = ZG(i, j, k) * (XG(i, j + 1, k) - XG(i, j, k + 1)) +
+        ZG(i, j + 1, k) * (XG(i, j, k + 1) - XG(i, j, k)) +
+        ZG(i, j, k + 1) * (XG(i, j, k) - XG(i, j + 1, k));
+    *Cc = XG(i, j, k) * (YG(i, j + 1, k) - YG(i, j, k + 1)) +
+         XG(i, j + 1, k) * (YG(i, j, k + 1) - YG(i, j, k)) +
+         XG(i, j, k + 1) * (YG(i, j, k) - YG(i, j + 1, k));
+    *D = XG(i, j, k) * (YG(i, j + 1, k) * ZG(i, j, k + 1) - YG(i, j, k + 1) * ZG(i, j + 1, k)) +
+        XG(i, j + 1, k) * (YG(i, j, k + 1) * ZG(i, j, k) - YG(i, j, k) * ZG(i, j, k + 1)) +
+        XG(i, j, k + 1) * (YG(i, j, k) * ZG(i, j + 1, k) - YG(i, j + 1, k) * ZG(i, j, k));
+  } else {
+    *A = YG(i, j + 1, k + 1) * (ZG(i, j, k + 1) - ZG(i, j, k)) +
+        YG(i, j, k + 1) * (ZG(i, j, k) - ZG(i, j + 1, k + 1)) +
+        YG(i, j, k) * (ZG(i, j + 1, k + 1) - ZG(i, j, k + 1));
+    *B = ZG(i, j + 1, k + 1) * (XG(i, j, k + 1) - XG(i, j, k)) +
+        ZG(i, j, k + 1) * (XG(i, j, k) - XG(i, j + 1, k + 1)) +
+        ZG(i, j, k) * (XG(i, j + 1, k + 1) - XG(i, j, k + 1));
+    *Cc = XG(i, j + 1, k + 1) * (YG(i, j, k + 1) - YG(i, j, k)) +
+         XG(i, j, k + 1) * (YG(i, j, k) - YG(i, j + 1, k + 1)) +
+         XG(i, j, k) * (YG(i, j + 1, k + 1) - YG(i, j, k + 1));
+    *D = XG(i, j + 1, k + 1) * (YG(i, j, k + 1) * ZG(i, j, k) - YG(i, j, k) * ZG(i, j, k + 1)) +
+        XG(i, j, k + 1) * (YG(i, j, k) * ZG(i, j + 1, k + 1) - YG(i, j + 1, k + 1) * ZG(i, j, k)) +
+        XG(i, j, k) * (YG(i, j + 1, k + 1) * ZG(i, j, k + 1) - YG(i, j, k + 1) * ZG(i, j + 1, k + 1));
+  }
+#undef XG
+#undef YG
+#undef ZG
+}
+
 void orc3d_calccoef(orc3d *o) {
   int ni = o->ni, nj = o->nj, nk = o->nk;
   (void)ni;
@@ -1140,40 +1218,8 @@ void orc3d_calccoef(orc3d *o) {
     for (int k = 1; k <= nk - 1; ++k)
       for (int j = 1; j <= nj - 1; ++j) {
         int i = 1;
-#define XG(i, j, k) o->xg[IX(o, i, j, k)]
-#define YG(i, j, k) o->yg[IX(o, i, j, k)]
-#define ZG(i, j, k) o->zg[IX(o, i, j, k)]
         double A, B, Cc, D;
-        if (tri == 0) {
-          A = YG(i, j, k) * (ZG(i, j + 1, k) - ZG(i, j, k + 1)) +
-              YG(i, j + 1, k) * (ZG(i, j, k + 1) - ZG(i, j, k)) +
-              YG(i, j, k + 1) * (ZG(i, j, k) - ZG(i, j + 1, k));
-          B = ZG(i, j, k) * (XG(i, j + 1, k) - XG(i, j, k + 1)) +
-              ZG(i, j + 1, k) * (XG(i, j, k + 1) - XG(i, j, k)) +
-              ZG(i, j, k + 1) * (XG(i, j, k) - XG(i, j + 1, k));
-          Cc = XG(i, j, k) * (YG(i, j + 1, k) - YG(i, j, k + 1)) +
-               XG(i, j + 1, k) * (YG(i, j, k + 1) - YG(i, j, k)) +
-               XG(i, j, k + 1) * (YG(i, j, k) - YG(i, j + 1, k));
-          D = XG(i, j, k) * (YG(i, j + 1, k) * ZG(i, j, k + 1) - YG(i, j, k + 1) * ZG(i, j + 1, k)) +
-              XG(i, j + 1, k) * (YG(i, j, k + 1) * ZG(i, j, k) - YG(i, j, k) * ZG(i, j, k + 1)) +
-              XG(i, j, k + 1) * (YG(i, j, k) * ZG(i, j + 1, k) - YG(i, j + 1, k) * ZG(i, j, k));
-        } else {
-          A = YG(i, j + 1, k + 1) * (ZG(i, j, k + 1) - ZG(i, j, k)) +
-              YG(i, j, k + 1) * (ZG(i, j, k) - ZG(i, j + 1, k + 1)) +
-              YG(i, j, k) * (ZG(i, j + 1, k + 1) - ZG(i, j, k + 1));
-          B = ZG(i, j + 1, k + 1) * (XG(i, j, k + 1) - XG(i, j, k)) +
-              ZG(i, j, k + 1) * (XG(i, j, k) - XG(i, j + 1, k + 1)) +
-              ZG(i, j, k) * (XG(i, j + 1, k + 1) - XG(i, j, k + 1));
-          Cc = XG(i, j + 1, k + 1) * (YG(i, j, k + 1) - YG(i, j, k)) +
-               XG(i, j, k + 1) * (YG(i, j, k) - YG(i, j + 1, k + 1)) +
-               XG(i, j, k) * (YG(i, j + 1, k + 1) - YG(i, j, k + 1));
-          D = XG(i, j + 1, k + 1) * (YG(i, j, k + 1) * ZG(i, j, k) - YG(i, j, k) * ZG(i, j, k + 1)) +
-              XG(i, j, k + 1) * (YG(i, j, k) * ZG(i, j + 1, k + 1) - YG(i, j + 1, k + 1) * ZG(i, j, k)) +
-              XG(i, j, k) * (YG(i, j + 1, k + 1) * ZG(i, j, k + 1) - YG(i, j, k + 1) * ZG(i, j + 1, k + 1));
-        }
-#undef XG
-#undef YG
-#undef ZG
+        wall_plane(o, tri, j, k, &A, &B, &Cc, &D);
         D = -1.0 * D;
         double dum = sqrt(A * A + B * B + Cc * Cc);
         long c = IX(o, i, j, k);
@@ -1202,6 +1248,101 @@ void orc3d_calccoef(orc3d *o) {
   o->clv = tly / o->ref_a;
   o->cd = o->cdp + o->cdv;
   o->cl = o->clp + o->clv;
+  if (o->ipost) post_accumulate(o);
+}
+
+/* POST_PROCESS with IPOST = 0 (post_process.f90:9-61): zero the running sums and start the
+ * accumulation; the caller switches DTIME to DTIME_POST (post_process.f90:59). */
+void orc3d_post_begin(orc3d *o) {
+  for (double **q = &o->pavg[0]; q <= &o->psgs[5]; ++q) memset(*q, 0, o->n * sizeof(double));
+  o->niter_pos = 0;
+  o->ipost = 1;
+}
+
+/* 0.125 x the eight cells around node (I,J,K) in post_process.f90's order (:125-134) */
+static void post_nodal(const orc3d *o, const double *P, double *dst) {
+  long sj = o->sj, sk = o->sk;
+  for (int k = 1; k <= o->nk; ++k)
+    for (int j = 1; j <= o->nj; ++j)
+      for (int i = 1; i <= o->ni; ++i) {
+        long c = IX(o, i, j, k);
+        dst[c] = 0.125 * (P[c] + P[c - sj] + P[c - 1] + P[c - 1 - sj] + P[c - sk] +
+                          P[c - sj - sk] + P[c - 1 - sk] + P[c - 1 - sj - sk]);
+      }
+}
+
+/* POST_PROCESS with IPOST = 1 (post_process.f90:62-276): the means (in place), the wall
+ * friction, pressure and Nusselt coefficients from the means, the node averages of the means
+ * (Avrg000.PLT), of the rms sqrt(<u_i u_i>) (Urms000.PLT) and of the resolved + SGS Reynolds
+ * stresses <u_i u_j> - <u_i><u_j> + <tau_ij> (Strs000.PLT, left in the moment sums), and the
+ * scaled surface coefficients (Surf000.PLT). */
+void orc3d_post_finalize(orc3d *o) {
+  int nj = o->nj, nk = o->nk;
+  long sj = o->sj, sk = o->sk, n = o->n;
+  const double np = (double)o->niter_pos;
+  for (int q = 0; q < 5; ++q)
+    for (long c = 0; c < n; ++c) o->pavg[q][c] = o->pavg[q][c] / np;
+  const double *UA = o->pavg[0], *VA = o->pavg[1], *WA = o->pavg[2], *PA = o->pavg[3],
+               *TA = o->pavg[4];
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j) {
+      long c = IX(o, 1, j, k);
+      double sh = 0.0, nu = 0.0;
+      for (int tri = 0; tri < 2; ++tri) {
+        double A, B, Cc, D;
+        wall_plane(o, tri, j, k, &A, &B, &Cc, &D);
+        D = -1.0 * D;
+        double dum = sqrt(A * A + B * B + Cc * Cc);
+        double dn = fabs(A * o->x[c] + B * o->y[c] + Cc * o->z[c] + D) / dum;
+        double U = UA[c], V = VA[c], W = WA[c];
+        double t1 = V * Cc / dum - W * B / dum, t2 = U * Cc / dum - W * A / dum,
+               t3 = U * B / dum - V * A / dum;
+        double dudn = sqrt(t1 * t1 + t2 * t2 + t3 * t3);
+        dudn = dudn / dn;
+        double dtdn = (TA[c] - 1.0) / dn; /* T_WALL = 1 (comdat_param.f90:40) */
+        double ar = tri == 0 ? o->area1[c] : o->area2[c];
+        double tot = o->area1[c] + o->area2[c];
+        if (tri == 0) {
+          sh = dudn * ar / tot;
+          nu = dtdn * ar / tot;
+          o->cp_s[c] = 0.125 * (PA[c] + PA[c - sj] + PA[c - 1] + PA[c - 1 - sj] + PA[c + sk] +
+                                PA[c - sj + sk] + PA[c - 1 + sk] + PA[c - 1 - sj + sk]);
+        } else {
+          sh = sh + dudn * ar / tot;
+          nu = nu + dtdn * ar / tot;
+        }
+      }
+      o->cf_s[c] = sh;
+      o->nu_s[c] = nu;
+    }
+  const int pf[5] = {3, 0, 1, 2, 4}; /* PG UG VG WG TG from P_AVG U_AVG V_AVG W_AVG T_AVG */
+  for (int q = 0; q < 5; ++q) post_nodal(o, o->pavg[pf[q]], o->pnode[q]);
+  for (int q = 0; q < 6; ++q)
+    for (long c = 0; c < n; ++c) {
+      o->pmom[q][c] = o->pmom[q][c] / np;
+      o->psgs[q][c] = o->psgs[q][c] / np;
+    }
+  double *rms = alloc0(n);
+  const int dg[3] = {0, 3, 5}; /* uu, vv, ww */
+  for (int q = 0; q < 3; ++q) {
+    for (long c = 0; c < n; ++c) rms[c] = sqrt(o->pmom[dg[q]][c]);
+    post_nodal(o, rms, o->pnode[5 + q]);
+  }
+  free(rms);
+  const int a[6] = {0, 0, 0, 1, 1, 2}, b[6] = {0, 1, 2, 1, 2, 2};
+  for (int q = 0; q < 6; ++q)
+    for (long c = 0; c < n; ++c)
+      o->pmom[q][c] = o->pmom[q][c] - o->pavg[a[q]][c] * o->pavg[b[q]][c] + o->psgs[q][c];
+  for (int q = 0; q < 6; ++q) post_nodal(o, o->pmom[q], o->pnode[8 + q]);
+  const double f = 2.0 / sqrt(o->re);
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j) {
+      long c = IX(o, 1, j, k);
+      o->cf_s[c] = f * o->cf_s[c];
+      o->cp_s[c] = 2.0 * o->cp_s[c];
+      o->nu_s[c] = -o->nu_s[c];
+    }
+  o->ipost = 0;
 }
 
 /* The predictor half of a time step (RLES/main.f90:39-88): the step copies, NL, SGS, DIF,

@@ -34,6 +34,9 @@ def lib():
         L.orc3d_set_state.argtypes = [vp] * 8
         L.orc3d_step.argtypes = [vp, ci, cd, ci]
         L.orc3d_predictor.argtypes = [vp, ci]
+        L.orc3d_post_begin.argtypes = [vp]
+        L.orc3d_post_finalize.argtypes = [vp]
+        L.orc3d_set_dtime.argtypes = [vp, cd]
         L.orc3d_set_no_pref.argtypes = [vp, ci]
         for f in ("orc3d_contra_vel", "orc3d_sgs", "orc3d_bcp", "orc3d_lisolv", "orc3d_calccoef"):
             getattr(L, f).argtypes = [vp]
@@ -116,6 +119,17 @@ class Oracle3D:
     def predictor(self, use_les=True):
         """The step up to and including the three ITER_U solves (u*, v*, w* left in u, v, w)."""
         self.L.orc3d_predictor(self.h, 1 if use_les else 0)
+
+    def post_begin(self):
+        """POST_PROCESS's start: zero the time-average sums; CALCCOEF then accumulates."""
+        self.L.orc3d_post_begin(self.h)
+
+    def post_finalize(self):
+        """POST_PROCESS's end: means, rms, Reynolds stresses, node averages, Cf / Cp / Nu."""
+        self.L.orc3d_post_finalize(self.h)
+
+    def set_dtime(self, dtime):
+        self.L.orc3d_set_dtime(self.h, float(dtime))
 
     def step(self, nswpp=20, resimax=1e-4, use_les=True):
         self.L.orc3d_step(self.h, int(nswpp), float(resimax), 1 if use_les else 0)

@@ -7,17 +7,26 @@
 !  module arrays in raw binary (stream access, column-major, ghosts included) so the
 !  project's CPU restatement and GPU kernels can be compared with the reference stage by stage.
 !
-!  usage: harness3d NSTEP   (run directory holds NNNxNNNxNNN.dat and RESTART000.DAT)
+!  usage: harness3d NSTEP [NPOST]   (run directory holds NNNxNNNxNNN.dat and RESTART000.DAT)
+!
+!  With NPOST > 0 the last NPOST steps run in the time-averaging phase of POST_PROCESS (its
+!  first call before step NSTEP-NPOST+1: sums zeroed, DTIME = DTIME_POST; CALCCOEF accumulates;
+!  its second call after the last step writes Avrg000/Urms000/Strs000/Surf000.PLT).  The sums
+!  are dumped before that second call and its in-place results after it; WRITEDATA is skipped
+!  in this mode (post_process.f90:259 scales THETA at the loop indices one past the array).
 PROGRAM HARNESS3D
   USE COMDAT_SHARED
   IMPLICIT NONE
   INCLUDE 'mpif.h'
-  INTEGER :: NSTEP, ISTEP
+  INTEGER :: NSTEP, ISTEP, NPOST, IST
   DOUBLE PRECISION :: PREF
   CHARACTER(64) :: ARG
 
   CALL GET_COMMAND_ARGUMENT(1, ARG)
   READ(ARG, *) NSTEP
+  NPOST = 0
+  CALL GET_COMMAND_ARGUMENT(2, ARG, STATUS=IST)
+  IF (IST .EQ. 0 .AND. LEN_TRIM(ARG) .GT. 0) READ(ARG, *) NPOST
 
   ! files the reference appends to with STATUS='OLD'
   OPEN(UNIT=11, FILE='NU_CD_CL.PLT', STATUS='REPLACE'); CLOSE(11)
@@ -41,6 +50,7 @@ PROGRAM HARNESS3D
   WRITE(61, '(A,2ES26.17E3)') 'area ', SUM(AREA1), SUM(AREA2)
 
   DO ISTEP = 1, NSTEP
+    IF (NPOST .GT. 0 .AND. ISTEP .EQ. NSTEP - NPOST + 1) CALL POST_PROCESS
     NITER_TIME = NITER_TIME + 1
     RTIME = RTIME + DTIME
     U_N(IS:IE,JS:JE,KS:KE) = U(IS:IE,JS:JE,KS:KE)
@@ -145,6 +155,39 @@ PROGRAM HARNESS3D
          CD, CL, CD_P, CD_V, NU
   ENDDO
   CLOSE(61)
+
+  IF (NPOST .GT. 0) THEN
+    CALL DUMP('post_sum_u', U_AVG); CALL DUMP('post_sum_v', V_AVG)
+    CALL DUMP('post_sum_w', W_AVG); CALL DUMP('post_sum_p', P_AVG)
+    CALL DUMP('post_sum_t', T_AVG)
+    CALL DUMP('post_sum_uu', UU_AVG); CALL DUMP('post_sum_uv', UV_AVG)
+    CALL DUMP('post_sum_uw', UW_AVG); CALL DUMP('post_sum_vv', VV_AVG)
+    CALL DUMP('post_sum_vw', VW_AVG); CALL DUMP('post_sum_ww', WW_AVG)
+    CALL DUMP('post_sum_sgs11', SGS11); CALL DUMP('post_sum_sgs12', SGS12)
+    CALL DUMP('post_sum_sgs13', SGS13); CALL DUMP('post_sum_sgs22', SGS22)
+    CALL DUMP('post_sum_sgs23', SGS23); CALL DUMP('post_sum_sgs33', SGS33)
+    OPEN(UNIT=63, FILE='post_niter.txt', STATUS='REPLACE'); WRITE(63, *) NITER_POS; CLOSE(63)
+    CALL POST_PROCESS
+    ! in-place results: the means, and the Reynolds stresses left in UU_AVG..WW_AVG
+    CALL DUMP('post_fin_u', U_AVG); CALL DUMP('post_fin_v', V_AVG)
+    CALL DUMP('post_fin_w', W_AVG); CALL DUMP('post_fin_p', P_AVG)
+    CALL DUMP('post_fin_t', T_AVG)
+    CALL DUMP('post_fin_uu', UU_AVG); CALL DUMP('post_fin_uv', UV_AVG)
+    CALL DUMP('post_fin_uw', UW_AVG); CALL DUMP('post_fin_vv', VV_AVG)
+    CALL DUMP('post_fin_vw', VW_AVG); CALL DUMP('post_fin_ww', WW_AVG)
+    ! node averages of the Strs000.PLT pass (uu vv ww in UG VG WG, uv uw vw in U..W_RMS) and
+    ! of the means of p and T (Avrg000.PLT); the other node averages are in the .PLT files
+    CALL DUMPN('post_node_uu', UG); CALL DUMPN('post_node_vv', VG)
+    CALL DUMPN('post_node_ww', WG); CALL DUMPN('post_node_pg', PG)
+    CALL DUMPN('post_node_tg', TG)
+    CALL DUMP('post_cell_uv', U_RMS); CALL DUMP('post_cell_uw', V_RMS)
+    CALL DUMP('post_cell_vw', W_RMS)
+    OPEN(UNIT=62, FILE='post_surf.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(62) SH_S, CP_S, NU_S
+    CLOSE(62)
+    CALL MPI_FINALIZE(ERR)
+    STOP
+  ENDIF
 
   ! field output of the last step (WRITEDATA, writedata.f90:8-189: CALC_VOR, lambda_2 by
   ! FIND_ROOT, nodal averages, Tecplot files).  VOR_M's I ghost planes are never assigned by the

@@ -48,6 +48,13 @@ CASES = {
                                     keep_tags=("u", "v", "w", "pref", "nut", "adv0", "adv",
                                                "sgst", "kappa", "that", "dtdx", "dtdy", "dtdz",
                                                "dift", "t")),
+    # the converged 8 x 20 x 4 thermal case run 1 step at dt, then 3 steps in POST_PROCESS's
+    # averaging phase (DTIME_POST = 1e-3) and its final pass: means, rms, Reynolds stresses
+    # with the SGS part, node averages, wall Cf / Cp / Nu
+    "rles_9x21x5_les_post": dict(ni=9, nj=21, nk=5, h1=0.08, r_out=4.0, sigma=0.05,
+                                 re=200.0, dt=4e-3, nswpp=200000, resimax="1.0D-13",
+                                 les=1, steps=4, npost=3, keep_steps=(4,), t_sigma=0.05,
+                                 keep_tags=("u", "v", "w", "pref", "t", "nut")),
     # (no USE_LES=0 fixture: the reference's laminar mode writes YPLUS in CALCCOEF
     # (calccoef.f90:121), an array INIT allocates only when USE_LES=1 (init.f90:79-81))
 }
@@ -67,6 +74,47 @@ def build(label, c):
                         str(c["nswpp"]), c["resimax"], str(c["les"]), "3.14159265358979D0"],
                        check=True)
     return exe
+
+
+def read_plt(path, ncol):
+    """The node records of one reference Tecplot file (three header lines, D16.9 columns)."""
+    rows = [l.split() for l in open(path).readlines()[3:] if l.strip()]
+    return np.array([[float(x.replace("D", "e")) for x in r] for r in rows]).reshape(-1, ncol)
+
+
+def post_fixture(out, d, ni, nj, nk, npost):
+    """Dumps and Tecplot files of POST_PROCESS (post_process.f90:62-276) after npost steps."""
+    shape = (ni + 1, nj + 1, nk + 1)
+    rd = lambda t: np.fromfile(os.path.join(d, t + ".bin")).reshape(shape, order="F")
+    out["npost"], out["dt_post"] = npost, 1e-3
+    out["niter_pos"] = int(open(os.path.join(d, "post_niter.txt")).read().split()[0])
+    for q in ("u", "v", "w", "p", "t", "uu", "uv", "uw", "vv", "vw", "ww", "sgs11", "sgs12",
+              "sgs13", "sgs22", "sgs23", "sgs33"):
+        out["post_sum_" + q] = rd("post_sum_" + q)
+    for q in ("u", "v", "w", "p", "t", "uu", "uv", "uw", "vv", "vw", "ww"):
+        out["post_fin_" + q] = rd("post_fin_" + q)
+    node = lambda t: np.fromfile(os.path.join(d, t + ".bin")).reshape((ni, nj, nk), order="F")
+    for q in ("uu", "vv", "ww", "pg", "tg"):
+        out["post_node_" + q] = node("post_node_" + q)
+    for q in ("uv", "uw", "vw"):  # node values held in cell arrays at I, J, K = 1..N
+        out["post_node_" + q] = rd("post_cell_" + q)[1:, 1:, 1:]
+    m = (nj - 1) * (nk - 1)
+    surf = np.fromfile(os.path.join(d, "post_surf.bin")).reshape(3, m)
+    for q, a in zip(("cf", "cp", "nu"), surf):
+        out["post_surf_" + q] = a.reshape((nj - 1, nk - 1), order="F")
+    # the text files: x y z + fields per node (I fastest), 9 digits after the point
+    for f, cols in (("Avrg000.PLT", ("pg", "ug", "vg", "wg", "tg")),
+                    ("Urms000.PLT", ("urms", "vrms", "wrms")),
+                    ("Strs000.PLT", ("uu", "uv", "uw", "vv", "vw", "ww"))):
+        a = read_plt(os.path.join(d, f), 3 + len(cols))
+        for q, col in zip(cols, a[:, 3:].T):
+            out["plt_" + q] = col.reshape((ni, nj, nk), order="F")
+    a = read_plt(os.path.join(d, "Surf000.PLT"), 7)
+    for q, col in zip(("theta", "cf", "cp", "nu"), a[:, 3:].T):
+        out["plt_surf_" + q] = col.reshape((nj - 1, nk - 1), order="F")
+    out["plt_headers"] = np.array([open(os.path.join(d, f)).read().split("\n", 3)[:3]
+                                   for f in ("Avrg000.PLT", "Urms000.PLT", "Strs000.PLT",
+                                             "Surf000.PLT")])
 
 
 def run_case(label, c):
@@ -90,7 +138,8 @@ def run_case(label, c):
             out["in_" + k] = rs[k]
         if c.get("t_sigma"):
             out["in_t"], out["in_adv"], out["pe"] = rs["t"], rs["adv"], 2769.0
-        subprocess.run([exe, str(c["steps"])], cwd=d, check=True, stdout=subprocess.DEVNULL)
+        args = [exe, str(c["steps"])] + ([str(c["npost"])] if c.get("npost") else [])
+        subprocess.run(args, cwd=d, check=True, stdout=subprocess.DEVNULL)
         rd = lambda t: np.fromfile(os.path.join(d, t + ".bin")).reshape(shape, order="F")
         setup_tags = SETUP_TAGS if ("keep_tags" not in c or c.get("t_sigma")) else ()
         for t in setup_tags:
@@ -103,7 +152,9 @@ def run_case(label, c):
         rows = [l.split() for l in open(os.path.join(d, "scalars.txt")) if l.startswith("step")]
         # columns: step, niter, resip, pref, cfl, cpb, cd, cl, cd_p, cd_v[, nu]
         out["scalars"] = np.array([[float(x) for x in r[1:]] for r in rows])
-        if c.get("t_sigma"):
+        if c.get("npost"):
+            post_fixture(out, d, ni, nj, nk, c["npost"])
+        elif c.get("t_sigma"):
             # field output of the last step (WRITEDATA): lambda_2, |omega| on the cells, the
             # nodal averages on nodes 1..N
             for t in ("out_lamda2", "out_vorm", "out_dum3d"):

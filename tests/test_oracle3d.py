@@ -298,3 +298,82 @@ def test_lambda2_closed_forms():
     I = (slice(2, ni - 1), slice(1, nj), slice(1, nk))
     assert np.abs(o.array("lamda2")[I] - lam[1]).max() < 1e-9
     assert np.abs(o.array("vorm")[I] - np.linalg.norm(curl)).max() < 1e-9
+
+
+POST_SUMS = ("u", "v", "w", "p", "t", "uu", "uv", "uw", "vv", "vw", "ww", "sgs11", "sgs12",
+             "sgs13", "sgs22", "sgs23", "sgs33")
+
+
+def run_post(g):
+    """Step 1 at DTIME, then POST_PROCESS's averaging phase at DTIME_POST for npost steps."""
+    o = make(g)
+    o.set_temperature(g["in_t"], g["in_adv"], float(g["pe"]))
+    steps, npost = int(g["scalars"].shape[0]), int(g["npost"])
+    for s in range(1, steps + 1):
+        if s == steps - npost + 1:
+            o.post_begin()
+            o.set_dtime(float(g["dt_post"]))
+        o.step(int(g["nswpp"]), float(g["resimax"]), True)
+    return o
+
+
+def test_post_process_bitexact():
+    """POST_PROCESS (post_process.f90:9-276) with the accumulation of CALCCOEF
+    (calccoef.f90:286-307): the running sums after three averaging steps, the in-place means
+    and Reynolds stresses, the node averages and the wall Cf / Cp / Nu equal the reference's
+    arrays bit for bit, and its Tecplot files to their 9 printed digits."""
+    g = load_golden("rles_9x21x5_les_post")
+    o = run_post(g)
+    for t in ("u", "v", "w", "pref", "t", "nut"):
+        assert np.array_equal(o.array("p" if t == "pref" else t), g["s4_" + t]), t
+    assert o.scalar("niter_pos") == int(g["niter_pos"]) == 3
+    for q in POST_SUMS:
+        assert np.array_equal(o.array("sum_" + q), g["post_sum_" + q]), q
+    o.post_finalize()
+    for q in ("u", "v", "w", "p", "t", "uu", "uv", "uw", "vv", "vw", "ww"):
+        assert np.array_equal(o.array("sum_" + q), g["post_fin_" + q]), q
+    n = tuple(slice(1, m) for m in o.shape)
+    for q, name in (("pg", "avg_pg"), ("tg", "avg_tg"), ("uu", "rs_uu"), ("uv", "rs_uv"),
+                    ("uw", "rs_uw"), ("vv", "rs_vv"), ("vw", "rs_vw"), ("ww", "rs_ww")):
+        assert np.array_equal(o.array(name)[n], g["post_node_" + q]), q
+    w = (1, slice(1, o.nj), slice(1, o.nk))
+    for q in ("cf", "cp", "nu"):
+        assert np.array_equal(o.array(q + "_s")[w], g["post_surf_" + q]), q
+    for q, name in (("pg", "avg_pg"), ("ug", "avg_ug"), ("vg", "avg_vg"), ("wg", "avg_wg"),
+                    ("tg", "avg_tg"), ("urms", "rms_ug"), ("vrms", "rms_vg"),
+                    ("wrms", "rms_wg"), ("uu", "rs_uu"), ("uv", "rs_uv"), ("uw", "rs_uw"),
+                    ("vv", "rs_vv"), ("vw", "rs_vw"), ("ww", "rs_ww")):
+        np.testing.assert_allclose(o.array(name)[n], g["plt_" + q], rtol=1e-8, atol=1e-300,
+                                   err_msg=q)
+    for q in ("cf", "cp", "nu"):
+        np.testing.assert_allclose(o.array(q + "_s")[w], g["plt_surf_" + q], rtol=1e-8,
+                                   err_msg=q)
+
+
+def test_post_process_closed_forms():
+    """A state held fixed over the averaging window: the means are the state, the rms is |u|,
+    the resolved Reynolds stresses vanish (only the SGS part is left), and on the wall Cp is
+    twice the 8-cell pressure average.  Checked on a seeded state by calling the finalization
+    after N identical accumulations (three CALCCOEF passes on the same fields)."""
+    g = load_golden("rles_9x21x5_les_post")
+    o = make(g)
+    o.set_temperature(g["in_t"], g["in_adv"], float(g["pe"]))
+    o.post_begin()
+    for _ in range(3):
+        o.calccoef()
+    u, v, w = (o.array(c).copy() for c in ("u", "v", "w"))
+    o.post_finalize()
+    np.testing.assert_allclose(o.array("sum_u"), u, rtol=1e-15, atol=0)
+    # nu_t = 0 before any SGS pass: the stresses are <u u> - <u><u> = 0 up to round-off
+    for q in ("uu", "uv", "uw", "vv", "vw", "ww"):
+        assert np.abs(o.array("sum_" + q)).max() < 1e-14 * np.abs(u).max() ** 2, q
+    n = tuple(slice(1, m) for m in o.shape)
+    P = o.array("sum_p")
+    uabs = np.sqrt(u * u)
+    ref = 0.125 * (uabs[1:, 1:, 1:] + uabs[1:, :-1, 1:] + uabs[:-1, 1:, 1:] + uabs[:-1, :-1, 1:]
+                   + uabs[1:, 1:, :-1] + uabs[1:, :-1, :-1] + uabs[:-1, 1:, :-1]
+                   + uabs[:-1, :-1, :-1])
+    np.testing.assert_allclose(o.array("rms_ug")[n], ref, rtol=1e-14)
+    cp = 0.25 * (P[1, 1:-1, 1:-1] + P[1, :-2, 1:-1] + P[0, 1:-1, 1:-1] + P[0, :-2, 1:-1]
+                 + P[1, 1:-1, 2:] + P[1, :-2, 2:] + P[0, 1:-1, 2:] + P[0, :-2, 2:])
+    np.testing.assert_allclose(o.array("cp_s")[1, 1:-1, 1:-1], cp, rtol=1e-13, atol=1e-300)
