@@ -23,6 +23,15 @@ FIELDS = {
     "ug": 47, "vg": 48, "wg": 49, "tg": 50, "vorg": 51, "lamdag": 52,
     "yplus": 53,
 }
+# time averages: the running sums (the means / Reynolds stresses after the finalization), the
+# node averages of the Avrg, Urms and Strs files and the wall coefficients of the Surf file
+POST_SUMS = ("u", "v", "w", "p", "t", "uu", "uv", "uw", "vv", "vw", "ww", "sgs11", "sgs12",
+             "sgs13", "sgs22", "sgs23", "sgs33")
+POST_OUT = ("avg_pg", "avg_ug", "avg_vg", "avg_wg", "avg_tg", "rms_ug", "rms_vg", "rms_wg",
+            "rs_uu", "rs_uv", "rs_uw", "rs_vv", "rs_vw", "rs_ww", "surf_cf", "surf_cp",
+            "surf_nu")
+FIELDS.update({"sum_" + q: 54 + i for i, q in enumerate(POST_SUMS)})
+FIELDS.update({q: 71 + i for i, q in enumerate(POST_OUT)})
 
 STATUS = {0: "ok", 1: "bad argument", 2: "HIP runtime error", 3: "non-finite value",
           4: "Poisson sweep cap reached", 5: "grid too large for a line kernel"}
@@ -61,7 +70,7 @@ EXPORTS = ("fvm_create", "fvm_destroy", "fvm_set_field", "fvm_get_field", "fvm_s
            "fvm_p_source", "fvm_iter_u", "fvm_correct", "fvm_iter_p", "fvm_pressure_reference",
            "fvm_calccoef", "fvm_profile_enable", "fvm_profile_get", "fvm_profile_reset",
            "fvm_sync", "fvm_set_dtime", "fvm_create2d", "fvm_set_thermal", "fvm_iter_t",
-           "fvm_writedata_fields")
+           "fvm_writedata_fields", "fvm_post_begin", "fvm_post_finalize")
 
 _LIB = None
 
@@ -92,6 +101,8 @@ def load(path: str = LIB_PATH):
     L.fvm_set_thermal.argtypes = [vp, cd]
     L.fvm_iter_t.argtypes = [vp]
     L.fvm_writedata_fields.argtypes = [vp]
+    L.fvm_post_begin.argtypes = [vp, vp]
+    L.fvm_post_finalize.argtypes = [vp, ctypes.POINTER(ci)]
     L.fvm_iter_p.argtypes = [vp, pd, ctypes.POINTER(ci)]
     L.fvm_pressure_reference.argtypes = [vp, pd]
     L.fvm_calccoef.argtypes = [vp, ctypes.POINTER(StepInfo)]

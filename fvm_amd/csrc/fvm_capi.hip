@@ -183,7 +183,11 @@ static double* field_ptr(FvmCtx* c, int f) {
     case FVM_PG: case FVM_UG: case FVM_VG: case FVM_WG: case FVM_TG: case FVM_VORG:
     case FVM_LAMDAG: return c->nodal[f - FVM_PG];
     case FVM_YPLUS: return c->yplus;
-    default: return nullptr;
+    default:
+      if (f >= FVM_SUM_U && f <= FVM_SUM_SGS33) return c->post_sum[f - FVM_SUM_U];
+      if (f >= FVM_AVG_PG && f <= FVM_RS_WW) return c->post_node[f - FVM_AVG_PG];
+      if (f >= FVM_SURF_CF && f <= FVM_SURF_NU) return c->post_surf[f - FVM_SURF_CF];
+      return nullptr;
   }
 }
 
@@ -435,6 +439,20 @@ int fvm_set_dtime(fvm_ctx* c, double dtime) {
   return fvm_k_set_dtime(c, dtime);
 }
 
+int fvm_post_begin(fvm_ctx* c, const double* t_const) {
+  return c ? fvm_k_post_begin(c, t_const) : FVM_ERR_ARG;
+}
+
+int fvm_post_finalize(fvm_ctx* c, int* niter_pos) {
+  if (!c) return FVM_ERR_ARG;
+  const long np = c->niter_pos;
+  int r = fvm_k_post_finalize(c);
+  if (r) return r;
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (niter_pos) *niter_pos = (int)np;
+  return FVM_OK;
+}
+
 int fvm_contra_vel(fvm_ctx* c) { return c ? fvm_k_contra_vel(c) : FVM_ERR_ARG; }
 int fvm_nl(fvm_ctx* c) { return c ? fvm_k_nl(c) : FVM_ERR_ARG; }
 int fvm_dif(fvm_ctx* c) { return c ? fvm_k_dif(c) : FVM_ERR_ARG; }
@@ -485,6 +503,8 @@ int fvm_calccoef(fvm_ctx* c, fvm_step_info* info) {
   double o[8];
   int r = c->two_d ? fvm_k_calccoef2d(c, o) : fvm_k_calccoef(c, o);
   if (r) return r;
+  r = fvm_k_post_accum(c);  // calccoef.f90:286-307 while the time averages run
+  if (r) return r;
   if (info) {
     info->cfl = o[0]; info->cpb = o[1]; info->cd = o[2]; info->cl = o[3];
     info->cd_p = o[4]; info->cd_v = o[5]; info->cl_p = o[6]; info->cl_v = o[7];
@@ -531,6 +551,7 @@ int fvm_step(fvm_ctx* c, fvm_step_info* info) {
   } else {
     const Mailbox mb = fvm_mail(c);               // the force kernel publishes scal[0..16]
     CK(fvm_k_calccoef(c, nullptr, true, &mb));    // :150
+    CK(fvm_k_post_accum(c));                      // calccoef.f90:286-307 (time averages)
     r = fvm_mail_wait(c, mb, 17, c->hscal);
   }
 #undef CK

@@ -124,6 +124,15 @@ struct FvmCtx {
   double *vorm = nullptr, *lam2 = nullptr;
   double* nodal[7] = {};
   double* yplus = nullptr;  // wall y+ of CALCCOEF on the I = 1 cells (zero elsewhere)
+  // time averages: 17 running sums per cell (u v w p T, uu uv uw vv vw ww, SGS 11 12 13 22 23
+  // 33), the cell rms of the finalization, 14 node averages and the wall Cf, Cp, Nu
+  double* post_sum[17] = {};
+  double* post_rms[3] = {};
+  double* post_node[14] = {};
+  double* post_surf[3] = {};
+  double* post_t = nullptr;  // averaged temperature without the passive temperature model
+  bool post_on = false;
+  long niter_pos = 0;
   // last-step diagnostics
   fvm_step_info last{};
   // profiling
@@ -180,6 +189,9 @@ int fvm_k_step_copy(FvmCtx* c);
 int fvm_k_iter_t(FvmCtx* c);
 int fvm_thermal_setup(FvmCtx* c);
 int fvm_k_writedata_fields(FvmCtx* c);
+int fvm_k_post_begin(FvmCtx* c, const double* t_const);
+int fvm_k_post_accum(FvmCtx* c);
+int fvm_k_post_finalize(FvmCtx* c);
 int fvm_k_correct(FvmCtx* c, double sign);
 int fvm_k_psource_correct(FvmCtx* c, double sign, bool with_cfl);
 int fvm_k_correct_bcu_contra(FvmCtx* c);

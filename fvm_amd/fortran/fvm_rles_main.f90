@@ -7,7 +7,11 @@
 ! lines of calccoef.f90:190-198, CS.PLT / CT.PLT (stress_tensor_dy.f90:519-522, 707-711), the
 ! nodal Tecplot fields PVT_nnn_000.PLT (x, y, z, p, u, v, w, T) and VOR_nnn_000.PLT (x, y, z,
 ! |omega|, lambda_2) every NITER_W steps (writedata.f90:8-189) and RESTART000.DAT every NITER_R
-! steps and at the end (writedata.f90:193-216).  Run constants
+! steps and at the end (writedata.f90:193-216).  With RTIME_END > RTIME_MAX the run continues
+! after RTIME_MAX in the time-averaging phase of POST_PROCESS (main.f90:164-166,
+! post_process.f90:9-276): DTIME = DTIME_POST up to RTIME_END (or NSTEP), then the means, rms,
+! Reynolds stresses and wall coefficients to Avrg000.PLT, Urms000.PLT, Strs000.PLT and
+! Surf000.PLT.  Run constants
 ! come from the namelist file fvm_rles.nml (the reference compiles them in,
 ! comdat_param.f90:6-48).  The passive temperature is advanced as in the reference
 ! (THERMAL = 1, Peclet number PE) or carried through the restart unchanged with NU = 0
@@ -21,16 +25,18 @@ PROGRAM FVM_RLES
   INTEGER :: NITER_R = HUGE(1), NSTEP = HUGE(1), AF_MODE = 0, THERMAL = 1, NITER_W = HUGE(1)
   DOUBLE PRECISION :: RE = 3900.D0, DTIME = 2.D-3, RESIMAX = 1.D-8, REF_A = -1.D0
   DOUBLE PRECISION :: RTIME_MAX = HUGE(1.D0), PE = 2769.D0, PR = 0.71D0
+  DOUBLE PRECISION :: RTIME_END = -1.D0, DTIME_POST = 1.D-3   ! comdat_param.f90:26, 44
   NAMELIST /FVM_RUN/ NI, NJ, NK, RE, DTIME, RESIMAX, NSWPP, USE_LES, REF_A, POISSON_MODE, &
-       RTIME_MAX, NITER_R, NSTEP, AF_MODE, THERMAL, PE, PR, NITER_W
+       RTIME_MAX, NITER_R, NSTEP, AF_MODE, THERMAL, PE, PR, NITER_W, RTIME_END, DTIME_POST
 
   TYPE(FVM_PARAMS) :: PRM
   TYPE(FVM_STEP_INFO) :: INFO
   TYPE(C_PTR) :: CTX
   DOUBLE PRECISION, ALLOCATABLE :: XG(:), YG(:), ZG(:)
-  DOUBLE PRECISION, ALLOCATABLE :: U(:), V(:), W(:), P(:), T(:), NLX(:), NLY(:), NLZ(:), ADV(:)
+  DOUBLE PRECISION, ALLOCATABLE :: U(:), V(:), W(:), P(:), NLX(:), NLY(:), NLZ(:), ADV(:)
+  DOUBLE PRECISION, ALLOCATABLE, TARGET :: T(:)
   DOUBLE PRECISION :: RE_F, PE_F, DT_F, RTIME, RESIP, C_AVG, PREF, NU
-  INTEGER :: NITER_TIME, NITER, ISTEP, NNODE, NCELL, Q, IU, IOS, NUM_WRITE = 0
+  INTEGER :: NITER_TIME, NITER, ISTEP, NNODE, NCELL, Q, IU, IOS, NUM_WRITE = 0, IPOST = 0
   CHARACTER(LEN=256) :: NMLFILE
   CHARACTER(LEN=15) :: GRIDFILE
   LOGICAL :: EXISTS
@@ -102,7 +108,8 @@ PROGRAM FVM_RLES
   NU = 0.D0
 
   ISTEP = 0
-  DO WHILE (RTIME .LT. RTIME_MAX .AND. ISTEP .LT. NSTEP)                 ! main.f90:35,161
+  DO                                                                     ! main.f90:35,161-166
+  DO WHILE (RTIME .LT. RTIME_MAX .AND. ISTEP .LT. NSTEP)
     ISTEP = ISTEP + 1
     NITER_TIME = NITER_TIME + 1
     RTIME = RTIME + DTIME
@@ -146,6 +153,15 @@ PROGRAM FVM_RLES
     WRITE (*, 105) NITER, RESIP, C_AVG, PREF
     IF (MOD(NITER_TIME, NITER_W) .EQ. 0) CALL WRITE_FIELDS()              ! main.f90:154
     IF (MOD(NITER_TIME, NITER_R) .EQ. 0) CALL WRITE_RESTART()
+  END DO
+  ! POST_PROCESS (main.f90:164-166): its first call starts the averaging phase, its second
+  ! writes the averages
+  IF (IPOST .EQ. 1) THEN
+    CALL POST_END()
+    EXIT
+  END IF
+  IF (RTIME_END .LE. RTIME_MAX .OR. ISTEP .GE. NSTEP) EXIT
+  CALL POST_BEGIN()
   END DO
   CALL WRITE_RESTART()
   CALL FVM_CHECK(FVM_DESTROY(CTX), 'fvm_destroy')
@@ -233,6 +249,101 @@ CONTAINS
 161 FORMAT(TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9)
 183 FORMAT(TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9)
   END SUBROUTINE WRITE_FIELDS
+
+  ! POST_PROCESS's first call (post_process.f90:9-61): zeroed sums, CALCCOEF accumulates from
+  ! the next step on, RTIME_MAX = RTIME_END, DTIME = DTIME_POST.  Without the passive
+  ! temperature the restart's T is the one averaged.
+  SUBROUTINE POST_BEGIN()
+    IF (THERMAL .EQ. 1) THEN
+      CALL FVM_CHECK(FVM_POST_BEGIN(CTX, C_NULL_PTR), 'POST_PROCESS')
+    ELSE
+      CALL FVM_CHECK(FVM_POST_BEGIN(CTX, C_LOC(T)), 'POST_PROCESS')
+    END IF
+    RTIME_MAX = RTIME_END
+    DTIME = DTIME_POST
+    CALL FVM_CHECK(FVM_SET_DTIME(CTX, DTIME), 'DTIME = DTIME_POST')
+    IPOST = 1
+  END SUBROUTINE POST_BEGIN
+
+  ! POST_PROCESS's second call (post_process.f90:62-276): the node averages of the means
+  ! (Avrg000.PLT), of the rms (Urms000.PLT) and of the Reynolds stresses with the SGS part
+  ! (Strs000.PLT) on the nodes I = 1..NI, J = 1..NJ, K = 1..NK, and the wall coefficients
+  ! on the wall cells J = 1..NJ-1, K = 1..NK-1 (Surf000.PLT; THETA in radians, as the
+  ! reference's degree conversion at post_process.f90:259 addresses no wall cell)
+  SUBROUTINE POST_END()
+    INTEGER :: JU, I, J, K, C, N
+    INTEGER(C_INT) :: NPOS, SEL(21)
+    DOUBLE PRECISION, ALLOCATABLE :: F(:,:)
+    SEL = (/ FVM_AVG_PG, FVM_AVG_UG, FVM_AVG_VG, FVM_AVG_WG, FVM_AVG_TG, FVM_RMS_UG, &
+             FVM_RMS_VG, FVM_RMS_WG, FVM_RS_UU, FVM_RS_UV, FVM_RS_UW, FVM_RS_VV, FVM_RS_VW, &
+             FVM_RS_WW, FVM_SURF_CF, FVM_SURF_CP, FVM_SURF_NU, FVM_X, FVM_Y, FVM_Z, FVM_THETA /)
+    CALL FVM_CHECK(FVM_POST_FINALIZE(CTX, NPOS), 'POST_PROCESS')
+    ALLOCATE (F(NCELL, 21))
+    DO N = 1, 21
+      CALL FVM_CHECK(FVM_GET_FIELD(CTX, SEL(N), F(:, N)), 'fvm_get_field')
+    END DO
+    OPEN (NEWUNIT=JU, FILE='Avrg000.PLT', STATUS='REPLACE')
+    WRITE (JU, *) 'TITLE="', 'RE', RE, 'PR', PR, '"'
+    WRITE (JU, *) 'variables=x,y,z,p,u,v,w,T'
+    WRITE (JU, 68) NI, NJ, NK, RTIME
+    DO K = 1, NK
+      DO J = 1, NJ
+        DO I = 1, NI
+          C = 1 + I + (NI + 1) * (J + (NJ + 1) * K)
+          Q = I + NI * ((J - 1) + NJ * (K - 1))
+          WRITE (JU, 73) XG(Q), YG(Q), ZG(Q), (F(C, N), N = 1, 5)
+        END DO
+      END DO
+    END DO
+    CLOSE (JU)
+    OPEN (NEWUNIT=JU, FILE='Urms000.PLT', STATUS='REPLACE')
+    WRITE (JU, *) 'TITLE="', 'RE', RE, 'PR', PR, '"'
+    WRITE (JU, *) 'variables=x,y,z,u_rms,v_rms,w_rms'
+    WRITE (JU, 68) NI, NJ, NK, RTIME
+    DO K = 1, NK
+      DO J = 1, NJ
+        DO I = 1, NI
+          C = 1 + I + (NI + 1) * (J + (NJ + 1) * K)
+          Q = I + NI * ((J - 1) + NJ * (K - 1))
+          WRITE (JU, 109) XG(Q), YG(Q), ZG(Q), (F(C, N), N = 6, 8)
+        END DO
+      END DO
+    END DO
+    CLOSE (JU)
+    OPEN (NEWUNIT=JU, FILE='Strs000.PLT', STATUS='REPLACE')
+    WRITE (JU, *) 'TITLE="', 'RE', RE, 'PR', PR, '"'
+    WRITE (JU, *) 'variables=x,y,z,uu,uv,uw,vv,vw,ww'
+    WRITE (JU, 68) NI, NJ, NK, RTIME
+    DO K = 1, NK
+      DO J = 1, NJ
+        DO I = 1, NI
+          C = 1 + I + (NI + 1) * (J + (NJ + 1) * K)
+          Q = I + NI * ((J - 1) + NJ * (K - 1))
+          WRITE (JU, 249) XG(Q), YG(Q), ZG(Q), (F(C, N), N = 9, 14)
+        END DO
+      END DO
+    END DO
+    CLOSE (JU)
+    OPEN (NEWUNIT=JU, FILE='Surf000.PLT', STATUS='REPLACE')
+    WRITE (JU, *) 'TITLE="SURFACE COEF."'
+    WRITE (JU, *) 'variables="x","y","z","Theta","Cf","Cp","Nu"'
+    WRITE (JU, 132) 1, NJ - 1, NK - 1
+    DO K = 1, NK - 1
+      DO J = 1, NJ - 1
+        C = 2 + (NI + 1) * (J + (NJ + 1) * K)
+        WRITE (JU, 137) F(C, 18), F(C, 19), F(C, 20), F(C, 21), F(C, 15), F(C, 16), F(C, 17)
+      END DO
+    END DO
+    CLOSE (JU)
+    DEALLOCATE (F)
+68  FORMAT(TR1,'ZONE T="WHOLE", I=',I6,', J=',I6,', K=',I6,', SOLUTIONTIME=',F10.4)
+73  FORMAT(TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9)
+109 FORMAT(TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9)
+132 FORMAT(TR1,'ZONE T="POST", I=',I6,', j=',I6,', k=',I6)
+137 FORMAT(TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9)
+249 FORMAT(TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9,TR1,D16.9, &
+           TR1,D16.9)
+  END SUBROUTINE POST_END
 
   ! RESTART000.DAT in the reference's layout (writedata.f90:193-216)
   SUBROUTINE WRITE_RESTART()

@@ -100,6 +100,21 @@ class Solver3D:
         capi.check(self.L.fvm_writedata_fields(self.h), "writedata_fields")
     def correct(self, sign): capi.check(self.L.fvm_correct(self.h, float(sign)), "correct")
 
+    def post_begin(self, t=None):
+        """Start the time averages of POST_PROCESS (t: the temperature averaged without the
+        passive temperature model)."""
+        tp = None
+        if t is not None:
+            tp = np.ascontiguousarray(np.asfortranarray(t, dtype=np.float64).ravel(order="F"))
+        capi.check(self.L.fvm_post_begin(self.h, tp.ctypes.data if tp is not None else None),
+                   "post_begin")
+
+    def post_finalize(self):
+        """Means, rms, Reynolds stresses, node averages, wall coefficients; returns NITER_POS."""
+        n = ctypes.c_int()
+        capi.check(self.L.fvm_post_finalize(self.h, ctypes.byref(n)), "post_finalize")
+        return n.value
+
     def stress_tensor_dy(self):
         c = ctypes.c_double()
         capi.check(self.L.fvm_stress_tensor_dy(self.h, ctypes.byref(c)), "stress_tensor_dy")

@@ -80,7 +80,24 @@ enum {
      (nodes I = 1..NI, J = 1..NJ, K = 1..NK of the cell-array layout); the wall y+ of CALCCOEF
      on the I = 1 cells (zero elsewhere) */
   FVM_VORM = 44, FVM_LAMDA2 = 45, FVM_PG = 46, FVM_UG = 47, FVM_VG = 48, FVM_WG = 49,
-  FVM_TG = 50, FVM_VORG = 51, FVM_LAMDAG = 52, FVM_YPLUS = 53, FVM_NFIELDS = 54
+  FVM_TG = 50, FVM_VORG = 51, FVM_LAMDAG = 52, FVM_YPLUS = 53,
+  /* time averages (after fvm_post_begin): the running sums of u, v, w, p, T, of uu, uv, uw, vv,
+     vw, ww and of the SGS stresses -2 nu_t S_ij (11, 12, 13, 22, 23, 33) on every cell, which
+     fvm_post_finalize leaves as the reference leaves U_AVG..WW_AVG and SGS11..SGS33: the means,
+     the Reynolds stresses and the SGS means */
+  FVM_SUM_U = 54, FVM_SUM_V = 55, FVM_SUM_W = 56, FVM_SUM_P = 57, FVM_SUM_T = 58,
+  FVM_SUM_UU = 59, FVM_SUM_UV = 60, FVM_SUM_UW = 61, FVM_SUM_VV = 62, FVM_SUM_VW = 63,
+  FVM_SUM_WW = 64, FVM_SUM_SGS11 = 65, FVM_SUM_SGS12 = 66, FVM_SUM_SGS13 = 67,
+  FVM_SUM_SGS22 = 68, FVM_SUM_SGS23 = 69, FVM_SUM_SGS33 = 70,
+  /* after fvm_post_finalize: node averages held at the node's cell index (nodes I = 1..NI,
+     J = 1..NJ, K = 1..NK) of the means of p, u, v, w, T (Avrg000.PLT), of the rms
+     sqrt(<u_i u_i>) (Urms000.PLT) and of the Reynolds stresses with the SGS part (Strs000.PLT),
+     and the wall friction, pressure and Nusselt coefficients on the I = 1 cells (Surf000.PLT,
+     zero elsewhere) */
+  FVM_AVG_PG = 71, FVM_AVG_UG = 72, FVM_AVG_VG = 73, FVM_AVG_WG = 74, FVM_AVG_TG = 75,
+  FVM_RMS_UG = 76, FVM_RMS_VG = 77, FVM_RMS_WG = 78, FVM_RS_UU = 79, FVM_RS_UV = 80,
+  FVM_RS_UW = 81, FVM_RS_VV = 82, FVM_RS_VW = 83, FVM_RS_WW = 84, FVM_SURF_CF = 85,
+  FVM_SURF_CP = 86, FVM_SURF_NU = 87, FVM_NFIELDS = 88
 };
 
 /* GRID + METRIC + the geometric part of INIT (RLES/grid.f90:1-367, RLES/metric.f90:1-538,
@@ -126,6 +143,23 @@ int fvm_set_thermal(fvm_ctx* ctx, double pe);
  * wall y+ that CALCCOEF computes and WRITEDATA writes (calccoef.f90:121, 159; writedata.f90:
  * 14-32), for the current state; read them with fvm_get_field.  3D contexts with NI >= 4. */
 int fvm_writedata_fields(fvm_ctx* ctx);
+
+/* Time averages of POST_PROCESS (RLES/post_process.f90:9-276) with their accumulation at the
+ * end of CALCCOEF (RLES/calccoef.f90:286-307).  fvm_post_begin is POST_PROCESS's first call: it
+ * zeroes the running sums (FVM_SUM_*) and starts the accumulation, after which every fvm_step
+ * and fvm_calccoef of a 3D context adds the state it leaves: u, v, w, p, T, the products u_i u_j
+ * and, with the LES model on, the SGS stresses -2 nu_t S_ij of the step's STRESS_TENSOR_DY.
+ * t_const is the temperature averaged when the passive temperature is off (the field the
+ * reference carries unchanged; NULL: zero); it is ignored after fvm_set_thermal.  The caller
+ * switches DTIME to DTIME_POST itself with fvm_set_dtime, as POST_PROCESS does
+ * (post_process.f90:59).  fvm_post_finalize is its second call: the means, the rms, the
+ * Reynolds stresses <u_i u_j> - <u_i><u_j> + <tau_ij>, their node averages and the wall
+ * coefficients Cf = 2/sqrt(Re) |du_t/dn|, Cp = 2 p, Nu = -dT/dn from the means
+ * (post_process.f90:62-276), read with fvm_get_field; it ends the accumulation and returns
+ * the number of accumulated steps in niter_pos (may be NULL).  FVM_ERR_ARG for 2D contexts and
+ * for a finalization with no accumulated step. */
+int fvm_post_begin(fvm_ctx* ctx, const double* t_const);
+int fvm_post_finalize(fvm_ctx* ctx, int* niter_pos);
 
 /* Change DTIME between steps.  DTIME is a run-time module variable of the reference, reassigned
  * by POST_PROCESS (RLES/post_process.f90:59) and by the CFL control of CALCCOEF

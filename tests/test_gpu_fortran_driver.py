@@ -76,6 +76,43 @@ def test_fortran_driver_two_steps(tmp_path):
         assert np.abs(vals[:, 1] - g[key]).max() <= 1e-10 * np.abs(g[key]).max(), fname
 
 
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built (flang absent)")
+def test_fortran_driver_post_process(tmp_path):
+    """One step to RTIME_MAX, then POST_PROCESS's averaging phase at DTIME_POST up to
+    RTIME_END (three steps) and its Avrg000 / Urms000 / Strs000 / Surf000.PLT files, against
+    the values and headers the reference wrote for the same run."""
+    from fvm_amd.grid import grid_filename, write_grid3d, write_restart3d
+    g = load_golden("rles_9x21x5_les_post")
+    ni, nj, nk = int(g["ni"]), int(g["nj"]), int(g["nk"])
+    write_grid3d(str(tmp_path / grid_filename(ni, nj, nk)), g["xg"], g["yg"], g["zg"])
+    f = {k: g["in_" + k] for k in ("u", "v", "w", "p", "nlx", "nly", "nlz", "t", "adv")}
+    write_restart3d(str(tmp_path / "RESTART000.DAT"), f, float(g["re"]), 0.0, float(g["dt"]), 0.0)
+    (tmp_path / "fvm_rles.nml").write_text(
+        f"&FVM_RUN NI={ni}, NJ={nj}, NK={nk}, RE={float(g['re'])!r}, DTIME={float(g['dt'])!r},\n"
+        f" RESIMAX=1.0D-13, NSWPP=50, USE_LES=1, REF_A={float(g['ref_a'])!r},"
+        f" POISSON_MODE=2, THERMAL=1, PE={float(g['pe'])!r}, RTIME_MAX={float(g['dt'])!r},"
+        f" RTIME_END=0.0065, DTIME_POST={float(g['dt_post'])!r} /\n")
+    print("fortran driver: rles_9x21x5, 1 step + 3 averaging steps", flush=True)
+    out = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    body = (tmp_path / "NU_CD_CL.PLT").read_text().split("\n", 1)[1]
+    assert np.array(body.split(), dtype=np.float64).reshape(-1, 10).shape == (4, 10)
+    headers = g["plt_headers"]
+    for h, (fname, cols, nrec) in enumerate((
+            ("Avrg000.PLT", ("pg", "ug", "vg", "wg", "tg"), ni * nj * nk),
+            ("Urms000.PLT", ("urms", "vrms", "wrms"), ni * nj * nk),
+            ("Strs000.PLT", ("uu", "uv", "uw", "vv", "vw", "ww"), ni * nj * nk),
+            ("Surf000.PLT", ("surf_theta", "surf_cf", "surf_cp", "surf_nu"),
+             (nj - 1) * (nk - 1)))):
+        lines = (tmp_path / fname).read_text().splitlines()
+        assert lines[:3] == list(headers[h]), fname
+        vals = np.array(" ".join(lines[3:]).replace("D", "e").split(), dtype=np.float64)
+        vals = vals.reshape(nrec, 3 + len(cols))
+        for q, key in enumerate(cols):
+            ref = g["plt_" + key].ravel(order="F")
+            assert np.abs(vals[:, 3 + q] - ref).max() <= 1e-8 * np.abs(ref).max(), (fname, key)
+
+
 EXE2D = os.path.join(REPO, "fvm_amd", "bin", "fvm_r2d")
 
 
