@@ -168,6 +168,58 @@ def cpu_baseline(w, log):
                        f"restatement on 1 host core, {t1 - t0:.2f} s")
 
 
+def run_ensemble(a, log):
+    """The ensemble of seeded 2D cases (uneven sizes, Re in [40, 300]) stepped together by
+    fvm_ensemble_step; the same steps with one host thread give the one-case-at-a-time rate."""
+    from fvm_amd.grid import ensemble_cases
+    from fvm_amd.solver import ensemble2d
+    cases = ensemble_cases(a.cases)
+    ncell = sum((c["ni"] - 1) * (c["nj"] - 1) for c in cases)
+    log(f"ensemble: {len(cases)} cases, {ncell} cells, {a.threads} host threads")
+    rates = {}
+    for nt in (a.threads, 1):
+        # the impulsive start (uniform flow through the wall cells) at dt/10 first, as for
+        # the 3D workloads, then each case's own dt
+        ens = ensemble2d(cases, nthreads=nt, dt_scale=a.startup_factor, nswpp=a.nswpp,
+                         resimax=a.resimax, poisson_mode=3, check_every=a.check_every)
+        for q in range(a.startup_steps):
+            infos = ens.step()
+        log(f"start-up: {a.startup_steps} steps at dt/{1 / a.startup_factor:g}, max cfl "
+            f"{max(i['cfl'] for i in infos):.3f}")
+        for sv, c in zip(ens.solvers, cases):
+            sv.set_dtime(c["dt"])
+        for q in range(a.warmup):
+            infos = ens.step()
+            log(f"warmup step {q + 1}/{a.warmup} threads {nt}: max cfl "
+                f"{max(i['cfl'] for i in infos):.3f} max resip {max(i['resip'] for i in infos):.2e}")
+        ens.sync()
+        t0 = time.perf_counter()
+        for q in range(a.steps):
+            infos = ens.step()
+        ens.sync()
+        secs = time.perf_counter() - t0
+        assert all(np.isfinite(i["cd"]) for i in infos)
+        rates[nt] = (secs, infos)
+        log(f"threads {nt}: {secs / a.steps * 1e3:.2f} ms/step "
+            f"{ncell * a.steps / secs / 1e6:.1f} Mcell-steps/s")
+        del ens
+    secs, infos = rates[a.threads]
+    out = {
+        "metric": "Mcell-steps/s, ensemble of 2D cylinder cases of uneven size and Re",
+        "value": ncell * a.steps / secs / 1e6, "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": secs / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": "ensemble2d", "cases": len(cases), "cells": ncell,
+                   "host_threads": a.threads, "resimax": a.resimax, "poisson": "mixed",
+                   "startup": f"{a.startup_steps} untimed steps at dt/{1 / a.startup_factor:g}"},
+        "one_thread_mcell_steps_per_s": ncell * a.steps / rates[1][0] / 1e6,
+        "resip_max": float(max(i["resip"] for i in infos)),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +237,8 @@ def main():
     ap.add_argument("--profile", action="store_true", help="print every kernel scope")
     ap.add_argument("--thermal", action="store_true",
                     help="advance the passive temperature too (not part of the headline step)")
+    ap.add_argument("--cases", type=int, default=96, help="ensemble2d: number of 2D cases")
+    ap.add_argument("--threads", type=int, default=8, help="ensemble2d: host threads")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the solver holds "
@@ -192,6 +246,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    if a.workload == "ensemble2d":
+        run_ensemble(a, log)
+        return
     from fvm_amd.grid import WORKLOADS
     w = WORKLOADS[a.workload]
     ncell = cells_of(w)

@@ -70,7 +70,7 @@ EXPORTS = ("fvm_create", "fvm_destroy", "fvm_set_field", "fvm_get_field", "fvm_s
            "fvm_p_source", "fvm_iter_u", "fvm_correct", "fvm_iter_p", "fvm_pressure_reference",
            "fvm_calccoef", "fvm_profile_enable", "fvm_profile_get", "fvm_profile_reset",
            "fvm_sync", "fvm_set_dtime", "fvm_create2d", "fvm_set_thermal", "fvm_iter_t",
-           "fvm_writedata_fields", "fvm_post_begin", "fvm_post_finalize")
+           "fvm_writedata_fields", "fvm_post_begin", "fvm_post_finalize", "fvm_ensemble_step")
 
 _LIB = None
 
@@ -103,6 +103,7 @@ def load(path: str = LIB_PATH):
     L.fvm_writedata_fields.argtypes = [vp]
     L.fvm_post_begin.argtypes = [vp, vp]
     L.fvm_post_finalize.argtypes = [vp, ctypes.POINTER(ci)]
+    L.fvm_ensemble_step.argtypes = [ctypes.POINTER(vp), ci, ci, ctypes.POINTER(StepInfo)]
     L.fvm_iter_p.argtypes = [vp, pd, ctypes.POINTER(ci)]
     L.fvm_pressure_reference.argtypes = [vp, pd]
     L.fvm_calccoef.argtypes = [vp, ctypes.POINTER(StepInfo)]

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <thread>
 #include <vector>
 
 #include "fvm_ctx.h"
@@ -437,6 +438,27 @@ int fvm_writedata_fields(fvm_ctx* c) { return c ? fvm_k_writedata_fields(c) : FV
 int fvm_set_dtime(fvm_ctx* c, double dtime) {
   if (!c || !(dtime > 0)) return FVM_ERR_ARG;
   return fvm_k_set_dtime(c, dtime);
+}
+
+int fvm_ensemble_step(fvm_ctx* const* ctxs, int n, int nthreads, fvm_step_info* infos) {
+  if (!ctxs || n < 0 || nthreads < 1) return FVM_ERR_ARG;
+  for (int q = 0; q < n; ++q)
+    if (!ctxs[q]) return FVM_ERR_ARG;
+  std::vector<int> status(n, FVM_OK);
+  const int nt = std::min(nthreads, std::max(n, 1));
+  auto work = [&](int t) {
+    for (int q = t; q < n; q += nt) status[q] = fvm_step(ctxs[q], infos ? infos + q : nullptr);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (std::thread& th : pool) th.join();
+  int r = FVM_OK;
+  for (int q = 0; q < n; ++q) {
+    if (status[q] != FVM_OK && status[q] != FVM_ERR_NOCONV) return status[q];
+    if (status[q] == FVM_ERR_NOCONV) r = FVM_ERR_NOCONV;
+  }
+  return r;
 }
 
 int fvm_post_begin(fvm_ctx* c, const double* t_const) {

@@ -273,6 +273,31 @@ WORKLOADS = {
 }
 
 
+# the ensemble of 2D cases: circumferential x radial node counts and the Reynolds range
+ENSEMBLE_NJ = (129, 193, 257, 385, 513)
+ENSEMBLE_NI = (65, 97, 129, 193)
+ENSEMBLE_RE = (40.0, 300.0)
+
+
+def ensemble_cases(n: int = 96, seed: int = SEED, h1: float = 0.002, r_out: float = 30.0,
+                   sigma: float = 1e-3, dt: float = 0.005):
+    """Seeded 2D cylinder cases of uneven size and Reynolds number: per case index c, one
+    Philox stream (key seed, stream 1000 + c) draws NJ, NI from the size sets and Re uniform in
+    [40, 300].  Stretching law and velocity noise as the single 2D case (wall spacing h1, outer
+    radius r_out, N(0, sigma) noise with the case's own stream); the time step keeps the 257-node
+    case's circumferential Courant number, dt (256 / (NJ - 1)) capped at dt."""
+    out = []
+    for c in range(n):
+        gen = np.random.Generator(np.random.Philox(np.random.SeedSequence([seed, 1000 + c])))
+        nj = int(ENSEMBLE_NJ[gen.integers(len(ENSEMBLE_NJ))])
+        ni = int(ENSEMBLE_NI[gen.integers(len(ENSEMBLE_NI))])
+        re = float(gen.uniform(*ENSEMBLE_RE))
+        (xg, yg, _), ic = make_case(ni, nj, 1, h1, r_out, sigma, seed=seed + 1 + c)
+        out.append(dict(ni=ni, nj=nj, re=re, dt=dt * min(1.0, 256.0 / (nj - 1)), xg=xg, yg=yg,
+                        ic=ic))
+    return out
+
+
 def make_case(ni, nj, nk, h1, r_out, sigma, seed=SEED, lz=math.pi, **_):
     """Grid nodes and seeded initial fields for a workload (3D when nk > 1)."""
     g = OGrid(ni, nj, nk, h1, r_out, lz)

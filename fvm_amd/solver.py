@@ -157,6 +157,42 @@ class Solver3D:
         return out
 
 
+class Ensemble:
+    """Independent cases stepped together by fvm_ensemble_step (one stream per case, nthreads
+    host threads)."""
+
+    def __init__(self, solvers, nthreads=8):
+        self.L = capi.load()
+        self.solvers = list(solvers)
+        self.nthreads = int(nthreads)
+        self._h = (ctypes.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+
+    def step(self):
+        n = len(self.solvers)
+        infos = (capi.StepInfo * n)()
+        code = self.L.fvm_ensemble_step(self._h, n, self.nthreads, infos)
+        if code not in (0, 4):
+            raise capi.FvmError(code, "fvm_ensemble_step")
+        return [infos[q].as_dict() for q in range(n)]
+
+    def sync(self):
+        for s in self.solvers:
+            s.sync()
+
+
+def ensemble2d(cases, nthreads=8, dt_scale=1.0, **kw):
+    """An Ensemble of Solver2D built from fvm_amd.grid.ensemble_cases entries, each at its
+    case's time step times dt_scale."""
+    solvers = []
+    for c in cases:
+        s = Solver2D(c["ni"], c["nj"], c["re"], c["dt"] * dt_scale, c["xg"][:, :, 0],
+                     c["yg"][:, :, 0], **kw)
+        ic = c["ic"]
+        s.set_state(*(ic[k][:, :, 0] for k in ("u", "v", "p", "nlx", "nly")))
+        solvers.append(s)
+    return Ensemble(solvers, nthreads)
+
+
 class Solver2D(Solver3D):
     """2D fractional 4-step solver (one-cell span slab of the 3D kernels, see fvm_create2d).
     Fields are (NI+1, NJ+1) arrays of the single plane, as in the 2D reference's RESTART.DAT."""
