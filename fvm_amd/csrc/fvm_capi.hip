@@ -446,12 +446,19 @@ int fvm_ensemble_step(fvm_ctx* const* ctxs, int n, int nthreads, fvm_step_info* 
     if (!ctxs[q]) return FVM_ERR_ARG;
   std::vector<int> status(n, FVM_OK);
   const int nt = std::min(nthreads, std::max(n, 1));
-  auto work = [&](int t) {
-    for (int q = t; q < n; q += nt) status[q] = fvm_step(ctxs[q], infos ? infos + q : nullptr);
+  // each thread takes the next case from a shared counter, so uneven cases balance and any
+  // number of started threads (the calling one included) covers every case
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (int q; (q = next.fetch_add(1)) < n;)
+      status[q] = fvm_step(ctxs[q], infos ? infos + q : nullptr);
   };
   std::vector<std::thread> pool;
-  for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
-  work(0);
+  try {
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+  } catch (...) {  // fewer threads: the cases are still all taken
+  }
+  work();
   for (std::thread& th : pool) th.join();
   int r = FVM_OK;
   for (int q = 0; q < n; ++q) {

@@ -165,9 +165,9 @@ int fvm_post_finalize(fvm_ctx* ctx, int* niter_pos);
  * together: the ensemble of 2D cylinder cases of uneven grid size and Reynolds number.  Each
  * case is the time step of fvm_step (2D: MAIN of cylinder/cylinder_v3.0.3_AF_original
  * 4step_face_nl/Main.f90:48-110; the reference has no ensemble driver).  Every context runs on
- * its own stream; nthreads host threads take the contexts round-robin, so the kernels of
- * different cases overlap on the device while a thread waits for one case's convergence
- * checks.  infos (n entries) may be NULL.  Returns the first error status, else
+ * its own stream; nthreads host threads take the next context from a shared counter, so the
+ * kernels of different cases overlap on the device while a thread waits for one case's
+ * convergence checks.  infos (n entries) may be NULL.  Returns the first error status, else
  * FVM_ERR_NOCONV if any case hit its Poisson sweep cap, else FVM_OK.  The contexts must be
  * distinct and not used by other threads during the call. */
 int fvm_ensemble_step(fvm_ctx* const* ctxs, int n, int nthreads, fvm_step_info* infos);
