@@ -608,14 +608,54 @@ __global__ void k_forces_final(Dims d, const double* __restrict__ p,
 // calccoef.f90:11-144) on the slab's plane K = 1, one lane, the reference's summation order.
 // wall2d per J: segment length AREA, centre-to-wall distance DN1, sin/cos TH2 (direction of the
 // wall normal through the centre), cos/sin THETA of the wall cell.
-__global__ void k_calccoef2d(Dims d, const double* __restrict__ wall2d,
-                             const double* __restrict__ p, const double* __restrict__ u,
-                             const double* __restrict__ v, double re, double vel_rot,
-                             const double* __restrict__ cflpart, int ncfl, double* out) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// One workgroup of 256: the CFL maximum over the partial maxima (order-free) and the wall terms
+// of every wall cell formed in parallel, then added by one lane in the reference's J order
+// (calccoef.f90:46-79), 256 wall cells at a time through LDS, so the sums round as the
+// reference's loop does.
+__global__ void __launch_bounds__(256) k_calccoef2d(Dims d, const double* __restrict__ wall2d,
+                                                    const double* __restrict__ p,
+                                                    const double* __restrict__ u,
+                                                    const double* __restrict__ v, double re,
+                                                    double vel_rot,
+                                                    const double* __restrict__ cflpart, int ncfl,
+                                                    double* out) {
+  __shared__ double tm[4][256];
+  __shared__ double cm[256];
+  const int t = threadIdx.x;
   const int k = 1;
   double cmax = 0.0;
-  for (int q = 0; q < ncfl; ++q) cmax = fmax(cmax, cflpart[q]);
+  for (int q = t; q < ncfl; q += 256) cmax = fmax(cmax, cflpart[q]);
+  cm[t] = cmax;
+  double pdrag = 0.0, plift = 0.0, tdrag = 0.0, tlift = 0.0;
+  for (int j0 = 1; j0 <= d.nj - 1; j0 += 256) {
+    const int j = j0 + t;
+    if (j <= d.nj - 1) {
+      const double* o = wall2d + 6 * (j - 1);
+      const double area = o[0], dn1 = o[1], s2 = o[2], c2 = o[3], ct = o[4], st = o[5];
+      const long c = cidx(d, 1, j, k);
+      const double dudn = (u[c] * s2 - v[c] * c2 - vel_rot) / dn1;
+      tm[0][t] = 2.0 * p[c] * area * ct;
+      tm[1][t] = 2.0 * p[c] * area * st;
+      tm[2][t] = (2.0 / re) * dudn * area * st;
+      tm[3][t] = (2.0 / re) * dudn * area * ct;
+    }
+    __syncthreads();
+    if (t == 0) {
+      const int m = min(256, d.nj - j0);
+      for (int q = 0; q < m; ++q) {
+        pdrag = pdrag - tm[0][q];
+        plift = plift - tm[1][q];
+        tdrag = tdrag + tm[2][q];
+        tlift = tlift - tm[3][q];
+      }
+    }
+    __syncthreads();
+  }
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) cm[t] = fmax(cm[t], cm[t + s]);
+    __syncthreads();
+  }
+  if (t != 0) return;
   // base pressure: corner average PG(1,1) (calccoef.f90:30-44).  The reference's far-field
   // value PG(NI,NJ/2) is not assigned by CALCCOEF (its loop covers I = 1..3); it holds what the
   // last field dump left there (writedata.f90:20), zero before the first dump, and zero here.
@@ -623,18 +663,7 @@ __global__ void k_calccoef2d(Dims d, const double* __restrict__ wall2d,
   const double dum_b = 0.25 * (p[cidx(d, 1, 1, k)] + p[cidx(d, 1, 0, k)] + p[cidx(d, 0, 1, k)] +
                                p[cidx(d, 0, 0, k)]);
   const double cpb = 2.0 * (dum_a - dum_b);
-  double pdrag = 0.0, plift = 0.0, tdrag = 0.0, tlift = 0.0;
-  for (int j = 1; j <= d.nj - 1; ++j) {
-    const double* o = wall2d + 6 * (j - 1);
-    const double area = o[0], dn1 = o[1], s2 = o[2], c2 = o[3], ct = o[4], st = o[5];
-    const long c = cidx(d, 1, j, k);
-    const double dudn = (u[c] * s2 - v[c] * c2 - vel_rot) / dn1;
-    pdrag = pdrag - 2.0 * p[c] * area * ct;
-    plift = plift - 2.0 * p[c] * area * st;
-    tdrag = tdrag + (2.0 / re) * dudn * area * st;
-    tlift = tlift - (2.0 / re) * dudn * area * ct;
-  }
-  out[0] = cmax;
+  out[0] = cm[0];
   out[1] = cpb;
   out[2] = pdrag + tdrag;
   out[3] = plift + tlift;
@@ -827,7 +856,7 @@ int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready) {
   if (!cfl_ready)
     hipLaunchKernelGGL(k_cfl, dim3(ncfl), dim3(256), 0, c->stream, c->d, (const double* const*)m,
                        c->u, c->v, c->w, c->prm.dtime, c->red);
-  hipLaunchKernelGGL(k_calccoef2d, dim3(1), dim3(64), 0, c->stream, c->d, c->wall2d, c->p, c->u,
+  hipLaunchKernelGGL(k_calccoef2d, dim3(1), dim3(256), 0, c->stream, c->d, c->wall2d, c->p, c->u,
                      c->v, c->prm.re, 0.0, cfl, ncfl, c->scal + 8);
   HIP_CHECK(hipGetLastError());
   if (out8) {
