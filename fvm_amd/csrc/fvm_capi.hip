@@ -575,8 +575,9 @@ int fvm_step(fvm_ctx* c, fvm_step_info* info) {
   CK(fvm_k_bcu_ghosts(c, f, 3));                  // :138-142
   if (c->thermal) CK(fvm_k_iter_t(c));            // :145-147 (ITER_T, BCT)
   if (c->two_d) {
-    CK(fvm_k_calccoef2d(c, nullptr, true));       // 2D CALCCOEF
-    r = fvm_fetch(c, c->scal, 17, c->hscal);
+    const Mailbox mb = fvm_mail(c);               // the 2D CALCCOEF publishes scal[0..16]
+    CK(fvm_k_calccoef2d(c, nullptr, true, &mb));  // 2D CALCCOEF
+    r = fvm_mail_wait(c, mb, 17, c->hscal);
   } else {
     const Mailbox mb = fvm_mail(c);               // the force kernel publishes scal[0..16]
     CK(fvm_k_calccoef(c, nullptr, true, &mb));    // :150

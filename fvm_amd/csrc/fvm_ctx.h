@@ -198,7 +198,8 @@ int fvm_k_correct_bcu_contra(FvmCtx* c);
 int fvm_k_pref(FvmCtx* c, double* pref_host);
 int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready = false,
                    const Mailbox* mb = nullptr);
-int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready = false);
+int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready = false,
+                     const Mailbox* mb = nullptr);
 int fvm_wall2d_setup(FvmCtx* c, const double* xg, const double* yg);
 int fvm_k_sgs(FvmCtx* c, bool sgs_out = true);
 int fvm_k_sgs_out(FvmCtx* c);

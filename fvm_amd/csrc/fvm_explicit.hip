@@ -618,7 +618,8 @@ __global__ void __launch_bounds__(256) k_calccoef2d(Dims d, const double* __rest
                                                     const double* __restrict__ v, double re,
                                                     double vel_rot,
                                                     const double* __restrict__ cflpart, int ncfl,
-                                                    double* out) {
+                                                    double* scal, Mailbox mb) {
+  double* out = scal + 8;
   __shared__ double tm[4][256];
   __shared__ double cm[256];
   const int t = threadIdx.x;
@@ -663,14 +664,19 @@ __global__ void __launch_bounds__(256) k_calccoef2d(Dims d, const double* __rest
   const double dum_b = 0.25 * (p[cidx(d, 1, 1, k)] + p[cidx(d, 1, 0, k)] + p[cidx(d, 0, 1, k)] +
                                p[cidx(d, 0, 0, k)]);
   const double cpb = 2.0 * (dum_a - dum_b);
-  out[0] = cm[0];
-  out[1] = cpb;
-  out[2] = pdrag + tdrag;
-  out[3] = plift + tlift;
-  out[4] = pdrag;
-  out[5] = tdrag;
-  out[6] = plift;
-  out[7] = tlift;
+  double sv[17];
+  for (int q = 0; q < 8; ++q) sv[q] = scal[q];  // the step's earlier scalars (stream order)
+  sv[8] = cm[0];
+  sv[9] = cpb;
+  sv[10] = pdrag + tdrag;
+  sv[11] = plift + tlift;
+  sv[12] = pdrag;
+  sv[13] = tdrag;
+  sv[14] = plift;
+  sv[15] = tlift;
+  sv[16] = scal[16];
+  for (int q = 0; q < 8; ++q) out[q] = sv[8 + q];
+  publish(mb, sv, 17);  // scal[0..16] to the host mailbox (no-op without one)
 }
 
 }  // namespace
@@ -848,7 +854,7 @@ int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready, const Mailbox* mb) {
   return FVM_OK;
 }
 
-int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready) {
+int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready, const Mailbox* mb) {
   double** m = c->dm;
   ProfScope ps(c, "calccoef");
   const int ncfl = cfl_ready ? c->ncflused : 512;
@@ -856,8 +862,9 @@ int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready) {
   if (!cfl_ready)
     hipLaunchKernelGGL(k_cfl, dim3(ncfl), dim3(256), 0, c->stream, c->d, (const double* const*)m,
                        c->u, c->v, c->w, c->prm.dtime, c->red);
+  const Mailbox none{nullptr, nullptr, 0};
   hipLaunchKernelGGL(k_calccoef2d, dim3(1), dim3(256), 0, c->stream, c->d, c->wall2d, c->p, c->u,
-                     c->v, c->prm.re, 0.0, cfl, ncfl, c->scal + 8);
+                     c->v, c->prm.re, 0.0, cfl, ncfl, c->scal, mb ? *mb : none);
   HIP_CHECK(hipGetLastError());
   if (out8) {
     HIP_CHECK(hipMemcpyAsync(out8, c->scal + 8, 8 * sizeof(double), hipMemcpyDeviceToHost,
