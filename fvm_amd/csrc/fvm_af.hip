@@ -765,7 +765,11 @@ int fvm_k_iter_u(FvmCtx* c) {
   if (d.ni - 1 > FVM_MAX_LINE || d.nj - 1 > FVM_MAX_LINE || d.nk - 1 > FVM_MAX_LINE)
     return FVM_ERR_SIZE;
   const dim3 gi((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
-  for (int s = 0; s < 3; ++s) {
+  // the 2D scheme has two sweeps per component, I then J (2D iter_u.f90 / iter_v.f90): the
+  // one-cell span of the 2D slab gets no K stage
+  const int nstage = c->two_d ? 2 : 3;
+  if (c->two_d) c->af_iters[2] = 0;
+  for (int s = 0; s < nstage; ++s) {
     AFArgs A;
     double* fld[3] = {c->u, c->v, c->w};
     const double* fn[3] = {c->un, c->vn, c->wn};
