@@ -627,7 +627,9 @@ __global__ void __launch_bounds__(256) k_calccoef2d(Dims d, const double* __rest
   double cmax = 0.0;
   for (int q = t; q < ncfl; q += 256) cmax = fmax(cmax, cflpart[q]);
   cm[t] = cmax;
-  double pdrag = 0.0, plift = 0.0, tdrag = 0.0, tlift = 0.0;
+  // lanes 0..3 each carry one of the four sums (pressure drag, pressure lift, viscous drag,
+  // viscous lift) through every wall cell in J order
+  double acc = 0.0;
   for (int j0 = 1; j0 <= d.nj - 1; j0 += 256) {
     const int j = j0 + t;
     if (j <= d.nj - 1) {
@@ -641,22 +643,22 @@ __global__ void __launch_bounds__(256) k_calccoef2d(Dims d, const double* __rest
       tm[3][t] = (2.0 / re) * dudn * area * ct;
     }
     __syncthreads();
-    if (t == 0) {
+    if (t < 4) {
       const int m = min(256, d.nj - j0);
-      for (int q = 0; q < m; ++q) {
-        pdrag = pdrag - tm[0][q];
-        plift = plift - tm[1][q];
-        tdrag = tdrag + tm[2][q];
-        tlift = tlift - tm[3][q];
-      }
+      if (t == 2)
+        for (int q = 0; q < m; ++q) acc = acc + tm[2][q];
+      else
+        for (int q = 0; q < m; ++q) acc = acc - tm[t][q];
     }
     __syncthreads();
   }
+  if (t < 4) tm[t][0] = acc;
   for (int s = 128; s > 0; s >>= 1) {
     if (t < s) cm[t] = fmax(cm[t], cm[t + s]);
     __syncthreads();
   }
   if (t != 0) return;
+  const double pdrag = tm[0][0], plift = tm[1][0], tdrag = tm[2][0], tlift = tm[3][0];
   // base pressure: corner average PG(1,1) (calccoef.f90:30-44).  The reference's far-field
   // value PG(NI,NJ/2) is not assigned by CALCCOEF (its loop covers I = 1..3); it holds what the
   // last field dump left there (writedata.f90:20), zero before the first dump, and zero here.
