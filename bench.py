@@ -168,6 +168,20 @@ def cpu_baseline(w, log):
                        f"restatement on 1 host core, {t1 - t0:.2f} s")
 
 
+def copy_calibration(mib, log):
+    """A plain device copy of a known size (torch's copy kernel, four times): under rocprofv3
+    --pmc FETCH_SIZE (or WRITE_SIZE) each copy dispatch reads (writes) exactly mib MiB, which
+    fixes the scale factor applied to the counters of the solver's kernels."""
+    import torch
+    n = mib * 1024 * 1024 // 8
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copied {mib} MiB x 4")
+
+
 def run_ensemble(a, log):
     """The ensemble of seeded 2D cases (uneven sizes, Re in [40, 300]) stepped together by
     fvm_ensemble_step; the same steps with one host thread give the one-case-at-a-time rate."""
@@ -239,6 +253,8 @@ def main():
                     help="advance the passive temperature too (not part of the headline step)")
     ap.add_argument("--cases", type=int, default=96, help="ensemble2d: number of 2D cases")
     ap.add_argument("--threads", type=int, default=8, help="ensemble2d: host threads")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB four times (HBM counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the solver holds "
@@ -246,6 +262,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration, log)
+        return
     if a.workload == "ensemble2d":
         run_ensemble(a, log)
         return

@@ -444,6 +444,11 @@ int fvm_ensemble_step(fvm_ctx* const* ctxs, int n, int nthreads, fvm_step_info* 
   if (!ctxs || n < 0 || nthreads < 1) return FVM_ERR_ARG;
   for (int q = 0; q < n; ++q)
     if (!ctxs[q]) return FVM_ERR_ARG;
+  {  // one context twice would be stepped by two threads at once on one stream and mailbox
+    std::vector<fvm_ctx*> sorted(ctxs, ctxs + n);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) return FVM_ERR_ARG;
+  }
   std::vector<int> status(n, FVM_OK);
   const int nt = std::min(nthreads, std::max(n, 1));
   // each thread takes the next case from a shared counter, so uneven cases balance and any

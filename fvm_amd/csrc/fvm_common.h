@@ -220,6 +220,14 @@ __global__ inline void k_ghost_fill_multi(Dims d, PtrSet8 set) {
   }
 }
 
+// Scaled-L1 residual sum|r| / sum|AP p| of ITER_P (iter_p.f90:45-59).  When both sums are
+// zero (p = 0 and a zero right-hand side) the reference's quotient is 0/0 = NaN, for which its
+// test 'RESIP .GT. RESIMAX' is false, so its sweep loop ends: that state is converged, and 0 is
+// returned for it.  Every other quotient, non-finite ones included, is returned as computed.
+__host__ __device__ inline double scaled_residual(double s0, double s1) {
+  return (s0 == 0.0 && s1 == 0.0) ? 0.0 : s0 / s1;
+}
+
 // 1D launch over n items.
 inline dim3 grid1(long n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
 

@@ -129,7 +129,7 @@ __global__ void k_p_res_final(const double* __restrict__ part, long nb, double* 
     if (t < s) { s0[t] += s0[t + s]; s1[t] += s1[t + s]; }
     __syncthreads();
   }
-  if (t == 0) { out[0] = s0[0]; out[1] = s1[0]; out[2] = s0[0] / s1[0]; }
+  if (t == 0) { out[0] = s0[0]; out[1] = s1[0]; out[2] = scaled_residual(s0[0], s1[0]); }
 }
 
 // BCP ghosts: wall p(0) = p(1), outflow cubic extrapolation (bcp.f90:22-32); the periodic

@@ -117,8 +117,9 @@ __global__ void k_res_final(const double* __restrict__ part, long nb, double* ou
     __syncthreads();
   }
   if (t == 0) {
-    out[0] = s0[0]; out[1] = s1[0]; out[2] = s0[0] / s1[0];
-    const double v[3] = {s0[0], s1[0], s0[0] / s1[0]};
+    const double q = scaled_residual(s0[0], s1[0]);
+    out[0] = s0[0]; out[1] = s1[0]; out[2] = q;
+    const double v[3] = {s0[0], s1[0], q};
     publish(mb, v, 3);
   }
 }

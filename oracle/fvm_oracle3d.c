@@ -814,6 +814,49 @@ void orc3d_iter_p(orc3d *o, int nswpp, double resimax) {
   }
 }
 
+/* The scaled-L1 residual of ITER_P (RLES/iter_p.f90:12-59) for the current p without a sweep:
+ * coefficients, DD(p), SU = div / dt - DD, BCP (folds and ghosts), then
+ * RESIP = sum |A_nb p_nb + SU - AP p| / sum |AP p| into o->resip.  p keeps its interior values. */
+void orc3d_p_residual(orc3d *o) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  long sj = o->sj, sk = o->sk;
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        o->ap[c] = -o->de[c] - o->dw[c] - o->dn[c] - o->ds[c] - o->dt[c] - o->db[c];
+        o->ae[c] = -o->de[c];
+        o->aw[c] = -o->dw[c];
+        o->an[c] = -o->dn[c];
+        o->as[c] = -o->ds[c];
+        o->at[c] = -o->dt[c];
+        o->ab[c] = -o->db[c];
+      }
+  double dtime = o->dtime;
+  orc3d_dif_diagonal(o, o->p, o->dd);
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        double dv = o->ue[c] - o->uw[c] + o->vnf[c] - o->vs[c] + o->wt[c] - o->wb[c];
+        o->su[c] = 1.0 / dtime * dv - o->dd[c];
+      }
+  orc3d_bcp(o);
+  double res = 0.0, dum = 0.0;
+  double *P = o->p;
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        double r = o->ae[c] * P[c + 1] + o->aw[c] * P[c - 1] + o->an[c] * P[c + sj] +
+                   o->as[c] * P[c - sj] + o->at[c] * P[c + sk] + o->ab[c] * P[c - sk] +
+                   o->su[c] - o->ap[c] * P[c];
+        res = res + fabs(r);
+        dum = dum + fabs(o->ap[c] * P[c]);
+      }
+  o->resip = res / dum;
+}
+
 /* TEST_FILTER(IN,OUT) (RLES/test_filter.f90:13-30): (1,2,1)^3/64 on index space. */
 void orc3d_test_filter(orc3d *o, const double *in, double *out) {
   long sj = o->sj, sk = o->sk;
@@ -1347,7 +1390,9 @@ void orc3d_post_finalize(orc3d *o) {
 
 /* The predictor half of a time step (RLES/main.f90:39-88): the step copies, NL, SGS, DIF,
  * P_SOURCE and the three AF solves ITER_U, leaving u*, v*, w* in u, v, w. */
-void orc3d_predictor(orc3d *o, int use_les) {
+/* Predictor, first part (RLES/main.f90:42-58): U_N = U, CONTRA_VEL, NL_NM1 = NL, NL(U_N..W_N)
+ * (the convective terms before the SGS divergence is added). */
+void orc3d_pred_nl(orc3d *o) {
   int ni = o->ni, nj = o->nj, nk = o->nk;
   long n = o->n;
   size_t b = n * sizeof(double);
@@ -1365,23 +1410,34 @@ void orc3d_predictor(orc3d *o, int use_les) {
   orc3d_nl(o, o->vn, o->nly);
   orc3d_nl(o, o->wn, o->nlz);
   if (o->thermal) orc3d_nl(o, o->tn, o->adv);
-  if (use_les) {
-    orc3d_sgs(o);
-    for (int k = 1; k <= nk - 1; ++k)
-      for (int j = 1; j <= nj - 1; ++j)
-        for (int i = 1; i <= ni - 1; ++i) {
-          long c = IX(o, i, j, k);
-          o->nlx[c] = o->nlx[c] + o->sgsx[c] * o->djac[c];
-          o->nly[c] = o->nly[c] + o->sgsy[c] * o->djac[c];
-          o->nlz[c] = o->nlz[c] + o->sgsz[c] * o->djac[c];
-          if (o->thermal) o->adv[c] = o->adv[c] + o->sgst[c] * o->djac[c];
-        }
-  }
+}
+
+/* STRESS_TENSOR_DY and NL += J^-1 SGS (RLES/main.f90:59-69). */
+void orc3d_pred_sgs(orc3d *o) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  orc3d_sgs(o);
+  for (int k = 1; k <= nk - 1; ++k)
+    for (int j = 1; j <= nj - 1; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        long c = IX(o, i, j, k);
+        o->nlx[c] = o->nlx[c] + o->sgsx[c] * o->djac[c];
+        o->nly[c] = o->nly[c] + o->sgsy[c] * o->djac[c];
+        o->nlz[c] = o->nlz[c] + o->sgsz[c] * o->djac[c];
+        if (o->thermal) o->adv[c] = o->adv[c] + o->sgst[c] * o->djac[c];
+      }
+}
+
+/* DIF(U_N..W_N) and P_SOURCE(P) (RLES/main.f90:72-78). */
+void orc3d_pred_dif_ps(orc3d *o) {
   orc3d_dif(o, o->un, o->difx);
   orc3d_dif(o, o->vn, o->dify);
   orc3d_dif(o, o->wn, o->difz);
   if (o->thermal) orc3d_dif(o, o->tn, o->dift);
   orc3d_p_source(o, o->p);
+}
+
+/* ITER_U for U, V, W (RLES/main.f90:82-88). */
+void orc3d_pred_iter_u(orc3d *o) {
   o->uin = 1.0;
   orc3d_iter_u(o, o->u, o->un, o->nlx, o->nlx1, o->difx, o->s_u);
   o->uin = 0.0;
@@ -1389,12 +1445,17 @@ void orc3d_predictor(orc3d *o, int use_les) {
   orc3d_iter_u(o, o->w, o->wn, o->nlz, o->nlz1, o->difz, o->s_w);
 }
 
-/* One time step of the reference time loop (RLES/main.f90:39-150, file output excluded; the
- * passive temperature when thermal is set). */
-void orc3d_step(orc3d *o, int nswpp, double resimax, int use_les) {
+/* The step up to and including the three ITER_U solves (RLES/main.f90:42-88). */
+void orc3d_predictor(orc3d *o, int use_les) {
+  orc3d_pred_nl(o);
+  if (use_les) orc3d_pred_sgs(o);
+  orc3d_pred_dif_ps(o);
+  orc3d_pred_iter_u(o);
+}
+
+/* Predictor correction u^ = u* - dt S / J, BCU x3 and CONTRA_VEL (RLES/main.f90:90-101). */
+void orc3d_correct_predictor(orc3d *o) {
   int ni = o->ni, nj = o->nj, nk = o->nk;
-  long n = o->n;
-  orc3d_predictor(o, use_les);
   double dt = o->dtime;
   for (int k = 1; k <= nk - 1; ++k)
     for (int j = 1; j <= nj - 1; ++j)
@@ -1407,7 +1468,14 @@ void orc3d_step(orc3d *o, int nswpp, double resimax, int use_les) {
   o->uin = 1.0; orc3d_bcu(o, o->u, o->un);
   o->uin = 0.0; orc3d_bcu(o, o->v, o->vn); orc3d_bcu(o, o->w, o->wn);
   orc3d_contra_vel(o);
-  orc3d_iter_p(o, nswpp, resimax);
+}
+
+/* The step after ITER_P (RLES/main.f90:105-150): BCP, P_SOURCE(P), projection, pressure
+ * reference, BCU x3, [ITER_T, BCT,] CALCCOEF. */
+void orc3d_finish_step(orc3d *o) {
+  int ni = o->ni, nj = o->nj, nk = o->nk;
+  long n = o->n;
+  double dt = o->dtime;
   orc3d_bcp(o);
   orc3d_p_source(o, o->p);
   for (int k = 1; k <= nk - 1; ++k)
@@ -1432,6 +1500,15 @@ void orc3d_step(orc3d *o, int nswpp, double resimax, int use_les) {
     orc3d_bct(o, o->t, o->tn);
   }
   orc3d_calccoef(o);
+}
+
+/* One time step of the reference time loop (RLES/main.f90:39-150, file output excluded; the
+ * passive temperature when thermal is set). */
+void orc3d_step(orc3d *o, int nswpp, double resimax, int use_les) {
+  orc3d_predictor(o, use_les);
+  orc3d_correct_predictor(o);
+  orc3d_iter_p(o, nswpp, resimax);
+  orc3d_finish_step(o);
 }
 
 /* ----------------------------------------------------------------------------------------- */

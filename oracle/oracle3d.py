@@ -38,7 +38,9 @@ def lib():
         L.orc3d_post_finalize.argtypes = [vp]
         L.orc3d_set_dtime.argtypes = [vp, cd]
         L.orc3d_set_no_pref.argtypes = [vp, ci]
-        for f in ("orc3d_contra_vel", "orc3d_sgs", "orc3d_bcp", "orc3d_lisolv", "orc3d_calccoef"):
+        for f in ("orc3d_contra_vel", "orc3d_sgs", "orc3d_bcp", "orc3d_lisolv", "orc3d_calccoef",
+                  "orc3d_pred_nl", "orc3d_pred_sgs", "orc3d_pred_dif_ps", "orc3d_pred_iter_u",
+                  "orc3d_correct_predictor", "orc3d_finish_step", "orc3d_p_residual"):
             getattr(L, f).argtypes = [vp]
         L.orc3d_iter_p.argtypes = [vp, ci, cd]
         L.orc3d_nl.argtypes = [vp, vp, vp]
@@ -119,6 +121,32 @@ class Oracle3D:
     def predictor(self, use_les=True):
         """The step up to and including the three ITER_U solves (u*, v*, w* left in u, v, w)."""
         self.L.orc3d_predictor(self.h, 1 if use_les else 0)
+
+    # the predictor in its stages (RLES/main.f90:42-58, 59-69, 72-78, 82-88)
+    def pred_nl(self):
+        self.L.orc3d_pred_nl(self.h)
+
+    def pred_sgs(self):
+        self.L.orc3d_pred_sgs(self.h)
+
+    def pred_dif_ps(self):
+        self.L.orc3d_pred_dif_ps(self.h)
+
+    def pred_iter_u(self):
+        self.L.orc3d_pred_iter_u(self.h)
+
+    def correct_predictor(self):
+        """u^ = u* - dt S / J, BCU, CONTRA_VEL (RLES/main.f90:90-101)."""
+        self.L.orc3d_correct_predictor(self.h)
+
+    def p_residual(self):
+        """ITER_P's scaled-L1 residual of the current p, no sweep (iter_p.f90:12-59)."""
+        self.L.orc3d_p_residual(self.h)
+        return self.scalar("resip")
+
+    def finish_step(self):
+        """BCP, P_SOURCE, projection, pressure reference, BCU, CALCCOEF (main.f90:105-150)."""
+        self.L.orc3d_finish_step(self.h)
 
     def post_begin(self):
         """POST_PROCESS's start: zero the time-average sums; CALCCOEF then accumulates."""

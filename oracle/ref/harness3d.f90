@@ -7,7 +7,11 @@
 !  module arrays in raw binary (stream access, column-major, ghosts included) so the
 !  project's CPU restatement and GPU kernels can be compared with the reference stage by stage.
 !
-!  usage: harness3d NSTEP [NPOST]   (run directory holds NNNxNNNxNNN.dat and RESTART000.DAT)
+!  usage: harness3d NSTEP [NPOST [NPRE]]   (run directory holds NNNxNNNxNNN.dat and
+!         RESTART000.DAT)
+!
+!  With NPRE > 0 the first NPRE steps run at DTIME/10 (a start-up from the impulsive start; the
+!  reference reassigns DTIME at run time the same way, post_process.f90:59), the rest at DTIME.
 !
 !  With NPOST > 0 the last NPOST steps run in the time-averaging phase of POST_PROCESS (its
 !  first call before step NSTEP-NPOST+1: sums zeroed, DTIME = DTIME_POST; CALCCOEF accumulates;
@@ -18,8 +22,8 @@ PROGRAM HARNESS3D
   USE COMDAT_SHARED
   IMPLICIT NONE
   INCLUDE 'mpif.h'
-  INTEGER :: NSTEP, ISTEP, NPOST, IST
-  DOUBLE PRECISION :: PREF
+  INTEGER :: NSTEP, ISTEP, NPOST, IST, NPRE
+  DOUBLE PRECISION :: PREF, DT_RUN
   CHARACTER(64) :: ARG
 
   CALL GET_COMMAND_ARGUMENT(1, ARG)
@@ -27,6 +31,9 @@ PROGRAM HARNESS3D
   NPOST = 0
   CALL GET_COMMAND_ARGUMENT(2, ARG, STATUS=IST)
   IF (IST .EQ. 0 .AND. LEN_TRIM(ARG) .GT. 0) READ(ARG, *) NPOST
+  NPRE = 0
+  CALL GET_COMMAND_ARGUMENT(3, ARG, STATUS=IST)
+  IF (IST .EQ. 0 .AND. LEN_TRIM(ARG) .GT. 0) READ(ARG, *) NPRE
 
   ! files the reference appends to with STATUS='OLD'
   OPEN(UNIT=11, FILE='NU_CD_CL.PLT', STATUS='REPLACE'); CLOSE(11)
@@ -49,7 +56,12 @@ PROGRAM HARNESS3D
   CALL DUMP('setup_theta', THETA)
   WRITE(61, '(A,2ES26.17E3)') 'area ', SUM(AREA1), SUM(AREA2)
 
+  DT_RUN = DTIME
   DO ISTEP = 1, NSTEP
+    IF (NPRE .GT. 0) THEN
+      DTIME = DT_RUN
+      IF (ISTEP .LE. NPRE) DTIME = 0.1D0*DT_RUN
+    ENDIF
     IF (NPOST .GT. 0 .AND. ISTEP .EQ. NSTEP - NPOST + 1) CALL POST_PROCESS
     NITER_TIME = NITER_TIME + 1
     RTIME = RTIME + DTIME

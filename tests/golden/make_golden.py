@@ -55,6 +55,23 @@ CASES = {
                                  re=200.0, dt=4e-3, nswpp=200000, resimax="1.0D-13",
                                  les=1, steps=4, npost=3, keep_steps=(4,), t_sigma=0.05,
                                  keep_tags=("u", "v", "w", "pref", "t", "nut")),
+    # the 192 x 256 x 64 workload's wall spacing (0.001 D), outer radius, Re and seeded start on a
+    # 32 x 64 x 16 grid, started like the benchmark: 20 steps at dt/10, then two steps at
+    # dt = 0.002, every Poisson solve to 1e-10.  Kept: the state after the start-up (the GPU
+    # starts from it) and the two steps at the workload's dt
+    "rles_33x65x17_re3900_warm": dict(ni=33, nj=65, nk=17, h1=0.001, r_out=25.0, sigma=0.05,
+                                      re=3900.0, dt=2e-3, nswpp=2000000, resimax="1.0D-10",
+                                      les=1, steps=22, npre=20,
+                                      keep={20: ("u", "v", "w", "pref", "nlx", "nly", "nlz"),
+                                            21: ("ustar", "vstar", "wstar", "u", "v", "w",
+                                                 "pref"),
+                                            22: ("u", "v", "w", "pref")}),
+    # a wavy cylinder: radius 0.5 (1 + 0.08 sin(2 pi z / LZ)), so xi_z, eta_z and the
+    # stage-coupling metric terms are not zero and the face coefficients vary along the span
+    # (not an extruded, separable grid); Poisson to 1e-12, 2 steps
+    "rles_13x17x9_wavy_converged": dict(ni=13, nj=17, nk=9, h1=0.05, r_out=5.0, sigma=0.05,
+                                        re=200.0, dt=4e-3, nswpp=2000000, resimax="1.0D-12",
+                                        les=1, steps=2, keep_steps=(1, 2), wavy=0.08),
     # (no USE_LES=0 fixture: the reference's laminar mode writes YPLUS in CALCCOEF
     # (calccoef.f90:121), an array INIT allocates only when USE_LES=1 (init.f90:79-81))
 }
@@ -117,11 +134,21 @@ def post_fixture(out, d, ni, nj, nk, npost):
                                              "Surf000.PLT")])
 
 
+def wavy_nodes(g, amp):
+    """The O-grid of g with every node's radius scaled by 1 + amp sin(2 pi z / LZ) at the wall,
+    relaxing linearly in the radial index to 1 at the outer boundary (periodic in z)."""
+    xg, yg, zg = g.nodes()
+    s = np.sin(2.0 * np.pi * zg / g.lz)
+    w = 1.0 - np.arange(g.ni, dtype=np.float64)[:, None, None] / (g.ni - 1)
+    f = 1.0 + amp * s * w
+    return np.asfortranarray(xg * f), np.asfortranarray(yg * f), zg
+
+
 def run_case(label, c):
     exe = build(label, c)
     ni, nj, nk = c["ni"], c["nj"], c["nk"]
     g = OGrid(ni, nj, nk, c["h1"], c["r_out"])
-    xg, yg, zg = g.nodes()
+    xg, yg, zg = wavy_nodes(g, c["wavy"]) if c.get("wavy") else g.nodes()
     ic = seeded_ic(ni, nj, nk, c["sigma"])
     if c.get("t_sigma"):
         ic["t"] = seeded_temperature(ni, nj, nk, c["t_sigma"])
@@ -139,13 +166,18 @@ def run_case(label, c):
         if c.get("t_sigma"):
             out["in_t"], out["in_adv"], out["pe"] = rs["t"], rs["adv"], 2769.0
         args = [exe, str(c["steps"])] + ([str(c["npost"])] if c.get("npost") else [])
+        if c.get("npre"):
+            args = [exe, str(c["steps"]), "0", str(c["npre"])]
+            out["npre"] = c["npre"]
         subprocess.run(args, cwd=d, check=True, stdout=subprocess.DEVNULL)
         rd = lambda t: np.fromfile(os.path.join(d, t + ".bin")).reshape(shape, order="F")
-        setup_tags = SETUP_TAGS if ("keep_tags" not in c or c.get("t_sigma")) else ()
+        setup_tags = SETUP_TAGS if (("keep_tags" not in c and "keep" not in c)
+                                    or c.get("t_sigma")) else ()
         for t in setup_tags:
             out["setup_" + t] = rd("setup_" + t)
-        for s in c["keep_steps"]:
-            for t in c.get("keep_tags", STEP_TAGS):
+        keep = c.get("keep") or {s: c.get("keep_tags", STEP_TAGS) for s in c["keep_steps"]}
+        for s, tags in keep.items():
+            for t in tags:
                 p = os.path.join(d, f"s{s:03d}_{t}.bin")
                 if os.path.exists(p):
                     out[f"s{s}_{t}"] = rd(f"s{s:03d}_{t}")
