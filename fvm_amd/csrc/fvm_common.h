@@ -194,7 +194,7 @@ struct PtrSet8 {
 
 // the periodic ghost copy of several arrays in one launch (blockIdx.y selects the array); every
 // ghost reads an interior (J, K) cell only, so the arrays and the cells are independent
-__global__ inline void k_ghost_fill_multi(Dims d, PtrSet8 set) {
+static __global__ void k_ghost_fill_multi(Dims d, PtrSet8 set) {
   double* __restrict__ a = set.p[blockIdx.y];
   const long nJ = (long)(d.ni + 1) * 2 * (d.nk + 1);
   const long nK = (long)(d.ni + 1) * (d.nj + 1) * 2;

@@ -7,20 +7,20 @@
 // so the orthogonal part L_D is diagonalised by real Fourier bases in J (periodic branch cut) and
 // K (periodic span): for every mode pair (mJ, mK) what is left is ONE radial tridiagonal line
 //   dW x(i-1) - (dE + dW + lamJ dN + lamK dT) x(i) + dE x(i+1) = r_hat(i),
-// lam = 4 sin^2(pi m / N).  The transforms are dense orthonormal DFT matrices applied as GEMMs
-// (rocBLAS, MFMA) on the compact interior array; the radial lines are solved by the batched
-// 64-lane PCR line kernel.  The outer loop keeps the reference's FP64 operator -- cross terms,
+// lam = 4 sin^2(pi m / N).  The transforms are this library's own mixed-radix FFTs in LDS
+// (fvm_fft.h): a real-to-half-complex transform along J that packs two radial columns into one
+// complex sequence, a complex transform along K; the radial mode lines are solved by 64-lane
+// affine-map scans, and the inverse J transform adds the correction to p in the same kernel.  The outer loop keeps the reference's FP64 operator -- cross terms,
 // boundary folds, periodic ghosts -- in the residual r = rhs - L p and its scaled-L1 norm
 // sum|r| / sum|AP p| (iter_p.f90:45-59), so it converges to the same fixed point as the
 // reference's line Gauss-Seidel; the approximate inverse runs in FP64 (plain form) or FP32
 // (mixed form).  Coefficients are averaged over J and K; their largest deviation from the
 // average is measured at set-up (fvm_ctx.sep_dev).
-#include <rocblas/rocblas.h>
-
 #include <type_traits>
 #include <vector>
 
 #include "fvm_ctx.h"
+#include "fvm_fft.h"
 
 namespace {
 
@@ -128,16 +128,17 @@ __global__ void k_res_final(const double* __restrict__ part, long nb, double* ou
 // lo = <D_W>, up = <D_E>, dg = -(<D_E> + <D_W>) - lambda_J <D_N> - lambda_K <D_T> (the J, K
 // second differences diagonalised, lambda = 4 sin^2(pi m / N)), the wall Neumann fold at r = 0
 // and a zero ghost beyond the last row.  Its Thomas factors 1/D(r) and B(r) = up(r)/D(r),
-// D(r) = dg(r) - lo(r) B(r-1), are stored once in the layout of the transformed array
-// [r + N1 (mJ + NJ1 mK)]; one thread per line (set-up only).
-__global__ void k_mode_factor(int n, int nj1, int nk1, const double* __restrict__ dE,
+// D(r) = dg(r) - lo(r) B(r-1), are stored once in the layout of the spectrum
+// [r + N1 (mJ + MJ mK)] (mJ = 0..NJ1/2, mK = 0..NK1-1); one thread per line (set-up only).
+__global__ void k_mode_factor(int n, int mj_n, int nj1, int nk1, const double* __restrict__ dE,
                               const double* __restrict__ dW, const double* __restrict__ dN,
-                              const double* __restrict__ dT, const double* __restrict__ lamJ,
-                              const double* __restrict__ lamK, double* invd, double* bf) {
+                              const double* __restrict__ dT, double* invd, double* bf) {
   const int mj = blockIdx.x * 64 + threadIdx.x, mk = blockIdx.y;
-  if (mj >= nj1) return;
-  const double lj = lamJ[mj], lk = lamK[mk];
-  const long base = (long)n * (mj + (long)nj1 * mk);
+  if (mj >= mj_n) return;
+  const double pi = 3.14159265358979323846;
+  const double sj = sin(pi * (double)mj / (double)nj1), sk = sin(pi * (double)mk / (double)nk1);
+  const double lj = 4.0 * sj * sj, lk = 4.0 * sk * sk;
+  const long base = (long)n * (mj + (long)mj_n * mk);
   double bprev = 0.0;
   for (int r = 0; r < n; ++r) {
     const double lo = r == 0 ? 0.0 : dW[r];
@@ -153,92 +154,204 @@ __global__ void k_mode_factor(int n, int nj1, int nk1, const double* __restrict_
   }
 }
 
-// Radial mode-line solves with the stored factors, in FP64 whatever the storage type T: one
-// 64-lane wavefront per line, lane l owning rows l*R .. l*R+R-1; forward elimination
-// y(r) = z(r)/D(r) - lo(r)/D(r) y(r-1) and back substitution x(r) = y(r) - B(r) x(r+1) as
-// affine-map scans across the lanes (shuffles, no LDS).
+// Radial mode-line solves of the complex spectrum with the stored factors, in FP64 whatever the
+// storage type T: one 64-lane wavefront per line, lane l owning rows l*R .. l*R+R-1; forward
+// elimination y(r) = z(r)/D(r) - lo(r)/D(r) y(r-1) and back substitution
+// x(r) = y(r) - B(r) x(r+1) as affine-map scans across the lanes (shuffles, no LDS); the real
+// and imaginary parts share the real line operator.
 template <typename T, int R>
-__global__ void __launch_bounds__(64) k_mode_scan(int n, const double* __restrict__ dW,
-                                                  const double* __restrict__ invd,
-                                                  const double* __restrict__ bf, T* Z) {
+__global__ void __launch_bounds__(64) k_mode_scan_c(int n, const double* __restrict__ dW,
+                                                    const double* __restrict__ invd,
+                                                    const double* __restrict__ bf, Cx<T>* Z) {
   const long base = (long)n * blockIdx.x;
   const int l = threadIdx.x;
   const int r0 = l * R;
-  double al[R], be[R], bb[R];
-  double At = 0.0, Bt = 1.0;
+  double alx[R], aly[R], be[R], bb[R];
+  double Ax = 0.0, Ay = 0.0, Bt = 1.0;
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int r = r0 + q;
     if (r < n) {
       const double iv = invd[base + r];
-      al[q] = (double)Z[base + r] * iv;
+      const Cx<T> z = Z[base + r];
+      alx[q] = (double)z.x * iv;
+      aly[q] = (double)z.y * iv;
       be[q] = r == 0 ? 0.0 : -dW[r] * iv;
       bb[q] = bf[base + r];
-      At = al[q] + be[q] * At;
+      Ax = alx[q] + be[q] * Ax;
+      Ay = aly[q] + be[q] * Ay;
       Bt = be[q] * Bt;
     } else {
-      al[q] = 0.0; be[q] = 0.0; bb[q] = 0.0;
+      alx[q] = 0.0; aly[q] = 0.0; be[q] = 0.0; bb[q] = 0.0;
     }
   }
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    const double Ao = __shfl_up(At, o, 64), Bo = __shfl_up(Bt, o, 64);
-    if (l >= o) { At = At + Bt * Ao; Bt = Bt * Bo; }
+    const double Aox = __shfl_up(Ax, o, 64), Aoy = __shfl_up(Ay, o, 64),
+                 Bo = __shfl_up(Bt, o, 64);
+    if (l >= o) { Ax = Ax + Bt * Aox; Ay = Ay + Bt * Aoy; Bt = Bt * Bo; }
   }
-  double yin = __shfl_up(At, 1, 64);  // y(-1) = 0: only the offset part survives
-  if (l == 0) yin = 0.0;
-  double y[R];
+  double yinx = __shfl_up(Ax, 1, 64), yiny = __shfl_up(Ay, 1, 64);  // y(-1) = 0
+  if (l == 0) { yinx = 0.0; yiny = 0.0; }
+  double yx[R], yy[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
-    y[q] = al[q] + be[q] * yin;
-    if (r0 + q < n) yin = y[q];
+    yx[q] = alx[q] + be[q] * yinx;
+    yy[q] = aly[q] + be[q] * yiny;
+    if (r0 + q < n) { yinx = yx[q]; yiny = yy[q]; }
   }
-  double Ct = 1.0, Dt = 0.0;
+  double Ct = 1.0, Dx = 0.0, Dy = 0.0;
 #pragma unroll
   for (int q = R - 1; q >= 0; --q) {
     if (r0 + q < n) {
-      Dt = y[q] - bb[q] * Dt;
+      Dx = yx[q] - bb[q] * Dx;
+      Dy = yy[q] - bb[q] * Dy;
       Ct = -bb[q] * Ct;
     }
   }
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    const double Do = __shfl_down(Dt, o, 64), Co = __shfl_down(Ct, o, 64);
-    if (l + o < 64) { Dt = Dt + Ct * Do; Ct = Ct * Co; }
+    const double Dox = __shfl_down(Dx, o, 64), Doy = __shfl_down(Dy, o, 64),
+                 Co = __shfl_down(Ct, o, 64);
+    if (l + o < 64) { Dx = Dx + Ct * Dox; Dy = Dy + Ct * Doy; Ct = Ct * Co; }
   }
-  double xin = __shfl_down(Dt, 1, 64);  // x(n) = 0
-  if (l == 63) xin = 0.0;
+  double xinx = __shfl_down(Dx, 1, 64), xiny = __shfl_down(Dy, 1, 64);  // x(n) = 0
+  if (l == 63) { xinx = 0.0; xiny = 0.0; }
 #pragma unroll
   for (int q = R - 1; q >= 0; --q) {
     const int r = r0 + q;
     if (r < n) {
-      const double x = y[q] - bb[q] * xin;
-      xin = x;
-      Z[base + r] = (T)x;
+      const double xx = yx[q] - bb[q] * xinx, xy = yy[q] - bb[q] * xiny;
+      xinx = xx;
+      xiny = xy;
+      Z[base + r] = Cx<T>{(T)xx, (T)xy};
     }
   }
 }
 
 template <typename T>
-void mode_lines(int n1, int nlines, const double* dW, const double* invd, const double* bf, T* Z,
-                hipStream_t st) {
+void mode_lines(int n1, int nlines, const double* dW, const double* invd, const double* bf,
+                Cx<T>* Z, hipStream_t st) {
   const int R = (n1 + 63) / 64;
-  if (R <= 3) hipLaunchKernelGGL((k_mode_scan<T, 3>), dim3(nlines), dim3(64), 0, st, n1, dW, invd, bf, Z);
-  else if (R <= 4) hipLaunchKernelGGL((k_mode_scan<T, 4>), dim3(nlines), dim3(64), 0, st, n1, dW, invd, bf, Z);
-  else if (R <= 6) hipLaunchKernelGGL((k_mode_scan<T, 6>), dim3(nlines), dim3(64), 0, st, n1, dW, invd, bf, Z);
-  else hipLaunchKernelGGL((k_mode_scan<T, 8>), dim3(nlines), dim3(64), 0, st, n1, dW, invd, bf, Z);
+  if (R <= 3) hipLaunchKernelGGL((k_mode_scan_c<T, 3>), dim3(nlines), dim3(64), 0, st, n1, dW, invd, bf, Z);
+  else if (R <= 4) hipLaunchKernelGGL((k_mode_scan_c<T, 4>), dim3(nlines), dim3(64), 0, st, n1, dW, invd, bf, Z);
+  else if (R <= 6) hipLaunchKernelGGL((k_mode_scan_c<T, 6>), dim3(nlines), dim3(64), 0, st, n1, dW, invd, bf, Z);
+  else hipLaunchKernelGGL((k_mode_scan_c<T, 8>), dim3(nlines), dim3(64), 0, st, n1, dW, invd, bf, Z);
 }
 
+// Forward real transform along J of the compact residual R[i + N1 (j + NJ1 k)] for a tile of
+// 2 ncol radial indices at one k: columns i and i+1 are packed into one complex sequence
+// z_j = R(i) + i R(i+1), transformed in LDS, and split into the half spectra
+// A_m = (Z_m + conj Z_{N-m}) / 2, B_m = (Z_m - conj Z_{N-m}) / 2i, m = 0..N/2, written to
+// S[i + N1 (m + MJ k)].  LDS: two buffers of ncol N complex values, then the twiddles.
 template <typename T>
-__global__ void k_add_compact(Dims d, double* p, const T* __restrict__ X) {
-  const uint3 bb = xcd_block();
-  const int i = 1 + bb.x * 64 + threadIdx.x;
-  const int j = 1 + bb.y * 4 + threadIdx.y;
-  const int k = 1 + bb.z;
-  if (i > d.ni - 1 || j > d.nj - 1) return;
-  const long c = cidx(d, i, j, k);
-  const long ci = (long)(i - 1) + (long)(d.ni - 1) * ((j - 1) + (long)(d.nj - 1) * (k - 1));
-  p[c] = p[c] + (double)X[ci];
+__global__ void __launch_bounds__(256) k_fft_j_fwd(int n1, int nk1, FftPlan pl, int ncol,
+                                                   const T* __restrict__ R,
+                                                   const Cx<T>* __restrict__ twg, Cx<T>* S) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = pl.n, MJ = N / 2 + 1;
+  Cx<T>* a = (Cx<T>*)smem;
+  Cx<T>* b = a + (size_t)ncol * N;
+  Cx<T>* tw = b + (size_t)ncol * N;
+  const int i0 = blockIdx.x * 2 * ncol, k = blockIdx.y;
+  for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = twg[t];
+  for (int q = threadIdx.x; q < ncol * N; q += blockDim.x) {
+    const int col = q % ncol, j = q / ncol;
+    const int i = i0 + 2 * col;
+    Cx<T> z{(T)0, (T)0};
+    if (i < n1) {
+      const long o = (long)i + (long)n1 * (j + (long)N * k);
+      z.x = R[o];
+      z.y = i + 1 < n1 ? R[o + 1] : (T)0;
+    }
+    a[j * ncol + col] = z;
+  }
+  __syncthreads();
+  const Cx<T>* Z = fft_lds<T, -1>(a, b, ncol, pl, tw);
+  for (int q = threadIdx.x; q < ncol * MJ; q += blockDim.x) {
+    const int col = q % ncol, m = q / ncol;
+    const int i = i0 + 2 * col;
+    if (i >= n1) continue;
+    const Cx<T> zm = Z[m * ncol + col], zc = Z[((N - m) % N) * ncol + col];
+    // A = (zm + conj zc) / 2, B = (zm - conj zc) / (2 i)
+    const Cx<T> A{(T)0.5 * (zm.x + zc.x), (T)0.5 * (zm.y - zc.y)};
+    const Cx<T> B{(T)0.5 * (zm.y + zc.y), (T)0.5 * (zc.x - zm.x)};
+    const long o = (long)i + (long)n1 * (m + (long)MJ * k);
+    S[o] = A;
+    if (i + 1 < n1) S[o + 1] = B;
+  }
+}
+
+// Complex transform along K of S[i + N1 (mJ + MJ k)] for ncol consecutive radial indices at one
+// mJ, in place.
+template <typename T, int SIGN>
+__global__ void __launch_bounds__(256) k_fft_k(int n1, int MJ, FftPlan pl, int ncol,
+                                               const Cx<T>* __restrict__ twg, Cx<T>* S) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = pl.n;
+  Cx<T>* a = (Cx<T>*)smem;
+  Cx<T>* b = a + (size_t)ncol * N;
+  Cx<T>* tw = b + (size_t)ncol * N;
+  const int i0 = blockIdx.x * ncol, mj = blockIdx.y;
+  for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = twg[t];
+  for (int q = threadIdx.x; q < ncol * N; q += blockDim.x) {
+    const int col = q % ncol, k = q / ncol;
+    const int i = i0 + col;
+    Cx<T> z{(T)0, (T)0};
+    if (i < n1) z = S[(long)i + (long)n1 * (mj + (long)MJ * k)];
+    a[k * ncol + col] = z;
+  }
+  __syncthreads();
+  const Cx<T>* Z = fft_lds<T, SIGN>(a, b, ncol, pl, tw);
+  for (int q = threadIdx.x; q < ncol * N; q += blockDim.x) {
+    const int col = q % ncol, k = q / ncol;
+    const int i = i0 + col;
+    if (i < n1) S[(long)i + (long)n1 * (mj + (long)MJ * k)] = Z[k * ncol + col];
+  }
+}
+
+// Inverse of k_fft_j_fwd fused with the correction: the half spectra of columns i and i+1 at one
+// k become Z_m = A_m + i B_m (m = 0..N/2, with the imaginary parts of A_0, B_0, A_{N/2},
+// B_{N/2} dropped: a real sequence has real coefficients there) and Z_{N-m} = conj A_m +
+// i conj B_m; after the inverse transform, p(i+1, j+1, k+1) += scale Re z_j and
+// p(i+2, j+1, k+1) += scale Im z_j (scale = 1 / (NJ1 NK1)).
+template <typename T>
+__global__ void __launch_bounds__(256) k_fft_j_inv_add(Dims d, FftPlan pl, int ncol,
+                                                       const Cx<T>* __restrict__ S,
+                                                       const Cx<T>* __restrict__ twg,
+                                                       double scale, double* p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = pl.n, MJ = N / 2 + 1;
+  const int n1 = d.ni - 1;
+  Cx<T>* a = (Cx<T>*)smem;
+  Cx<T>* b = a + (size_t)ncol * N;
+  Cx<T>* tw = b + (size_t)ncol * N;
+  const int i0 = blockIdx.x * 2 * ncol, k = blockIdx.y;
+  for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = twg[t];
+  for (int q = threadIdx.x; q < ncol * MJ; q += blockDim.x) {
+    const int col = q % ncol, m = q / ncol;
+    const int i = i0 + 2 * col;
+    Cx<T> A{(T)0, (T)0}, B{(T)0, (T)0};
+    if (i < n1) {
+      const long o = (long)i + (long)n1 * (m + (long)MJ * k);
+      A = S[o];
+      if (i + 1 < n1) B = S[o + 1];
+    }
+    if (m == 0 || 2 * m == N) { A.y = (T)0; B.y = (T)0; }
+    a[m * ncol + col] = Cx<T>{A.x - B.y, A.y + B.x};
+    if (m > 0 && 2 * m < N) a[(N - m) * ncol + col] = Cx<T>{A.x + B.y, B.x - A.y};
+  }
+  __syncthreads();
+  const Cx<T>* Z = fft_lds<T, +1>(a, b, ncol, pl, tw);
+  for (int q = threadIdx.x; q < ncol * N; q += blockDim.x) {
+    const int col = q % ncol, j = q / ncol;
+    const int i = i0 + 2 * col;
+    if (i >= n1) continue;
+    const Cx<T> z = Z[j * ncol + col];
+    const long c = cidx(d, i + 1, j + 1, k + 1);
+    p[c] = p[c] + scale * (double)z.x;
+    if (i + 1 < n1) p[c + 1] = p[c + 1] + scale * (double)z.y;
+  }
 }
 
 // J,K-averaged face coefficients per radial index and their largest relative deviation.
@@ -292,196 +405,150 @@ __global__ void k_avg_faces(Dims d, const double* __restrict__ pdI, const double
   }
 }
 
-// orthonormal real Fourier basis of length N (rows: cos 0, cos 1, sin 1, ..., [cos N/2]) and the
-// eigenvalues 4 sin^2(pi m / N) of the periodic second difference
-void fourier_basis(int N, std::vector<double>& F, std::vector<double>& lam) {
-  F.assign((size_t)N * N, 0.0);  // column-major: F[row + N*col]
-  lam.assign(N, 0.0);
-  const double pi = 4.0 * atan(1.0);
-  int row = 0;
-  auto put = [&](int m, bool sine, double scale) {
-    for (int j = 0; j < N; ++j) {
-      double a = 2.0 * pi * (double)m * (double)j / (double)N;
-      F[row + (size_t)N * j] = scale * (sine ? sin(a) : cos(a));
-    }
-    double sm = sin(pi * (double)m / (double)N);
-    lam[row] = 4.0 * sm * sm;
-    ++row;
-  };
-  put(0, false, 1.0 / sqrt((double)N));
-  for (int m = 1; 2 * m < N; ++m) {
-    put(m, false, sqrt(2.0 / N));
-    put(m, true, sqrt(2.0 / N));
-  }
-  if (N % 2 == 0) put(N / 2, false, 1.0 / sqrt((double)N));
+// Largest power-of-two column count whose two LDS buffers and twiddle table fit in lds_max
+// bytes (at least 1).
+inline int fft_cols(int n, size_t elem, size_t lds_max, int cap) {
+  int c = 1;
+  while (2 * c <= cap && (size_t)(2 * (2 * c) * n + n) * elem <= lds_max) c *= 2;
+  return c;
 }
-
-template <typename T>
-struct Gemm;
-template <>
-struct Gemm<double> {
-  static rocblas_status mm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m,
-                           int n, int k, const double* A, int lda, const double* B, int ldb,
-                           double* C, int ldc) {
-    const double one = 1.0, zero = 0.0;
-    return rocblas_dgemm(h, ta, tb, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc);
-  }
-  static rocblas_status sb(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m,
-                           int n, int k, const double* A, int lda, long sa, const double* B,
-                           int ldb, long sb_, double* C, int ldc, long sc, int batch) {
-    const double one = 1.0, zero = 0.0;
-    return rocblas_dgemm_strided_batched(h, ta, tb, m, n, k, &one, A, lda, sa, B, ldb, sb_,
-                                         &zero, C, ldc, sc, batch);
-  }
-};
-template <>
-struct Gemm<float> {
-  static rocblas_status mm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m,
-                           int n, int k, const float* A, int lda, const float* B, int ldb,
-                           float* C, int ldc) {
-    const float one = 1.0f, zero = 0.0f;
-    return rocblas_sgemm(h, ta, tb, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc);
-  }
-  static rocblas_status sb(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m,
-                           int n, int k, const float* A, int lda, long sa, const float* B,
-                           int ldb, long sb_, float* C, int ldc, long sc, int batch) {
-    const float one = 1.0f, zero = 0.0f;
-    return rocblas_sgemm_strided_batched(h, ta, tb, m, n, k, &one, A, lda, sa, B, ldb, sb_,
-                                         &zero, C, ldc, sc, batch);
-  }
-};
-
-#define RB_CHECK(x)                                                                 \
-  do {                                                                              \
-    rocblas_status s_ = (x);                                                        \
-    if (s_ != rocblas_status_success) {                                             \
-      fprintf(stderr, "fvm_amd: rocBLAS status %d at %s:%d\n", (int)s_, __FILE__,   \
-              __LINE__);                                                            \
-      return FVM_ERR_HIP;                                                           \
-    }                                                                               \
-  } while (0)
 
 template <typename T>
 struct SepData {
-  T *FJ = nullptr, *FK = nullptr;                       // transform matrices (storage type)
-  double *lamJ = nullptr, *lamK = nullptr;              // line operators in FP64
-  double *dE = nullptr, *dW = nullptr, *dN = nullptr, *dT = nullptr;
-  T *X = nullptr, *Y = nullptr;
+  T* R = nullptr;                      // residual, compact interior array (I fastest)
+  Cx<T>* S = nullptr;                  // spectrum [i + N1 (mJ + MJ mK)]
+  Cx<T>*twJ = nullptr, *twK = nullptr; // exp(-2 pi i t / N) tables
+  double* dW = nullptr;                // <D_W> per radial row (mode-line lower coefficient)
   double *invd = nullptr, *bf = nullptr;  // mode-line Thomas factors (FP64)
+  int colsJ = 1, colsK = 1;            // columns per workgroup of the J and K transforms
 };
 
-template <typename T>
-int sep_init(FvmCtx* c, SepData<T>& S, const std::vector<double>& FJ,
-             const std::vector<double>& FK, const std::vector<double>& lj,
-             const std::vector<double>& lk, const std::vector<double>& avg) {
-  const int n1 = c->d.ni - 1, nj1 = c->d.nj - 1, nk1 = c->d.nk - 1;
-  auto up = [&](auto** dst, const std::vector<double>& src, size_t off, size_t cnt) -> int {
-    using V = typename std::remove_reference<decltype(**dst)>::type;
-    std::vector<V> h(cnt);
-    for (size_t q = 0; q < cnt; ++q) h[q] = (V)src[off + q];
-    if (hipMalloc((void**)dst, cnt * sizeof(V)) != hipSuccess) return FVM_ERR_HIP;
-    c->allocs.push_back(*dst);
-    if (hipMemcpy(*dst, h.data(), cnt * sizeof(V), hipMemcpyHostToDevice) != hipSuccess)
-      return FVM_ERR_HIP;
-    return FVM_OK;
-  };
-  int r = 0;
-  r |= up(&S.FJ, FJ, 0, FJ.size());
-  r |= up(&S.FK, FK, 0, FK.size());
-  r |= up(&S.lamJ, lj, 0, lj.size());
-  r |= up(&S.lamK, lk, 0, lk.size());
-  // row r (radial index i = r+1): dE = <D_E>(i+1/2), dW = <D_E>(i-1/2)
-  r |= up(&S.dE, avg, 1, n1);
-  r |= up(&S.dW, avg, 0, n1);
-  r |= up(&S.dN, avg, n1 + 1, n1);
-  r |= up(&S.dT, avg, 2 * n1 + 1, n1);
-  size_t ncomp = (size_t)n1 * nj1 * nk1;
-  if (hipMalloc((void**)&S.X, ncomp * sizeof(T)) != hipSuccess) return FVM_ERR_HIP;
-  c->allocs.push_back(S.X);
-  if (hipMalloc((void**)&S.Y, ncomp * sizeof(T)) != hipSuccess) return FVM_ERR_HIP;
-  c->allocs.push_back(S.Y);
-  if (r) return r;
-  double** fac[2] = {&S.invd, &S.bf};
-  for (double** q : fac) {
-    if (hipMalloc((void**)q, ncomp * sizeof(double)) != hipSuccess) return FVM_ERR_HIP;
-    c->allocs.push_back(*q);
-  }
-  hipLaunchKernelGGL(k_mode_factor, dim3((nj1 + 63) / 64, nk1), dim3(64), 0, c->stream, n1, nj1,
-                     nk1, S.dE, S.dW, S.dN, S.dT, S.lamJ, S.lamK, S.invd, S.bf);
-  if (hipGetLastError() != hipSuccess) return FVM_ERR_HIP;
-  if (hipStreamSynchronize(c->stream) != hipSuccess) return FVM_ERR_HIP;
-  return FVM_OK;
-}
-
-// e = M^-1 r on the compact array X (in place through Y)
-template <typename T>
-int sep_apply(FvmCtx* c, SepData<T>& S, rocblas_handle h) {
-  const int n1 = c->d.ni - 1, nj1 = c->d.nj - 1, nk1 = c->d.nk - 1;
-  const int m = n1 * nj1;
-  const long slab = (long)n1 * nj1;
-  {
-    ProfScope ps(c, "p_dft");
-    // K transform: Y = X FK^T ; J transform per K slab: X = Y FJ^T
-    RB_CHECK(Gemm<T>::mm(h, rocblas_operation_none, rocblas_operation_transpose, m, nk1, nk1,
-                         S.X, m, S.FK, nk1, S.Y, m));
-    RB_CHECK(Gemm<T>::sb(h, rocblas_operation_none, rocblas_operation_transpose, n1, nj1, nj1,
-                         S.Y, n1, slab, S.FJ, nj1, 0, S.X, n1, slab, nk1));
-  }
-  {
-    ProfScope ps(c, "p_mode_lines");
-    mode_lines<T>(n1, nj1 * nk1, S.dW, S.invd, S.bf, S.X, c->stream);
-    HIP_CHECK(hipGetLastError());
-  }
-  {
-    ProfScope ps(c, "p_dft");
-    RB_CHECK(Gemm<T>::sb(h, rocblas_operation_none, rocblas_operation_none, n1, nj1, nj1, S.X,
-                         n1, slab, S.FJ, nj1, 0, S.Y, n1, slab, nk1));
-    RB_CHECK(Gemm<T>::mm(h, rocblas_operation_none, rocblas_operation_none, m, nk1, nk1, S.Y, m,
-                         S.FK, nk1, S.X, m));
-  }
-  return FVM_OK;
-}
-
 struct SepState {
-  rocblas_handle h = nullptr;
+  FftPlan pJ, pK;
+  int MJ = 0;
   SepData<double> d64;
   SepData<float> d32;
   bool ready = false;
 };
+
+template <typename T>
+int sep_init(FvmCtx* c, SepState& st, SepData<T>& S, const std::vector<double>& avg,
+             const double* dE_dev, const double* dW_dev, const double* dN_dev,
+             const double* dT_dev) {
+  const int n1 = c->d.ni - 1, nj1 = c->d.nj - 1, nk1 = c->d.nk - 1;
+  auto dev_alloc = [&](void** p, size_t bytes) -> int {
+    if (hipMalloc(p, bytes) != hipSuccess) return FVM_ERR_HIP;
+    c->allocs.push_back(*p);
+    return FVM_OK;
+  };
+  int r = 0;
+  const size_t ncomp = (size_t)n1 * nj1 * nk1;
+  const size_t nspec = (size_t)n1 * st.MJ * nk1;
+  r |= dev_alloc((void**)&S.R, ncomp * sizeof(T));
+  r |= dev_alloc((void**)&S.S, nspec * sizeof(Cx<T>));
+  r |= dev_alloc((void**)&S.twJ, (size_t)nj1 * sizeof(Cx<T>));
+  r |= dev_alloc((void**)&S.twK, (size_t)nk1 * sizeof(Cx<T>));
+  r |= dev_alloc((void**)&S.invd, nspec * sizeof(double));
+  r |= dev_alloc((void**)&S.bf, nspec * sizeof(double));
+  if (r) return FVM_ERR_HIP;
+  S.dW = (double*)dW_dev;
+  for (int dir = 0; dir < 2; ++dir) {
+    const int N = dir == 0 ? nj1 : nk1;
+    std::vector<double> w = fft_twiddles(N);
+    std::vector<Cx<T>> h(N);
+    for (int t = 0; t < N; ++t) h[t] = Cx<T>{(T)w[2 * t], (T)w[2 * t + 1]};
+    HIP_CHECK(hipMemcpy(dir == 0 ? S.twJ : S.twK, h.data(), N * sizeof(Cx<T>),
+                        hipMemcpyHostToDevice));
+  }
+  // columns per workgroup: two buffers and the twiddles in at most 64 KB of LDS
+  S.colsJ = fft_cols(nj1, sizeof(Cx<T>), 64 * 1024, 32);
+  S.colsK = fft_cols(nk1, sizeof(Cx<T>), 64 * 1024, 64);
+  hipLaunchKernelGGL(k_mode_factor, dim3((st.MJ + 63) / 64, nk1), dim3(64), 0, c->stream, n1,
+                     st.MJ, nj1, nk1, dE_dev, dW_dev, dN_dev, dT_dev, S.invd, S.bf);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return FVM_OK;
+}
+
+// p += M^-1 R: forward transforms of the residual, radial mode lines, inverse transforms with
+// the correction added to p
+template <typename T>
+int sep_apply(FvmCtx* c, SepState& st, SepData<T>& S) {
+  const Dims d = c->d;
+  const int n1 = d.ni - 1, nj1 = d.nj - 1, nk1 = d.nk - 1;
+  const size_t cb = sizeof(Cx<T>);
+  const size_t ldsJ = (size_t)(2 * S.colsJ * nj1 + nj1) * cb;
+  const size_t ldsK = (size_t)(2 * S.colsK * nk1 + nk1) * cb;
+  const dim3 gJ((n1 + 2 * S.colsJ - 1) / (2 * S.colsJ), nk1);
+  const dim3 gK((n1 + S.colsK - 1) / S.colsK, st.MJ);
+  {
+    ProfScope ps(c, "p_fft");
+    hipLaunchKernelGGL(k_fft_j_fwd<T>, gJ, dim3(256), ldsJ, c->stream, n1, nk1, st.pJ, S.colsJ,
+                       S.R, S.twJ, S.S);
+    if (nk1 > 1)
+      hipLaunchKernelGGL((k_fft_k<T, -1>), gK, dim3(256), ldsK, c->stream, n1, st.MJ, st.pK,
+                         S.colsK, S.twK, S.S);
+    HIP_CHECK(hipGetLastError());
+  }
+  {
+    ProfScope ps(c, "p_mode_lines");
+    mode_lines<T>(n1, st.MJ * nk1, S.dW, S.invd, S.bf, S.S, c->stream);
+    HIP_CHECK(hipGetLastError());
+  }
+  {
+    ProfScope ps(c, "p_fft");
+    if (nk1 > 1)
+      hipLaunchKernelGGL((k_fft_k<T, +1>), gK, dim3(256), ldsK, c->stream, n1, st.MJ, st.pK,
+                         S.colsK, S.twK, S.S);
+    const double scale = 1.0 / ((double)nj1 * (double)nk1);
+    hipLaunchKernelGGL(k_fft_j_inv_add<T>, gJ, dim3(256), ldsJ, c->stream, d, st.pJ, S.colsJ,
+                       S.S, S.twJ, scale, c->p);
+    HIP_CHECK(hipGetLastError());
+  }
+  return FVM_OK;
+}
 
 }  // namespace
 
 int fvm_sep_setup(FvmCtx* c) {
   const Dims d = c->d;
   const int n1 = d.ni - 1, nj1 = d.nj - 1, nk1 = d.nk - 1;
-  if (n1 > 512) return FVM_ERR_SIZE;  // mode lines: at most 8 rows per lane of a wavefront
   SepState* S = new SepState();
   c->sep = S;
-  if (rocblas_create_handle(&S->h) != rocblas_status_success) return FVM_ERR_HIP;
-  if (rocblas_set_stream(S->h, c->stream) != rocblas_status_success) return FVM_ERR_HIP;
+  // mode lines: at most 8 rows per lane of a wavefront; transforms: lengths with prime factors
+  // 2, 3, 5, 7 and an even J length (two radial columns per complex sequence need none, the
+  // half spectrum needs N / 2).  Other grids keep the line Gauss-Seidel forms only.
+  if (n1 > 512 || !fft_plan(nj1, &S->pJ) || !fft_plan(nk1, &S->pK) || nj1 % 2 != 0) {
+    fprintf(stderr, "fvm_amd: note: no separable pressure inverse for %d x %d x %d cells "
+                    "(Poisson modes 0 and 1 only)\n", n1, nj1, nk1);
+    return FVM_OK;
+  }
+  S->MJ = nj1 / 2 + 1;
   double *avg = nullptr, *dev = nullptr;
   const int navg = 3 * n1 + 1;
   HIP_CHECK(hipMalloc((void**)&avg, navg * sizeof(double)));
+  c->allocs.push_back(avg);
   HIP_CHECK(hipMalloc((void**)&dev, (n1 + 1) * sizeof(double)));
+  c->allocs.push_back(dev);
   hipLaunchKernelGGL(k_avg_faces, dim3(n1 + 1), dim3(256), 0, c->stream, d, c->pdI, c->pdJ,
                      c->pdK, avg, dev);
   HIP_CHECK(hipGetLastError());
-  std::vector<double> havg(navg), hdev(n1 + 1);
-  HIP_CHECK(hipMemcpyAsync(havg.data(), avg, navg * sizeof(double), hipMemcpyDeviceToHost,
-                           c->stream));
+  std::vector<double> hdev(n1 + 1);
   HIP_CHECK(hipMemcpyAsync(hdev.data(), dev, (n1 + 1) * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
-  HIP_CHECK(hipFree(avg));
-  HIP_CHECK(hipFree(dev));
   double mx = 0.0;
   for (double v : hdev) mx = std::max(mx, v);
   c->sep_dev = mx;
-  std::vector<double> FJ, FK, lj, lk;
-  fourier_basis(nj1, FJ, lj);
-  fourier_basis(nk1, FK, lk);
-  int r = sep_init<double>(c, S->d64, FJ, FK, lj, lk, havg);
-  r |= sep_init<float>(c, S->d32, FJ, FK, lj, lk, havg);
+  // row r (radial index i = r+1): dE = <D_E>(i+1/2) = avg[r+1], dW = <D_E>(i-1/2) = avg[r],
+  // dN = avg[n1+1+r], dT = avg[2n1+1+r]
+  const double* dW = avg;
+  const double* dE = avg + 1;
+  const double* dN = avg + n1 + 1;
+  const double* dT = avg + 2 * n1 + 1;
+  std::vector<double> none;
+  int r = sep_init<double>(c, *S, S->d64, none, dE, dW, dN, dT);
+  r |= sep_init<float>(c, *S, S->d32, none, dE, dW, dN, dT);
   if (r) return FVM_ERR_HIP;
   S->ready = true;
   return FVM_OK;
@@ -490,7 +557,6 @@ int fvm_sep_setup(FvmCtx* c) {
 void fvm_sep_destroy(FvmCtx* c) {
   SepState* S = (SepState*)c->sep;
   if (!S) return;
-  if (S->h) rocblas_destroy_handle(S->h);
   delete S;
   c->sep = nullptr;
 }
@@ -498,7 +564,11 @@ void fvm_sep_destroy(FvmCtx* c) {
 // Defect correction p += M^-1 (rhs - L p) until sum|r| / sum|AP p| <= RESIMAX.
 int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
   SepState* S = (SepState*)c->sep;
-  if (!S || !S->ready) return FVM_ERR_ARG;
+  if (!S || !S->ready) {
+    fprintf(stderr, "fvm_amd: Poisson modes 2 and 3 need the separable inverse, which this "
+                    "grid does not have\n");
+    return FVM_ERR_ARG;
+  }
   const Dims d = c->d;
   dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
   long nb = (long)g.x * g.y * g.z;
@@ -515,10 +585,10 @@ int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
       ProfScope pk(c, "k_residual");
       if (use32)
         hipLaunchKernelGGL(k_residual_compact<float>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
-                           c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d32.X, c->red);
+                           c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d32.R, c->red);
       else
         hipLaunchKernelGGL(k_residual_compact<double>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
-                           c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d64.X, c->red);
+                           c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d64.R, c->red);
       mb = (outer > 0 || cap < 1) ? fvm_mail(c) : Mailbox{nullptr, nullptr, 0};
       hipLaunchKernelGGL(k_res_final, dim3(1), dim3(256), 0, c->stream, c->red, nb, c->scal + 4,
                          mb);
@@ -540,21 +610,13 @@ int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
       // the FP32 approximate inverse stopped contracting: finish with the FP64 inverse
       use32 = false;
       hipLaunchKernelGGL(k_residual_compact<double>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
-                         c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d64.X, c->red);
+                         c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d64.R, c->red);
       HIP_CHECK(hipGetLastError());
       ++c->sep_fallbacks;
     }
     res_prev = res;
-    r = use32 ? sep_apply<float>(c, S->d32, S->h) : sep_apply<double>(c, S->d64, S->h);
+    r = use32 ? sep_apply<float>(c, *S, S->d32) : sep_apply<double>(c, *S, S->d64);
     if (r) return r;
-    {
-      ProfScope ps(c, "p_add");
-      if (use32)
-        hipLaunchKernelGGL(k_add_compact<float>, g, kBlk, 0, c->stream, d, c->p, S->d32.X);
-      else
-        hipLaunchKernelGGL(k_add_compact<double>, g, kBlk, 0, c->stream, d, c->p, S->d64.X);
-      HIP_CHECK(hipGetLastError());
-    }
     ++outer;
   }
   int r = fvm_k_bcp(c);

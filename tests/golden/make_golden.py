@@ -68,9 +68,9 @@ CASES = {
                                             22: ("u", "v", "w", "pref")}),
     # a wavy cylinder: radius 0.5 (1 + 0.08 sin(2 pi z / LZ)), so xi_z, eta_z and the
     # stage-coupling metric terms are not zero and the face coefficients vary along the span
-    # (not an extruded, separable grid); Poisson to 1e-12, 2 steps
+    # (not an extruded, separable grid); Poisson to 1e-14, 2 steps
     "rles_13x17x9_wavy_converged": dict(ni=13, nj=17, nk=9, h1=0.05, r_out=5.0, sigma=0.05,
-                                        re=200.0, dt=4e-3, nswpp=2000000, resimax="1.0D-12",
+                                        re=200.0, dt=4e-3, nswpp=2000000, resimax="1.0D-14",
                                         les=1, steps=2, keep_steps=(1, 2), wavy=0.08),
     # (no USE_LES=0 fixture: the reference's laminar mode writes YPLUS in CALCCOEF
     # (calccoef.f90:121), an array INIT allocates only when USE_LES=1 (init.f90:79-81))
