@@ -70,7 +70,13 @@ EXPORTS = ("fvm_create", "fvm_destroy", "fvm_set_field", "fvm_get_field", "fvm_s
            "fvm_p_source", "fvm_iter_u", "fvm_correct", "fvm_iter_p", "fvm_pressure_reference",
            "fvm_calccoef", "fvm_profile_enable", "fvm_profile_get", "fvm_profile_reset",
            "fvm_sync", "fvm_set_dtime", "fvm_create2d", "fvm_set_thermal", "fvm_iter_t",
-           "fvm_writedata_fields", "fvm_post_begin", "fvm_post_finalize", "fvm_ensemble_step")
+           "fvm_writedata_fields", "fvm_post_begin", "fvm_post_finalize", "fvm_ensemble_step",
+           "fvm_batch2d_create", "fvm_batch2d_destroy",
+           "fvm_batch2d_set_field", "fvm_batch2d_get_field", "fvm_batch2d_set_dtime",
+           "fvm_batch2d_set_wall_velocity", "fvm_batch2d_step", "fvm_batch2d_history",
+           "fvm_batch2d_set_tuning", "fvm_batch2d_get_tuning", "fvm_batch2d_autotune",
+           "fvm_batch2d_adjoint_init", "fvm_batch2d_adjoint_step", "fvm_batch2d_adjoint_wall",
+           "fvm_batch2d_adjoint_get", "fvm_batch2d_save", "fvm_batch2d_load", "fvm_geometry2d")
 
 _LIB = None
 

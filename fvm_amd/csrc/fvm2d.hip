@@ -1,0 +1,711 @@
+// The batched 2D time step: kernels (one launch per phase over all cases), the fixed-shape step
+// captured as a hipGraph, and the C ABI of include/fvm_capi.h (fvm_batch2d_*).
+//
+// Step (Main.f90:48-118): U_N copy, CONTRA_VEL + NL + DIF + P_SOURCE + the stage-1 right-hand
+// side of ITER_U/ITER_V (one fused stencil pass), the stage-1 BCU and I-line sweep, the
+// stage-2 right-hand side, BCU and J-line sweep, the predictor correction with BCU, the
+// Poisson right-hand side, ITER_P as FP64 defect correction with the separable inverse (fixed
+// pass count, converged cases skip), BCP, P_SOURCE + projection + BCU + CFL, CALCCOEF.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/fvm_capi.h"
+#include "fvm2d_host.h"
+#include "fvm2d_kernels.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// Phase A: U_N = U (whole plane), NL_NM1 = NL, CONTRA_VEL, NL(U_N), NL(V_N), DIF(U_N), DIF(V_N),
+// P_SOURCE, the stage-1 right-hand sides SU (iter_u.f90:43-46 / iter_v.f90) with the cross
+// terms DD of the current field, the far-field U_F of BCU/BCV (bcu.f90:22-33) from U_N and the
+// stage-1 right-hand-side folds (bcu.f90:15, 34).
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k2_explicit(const Case2D* __restrict__ tab,
+                                                   const int* __restrict__ bcase,
+                                                   const int* __restrict__ btile) {
+  const TileIdx t = tile_index(bcase, btile, tab);
+  const Case2D& cs = tab[t.c];
+  const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
+  const int i = t.i, j = t.j;
+  if (i > ni - 1 || j > nj - 1) return;
+  const long c = (long)i + (long)sj * j;
+  const double* U = cs.f[F_U];
+  const double* V = cs.f[F_V];
+  const double* P = cs.f[F_P];
+  double* UN = cs.f[F_UN];
+  double* VN = cs.f[F_VN];
+  // U_N = U over the whole plane: every interior thread copies its cell and the ghosts next to
+  // it (corners from the corner cells)
+  {
+    UN[c] = U[c];
+    VN[c] = V[c];
+    const int gi = i == 1 ? -1 : (i == ni - 1 ? 1 : 0);
+    const int gj = j == 1 ? -1 : (j == nj - 1 ? 1 : 0);
+    if (gi) { UN[c + gi] = U[c + gi]; VN[c + gi] = V[c + gi]; }
+    if (gj) { UN[c + gj * sj] = U[c + gj * sj]; VN[c + gj * sj] = V[c + gj * sj]; }
+    if (gi && gj) {
+      const long q = c + gi + (long)gj * sj;
+      UN[q] = U[q];
+      VN[q] = V[q];
+    }
+  }
+  const Geom g{cs.f[G_DJAC], cs.f[G_XIX], cs.f[G_XIY], cs.f[G_ETX], cs.f[G_ETY]};
+  const double j1 = ju1(g, U, V, c), j2 = ju2(g, U, V, c);
+  const double ue = 0.5 * (j1 + ju1(g, U, V, c + 1));
+  const double uw = i == 1 ? 0.0 : 0.5 * (j1 + ju1(g, U, V, c - 1));  // UW(1) = 0
+  const double vn = 0.5 * (j2 + ju2(g, U, V, c + sj));
+  const double vs = 0.5 * (j2 + ju2(g, U, V, c - sj));
+  double* NLX = cs.f[F_NLX];
+  double* NLY = cs.f[F_NLY];
+  const double nlx1 = NLX[c], nly1 = NLY[c];
+  const double nlx = nl2(U, c, sj, ue, uw, vn, vs);
+  const double nly = nl2(V, c, sj, ue, uw, vn, vs);
+  const double* de = cs.f[G_DE];
+  const double* dn = cs.f[G_DN];
+  const double* jq = cs.f[G_JQ12];
+  const double difx = dif2(de, dn, jq, U, c, sj);
+  const double dify = dif2(de, dn, jq, V, c, sj);
+  const double su = psrc(g.dj, g.xix, g.etx, P, c, sj);
+  const double sv = psrc(g.dj, g.xiy, g.ety, P, c, sj);
+  const double re = cs.re;
+  const double ap0 = cs.f[G_AP0][c];
+  double r1u = ap0 * U[c] - 3.0 * re * nlx + 1.0 * re * nlx1 + difx + 2.0 * re * su +
+               cross2(cs.f[G_JXE1], U, c, sj);
+  double r1v = ap0 * V[c] - 3.0 * re * nly + 1.0 * re * nly1 + dify + 2.0 * re * sv +
+               cross2(cs.f[G_JXE1], V, c, sj);
+  if (i == 1) {  // wall fold SU(1) += 2 AW(1) U_F
+    r1u = r1u + 2.0 * cs.b[B_AW1W][j] * cs.b[B_UWALL][j];
+    r1v = r1v + 2.0 * cs.b[B_AW1W][j] * cs.b[B_VWALL][j];
+  }
+  if (i == ni - 1) {
+    // far-field U_F: inflow Dirichlet (UIN = 1, VIN = 0) where THETA(NI, J) is in [pi/2, 3pi/2],
+    // else the convective outflow value from U_N with convection speed UIN (bcu.f90:26-33)
+    const double pi = 3.14159265358979323846;
+    const long f = c + 1;  // (NI, J)
+    const double th = cs.f[G_THETA][f];
+    double ufu, ufv;
+    if (th >= 0.5 * pi && th <= 1.5 * pi) {
+      ufu = 1.0;
+      ufv = 0.0;
+    } else {
+      const double ax = 0.5 * (g.xix[f] + g.xix[c]);
+      const double ae = 0.5 * (g.etx[f] + g.etx[c]);
+      const double* F2[2] = {U, V};
+      double uf[2];
+      for (int q = 0; q < 2; ++q) {
+        const double* W = F2[q];
+        const double s = W[f + sj] + W[c + sj] - W[f - sj] - W[c - sj];
+        uf[q] = 0.5 * (W[f] + W[c]) - cs.dt * 1.0 * (ax * (W[f] - W[c]) + ae * 0.25 * s);
+      }
+      ufu = uf[0];
+      ufv = uf[1];
+    }
+    cs.b[B_UFAR][j] = ufu;
+    cs.b[B_VFAR][j] = ufv;
+    r1u = r1u + 2.0 * cs.b[B_AE1F][j] * ufu;
+    r1v = r1v + 2.0 * cs.b[B_AE1F][j] * ufv;
+  }
+  cs.f[F_NLX1][c] = nlx1;
+  cs.f[F_NLY1][c] = nly1;
+  NLX[c] = nlx;
+  NLY[c] = nly;
+  cs.f[F_DIFX][c] = difx;
+  cs.f[F_DIFY][c] = dify;
+  cs.f[F_SU][c] = su;
+  cs.f[F_SV][c] = sv;
+  cs.f[F_SU1][c] = r1u;
+  cs.f[F_SV1][c] = r1v;
+}
+
+// BCU / BCV ghosts (bcu.f90:10-51) of field F from its interior: wall row I = 0 and far-field
+// row I = NI for J = 0..NJ, then the branch-cut columns J = 0, NJ for I = 0..NI.  One
+// workgroup of 64 lanes.
+__device__ inline void bcu_ghosts_wave(double* F, int ni, int nj, int sj, const double* uwall,
+                                       const double* ufar) {
+  const int l = threadIdx.x;
+  for (int jj = l; jj <= nj; jj += 64) {
+    const bool in = jj >= 1 && jj <= nj - 1;
+    const long c = (long)sj * jj;
+    F[c] = -F[c + 1] + 2.0 * (in ? uwall[jj] : 0.0);
+    F[c + ni] = -F[c + ni - 1] + 2.0 * (in ? ufar[jj] : 0.0);
+  }
+  __syncthreads();
+  for (int ii = l; ii <= ni; ii += 64) {
+    F[ii] = F[ii + (long)sj * (nj - 1)];
+    F[ii + (long)sj * nj] = F[ii + sj];
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------------
+// The AF line Gauss-Seidel sweep of one stage (iter_u.f90:50-66 I lines, 103-119 J lines), in
+// the reference's order: lines one after the other, each line solved with the already updated
+// previous line and the old next line.  One 64-lane wavefront per (case, component): BCU's
+// ghosts first, then per line the Thomas recursion as two affine-map scans over the lanes
+// (lane l holds rows l R .. l R + R - 1 and keeps the previous line's values in registers).
+// DIR 0: I lines (rows along I, lines J = 1..NJ-1, lagged couplings AN, AS = AN(J-1));
+// DIR 1: J lines (rows along J, lines I = 1..NI-1, lagged couplings AE, AW); the line ends take
+// the ghost values as Dirichlet data.
+// ---------------------------------------------------------------------------------------------
+template <int R, int DIR>
+__global__ void __launch_bounds__(64) k2_sweep(const Case2D* __restrict__ tab,
+                                               const int* __restrict__ cases, int with_bcu) {
+  const Case2D& cs = tab[cases[blockIdx.x >> 1]];
+  const int comp = blockIdx.x & 1;
+  const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
+  double* F = cs.f[comp ? F_V : F_U];
+  if (with_bcu)
+    bcu_ghosts_wave(F, ni, nj, sj, cs.b[comp ? B_VWALL : B_UWALL], cs.b[comp ? B_VFAR : B_UFAR]);
+  const double* Q = cs.f[comp ? F_SV1 : F_SU1];
+  const double* invd = cs.f[DIR == 0 ? G_INVD1 : G_INVD2];
+  const double* na = cs.f[DIR == 0 ? G_NA1 : G_NA2];
+  const double* bf = cs.f[DIR == 0 ? G_B1 : G_B2];
+  const double* cup = cs.f[DIR == 0 ? G_AN1 : G_AE2];
+  const double* cdn = cs.f[DIR == 0 ? G_AN1 : G_AW2];
+  const int n = DIR == 0 ? ni - 1 : nj - 1;       // rows per line
+  const int nl = DIR == 0 ? nj - 1 : ni - 1;      // lines
+  const long rs = DIR == 0 ? 1 : sj;              // row stride
+  const long ls = DIR == 0 ? sj : 1;              // line stride
+  const long dnoff = DIR == 0 ? -(long)sj : 0;    // lower coupling of row c at cdn[c + dnoff]
+  const int l = threadIdx.x;
+  const int r0 = l * R;
+  double prev[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int r = r0 + q;
+    prev[q] = r < n ? F[rs * (r + 1)] : 0.0;  // line 0: the ghost line
+  }
+  for (int L = 1; L <= nl; ++L) {
+    const long base = ls * L;
+    double al[R], be[R], bb[R];
+    double At = 0.0, Bt = 1.0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int r = r0 + q;
+      if (r < n) {
+        const long c = base + rs * (r + 1);
+        const double C = cup[c] * F[c + ls] + cdn[c + dnoff] * prev[q] + Q[c];
+        al[q] = C * invd[c];
+        be[q] = na[c];
+        bb[q] = bf[c];
+      } else {
+        al[q] = 0.0; be[q] = 1.0; bb[q] = 0.0;
+      }
+      At = al[q] + be[q] * At;
+      Bt = be[q] * Bt;
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const double Ao = __shfl_up(At, o, 64), Bo = __shfl_up(Bt, o, 64);
+      if (l >= o) { At = At + Bt * Ao; Bt = Bt * Bo; }
+    }
+    // lane l starts from the exclusive prefix applied to the ghost before the first row
+    const double g0 = F[base];
+    const double Ae = __shfl_up(At, 1, 64), Be = __shfl_up(Bt, 1, 64);
+    double yin = l == 0 ? g0 : Ae + Be * g0;
+    double y[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      y[q] = al[q] + be[q] * yin;
+      yin = y[q];
+    }
+    double Ct = 1.0, Dt = 0.0;
+#pragma unroll
+    for (int q = R - 1; q >= 0; --q) {
+      if (r0 + q < n) {
+        Dt = y[q] - bb[q] * Dt;
+        Ct = -bb[q] * Ct;
+      }
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const double Do = __shfl_down(Dt, o, 64), Co = __shfl_down(Ct, o, 64);
+      if (l + o < 64) { Dt = Dt + Ct * Do; Ct = Ct * Co; }
+    }
+    // and from the exclusive suffix applied to the ghost after the last row
+    const double g1 = F[base + rs * (n + 1)];
+    const double De = __shfl_down(Dt, 1, 64), Ce = __shfl_down(Ct, 1, 64);
+    double xin = l == 63 ? g1 : De + Ce * g1;
+#pragma unroll
+    for (int q = R - 1; q >= 0; --q) {
+      const int r = r0 + q;
+      if (r < n) {
+        const double x = y[q] - bb[q] * xin;
+        xin = x;
+        prev[q] = x;
+        F[base + rs * (r + 1)] = x;
+      }
+    }
+  }
+}
+
+// Stage-2 right-hand side SU = 2 Re DJAC / dt U + DD(U) with the stage-2 cross weights
+// (iter_u.f90:84-98) and BCU's folds with the stage-2 AW(1), AE(NI-1) (bcu.f90:15, 34).
+__global__ void __launch_bounds__(256) k2_rhs2(const Case2D* __restrict__ tab,
+                                               const int* __restrict__ bcase,
+                                               const int* __restrict__ btile) {
+  const TileIdx t = tile_index(bcase, btile, tab);
+  const Case2D& cs = tab[t.c];
+  const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
+  const int i = t.i, j = t.j;
+  if (i > ni - 1 || j > nj - 1) return;
+  const long c = (long)i + (long)sj * j;
+  const double* U = cs.f[F_U];
+  const double* V = cs.f[F_V];
+  const double ap0 = cs.f[G_AP0][c];
+  double ru = ap0 * U[c] + cross2(cs.f[G_JXE2], U, c, sj);
+  double rv = ap0 * V[c] + cross2(cs.f[G_JXE2], V, c, sj);
+  if (i == 1) {
+    ru = ru + 2.0 * cs.b[B_AW2W][j] * cs.b[B_UWALL][j];
+    rv = rv + 2.0 * cs.b[B_AW2W][j] * cs.b[B_VWALL][j];
+  }
+  if (i == ni - 1) {
+    ru = ru + 2.0 * cs.b[B_AE2F][j] * cs.b[B_UFAR][j];
+    rv = rv + 2.0 * cs.b[B_AE2F][j] * cs.b[B_VFAR][j];
+  }
+  cs.f[F_SU1][c] = ru;
+  cs.f[F_SV1][c] = rv;
+}
+
+// BCU ghosts written by the interior thread that owns their value after a full BCU: the wall
+// and far-field rows, the branch-cut columns and the corners (bcu.f90:10-51)
+__device__ inline void own_ghosts(double* F, const Case2D& cs, int i, int j, long c, double v,
+                                  const double* uwall, const double* ufar) {
+  const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
+  double gw = 0.0, gf = 0.0;
+  if (i == 1) { gw = -v + 2.0 * uwall[j]; F[c - 1] = gw; }
+  if (i == ni - 1) { gf = -v + 2.0 * ufar[j]; F[c + 1] = gf; }
+  if (j == nj - 1) {  // J = 0 <- NJ-1
+    const long g = c - (long)sj * (nj - 1);
+    F[g] = v;
+    if (i == 1) F[g - 1] = gw;
+    if (i == ni - 1) F[g + 1] = gf;
+  }
+  if (j == 1) {  // J = NJ <- 1
+    const long g = c + (long)sj * (nj - 1);
+    F[g] = v;
+    if (i == 1) F[g - 1] = gw;
+    if (i == ni - 1) F[g + 1] = gf;
+  }
+}
+
+// Predictor correction U = U - S_U DTIME / DJAC (Main.f90:80-84) and the BCU ghosts
+// (Main.f90:86-87).
+__global__ void __launch_bounds__(256) k2_correct(const Case2D* __restrict__ tab,
+                                                  const int* __restrict__ bcase,
+                                                  const int* __restrict__ btile) {
+  const TileIdx t = tile_index(bcase, btile, tab);
+  const Case2D& cs = tab[t.c];
+  const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
+  const int i = t.i, j = t.j;
+  if (i > ni - 1 || j > nj - 1) return;
+  const long c = (long)i + (long)sj * j;
+  const double dj = cs.f[G_DJAC][c];
+  double* U = cs.f[F_U];
+  double* V = cs.f[F_V];
+  const double u = U[c] - cs.f[F_SU][c] * cs.dt / dj;
+  const double v = V[c] - cs.f[F_SV][c] * cs.dt / dj;
+  U[c] = u;
+  V[c] = v;
+  own_ghosts(U, cs, i, j, c, u, cs.b[B_UWALL], cs.b[B_UFAR]);
+  own_ghosts(V, cs, i, j, c, v, cs.b[B_VWALL], cs.b[B_VFAR]);
+}
+
+// CONTRA_VEL of the corrected velocity and the source of ITER_P (iter_p.f90:29-32):
+// RHS = 1 / DTIME (UE - UW + VN - VS)
+__global__ void __launch_bounds__(256) k2_prhs(const Case2D* __restrict__ tab,
+                                               const int* __restrict__ bcase,
+                                               const int* __restrict__ btile) {
+  const TileIdx t = tile_index(bcase, btile, tab);
+  const Case2D& cs = tab[t.c];
+  const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
+  const int i = t.i, j = t.j;
+  if (i > ni - 1 || j > nj - 1) return;
+  const long c = (long)i + (long)sj * j;
+  const double* U = cs.f[F_U];
+  const double* V = cs.f[F_V];
+  const Geom g{cs.f[G_DJAC], cs.f[G_XIX], cs.f[G_XIY], cs.f[G_ETX], cs.f[G_ETY]};
+  const double j1 = ju1(g, U, V, c), j2 = ju2(g, U, V, c);
+  const double ue = 0.5 * (j1 + ju1(g, U, V, c + 1));
+  const double uw = i == 1 ? 0.0 : 0.5 * (j1 + ju1(g, U, V, c - 1));
+  const double vn = 0.5 * (j2 + ju2(g, U, V, c + sj));
+  const double vs = 0.5 * (j2 + ju2(g, U, V, c - sj));
+  cs.f[F_RHS][c] = 1.0 / cs.dt * (ue - uw + vn - vs);
+}
+
+// BCP (bcp.f90:12-43) of p: wall Neumann p(0) = p(1), outflow cubic extrapolation
+// p(NI) = 3 p(NI-1) - 3 p(NI-2) + p(NI-3), branch cut.  One workgroup per case; skipped for a
+// converged case unless force.
+__global__ void __launch_bounds__(256) k2_bcp(const Case2D* __restrict__ tab, int force) {
+  const Case2D& cs = tab[blockIdx.x];
+  if (!force && C2_DONE(cs)) return;
+  const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
+  double* P = cs.f[F_P];
+  for (int j = 1 + threadIdx.x; j <= nj - 1; j += blockDim.x) {
+    const long c = (long)sj * j;
+    P[c] = P[c + 1];
+    P[c + ni] = 3.0 * P[c + ni - 1] - 3.0 * P[c + ni - 2] + P[c + ni - 3];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i <= ni; i += blockDim.x) {
+    P[i] = P[i + (long)sj * (nj - 1)];
+    P[i + (long)sj * nj] = P[i + sj];
+  }
+}
+
+// Residual of ITER_P's operator (iter_p.f90:12-51 with BCP's folds AP(1) -= AW(1), AW(1) = 0,
+// AE(NI-1) = 0): r = AE p_E + AW p_W + AN p_N + AS p_S + SU - AP p, SU = RHS - DD(p), into the
+// compact array R [(i-1) + (NI-1)(j-1)], with the block's sums of |r| and |AP p|.
+template <typename T>
+__global__ void __launch_bounds__(256) k2_residual(const Case2D* __restrict__ tab,
+                                                   const int* __restrict__ bcase,
+                                                   const int* __restrict__ btile) {
+  __shared__ double sh[256];
+  const TileIdx t = tile_index(bcase, btile, tab);
+  const Case2D& cs = tab[t.c];
+  if (C2_DONE(cs)) return;  // uniform over the workgroup
+  const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
+  const int i = t.i, j = t.j;
+  double ra = 0.0, rb = 0.0;
+  if (i <= ni - 1 && j <= nj - 1) {
+    const long c = (long)i + (long)sj * j;
+    const double* P = cs.f[F_P];
+    const double* de = cs.f[G_DE];
+    const double* dn = cs.f[G_DN];
+    const double dE = de[c], dW = de[c - 1], dN = dn[c], dS = dn[c - sj];
+    double ap = -dE - dW - dN - dS;
+    double ae = -dE, aw = -dW;
+    const double an = -dN, as = -dS;
+    if (i == 1) { ap = ap - aw; aw = 0.0; }
+    if (i == ni - 1) ae = 0.0;
+    const double su = cs.f[F_RHS][c] - cross2(cs.f[G_JQ12], P, c, sj);
+    const double r = ae * P[c + 1] + aw * P[c - 1] + an * P[c + sj] + as * P[c - sj] + su -
+                     ap * P[c];
+    ra = fabs(r);
+    rb = fabs(ap * P[c]);
+    ((T*)(sizeof(T) == 8 ? cs.R : cs.R32))[(i - 1) + (long)(ni - 1) * (j - 1)] = (T)r;
+  }
+  ra = wg_sum(ra, sh);
+  rb = wg_sum(rb, sh);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    cs.part[2 * t.tile] = ra;
+    cs.part[2 * t.tile + 1] = rb;
+  }
+}
+
+// Per case: RESIP = sum|r| / sum|AP p| from the tile sums (fixed order); pass q >= 1 ends the
+// case's correction when RESIP <= RESIMAX (the reference tests after a sweep,
+// iter_p.f90:36-51) or when q is the last pass; q = 0 only records the starting residual.
+__global__ void __launch_bounds__(256) k2_res_final(const Case2D* __restrict__ tab, int q,
+                                                    int last) {
+  __shared__ double sh[256];
+  const Case2D& cs = tab[blockIdx.x];
+  if (C2_DONE(cs)) return;
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < cs.ntile; k += blockDim.x) {
+    a = a + cs.part[2 * k];
+    b = b + cs.part[2 * k + 1];
+  }
+  a = wg_sum(a, sh);
+  b = wg_sum(b, sh);
+  if (threadIdx.x != 0) return;
+  const double res = scaled_residual(a, b);
+  double* st = cs.stat;
+  st[ST_S0] = a;
+  st[ST_S1] = b;
+  st[ST_RESIP] = res;
+  st[ST_NPASS] = (double)q;
+  if (q == 0) st[ST_RES0] = res;
+  if (!(res == res)) st[ST_NAN] = 1.0;
+  if (q >= 1 && (res <= cs.resimax || q >= last || !(res == res))) st[ST_DONE] = 1.0;
+}
+
+__global__ void k2_begin_p(const Case2D* __restrict__ tab) {
+  const Case2D& cs = tab[blockIdx.x];
+  if (threadIdx.x == 0) {
+    cs.stat[ST_DONE] = 0.0;
+    cs.stat[ST_NAN] = 0.0;
+  }
+}
+
+// Forward real J transform of the residual (two radial rows per complex sequence) into the
+// half spectrum S[i + n1 m], one workgroup per (case, tile of 2 ncol rows); see
+// fvm_poisson_spec.hip for the 3D form.
+template <typename T>
+__global__ void __launch_bounds__(256) k2_fft_fwd(const Case2D* __restrict__ tab,
+                                                  const int* __restrict__ bcase,
+                                                  const int* __restrict__ btile, int ncol) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const Case2D& cs = tab[bcase[blockIdx.x]];
+  if (C2_DONE(cs)) return;
+  const int N = cs.pj.n, MJ = cs.mj, n1 = cs.ni - 1;
+  const T* R = (const T*)(sizeof(T) == 8 ? cs.R : cs.R32);
+  Cx<T>* S = (Cx<T>*)(sizeof(T) == 8 ? cs.S : cs.S32);
+  const Cx<T>* twg = (const Cx<T>*)(sizeof(T) == 8 ? cs.twj : cs.twj32);
+  Cx<T>* a = (Cx<T>*)smem;
+  Cx<T>* b = a + (size_t)ncol * N;
+  Cx<T>* tw = b + (size_t)ncol * N;
+  const int i0 = btile[blockIdx.x] * 2 * ncol;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  for (int q = threadIdx.x; q < ncol * N; q += blockDim.x) {
+    const int col = q % ncol, j = q / ncol;
+    const int i = i0 + 2 * col;
+    Cx<T> z{(T)0, (T)0};
+    if (i < n1) {
+      const long o = (long)i + (long)n1 * j;
+      z.x = R[o];
+      z.y = i + 1 < n1 ? R[o + 1] : (T)0;
+    }
+    a[j * ncol + col] = z;
+  }
+  __syncthreads();
+  const Cx<T>* Z = fft_lds<T, -1>(a, b, ncol, cs.pj, tw);
+  for (int q = threadIdx.x; q < ncol * MJ; q += blockDim.x) {
+    const int col = q % ncol, m = q / ncol;
+    const int i = i0 + 2 * col;
+    if (i >= n1) continue;
+    const Cx<T> zm = Z[m * ncol + col], zc = Z[((N - m) % N) * ncol + col];
+    const long o = (long)i + (long)n1 * m;
+    S[o] = Cx<T>{(T)0.5 * (zm.x + zc.x), (T)0.5 * (zm.y - zc.y)};
+    if (i + 1 < n1) S[o + 1] = Cx<T>{(T)0.5 * (zm.y + zc.y), (T)0.5 * (zc.x - zm.x)};
+  }
+}
+
+// Radial mode lines of the half spectrum, one wavefront per (case, mode) from the block table.
+template <typename T, int R>
+__global__ void __launch_bounds__(64) k2_modes(const Case2D* __restrict__ tab,
+                                               const int* __restrict__ bcase,
+                                               const int* __restrict__ btile) {
+  const Case2D& cs = tab[bcase[blockIdx.x]];
+  if (C2_DONE(cs)) return;
+  const int n = cs.ni - 1;
+  const long base = (long)n * btile[blockIdx.x];
+  Cx<T>* Z = (Cx<T>*)(sizeof(T) == 8 ? cs.S : cs.S32);
+  const double* dW = cs.pdw;
+  const double* invd = cs.invd;
+  const double* bf = cs.bf;
+  const int l = threadIdx.x;
+  const int r0 = l * R;
+  double alx[R], aly[R], be[R], bb[R];
+  double Ax = 0.0, Ay = 0.0, Bt = 1.0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int r = r0 + q;
+    if (r < n) {
+      const double iv = invd[base + r];
+      const Cx<T> z = Z[base + r];
+      alx[q] = (double)z.x * iv;
+      aly[q] = (double)z.y * iv;
+      be[q] = r == 0 ? 0.0 : -dW[r] * iv;
+      bb[q] = bf[base + r];
+    } else {
+      alx[q] = 0.0; aly[q] = 0.0; be[q] = 1.0; bb[q] = 0.0;
+    }
+    Ax = alx[q] + be[q] * Ax;
+    Ay = aly[q] + be[q] * Ay;
+    Bt = be[q] * Bt;
+  }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double Aox = __shfl_up(Ax, o, 64), Aoy = __shfl_up(Ay, o, 64),
+                 Bo = __shfl_up(Bt, o, 64);
+    if (l >= o) { Ax = Ax + Bt * Aox; Ay = Ay + Bt * Aoy; Bt = Bt * Bo; }
+  }
+  double yx_in = __shfl_up(Ax, 1, 64), yy_in = __shfl_up(Ay, 1, 64);
+  if (l == 0) { yx_in = 0.0; yy_in = 0.0; }
+  double yx[R], yy[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    yx[q] = alx[q] + be[q] * yx_in;
+    yy[q] = aly[q] + be[q] * yy_in;
+    yx_in = yx[q];
+    yy_in = yy[q];
+  }
+  double Ct = 1.0, Dx = 0.0, Dy = 0.0;
+#pragma unroll
+  for (int q = R - 1; q >= 0; --q) {
+    if (r0 + q < n) {
+      Dx = yx[q] - bb[q] * Dx;
+      Dy = yy[q] - bb[q] * Dy;
+      Ct = -bb[q] * Ct;
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double Dox = __shfl_down(Dx, o, 64), Doy = __shfl_down(Dy, o, 64),
+                 Co = __shfl_down(Ct, o, 64);
+    if (l + o < 64) { Dx = Dx + Ct * Dox; Dy = Dy + Ct * Doy; Ct = Ct * Co; }
+  }
+  double xx_in = __shfl_down(Dx, 1, 64), xy_in = __shfl_down(Dy, 1, 64);
+  if (l == 63) { xx_in = 0.0; xy_in = 0.0; }
+#pragma unroll
+  for (int q = R - 1; q >= 0; --q) {
+    const int r = r0 + q;
+    if (r < n) {
+      const double xx = yx[q] - bb[q] * xx_in, xy = yy[q] - bb[q] * xy_in;
+      xx_in = xx;
+      xy_in = xy;
+      Z[base + r] = Cx<T>{(T)xx, (T)xy};
+    }
+  }
+}
+
+// Inverse J transform of the corrected spectrum, added to p (scale 1 / (NJ-1)).
+template <typename T>
+__global__ void __launch_bounds__(256) k2_fft_inv_add(const Case2D* __restrict__ tab,
+                                                      const int* __restrict__ bcase,
+                                                      const int* __restrict__ btile, int ncol,
+                                                      int to_mu) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const Case2D& cs = tab[bcase[blockIdx.x]];
+  if (C2_DONE(cs)) return;
+  const int N = cs.pj.n, MJ = cs.mj, n1 = cs.ni - 1, sj = cs.sj;
+  const Cx<T>* S = (const Cx<T>*)(sizeof(T) == 8 ? cs.S : cs.S32);
+  const Cx<T>* twg = (const Cx<T>*)(sizeof(T) == 8 ? cs.twj : cs.twj32);
+  double* P = to_mu ? cs.a[A_MU] : cs.f[F_P];  // the adjoint solve corrects its own solution
+  Cx<T>* a = (Cx<T>*)smem;
+  Cx<T>* b = a + (size_t)ncol * N;
+  Cx<T>* tw = b + (size_t)ncol * N;
+  const int i0 = btile[blockIdx.x] * 2 * ncol;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  for (int q = threadIdx.x; q < ncol * MJ; q += blockDim.x) {
+    const int col = q % ncol, m = q / ncol;
+    const int i = i0 + 2 * col;
+    Cx<T> A{(T)0, (T)0}, B{(T)0, (T)0};
+    if (i < n1) {
+      const long o = (long)i + (long)n1 * m;
+      A = S[o];
+      if (i + 1 < n1) B = S[o + 1];
+    }
+    if (m == 0 || 2 * m == N) { A.y = (T)0; B.y = (T)0; }
+    a[m * ncol + col] = Cx<T>{A.x - B.y, A.y + B.x};
+    if (m > 0 && 2 * m < N) a[(N - m) * ncol + col] = Cx<T>{A.x + B.y, B.x - A.y};
+  }
+  __syncthreads();
+  const Cx<T>* Z = fft_lds<T, +1>(a, b, ncol, cs.pj, tw);
+  const double scale = 1.0 / (double)N;
+  for (int q = threadIdx.x; q < ncol * N; q += blockDim.x) {
+    const int col = q % ncol, j = q / ncol;
+    const int i = i0 + 2 * col;
+    if (i >= n1) continue;
+    const Cx<T> z = Z[j * ncol + col];
+    const long c = (long)(i + 1) + (long)sj * (j + 1);
+    P[c] = P[c] + scale * (double)z.x;
+    if (i + 1 < n1) P[c + 1] = P[c + 1] + scale * (double)z.y;
+  }
+}
+
+// P_SOURCE(p^{n+1}), the projection U = U + S_U DTIME / DJAC (Main.f90:93-103), BCU's ghosts
+// and the block maxima of the local CFL number (calccoef.f90:9-17)
+__global__ void __launch_bounds__(256) k2_project(const Case2D* __restrict__ tab,
+                                                  const int* __restrict__ bcase,
+                                                  const int* __restrict__ btile) {
+  __shared__ double sh[256];
+  const TileIdx t = tile_index(bcase, btile, tab);
+  const Case2D& cs = tab[t.c];
+  const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
+  const int i = t.i, j = t.j;
+  double cfl = 0.0;
+  if (i <= ni - 1 && j <= nj - 1) {
+    const long c = (long)i + (long)sj * j;
+    const Geom g{cs.f[G_DJAC], cs.f[G_XIX], cs.f[G_XIY], cs.f[G_ETX], cs.f[G_ETY]};
+    const double* P = cs.f[F_P];
+    const double su = psrc(g.dj, g.xix, g.etx, P, c, sj);
+    const double sv = psrc(g.dj, g.xiy, g.ety, P, c, sj);
+    double* U = cs.f[F_U];
+    double* V = cs.f[F_V];
+    const double dj = g.dj[c];
+    const double u = U[c] + su * cs.dt / dj;
+    const double v = V[c] + sv * cs.dt / dj;
+    U[c] = u;
+    V[c] = v;
+    cs.f[F_SU][c] = su;
+    cs.f[F_SV][c] = sv;
+    own_ghosts(U, cs, i, j, c, u, cs.b[B_UWALL], cs.b[B_UFAR]);
+    own_ghosts(V, cs, i, j, c, v, cs.b[B_VWALL], cs.b[B_VFAR]);
+    cfl = cs.dt * (fabs(g.xix[c] * u + g.xiy[c] * v) + fabs(g.etx[c] * u + g.ety[c] * v));
+  }
+  cfl = wg_max(cfl, sh);
+  if (threadIdx.x == 0 && threadIdx.y == 0) cs.part[2 * t.tile] = cfl;
+}
+
+// CALCCOEF (calccoef.f90:9-119): CFL maximum, C_pb from the corner average PG(1,1) (the far-field
+// PG(NI,NJ/2) is not assigned by CALCCOEF: zero), and the pressure and friction drag and lift
+// summed over the wall cells in J order, each sum carried by one lane; one workgroup per case.
+// The step's scalars go to stat and to the history ring at slot *ctr.
+__global__ void __launch_bounds__(256) k2_calccoef(const Case2D* __restrict__ tab, int ncase,
+                                                   const long* __restrict__ ctr, int hcap,
+                                                   double* __restrict__ hist) {
+  __shared__ double sh[256];
+  __shared__ double tm[4][256];
+  const Case2D& cs = tab[blockIdx.x];
+  const int nj = cs.nj, sj = cs.sj;
+  const int t = threadIdx.x;
+  double cm = 0.0;
+  for (int k = t; k < cs.ntile; k += blockDim.x) cm = fmax(cm, cs.part[2 * k]);
+  cm = wg_max(cm, sh);
+  const double* P = cs.f[F_P];
+  const double* U = cs.f[F_U];
+  const double* V = cs.f[F_V];
+  const double* w = cs.b[B_WALL];
+  double acc = 0.0;
+  for (int j0 = 1; j0 <= nj - 1; j0 += 256) {
+    const int j = j0 + t;
+    if (j <= nj - 1) {
+      const double* o = w + 6 * j;
+      const double dn1 = o[0], s2 = o[1], c2 = o[2], area = o[3], ct = o[4], st = o[5];
+      const long c = 1 + (long)sj * j;
+      const double dudn = (U[c] * s2 - V[c] * c2 - 0.0) / dn1;  // VEL_ROT = 0
+      tm[0][t] = 2.0 * P[c] * area * ct;
+      tm[1][t] = 2.0 * P[c] * area * st;
+      tm[2][t] = (2.0 / cs.re) * dudn * area * st;
+      tm[3][t] = (2.0 / cs.re) * dudn * area * ct;
+    }
+    __syncthreads();
+    if (t < 4) {
+      const int m = min(256, nj - j0);
+      if (t == 2)
+        for (int q = 0; q < m; ++q) acc = acc + tm[2][q];
+      else
+        for (int q = 0; q < m; ++q) acc = acc - tm[t][q];
+    }
+    __syncthreads();
+  }
+  if (t < 4) sh[t] = acc;
+  __syncthreads();
+  if (t != 0) return;
+  const double pd = sh[0], pl = sh[1], td = sh[2], tl = sh[3];
+  const double dum_b = 0.25 * (P[1 + sj] + P[1] + P[sj] + P[0]);
+  const double cpb = 2.0 * (0.0 - dum_b);
+  double* st = cs.stat;
+  st[ST_CFL] = cm;
+  st[ST_CPB] = cpb;
+  st[ST_CD] = pd + td;
+  st[ST_CL] = pl + tl;
+  st[ST_CDP] = pd;
+  st[ST_CDV] = td;
+  st[ST_CLP] = pl;
+  st[ST_CLV] = tl;
+  double* h = hist + ((long)(*ctr % hcap) * ncase + blockIdx.x) * 8;
+  h[0] = st[ST_CD];
+  h[1] = st[ST_CL];
+  h[2] = st[ST_CDP];
+  h[3] = st[ST_CDV];
+  h[4] = st[ST_CLP];
+  h[5] = st[ST_CLV];
+  h[6] = st[ST_RESIP];
+  h[7] = st[ST_CFL];
+}
+
+__global__ void k2_tick(long* ctr) {
+  if (threadIdx.x == 0) *ctr = *ctr + 1;
+}
+
+}  // namespace
+
+#include "fvm2d_batch.inc"
+#include "fvm2d_adjoint.inc"

@@ -172,6 +172,78 @@ int fvm_post_finalize(fvm_ctx* ctx, int* niter_pos);
  * distinct and not used by other threads during the call. */
 int fvm_ensemble_step(fvm_ctx* const* ctxs, int n, int nthreads, fvm_step_info* infos);
 
+/* ---------------------------------------------------------------------------------------------
+ * Batched native 2D solver: many independent 2D cylinder cases of uneven grid size and Reynolds
+ * number (the time step of cylinder/cylinder_v3.0.3_AF_original 4step_face_nl/, Main.f90:48-118;
+ * the reference has no ensemble driver: it runs one case per process) advanced together.  Every
+ * phase of the step is one launch over all cases; the Poisson defect correction runs a fixed
+ * number of passes (converged cases skip them), so the whole step is one captured hipGraph,
+ * replayed per step.  AF sweeps keep the reference's line order (lines one after the other).
+ * Fields are (NI+1)(NJ+1) planes, I fastest, as the 2D reference's RESTART.DAT holds them.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct fvm_batch2d fvm_batch2d;
+
+typedef struct fvm_case2d {
+  int ni, nj;            /* node counts NI (radial), NJ (circumferential)                    */
+  double re, dtime;      /* RE, DTIME                                                        */
+  const double* xg;      /* NI*NJ node coordinates, I fastest (the 2D grid file's records)  */
+  const double* yg;
+} fvm_case2d;
+
+/* GRID, METRIC, INIT geometry and the AF / Poisson factors of every case (host set-up in the
+ * reference's operations), device planes, block tables and the captured step.  mixed: the
+ * Poisson correction's transforms and mode-line storage in FP32 (residual, metrics, p in FP64);
+ * npass: Poisson correction passes per step (each case stops once RESIP <= resimax).
+ * NJ-1 must be even with prime factors 2, 3, 5, 7, 11, 13; NI-1, NJ-1 <= 512. */
+int fvm_batch2d_create(const fvm_case2d* cases, int n, int mixed, double resimax, int npass,
+                       fvm_batch2d** out);
+int fvm_batch2d_destroy(fvm_batch2d* b);
+/* Restart I/O of case c: FVM_U, FVM_V, FVM_P, FVM_NLX, FVM_NLY (set and get); get also
+ * FVM_DIFX, FVM_DIFY, FVM_SU, FVM_SV (P_SOURCE of the last pressure), FVM_UN, FVM_VN, the
+ * geometry FVM_X, FVM_Y, FVM_DJAC, FVM_XIX, FVM_XIY, FVM_ETX, FVM_ETY, FVM_THETA. */
+int fvm_batch2d_set_field(fvm_batch2d* b, int c, int field, const double* host);
+int fvm_batch2d_get_field(fvm_batch2d* b, int c, int field, double* host);
+int fvm_batch2d_set_dtime(fvm_batch2d* b, int c, double dtime);
+/* Wall velocity U_F, V_F of case c per wall cell J (NJ+1 values, J = 1..NJ-1 used): the hook
+ * of BCU/BCV (bcu.f90:10-20, bcv.f90:10-20; zero: the reference's fixed wall). */
+int fvm_batch2d_set_wall_velocity(fvm_batch2d* b, int c, const double* uf, const double* vf);
+/* nsteps time steps of every case; infos (n entries, may be NULL): the last step's RESIP,
+ * Poisson passes, CFL, C_pb, C_D, C_L and parts.  FVM_ERR_NAN / FVM_ERR_NOCONV as fvm_step. */
+int fvm_batch2d_step(fvm_batch2d* b, int nsteps, fvm_step_info* infos);
+/* The last nsteps steps' C_D, C_L, C_D pressure / friction, C_L pressure / friction, RESIP, CFL
+ * per case, oldest first: out[(s n + c) 8 + q]. */
+int fvm_batch2d_history(fvm_batch2d* b, int nsteps, double* out);
+/* Tile choices {stencil tile height 1|2|4, FFT columns per workgroup 1..16}; the autotuner times
+ * every admissible pair with reps steps each (state saved and restored) and keeps the fastest;
+ * ms_out (15 values, may be NULL) receives the timings. */
+int fvm_batch2d_set_tuning(fvm_batch2d* b, const int* choice);
+int fvm_batch2d_get_tuning(fvm_batch2d* b, int* choice);
+int fvm_batch2d_autotune(fvm_batch2d* b, int reps, int* choice, double* ms_out);
+
+/* Discrete adjoint of the batched step (the reference has no adjoint: these serve gradient-based
+ * flow control on top of its time step).  adjoint_init allocates the adjoint planes (once) and
+ * zeroes the adjoint state and the wall-velocity gradients.  adjoint_step maps the adjoint of
+ * the state after a step to the adjoint before it; the batch's state must be the state BEFORE
+ * that step (recomputed from a checkpoint).  weights (n values): the weight of each case's C_D
+ * after the step in the objective.  The adjoint Poisson equation is solved to the scaled
+ * residual tol (at most maxpass corrections; ares, may be NULL, gets the worst case's residual).
+ * adjoint_wall: the accumulated d(objective)/d(U_F, V_F) of case c per wall cell J (NJ+1
+ * values).  adjoint_get: the adjoint state of case c (0 u, 1 v, 2 p, 3 NLX, 4 NLY).
+ * save / load: device snapshots of every case's u, v, p, NLX, NLY in numbered slots (the
+ * checkpoints of the recompute). */
+int fvm_batch2d_adjoint_init(fvm_batch2d* b);
+int fvm_batch2d_adjoint_step(fvm_batch2d* b, const double* weights, double tol, int maxpass,
+                             double* ares);
+int fvm_batch2d_adjoint_wall(fvm_batch2d* b, int c, double* gu, double* gv);
+int fvm_batch2d_adjoint_get(fvm_batch2d* b, int c, int which, double* host);
+int fvm_batch2d_save(fvm_batch2d* b, int slot);
+int fvm_batch2d_load(fvm_batch2d* b, int slot);
+
+/* Host-only (no device call): the 2D set-up arrays X, Y, DJAC, XIX, XIY, ETX, ETY, THETA,
+ * DJAC Q12 of one case (9 planes of (NI+1)(NJ+1)) (grid.f90:47-115, metric.f90:18-145,
+ * init.f90:71-99). */
+int fvm_geometry2d(int ni, int nj, const double* xg, const double* yg, double* out);
+
 /* Change DTIME between steps.  DTIME is a run-time module variable of the reference, reassigned
  * by POST_PROCESS (RLES/post_process.f90:59) and by the CFL control of CALCCOEF
  * (RLES/calccoef.f90:27-33, commented out there); the dt-dependent coefficient 2 Re J^-1/DTIME
