@@ -192,6 +192,27 @@ static double* field_ptr(FvmCtx* c, int f) {
   }
 }
 
+// The rest of CALCCOEF after the force kernels: the time-average accumulation and the scalars
+// of info from the eight coefficients o8 and the step's dynamic coefficients.
+int fvm_calccoef_info(FvmCtx* c, const double* o, fvm_step_info* info) {
+  int r = fvm_k_post_accum(c);  // calccoef.f90:286-307 while the time averages run
+  if (r) return r;
+  if (info) {
+    info->cfl = o[0]; info->cpb = o[1]; info->cd = o[2]; info->cl = o[3];
+    info->cd_p = o[4]; info->cd_v = o[5]; info->cl_p = o[6]; info->cl_v = o[7];
+    // the dynamic coefficients of this step's STRESS_TENSOR_DY and the Nusselt number
+    double sc[2] = {0.0, 0.0};
+    HIP_CHECK(hipMemcpy(sc, c->scal, 2 * sizeof(double), hipMemcpyDeviceToHost));
+    info->c_avg = c->prm.use_les ? sc[0] : 0.0;
+    info->c_t = (c->thermal && c->prm.use_les) ? sc[1] : 0.0;
+    info->nu = 0.0;
+    if (c->thermal && !c->two_d) {
+      HIP_CHECK(hipMemcpy(&info->nu, c->scal + 16, sizeof(double), hipMemcpyDeviceToHost));
+    }
+  }
+  return FVM_OK;
+}
+
 extern "C" {
 
 static int create_impl(const fvm_params* prm, const double* xg, const double* yg,
@@ -537,22 +558,7 @@ int fvm_calccoef(fvm_ctx* c, fvm_step_info* info) {
   double o[8];
   int r = c->two_d ? fvm_k_calccoef2d(c, o) : fvm_k_calccoef(c, o);
   if (r) return r;
-  r = fvm_k_post_accum(c);  // calccoef.f90:286-307 while the time averages run
-  if (r) return r;
-  if (info) {
-    info->cfl = o[0]; info->cpb = o[1]; info->cd = o[2]; info->cl = o[3];
-    info->cd_p = o[4]; info->cd_v = o[5]; info->cl_p = o[6]; info->cl_v = o[7];
-    // the dynamic coefficients of this step's STRESS_TENSOR_DY and the Nusselt number
-    double sc[2] = {0.0, 0.0};
-    HIP_CHECK(hipMemcpy(sc, c->scal, 2 * sizeof(double), hipMemcpyDeviceToHost));
-    info->c_avg = c->prm.use_les ? sc[0] : 0.0;
-    info->c_t = (c->thermal && c->prm.use_les) ? sc[1] : 0.0;
-    info->nu = 0.0;
-    if (c->thermal && !c->two_d) {
-      HIP_CHECK(hipMemcpy(&info->nu, c->scal + 16, sizeof(double), hipMemcpyDeviceToHost));
-    }
-  }
-  return FVM_OK;
+  return fvm_calccoef_info(c, o, info);
 }
 
 int fvm_step(fvm_ctx* c, fvm_step_info* info) {

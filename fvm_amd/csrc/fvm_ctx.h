@@ -196,10 +196,14 @@ int fvm_k_correct(FvmCtx* c, double sign);
 int fvm_k_psource_correct(FvmCtx* c, double sign, bool with_cfl);
 int fvm_k_correct_bcu_contra(FvmCtx* c);
 int fvm_k_pref(FvmCtx* c, double* pref_host);
+// CALCCOEF; part (cap doubles) holds the CFL and force partial sums, nullptr: the context's own
+// reduction buffer.  fvm_calccoef_scratch: the doubles of part one call needs.
 int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready = false,
-                   const Mailbox* mb = nullptr);
+                   const Mailbox* mb = nullptr, double* part = nullptr, long cap = 0);
 int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready = false,
-                     const Mailbox* mb = nullptr);
+                     const Mailbox* mb = nullptr, double* part = nullptr, long cap = 0);
+long fvm_calccoef_scratch(const FvmCtx* c, bool cfl_ready);
+int fvm_calccoef_info(FvmCtx* c, const double* o8, fvm_step_info* info);
 int fvm_wall2d_setup(FvmCtx* c, const double* xg, const double* yg);
 int fvm_k_sgs(FvmCtx* c, bool sgs_out = true);
 int fvm_k_sgs_out(FvmCtx* c);

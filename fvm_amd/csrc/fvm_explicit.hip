@@ -830,21 +830,30 @@ int fvm_k_pref(FvmCtx* c, double* pref_host) {
 }
 
 // cfl_ready: the CFL partial maxima were left in c->cflpart by the projection of this step
-int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready, const Mailbox* mb) {
+long fvm_calccoef_scratch(const FvmCtx* c, bool cfl_ready) {
+  const long ncfl = cfl_ready ? 0 : 512;
+  if (c->two_d) return ncfl;
+  const long nw2 = 2L * (c->d.nj - 1) * (c->d.nk - 1);
+  return ncfl + 5L * ((nw2 + 255) / 256);
+}
+
+int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready, const Mailbox* mb, double* part,
+                   long cap) {
   double** m = c->dm;
   ProfScope ps(c, "calccoef");
+  if (!part) { part = c->red; cap = c->nred; }
+  if (fvm_calccoef_scratch(c, cfl_ready) > cap) return FVM_ERR_SIZE;
   const int ncfl = cfl_ready ? c->ncflused : 512;
-  const double* cfl = cfl_ready ? c->cflpart : c->red;
+  const double* cfl = cfl_ready ? c->cflpart : part;
   const int off = cfl_ready ? 0 : ncfl;
   if (!cfl_ready)
     hipLaunchKernelGGL(k_cfl, dim3(ncfl), dim3(256), 0, c->stream, c->d, (const double* const*)m,
-                       c->u, c->v, c->w, c->prm.dtime, c->red);
+                       c->u, c->v, c->w, c->prm.dtime, part);
   const long nw2 = 2L * (c->d.nj - 1) * (c->d.nk - 1);
   const int nfb = (int)((nw2 + 255) / 256);
-  if (off + 5L * nfb > c->nred) return FVM_ERR_SIZE;
   hipLaunchKernelGGL(k_forces_part, dim3(nfb), dim3(256), 0, c->stream, c->d, c->wall, c->p,
-                     c->u, c->v, c->w, c->thermal ? c->t : nullptr, c->prm.re, c->red + off);
-  hipLaunchKernelGGL(k_forces_final, dim3(1), dim3(256), 0, c->stream, c->d, c->p, c->red + off,
+                     c->u, c->v, c->w, c->thermal ? c->t : nullptr, c->prm.re, part + off);
+  hipLaunchKernelGGL(k_forces_final, dim3(1), dim3(256), 0, c->stream, c->d, c->p, part + off,
                      nfb, c->prm.ref_a, c->thermal ? c->sum_area : 0.0, cfl, ncfl, c->scal + 8,
                      mb ? *mb : Mailbox{nullptr, nullptr, 0});
   HIP_CHECK(hipGetLastError());
@@ -856,14 +865,17 @@ int fvm_k_calccoef(FvmCtx* c, double* out8, bool cfl_ready, const Mailbox* mb) {
   return FVM_OK;
 }
 
-int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready, const Mailbox* mb) {
+int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready, const Mailbox* mb, double* part,
+                     long cap) {
   double** m = c->dm;
   ProfScope ps(c, "calccoef");
+  if (!part) { part = c->red; cap = c->nred; }
+  if (fvm_calccoef_scratch(c, cfl_ready) > cap) return FVM_ERR_SIZE;
   const int ncfl = cfl_ready ? c->ncflused : 512;
-  const double* cfl = cfl_ready ? c->cflpart : c->red;
+  const double* cfl = cfl_ready ? c->cflpart : part;
   if (!cfl_ready)
     hipLaunchKernelGGL(k_cfl, dim3(ncfl), dim3(256), 0, c->stream, c->d, (const double* const*)m,
-                       c->u, c->v, c->w, c->prm.dtime, c->red);
+                       c->u, c->v, c->w, c->prm.dtime, part);
   const Mailbox none{nullptr, nullptr, 0};
   hipLaunchKernelGGL(k_calccoef2d, dim3(1), dim3(256), 0, c->stream, c->d, c->wall2d, c->p, c->u,
                      c->v, c->prm.re, 0.0, cfl, ncfl, c->scal, mb ? *mb : none);

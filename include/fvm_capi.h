@@ -268,6 +268,14 @@ int fvm_iter_p(fvm_ctx* ctx, double* resip, int* niter); /* ITER_P + BCP (iter_p
 int fvm_pressure_reference(fvm_ctx* ctx, double* pref); /* RLES/main.f90:125-135          */
 int fvm_calccoef(fvm_ctx* ctx, fvm_step_info* info);    /* CALCCOEF (calccoef.f90:1-201)  */
 
+/* fvm_calccoef without any device allocation: the same kernels and results, with the CFL and
+ * force partial sums in a static device array of 1 MiB (131072 doubles) per GPU instead of the
+ * context's reduction buffer.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory at call time (graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Grids needing more than the array holds (about 3.3 million wall cells) return
+ * FVM_ERR_SIZE.  Every call on one GPU shares the array: calls that overlap on one GPU race. */
+int fvm_calccoef_noalloc(fvm_ctx* ctx, fvm_step_info* info);
+
 /* Kernel timing (HIP events on the context's stream).  fvm_profile_get fills up to cap
  * entries of names (each <= 47 chars) / total ms / call counts and returns the count. */
 int fvm_profile_enable(fvm_ctx* ctx, int on);
