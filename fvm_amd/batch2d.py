@@ -46,6 +46,8 @@ def _bind(L):
     L.fvm_batch2d_set_tuning.argtypes = [vp, pi]
     L.fvm_batch2d_get_tuning.argtypes = [vp, pi]
     L.fvm_batch2d_autotune.argtypes = [vp, ci, pi, pd]
+    L.fvm_batch2d_profile.argtypes = [vp, ci, ctypes.c_char_p, pd, ctypes.POINTER(ctypes.c_long),
+                                      ci]
     L.fvm_batch2d_adjoint_init.argtypes = [vp]
     L.fvm_batch2d_adjoint_step.argtypes = [vp, pd, cd, ci, pd]
     L.fvm_batch2d_adjoint_wall.argtypes = [vp, ci, pd, pd]
@@ -57,7 +59,7 @@ def _bind(L):
               "fvm_batch2d_step", "fvm_batch2d_history", "fvm_batch2d_set_tuning",
               "fvm_batch2d_get_tuning", "fvm_batch2d_autotune", "fvm_batch2d_adjoint_init",
               "fvm_batch2d_adjoint_step", "fvm_batch2d_adjoint_wall", "fvm_batch2d_adjoint_get",
-              "fvm_batch2d_save", "fvm_batch2d_load"):
+              "fvm_batch2d_save", "fvm_batch2d_load", "fvm_batch2d_profile"):
         getattr(L, f).restype = ci
     L._batch2d_bound = True
     return L
@@ -146,6 +148,19 @@ class Batch2D:
         capi.check(self.L.fvm_batch2d_history(self.h, int(nsteps), capi._dptr(out)),
                    "batch2d history")
         return out.reshape(nsteps, self.n, 8)
+
+    def profile(self, nsteps):
+        """nsteps eager steps with HIP events around every launch: {kernel: (total ms, calls)}
+        (these steps advance the run like ordinary steps)."""
+        cap = 64
+        names = ctypes.create_string_buffer(48 * cap)
+        ms = np.zeros(cap)
+        calls = (ctypes.c_long * cap)()
+        q = self.L.fvm_batch2d_profile(self.h, int(nsteps), names, capi._dptr(ms), calls, cap)
+        if q < 0:
+            raise capi.FvmError(-q, "fvm_batch2d_profile")
+        return {names.raw[48 * k:48 * k + 48].split(b"\0")[0].decode(): (float(ms[k]), int(calls[k]))
+                for k in range(q)}
 
     def tuning(self):
         ch = (ctypes.c_int * 2)()

@@ -71,10 +71,13 @@ EXPORTS = ("fvm_create", "fvm_destroy", "fvm_set_field", "fvm_get_field", "fvm_s
            "fvm_calccoef", "fvm_profile_enable", "fvm_profile_get", "fvm_profile_reset",
            "fvm_sync", "fvm_set_dtime", "fvm_create2d", "fvm_set_thermal", "fvm_iter_t",
            "fvm_writedata_fields", "fvm_post_begin", "fvm_post_finalize", "fvm_ensemble_step",
-           "fvm_calccoef_noalloc", "fvm_batch2d_create", "fvm_batch2d_destroy",
+           "fvm_calccoef_noalloc", "fvm_copy_field", "fvm_nl_c", "fvm_dif_c",
+           "fvm_stress_tensor_dy_c", "fvm_add_sgs_c", "fvm_iter_u_c", "fvm_correct_c", "fvm_bcu",
+           "fvm_bcp", "fvm_batch2d_create", "fvm_batch2d_destroy",
            "fvm_batch2d_set_field", "fvm_batch2d_get_field", "fvm_batch2d_set_dtime",
            "fvm_batch2d_set_wall_velocity", "fvm_batch2d_step", "fvm_batch2d_history",
            "fvm_batch2d_set_tuning", "fvm_batch2d_get_tuning", "fvm_batch2d_autotune",
+           "fvm_batch2d_profile",
            "fvm_batch2d_adjoint_init", "fvm_batch2d_adjoint_step", "fvm_batch2d_adjoint_wall",
            "fvm_batch2d_adjoint_get", "fvm_batch2d_save", "fvm_batch2d_load", "fvm_geometry2d")
 
@@ -114,6 +117,12 @@ def load(path: str = LIB_PATH):
     L.fvm_pressure_reference.argtypes = [vp, pd]
     L.fvm_calccoef.argtypes = [vp, ctypes.POINTER(StepInfo)]
     L.fvm_calccoef_noalloc.argtypes = [vp, ctypes.POINTER(StepInfo)]
+    for f in ("fvm_copy_field", "fvm_nl_c", "fvm_dif_c", "fvm_add_sgs_c", "fvm_bcu"):
+        getattr(L, f).argtypes = [vp, ci, ci]
+    L.fvm_stress_tensor_dy_c.argtypes = [vp, pd]
+    L.fvm_iter_u_c.argtypes = [vp, ci, ci, ci, ci, ci, ci]
+    L.fvm_correct_c.argtypes = [vp, ci, ci, cd]
+    L.fvm_bcp.argtypes = [vp]
     L.fvm_profile_enable.argtypes = [vp, ci]
     L.fvm_profile_get.argtypes = [vp, ctypes.c_char_p, pd, ctypes.POINTER(ctypes.c_long), ci]
     for f in EXPORTS:

@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "../../include/fvm_capi.h"

@@ -133,14 +133,14 @@ __global__ void k_af_products(Dims d, const double* __restrict__ J, const double
 // A.dif = DIFT) and BCT's wall fold adds 2 AW T_WALL (bct.f90:16-19).
 constexpr int RK_KC = 16;
 
-template <int S, bool TH>
+template <int S, bool TH, int NCV = 3>
 __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
                                                    const double* const* __restrict__ P,
                                                    const double* __restrict__ ap0, double re,
                                                    const double* const* __restrict__ fa,
                                                    int qform, int fn_alias) {
   constexpr double kTWall = 1.0;  // T_WALL (comdat_param.f90:40)
-  constexpr int NC = TH ? 1 : 3;  // components: T, or u, v, w
+  constexpr int NC = TH ? 1 : NCV;  // components: T, or u, v, w (NCV = 1: one of them)
   const uint3 bb = xcd_block();
   __shared__ double sh[NC][3][SF_P];  // [component][plane slot][tile cell]
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
@@ -838,6 +838,89 @@ int fvm_k_iter_u(FvmCtx* c) {
       if (s == 2) r = chunk_stage<2, 8>(c, A, &it);
       if (r) return r;
       c->af_iters[s] = it;
+    }
+  }
+  return FVM_OK;
+}
+
+// ITER_U(IN, INM, INA, INB, INC, IND) for one velocity component (RLES/iter_u.f90:1-262 as
+// MAIN calls it, main.f90:82-88): f = IN (the field, updated), fn = INM (its step-start copy),
+// nl / nl1 / dif / s = INA..IND; comp (0 u, 1 v, 2 w) picks BCU's far-field values, i.e. the UIN
+// MAIN sets before the call.  The same stages as fvm_k_iter_u with one component per launch;
+// grids whose stage couplings need the chunked fallback are refused (FVM_ERR_ARG).
+int fvm_k_iter_u_one(FvmCtx* c, int comp, double* f, const double* fn, const double* nl,
+                     const double* nl1, const double* dif, const double* s_) {
+  const Dims d = c->d;
+  hipStream_t st = c->stream;
+  if (comp < 0 || comp > 2) return FVM_ERR_ARG;
+  if (d.ni - 1 > FVM_MAX_LINE || d.nj - 1 > FVM_MAX_LINE || d.nk - 1 > FVM_MAX_LINE)
+    return FVM_ERR_SIZE;
+  c->u_is_un = false;
+  const dim3 gi((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
+  const int nstage = c->two_d ? 2 : 3;
+  const long nf = (long)(d.nj + 1) * (d.nk + 1);
+  for (int s = 0; s < nstage; ++s) {
+    const bool exact_k = s == 2 && c->prm.af_mode == 0 && c->tk_a && c->af_zero_a[2] &&
+                         c->af_zero_b[2];
+    const bool scan = s < 2 && c->prm.af_mode == 0 && c->af_scan[s];
+    if (!exact_k && !scan && c->prm.af_mode != 1) {
+      fprintf(stderr, "fvm_amd: ITER_U of one component needs line-separable stage couplings "
+                      "(stage %d is not); use fvm_iter_u\n", s);
+      return FVM_ERR_ARG;
+    }
+    AFArgs A;
+    A.nc = 1;
+    for (int q = 0; q < 3; ++q) {
+      A.f[q] = f; A.fn[q] = fn; A.nl[q] = nl; A.nl1[q] = nl1; A.dif[q] = dif; A.s[q] = s_;
+      A.su[q] = c->su[comp];
+    }
+    A.ufar = c->ufar + comp * nf;
+    {
+      ProfScope ps(c, "af_rhs");
+      const int qf = scan ? 1 : 0;
+      const double* const* fas = (const double* const*)c->dfa[s];
+      const int dz = c->dd2_zero ? 1 : 0;
+      const dim3 gk((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + RK_KC - 1) / RK_KC);
+      if (s == 0)
+        hipLaunchKernelGGL((k_af_rhs_km<0, false, 1>), gk, kBlk, 0, st, d, A,
+                           (const double* const*)c->dafp[0], c->ap0, c->prm.re, fas, qf, 0);
+      if (s == 1)
+        hipLaunchKernelGGL((k_af_rhs_km<1, false, 1>), gk, kBlk, 0, st, d, A,
+                           (const double* const*)c->dafp[1], c->ap0, c->prm.re, fas, qf, 0);
+      if (s == 2)
+        hipLaunchKernelGGL(k_af_rhs<2>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm,
+                           c->djac, c->ap0, c->prm.re, fas, qf, dz);
+      HIP_CHECK(hipGetLastError());
+    }
+    int r = fvm_k_bcu_ghosts_one(c, f, comp);  // BCU(IN, INM), iter_u.f90:83,160,237
+    if (r) return r;
+    ProfScope ps(c, s == 0 ? "af_sweep_i" : (s == 1 ? "af_sweep_j" : "af_sweep_k"));
+    if (exact_k) {
+      const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, 1);
+      if (d.nk - 1 <= 64)
+        hipLaunchKernelGGL(k_af_thomas_k_reg<64>, g, dim3(64, 4, 1), 0, st, d, A, c->tk_a,
+                           c->tk_d, c->tk_b);
+      else
+        hipLaunchKernelGGL(k_af_thomas_k, g, dim3(64, 4, 1), 0, st, d, A, c->tk_a, c->tk_d,
+                           c->tk_b);
+      HIP_CHECK(hipGetLastError());
+    } else if (scan) {
+      int it = 0;
+      r = fvm_af_scan_stage(c, s, A, &it);
+      if (r) return r;
+    } else {  // af_mode 1: anti-diagonal wavefront launches
+      const int na = s == 0 ? d.nj - 1 : d.ni - 1;
+      const int nb = s == 2 ? d.nj - 1 : d.nk - 1;
+      const double* const* fa = (const double* const*)c->dfa[s];
+      for (int h = 0; h <= na + nb - 2; ++h) {
+        int lo = h - (nb - 1) > 0 ? h - (nb - 1) : 0;
+        int hi = h < na - 1 ? h : na - 1;
+        dim3 g(hi - lo + 1, 1);
+        if (s == 0) hipLaunchKernelGGL(k_af_sweep<0>, g, dim3(64), 0, st, d, A, fa, c->ap0, h);
+        if (s == 1) hipLaunchKernelGGL(k_af_sweep<1>, g, dim3(64), 0, st, d, A, fa, c->ap0, h);
+        if (s == 2) hipLaunchKernelGGL(k_af_sweep<2>, g, dim3(64), 0, st, d, A, fa, c->ap0, h);
+      }
+      HIP_CHECK(hipGetLastError());
     }
   }
   return FVM_OK;

@@ -184,12 +184,31 @@ static double* field_ptr(FvmCtx* c, int f) {
     case FVM_PG: case FVM_UG: case FVM_VG: case FVM_WG: case FVM_TG: case FVM_VORG:
     case FVM_LAMDAG: return c->nodal[f - FVM_PG];
     case FVM_YPLUS: return c->yplus;
+    case FVM_NLX_NM1: return c->nl1[0];
+    case FVM_NLY_NM1: return c->nl1[1];
+    case FVM_NLZ_NM1: return c->nl1[2];
+    case FVM_ADV_NM1: return c->adv1;
     default:
       if (f >= FVM_SUM_U && f <= FVM_SUM_SGS33) return c->post_sum[f - FVM_SUM_U];
       if (f >= FVM_AVG_PG && f <= FVM_RS_WW) return c->post_node[f - FVM_AVG_PG];
       if (f >= FVM_SURF_CF && f <= FVM_SURF_NU) return c->post_surf[f - FVM_SURF_CF];
       return nullptr;
   }
+}
+
+double* fvm_field(FvmCtx* c, int f) { return field_ptr(c, f); }
+
+// The far-field values of BCU are formed from U_N; after U_N changed through the
+// per-subroutine entry points they are formed again before the next BCU.
+static int ufar_fresh(FvmCtx* c) {
+  if (!c->ufar_stale) return FVM_OK;
+  c->ufar_stale = false;
+  return fvm_k_ufar(c);
+}
+
+// velocity component (0 u, 1 v, 2 w) of a field selector, -1 for other fields
+static int vel_comp(int f) {
+  return f == FVM_U ? 0 : (f == FVM_V ? 1 : (f == FVM_W ? 2 : -1));
 }
 
 // The rest of CALCCOEF after the force kernels: the time-average accumulation and the scalars
@@ -397,6 +416,7 @@ int fvm_set_field(fvm_ctx* c, int field, const double* host) {
   rs = fvm_k_nut_out(c);
   if (rs) return rs;
   if (field >= FVM_XIX && field <= FVM_ZTZ) c->xz_zero = false;  // re-derived only at set-up
+  if (field == FVM_UN || field == FVM_VN || field == FVM_WN) c->ufar_stale = true;
   std::vector<double> slab;
   if (c->two_d) {  // one (NI+1)(NJ+1) plane, repeated on the slab's three K planes
     slab.resize(c->d.n);
@@ -443,6 +463,7 @@ int fvm_begin_step(fvm_ctx* c) {
     if (r) return r;
   }
   // far-field boundary values depend only on U_N: one evaluation serves every BCU of the step
+  c->ufar_stale = false;
   return fvm_k_ufar(c);
 }
 
@@ -507,6 +528,98 @@ int fvm_post_finalize(fvm_ctx* c, int* niter_pos) {
   if (niter_pos) *niter_pos = (int)np;
   return FVM_OK;
 }
+
+// ---- the reference's subroutines with their own arguments (one array per call) ----
+
+int fvm_copy_field(fvm_ctx* c, int dst, int src) {
+  if (!c) return FVM_ERR_ARG;
+  double* a = field_ptr(c, dst);
+  const double* b = field_ptr(c, src);
+  if (!a || !b) return FVM_ERR_ARG;
+  if (a == b) return FVM_OK;
+  int rs = fvm_k_sgs_out(c);  // the lazily formed SGS_x,y,z and nu_t, before inputs change
+  if (rs) return rs;
+  rs = fvm_k_nut_out(c);
+  if (rs) return rs;
+  HIP_CHECK(hipMemcpyAsync(a, b, c->d.n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  if (dst == FVM_UN || dst == FVM_VN || dst == FVM_WN) c->ufar_stale = true;
+  if (dst == FVM_U || dst == FVM_V || dst == FVM_W || dst == FVM_UN || dst == FVM_VN ||
+      dst == FVM_WN)
+    c->u_is_un = false;
+  return FVM_OK;
+}
+
+int fvm_nl_c(fvm_ctx* c, int in, int out) {
+  if (!c) return FVM_ERR_ARG;
+  const double* a = field_ptr(c, in);
+  double* b = field_ptr(c, out);
+  if (!a || !b || a == b) return FVM_ERR_ARG;
+  return fvm_k_nl_one(c, a, b);
+}
+
+int fvm_dif_c(fvm_ctx* c, int in, int out) {
+  if (!c) return FVM_ERR_ARG;
+  const double* a = field_ptr(c, in);
+  double* b = field_ptr(c, out);
+  if (!a || !b || a == b) return FVM_ERR_ARG;
+  return fvm_k_dif_one(c, a, b);
+}
+
+int fvm_stress_tensor_dy_c(fvm_ctx* c, double* c_avg) {
+  if (!c || c->two_d) return FVM_ERR_ARG;
+  int r = ufar_fresh(c);  // BCU of the test-filtered velocities takes the far-field values
+  if (r) return r;
+  r = fvm_k_sgs(c, true, false);
+  if (r) return r;
+  if (c_avg) {
+    HIP_CHECK(hipMemcpyAsync(c_avg, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
+  return FVM_OK;
+}
+
+int fvm_add_sgs_c(fvm_ctx* c, int nl, int sgs) {
+  if (!c) return FVM_ERR_ARG;
+  double* a = field_ptr(c, nl);
+  const double* b = field_ptr(c, sgs);
+  if (!a || !b || a == b) return FVM_ERR_ARG;
+  return fvm_k_add_sgs(c, a, b);
+}
+
+int fvm_iter_u_c(fvm_ctx* c, int in, int inm, int ina, int inb, int inc, int ind) {
+  if (!c) return FVM_ERR_ARG;
+  const int comp = vel_comp(in);
+  double* f = field_ptr(c, in);
+  const double* fn = field_ptr(c, inm);
+  const double* nl = field_ptr(c, ina);
+  const double* nl1 = field_ptr(c, inb);
+  const double* dif = field_ptr(c, inc);
+  const double* s = field_ptr(c, ind);
+  if (comp < 0 || !f || !fn || !nl || !nl1 || !dif || !s || fn == f) return FVM_ERR_ARG;
+  int r = ufar_fresh(c);
+  if (r) return r;
+  return fvm_k_iter_u_one(c, comp, f, fn, nl, nl1, dif, s);
+}
+
+int fvm_correct_c(fvm_ctx* c, int f, int s, double sign) {
+  if (!c) return FVM_ERR_ARG;
+  double* a = field_ptr(c, f);
+  const double* b = field_ptr(c, s);
+  if (!a || !b || a == b) return FVM_ERR_ARG;
+  c->u_is_un = false;
+  return fvm_k_correct_one(c, a, b, sign);
+}
+
+int fvm_bcu(fvm_ctx* c, int a, int b) {
+  if (!c) return FVM_ERR_ARG;
+  const int comp = vel_comp(a);
+  if (comp < 0 || b != FVM_UN + comp) return FVM_ERR_ARG;
+  int r = ufar_fresh(c);
+  if (r) return r;
+  return fvm_k_bcu_ghosts_one(c, field_ptr(c, a), comp);
+}
+
+int fvm_bcp(fvm_ctx* c) { return c ? fvm_k_bcp(c) : FVM_ERR_ARG; }
 
 int fvm_contra_vel(fvm_ctx* c) { return c ? fvm_k_contra_vel(c) : FVM_ERR_ARG; }
 int fvm_nl(fvm_ctx* c) { return c ? fvm_k_nl(c) : FVM_ERR_ARG; }

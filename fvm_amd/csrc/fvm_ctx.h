@@ -42,6 +42,7 @@ struct FvmCtx {
   // u, v, w still equal U_N, V_N, W_N bit for bit (set by the step-start copy, cleared by every
   // entry point that may write them): the first AF stage then takes U_N from its staged input
   bool u_is_un = false;
+  bool ufar_stale = false;  // U_N changed since the far-field values were formed
   double* nl[3] = {};
   double* nl1[3] = {};
   double* dif[3] = {};
@@ -193,6 +194,15 @@ int fvm_k_post_begin(FvmCtx* c, const double* t_const);
 int fvm_k_post_accum(FvmCtx* c);
 int fvm_k_post_finalize(FvmCtx* c);
 int fvm_k_correct(FvmCtx* c, double sign);
+// one component at a time (the per-subroutine entry points)
+int fvm_k_nl_one(FvmCtx* c, const double* in, double* out);
+int fvm_k_dif_one(FvmCtx* c, const double* in, double* out);
+int fvm_k_correct_one(FvmCtx* c, double* f, const double* s, double sign);
+int fvm_k_add_sgs(FvmCtx* c, double* nl, const double* sgs);
+int fvm_k_bcu_ghosts_one(FvmCtx* c, double* f, int comp);
+int fvm_k_iter_u_one(FvmCtx* c, int comp, double* f, const double* fn, const double* nl,
+                     const double* nl1, const double* dif, const double* s);
+double* fvm_field(FvmCtx* c, int f);
 int fvm_k_psource_correct(FvmCtx* c, double sign, bool with_cfl);
 int fvm_k_correct_bcu_contra(FvmCtx* c);
 int fvm_k_pref(FvmCtx* c, double* pref_host);
@@ -205,7 +215,7 @@ int fvm_k_calccoef2d(FvmCtx* c, double* out8, bool cfl_ready = false,
 long fvm_calccoef_scratch(const FvmCtx* c, bool cfl_ready);
 int fvm_calccoef_info(FvmCtx* c, const double* o8, fvm_step_info* info);
 int fvm_wall2d_setup(FvmCtx* c, const double* xg, const double* yg);
-int fvm_k_sgs(FvmCtx* c, bool sgs_out = true);
+int fvm_k_sgs(FvmCtx* c, bool sgs_out = true, bool nl_update = true);
 int fvm_k_sgs_out(FvmCtx* c);
 int fvm_k_nut_out(FvmCtx* c);
 int fvm_k_iter_u(FvmCtx* c);

@@ -412,6 +412,25 @@ __global__ void k_correct(Dims d, double sign, double dtime, const double* __res
   }
 }
 
+// one component of the predictor correction / projection: f = f + sign DTIME s / J
+// (MAIN's array statements, RLES/main.f90:90-92, 108-111)
+__global__ void k_correct1(Dims d, double sign, double dtime, const double* __restrict__ djac,
+                           double* f, const double* __restrict__ s) {
+  INTERIOR_IJK(d);
+  const double J = djac[c];
+  if (sign < 0)
+    f[c] = f[c] - dtime * s[c] / J;
+  else
+    f[c] = f[c] + dtime * s[c] / J;
+}
+
+// NL = NL + SGS J (RLES/main.f90:61-66) for one component
+__global__ void k_add_sgs(Dims d, const double* __restrict__ djac, double* nl,
+                          const double* __restrict__ sgs) {
+  INTERIOR_IJK(d);
+  nl[c] = nl[c] + sgs[c] * djac[c];
+}
+
 // span mean of p at (IE, NJ/4, K), summed in the reference's order by one lane
 // (RLES/main.f90:125-131)
 __global__ void k_pref(Dims d, const double* __restrict__ p, double* out) {
@@ -803,6 +822,60 @@ int fvm_k_bcu_ghosts(FvmCtx* c, double** f, int cnt) {
   for (int q = 0; q < cnt; ++q) set.p[q] = f[q];
   hipLaunchKernelGGL(k_bc_ghosts, dim3(grid1(bc_ghost_count(c->d), 256).x, cnt), dim3(256), 0,
                      c->stream, c->d, set, c->ufar, 0.0);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// one component at a time: the reference's subroutines with their own arguments
+// ----------------------------------------------------------------------------------------
+int fvm_k_nl_one(FvmCtx* c, const double* in, double* out) {
+  ProfScope ps(c, "nl");
+  CompSet cs{};
+  cs.in[0] = in;
+  cs.out[0] = out;
+  hipLaunchKernelGGL(k_nl, interior_grid(c->d), kBlk, 0, c->stream, c->d, c->ju[0], c->ju[1],
+                     c->ju[2], cs, 1);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_k_dif_one(FvmCtx* c, const double* in, double* out) {
+  ProfScope ps(c, "dif");
+  const Dims d = c->d;
+  const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + DF_KC - 1) / DF_KC);
+  CompSet cs{};
+  cs.in[0] = in;
+  cs.out[0] = out;
+  hipLaunchKernelGGL(k_dif_km<1>, g, kBlk, 0, c->stream, c->d, c->pdI, c->pdJ, c->pdK, c->jq12,
+                     c->jq13, c->jq23, cs);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_k_correct_one(FvmCtx* c, double* f, const double* s, double sign) {
+  ProfScope ps(c, "correct");
+  hipLaunchKernelGGL(k_correct1, interior_grid(c->d), kBlk, 0, c->stream, c->d, sign,
+                     c->prm.dtime, c->djac, f, s);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_k_add_sgs(FvmCtx* c, double* nl, const double* sgs) {
+  hipLaunchKernelGGL(k_add_sgs, interior_grid(c->d), kBlk, 0, c->stream, c->d, c->djac, nl, sgs);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+// BCU ghosts of one array with the far-field values of component comp (0 u, 1 v, 2 w)
+int fvm_k_bcu_ghosts_one(FvmCtx* c, double* f, int comp) {
+  if (comp < 0 || comp > 2) return FVM_ERR_ARG;
+  ProfScope ps(c, "bcu");
+  PtrSet8 set{};
+  set.p[0] = f;
+  const long nf = (long)(c->d.nj + 1) * (c->d.nk + 1);
+  hipLaunchKernelGGL(k_bc_ghosts, dim3(grid1(bc_ghost_count(c->d), 256).x, 1), dim3(256), 0,
+                     c->stream, c->d, set, c->ufar + comp * nf, 0.0);
   HIP_CHECK(hipGetLastError());
   return FVM_OK;
 }

@@ -680,7 +680,7 @@ int fvm_k_sgs_out(FvmCtx* c) {
   return FVM_OK;
 }
 
-int fvm_k_sgs(FvmCtx* c, bool sgs_out) {
+int fvm_k_sgs(FvmCtx* c, bool sgs_out, bool nl_update) {
   const Dims d = c->d;
   double** m = c->dm;
   ProfScope ps(c, "sgs");
@@ -735,7 +735,8 @@ int fvm_k_sgs(FvmCtx* c, bool sgs_out) {
     auto kern = c->xz_zero ? k_sgs_div<true> : k_sgs_div<false>;
     hipLaunchKernelGGL(kern, gs, kBlk, 0, st, d, (const double* const*)m, c->filt2, c->scal + 0,
                        (const double* const*)c->dlss, c->djac, c->nl[0], c->nl[1], c->nl[2],
-                       c->sgs[0], c->sgs[1], c->sgs[2], sgs_out ? 3 : 1);
+                       c->sgs[0], c->sgs[1], c->sgs[2],
+                       (nl_update ? 1 : 0) | (sgs_out ? 2 : 0));
     HIP_CHECK(hipGetLastError());
     c->sgs_stale = !sgs_out;
   }

@@ -97,7 +97,9 @@ enum {
   FVM_AVG_PG = 71, FVM_AVG_UG = 72, FVM_AVG_VG = 73, FVM_AVG_WG = 74, FVM_AVG_TG = 75,
   FVM_RMS_UG = 76, FVM_RMS_VG = 77, FVM_RMS_WG = 78, FVM_RS_UU = 79, FVM_RS_UV = 80,
   FVM_RS_UW = 81, FVM_RS_VV = 82, FVM_RS_VW = 83, FVM_RS_WW = 84, FVM_SURF_CF = 85,
-  FVM_SURF_CP = 86, FVM_SURF_NU = 87, FVM_NFIELDS = 88
+  FVM_SURF_CP = 86, FVM_SURF_NU = 87,
+  /* the convective history NLX_NM1, NLY_NM1, NLZ_NM1, ADV_NM1 (the previous step's NL) */
+  FVM_NLX_NM1 = 88, FVM_NLY_NM1 = 89, FVM_NLZ_NM1 = 90, FVM_ADV_NM1 = 91, FVM_NFIELDS = 92
 };
 
 /* GRID + METRIC + the geometric part of INIT (RLES/grid.f90:1-367, RLES/metric.f90:1-538,
@@ -219,6 +221,11 @@ int fvm_batch2d_history(fvm_batch2d* b, int nsteps, double* out);
 int fvm_batch2d_set_tuning(fvm_batch2d* b, const int* choice);
 int fvm_batch2d_get_tuning(fvm_batch2d* b, int* choice);
 int fvm_batch2d_autotune(fvm_batch2d* b, int reps, int* choice, double* ms_out);
+/* nsteps steps launched eagerly (the captured step's launches) with HIP events around each
+ * launch: per kernel the total ms and launch count, up to cap entries, names 48 chars each.
+ * Returns the number of kernels (negative: an error status). */
+int fvm_batch2d_profile(fvm_batch2d* b, int nsteps, char* names, double* ms, long* calls,
+                        int cap);
 
 /* Discrete adjoint of the batched step (the reference has no adjoint: these serve gradient-based
  * flow control on top of its time step).  adjoint_init allocates the adjoint planes (once) and
@@ -275,6 +282,35 @@ int fvm_calccoef(fvm_ctx* ctx, fvm_step_info* info);    /* CALCCOEF (calccoef.f9
  * hipMalloc).  Grids needing more than the array holds (about 3.3 million wall cells) return
  * FVM_ERR_SIZE.  Every call on one GPU shares the array: calls that overlap on one GPU race. */
 int fvm_calccoef_noalloc(fvm_ctx* ctx, fvm_step_info* info);
+
+/* The reference's subroutines one array at a time, with their own argument lists: each array
+ * argument is a field selector (the device array standing for the reference's module array).
+ * For a driver that keeps the reference's MAIN statement for statement (the Fortran shims of
+ * fvm_amd/fortran/fvm_shims.f90 bind the reference's names to these):
+ *   fvm_copy_field   MAIN's whole-array assignments: U_N = U (RLES/main.f90:41-44),
+ *                    NLX_NM1 = NLX (main.f90:51-54)
+ *   fvm_nl_c         NL(IN, OUT) (nl.f90:1-49): OUT from IN and CONTRA_VEL's face fluxes
+ *   fvm_dif_c        DIF(IN, OUT) (dif.f90:1-40, dif_diagonal.f90)
+ *   fvm_stress_tensor_dy_c  STRESS_TENSOR_DY (stress_tensor_dy.f90): SGS_X..Z and NU_T, NL
+ *                    unchanged; returns the dynamic coefficient
+ *   fvm_add_sgs_c    NLX = NLX + SGS_X * DJAC (main.f90:61-66)
+ *   fvm_iter_u_c     ITER_U(IN, INM, INA, INB, INC, IND) (iter_u.f90:1-262): IN = U, V or W
+ *                    (its BCU far-field value is the UIN MAIN sets for it: 1 for U, 0 else);
+ *                    the 2D context's ITER_U / ITER_V (2D iter_u.f90, iter_v.f90) likewise
+ *   fvm_correct_c    U = U + sign DTIME S_U / DJAC (main.f90:90-92, 108-111)
+ *   fvm_bcu          BCU(A, B) (bcu.f90:1-50): A = U, V or W, B its step-start copy
+ *   fvm_bcp          BCP (bcp.f90:1-38)
+ * P_SOURCE(P), CONTRA_VEL, ITER_P, CALCCOEF are fvm_p_source, fvm_contra_vel, fvm_iter_p,
+ * fvm_calccoef below. */
+int fvm_copy_field(fvm_ctx* ctx, int dst, int src);
+int fvm_nl_c(fvm_ctx* ctx, int in, int out);
+int fvm_dif_c(fvm_ctx* ctx, int in, int out);
+int fvm_stress_tensor_dy_c(fvm_ctx* ctx, double* c_avg);
+int fvm_add_sgs_c(fvm_ctx* ctx, int nl, int sgs);
+int fvm_iter_u_c(fvm_ctx* ctx, int in, int inm, int ina, int inb, int inc, int ind);
+int fvm_correct_c(fvm_ctx* ctx, int f, int s, double sign);
+int fvm_bcu(fvm_ctx* ctx, int a, int b);
+int fvm_bcp(fvm_ctx* ctx);
 
 /* Kernel timing (HIP events on the context's stream).  fvm_profile_get fills up to cap
  * entries of names (each <= 47 chars) / total ms / call counts and returns the count. */
