@@ -182,55 +182,142 @@ def copy_calibration(mib, log):
     log(f"copied {mib} MiB x 4")
 
 
+# Algorithmic bytes per interior cell of the batched 2D kernels, counted as KERNEL_BYTES (each
+# distinct operand once, FP64 8 B, FP32 4 B; line factors shared by the u and v waves once).
+KERNEL_BYTES_2D = {
+    # phase A: {u, v, p, J, xi_x, xi_y, eta_x, eta_y, D_E, D_N, J Q12, 2 Re J/dt, stage-1 cross
+    # weights, NLX, NLY} -> {U_N, V_N, NL_NM1 2, NL 2, DIF 2, S 2, SU 2}
+    "k2_explicit": 8 * 27,
+    # I-line sweep: factors {1/D, beta, B, AN} + per component {SU, x_old, x}
+    "k2_sweep_i": 8 * 10,
+    # J-line sweep: factors {1/D, beta, B, AE, AW} + per component {SU, x_old, x}
+    "k2_sweep_j": 8 * 11,
+    # stage-2 right-hand side: {u, v, 2 Re J/dt, cross weights} -> SU 2
+    "k2_rhs2": 8 * 6,
+    # predictor correction: {u, v, S 2, J} -> {u, v}
+    "k2_correct": 8 * 7,
+    # Poisson source: {u, v, J, 4 metrics} -> RHS
+    "k2_prhs": 8 * 8,
+    # residual: {p, RHS, D_E, D_N, J Q12} -> r (FP32 in the mixed form)
+    "k2_residual": 8 * 5 + 4,
+    # projection: {p, J, 4 metrics, u, v} -> {u, v, S 2}
+    "k2_project": 8 * 12,
+}
+
+
+def cpu_baseline_ensemble(cases, log, budget_s=20.0):
+    """The oracle (C restatement of the step on a one-cell span slab, W = 0, Poisson line
+    Gauss-Seidel to RESIP 1e-8) on a sample of the ensemble's cases, one host core per worker
+    process as the reference runs one case per process: per case one untimed and one timed step;
+    the rate is the sum of the per-case rates over the worker count."""
+    import concurrent.futures as cf
+    nproc = max(1, min(16, os.cpu_count() or 1))
+    sample = cases[:nproc]
+    t0 = time.perf_counter()
+    with cf.ProcessPoolExecutor(nproc) as ex:
+        res = list(ex.map(_oracle_case_step, [(c["ni"], c["nj"], c["re"], c["dt"], c["xg"],
+                                               c["yg"], c["ic"]) for c in sample]))
+    wall = time.perf_counter() - t0
+    cells = sum(r[0] for r in res)
+    secs = sum(r[1] for r in res)
+    log(f"cpu oracle ensemble sample: {len(sample)} cases on {nproc} processes, step time "
+        f"{secs:.2f} s summed, wall {wall:.1f} s, sweeps {min(r[2] for r in res)}-"
+        f"{max(r[2] for r in res)}, RESIP max {max(r[3] for r in res):.1e}")
+    return dict(value=cells / secs * nproc / 1e6, unit="Mcell-steps/s", cores=nproc,
+                kind="port",
+                sample=f"{len(sample)} of the ensemble's cases, one step each after one untimed "
+                       f"step, oracle C restatement (one-cell span slab, Poisson line GS to "
+                       f"RESIP 1e-8, cap 20000 sweeps), {nproc} worker processes of one core; "
+                       f"per-case step times summed {secs:.1f} s")
+
+
+def _oracle_case_step(args):
+    ni, nj, re, dt, xg, yg, ic = args
+    from oracle.oracle3d import Oracle3D
+    lz = 1.0e6
+    slab = lambda a, k: np.asfortranarray(np.repeat(np.asarray(a)[:, :, :1], k, axis=2))
+    zg = np.zeros((ni, nj, 2), order="F")
+    zg[:, :, 1] = lz
+    o = Oracle3D(ni, nj, 2, re, dt, lz, slab(xg, 2), slab(yg, 2), zg)
+    o.set_no_pref(True)
+    z = np.zeros((ni + 1, nj + 1, 3), order="F")
+    o.set_state(slab(ic["u"], 3), slab(ic["v"], 3), z, slab(ic["p"], 3), slab(ic["nlx"], 3),
+                slab(ic["nly"], 3), z)
+    o.step(20000, 1e-8, False)
+    t0 = time.perf_counter()
+    o.step(20000, 1e-8, False)
+    secs = time.perf_counter() - t0
+    return (ni - 1) * (nj - 1), secs, int(o.scalar("niter")), float(o.scalar("resip"))
+
+
 def run_ensemble(a, log):
-    """The ensemble of seeded 2D cases (uneven sizes, Re in [40, 300]) stepped together by
-    fvm_ensemble_step; the same steps with one host thread give the one-case-at-a-time rate."""
+    """The ensemble of seeded 2D cases (uneven sizes, Re in [40, 300]) as one batch
+    (fvm_batch2d_*): one launch per phase over all cases, the step captured as a hipGraph and
+    replayed, tiles chosen by the batch's autotuner.  Start-up at dt/10, then the time step."""
+    from fvm_amd.batch2d import batch_from_cases
     from fvm_amd.grid import ensemble_cases
-    from fvm_amd.solver import ensemble2d
-    cases = ensemble_cases(a.cases)
+    cases = ensemble_cases(a.cases, dt=a.ens_dt, dt_rule="uniform")
     ncell = sum((c["ni"] - 1) * (c["nj"] - 1) for c in cases)
-    log(f"ensemble: {len(cases)} cases, {ncell} cells, {a.threads} host threads")
-    rates = {}
-    for nt in (a.threads, 1):
-        # the impulsive start (uniform flow through the wall cells) at dt/10 first, as for
-        # the 3D workloads, then each case's own dt
-        ens = ensemble2d(cases, nthreads=nt, dt_scale=a.startup_factor, nswpp=a.nswpp,
-                         resimax=a.resimax, poisson_mode=3, check_every=a.check_every)
-        for q in range(a.startup_steps):
-            infos = ens.step()
-        log(f"start-up: {a.startup_steps} steps at dt/{1 / a.startup_factor:g}, max cfl "
-            f"{max(i['cfl'] for i in infos):.3f}")
-        for sv, c in zip(ens.solvers, cases):
-            sv.set_dtime(c["dt"])
-        for q in range(a.warmup):
-            infos = ens.step()
-            log(f"warmup step {q + 1}/{a.warmup} threads {nt}: max cfl "
-                f"{max(i['cfl'] for i in infos):.3f} max resip {max(i['resip'] for i in infos):.2e}")
-        ens.sync()
-        t0 = time.perf_counter()
-        for q in range(a.steps):
-            infos = ens.step()
-        ens.sync()
-        secs = time.perf_counter() - t0
-        assert all(np.isfinite(i["cd"]) for i in infos)
-        rates[nt] = (secs, infos)
-        log(f"threads {nt}: {secs / a.steps * 1e3:.2f} ms/step "
-            f"{ncell * a.steps / secs / 1e6:.1f} Mcell-steps/s")
-        del ens
-    secs, infos = rates[a.threads]
+    log(f"ensemble: {len(cases)} cases, {ncell} cells, dt {a.ens_dt}")
+    b = batch_from_cases(cases, mixed=True, resimax=a.resimax, npass=a.npass,
+                         dt_scale=a.startup_factor)
+    infos = b.step(a.startup_steps)
+    log(f"start-up: {a.startup_steps} steps at dt/{1 / a.startup_factor:g}, max cfl "
+        f"{max(i['cfl'] for i in infos):.3f}")
+    for q, c in enumerate(cases):
+        b.set_dtime(q, c["dt"])
+    tune = b.autotune(reps=3)
+    log(f"autotuner: tile height {tune['tile_j']}, FFT columns {tune['fft_cols']}; "
+        + ", ".join(f"{k} {v:.3f} ms" for k, v in sorted(tune["ms_per_step"].items())))
+    for q in range(a.warmup):
+        infos = b.step(1)
+        log(f"warmup step {q + 1}/{a.warmup}: max cfl {max(i['cfl'] for i in infos):.3f} "
+            f"max resip {max(i['resip'] for i in infos):.2e} max passes "
+            f"{max(i['niter'] for i in infos)}")
+    b.step(0)  # synchronizes the stream
+    t0 = time.perf_counter()
+    infos = b.step(a.steps)  # a.steps graph replays, one synchronization at the end
+    secs = time.perf_counter() - t0
+    hist = b.history(a.steps)
+    assert np.isfinite(hist).all(), "non-finite C_D / C_L in the timed steps"
+    log(f"batch: {secs / a.steps * 1e3:.3f} ms/step {ncell * a.steps / secs / 1e6:.1f} "
+        f"Mcell-steps/s; RESIP max {hist[:, :, 6].max():.2e}, CFL max {hist[:, :, 7].max():.3f}")
+    prof = b.profile(a.steps)
+    tot = sum(v[0] for v in prof.values())
+    for k, (ms, calls) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+        log(f"  {k:16s} {ms:9.3f} ms {calls:6d} launches {ms / calls * 1e3:9.1f} us/launch "
+            f"({100 * ms / tot:.1f}%)")
+    rows = []
+    for k, nb in KERNEL_BYTES_2D.items():
+        if k in prof:
+            ms, calls = prof[k]
+            avg = ms / calls * 1e-3
+            rows.append(dict(kernel=k, calls=calls, avg_us=avg * 1e6, total_ms=ms,
+                             bytes_per_launch=nb * ncell, achieved_gbs=nb * ncell / avg / 1e9,
+                             frac=nb * ncell / avg / 1e9 / HBM_PEAK_GBS))
+    rows.sort(key=lambda r: -r["total_ms"])
+    top = rows[0]
     out = {
-        "metric": "Mcell-steps/s, ensemble of 2D cylinder cases of uneven size and Re",
+        "metric": "Mcell-steps/s, batched ensemble of 2D cylinder cases of uneven size and Re",
         "value": ncell * a.steps / secs / 1e6, "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": secs / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
         "data": "synthetic",
         "config": {"workload": "ensemble2d", "cases": len(cases), "cells": ncell,
-                   "host_threads": a.threads, "resimax": a.resimax, "poisson": "mixed",
+                   "dt": a.ens_dt, "resimax": a.resimax, "poisson_passes": a.npass,
+                   "poisson": "mixed", "tuning": {"tile_j": tune["tile_j"],
+                                                  "fft_cols": tune["fft_cols"]},
                    "startup": f"{a.startup_steps} untimed steps at dt/{1 / a.startup_factor:g}"},
-        "one_thread_mcell_steps_per_s": ncell * a.steps / rates[1][0] / 1e6,
-        "resip_max": float(max(i["resip"] for i in infos)),
-        "roofline": None, "cpu_baseline": None,
+        "resip_max": float(hist[:, :, 6].max()),
+        "roofline": {"bound": "hbm", "achieved": top["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": top["frac"], "traffic": None,
+                     "kernel": top["kernel"], "bytes_per_launch": top["bytes_per_launch"],
+                     "avg_us": top["avg_us"], "launches": top["calls"]},
+        "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+                    for r in rows],
+        "profile_share": {k: round(v[0] / tot, 4) for k, v in prof.items()},
     }
+    out["cpu_baseline"] = None if a.no_cpu else cpu_baseline_ensemble(cases, log)
     print(json.dumps(out), flush=True)
 
 
@@ -252,7 +339,11 @@ def main():
     ap.add_argument("--thermal", action="store_true",
                     help="advance the passive temperature too (not part of the headline step)")
     ap.add_argument("--cases", type=int, default=96, help="ensemble2d: number of 2D cases")
-    ap.add_argument("--threads", type=int, default=8, help="ensemble2d: host threads")
+    ap.add_argument("--npass", type=int, default=3,
+                    help="ensemble2d: Poisson correction passes per step")
+    ap.add_argument("--ens-dt", type=float, default=0.001,
+                    help="ensemble2d: the time step of every case (the reference scheme diverges "
+                         "on some of the drawn cases at 0.0015 and above)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB four times (HBM counter calibration) and exit")
     a = ap.parse_args()

@@ -145,22 +145,48 @@ __device__ inline void bcu_ghosts_wave(double* F, int ni, int nj, int sj, const 
 // ---------------------------------------------------------------------------------------------
 // The AF line Gauss-Seidel sweep of one stage (iter_u.f90:50-66 I lines, 103-119 J lines), in
 // the reference's order: lines one after the other, each line solved with the already updated
-// previous line and the old next line.  One 64-lane wavefront per (case, component): BCU's
-// ghosts first, then per line the Thomas recursion as two affine-map scans over the lanes
-// (lane l holds rows l R .. l R + R - 1 and keeps the previous line's values in registers).
+// previous line and the old next line.  One 64-lane wavefront per (case, component), all cases
+// in one launch: BCU's ghosts first, then per line the Thomas recursion as two affine-map scans
+// over the lanes (lane l holds rows l R .. l R + R - 1, R = ceil(rows / 64) of the case, and
+// keeps the previous line's values in registers).  The operands of line L + 1 (right-hand side,
+// factors, lagged couplings with the old next-but-one line, ghosts) are loaded while line L is
+// solved: none of them is written by the sweep before line L + 1 reads it.
 // DIR 0: I lines (rows along I, lines J = 1..NJ-1, lagged couplings AN, AS = AN(J-1));
 // DIR 1: J lines (rows along J, lines I = 1..NI-1, lagged couplings AE, AW); the line ends take
 // the ghost values as Dirichlet data.
 // ---------------------------------------------------------------------------------------------
+template <int R>
+struct LineOps {
+  double cupf[R], cdn[R], q[R], iv[R], na[R], bf[R];  // cup x_old(next line) formed at load
+  double g0, g1;                                       // ghosts before / after the line
+};
+
 template <int R, int DIR>
-__global__ void __launch_bounds__(64) k2_sweep(const Case2D* __restrict__ tab,
-                                               const int* __restrict__ cases, int with_bcu) {
-  const Case2D& cs = tab[cases[blockIdx.x >> 1]];
-  const int comp = blockIdx.x & 1;
+__device__ inline void sweep_load(LineOps<R>& o, const double* F, const double* Q,
+                                  const double* invd, const double* na, const double* bf,
+                                  const double* cup, const double* cdn, long base, long rs,
+                                  long ls, long dnoff, int n, int r0, int l) {
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int r = r0 + q;
+    if (r < n) {
+      const long c = base + rs * (r + 1);
+      o.cupf[q] = cup[c] * F[c + ls];
+      o.cdn[q] = cdn[c + dnoff];
+      o.q[q] = Q[c];
+      o.iv[q] = invd[c];
+      o.na[q] = na[c];
+      o.bf[q] = bf[c];
+    }
+  }
+  o.g0 = l == 0 ? F[base] : 0.0;
+  o.g1 = l == 63 ? F[base + rs * (n + 1)] : 0.0;
+}
+
+template <int R, int DIR>
+__device__ void sweep_lines(const Case2D& cs, int comp) {
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
   double* F = cs.f[comp ? F_V : F_U];
-  if (with_bcu)
-    bcu_ghosts_wave(F, ni, nj, sj, cs.b[comp ? B_VWALL : B_UWALL], cs.b[comp ? B_VFAR : B_UFAR]);
   const double* Q = cs.f[comp ? F_SV1 : F_SU1];
   const double* invd = cs.f[DIR == 0 ? G_INVD1 : G_INVD2];
   const double* na = cs.f[DIR == 0 ? G_NA1 : G_NA2];
@@ -180,19 +206,20 @@ __global__ void __launch_bounds__(64) k2_sweep(const Case2D* __restrict__ tab,
     const int r = r0 + q;
     prev[q] = r < n ? F[rs * (r + 1)] : 0.0;  // line 0: the ghost line
   }
+  LineOps<R> cur, nxt;
+  sweep_load<R, DIR>(cur, F, Q, invd, na, bf, cup, cdn, ls, rs, ls, dnoff, n, r0, l);
   for (int L = 1; L <= nl; ++L) {
     const long base = ls * L;
+    if (L < nl) sweep_load<R, DIR>(nxt, F, Q, invd, na, bf, cup, cdn, base + ls, rs, ls, dnoff, n, r0, l);
     double al[R], be[R], bb[R];
     double At = 0.0, Bt = 1.0;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      const int r = r0 + q;
-      if (r < n) {
-        const long c = base + rs * (r + 1);
-        const double C = cup[c] * F[c + ls] + cdn[c + dnoff] * prev[q] + Q[c];
-        al[q] = C * invd[c];
-        be[q] = na[c];
-        bb[q] = bf[c];
+      if (r0 + q < n) {
+        const double C = cur.cupf[q] + cur.cdn[q] * prev[q] + cur.q[q];
+        al[q] = C * cur.iv[q];
+        be[q] = cur.na[q];
+        bb[q] = cur.bf[q];
       } else {
         al[q] = 0.0; be[q] = 1.0; bb[q] = 0.0;
       }
@@ -205,7 +232,7 @@ __global__ void __launch_bounds__(64) k2_sweep(const Case2D* __restrict__ tab,
       if (l >= o) { At = At + Bt * Ao; Bt = Bt * Bo; }
     }
     // lane l starts from the exclusive prefix applied to the ghost before the first row
-    const double g0 = F[base];
+    const double g0 = __shfl(cur.g0, 0, 64);
     const double Ae = __shfl_up(At, 1, 64), Be = __shfl_up(Bt, 1, 64);
     double yin = l == 0 ? g0 : Ae + Be * g0;
     double y[R];
@@ -228,7 +255,7 @@ __global__ void __launch_bounds__(64) k2_sweep(const Case2D* __restrict__ tab,
       if (l + o < 64) { Dt = Dt + Ct * Do; Ct = Ct * Co; }
     }
     // and from the exclusive suffix applied to the ghost after the last row
-    const double g1 = F[base + rs * (n + 1)];
+    const double g1 = __shfl(cur.g1, 63, 64);
     const double De = __shfl_down(Dt, 1, 64), Ce = __shfl_down(Ct, 1, 64);
     double xin = l == 63 ? g1 : De + Ce * g1;
 #pragma unroll
@@ -241,6 +268,27 @@ __global__ void __launch_bounds__(64) k2_sweep(const Case2D* __restrict__ tab,
         F[base + rs * (r + 1)] = x;
       }
     }
+    cur = nxt;
+  }
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(64) k2_sweep(const Case2D* __restrict__ tab, int with_bcu) {
+  const Case2D& cs = tab[blockIdx.x >> 1];
+  const int comp = blockIdx.x & 1;
+  if (with_bcu)
+    bcu_ghosts_wave(cs.f[comp ? F_V : F_U], cs.ni, cs.nj, cs.sj, cs.b[comp ? B_VWALL : B_UWALL],
+                    cs.b[comp ? B_VFAR : B_UFAR]);
+  const int n = DIR == 0 ? cs.ni - 1 : cs.nj - 1;
+  switch ((n + 63) >> 6) {  // the case's own rows per lane
+    case 1: sweep_lines<1, DIR>(cs, comp); break;
+    case 2: sweep_lines<2, DIR>(cs, comp); break;
+    case 3: sweep_lines<3, DIR>(cs, comp); break;
+    case 4: sweep_lines<4, DIR>(cs, comp); break;
+    case 5: sweep_lines<5, DIR>(cs, comp); break;
+    case 6: sweep_lines<6, DIR>(cs, comp); break;
+    case 7: sweep_lines<7, DIR>(cs, comp); break;
+    default: sweep_lines<8, DIR>(cs, comp); break;
   }
 }
 

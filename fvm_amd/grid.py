@@ -280,12 +280,13 @@ ENSEMBLE_RE = (40.0, 300.0)
 
 
 def ensemble_cases(n: int = 96, seed: int = SEED, h1: float = 0.002, r_out: float = 30.0,
-                   sigma: float = 1e-3, dt: float = 0.005):
+                   sigma: float = 1e-3, dt: float = 0.005, dt_rule: str = "courant"):
     """Seeded 2D cylinder cases of uneven size and Reynolds number: per case index c, one
     Philox stream (key seed, stream 1000 + c) draws NJ, NI from the size sets and Re uniform in
     [40, 300].  Stretching law and velocity noise as the single 2D case (wall spacing h1, outer
-    radius r_out, N(0, sigma) noise with the case's own stream); the time step keeps the 257-node
-    case's circumferential Courant number, dt (256 / (NJ - 1)) capped at dt."""
+    radius r_out, N(0, sigma) noise with the case's own stream).  dt_rule "courant": the time
+    step keeps the 257-node case's circumferential Courant number, dt (256 / (NJ - 1)) capped at
+    dt; "uniform": dt for every case."""
     out = []
     for c in range(n):
         gen = np.random.Generator(np.random.Philox(np.random.SeedSequence([seed, 1000 + c])))
@@ -293,8 +294,8 @@ def ensemble_cases(n: int = 96, seed: int = SEED, h1: float = 0.002, r_out: floa
         ni = int(ENSEMBLE_NI[gen.integers(len(ENSEMBLE_NI))])
         re = float(gen.uniform(*ENSEMBLE_RE))
         (xg, yg, _), ic = make_case(ni, nj, 1, h1, r_out, sigma, seed=seed + 1 + c)
-        out.append(dict(ni=ni, nj=nj, re=re, dt=dt * min(1.0, 256.0 / (nj - 1)), xg=xg, yg=yg,
-                        ic=ic))
+        cdt = dt if dt_rule == "uniform" else dt * min(1.0, 256.0 / (nj - 1))
+        out.append(dict(ni=ni, nj=nj, re=re, dt=cdt, xg=xg, yg=yg, ic=ic, draw=c))
     return out
 
 
