@@ -16,6 +16,7 @@
 #include "../../include/fvm_capi.h"
 #include "fvm2d_host.h"
 #include "fvm2d_kernels.h"
+#include "fvm_wave.h"
 
 namespace {
 
@@ -226,15 +227,8 @@ __device__ void sweep_lines(const Case2D& cs, int comp) {
       At = al[q] + be[q] * At;
       Bt = be[q] * Bt;
     }
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const double Ao = __shfl_up(At, o, 64), Bo = __shfl_up(Bt, o, 64);
-      if (l >= o) { At = At + Bt * Ao; Bt = Bt * Bo; }
-    }
     // lane l starts from the exclusive prefix applied to the ghost before the first row
-    const double g0 = __shfl(cur.g0, 0, 64);
-    const double Ae = __shfl_up(At, 1, 64), Be = __shfl_up(Bt, 1, 64);
-    double yin = l == 0 ? g0 : Ae + Be * g0;
+    double yin = wave::prefix_apply(At, Bt, wave::lane_value(cur.g0, 0));
     double y[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -249,15 +243,8 @@ __device__ void sweep_lines(const Case2D& cs, int comp) {
         Ct = -bb[q] * Ct;
       }
     }
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const double Do = __shfl_down(Dt, o, 64), Co = __shfl_down(Ct, o, 64);
-      if (l + o < 64) { Dt = Dt + Ct * Do; Ct = Ct * Co; }
-    }
     // and from the exclusive suffix applied to the ghost after the last row
-    const double g1 = __shfl(cur.g1, 63, 64);
-    const double De = __shfl_down(Dt, 1, 64), Ce = __shfl_down(Ct, 1, 64);
-    double xin = l == 63 ? g1 : De + Ce * g1;
+    double xin = wave::suffix_apply(Dt, Ct, wave::lane_value(cur.g1, 63));
 #pragma unroll
     for (int q = R - 1; q >= 0; --q) {
       const int r = r0 + q;
@@ -558,14 +545,13 @@ __global__ void __launch_bounds__(64) k2_modes(const Case2D* __restrict__ tab,
     Ay = aly[q] + be[q] * Ay;
     Bt = be[q] * Bt;
   }
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double Aox = __shfl_up(Ax, o, 64), Aoy = __shfl_up(Ay, o, 64),
-                 Bo = __shfl_up(Bt, o, 64);
-    if (l >= o) { Ax = Ax + Bt * Aox; Ay = Ay + Bt * Aoy; Bt = Bt * Bo; }
+  double yx_in, yy_in;
+  {
+    double A2[2] = {Ax, Ay}, z2[2] = {0.0, 0.0}, o2[2];
+    wave::prefix_apply<2>(A2, Bt, z2, o2);  // y(-1) = 0
+    yx_in = o2[0];
+    yy_in = o2[1];
   }
-  double yx_in = __shfl_up(Ax, 1, 64), yy_in = __shfl_up(Ay, 1, 64);
-  if (l == 0) { yx_in = 0.0; yy_in = 0.0; }
   double yx[R], yy[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
@@ -583,14 +569,13 @@ __global__ void __launch_bounds__(64) k2_modes(const Case2D* __restrict__ tab,
       Ct = -bb[q] * Ct;
     }
   }
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double Dox = __shfl_down(Dx, o, 64), Doy = __shfl_down(Dy, o, 64),
-                 Co = __shfl_down(Ct, o, 64);
-    if (l + o < 64) { Dx = Dx + Ct * Dox; Dy = Dy + Ct * Doy; Ct = Ct * Co; }
+  double xx_in, xy_in;
+  {
+    double D2[2] = {Dx, Dy}, z2[2] = {0.0, 0.0}, o2[2];
+    wave::suffix_apply<2>(D2, Ct, z2, o2);  // x(n) = 0
+    xx_in = o2[0];
+    xy_in = o2[1];
   }
-  double xx_in = __shfl_down(Dx, 1, 64), xy_in = __shfl_down(Dy, 1, 64);
-  if (l == 63) { xx_in = 0.0; xy_in = 0.0; }
 #pragma unroll
   for (int q = R - 1; q >= 0; --q) {
     const int r = r0 + q;

@@ -497,15 +497,17 @@ __global__ void k_copy_ghosts(Dims d, const double* __restrict__ src, double* ds
 }
 
 // The same sweep with the forward results held in registers (lines of at most NMAX rows): one
-// read of SU and the factors, one write of the result.
+// read of SU and the factors, one write of the result.  The components of a line sit in the
+// wavefronts of one workgroup (threadIdx.y), which load the shared factors together: one HBM
+// read of them serves all three.
 template <int NMAX>
-__global__ void __launch_bounds__(256) k_af_thomas_k_reg(Dims d, AFArgs A,
+__global__ void __launch_bounds__(192) k_af_thomas_k_reg(Dims d, AFArgs A,
                                                          const double* __restrict__ ta,
                                                          const double* __restrict__ td,
                                                          const double* __restrict__ tb) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int comp = blockIdx.z;
+  const int j = 1 + blockIdx.y;
+  const int comp = threadIdx.y;
   if (i > d.ni - 1 || j > d.nj - 1) return;
   const GPW f = gpw(A.f[comp]);
   const GP su = gp(A.su[comp]);
@@ -670,13 +672,13 @@ __global__ void k_af_factor_k(Dims d, const double* const* __restrict__ fa,
 // along I (coalesced), the reference's forward elimination and back substitution in its
 // operation order, in place.  With the I and J couplings identically zero the lagged-neighbour
 // sum C of the reference is exactly SU (the BCU additions to SU multiply the same zeros).
-__global__ void __launch_bounds__(256) k_af_thomas_k(Dims d, AFArgs A,
+__global__ void __launch_bounds__(192) k_af_thomas_k(Dims d, AFArgs A,
                                                      const double* __restrict__ ta,
                                                      const double* __restrict__ td,
                                                      const double* __restrict__ tb) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int comp = blockIdx.z;
+  const int j = 1 + blockIdx.y;
+  const int comp = threadIdx.y;
   if (i > d.ni - 1 || j > d.nj - 1) return;
   double* f = A.f[comp];
   const double* su = A.su[comp];
@@ -802,12 +804,12 @@ int fvm_k_iter_u(FvmCtx* c) {
     if (s == 2 && c->prm.af_mode == 0 && c->tk_a && c->af_zero_a[2] && c->af_zero_b[2]) {
       // independent K-lines: the exact sweep in one pass
       ProfScope pk(c, "k_af_thomas_k");
-      const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, A.nc);
+      const dim3 g((d.ni - 1 + 63) / 64, d.nj - 1, 1);
       if (d.nk - 1 <= 64)
-        hipLaunchKernelGGL(k_af_thomas_k_reg<64>, g, dim3(64, 4, 1), 0, st, d, A, c->tk_a,
+        hipLaunchKernelGGL(k_af_thomas_k_reg<64>, g, dim3(64, A.nc, 1), 0, st, d, A, c->tk_a,
                            c->tk_d, c->tk_b);
       else
-        hipLaunchKernelGGL(k_af_thomas_k, g, dim3(64, 4, 1), 0, st, d, A, c->tk_a, c->tk_d,
+        hipLaunchKernelGGL(k_af_thomas_k, g, dim3(64, A.nc, 1), 0, st, d, A, c->tk_a, c->tk_d,
                            c->tk_b);
       HIP_CHECK(hipGetLastError());
       c->af_iters[s] = 1;
@@ -896,12 +898,12 @@ int fvm_k_iter_u_one(FvmCtx* c, int comp, double* f, const double* fn, const dou
     if (r) return r;
     ProfScope ps(c, s == 0 ? "af_sweep_i" : (s == 1 ? "af_sweep_j" : "af_sweep_k"));
     if (exact_k) {
-      const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, 1);
+      const dim3 g((d.ni - 1 + 63) / 64, d.nj - 1, 1);
       if (d.nk - 1 <= 64)
-        hipLaunchKernelGGL(k_af_thomas_k_reg<64>, g, dim3(64, 4, 1), 0, st, d, A, c->tk_a,
+        hipLaunchKernelGGL(k_af_thomas_k_reg<64>, g, dim3(64, 1, 1), 0, st, d, A, c->tk_a,
                            c->tk_d, c->tk_b);
       else
-        hipLaunchKernelGGL(k_af_thomas_k, g, dim3(64, 4, 1), 0, st, d, A, c->tk_a, c->tk_d,
+        hipLaunchKernelGGL(k_af_thomas_k, g, dim3(64, 1, 1), 0, st, d, A, c->tk_a, c->tk_d,
                            c->tk_b);
       HIP_CHECK(hipGetLastError());
     } else if (scan) {
@@ -1035,12 +1037,12 @@ int fvm_k_iter_t(FvmCtx* c) {
       r = fvm_af_scan_stage(c, s, A, &it, true);
       if (r) return r;
     } else {
-      const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, 1);
+      const dim3 g((d.ni - 1 + 63) / 64, d.nj - 1, 1);
       if (d.nk - 1 <= 64)
-        hipLaunchKernelGGL(k_af_thomas_k_reg<64>, g, dim3(64, 4, 1), 0, st, d, A, c->tk_at,
+        hipLaunchKernelGGL(k_af_thomas_k_reg<64>, g, dim3(64, 1, 1), 0, st, d, A, c->tk_at,
                            c->tk_dt, c->tk_bt);
       else
-        hipLaunchKernelGGL(k_af_thomas_k, g, dim3(64, 4, 1), 0, st, d, A, c->tk_at, c->tk_dt,
+        hipLaunchKernelGGL(k_af_thomas_k, g, dim3(64, 1, 1), 0, st, d, A, c->tk_at, c->tk_dt,
                            c->tk_bt);
       HIP_CHECK(hipGetLastError());
     }

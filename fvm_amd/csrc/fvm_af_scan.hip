@@ -16,7 +16,7 @@
 // * A line solve is the forward recurrence y(p) = alpha(p) + beta(p) y(p-1),
 //   alpha = (Q + cam x_{a-1}) / D, and the back substitution x(p) = y(p) - B(p) x(p+1); both are
 //   first-order affine recurrences, evaluated by one 64-lane wavefront as an affine-map scan
-//   (each lane composes its block of rows, then a log2(64)-step shuffle scan, no LDS barriers).
+//   (each lane composes its block of rows, then a six-step scan of DPP lane moves, no LDS).
 //   Q = SU' + (coupling to line a+1) x_in(a+1) is formed once per stage.
 // * A wavefront owns SC_C consecutive lines of one plane and sweeps them in order; chunks are
 //   coupled through the last line of the preceding chunk from the previous pass (block Jacobi
@@ -27,6 +27,7 @@
 //   J-lines the line factors, Q and the iterate live in a line-major copy (J fastest), and the
 //   converged iterate is transposed back into the field.
 #include "fvm_ctx.h"
+#include "fvm_wave.h"
 
 #include <string.h>
 
@@ -254,17 +255,7 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
       }
       load(min(a + 1, a1));  // next line's operands in flight during the scans
       __builtin_amdgcn_sched_barrier(0);  // keep those loads ahead of the scans
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const double Ao = shfl_up_d(At, o), Bo = shfl_up_d(Bt, o);
-        const bool up = l >= o;
-        At = up ? At + Bt * Ao : At;
-        Bt = up ? Bt * Bo : Bt;
-      }
-      double Ae = shfl_up_d(At, 1), Be = shfl_up_d(Bt, 1);
-      Ae = l == 0 ? 0.0 : Ae;
-      Be = l == 0 ? 1.0 : Be;
-      double yin = Ae + Be * y0;
+      double yin = wave::prefix_apply(At, Bt, y0);  // y0, xn: the same on every lane
       double y[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -279,17 +270,7 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
         Dt = in ? y[r] - bw[r] * Dt : Dt;
         Ct = in ? -bw[r] * Ct : Ct;
       }
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const double Do = shfl_down_d(Dt, o), Co = shfl_down_d(Ct, o);
-        const bool dn = l + o < 64;
-        Dt = dn ? Dt + Ct * Do : Dt;
-        Ct = dn ? Ct * Co : Ct;
-      }
-      double De = shfl_down_d(Dt, 1), Ce = shfl_down_d(Ct, 1);
-      De = l == 63 ? 0.0 : De;
-      Ce = l == 63 ? 1.0 : Ce;
-      double xin = De + Ce * xn;
+      double xin = wave::suffix_apply(Dt, Ct, xn);
       double dl = 0.0;
 #pragma unroll
       for (int r = R - 1; r >= 0; --r) {

@@ -21,6 +21,7 @@
 
 #include "fvm_ctx.h"
 #include "fvm_fft.h"
+#include "fvm_wave.h"
 
 namespace {
 
@@ -157,7 +158,7 @@ __global__ void k_mode_factor(int n, int mj_n, int nj1, int nk1, const double* _
 // Radial mode-line solves of the complex spectrum with the stored factors, in FP64 whatever the
 // storage type T: one 64-lane wavefront per line, lane l owning rows l*R .. l*R+R-1; forward
 // elimination y(r) = z(r)/D(r) - lo(r)/D(r) y(r-1) and back substitution
-// x(r) = y(r) - B(r) x(r+1) as affine-map scans across the lanes (shuffles, no LDS); the real
+// x(r) = y(r) - B(r) x(r+1) as affine-map scans across the lanes (DPP lane moves, no LDS); the real
 // and imaginary parts share the real line operator.
 template <typename T, int R>
 __global__ void __launch_bounds__(64) k_mode_scan_c(int n, const double* __restrict__ dW,
@@ -185,14 +186,13 @@ __global__ void __launch_bounds__(64) k_mode_scan_c(int n, const double* __restr
       alx[q] = 0.0; aly[q] = 0.0; be[q] = 0.0; bb[q] = 0.0;
     }
   }
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double Aox = __shfl_up(Ax, o, 64), Aoy = __shfl_up(Ay, o, 64),
-                 Bo = __shfl_up(Bt, o, 64);
-    if (l >= o) { Ax = Ax + Bt * Aox; Ay = Ay + Bt * Aoy; Bt = Bt * Bo; }
+  double yinx, yiny;
+  {
+    double A2[2] = {Ax, Ay}, z2[2] = {0.0, 0.0}, o2[2];
+    wave::prefix_apply<2>(A2, Bt, z2, o2);  // y(-1) = 0
+    yinx = o2[0];
+    yiny = o2[1];
   }
-  double yinx = __shfl_up(Ax, 1, 64), yiny = __shfl_up(Ay, 1, 64);  // y(-1) = 0
-  if (l == 0) { yinx = 0.0; yiny = 0.0; }
   double yx[R], yy[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
@@ -209,14 +209,13 @@ __global__ void __launch_bounds__(64) k_mode_scan_c(int n, const double* __restr
       Ct = -bb[q] * Ct;
     }
   }
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double Dox = __shfl_down(Dx, o, 64), Doy = __shfl_down(Dy, o, 64),
-                 Co = __shfl_down(Ct, o, 64);
-    if (l + o < 64) { Dx = Dx + Ct * Dox; Dy = Dy + Ct * Doy; Ct = Ct * Co; }
+  double xinx, xiny;
+  {
+    double D2[2] = {Dx, Dy}, z2[2] = {0.0, 0.0}, o2[2];
+    wave::suffix_apply<2>(D2, Ct, z2, o2);  // x(n) = 0
+    xinx = o2[0];
+    xiny = o2[1];
   }
-  double xinx = __shfl_down(Dx, 1, 64), xiny = __shfl_down(Dy, 1, 64);  // x(n) = 0
-  if (l == 63) { xinx = 0.0; xiny = 0.0; }
 #pragma unroll
   for (int q = R - 1; q >= 0; --q) {
     const int r = r0 + q;
