@@ -406,6 +406,23 @@ int fvm_destroy(fvm_ctx* c) {
   return FVM_OK;
 }
 
+// Fields that carry the cell volume J^-1 (fluxes summed over the faces, NL, DIF, S_U, the
+// volume itself): on the 2D slab they are LZ2D times the 2D solver's per-unit-span values
+// (cylinder/.../init.f90:46-56 reads and writedata.f90 writes NLX, NLY, ADV per unit span), so
+// a 2D context exchanges them with the host per unit span.
+static bool span_scaled(int f) {
+  switch (f) {
+    case FVM_NLX: case FVM_NLY: case FVM_NLZ:
+    case FVM_NLX_NM1: case FVM_NLY_NM1: case FVM_NLZ_NM1:
+    case FVM_DIFX: case FVM_DIFY: case FVM_DIFZ:
+    case FVM_SU: case FVM_SV: case FVM_SW:
+    case FVM_DJAC: case FVM_ADV: case FVM_ADV_NM1: case FVM_DIFT:
+      return true;
+    default:
+      return false;
+  }
+}
+
 int fvm_set_field(fvm_ctx* c, int field, const double* host) {
   if (!c || !host) return FVM_ERR_ARG;
   c->u_is_un = false;
@@ -420,8 +437,9 @@ int fvm_set_field(fvm_ctx* c, int field, const double* host) {
   std::vector<double> slab;
   if (c->two_d) {  // one (NI+1)(NJ+1) plane, repeated on the slab's three K planes
     slab.resize(c->d.n);
+    const double sc = span_scaled(field) ? LZ2D : 1.0;
     for (int k = 0; k < 3; ++k)
-      memcpy(slab.data() + k * c->d.sk, host, c->d.sk * sizeof(double));
+      for (long q = 0; q < c->d.sk; ++q) slab[k * c->d.sk + q] = host[q] * sc;
     host = slab.data();
   }
   HIP_CHECK(hipMemcpyAsync(dptr, host, c->d.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -451,6 +469,8 @@ int fvm_get_field(fvm_ctx* c, int field, double* host) {
   const long cnt = c->two_d ? c->d.sk : c->d.n;
   HIP_CHECK(hipMemcpyAsync(host, dptr, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (c->two_d && span_scaled(field))
+    for (long q = 0; q < cnt; ++q) host[q] /= LZ2D;
   return FVM_OK;
 }
 

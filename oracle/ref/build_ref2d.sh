@@ -8,12 +8,13 @@
 # meaning: INT4(x) -> INT(x) (grid.f90, writedata.f90), IF(FLAG) -> IF(FLAG.NE.0)
 # (contra_vel.f90), IF(IPOST) -> IF(IPOST.NE.0) (calccoef.f90).  No other source line changes.
 #
-# usage: build_ref2d.sh LABEL NI NJ NSWPP RESIMAX
+# usage: build_ref2d.sh LABEL NI NJ NSWPP RESIMAX [HARNESS]   (HARNESS: harness2d (default) or
+#        harness2d_post, the trajectory harness with the post-processing phase)
 set -euo pipefail
 REF_SRC="/root/reference/cylinder/cylinder_v3.0.3_AF_original 4step_face_nl"
 HERE="$(cd "$(dirname "$0")" && pwd)"
 REPO="$(cd "$HERE/../.." && pwd)"
-LABEL=$1; NI=$2; NJ=$3; NSWPP=$4; RESIMAX=$5
+LABEL=$1; NI=$2; NJ=$3; NSWPP=$4; RESIMAX=$5; HARNESS=${6:-harness2d}
 FLANG=${FLANG:-/opt/rocm/lib/llvm/bin/flang}
 
 if [ ! -d "$REF_SRC" ]; then
@@ -35,13 +36,13 @@ sed -i -e 's/INT4(/INT(/g' "$SCRATCH/grid.f90" "$SCRATCH/writedata.f90"
 sed -i -e 's/IF(FLAG) THEN/IF(FLAG.NE.0) THEN/' "$SCRATCH/contra_vel.f90"
 sed -i -e 's/IF(IPOST) THEN/IF(IPOST.NE.0) THEN/' "$SCRATCH/calccoef.f90"
 grep -E "NI|NJ|NSWPP|RESIMAX" "$P" | grep -v '^[[:space:]]*!' >&2
-cp "$HERE/harness2d.f90" "$SCRATCH/"
+cp "$HERE/$HARNESS.f90" "$SCRATCH/"
 cd "$SCRATCH"
 FFLAGS="-O2 -ffp-contract=off"
 $FLANG $FFLAGS -c comdat_param.f90 comdat_shared.f90
-SRCS=$(ls *.f90 | grep -v -e '^comdat_param.f90$' -e '^comdat_shared.f90$' -e '^harness2d.f90$')
-$FLANG $FFLAGS -c $SRCS harness2d.f90 2>"$SCRATCH/warn.txt" || { cat "$SCRATCH/warn.txt" >&2; exit 1; }
+SRCS=$(ls *.f90 | grep -v -e '^comdat_param.f90$' -e '^comdat_shared.f90$' -e "^$HARNESS.f90\$")
+$FLANG $FFLAGS -c $SRCS $HARNESS.f90 2>"$SCRATCH/warn.txt" || { cat "$SCRATCH/warn.txt" >&2; exit 1; }
 OUT="$REPO/oracle/_ref/$LABEL"
 mkdir -p "$OUT"
-$FLANG $FFLAGS -o "$OUT/harness2d" *.o
-echo "built $OUT/harness2d" >&2
+$FLANG $FFLAGS -o "$OUT/$HARNESS" *.o
+echo "built $OUT/$HARNESS" >&2
