@@ -168,6 +168,64 @@ def cpu_baseline(w, log):
                        f"restatement on 1 host core, {t1 - t0:.2f} s")
 
 
+REF_MPI = os.path.join(HERE, "oracle", "_ref", "rles_c1_mpi16", "rles")
+MPIRUN = "/opt/conda/bin/mpirun"
+
+
+def cpu_baseline_reference(w, log, nranks=16, nsteps=4, timeout_s=420):
+    """The reference's own program on the host cores: LES/v2.3_LES-Reynolds stresses built by
+    oracle/ref/build_ref_mpi.sh (its main.f90 and MPI (J, K) block decomposition unchanged,
+    4 x 4 ranks) for the 192 x 256 x 64 grid, cold start at dt 2e-4, the reference's own Poisson
+    settings (NSWPP = 20 line-GS sweeps per step, RESIMAX 1e-4), run with mpirun -np 16.  The
+    step time is the reference's own per-step report (CPU_TIME of rank 0, main.f90:155-158),
+    median over the steps after the first; the run is stopped after nsteps steps (before the
+    reference's end-of-run output).  None when the build or mpirun is absent or the run fails."""
+    import re
+    import signal
+    import subprocess
+    import tempfile
+    if not (os.path.exists(REF_MPI) and os.path.exists(MPIRUN)):
+        return None
+    from fvm_amd.grid import OGrid, grid_filename, write_grid3d
+    g = OGrid(w["ni"], w["nj"], w["nk"], w["h1"], w["r_out"], math.pi)
+    secs = []
+    with tempfile.TemporaryDirectory() as d:
+        write_grid3d(os.path.join(d, grid_filename(w["ni"], w["nj"], w["nk"])), *g.nodes())
+        t0 = time.perf_counter()
+        proc = subprocess.Popen([MPIRUN, "-np", str(nranks), REF_MPI], cwd=d,
+                                stdin=subprocess.DEVNULL, stdout=subprocess.PIPE,
+                                stderr=subprocess.STDOUT, text=True, start_new_session=True)
+        try:
+            for line in proc.stdout:
+                m = re.search(r"Time of operation =\s*([0-9.]+)", line)
+                if m:
+                    secs.append(float(m.group(1)))
+                    log(f"cpu reference step {len(secs)}: {secs[-1]:.3f} s")
+                if len(secs) >= nsteps or time.perf_counter() - t0 > timeout_s:
+                    break
+        finally:
+            if proc.poll() is None:
+                os.killpg(proc.pid, signal.SIGTERM)  # the process group started above
+                try:
+                    proc.wait(timeout=30)
+                except subprocess.TimeoutExpired:
+                    os.killpg(proc.pid, signal.SIGKILL)
+                    proc.wait()
+        wall = time.perf_counter() - t0
+    if len(secs) < 2:
+        log(f"cpu reference: {len(secs)} steps reported")
+        return None
+    step = float(np.median(secs[1:]))
+    log(f"cpu reference: {len(secs)} steps on {nranks} ranks, wall {wall:.1f} s")
+    return dict(value=cells_of(w) / step / 1e6, unit="Mcell-steps/s", cores=nranks,
+                kind="reference",
+                sample=f"{len(secs)} steps of the reference program (flang build, mpirun -np "
+                       f"{nranks}, NDIMJ x NDIMK = 4 x 4) on the 192x256x64 grid from its cold "
+                       f"start at dt 2e-4 with its own Poisson settings (20 line-GS sweeps per "
+                       f"step, RESIMAX 1e-4: not the 1e-8 of the GPU figure); median of the "
+                       f"reference's per-step times after the first, {step:.3f} s")
+
+
 def copy_calibration(mib, log):
     """A plain device copy of a known size (torch's copy kernel, four times): under rocprofv3
     --pmc FETCH_SIZE (or WRITE_SIZE) each copy dispatch reads (writes) exactly mib MiB, which
@@ -407,7 +465,10 @@ def main():
         out["mixed_over_fp64"] = results["fp64"]["secs"] / results["mixed"]["secs"]
     out["kernels"] = [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in row.items()}
                       for row in rows]
-    out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(w, log)
+    if a.no_cpu:
+        out["cpu_baseline"] = None
+    else:
+        out["cpu_baseline"] = cpu_baseline_reference(w, log) or cpu_baseline(w, log)
     print(json.dumps(out), flush=True)
 
 
