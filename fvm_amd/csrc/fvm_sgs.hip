@@ -185,12 +185,17 @@ __global__ void __launch_bounds__(256) k_strain_filter(Dims d, const double* con
 // products once per plane and combines three consecutive planes with (1,2,1) weights.  The sum
 // is the reference's 27-point filter with its terms grouped by plane (rounding-level
 // differences only).
+// Latency: the loads of plane k+1's tile and of the finishing cell's own operands (metrics,
+// test-filtered velocity stencil, Delta^2) are issued before plane k's in-plane sums are formed,
+// so they arrive while the LDS work runs; the three-plane combination is carried as two running
+// sums, X = P(k-1) + 2 P(k) for the cell finished next and Y = P(k) for the one after.
 constexpr int LM_KC = 16;
 constexpr int SD_KC = 16;  // planes per workgroup of the K-marching SGS divergences
 constexpr int LM_W = 66, LM_H = 6;  // tile with halo
+constexpr int LM_NQ = (LM_W * LM_H + 255) / 256;  // tile cells staged per thread
 
 template <bool XZ0>
-__global__ void __launch_bounds__(256, 3) k_lm(Dims d, const double* const* __restrict__ m,
+__global__ void __launch_bounds__(256, 2) k_lm(Dims d, const double* const* __restrict__ m,
                                             const double* __restrict__ U,
                                             const double* __restrict__ V,
                                             const double* __restrict__ W,
@@ -199,40 +204,76 @@ __global__ void __launch_bounds__(256, 3) k_lm(Dims d, const double* const* __re
                                             const double* __restrict__ WH,
                                             const double* const* __restrict__ L,
                                             const double* __restrict__ filt2, double* part,
-                                            double* magh_out) {
+                                            double* magh_out, int kc) {
   const uint3 bb = xcd_block();
   __shared__ double sh[10][LM_W * LM_H];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
   const int i = 1 + bb.x * 64 + tx;
   const int j = 1 + bb.y * 4 + ty;
-  const int k0 = 1 + bb.z * LM_KC;
-  const int k1 = min(k0 + LM_KC - 1, d.nk - 1);
+  const int k0 = 1 + bb.z * kc;
+  const int k1 = min(k0 + kc - 1, d.nk - 1);
   const bool act = i <= d.ni - 1 && j <= d.nj - 1;
   const GP L0 = gp(L[0]), L1 = gp(L[1]), L2 = gp(L[2]);
   const GP L3 = gp(L[3]), L4 = gp(L[4]), L5 = gp(L[5]);
-  // rolling in-plane sums of the products for planes k-1, k, k+1; product order:
-  // uu vv ww uv uw vw, |S|S11 |S|S22 |S|S33 |S|S12 |S|S13 |S|S23
-  double Pa[12], Pb[12], Pc[12];
+  const long sj = d.sj, sk = d.sk;
+  // staged tile values of the next plane: u v w S11 S12 S13 S22 S23 S33 per tile cell
+  double tv[LM_NQ][9];
+  auto load_tile = [&](int kp) {
 #pragma unroll
-  for (int q = 0; q < 12; ++q) Pa[q] = Pb[q] = Pc[q] = 0.0;
-  // this thread's sums of LM, MM and the cell count over the cells with 0.5 LM/MM >= 0
-  double sa = 0.0, sb = 0.0, sn = 0.0;
-  for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
-    __syncthreads();
-    for (int q = tid; q < LM_W * LM_H; q += 256) {
+    for (int h = 0; h < LM_NQ; ++h) {
+      const int q = min(tid + 256 * h, LM_W * LM_H - 1);
       const int gi = min(bb.x * 64 + q % LM_W, d.ni);
       const int gj = min(bb.y * 4 + q / LM_W, d.nj);
       const long c = cidx(d, gi, gj, kp);
-      const double l11 = L0[c], l12 = L1[c], l13 = L2[c], l22 = L3[c], l23 = L4[c], l33 = L5[c];
-      sh[0][q] = U[c]; sh[1][q] = V[c]; sh[2][q] = W[c];
-      sh[4][q] = l11; sh[5][q] = l12; sh[6][q] = l13;
-      sh[7][q] = l22; sh[8][q] = l23; sh[9][q] = l33;
-      // |S| from the stored strain rates in strain_store's order: the value it wrote to |S|
-      const double s2 = l11 * l11 + l12 * l12 + l13 * l13 + l12 * l12 + l22 * l22 + l23 * l23 +
-                        l13 * l13 + l23 * l23 + l33 * l33;
-      sh[3][q] = sqrt(2.0 * s2);
+      tv[h][0] = U[c]; tv[h][1] = V[c]; tv[h][2] = W[c];
+      tv[h][3] = L0[c]; tv[h][4] = L1[c]; tv[h][5] = L2[c];
+      tv[h][6] = L3[c]; tv[h][7] = L4[c]; tv[h][8] = L5[c];
+    }
+  };
+  // running plane sums in product order uu vv ww uv uw vw, |S|S11 |S|S22 |S|S33 |S|S12 |S|S13
+  // |S|S23
+  double X[12], Y[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) X[q] = Y[q] = 0.0;
+  // this thread's sums of LM, MM and the cell count over the cells with 0.5 LM/MM >= 0
+  double sa = 0.0, sb = 0.0, sn = 0.0;
+  load_tile(k0 - 1);
+  for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < LM_NQ; ++h) {
+      const int q = tid + 256 * h;
+      if (q < LM_W * LM_H) {
+        const double l11 = tv[h][3], l12 = tv[h][4], l13 = tv[h][5], l22 = tv[h][6],
+                     l23 = tv[h][7], l33 = tv[h][8];
+        sh[0][q] = tv[h][0]; sh[1][q] = tv[h][1]; sh[2][q] = tv[h][2];
+        sh[4][q] = l11; sh[5][q] = l12; sh[6][q] = l13;
+        sh[7][q] = l22; sh[8][q] = l23; sh[9][q] = l33;
+        // |S| from the stored strain rates in strain_store's order: the value it wrote to |S|
+        const double s2 = l11 * l11 + l12 * l12 + l13 * l13 + l12 * l12 + l22 * l22 +
+                          l23 * l23 + l13 * l13 + l23 * l23 + l33 * l33;
+        sh[3][q] = sqrt(2.0 * s2);
+      }
     }
     __syncthreads();
+    if (kp + 1 <= k1 + 1) load_tile(kp + 1);  // in flight during this plane's sums
+    // operands of the cell finished in this iteration, (i, j, kp - 1)
+    const bool fin = kp >= k0 + 1 && act;
+    const long c = fin ? cidx(d, i, j, kp - 1) : cidx(d, 1, 1, 1);
+    const double xix = gp(m[0])[c], etx = gp(m[1])[c], ztx = gp(m[2])[c];
+    const double xiy = gp(m[3])[c], ety = gp(m[4])[c], zty = gp(m[5])[c];
+    const double xiz = mget<XZ0>(m, 6, c), etz = mget<XZ0>(m, 7, c), ztz = gp(m[8])[c];
+    double hs[3][6];  // u^, v^, w^ at i+1, i-1, j+1, j-1, k+1, k-1
+    const double* hf[3] = {UH, VH, WH};
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const GP h = gp(hf[f]);
+      hs[f][0] = h[c + 1]; hs[f][1] = h[c - 1];
+      hs[f][2] = h[c + sj]; hs[f][3] = h[c - sj];
+      hs[f][4] = h[c + sk]; hs[f][5] = h[c - sk];
+    }
+    const double uh = gp(UH)[c], vh = gp(VH)[c], wh = gp(WH)[c];
+    const double fw2 = gp(filt2)[c];
     double Pn[12];
 #pragma unroll
     for (int q = 0; q < 12; ++q) Pn[q] = 0.0;
@@ -256,43 +297,30 @@ __global__ void __launch_bounds__(256, 3) k_lm(Dims d, const double* const* __re
         Pn[10] += wgt * (mg * sh[6][q]);
         Pn[11] += wgt * (mg * sh[8][q]);
       }
+    double F[12];
 #pragma unroll
-    for (int q = 0; q < 12; ++q) { Pa[q] = Pb[q]; Pb[q] = Pc[q]; Pc[q] = Pn[q]; }
-    if (kp < k0 + 1 || !act) continue;
-    // finish cell (i, j, kp - 1)
-    const int k = kp - 1;
-    const long c = cidx(d, i, j, k);
-    const long sj = d.sj, sk = d.sk;
-    const double xix = gp(m[0])[c], etx = gp(m[1])[c], ztx = gp(m[2])[c];
-    const double xiy = gp(m[3])[c], ety = gp(m[4])[c], zty = gp(m[5])[c];
-    const double xiz = mget<XZ0>(m, 6, c), etz = mget<XZ0>(m, 7, c), ztz = gp(m[8])[c];
-#define DI(F) (F[c + 1] - F[c - 1])
-#define DJ(F) (F[c + sj] - F[c - sj])
-#define DK(F) (F[c + sk] - F[c - sk])
+    for (int q = 0; q < 12; ++q) {
+      F[q] = (X[q] + Pn[q]) / 64.0;  // (P(k-1) + 2 P(k)) + P(k+1)
+      X[q] = Y[q] + 2.0 * Pn[q];
+      Y[q] = Pn[q];
+    }
+    if (!fin) continue;
     // raw (unscaled) test-filtered strain sums; S11 = 0.5 r11, S12 = 0.25 r12, ...
-    const double r11 = xix * DI(UH) + etx * DJ(UH) + ztx * DK(UH);
-    const double r12 = xiy * DI(UH) + ety * DJ(UH) + zty * DK(UH) + xix * DI(VH) +
-                       etx * DJ(VH) + ztx * DK(VH);
-    const double r13 = xiz * DI(UH) + etz * DJ(UH) + ztz * DK(UH) + xix * DI(WH) +
-                       etx * DJ(WH) + ztx * DK(WH);
-    const double r22 = xiy * DI(VH) + ety * DJ(VH) + zty * DK(VH);
-    const double r23 = xiz * DI(VH) + etz * DJ(VH) + ztz * DK(VH) + xiy * DI(WH) +
-                       ety * DJ(WH) + zty * DK(WH);
-    const double r33 = xiz * DI(WH) + etz * DJ(WH) + ztz * DK(WH);
-#undef DI
-#undef DJ
-#undef DK
+    const double dui = hs[0][0] - hs[0][1], duj = hs[0][2] - hs[0][3], duk = hs[0][4] - hs[0][5];
+    const double dvi = hs[1][0] - hs[1][1], dvj = hs[1][2] - hs[1][3], dvk = hs[1][4] - hs[1][5];
+    const double dwi = hs[2][0] - hs[2][1], dwj = hs[2][2] - hs[2][3], dwk = hs[2][4] - hs[2][5];
+    const double r11 = xix * dui + etx * duj + ztx * duk;
+    const double r12 = xiy * dui + ety * duj + zty * duk + xix * dvi + etx * dvj + ztx * dvk;
+    const double r13 = xiz * dui + etz * duj + ztz * duk + xix * dwi + etx * dwj + ztx * dwk;
+    const double r22 = xiy * dvi + ety * dvj + zty * dvk;
+    const double r23 = xiz * dvi + etz * dvj + ztz * dvk + xiy * dwi + ety * dwj + zty * dwk;
+    const double r33 = xiz * dwi + etz * dwj + ztz * dwk;
     const double s11 = 0.5 * r11, s12 = 0.25 * r12, s13 = 0.25 * r13, s22 = 0.5 * r22,
                  s23 = 0.25 * r23, s33 = 0.5 * r33;
     const double dum = 2.0 * (s11 * s11 + s12 * s12 + s13 * s13 + s12 * s12 + s22 * s22 +
                               s23 * s23 + s13 * s13 + s23 * s23 + s33 * s33);
     const double mh = sqrt(dum);
     if (magh_out) magh_out[c] = mh;
-    const double fw2 = filt2[c];
-    const double uh = UH[c], vh = VH[c], wh = WH[c];
-    double F[12];
-#pragma unroll
-    for (int q = 0; q < 12; ++q) F[q] = (Pa[q] + 2.0 * Pb[q] + Pc[q]) / 64.0;
     // reference component order: 11, 22, 33, 12, 13, 23
     const double hA[6] = {uh, vh, wh, uh, uh, vh};
     const double hB[6] = {uh, vh, wh, vh, wh, wh};
@@ -713,11 +741,12 @@ int fvm_k_sgs(FvmCtx* c, bool sgs_out, bool nl_update) {
   if (r) return r;
   {
     ProfScope pk(c, "k_lm");
-    const dim3 gl((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + LM_KC - 1) / LM_KC);
+    const int kc = LM_KC;
+    const dim3 gl((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + kc - 1) / kc);
     auto kern = c->xz_zero ? k_lm<true> : k_lm<false>;
     hipLaunchKernelGGL(kern, gl, kBlk, 0, st, d, (const double* const*)m, c->u, c->v, c->w,
                        c->uh[0], c->uh[1], c->uh[2], (const double* const*)c->dlss, c->filt2,
-                       c->red, th ? c->magh : nullptr);
+                       c->red, th ? c->magh : nullptr, kc);
     HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_lm_final, dim3(1), dim3(256), 0, st, c->red, (int)(gl.x * gl.y * gl.z),
                        c->scal + 0);
