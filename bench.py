@@ -38,6 +38,9 @@ KERNEL_BYTES = {
     # one pass of the line-scan AF stage: line factors {beta, 1/D, B, c_(a-1)} shared by the
     # three components, {Q} -> {x} per component: 32 + 3 x 16 bytes
     "k_scan_pass_i": (80, 1), "k_scan_pass_j": (80, 1),
+    # the same pass in the FP32 change form: FP32 factors 16 bytes shared, {r} -> {delta}
+    # 3 x 8 bytes
+    "k_scan_pass_i/f32": (40, 1), "k_scan_pass_j/f32": (40, 1),
     # exact K-line sweep: Thomas factors {A, D, B} shared, {SU} -> {x} per component
     "k_af_thomas_k": (72, 1),
     # chunked fallback pass over the 3 components: {phi_in, SU, J, 3 stage metric products}
@@ -66,6 +69,13 @@ KERNEL_BYTES = {
 }
 
 
+# precision forms of the 3D step: (poisson_mode, af_mode; None: --af-mode).  mixed: FP32
+# separable Poisson corrections and FP32 changes of the I and J AF stages, each inside FP64
+# defect correction; mixed-poisson: the FP32 Poisson corrections only
+MODES = {"mixed": (3, 2), "mixed-poisson": (3, None), "fp64": (2, None), "zebra64": (0, None),
+         "zebra32": (1, None)}
+
+
 def cells_of(w):
     return (w["ni"] - 1) * (w["nj"] - 1) * (w["nk"] - 1)
 
@@ -84,10 +94,10 @@ def build_solver(w, mode, resimax, nswpp, check_every, dt, af_mode=0, thermal=Fa
     return s
 
 
-def run_mode(w, mode, a, log):
+def run_mode(w, mode, a, log, af_mode=None):
     dt = w["dt"]
     s = build_solver(w, mode, a.resimax, a.nswpp, a.check_every, dt * a.startup_factor,
-                     a.af_mode, a.thermal)
+                     a.af_mode if af_mode is None else af_mode, a.thermal)
     for q in range(a.startup_steps):
         info = s.step()
         if q == 0 or q + 1 == a.startup_steps:
@@ -122,9 +132,12 @@ def run_mode(w, mode, a, log):
     return dict(secs=t1 - t0, infos=infos, prof=prof)
 
 
-def roofline(prof, ncell):
+def roofline(prof, ncell, af32=False):
     rows = []
-    for name, (b, units) in KERNEL_BYTES.items():
+    for key, (b, units) in KERNEL_BYTES.items():
+        name, _, form = key.partition("/")
+        if (form == "f32") != (af32 and name.startswith("k_scan_pass")):
+            continue
         if name not in prof or prof[name][1] == 0:
             continue
         ms, calls = prof[name]
@@ -422,8 +435,9 @@ def main():
     ncell = cells_of(w)
     results = {}
     for mname in a.modes.split(","):
-        mode = {"mixed": 3, "fp64": 2, "zebra64": 0, "zebra32": 1}[mname]
-        r = run_mode(w, mode, a, log)
+        # precision forms: Poisson form (poisson_mode), AF form (af_mode; None: --af-mode)
+        mode, afm = MODES[mname]
+        r = run_mode(w, mode, a, log, afm)
         results[mname] = r
         log(f"{mname}: {r['secs'] / a.steps * 1e3:.2f} ms/step "
             f"{ncell * a.steps / r['secs'] / 1e6:.1f} Mcell-steps/s")
@@ -436,7 +450,7 @@ def main():
     head = "mixed" if "mixed" in results else list(results)[0]
     r = results[head]
     value = ncell * a.steps / r["secs"] / 1e6
-    rows = roofline(r["prof"], ncell)
+    rows = roofline(r["prof"], ncell, af32=MODES[head][1] == 2)
     roof = None
     if rows:
         top = rows[0]
@@ -449,10 +463,11 @@ def main():
         "value": value, "unit": "Mcell-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": r["secs"] / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64+f32" if head == "mixed" else "f64", "data": "synthetic",
+        "dtype": "f64+f32" if head.startswith("mixed") else "f64", "data": "synthetic",
         "config": {"workload": a.workload, "cells": [w["nj"] - 1, w["ni"] - 1, w["nk"] - 1],
                    "re": w["re"], "dt": w["dt"], "resimax": a.resimax,
-                   "poisson": "mixed" if head == "mixed" else "fp64",
+                   "poisson": "mixed" if MODES[head][0] in (1, 3) else "fp64",
+                   "af": "fp32 change form" if MODES[head][1] == 2 else "fp64",
                    "startup": f"{a.startup_steps} untimed steps at dt={w['dt'] * a.startup_factor:g}",
                    "passive_temperature": bool(a.thermal)},
         "roofline": roof,

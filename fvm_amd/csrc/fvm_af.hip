@@ -133,7 +133,13 @@ __global__ void k_af_products(Dims d, const double* __restrict__ J, const double
 // A.dif = DIFT) and BCT's wall fold adds 2 AW T_WALL (bct.f90:16-19).
 constexpr int RK_KC = 16;
 
-template <int S, bool TH, int NCV = 3>
+// With DR set (stages 0 and 1 of the FP32 correction form, qform on) the kernel stores, as FP32
+// in SU's memory, the residual of the stage's line system at the stage input instead of Q:
+// r = Q + c_(a-1) x_in(a-1) - (AP x_in - lo x_in(p-1) - up x_in(p+1)), with k_scan_factor's AP,
+// folds and couplings; the branch-cut ghosts enter as the periodic images of the interior, the
+// values BCU gives them before the sweep.  The line sweep then solves for the stage's change
+// delta = x - x_in in FP32 and adds it to the FP64 field.
+template <int S, bool TH, int NCV = 3, bool DR = false>
 __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
                                                    const double* const* __restrict__ P,
                                                    const double* __restrict__ ap0, double re,
@@ -204,8 +210,12 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
         }
       }
     }
-    double qE = 0.0, qW = 0.0, qN = 0.0;
+    double qE = 0.0, qW = 0.0, qN = 0.0, rN = 0.0, rS = 0.0;
     const long nf = (long)(d.nj + 1) * (d.nk + 1);
+    if (DR) {
+      rN = fN[c];
+      rS = fN[c - sj];
+    }
     if (qform) {
       qE = fE[c];
       qW = fE[c - 1];
@@ -260,6 +270,25 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
         su = S == 0 ? su + qN * F(0, 1, 0) : su + ae * F(0, 0, 1);
         su = su + at[1] * F(1, 0, 0);
         su = su + at[0] * F(-1, 0, 0);
+      }
+      if (DR) {
+        // the periodic images of the branch-cut ghosts j = 0 and j = NJ
+        const GP fc = gp(A.f[comp]);
+        const double xjm = j == 1 ? fc[cidx(d, i, d.nj - 1, k)] : F(0, -1, 0);
+        const double xjp = j == d.nj - 1 ? fc[cidx(d, i, 1, k)] : F(0, 1, 0);
+        double fe = qE, fw = qW, ap = a0 + fe + fw + rN + rS + at[1] + at[0];
+        if (i == 1) { ap = ap + fw; fw = 0.0; }
+        if (i == d.ni - 1) { ap = ap + fe; fe = 0.0; }
+        double r;
+        if (S == 0) {  // rows along I, lines along J: lo = AW, up = AE, c_(a-1) = AS
+          r = su + rS * xjm;
+          r = r - (ap * F(0, 0, 0) - fw * F(0, 0, -1) - fe * F(0, 0, 1));
+        } else {       // rows along J, lines along I: lo = AS, up = AN, c_(a-1) = AW
+          r = su + fw * F(0, 0, -1);
+          r = r - (ap * F(0, 0, 0) - rS * xjm - rN * xjp);
+        }
+        reinterpret_cast<float*>(A.su[comp])[c] = (float)r;
+        continue;
       }
 #undef F
       gpw(A.su[comp])[c] = su;
@@ -771,6 +800,9 @@ int fvm_k_iter_u(FvmCtx* c) {
   // one-cell span of the 2D slab gets no K stage
   const int nstage = c->two_d ? 2 : 3;
   if (c->two_d) c->af_iters[2] = 0;
+  // af_mode 0 and 2: chunked line scans; 2: the I and J stages in the FP32 change form
+  const bool chunked = c->prm.af_mode == 0 || c->prm.af_mode == 2;
+  const bool f32 = c->prm.af_mode == 2 && c->lf32[0][0] && c->lf32[1][0];
   for (int s = 0; s < nstage; ++s) {
     AFArgs A;
     double* fld[3] = {c->u, c->v, c->w};
@@ -782,17 +814,19 @@ int fvm_k_iter_u(FvmCtx* c) {
     A.ufar = c->ufar;
     {
       ProfScope ps(c, "af_rhs");
-      const int qf = (s < 2 && c->prm.af_mode == 0 && c->af_scan[s]) ? 1 : 0;
+      const int qf = (s < 2 && chunked && c->af_scan[s]) ? 1 : 0;
       const double* const* fas = (const double* const*)c->dfa[s];
       const int dz = c->dd2_zero ? 1 : 0;
       const dim3 gk((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + RK_KC - 1) / RK_KC);
       if (s == 0) {
         ProfScope pk(c, "k_af_rhs_i");
-        hipLaunchKernelGGL((k_af_rhs_km<0, false>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[0], c->ap0, c->prm.re, fas, qf, fn_alias);
+        auto kern = (qf && f32) ? k_af_rhs_km<0, false, 3, true> : k_af_rhs_km<0, false, 3, false>;
+        hipLaunchKernelGGL(kern, gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[0], c->ap0, c->prm.re, fas, qf, fn_alias);
       }
       if (s == 1) {
         ProfScope pk(c, "k_af_rhs_j");
-        hipLaunchKernelGGL((k_af_rhs_km<1, false>), gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[1], c->ap0, c->prm.re, fas, qf, 0);
+        auto kern = (qf && f32) ? k_af_rhs_km<1, false, 3, true> : k_af_rhs_km<1, false, 3, false>;
+        hipLaunchKernelGGL(kern, gk, kBlk, 0, st, d, A, (const double* const*)c->dafp[1], c->ap0, c->prm.re, fas, qf, 0);
       }
       if (s == 2) hipLaunchKernelGGL(k_af_rhs<2>, gi, kBlk, 0, st, d, A, (const double* const*)c->dm, c->djac, c->ap0, c->prm.re, fas, qf, dz);
       HIP_CHECK(hipGetLastError());
@@ -801,7 +835,7 @@ int fvm_k_iter_u(FvmCtx* c) {
     int r = fvm_k_bcu_ghosts(c, fld, 3);
     if (r) return r;
     ProfScope ps(c, s == 0 ? "af_sweep_i" : (s == 1 ? "af_sweep_j" : "af_sweep_k"));
-    if (s == 2 && c->prm.af_mode == 0 && c->tk_a && c->af_zero_a[2] && c->af_zero_b[2]) {
+    if (s == 2 && chunked && c->tk_a && c->af_zero_a[2] && c->af_zero_b[2]) {
       // independent K-lines: the exact sweep in one pass
       ProfScope pk(c, "k_af_thomas_k");
       const dim3 g((d.ni - 1 + 63) / 64, d.nj - 1, 1);
@@ -813,9 +847,9 @@ int fvm_k_iter_u(FvmCtx* c) {
                            c->tk_b);
       HIP_CHECK(hipGetLastError());
       c->af_iters[s] = 1;
-    } else if (s < 2 && c->prm.af_mode == 0 && c->af_scan[s]) {
+    } else if (s < 2 && chunked && c->af_scan[s]) {
       int it = 0;
-      r = fvm_af_scan_stage(c, s, A, &it);
+      r = fvm_af_scan_stage(c, s, A, &it, false, f32);
       if (r) return r;
       c->af_iters[s] = it;
     } else if (c->prm.af_mode == 1) {
@@ -862,9 +896,9 @@ int fvm_k_iter_u_one(FvmCtx* c, int comp, double* f, const double* fn, const dou
   const int nstage = c->two_d ? 2 : 3;
   const long nf = (long)(d.nj + 1) * (d.nk + 1);
   for (int s = 0; s < nstage; ++s) {
-    const bool exact_k = s == 2 && c->prm.af_mode == 0 && c->tk_a && c->af_zero_a[2] &&
-                         c->af_zero_b[2];
-    const bool scan = s < 2 && c->prm.af_mode == 0 && c->af_scan[s];
+    const bool chunked = c->prm.af_mode == 0 || c->prm.af_mode == 2;  // FP64 scans here
+    const bool exact_k = s == 2 && chunked && c->tk_a && c->af_zero_a[2] && c->af_zero_b[2];
+    const bool scan = s < 2 && chunked && c->af_scan[s];
     if (!exact_k && !scan && c->prm.af_mode != 1) {
       fprintf(stderr, "fvm_amd: ITER_U of one component needs line-separable stage couplings "
                       "(stage %d is not); use fvm_iter_u\n", s);

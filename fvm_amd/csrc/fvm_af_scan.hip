@@ -32,6 +32,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -93,14 +94,15 @@ __global__ void k_scan_factor(Dims d, const double* const* __restrict__ fa,
 
 // Q of the J stage (formed by the right-hand-side kernel in the natural layout) into the
 // line-major layout, 32 x 32 (I x J) tiles transposed through LDS.
+template <typename T>
 __global__ void __launch_bounds__(256) k_scan_q1v(Dims d, PtrSet8 qn, double* const* q) {
-  __shared__ double tile[32][33];
+  __shared__ T tile[32][33];
   const int i0 = 1 + blockIdx.x * 32, j0 = 1 + blockIdx.y * 32;
   const int k = 1 + (int)(blockIdx.z % (d.nk - 1));
   const int comp = (int)(blockIdx.z / (d.nk - 1));
   const int t = threadIdx.x;
-  const GP src = gp(qn.p[comp]);
-  const GPW dst = gpw(q[comp]);
+  const T* __restrict__ src = reinterpret_cast<const T*>(qn.p[comp]);
+  T* __restrict__ dst = reinterpret_cast<T*>(q[comp]);
   for (int r = 0; r < 4; ++r) {
     const int li = t % 32, lj = t / 32 + 8 * r;
     const int i = i0 + li, j = j0 + lj;
@@ -115,17 +117,19 @@ __global__ void __launch_bounds__(256) k_scan_q1v(Dims d, PtrSet8 qn, double* co
   }
 }
 
-// converged J-line iterate (line-major) back into the field interior
+// converged J-line iterate (line-major) back into the field interior; T = float: the iterate is
+// the stage's change, added to the field
+template <typename T>
 __global__ void __launch_bounds__(256) k_scan_unpack1(Dims d, const double* const* xo,
                                                       double* const* f) {
-  __shared__ double tile[32][33];
+  __shared__ T tile[32][33];
   const int i0 = 1 + blockIdx.x * 32, j0 = 1 + blockIdx.y * 32;
   const int k = 1 + (int)(blockIdx.z % (d.nk - 1));
   const int comp = (int)(blockIdx.z / (d.nk - 1));
   const int t = threadIdx.x;
   const long lsa = d.nj + 1, lsb = (long)(d.nj + 1) * (d.ni + 1);
-  const GP src = gp(xo[comp]);
-  const GPW dst = gpw(f[comp]);
+  const T* __restrict__ src = reinterpret_cast<const T*>(xo[comp]);
+  double* __restrict__ dst = f[comp];
   for (int r = 0; r < 4; ++r) {
     const int lj = t % 32, li = t / 32 + 8 * r;
     const int i = i0 + li, j = j0 + lj;
@@ -135,14 +139,37 @@ __global__ void __launch_bounds__(256) k_scan_unpack1(Dims d, const double* cons
   for (int r = 0; r < 4; ++r) {
     const int li = t % 32, lj = t / 32 + 8 * r;
     const int i = i0 + li, j = j0 + lj;
-    if (i <= d.ni - 1 && j <= d.nj - 1) dst[cidx(d, i, j, k)] = tile[lj][li];
+    if (i <= d.ni - 1 && j <= d.nj - 1) {
+      const long c = cidx(d, i, j, k);
+      dst[c] = std::is_same<T, float>::value ? dst[c] + (double)tile[lj][li]
+                                             : (double)tile[lj][li];
+    }
   }
+}
+
+// FP32 correction form of the I stage: the converged change (natural layout) added to the field
+__global__ void __launch_bounds__(256) k_scan_add0(Dims d, PtrSet8 dl, PtrSet8 f) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + (int)(blockIdx.z % (d.nk - 1));
+  const int comp = (int)(blockIdx.z / (d.nk - 1));
+  if (i > d.ni - 1 || j > d.nj - 1) return;
+  const long c = cidx(d, i, j, k);
+  const float* __restrict__ x = reinterpret_cast<const float*>(dl.p[comp]);
+  f.p[comp][c] = f.p[comp][c] + (double)x[c];
+}
+
+__global__ void k_to_float(long n, const double* __restrict__ a, float* b) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x)
+    b[t] = (float)a[t];
 }
 
 // boundary lines of the first pass: the stage input at the last line of every chunk
 // (bnd[((comp nb + b-1) nchunk + ca) n + p-1])
-template <int S>
-__global__ void k_scan_seed(Dims d, AFArgs A, int nchunk, double* bnd) {
+// (T = float: the change form, whose stage input is the zero change)
+template <int S, typename T>
+__global__ void k_scan_seed(Dims d, AFArgs A, int nchunk, T* bnd) {
   const ScanGeo g = scan_geo(d, S);
   const long tot = (long)A.nc * g.nb * nchunk * g.n;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot;
@@ -154,7 +181,8 @@ __global__ void k_scan_seed(Dims d, AFArgs A, int nchunk, double* bnd) {
     const int b = 1 + (int)(r % g.nb);
     const int comp = (int)(r / g.nb);
     const int a = min((ca + 1) * SC_C, g.na);
-    bnd[t] = A.f[comp][g.ns_p * p + g.ns_a * a + g.ns_b * b];
+    bnd[t] = std::is_same<T, float>::value ? (T)0
+                                           : (T)A.f[comp][g.ns_p * p + g.ns_a * a + g.ns_b * b];
   }
 }
 
@@ -176,20 +204,27 @@ __device__ inline double shfl_down_d(double v, int o) { return __shfl_down(v, o,
 // remaining lines of the chunk would change by less than that too and keep their values from
 // the previous pass; the chunk's last line is then unchanged (bnd_out = bnd_in).  Where rho is
 // small (rho ~ dt |grad a|^2 / 2 Re), a pass after the first re-solves a few lines per chunk.
-template <int S, int R, bool EARLY>
-__global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double* __restrict__ beta,
-                                                  const double* __restrict__ invd,
-                                                  const double* __restrict__ bb,
-                                                  const double* __restrict__ cam,
-                                                  const double* const* q, double* const* xo,
-                                                  const double* __restrict__ bnd_in,
-                                                  double* bnd_out, int nchunk, double* part,
-                                                  double tol, double* sink) {
+//
+// T = float: the FP32 correction form.  q holds the FP32 residual of the line systems at the stage
+// input (k_af_rhs_km with DR), the factors are FP32 copies, the iterate is the change of the
+// stage (zero at the line ends and on the ghost line before the first chunk) and the arithmetic
+// is FP32; the FP64 field is updated with the converged change afterwards.
+template <int S, int R, bool EARLY, typename T>
+__global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const T* __restrict__ beta,
+                                                  const T* __restrict__ invd,
+                                                  const T* __restrict__ bb,
+                                                  const T* __restrict__ cam,
+                                                  const double* const* qt, double* const* xot,
+                                                  const T* __restrict__ bnd_in,
+                                                  T* bnd_out, int nchunk, double* part,
+                                                  double tol, double* sinkd) {
+  constexpr bool DELTA = std::is_same<T, float>::value;
   const ScanGeo g = scan_geo(d, S);
   const long G = gridDim.x;
   const long lin = blockIdx.x;
   const long w = (lin % 8) * (G / 8) + lin / 8;  // XCD-aware: neighbouring items share an XCD
   const int l = threadIdx.x;
+  T* const sink = reinterpret_cast<T*>(sinkd);
   double dmax = 0.0, xmax = 0.0;
   if (w < (long)A.nc * nchunk * g.nb) {
     const int comp = (int)(w % A.nc);
@@ -199,19 +234,19 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
     const int a0 = 1 + ca * SC_C;
     const int a1 = min(a0 + SC_C - 1, g.na);
     const int p0 = l * R + 1;
-    const GP qc = gp(q[comp]);
+    const T* __restrict__ qc = reinterpret_cast<const T*>(qt[comp]);
     const GP fc = gp(A.f[comp]);
-    const GPW xc = gpw(xo[comp]);
-    double xp[R];
+    T* __restrict__ xc = reinterpret_cast<T*>(xot[comp]);
+    T xp[R];
     // line a0-1: the stage input's ghost line for the first chunk, else the preceding
     // chunk's last line of the previous pass
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int p = p0 + r;
-      double v = 0.0;
+      T v = (T)0;
       if (p <= g.n) {
         if (ca == 0)
-          v = fc[g.ns_p * p + g.ns_a * (a0 - 1) + g.ns_b * b];
+          v = DELTA ? (T)0 : (T)fc[g.ns_p * p + g.ns_a * (a0 - 1) + g.ns_b * b];
         else
           v = bnd_in[(((long)comp * g.nb + (b - 1)) * nchunk + (ca - 1)) * g.n + (p - 1)];
       }
@@ -221,28 +256,34 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
     // and the next line is always fetched (the last line again at the end of the chunk): with
     // no load or store behind a branch the compiler counts the loads in flight exactly and
     // waits only for the ones it needs, so the next line's loads overlap this line's scans.
-    double be[R], iv[R], bv[R], cm[R], qv[R], xv[R], y0n, xnn;
+    T be[R], iv[R], bv[R], cm[R], qv[R], xv[R], y0n, xnn;
     auto load = [&](int a) {
       const long lb = g.ls_a * a + g.ls_b * b;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const long o = lb + min(p0 + r, g.n);
         be[r] = beta[o]; iv[r] = invd[o]; bv[r] = bb[o]; cm[r] = cam[o]; qv[r] = qc[o];
-        xv[r] = EARLY ? xc[o] : 0.0;  // the previous pass's value of the line
+        xv[r] = EARLY ? xc[o] : (T)0;  // the previous pass's value of the line
       }
-      const long nb0 = g.ns_a * a + g.ns_b * b;  // natural base of the line
-      y0n = fc[nb0];                             // row 0 ghost
-      xnn = fc[nb0 + g.ns_p * (g.n + 1)];        // row n+1 ghost
+      if (DELTA) {
+        y0n = (T)0;
+        xnn = (T)0;
+      } else {
+        const long nb0 = g.ns_a * a + g.ns_b * b;  // natural base of the line
+        y0n = (T)fc[nb0];                          // row 0 ghost
+        xnn = (T)fc[nb0 + g.ns_p * (g.n + 1)];     // row n+1 ghost
+      }
     };
     load(a0);
     bool stopped = false;
-    double xrun = 0.0;  // largest |x| of the lines this wavefront solved in this pass
+    T xrun = (T)0;  // largest |x| of the lines this wavefront solved in this pass
+    const T ttol = (T)tol;
     for (int a = a0; a <= a1; ++a) {
       const long lb = g.ls_a * a + g.ls_b * b;
-      const double y0 = y0n, xn = xnn;
+      const T y0 = y0n, xn = xnn;
       // forward: y(p) = al(p) + be(p) y(p-1)
-      double al[R], bt[R], bw[R], xo[R];
-      double Bt = 1.0, At = 0.0;
+      T al[R], bt[R], bw[R], xo[R];
+      T Bt = (T)1, At = (T)0;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         al[r] = (qv[r] + cm[r] * xp[r]) * iv[r];
@@ -255,33 +296,33 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
       }
       load(min(a + 1, a1));  // next line's operands in flight during the scans
       __builtin_amdgcn_sched_barrier(0);  // keep those loads ahead of the scans
-      double yin = wave::prefix_apply(At, Bt, y0);  // y0, xn: the same on every lane
-      double y[R];
+      T yin = wave::prefix_apply(At, Bt, y0);  // y0, xn: the same on every lane
+      T y[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         y[r] = al[r] + bt[r] * yin;
         yin = p0 + r <= g.n ? y[r] : yin;
       }
       // backward: x(p) = y(p) - B(p) x(p+1)
-      double Ct = 1.0, Dt = 0.0;
+      T Ct = (T)1, Dt = (T)0;
 #pragma unroll
       for (int r = R - 1; r >= 0; --r) {
         const bool in = p0 + r <= g.n;
         Dt = in ? y[r] - bw[r] * Dt : Dt;
         Ct = in ? -bw[r] * Ct : Ct;
       }
-      double xin = wave::suffix_apply(Dt, Ct, xn);
-      double dl = 0.0;
+      T xin = wave::suffix_apply(Dt, Ct, xn);
+      T dl = (T)0;
 #pragma unroll
       for (int r = R - 1; r >= 0; --r) {
         const int p = p0 + r;
         const bool in = p <= g.n;
-        const double x = y[r] - bw[r] * xin;
+        const T x = y[r] - bw[r] * xin;
         xin = in ? x : xin;
         dl = in ? fmax(dl, fabs(x - xo[r])) : dl;
         xrun = in ? fmax(xrun, fabs(x)) : xrun;
         // a row past n stores to this lane's slot of the sink instead of a branch
-        const GPW dst = in ? xc + (lb + p) : gpw(sink) + l;
+        T* const dst = in ? xc + (lb + p) : sink + l;
         *dst = x;
         xp[r] = in ? x : xp[r];
       }
@@ -291,7 +332,7 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
           dl = fmax(dl, __shfl_xor(dl, o, 64));
           xrun = fmax(xrun, __shfl_xor(xrun, o, 64));
         }
-        if (dl <= tol * xrun) { stopped = true; break; }  // uniform over the wavefront
+        if (dl <= ttol * xrun) { stopped = true; break; }  // uniform over the wavefront
       }
     }
     if (stopped) {  // the chunk's last line keeps the previous pass's value
@@ -307,9 +348,9 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const double
       const int p = p0 + r;
       if (p <= g.n) {
         const long o = (((long)comp * g.nb + (b - 1)) * nchunk + ca) * g.n + (p - 1);
-        const double x = xp[r];
-        dmax = fmax(dmax, fabs(x - bnd_in[o]));
-        xmax = fmax(xmax, fabs(x));
+        const T x = xp[r];
+        dmax = fmax(dmax, (double)fabs(x - bnd_in[o]));
+        xmax = fmax(xmax, (double)fabs(x));
         bnd_out[o] = x;
       }
     }
@@ -341,8 +382,9 @@ __global__ void k_scan_max(const double* __restrict__ part, long nb, double* out
   }
 }
 
-template <int S, int R>
+template <int S, int R, typename T>
 int scan_stage(FvmCtx* c, const AFArgs& A, int* iters, bool temp) {
+  constexpr bool DELTA = std::is_same<T, float>::value;
   const Dims d = c->d;
   hipStream_t st = c->stream;
   const ScanGeo g = scan_geo(d, S);
@@ -351,12 +393,13 @@ int scan_stage(FvmCtx* c, const AFArgs& A, int* iters, bool temp) {
   const long nwork = (long)nc * nchunk * g.nb;
   const long G = (nwork + 7) / 8 * 8;
   if (2 * G > c->nred) return FVM_ERR_SIZE;
-  // Q: the right-hand-side kernel left it in A.su (natural layout); the J stage uses a
-  // line-major copy
+  // Q (or the FP32 residual): the right-hand-side kernel left it in A.su (natural layout); the
+  // J stage uses a line-major copy.  The iterate is the field itself for the FP64 I stage, else
+  // a work array (line-major for J).
   double* q[3] = {};
   for (int k = 0; k < nc; ++k) q[k] = S == 0 ? A.su[k] : c->afb[k][1];
   double* xo[3] = {};
-  for (int k = 0; k < nc; ++k) xo[k] = S == 0 ? A.f[k] : c->afb[k][0];
+  for (int k = 0; k < nc; ++k) xo[k] = (S == 0 && !DELTA) ? A.f[k] : c->afb[k][0];
   double* fo[3] = {};
   for (int k = 0; k < nc; ++k) fo[k] = A.f[k];
   // device pointer table of this stage: q at [0..2], iterate at [3..5], field at [6..8]; the
@@ -371,28 +414,35 @@ int scan_stage(FvmCtx* c, const AFArgs& A, int* iters, bool temp) {
   }
   double* const* dq = dtab;
   double* const* dxo = dtab + 3;
+  T* bnd[2] = {reinterpret_cast<T*>(c->bnd[0]), reinterpret_cast<T*>(c->bnd[1])};
   {
     ProfScope pk(c, "af_scan_q");
     if (S == 1) {
       PtrSet8 qn{};
       for (int k = 0; k < nc; ++k) qn.p[k] = A.su[k];
-      hipLaunchKernelGGL(k_scan_q1v, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, nc * (d.nk - 1)),
+      hipLaunchKernelGGL(k_scan_q1v<T>, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, nc * (d.nk - 1)),
                          dim3(256), 0, st, d, qn, dq);
     }
-    hipLaunchKernelGGL(k_scan_seed<S>, grid1((long)nc * g.nb * nchunk * g.n, 256), dim3(256), 0,
-                       st, d, A, nchunk, c->bnd[0]);
+    hipLaunchKernelGGL((k_scan_seed<S, T>), grid1((long)nc * g.nb * nchunk * g.n, 256), dim3(256),
+                       0, st, d, A, nchunk, bnd[0]);
     HIP_CHECK(hipGetLastError());
   }
-  const double tol = c->prm.af_tol > 0 ? c->prm.af_tol : 1e-14;
+  // the FP32 change is resolved to its own precision
+  double tol = c->prm.af_tol > 0 ? c->prm.af_tol : 1e-14;
+  if (DELTA) tol = std::max(tol, 6e-8);
   const int cap = nchunk + 1;
   int slot = 0, it = 0;
-  double* const* lf = temp ? c->lft[S] : c->lf[S];
+  const T* const* lf;
+  if (DELTA)
+    lf = reinterpret_cast<const T* const*>(c->lf32[S]);
+  else
+    lf = reinterpret_cast<const T* const*>(temp ? c->lft[S] : c->lf[S]);
   for (;;) {
     {
       ProfScope pk(c, S == 0 ? "k_scan_pass_i" : "k_scan_pass_j");
-      auto kern = it > 0 ? k_scan_pass<S, R, true> : k_scan_pass<S, R, false>;
+      auto kern = it > 0 ? k_scan_pass<S, R, true, T> : k_scan_pass<S, R, false, T>;
       hipLaunchKernelGGL(kern, dim3((unsigned)G), dim3(64), 0, st, d, A, lf[0], lf[1], lf[2],
-                         lf[3], (const double* const*)dq, dxo, c->bnd[slot], c->bnd[slot ^ 1],
+                         lf[3], (const double* const*)dq, dxo, bnd[slot], bnd[slot ^ 1],
                          nchunk, c->red, tol, c->sink);
     }
     // the first pass never suffices on its own: its maxima are not read
@@ -413,8 +463,15 @@ int scan_stage(FvmCtx* c, const AFArgs& A, int* iters, bool temp) {
   }
   if (S == 1) {
     ProfScope pk(c, "af_scan_unpack");
-    hipLaunchKernelGGL(k_scan_unpack1, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, nc * (d.nk - 1)),
+    hipLaunchKernelGGL(k_scan_unpack1<T>, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, nc * (d.nk - 1)),
                        dim3(256), 0, st, d, (const double* const*)dxo, dtab + 6);
+    HIP_CHECK(hipGetLastError());
+  } else if (DELTA) {
+    ProfScope pk(c, "af_scan_add");
+    PtrSet8 dl{}, f{};
+    for (int k = 0; k < nc; ++k) { dl.p[k] = xo[k]; f.p[k] = A.f[k]; }
+    hipLaunchKernelGGL(k_scan_add0, dim3((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, nc * (d.nk - 1)),
+                       dim3(64, 4, 1), 0, st, d, dl, f);
     HIP_CHECK(hipGetLastError());
   }
   *iters = it;
@@ -453,6 +510,17 @@ int fvm_af_scan_setup(FvmCtx* c) {
                            (const double* const*)c->dfa[1], a0, L[0], L[1], L[2], L[3]);
       HIP_CHECK(hipGetLastError());
     }
+    if (c->prm.af_mode == 2) {  // FP32 copies of the momentum factors (correction form)
+      for (int q = 0; q < 4; ++q) {
+        if (!c->lf32[s][q]) {
+          HIP_CHECK(hipMalloc((void**)&c->lf32[s][q], d.n * sizeof(float)));
+          c->allocs.push_back(c->lf32[s][q]);
+        }
+        hipLaunchKernelGGL(k_to_float, grid1(d.n, 256), dim3(256), 0, st, d.n, c->lf[s][q],
+                           c->lf32[s][q]);
+      }
+      HIP_CHECK(hipGetLastError());
+    }
   }
   if (nbnd > c->nbnd) {
     for (int q = 0; q < 2; ++q) {
@@ -465,18 +533,27 @@ int fvm_af_scan_setup(FvmCtx* c) {
   return FVM_OK;
 }
 
-int fvm_af_scan_stage(FvmCtx* c, int s, const AFArgs& A, int* iters, bool temp) {
+template <typename T>
+static int scan_stage_r(FvmCtx* c, int s, const AFArgs& A, int* iters, bool temp) {
   const ScanGeo g = scan_geo(c->d, s);
   const int R = (g.n + 63) / 64;
-  if (A.nc < 1 || A.nc > 3 || (temp && !c->lft[s][0])) return FVM_ERR_ARG;
   if (s == 0) {
-    if (R <= 3) return scan_stage<0, 3>(c, A, iters, temp);
-    if (R <= 4) return scan_stage<0, 4>(c, A, iters, temp);
-    if (R <= 6) return scan_stage<0, 6>(c, A, iters, temp);
-    return scan_stage<0, SC_R>(c, A, iters, temp);
+    if (R <= 3) return scan_stage<0, 3, T>(c, A, iters, temp);
+    if (R <= 4) return scan_stage<0, 4, T>(c, A, iters, temp);
+    if (R <= 6) return scan_stage<0, 6, T>(c, A, iters, temp);
+    return scan_stage<0, SC_R, T>(c, A, iters, temp);
   }
-  if (R <= 3) return scan_stage<1, 3>(c, A, iters, temp);
-  if (R <= 4) return scan_stage<1, 4>(c, A, iters, temp);
-  if (R <= 6) return scan_stage<1, 6>(c, A, iters, temp);
-  return scan_stage<1, SC_R>(c, A, iters, temp);
+  if (R <= 3) return scan_stage<1, 3, T>(c, A, iters, temp);
+  if (R <= 4) return scan_stage<1, 4, T>(c, A, iters, temp);
+  if (R <= 6) return scan_stage<1, 6, T>(c, A, iters, temp);
+  return scan_stage<1, SC_R, T>(c, A, iters, temp);
+}
+
+int fvm_af_scan_stage(FvmCtx* c, int s, const AFArgs& A, int* iters, bool temp, bool delta) {
+  if (A.nc < 1 || A.nc > 3 || (temp && !c->lft[s][0])) return FVM_ERR_ARG;
+  if (delta) {
+    if (temp || !c->lf32[s][0]) return FVM_ERR_ARG;
+    return scan_stage_r<float>(c, s, A, iters, false);
+  }
+  return scan_stage_r<double>(c, s, A, iters, temp);
 }

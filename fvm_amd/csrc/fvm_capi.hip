@@ -121,12 +121,12 @@ int fvm_thermal_setup(FvmCtx* c) {
   if (r) return r;
   // the temperature uses the exact K-line stage and the line-scan form of the I and J stages
   const double ratio = c->prm.re / c->pe;
-  const bool ok = c->prm.af_mode == 0 && c->tk_at && c->af_zero_a[2] && c->af_zero_b[2] &&
+  const bool ok = (c->prm.af_mode == 0 || c->prm.af_mode == 2) && c->tk_at && c->af_zero_a[2] && c->af_zero_b[2] &&
                   c->af_scan[0] && c->af_scan[1] && c->af_b_ratio[0] * ratio <= 1e-15 &&
                   c->af_b_ratio[1] * ratio <= 1e-15 && c->lft[0][0] && c->lft[1][0];
   if (!ok) {
     fprintf(stderr, "fvm_amd: the passive temperature needs an extruded grid (independent K "
-                    "lines, negligible K couplings of the I and J stages) and af_mode 0\n");
+                    "lines, negligible K couplings of the I and J stages) and af_mode 0 or 2\n");
     c->thermal = false;
     return FVM_ERR_ARG;
   }

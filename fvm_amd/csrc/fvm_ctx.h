@@ -78,6 +78,7 @@ struct FvmCtx {
   // factors (beta = -A/D, 1/D, B, coupling to the previous line) and the chunk boundary lines
   bool af_scan[2] = {};
   double* lf[2][4] = {};
+  float* lf32[2][4] = {};   // FP32 copies of lf (af_mode 2: FP32 change form)
   double* bnd[2] = {};
   long nbnd = 0;
   // reductions
@@ -175,7 +176,8 @@ int fvm_geom_setup(FvmCtx* c);
 int fvm_k_set_dtime(FvmCtx* c, double dtime);
 int fvm_af_setup(FvmCtx* c);
 int fvm_af_scan_setup(FvmCtx* c);
-int fvm_af_scan_stage(FvmCtx* c, int s, const AFArgs& A, int* iters, bool temp = false);
+int fvm_af_scan_stage(FvmCtx* c, int s, const AFArgs& A, int* iters, bool temp = false,
+                      bool delta = false);
 int fvm_ghost_fill(FvmCtx* c, double* a);
 int fvm_ghost_fill_n(FvmCtx* c, double** a, int cnt);
 int fvm_k_contra_vel(FvmCtx* c);

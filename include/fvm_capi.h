@@ -43,7 +43,12 @@ typedef struct fvm_params {
   int check_every;      /* sweeps between residual evaluations (>=1)                     */
   int af_mode;          /* AF line sweeps in the reference's lexicographic order, computed
                            0: by chunked block iteration to convergence (default);
-                           1: by anti-diagonal wavefront launches                          */
+                           1: by anti-diagonal wavefront launches;
+                           2: as 0, with the I and J stages of the momentum step (fvm_step,
+                              fvm_iter_u) solved for their change in FP32 against the FP64
+                              residual of the stage input, the change added to the FP64
+                              field (mixed form; the K stage, ITER_T and the one-component
+                              entries stay FP64)                                            */
   double af_tol;        /* chunked iteration: stop when max change <= af_tol * max |u|
                            (<= 0: 1e-14)                                                 */
 } fvm_params;

@@ -53,3 +53,32 @@ def test_precision_forms_reach_same_fixed_point():
     for c in ("u", "v", "w"):
         assert rel(s32.get(c), s64.get(c)) < 1e-8, c
     assert rel(s32.get("p"), s64.get("p")) < 1e-7
+
+
+def test_fp32_change_form_of_af_stages():
+    """af_mode 2: the I and J stages solve for their change in FP32 against the FP64 residual of
+    the stage input (delta form); the predictor equals the FP64 sweeps' to the FP32 resolution
+    of that change."""
+    s0 = make(af_mode=0)
+    start = {c: s0.get(c) for c in ("u", "v", "w")}
+    a = predictor(s0)
+    b = predictor(make(af_mode=2))
+    for c in ("u", "v", "w"):
+        # the difference against the size of the change the stages made (first step from the
+        # seeded start: the change is O(1) near the wall)
+        e = float(np.abs(b[c] - a[c]).max() / np.abs(a[c] - start[c]).max())
+        print(f"predictor {c}: FP32 change form vs FP64 sweeps, |diff| / |change| {e:.2e}",
+              flush=True)
+        assert e < 2e-6, c
+
+
+def test_fp32_change_form_steps_against_fp64():
+    s64, s32 = make(poisson_mode=2), make(poisson_mode=3, af_mode=2)
+    for st in range(3):
+        i64, i32 = s64.step(), s32.step()
+        assert i64["resip"] <= 1e-10 and i32["resip"] <= 1e-10
+    for c in ("u", "v", "w", "p"):
+        e = rel(s32.get(c), s64.get(c))
+        print(f"3 steps {c}: mixed form (FP32 Poisson and AF corrections) vs FP64, rel {e:.2e}",
+              flush=True)
+        assert e < (2e-5 if c == "p" else 2e-6), c
