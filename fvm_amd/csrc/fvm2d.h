@@ -70,12 +70,19 @@ struct Case2D {
   double* part;        // per-block partial sums of this case (residual pairs, CFL maxima)
   double* stat;        // per-case scalars (see ST_*)
   int ntile;           // stencil tiles of this case (partials per pass)
+  double* swb;         // AF sweep chunk boundary lines [pass slot][component][chunk][row]
   double* a[NA2];      // adjoint planes (null until the adjoint is initialised)
   double* ab[NAB2];
 };
 
+// lines per chunk of the batched AF sweeps, and chunks of a sweep over nl lines
+constexpr int SW_C = 16;
+__host__ __device__ inline int sw_nchunk(int nl) { return (nl + SW_C - 1) / SW_C; }
+
 // per-case device scalars
 enum ST {
   ST_RESIP = 0, ST_S0, ST_S1, ST_DONE, ST_NPASS, ST_CFL, ST_CPB, ST_CD, ST_CL, ST_CDP, ST_CDV,
-  ST_CLP, ST_CLV, ST_RES0, ST_NAN, ST_ARES, NST = 16
+  ST_CLP, ST_CLV, ST_RES0, ST_NAN, ST_ARES,
+  ST_AFCH,  // largest relative change of a chunk's boundary line in the last AF sweep pass
+  NST = 17
 };

@@ -119,6 +119,14 @@ __global__ void __launch_bounds__(256) k2_explicit(const Case2D* __restrict__ ta
   cs.f[F_DIFY][c] = dify;
   cs.f[F_SU][c] = su;
   cs.f[F_SV][c] = sv;
+  // the old next line's coupling AN U(J+1), taken into the right-hand side before the sweep; the
+  // branch-cut ghost J = NJ is BCU's periodic image J = 1
+  {
+    const double an = cs.f[G_AN1][c];
+    const long cn = j == nj - 1 ? (long)i + sj : c + sj;
+    r1u = r1u + an * U[cn];
+    r1v = r1v + an * V[cn];
+  }
   cs.f[F_SU1][c] = r1u;
   cs.f[F_SV1][c] = r1v;
 }
@@ -144,80 +152,95 @@ __device__ inline void bcu_ghosts_wave(double* F, int ni, int nj, int sj, const 
 }
 
 // ---------------------------------------------------------------------------------------------
-// The AF line Gauss-Seidel sweep of one stage (iter_u.f90:50-66 I lines, 103-119 J lines), in
-// the reference's order: lines one after the other, each line solved with the already updated
-// previous line and the old next line.  One 64-lane wavefront per (case, component), all cases
-// in one launch: BCU's ghosts first, then per line the Thomas recursion as two affine-map scans
-// over the lanes (lane l holds rows l R .. l R + R - 1, R = ceil(rows / 64) of the case, and
-// keeps the previous line's values in registers).  The operands of line L + 1 (right-hand side,
-// factors, lagged couplings with the old next-but-one line, ghosts) are loaded while line L is
-// solved: none of them is written by the sweep before line L + 1 reads it.
-// DIR 0: I lines (rows along I, lines J = 1..NJ-1, lagged couplings AN, AS = AN(J-1));
-// DIR 1: J lines (rows along J, lines I = 1..NI-1, lagged couplings AE, AW); the line ends take
-// the ghost values as Dirichlet data.
+// The AF line Gauss-Seidel sweep of one stage (iter_u.f90:50-66 I lines, 103-119 J lines): lines
+// in the reference's order, each solved with the already updated previous line and the old next
+// line.  The old next line's coupling is part of the right-hand side Q formed before the sweep
+// (k2_explicit, k2_rhs2), so a line needs only the previous line.  Per line the Thomas recursion
+// runs as two affine-map scans over the 64 lanes (lane l holds rows l R .. l R + R - 1,
+// R = ceil(rows / 64) of the case).
+//
+// Chunked form: the lines of a case are cut into chunks of SW_C consecutive lines and each
+// (case, component, chunk) is one wavefront; a chunk takes the line before it from the
+// preceding chunk's last line of the previous pass (block Jacobi across chunks, the stage input
+// in the first pass).  The passes are nilpotent (exact after as many passes as chunks) and each
+// contracts the chunk-boundary error by rho^SW_C, rho < 1 the ratio of the lagged coupling to
+// the diagonal dominance of the line operator; a pass after the first re-solves a chunk's lines
+// only until a line stops changing (max change <= tol max |x|), the rest keep their values.
+// The last pass books the largest relative change of a chunk's last line in the case's
+// ST_AFCH.  The operands of line L + 1 are loaded while line L is solved.
+// DIR 0: I lines (rows along I, lines J = 1..NJ-1, lagged coupling AS = AN(J-1));
+// DIR 1: J lines (rows along J, lines I = 1..NI-1, lagged coupling AW); the line ends take the
+// ghost values (BCU, applied by k2_sweep_seed) as Dirichlet data.
 // ---------------------------------------------------------------------------------------------
 template <int R>
 struct LineOps {
-  double cupf[R], cdn[R], q[R], iv[R], na[R], bf[R];  // cup x_old(next line) formed at load
-  double g0, g1;                                       // ghosts before / after the line
+  double cdn[R], q[R], iv[R], na[R], bf[R], xo[R];
+  double g0, g1;                                   // ghosts before / after the line
 };
 
-template <int R, int DIR>
+template <int R, int DIR, bool EARLY>
 __device__ inline void sweep_load(LineOps<R>& o, const double* F, const double* Q,
                                   const double* invd, const double* na, const double* bf,
-                                  const double* cup, const double* cdn, long base, long rs,
-                                  long ls, long dnoff, int n, int r0, int l) {
+                                  const double* cdn, long base, long rs, long dnoff, int n, int r0,
+                                  int l) {
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int r = r0 + q;
     if (r < n) {
       const long c = base + rs * (r + 1);
-      o.cupf[q] = cup[c] * F[c + ls];
       o.cdn[q] = cdn[c + dnoff];
       o.q[q] = Q[c];
       o.iv[q] = invd[c];
       o.na[q] = na[c];
       o.bf[q] = bf[c];
+      o.xo[q] = EARLY ? F[c] : 0.0;  // the previous pass's value of the row
     }
   }
   o.g0 = l == 0 ? F[base] : 0.0;
   o.g1 = l == 63 ? F[base + rs * (n + 1)] : 0.0;
 }
 
-template <int R, int DIR>
-__device__ void sweep_lines(const Case2D& cs, int comp) {
+template <int R, int DIR, bool EARLY>
+__device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double tol, int final) {
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
   double* F = cs.f[comp ? F_V : F_U];
   const double* Q = cs.f[comp ? F_SV1 : F_SU1];
   const double* invd = cs.f[DIR == 0 ? G_INVD1 : G_INVD2];
   const double* na = cs.f[DIR == 0 ? G_NA1 : G_NA2];
   const double* bf = cs.f[DIR == 0 ? G_B1 : G_B2];
-  const double* cup = cs.f[DIR == 0 ? G_AN1 : G_AE2];
   const double* cdn = cs.f[DIR == 0 ? G_AN1 : G_AW2];
   const int n = DIR == 0 ? ni - 1 : nj - 1;       // rows per line
   const int nl = DIR == 0 ? nj - 1 : ni - 1;      // lines
   const long rs = DIR == 0 ? 1 : sj;              // row stride
   const long ls = DIR == 0 ? sj : 1;              // line stride
   const long dnoff = DIR == 0 ? -(long)sj : 0;    // lower coupling of row c at cdn[c + dnoff]
+  const int nch = sw_nchunk(nl);
+  const int a0 = 1 + ca * SW_C, a1 = min(a0 + SW_C - 1, nl);
+  double* const bnd = cs.swb;
+  const long bin = ((long)(slot * 2 + comp) * nch) * n, bout = ((long)((slot ^ 1) * 2 + comp) * nch) * n;
   const int l = threadIdx.x;
   const int r0 = l * R;
   double prev[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int r = r0 + q;
-    prev[q] = r < n ? F[rs * (r + 1)] : 0.0;  // line 0: the ghost line
+    // the line before the chunk: the ghost line 0, or the preceding chunk's last line
+    prev[q] = r < n ? (ca == 0 ? F[rs * (r + 1)] : bnd[bin + (long)(ca - 1) * n + r]) : 0.0;
   }
   LineOps<R> cur, nxt;
-  sweep_load<R, DIR>(cur, F, Q, invd, na, bf, cup, cdn, ls, rs, ls, dnoff, n, r0, l);
-  for (int L = 1; L <= nl; ++L) {
+  sweep_load<R, DIR, EARLY>(cur, F, Q, invd, na, bf, cdn, ls * a0, rs, dnoff, n, r0, l);
+  bool stopped = false;
+  double xrun = 0.0;
+  for (int L = a0; L <= a1; ++L) {
     const long base = ls * L;
-    if (L < nl) sweep_load<R, DIR>(nxt, F, Q, invd, na, bf, cup, cdn, base + ls, rs, ls, dnoff, n, r0, l);
+    if (L < a1)
+      sweep_load<R, DIR, EARLY>(nxt, F, Q, invd, na, bf, cdn, base + ls, rs, dnoff, n, r0, l);
     double al[R], be[R], bb[R];
     double At = 0.0, Bt = 1.0;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       if (r0 + q < n) {
-        const double C = cur.cupf[q] + cur.cdn[q] * prev[q] + cur.q[q];
+        const double C = cur.cdn[q] * prev[q] + cur.q[q];
         al[q] = C * cur.iv[q];
         be[q] = cur.na[q];
         bb[q] = cur.bf[q];
@@ -245,6 +268,7 @@ __device__ void sweep_lines(const Case2D& cs, int comp) {
     }
     // and from the exclusive suffix applied to the ghost after the last row
     double xin = wave::suffix_apply(Dt, Ct, wave::lane_value(cur.g1, 63));
+    double dl = 0.0;
 #pragma unroll
     for (int q = R - 1; q >= 0; --q) {
       const int r = r0 + q;
@@ -253,29 +277,92 @@ __device__ void sweep_lines(const Case2D& cs, int comp) {
         xin = x;
         prev[q] = x;
         F[base + rs * (r + 1)] = x;
+        if (EARLY) {
+          dl = fmax(dl, fabs(x - cur.xo[q]));
+          xrun = fmax(xrun, fabs(x));
+        }
       }
+    }
+    if (EARLY && L < a1) {
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        dl = fmax(dl, __shfl_xor(dl, o, 64));
+        xrun = fmax(xrun, __shfl_xor(xrun, o, 64));
+      }
+      if (dl <= tol * xrun) { stopped = true; break; }  // uniform over the wavefront
     }
     cur = nxt;
   }
+  double dm = 0.0, xm = 0.0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int r = r0 + q;
+    if (r < n) {
+      const double old = bnd[bin + (long)ca * n + r];
+      const double x = stopped ? old : prev[q];  // a stopped chunk's last line keeps its value
+      dm = fmax(dm, fabs(x - old));
+      xm = fmax(xm, fabs(x));
+      bnd[bout + (long)ca * n + r] = x;
+    }
+  }
+  if (final) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      dm = fmax(dm, __shfl_xor(dm, o, 64));
+      xm = fmax(xm, __shfl_xor(xm, o, 64));
+    }
+    if (l == 0 && ca + 1 < nch) {  // the last chunk's boundary line feeds no other chunk
+      const double rel = dm / fmax(xm, 1e-300);
+      atomicMax(reinterpret_cast<unsigned long long*>(cs.stat + ST_AFCH),
+                (unsigned long long)__double_as_longlong(rel));
+    }
+  }
 }
 
+// BCU ghosts of the stage input and the first pass's chunk boundary lines (the stage input's
+// last line of every chunk); one wavefront per (case, component).
 template <int DIR>
-__global__ void __launch_bounds__(64) k2_sweep(const Case2D* __restrict__ tab, int with_bcu) {
+__global__ void __launch_bounds__(64) k2_sweep_seed(const Case2D* __restrict__ tab) {
   const Case2D& cs = tab[blockIdx.x >> 1];
   const int comp = blockIdx.x & 1;
-  if (with_bcu)
-    bcu_ghosts_wave(cs.f[comp ? F_V : F_U], cs.ni, cs.nj, cs.sj, cs.b[comp ? B_VWALL : B_UWALL],
-                    cs.b[comp ? B_VFAR : B_UFAR]);
+  double* F = cs.f[comp ? F_V : F_U];
+  bcu_ghosts_wave(F, cs.ni, cs.nj, cs.sj, cs.b[comp ? B_VWALL : B_UWALL],
+                  cs.b[comp ? B_VFAR : B_UFAR]);
+  const int n = DIR == 0 ? cs.ni - 1 : cs.nj - 1;
+  const int nl = DIR == 0 ? cs.nj - 1 : cs.ni - 1;
+  const long rs = DIR == 0 ? 1 : cs.sj, ls = DIR == 0 ? cs.sj : 1;
+  const int nch = sw_nchunk(nl);
+  const long b0 = (long)comp * nch * n;  // slot 0
+  for (int t = threadIdx.x; t < nch * n; t += 64) {
+    const int ca = t / n, r = t - ca * n;
+    const int a1 = min((ca + 1) * SW_C, nl);
+    cs.swb[b0 + t] = F[ls * a1 + rs * (r + 1)];
+  }
+  if (DIR == 0 && comp == 0 && threadIdx.x == 0) cs.stat[ST_AFCH] = 0.0;
+}
+
+// One pass over the chunks of every case (block table: case, 2 chunk + component; entries with
+// case -1 pad the table to whole XCD rounds).
+template <int DIR, bool EARLY>
+__global__ void __launch_bounds__(64) k2_sweep_pass(const Case2D* __restrict__ tab,
+                                                    const int* __restrict__ bcase,
+                                                    const int* __restrict__ btile, int slot,
+                                                    double tol, int final) {
+  const int cc = bcase[blockIdx.x];
+  if (cc < 0) return;
+  const Case2D& cs = tab[cc];
+  const int t = btile[blockIdx.x];
+  const int comp = t & 1, ca = t >> 1;
   const int n = DIR == 0 ? cs.ni - 1 : cs.nj - 1;
   switch ((n + 63) >> 6) {  // the case's own rows per lane
-    case 1: sweep_lines<1, DIR>(cs, comp); break;
-    case 2: sweep_lines<2, DIR>(cs, comp); break;
-    case 3: sweep_lines<3, DIR>(cs, comp); break;
-    case 4: sweep_lines<4, DIR>(cs, comp); break;
-    case 5: sweep_lines<5, DIR>(cs, comp); break;
-    case 6: sweep_lines<6, DIR>(cs, comp); break;
-    case 7: sweep_lines<7, DIR>(cs, comp); break;
-    default: sweep_lines<8, DIR>(cs, comp); break;
+    case 1: sweep_chunk<1, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
+    case 2: sweep_chunk<2, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
+    case 3: sweep_chunk<3, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
+    case 4: sweep_chunk<4, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
+    case 5: sweep_chunk<5, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
+    case 6: sweep_chunk<6, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
+    case 7: sweep_chunk<7, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
+    default: sweep_chunk<8, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
   }
 }
 
@@ -302,6 +389,12 @@ __global__ void __launch_bounds__(256) k2_rhs2(const Case2D* __restrict__ tab,
   if (i == ni - 1) {
     ru = ru + 2.0 * cs.b[B_AE2F][j] * cs.b[B_UFAR][j];
     rv = rv + 2.0 * cs.b[B_AE2F][j] * cs.b[B_VFAR][j];
+  }
+  // the old next line's coupling AE U(I+1) (the far-field fold leaves AE(NI-1) = 0)
+  {
+    const double ae = cs.f[G_AE2][c];
+    ru = ru + ae * U[c + 1];
+    rv = rv + ae * V[c + 1];
   }
   cs.f[F_SU1][c] = ru;
   cs.f[F_SV1][c] = rv;
