@@ -27,10 +27,13 @@ enum F2 {
   G_INVD2, G_NA2, G_B2,  // stage-2 (J-line) Thomas factors
   G_NAT1, G_BT1, G_NAT2, G_BT2,  // transposed line factors of the two stages (adjoint)
   G_GA2, G_GC2,        // stage-2 line couplings AS, AN (the line ends' ghost terms)
+  G_TINVD2, G_TNA2, G_TB2, G_TAW2,  // stage-2 factors and AW in the J-line-major layout
+                                    // [J + (NJ+1) I] (J-line sweeps read rows contiguously)
   // state
   F_U, F_V, F_P, F_NLX, F_NLY,
   // step work arrays
   F_UN, F_VN, F_NLX1, F_NLY1, F_DIFX, F_DIFY, F_SU, F_SV, F_SU1, F_SV1, F_RHS,
+  F_TQU, F_TQV, F_TXU, F_TXV,  // J-line sweep: right-hand sides and iterates, J-line-major
   NF2
 };
 

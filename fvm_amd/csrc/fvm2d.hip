@@ -200,19 +200,21 @@ __device__ inline void sweep_load(LineOps<R>& o, const double* F, const double* 
   o.g1 = l == 63 ? F[base + rs * (n + 1)] : 0.0;
 }
 
+// DIR 1 runs on the J-line-major copies (rows contiguous): iterate F_TX*, right-hand side F_TQ*,
+// factors and AW G_T*, made by k2_tr_in and returned by k2_tr_out.
 template <int R, int DIR, bool EARLY>
 __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double tol, int final) {
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
-  double* F = cs.f[comp ? F_V : F_U];
-  const double* Q = cs.f[comp ? F_SV1 : F_SU1];
-  const double* invd = cs.f[DIR == 0 ? G_INVD1 : G_INVD2];
-  const double* na = cs.f[DIR == 0 ? G_NA1 : G_NA2];
-  const double* bf = cs.f[DIR == 0 ? G_B1 : G_B2];
-  const double* cdn = cs.f[DIR == 0 ? G_AN1 : G_AW2];
+  double* F = DIR == 0 ? cs.f[comp ? F_V : F_U] : cs.f[comp ? F_TXV : F_TXU];
+  const double* Q = DIR == 0 ? cs.f[comp ? F_SV1 : F_SU1] : cs.f[comp ? F_TQV : F_TQU];
+  const double* invd = cs.f[DIR == 0 ? G_INVD1 : G_TINVD2];
+  const double* na = cs.f[DIR == 0 ? G_NA1 : G_TNA2];
+  const double* bf = cs.f[DIR == 0 ? G_B1 : G_TB2];
+  const double* cdn = cs.f[DIR == 0 ? G_AN1 : G_TAW2];
   const int n = DIR == 0 ? ni - 1 : nj - 1;       // rows per line
   const int nl = DIR == 0 ? nj - 1 : ni - 1;      // lines
-  const long rs = DIR == 0 ? 1 : sj;              // row stride
-  const long ls = DIR == 0 ? sj : 1;              // line stride
+  const long rs = 1;                              // row stride
+  const long ls = DIR == 0 ? sj : nj + 1;         // line stride
   const long dnoff = DIR == 0 ? -(long)sj : 0;    // lower coupling of row c at cdn[c + dnoff]
   const int nch = sw_nchunk(nl);
   const int a0 = 1 + ca * SW_C, a1 = min(a0 + SW_C - 1, nl);
@@ -315,6 +317,69 @@ __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double
       const double rel = dm / fmax(xm, 1e-300);
       atomicMax(reinterpret_cast<unsigned long long*>(cs.stat + ST_AFCH),
                 (unsigned long long)__double_as_longlong(rel));
+    }
+  }
+}
+
+// J-line-major copies for the J-line sweep: the right-hand sides and the stage input (ghosts
+// included, after BCU) of both components, 32 x 32 tiles through LDS; block table (case, tile).
+__global__ void __launch_bounds__(256) k2_tr_in(const Case2D* __restrict__ tab,
+                                                const int* __restrict__ bcase,
+                                                const int* __restrict__ btile) {
+  __shared__ double tile[4][32][33];
+  const Case2D& cs = tab[bcase[blockIdx.x]];
+  const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
+  const int tx = (ni + 1 + 31) / 32, t = btile[blockIdx.x];
+  const int i0 = (t % tx) * 32, j0 = (t / tx) * 32;
+  const double* src[4] = {cs.f[F_SU1], cs.f[F_SV1], cs.f[F_U], cs.f[F_V]};
+  double* dst[4] = {cs.f[F_TQU], cs.f[F_TQV], cs.f[F_TXU], cs.f[F_TXV]};
+  const int l = threadIdx.x;
+  for (int r = 0; r < 4; ++r) {
+    const int li = l % 32, lj = l / 32 + 8 * r;
+    const int i = i0 + li, j = j0 + lj;
+    if (i <= ni && j <= nj) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) tile[a][lj][li] = src[a][i + (long)sj * j];
+    }
+  }
+  __syncthreads();
+  for (int r = 0; r < 4; ++r) {
+    const int lj = l % 32, li = l / 32 + 8 * r;
+    const int i = i0 + li, j = j0 + lj;
+    if (i <= ni && j <= nj) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) dst[a][j + (long)(nj + 1) * i] = tile[a][lj][li];
+    }
+  }
+}
+
+// The J-line sweep's result back into the field interior.
+__global__ void __launch_bounds__(256) k2_tr_out(const Case2D* __restrict__ tab,
+                                                 const int* __restrict__ bcase,
+                                                 const int* __restrict__ btile) {
+  __shared__ double tile[2][32][33];
+  const Case2D& cs = tab[bcase[blockIdx.x]];
+  const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
+  const int tx = (ni + 1 + 31) / 32, t = btile[blockIdx.x];
+  const int i0 = (t % tx) * 32, j0 = (t / tx) * 32;
+  const double* src[2] = {cs.f[F_TXU], cs.f[F_TXV]};
+  double* dst[2] = {cs.f[F_U], cs.f[F_V]};
+  const int l = threadIdx.x;
+  for (int r = 0; r < 4; ++r) {
+    const int lj = l % 32, li = l / 32 + 8 * r;
+    const int i = i0 + li, j = j0 + lj;
+    if (i <= ni && j <= nj) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a) tile[a][lj][li] = src[a][j + (long)(nj + 1) * i];
+    }
+  }
+  __syncthreads();
+  for (int r = 0; r < 4; ++r) {
+    const int li = l % 32, lj = l / 32 + 8 * r;
+    const int i = i0 + li, j = j0 + lj;
+    if (i >= 1 && i <= ni - 1 && j >= 1 && j <= nj - 1) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a) dst[a][i + (long)sj * j] = tile[a][lj][li];
     }
   }
 }
