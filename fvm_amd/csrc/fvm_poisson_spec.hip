@@ -27,68 +27,100 @@ namespace {
 
 const dim3 kBlk(64, 4, 1);
 
-template <typename T>
-__device__ inline double cross_s(const Dims& d, long c, const double* __restrict__ f,
-                                 const double* __restrict__ jq12, const double* __restrict__ jq13,
-                                 const double* __restrict__ jq23) {
-  const long sj = d.sj, sk = d.sk;
-  const long e = c + 1, w = c - 1, n = c + sj, s = c - sj, t = c + sk, b = c - sk;
-  double r = jq12[e] * (f[e + sj] - f[e - sj]);
-  r = r - jq12[w] * (f[w + sj] - f[w - sj]);
-  r = r + jq13[e] * (f[e + sk] - f[e - sk]);
-  r = r - jq13[w] * (f[w + sk] - f[w - sk]);
-  r = r + jq12[n] * (f[n + 1] - f[n - 1]);
-  r = r - jq12[s] * (f[s + 1] - f[s - 1]);
-  r = r + jq23[n] * (f[n + sk] - f[n - sk]);
-  r = r - jq23[s] * (f[s + sk] - f[s - sk]);
-  r = r + jq13[t] * (f[t + 1] - f[t - 1]);
-  r = r - jq13[b] * (f[b + 1] - f[b - 1]);
-  r = r + jq23[t] * (f[t + sj] - f[t - sj]);
-  r = r - jq23[b] * (f[b + sj] - f[b - sj]);
-  return 0.25 * r;
-}
-
 // residual r = sum A_nb p_nb + (rhs - DD(p)) - AP p of every interior cell (iter_p.f90:50-53
-// with SU = rhs - DD), written to the compact interior array R (I fastest), plus per-block sums
-// of |r| and |AP p|
+// with SU = rhs - DD, DD the twelve cross terms of p weighted by J^-1 Q12, Q13, Q23), written to
+// the compact interior array R (I fastest), plus per-workgroup sums of |r| and |AP p|.
+// K-marching: a workgroup owns a 64 x 4 (I x J) column over RS_KC planes and
+// keeps three consecutive p planes (one-cell halo) in an LDS ring, so every p value is read from
+// memory once per column instead of once per plane that needs it; the own-column values of the
+// planes below and above (J^-1 Q13, J^-1 Q23 at K +- 1, D_T at K - 1) ride along in registers.
+// One pair of partial sums per workgroup.
+constexpr int RS_KC = 16;
+
 template <typename T>
-__global__ void k_residual_compact(Dims d, const double* __restrict__ p,
-                                   const double* __restrict__ rhs,
-                                   const double* __restrict__ pdI, const double* __restrict__ pdJ,
-                                   const double* __restrict__ pdK, const double* __restrict__ jq12,
-                                   const double* __restrict__ jq13, const double* __restrict__ jq23,
-                                   T* __restrict__ R, double* part) {
+__global__ void __launch_bounds__(256) k_residual_km(Dims d, const double* __restrict__ p,
+                                                     const double* __restrict__ rhs,
+                                                     const double* __restrict__ pdI,
+                                                     const double* __restrict__ pdJ,
+                                                     const double* __restrict__ pdK,
+                                                     const double* __restrict__ jq12,
+                                                     const double* __restrict__ jq13,
+                                                     const double* __restrict__ jq23,
+                                                     T* __restrict__ R, double* part) {
   const uint3 bb = xcd_block();
+  __shared__ double sh[3][SF_P];
   __shared__ double s0[256], s1[256];
-  const int i = 1 + bb.x * 64 + threadIdx.x;
-  const int j = 1 + bb.y * 4 + threadIdx.y;
-  const int k = 1 + bb.z;
-  const int t = threadIdx.x + 64 * threadIdx.y;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+  const int i = 1 + bb.x * 64 + tx;
+  const int j = 1 + bb.y * 4 + ty;
+  const int k0 = 1 + bb.z * RS_KC;
+  const int k1 = min(k0 + RS_KC - 1, d.nk - 1);
+  const bool act = i <= d.ni - 1 && j <= d.nj - 1;
+  const int q0 = (ty + 1) * SF_W + tx + 1;
+  const long sj = d.sj;
+  // own-column values of planes kp-2, kp-1, kp
+  double z13[3] = {0.0, 0.0, 0.0}, z23[3] = {0.0, 0.0, 0.0}, zk[3] = {0.0, 0.0, 0.0};
   double ra = 0.0, rb = 0.0;
-  if (i <= d.ni - 1 && j <= d.nj - 1) {
+  for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
+    const int slot = (kp - (k0 - 1)) % 3;
+    __syncthreads();  // plane kp - 3, held in this slot, is no longer read
+    for (int q = tid; q < SF_P; q += 256) {
+      const int gi = min(bb.x * 64 + q % SF_W, d.ni);
+      const int gj = min(bb.y * 4 + q / SF_W, d.nj);
+      sh[slot][q] = p[cidx(d, gi, gj, kp)];
+    }
+    z13[0] = z13[1]; z13[1] = z13[2];
+    z23[0] = z23[1]; z23[1] = z23[2];
+    zk[0] = zk[1]; zk[1] = zk[2];
+    if (act) {
+      const long cp = cidx(d, i, j, kp);
+      z13[2] = jq13[cp];
+      z23[2] = jq23[cp];
+      zk[2] = pdK[cp];
+    }
+    __syncthreads();
+    if (kp < k0 + 1 || !act) continue;
+    const int k = kp - 1;
+    const int pl[3] = {(slot + 1) % 3, (slot + 2) % 3, slot};
+#define P(dk, dj, di) sh[pl[(dk) + 1]][q0 + (dj) * SF_W + (di)]
     const long c = cidx(d, i, j, k);
-    const double de = pdI[c], dw = pdI[c - 1], dn = pdJ[c], ds = pdJ[c - d.sj], dt = pdK[c],
-                 db = pdK[c - d.sk];
+    const double de = pdI[c], dw = pdI[c - 1], dn = pdJ[c], ds = pdJ[c - sj], dt = zk[1],
+                 db = zk[0];
     double ap = -de - dw - dn - ds - dt - db;
     double ae = -de, aw = -dw, an = -dn, as = -ds, at = -dt, ab = -db;
     if (i == 1) { ap = ap - aw; aw = 0.0; }
     if (i == d.ni - 1) ae = 0.0;
-    double su = rhs[c] - cross_s<double>(d, c, p, jq12, jq13, jq23);
-    double r = ae * p[c + 1] + aw * p[c - 1] + an * p[c + d.sj] + as * p[c - d.sj] +
-               at * p[c + d.sk] + ab * p[c - d.sk] + su - ap * p[c];
-    ra = fabs(r);
-    rb = fabs(ap * p[c]);
+    // the cross terms DD
+    double x = jq12[c + 1] * (P(0, 1, 1) - P(0, -1, 1));
+    x = x - jq12[c - 1] * (P(0, 1, -1) - P(0, -1, -1));
+    x = x + jq13[c + 1] * (P(1, 0, 1) - P(-1, 0, 1));
+    x = x - jq13[c - 1] * (P(1, 0, -1) - P(-1, 0, -1));
+    x = x + jq12[c + sj] * (P(0, 1, 1) - P(0, 1, -1));
+    x = x - jq12[c - sj] * (P(0, -1, 1) - P(0, -1, -1));
+    x = x + jq23[c + sj] * (P(1, 1, 0) - P(-1, 1, 0));
+    x = x - jq23[c - sj] * (P(1, -1, 0) - P(-1, -1, 0));
+    x = x + z13[2] * (P(1, 0, 1) - P(1, 0, -1));
+    x = x - z13[0] * (P(-1, 0, 1) - P(-1, 0, -1));
+    x = x + z23[2] * (P(1, 1, 0) - P(1, -1, 0));
+    x = x - z23[0] * (P(-1, 1, 0) - P(-1, -1, 0));
+    const double su = rhs[c] - 0.25 * x;
+    const double pc = P(0, 0, 0);
+    const double r = ae * P(0, 0, 1) + aw * P(0, 0, -1) + an * P(0, 1, 0) + as * P(0, -1, 0) +
+                     at * P(1, 0, 0) + ab * P(-1, 0, 0) + su - ap * pc;
+#undef P
+    ra = ra + fabs(r);
+    rb = rb + fabs(ap * pc);
     const long ci = (long)(i - 1) + (long)(d.ni - 1) * ((j - 1) + (long)(d.nj - 1) * (k - 1));
     R[ci] = (T)r;
   }
-  s0[t] = ra; s1[t] = rb;
+  s0[tid] = ra; s1[tid] = rb;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) { s0[t] += s0[t + s]; s1[t] += s1[t + s]; }
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) { s0[tid] += s0[tid + st]; s1[tid] += s1[tid + st]; }
     __syncthreads();
   }
-  if (t == 0) {
-    long bid = bb.x + (long)gridDim.x * (bb.y + (long)gridDim.y * bb.z);
+  if (tid == 0) {
+    const long bid = bb.x + (long)gridDim.x * (bb.y + (long)gridDim.y * bb.z);
     part[2 * bid] = s0[0];
     part[2 * bid + 1] = s1[0];
   }
@@ -569,7 +601,7 @@ int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
     return FVM_ERR_ARG;
   }
   const Dims d = c->d;
-  dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
+  dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + RS_KC - 1) / RS_KC);
   long nb = (long)g.x * g.y * g.z;
   int outer = 0;
   double res = INFINITY, res_prev = INFINITY;
@@ -583,10 +615,10 @@ int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
       ProfScope ps(c, "p_residual");
       ProfScope pk(c, "k_residual");
       if (use32)
-        hipLaunchKernelGGL(k_residual_compact<float>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
+        hipLaunchKernelGGL(k_residual_km<float>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
                            c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d32.R, c->red);
       else
-        hipLaunchKernelGGL(k_residual_compact<double>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
+        hipLaunchKernelGGL(k_residual_km<double>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
                            c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d64.R, c->red);
       mb = (outer > 0 || cap < 1) ? fvm_mail(c) : Mailbox{nullptr, nullptr, 0};
       hipLaunchKernelGGL(k_res_final, dim3(1), dim3(256), 0, c->stream, c->red, nb, c->scal + 4,
@@ -608,7 +640,7 @@ int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
     if (use32 && outer > 0 && !(res < 0.5 * res_prev)) {
       // the FP32 approximate inverse stopped contracting: finish with the FP64 inverse
       use32 = false;
-      hipLaunchKernelGGL(k_residual_compact<double>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
+      hipLaunchKernelGGL(k_residual_km<double>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
                          c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d64.R, c->red);
       HIP_CHECK(hipGetLastError());
       ++c->sep_fallbacks;
