@@ -267,6 +267,10 @@ WORKLOADS = {
     # the same LES on 384 x 512 x 128 cells
     "les3d_re3900_25m": dict(ni=385, nj=513, nk=129, h1=0.001, r_out=25.0, re=3900.0,
                              dt=0.002, sigma=0.05),
+    # the LES on a coarse 96 x 128 x 16 cell grid (wall spacing 0.002 D): long shedding runs
+    # (stable over tens of thousands of steps at dt = 0.001)
+    "les3d_re3900_coarse": dict(ni=97, nj=129, nk=17, h1=0.002, r_out=25.0, re=3900.0,
+                                dt=0.001, sigma=0.05),
     # differentiable 2D case, Re=150: 97 x 193 nodes
     "cyl2d_re150_adj": dict(ni=97, nj=193, nk=1, h1=0.002, r_out=30.0, re=150.0, dt=0.005,
                             sigma=1e-3),

@@ -1,19 +1,20 @@
-"""Mixed-precision form against the FP64 form on the same seeded start.
+"""Mixed-precision forms against the FP64 form over a shedding window.
 
-python -m fvm_amd.precision_report [--workload les3d_re3900_25m] [--steps 500]
+python -m fvm_amd.precision_report [--workload les3d_re3900] [--spinup 10000] [--steps 10000]
 python -m fvm_amd.precision_report --state tests/golden/r2d_65x129_re100_shedding.npz --steps 5600
 
-The second form runs a 2D case from a developed, shedding state (the trajectory fixture's
-inputs) on the batched 2D solver, FP64 Poisson correction against the FP32 one inside the FP64
-defect correction, both to the same RESIMAX, and reports the same differences plus the C_L
-amplitude and the Strouhal number from the C_L zero crossings of the window.
+3D: the LES is started from its seeded field (20 steps at dt/10) and spun up in the FP64 form
+for --spinup steps at dt; from that developed state each precision form runs --steps steps --
+fp64 (FP64 separable Poisson inverse, FP64 AF sweeps), mixed-poisson (the separable inverse in
+FP32 inside the FP64 defect correction) and mixed (that, plus the I and J AF stages solved for
+their change in FP32 against the FP64 residual), all to the same scaled-L1 Poisson residual --
+and one JSON line reports the end-of-window relative differences of u, v, w, p against fp64
+(max-norm over the field maximum), the largest C_L difference over the window and the window's
+mean C_D, rms C_L, C_L amplitude and Strouhal number (C_L zero crossings) of every form.
 
-Runs the LES twice from the same seeded start -- the FP64 Poisson inverse (poisson_mode 2) and
-the FP32 inverse inside FP64 defect correction (poisson_mode 3), both to the same scaled-L1
-residual -- through a start-up of 20 steps at dt/10 and then --steps steps at dt, and prints one
-JSON line with the relative differences of u, v, w, p (max-norm over the field maximum) and of
-the force-coefficient statistics (mean C_D, rms C_L, Strouhal number from the C_L extrema as in
-the 2D reference's post-processing).
+2D (--state): a 2D case from a developed, shedding state (the trajectory fixture's inputs) on the
+batched 2D solver, FP64 Poisson correction against the FP32 one inside the FP64 defect
+correction, both to the same RESIMAX, with the same differences and statistics.
 """
 from __future__ import annotations
 
@@ -27,38 +28,84 @@ import time
 import numpy as np
 
 
-def run(w, mode, steps, resimax, startup, log):
+FORMS3D = {"fp64": (2, 0), "mixed-poisson": (3, 0), "mixed": (3, 2)}
+
+
+def solver3d(w, dt, poisson_mode, af_mode, resimax):
     from fvm_amd.grid import make_case
-    from fvm_amd.solver import Post2D, Solver3D
+    from fvm_amd.solver import Solver3D
     (xg, yg, zg), ic = make_case(**w)
-    s = Solver3D(w["ni"], w["nj"], w["nk"], w["re"], w["dt"] * 0.1, xg, yg, zg, ref_a=math.pi,
-                 use_les=True, nswpp=50, resimax=resimax, poisson_mode=mode, check_every=8)
-    s.set_state(*(ic[k] for k in ("u", "v", "w", "p", "nlx", "nly", "nlz")))
+    s = Solver3D(w["ni"], w["nj"], w["nk"], w["re"], dt, xg, yg, zg, ref_a=math.pi,
+                 use_les=True, nswpp=50, resimax=resimax, poisson_mode=poisson_mode,
+                 check_every=8, af_mode=af_mode)
+    return s, ic
+
+
+STATE3D = ("u", "v", "w", "p", "nlx", "nly", "nlz")
+
+
+def spinup3d(w, steps, resimax, startup, log):
+    s, ic = solver3d(w, w["dt"] * 0.1, 2, 0, resimax)
+    s.set_state(*(ic[k] for k in STATE3D))
     for _ in range(startup):
         s.step()
     s.set_dtime(w["dt"])
-    post = Post2D()
-    cds, cls = [], []
-    t = 0.0
-    t0 = time.perf_counter()
-    out = None
     for q in range(steps):
         info = s.step()
-        t += w["dt"]
+        if (q + 1) % 1000 == 0:
+            log(f"spin-up: step {q + 1}/{steps} cd {info['cd']:.5f} cl {info['cl']:.3e} "
+                f"cfl {info['cfl']:.3f} resip {info['resip']:.2e}")
+    state = {k: s.get(k) for k in STATE3D}
+    s.close()
+    return state
+
+
+def window3d(w, state, form, steps, resimax, log):
+    pm, am = FORMS3D[form]
+    s, _ = solver3d(w, w["dt"], pm, am, resimax)
+    s.set_state(*(state[k] for k in STATE3D))
+    cds, cls, res = [], [], []
+    t0 = time.perf_counter()
+    for q in range(steps):
+        info = s.step()
         cds.append(info["cd"])
         cls.append(info["cl"])
-        out = post.update(t, info["cd"], info["cl"], info["cpb"])
-        if (q + 1) % 25 == 0:
-            log(f"mode {mode}: step {q + 1}/{steps} cd {info['cd']:.5f} cl {info['cl']:.3e} "
+        res.append(info["resip"])
+        if (q + 1) % 1000 == 0:
+            log(f"{form}: step {q + 1}/{steps} cd {info['cd']:.5f} cl {info['cl']:.3e} "
                 f"cfl {info['cfl']:.3f} resip {info['resip']:.2e} passes {info['niter']}")
     s.sync()
     secs = time.perf_counter() - t0
     fields = {c: s.get(c) for c in ("u", "v", "w", "p")}
-    cl = np.array(cls)
-    stats = {"cd_mean": float(np.mean(cds)), "cl_rms": float(np.sqrt(np.mean((cl - cl.mean()) ** 2))),
-             "st": out["st"], "secs": secs}
     s.close()
-    return fields, stats
+    cd, cl = np.array(cds), np.array(cls)
+    t = w["dt"] * np.arange(1, steps + 1)
+    stats = {"cd_mean": float(cd.mean()), "cl_rms": float(np.sqrt(np.mean((cl - cl.mean()) ** 2))),
+             "cl_amp": float(0.5 * (cl.max() - cl.min())), "st": crossings_st(t, cl),
+             "resip_max": float(max(res)), "secs": secs}
+    log(f"{form}: {stats}")
+    return fields, stats, cl
+
+
+def main3d(a, log):
+    from fvm_amd.grid import WORKLOADS
+    w = dict(WORKLOADS[a.workload])
+    if a.dt > 0:
+        w["dt"] = a.dt
+    state = spinup3d(w, a.spinup, a.resimax, a.startup, log)
+    res = {f: window3d(w, state, f, a.steps, a.resimax, log) for f in FORMS3D}
+    f64, s64, cl64 = res["fp64"]
+    out = {"workload": a.workload, "dt": w["dt"], "spinup": a.spinup, "steps": a.steps,
+           "resimax": a.resimax, "fp64": s64}
+    for f in ("mixed-poisson", "mixed"):
+        fx, sx, clx = res[f]
+        rd = lambda k: abs(sx[k] - s64[k]) / abs(s64[k])
+        out[f] = dict(sx, field_rel_diff_end={c: float(np.abs(fx[c] - f64[c]).max() /
+                                                       np.abs(f64[c]).max()) for c in f64},
+                      cl_max_abs_diff=float(np.abs(clx - cl64).max()),
+                      cd_mean_rel_diff=rd("cd_mean"), cl_rms_rel_diff=rd("cl_rms"),
+                      cl_amp_rel_diff=rd("cl_amp"), st_rel_diff=rd("st"))
+    print(json.dumps(out), flush=True)
 
 
 def crossings_st(t, cl):
@@ -115,8 +162,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--state", default="",
                     help="2D: an .npz of a developed state (the trajectory fixture)")
-    ap.add_argument("--workload", default="les3d_re3900_25m")
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--workload", default="les3d_re3900")
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--spinup", type=int, default=10000, help="3D: FP64 steps before the window")
     ap.add_argument("--resimax", type=float, default=1e-8)
     ap.add_argument("--startup", type=int, default=20)
     ap.add_argument("--dt", type=float, default=0.0, help="override the workload's dt")
@@ -125,20 +173,7 @@ def main():
     if a.state:
         main2d(a, log)
         return
-    from fvm_amd.grid import WORKLOADS
-    w = dict(WORKLOADS[a.workload])
-    if a.dt > 0:
-        w["dt"] = a.dt
-    f64, s64 = run(w, 2, a.steps, a.resimax, a.startup, log)
-    f32, s32 = run(w, 3, a.steps, a.resimax, a.startup, log)
-    rel = {c: float(np.abs(f32[c] - f64[c]).max() / np.abs(f64[c]).max()) for c in f64}
-    out = {"workload": a.workload, "dt": w["dt"], "steps": a.steps, "resimax": a.resimax,
-           "field_rel_diff": rel,
-           "fp64": s64, "mixed": s32,
-           "cd_mean_rel_diff": abs(s32["cd_mean"] - s64["cd_mean"]) / abs(s64["cd_mean"]),
-           "cl_rms_rel_diff": abs(s32["cl_rms"] - s64["cl_rms"]) / max(abs(s64["cl_rms"]), 1e-300),
-           "st_diff": s32["st"] - s64["st"]}
-    print(json.dumps(out), flush=True)
+    main3d(a, log)
 
 
 if __name__ == "__main__":
