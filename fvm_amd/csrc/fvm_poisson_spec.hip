@@ -46,39 +46,51 @@ __global__ void __launch_bounds__(256) k_residual_km(Dims d, const double* __res
                                                      const double* __restrict__ jq12,
                                                      const double* __restrict__ jq13,
                                                      const double* __restrict__ jq23,
-                                                     T* __restrict__ R, double* part) {
+                                                     T* __restrict__ R, double* part, int kc) {
   const uint3 bb = xcd_block();
   __shared__ double sh[3][SF_P];
   __shared__ double s0[256], s1[256];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
   const int i = 1 + bb.x * 64 + tx;
   const int j = 1 + bb.y * 4 + ty;
-  const int k0 = 1 + bb.z * RS_KC;
-  const int k1 = min(k0 + RS_KC - 1, d.nk - 1);
+  const int k0 = 1 + bb.z * kc;
+  const int k1 = min(k0 + kc - 1, d.nk - 1);
   const bool act = i <= d.ni - 1 && j <= d.nj - 1;
   const int q0 = (ty + 1) * SF_W + tx + 1;
   const long sj = d.sj;
   // own-column values of planes kp-2, kp-1, kp
   double z13[3] = {0.0, 0.0, 0.0}, z23[3] = {0.0, 0.0, 0.0}, zk[3] = {0.0, 0.0, 0.0};
   double ra = 0.0, rb = 0.0;
+  // the next plane's tile values and own-column values, loaded one plane ahead
+  constexpr int NQ = (SF_P + 255) / 256;
+  double pv[NQ], n13 = 0.0, n23 = 0.0, nk = 0.0;
+  auto load = [&](int kp) {
+#pragma unroll
+    for (int h = 0; h < NQ; ++h) {
+      const int q = min(tid + 256 * h, SF_P - 1);
+      const int gi = min(bb.x * 64 + q % SF_W, d.ni);
+      const int gj = min(bb.y * 4 + q / SF_W, d.nj);
+      pv[h] = p[cidx(d, gi, gj, kp)];
+    }
+    const long cp = cidx(d, act ? i : 1, act ? j : 1, kp);
+    n13 = jq13[cp];
+    n23 = jq23[cp];
+    nk = pdK[cp];
+  };
+  load(k0 - 1);
   for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
     const int slot = (kp - (k0 - 1)) % 3;
     __syncthreads();  // plane kp - 3, held in this slot, is no longer read
-    for (int q = tid; q < SF_P; q += 256) {
-      const int gi = min(bb.x * 64 + q % SF_W, d.ni);
-      const int gj = min(bb.y * 4 + q / SF_W, d.nj);
-      sh[slot][q] = p[cidx(d, gi, gj, kp)];
+#pragma unroll
+    for (int h = 0; h < NQ; ++h) {
+      const int q = tid + 256 * h;
+      if (q < SF_P) sh[slot][q] = pv[h];
     }
-    z13[0] = z13[1]; z13[1] = z13[2];
-    z23[0] = z23[1]; z23[1] = z23[2];
-    zk[0] = zk[1]; zk[1] = zk[2];
-    if (act) {
-      const long cp = cidx(d, i, j, kp);
-      z13[2] = jq13[cp];
-      z23[2] = jq23[cp];
-      zk[2] = pdK[cp];
-    }
+    z13[0] = z13[1]; z13[1] = z13[2]; z13[2] = n13;
+    z23[0] = z23[1]; z23[1] = z23[2]; z23[2] = n23;
+    zk[0] = zk[1]; zk[1] = zk[2]; zk[2] = nk;
     __syncthreads();
+    if (kp + 1 <= k1 + 1) load(kp + 1);  // in flight while this plane's cell is formed
     if (kp < k0 + 1 || !act) continue;
     const int k = kp - 1;
     const int pl[3] = {(slot + 1) % 3, (slot + 2) % 3, slot};
@@ -601,7 +613,8 @@ int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
     return FVM_ERR_ARG;
   }
   const Dims d = c->d;
-  dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + RS_KC - 1) / RS_KC);
+  const int kc = RS_KC;
+  dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + kc - 1) / kc);
   long nb = (long)g.x * g.y * g.z;
   int outer = 0;
   double res = INFINITY, res_prev = INFINITY;
@@ -616,10 +629,10 @@ int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
       ProfScope pk(c, "k_residual");
       if (use32)
         hipLaunchKernelGGL(k_residual_km<float>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
-                           c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d32.R, c->red);
+                           c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d32.R, c->red, kc);
       else
         hipLaunchKernelGGL(k_residual_km<double>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
-                           c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d64.R, c->red);
+                           c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d64.R, c->red, kc);
       mb = (outer > 0 || cap < 1) ? fvm_mail(c) : Mailbox{nullptr, nullptr, 0};
       hipLaunchKernelGGL(k_res_final, dim3(1), dim3(256), 0, c->stream, c->red, nb, c->scal + 4,
                          mb);
@@ -641,7 +654,7 @@ int fvm_k_iter_p_sep(FvmCtx* c, bool mixed, double* resip, int* niter) {
       // the FP32 approximate inverse stopped contracting: finish with the FP64 inverse
       use32 = false;
       hipLaunchKernelGGL(k_residual_km<double>, g, kBlk, 0, c->stream, d, c->p, c->rhs,
-                         c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d64.R, c->red);
+                         c->pdI, c->pdJ, c->pdK, c->jq12, c->jq13, c->jq23, S->d64.R, c->red, kc);
       HIP_CHECK(hipGetLastError());
       ++c->sep_fallbacks;
     }
