@@ -162,26 +162,63 @@ __global__ void __launch_bounds__(256) k_af_rhs_km(Dims d, AFArgs A,
   const GP fE = gp(fa[0]), fN = gp(fa[1]), fT = gp(fa[2]);
   // own-column values of planes kp-2, kp-1, kp: (J Z) X, (J Z) E and AT
   double zx[3] = {0.0, 0.0, 0.0}, ze[3] = {0.0, 0.0, 0.0}, at[3] = {0.0, 0.0, 0.0};
-  for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
-    const int slot = (kp - (k0 - 1)) % 3;
-    __syncthreads();  // plane kp - 3, held in this slot, is no longer read
-    for (int q = tid; q < SF_P; q += 256) {
+  // the next plane's tile values and own-column values, loaded one plane ahead (in flight
+  // while the current plane's cells are formed) in stage 1; stage 0's cells hold many more
+  // operands, and there the registers are worth more than the overlap
+  constexpr bool PF = S == 1;
+  constexpr int NQ = (SF_P + 255) / 256;
+  double tv[NQ][NC], nzx = 0.0, nze = 0.0, nat = 0.0;
+  auto load = [&](int kp) {
+#pragma unroll
+    for (int h = 0; h < NQ; ++h) {
+      const int q = min(tid + 256 * h, SF_P - 1);
       const int gi = min(bb.x * 64 + q % SF_W, d.ni);
       const int gj = min(bb.y * 4 + q / SF_W, d.nj);
       const long c = cidx(d, gi, gj, kp);
 #pragma unroll
-      for (int comp = 0; comp < NC; ++comp) sh[comp][slot][q] = gp(A.f[comp])[c];
+      for (int comp = 0; comp < NC; ++comp) tv[h][comp] = gp(A.f[comp])[c];
     }
-    zx[0] = zx[1]; zx[1] = zx[2];
-    ze[0] = ze[1]; ze[1] = ze[2];
-    at[0] = at[1]; at[1] = at[2];
-    if (act) {
-      const long cp = cidx(d, i, j, kp);
-      zx[2] = PZX[cp];
-      ze[2] = PZE[cp];
-      at[2] = (S < 2 && qform) ? fT[cp] : 0.0;
+    const long cp = cidx(d, act ? i : 1, act ? j : 1, kp);
+    nzx = PZX[cp];
+    nze = PZE[cp];
+    nat = (S < 2 && qform) ? fT[cp] : 0.0;
+  };
+  if (PF) load(k0 - 1);
+  for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
+    const int slot = (kp - (k0 - 1)) % 3;
+    __syncthreads();  // plane kp - 3, held in this slot, is no longer read
+    if constexpr (PF) {
+#pragma unroll
+      for (int h = 0; h < NQ; ++h) {
+        const int q = tid + 256 * h;
+        if (q < SF_P) {
+#pragma unroll
+          for (int comp = 0; comp < NC; ++comp) sh[comp][slot][q] = tv[h][comp];
+        }
+      }
+      zx[0] = zx[1]; zx[1] = zx[2]; zx[2] = nzx;
+      ze[0] = ze[1]; ze[1] = ze[2]; ze[2] = nze;
+      at[0] = at[1]; at[1] = at[2]; at[2] = nat;
+    } else {
+      for (int q = tid; q < SF_P; q += 256) {
+        const int gi = min(bb.x * 64 + q % SF_W, d.ni);
+        const int gj = min(bb.y * 4 + q / SF_W, d.nj);
+        const long c = cidx(d, gi, gj, kp);
+#pragma unroll
+        for (int comp = 0; comp < NC; ++comp) sh[comp][slot][q] = gp(A.f[comp])[c];
+      }
+      zx[0] = zx[1]; zx[1] = zx[2];
+      ze[0] = ze[1]; ze[1] = ze[2];
+      at[0] = at[1]; at[1] = at[2];
+      if (act) {
+        const long cp = cidx(d, i, j, kp);
+        zx[2] = PZX[cp];
+        ze[2] = PZE[cp];
+        at[2] = (S < 2 && qform) ? fT[cp] : 0.0;
+      }
     }
     __syncthreads();
+    if (PF && kp + 1 <= k1 + 1) load(kp + 1);
     if (kp < k0 + 1 || !act) continue;
     // cell (i, j, kp - 1): planes k-1, k, k+1 sit in slots slot+1, slot+2, slot (mod 3)
     const int k = kp - 1;
