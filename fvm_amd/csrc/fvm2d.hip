@@ -384,26 +384,30 @@ __global__ void __launch_bounds__(256) k2_tr_out(const Case2D* __restrict__ tab,
   }
 }
 
-// BCU ghosts of the stage input and the first pass's chunk boundary lines (the stage input's
-// last line of every chunk); one wavefront per (case, component).
+// The first pass's chunk boundary lines (the stage input's last line of every chunk) and BCU's
+// ghosts of the stage input (by each (case, component)'s chunk-0 wavefront; the ghosts and the
+// interior lines copied here do not overlap); block table as the passes'.
 template <int DIR>
-__global__ void __launch_bounds__(64) k2_sweep_seed(const Case2D* __restrict__ tab) {
-  const Case2D& cs = tab[blockIdx.x >> 1];
-  const int comp = blockIdx.x & 1;
+__global__ void __launch_bounds__(64) k2_sweep_seed(const Case2D* __restrict__ tab,
+                                                    const int* __restrict__ bcase,
+                                                    const int* __restrict__ btile) {
+  const int cc = bcase[blockIdx.x];
+  if (cc < 0) return;
+  const Case2D& cs = tab[cc];
+  const int t = btile[blockIdx.x];
+  const int comp = t & 1, ca = t >> 1;
   double* F = cs.f[comp ? F_V : F_U];
-  bcu_ghosts_wave(F, cs.ni, cs.nj, cs.sj, cs.b[comp ? B_VWALL : B_UWALL],
-                  cs.b[comp ? B_VFAR : B_UFAR]);
+  if (ca == 0)
+    bcu_ghosts_wave(F, cs.ni, cs.nj, cs.sj, cs.b[comp ? B_VWALL : B_UWALL],
+                    cs.b[comp ? B_VFAR : B_UFAR]);
   const int n = DIR == 0 ? cs.ni - 1 : cs.nj - 1;
   const int nl = DIR == 0 ? cs.nj - 1 : cs.ni - 1;
   const long rs = DIR == 0 ? 1 : cs.sj, ls = DIR == 0 ? cs.sj : 1;
   const int nch = sw_nchunk(nl);
-  const long b0 = (long)comp * nch * n;  // slot 0
-  for (int t = threadIdx.x; t < nch * n; t += 64) {
-    const int ca = t / n, r = t - ca * n;
-    const int a1 = min((ca + 1) * SW_C, nl);
-    cs.swb[b0 + t] = F[ls * a1 + rs * (r + 1)];
-  }
-  if (DIR == 0 && comp == 0 && threadIdx.x == 0) cs.stat[ST_AFCH] = 0.0;
+  const int a1 = min((ca + 1) * SW_C, nl);
+  double* dst = cs.swb + ((long)comp * nch + ca) * n;  // slot 0
+  for (int r = threadIdx.x; r < n; r += 64) dst[r] = F[ls * a1 + rs * (r + 1)];
+  if (DIR == 0 && comp == 0 && ca == 0 && threadIdx.x == 0) cs.stat[ST_AFCH] = 0.0;
 }
 
 // One pass over the chunks of every case (block table: case, 2 chunk + component; entries with
