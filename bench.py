@@ -58,6 +58,9 @@ KERNEL_BYTES = {
     "k_residual": (8 * 8 + 4, 1),
     # DIF: {u_n, v_n, w_n, D_E, D_N, D_T, J^-1 Q12, Q13, Q23} -> DIF 3
     "dif": (8 * 12, 1),
+    # NL and DIF in one pass (the step): {u_n, v_n, w_n, JU 3, D_E, D_N, D_T, J^-1 Q12, Q13,
+    # Q23} -> {NL 3, DIF 3}
+    "nl_dif": (8 * 18, 1),
     # P_SOURCE + projection: {p, J, xi 7, u, v, w} -> {S 3, u, v, w}
     "psource_correct": (8 * 18, 1),
     # Germano identity terms: {u,v,w, u^ 3, xi 7, S 6, Delta^2} -> workgroup sums of LM, MM

@@ -701,11 +701,10 @@ int fvm_step(fvm_ctx* c, fvm_step_info* info) {
 #define CK(x) do { r = (x); if (r && r != FVM_ERR_NOCONV) return r; } while (0)
   CK(fvm_begin_step(c));                          // main.f90:42-45
   CK(fvm_k_contra_vel(c));                        // :48
-  CK(fvm_k_nl(c));                                // :51-57
+  CK(fvm_k_nl_dif(c));                            // :51-57 and DIF :72-75 in one pass
   if (c->prm.use_les) {                           // :59-69
     CK(fvm_k_sgs(c, false));  // SGS_x,y,z formed on first read (fvm_get_field)
   }
-  CK(fvm_k_dif(c));                               // :72-75
   CK(fvm_k_p_source(c));                          // :78
   CK(fvm_k_iter_u(c));                            // :82-88
   CK(fvm_k_correct_bcu_contra(c));                // :90-101 (correction, BCU, CONTRA_VEL)

@@ -182,6 +182,7 @@ int fvm_ghost_fill(FvmCtx* c, double* a);
 int fvm_ghost_fill_n(FvmCtx* c, double** a, int cnt);
 int fvm_k_contra_vel(FvmCtx* c);
 int fvm_k_nl(FvmCtx* c);
+int fvm_k_nl_dif(FvmCtx* c);
 int fvm_k_dif(FvmCtx* c);
 int fvm_k_p_source(FvmCtx* c);
 int fvm_k_ufar(FvmCtx* c);

@@ -97,13 +97,21 @@ __global__ void k_nl(Dims d, const double* __restrict__ ju0, const double* __res
 // ride along in registers.
 constexpr int DF_KC = 4;
 
-template <int NC>
+// With WNL set the same pass also forms NL(IN, NLOUT) of k_nl (RLES/nl.f90:23-31, its operation
+// order) from the staged planes and the face fluxes of ju (ju2 of the planes K+-1 in registers):
+// the whole step runs NL and DIF in one pass over u_n, v_n, w_n (DIF does not depend on NL or on
+// the SGS term added to NL in between).
+template <int NC, bool WNL = false>
 __global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict__ pdI,
                                                 const double* __restrict__ pdJ,
                                                 const double* __restrict__ pdK,
                                                 const double* __restrict__ jq12,
                                                 const double* __restrict__ jq13,
-                                                const double* __restrict__ jq23, CompSet cs) {
+                                                const double* __restrict__ jq23, CompSet cs,
+                                                const double* __restrict__ ju0 = nullptr,
+                                                const double* __restrict__ ju1 = nullptr,
+                                                const double* __restrict__ ju2 = nullptr,
+                                                CompSet nlo = CompSet{}) {
   const uint3 bb = xcd_block();
   __shared__ double sh[NC][3][SF_P];  // [component][plane slot][tile cell]
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
@@ -115,6 +123,7 @@ __global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict
   const int q0 = (ty + 1) * SF_W + tx + 1;
   const long sj = d.sj;
   double j13[3] = {0.0, 0.0, 0.0}, j23[3] = {0.0, 0.0, 0.0}, pk[3] = {0.0, 0.0, 0.0};
+  double z2[3] = {0.0, 0.0, 0.0};  // ju2 of the own column (NL's K faces)
   for (int kp = k0 - 1; kp <= k1 + 1; ++kp) {
     const int slot = (kp - (k0 - 1)) % 3;
     __syncthreads();  // plane kp - 3, held in this slot, is no longer read
@@ -128,17 +137,40 @@ __global__ void __launch_bounds__(256) k_dif_km(Dims d, const double* __restrict
     j13[0] = j13[1]; j13[1] = j13[2];
     j23[0] = j23[1]; j23[1] = j23[2];
     pk[0] = pk[1]; pk[1] = pk[2];
+    z2[0] = z2[1]; z2[1] = z2[2];
     if (act) {
       const long cp = cidx(d, i, j, kp);
       j13[2] = jq13[cp];
       j23[2] = jq23[cp];
       pk[2] = pdK[cp];
+      if (WNL) z2[2] = ju2[cp];
     }
     __syncthreads();
     if (kp < k0 + 1 || !act) continue;
     const int k = kp - 1;
     const int pl[3] = {(slot + 1) % 3, (slot + 2) % 3, slot};
     const long c = cidx(d, i, j, k);
+    if (WNL) {  // NL(IN, NLOUT): the face fluxes of k_nl's faces(), in its order
+      const double ue = 0.5 * (ju0[c] + ju0[c + 1]);
+      const double uw = i == 1 ? 0.0 : 0.5 * (ju0[c] + ju0[c - 1]);  // UW(1) = 0
+      const double vn = 0.5 * (ju1[c] + ju1[c + sj]);
+      const double vs = 0.5 * (ju1[c] + ju1[c - sj]);
+      const double wt = 0.5 * (z2[1] + z2[2]);
+      const double wb = 0.5 * (z2[1] + z2[0]);
+#pragma unroll
+      for (int comp = 0; comp < NC; ++comp) {
+#define F(dk, dj, di) sh[comp][pl[(dk) + 1]][q0 + (dj) * SF_W + (di)]
+        const double fc = F(0, 0, 0);
+        double t = ue * (F(0, 0, 1) + fc);
+        t = t - uw * (F(0, 0, -1) + fc);
+        t = t + vn * (F(0, 1, 0) + fc);
+        t = t - vs * (F(0, -1, 0) + fc);
+        t = t + wt * (F(1, 0, 0) + fc);
+        t = t - wb * (F(-1, 0, 0) + fc);
+#undef F
+        gpw(nlo.out[comp])[c] = 0.5 * t;
+      }
+    }
     const double de = pdI[c], dw = pdI[c - 1], dn = pdJ[c], ds = pdJ[c - sj], dt = pk[1],
                  db = pk[0];
     const double e12 = jq12[c + 1], w12 = jq12[c - 1], e13 = jq13[c + 1], w13 = jq13[c - 1];
@@ -744,6 +776,26 @@ int fvm_k_dif(FvmCtx* c) {
   else
     hipLaunchKernelGGL(k_dif_km<3>, g, kBlk, 0, c->stream, c->d, c->pdI, c->pdJ, c->pdK,
                        c->jq12, c->jq13, c->jq23, cs);
+  HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+// NL and DIF of the step in one pass (k_dif_km with WNL): the AB2 history swap of fvm_k_nl, then
+// NL(U_N..) and DIF(U_N..) of every component (and of T_N with the passive temperature)
+int fvm_k_nl_dif(FvmCtx* c) {
+  for (int q = 0; q < 3; ++q) std::swap(c->nl[q], c->nl1[q]);
+  if (c->thermal) std::swap(c->adv, c->adv1);
+  ProfScope ps(c, "nl_dif");
+  const Dims d = c->d;
+  const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, (d.nk - 1 + DF_KC - 1) / DF_KC);
+  const CompSet cs = {{c->un, c->vn, c->wn, c->tn}, {c->dif[0], c->dif[1], c->dif[2], c->dift}};
+  const CompSet nlo = {{nullptr, nullptr, nullptr, nullptr}, {c->nl[0], c->nl[1], c->nl[2], c->adv}};
+  if (c->thermal)
+    hipLaunchKernelGGL((k_dif_km<4, true>), g, kBlk, 0, c->stream, c->d, c->pdI, c->pdJ, c->pdK,
+                       c->jq12, c->jq13, c->jq23, cs, c->ju[0], c->ju[1], c->ju[2], nlo);
+  else
+    hipLaunchKernelGGL((k_dif_km<3, true>), g, kBlk, 0, c->stream, c->d, c->pdI, c->pdJ, c->pdK,
+                       c->jq12, c->jq13, c->jq23, cs, c->ju[0], c->ju[1], c->ju[2], nlo);
   HIP_CHECK(hipGetLastError());
   return FVM_OK;
 }
