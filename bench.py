@@ -277,6 +277,11 @@ KERNEL_BYTES_2D = {
     # projection: {p, J, 4 metrics, u, v} -> {u, v, S 2}
     "k2_project": 8 * 12,
 }
+# the FP32 change form of the AF sweeps (--ens-form mixed-af): phase A and the stage-2 right-hand
+# side also read the line operators {AP, AE} / {AP, AW, AS, AN} and store the residual in FP32;
+# the sweeps move the same operands in FP32
+KERNEL_BYTES_2D_AF32 = dict(KERNEL_BYTES_2D, k2_explicit=8 * 27 + 8, k2_sweep_i=4 * 10,
+                            k2_sweep_j=4 * 11, k2_rhs2=8 * 9)
 
 
 def cpu_baseline_ensemble(cases, log, budget_s=20.0):
@@ -333,8 +338,8 @@ def run_ensemble(a, log):
     cases = ensemble_cases(a.cases, dt=a.ens_dt, dt_rule="uniform")
     ncell = sum((c["ni"] - 1) * (c["nj"] - 1) for c in cases)
     log(f"ensemble: {len(cases)} cases, {ncell} cells, dt {a.ens_dt}")
-    b = batch_from_cases(cases, mixed=True, resimax=a.resimax, npass=a.npass,
-                         dt_scale=a.startup_factor)
+    b = batch_from_cases(cases, mixed=a.ens_form != "fp64", resimax=a.resimax, npass=a.npass,
+                         dt_scale=a.startup_factor, af32=a.ens_form == "mixed-af")
     infos = b.step(a.startup_steps)
     log(f"start-up: {a.startup_steps} steps at dt/{1 / a.startup_factor:g}, max cfl "
         f"{max(i['cfl'] for i in infos):.3f}")
@@ -362,7 +367,7 @@ def run_ensemble(a, log):
         log(f"  {k:16s} {ms:9.3f} ms {calls:6d} launches {ms / calls * 1e3:9.1f} us/launch "
             f"({100 * ms / tot:.1f}%)")
     rows = []
-    for k, nb in KERNEL_BYTES_2D.items():
+    for k, nb in (KERNEL_BYTES_2D_AF32 if a.ens_form == "mixed-af" else KERNEL_BYTES_2D).items():
         if k in prof:
             ms, calls = prof[k]
             avg = ms / calls * 1e-3
@@ -375,11 +380,14 @@ def run_ensemble(a, log):
         "metric": "Mcell-steps/s, batched ensemble of 2D cylinder cases of uneven size and Re",
         "value": ncell * a.steps / secs / 1e6, "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": secs / a.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.ens_form == "fp64" else "f64+f32",
         "data": "synthetic",
         "config": {"workload": "ensemble2d", "cases": len(cases), "cells": ncell,
                    "dt": a.ens_dt, "resimax": a.resimax, "poisson_passes": a.npass,
-                   "poisson": "mixed", "tuning": {"tile_j": tune["tile_j"],
+                   "poisson": "fp64" if a.ens_form == "fp64" else "mixed",
+                   "af": "fp32-change" if a.ens_form == "mixed-af" else "fp64",
+                   "tuning": {"tile_j": tune["tile_j"],
                                                   "fft_cols": tune["fft_cols"]},
                    "startup": f"{a.startup_steps} untimed steps at dt/{1 / a.startup_factor:g}"},
         "resip_max": float(hist[:, :, 6].max()),
@@ -418,6 +426,9 @@ def main():
     ap.add_argument("--ens-dt", type=float, default=0.001,
                     help="ensemble2d: the time step of every case (the reference scheme diverges "
                          "on some of the drawn cases at 0.0015 and above)")
+    ap.add_argument("--ens-form", default="mixed", choices=("mixed", "mixed-af", "fp64"),
+                    help="ensemble2d: mixed: FP32 Poisson corrections; mixed-af: also the AF "
+                         "sweeps in the FP32 change form; fp64: neither")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB four times (HBM counter calibration) and exit")
     a = ap.parse_args()

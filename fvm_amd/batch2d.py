@@ -66,9 +66,11 @@ def _bind(L):
 
 
 class Batch2D:
-    """cases: dicts with ni, nj, re, dt and the (NI, NJ) node arrays xg, yg."""
+    """cases: dicts with ni, nj, re, dt and the (NI, NJ) node arrays xg, yg.  mixed: the Poisson
+    corrections in FP32; af32: the AF line sweeps solve for each stage's change in FP32 against
+    the stage input's FP64 residual (no adjoint for such a batch)."""
 
-    def __init__(self, cases, mixed=True, resimax=1e-8, npass=3):
+    def __init__(self, cases, mixed=True, resimax=1e-8, npass=3, af32=False):
         self.L = _bind(capi.load())
         self.cases = [dict(c) for c in cases]
         n = len(cases)
@@ -83,7 +85,8 @@ class Batch2D:
             arr[q] = Case2D(int(c["ni"]), int(c["nj"]), float(c["re"]), float(c["dt"]),
                             capi._dptr(xg), capi._dptr(yg))
         h = ctypes.c_void_p()
-        capi.check(self.L.fvm_batch2d_create(arr, n, 1 if mixed else 0, float(resimax),
+        flags = (1 if mixed else 0) | (2 if af32 else 0)
+        capi.check(self.L.fvm_batch2d_create(arr, n, flags, float(resimax),
                                              int(npass), ctypes.byref(h)), "fvm_batch2d_create")
         self.h = h
         self.n = n
@@ -241,12 +244,12 @@ class Batch2D:
         return f"n{self.n}:" + ",".join(f"{a}x{b}" for a, b in sizes)
 
 
-def batch_from_cases(cases, mixed=True, resimax=1e-8, npass=3, dt_scale=1.0):
+def batch_from_cases(cases, mixed=True, resimax=1e-8, npass=3, dt_scale=1.0, af32=False):
     """A Batch2D of fvm_amd.grid.ensemble_cases entries (grid, seeded start), each at its
     case's time step times dt_scale."""
     specs = [dict(ni=c["ni"], nj=c["nj"], re=c["re"], dt=c["dt"] * dt_scale, xg=c["xg"][:, :, 0],
                   yg=c["yg"][:, :, 0]) for c in cases]
-    b = Batch2D(specs, mixed=mixed, resimax=resimax, npass=npass)
+    b = Batch2D(specs, mixed=mixed, resimax=resimax, npass=npass, af32=af32)
     for q, c in enumerate(cases):
         b.set_state(q, *(c["ic"][k][:, :, 0] for k in ("u", "v", "p", "nlx", "nly")))
     return b

@@ -29,6 +29,8 @@ enum F2 {
   G_GA2, G_GC2,        // stage-2 line couplings AS, AN (the line ends' ghost terms)
   G_TINVD2, G_TNA2, G_TB2, G_TAW2,  // stage-2 factors and AW in the J-line-major layout
                                     // [J + (NJ+1) I] (J-line sweeps read rows contiguously)
+  G_AE1,               // stage-1 AE (folded AE(NI-1) = 0; AW(I) = AE(I-1)): the FP64 residual
+                       // of the FP32 change form
   // state
   F_U, F_V, F_P, F_NLX, F_NLY,
   // step work arrays
@@ -49,6 +51,13 @@ enum B2 { B_UFAR, B_VFAR, B_AW1W, B_AE1F, B_AW2W, B_AE2F, B_UWALL, B_VWALL, B_WA
 enum A2 { A_LSU, A_LSV, A_LP, A_LNX, A_LNY, A_W1U, A_W1V, A_W2U, A_W2V, A_LQU, A_LQV, A_LSXU,
           A_LSXV, A_MU, A_B, A_WP, NA2 };
 enum AB2 { AB_UFW, AB_VFW, AB_UFF, AB_VFF, NAB2 };
+
+// FP32 planes of the change form of the AF sweeps (a stage solves T (x - x0) = r, r the FP64
+// residual of the stage input x0, in FP32): the residual (natural layout), the change iterate
+// (natural layout in the I-line stage, J-line-major in the J-line stage), the J-line-major
+// residual, and the line factors and lagged couplings of the two stages in FP32
+enum H2 { H_RU, H_RV, H_DU, H_DV, H_TRU, H_TRV, H_INVD1, H_NA1, H_B1, H_AN1, H_TINVD2, H_TNA2,
+          H_TB2, H_TAW2, NH2 };
 
 struct Case2D {
   int ni, nj;          // node counts NI (radial), NJ (circumferential)
@@ -74,6 +83,7 @@ struct Case2D {
   double* stat;        // per-case scalars (see ST_*)
   int ntile;           // stencil tiles of this case (partials per pass)
   double* swb;         // AF sweep chunk boundary lines [pass slot][component][chunk][row]
+  float* h[NH2];       // FP32 change form planes
   double* a[NA2];      // adjoint planes (null until the adjoint is initialised)
   double* ab[NAB2];
 };

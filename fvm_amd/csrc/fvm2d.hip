@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -24,8 +25,10 @@ namespace {
 // Phase A: U_N = U (whole plane), NL_NM1 = NL, CONTRA_VEL, NL(U_N), NL(V_N), DIF(U_N), DIF(V_N),
 // P_SOURCE, the stage-1 right-hand sides SU (iter_u.f90:43-46 / iter_v.f90) with the cross
 // terms DD of the current field, the far-field U_F of BCU/BCV (bcu.f90:22-33) from U_N and the
-// stage-1 right-hand-side folds (bcu.f90:15, 34).
+// stage-1 right-hand-side folds (bcu.f90:15, 34).  AF32: instead of SU, the FP64 residual of the
+// stage input against the stage-1 line operator, in FP32, for the change form of the sweep.
 // ---------------------------------------------------------------------------------------------
+template <bool AF32>
 __global__ void __launch_bounds__(256) k2_explicit(const Case2D* __restrict__ tab,
                                                    const int* __restrict__ bcase,
                                                    const int* __restrict__ btile) {
@@ -74,14 +77,17 @@ __global__ void __launch_bounds__(256) k2_explicit(const Case2D* __restrict__ ta
   const double su = psrc(g.dj, g.xix, g.etx, P, c, sj);
   const double sv = psrc(g.dj, g.xiy, g.ety, P, c, sj);
   const double re = cs.re;
-  const double ap0 = cs.f[G_AP0][c];
+  // the change form leaves out the 2 Re DJAC / dt U term: it cancels against the diagonal
+  const double ap0 = AF32 ? 0.0 : cs.f[G_AP0][c];
   double r1u = ap0 * U[c] - 3.0 * re * nlx + 1.0 * re * nlx1 + difx + 2.0 * re * su +
                cross2(cs.f[G_JXE1], U, c, sj);
   double r1v = ap0 * V[c] - 3.0 * re * nly + 1.0 * re * nly1 + dify + 2.0 * re * sv +
                cross2(cs.f[G_JXE1], V, c, sj);
-  if (i == 1) {  // wall fold SU(1) += 2 AW(1) U_F
-    r1u = r1u + 2.0 * cs.b[B_AW1W][j] * cs.b[B_UWALL][j];
-    r1v = r1v + 2.0 * cs.b[B_AW1W][j] * cs.b[B_VWALL][j];
+  if (i == 1) {  // wall fold SU(1) += 2 AW(1) U_F (change form: 2 AW(1) (U_F - U))
+    const double uw = AF32 ? cs.b[B_UWALL][j] - U[c] : cs.b[B_UWALL][j];
+    const double vw = AF32 ? cs.b[B_VWALL][j] - V[c] : cs.b[B_VWALL][j];
+    r1u = r1u + 2.0 * cs.b[B_AW1W][j] * uw;
+    r1v = r1v + 2.0 * cs.b[B_AW1W][j] * vw;
   }
   if (i == ni - 1) {
     // far-field U_F: inflow Dirichlet (UIN = 1, VIN = 0) where THETA(NI, J) is in [pi/2, 3pi/2],
@@ -108,8 +114,8 @@ __global__ void __launch_bounds__(256) k2_explicit(const Case2D* __restrict__ ta
     }
     cs.b[B_UFAR][j] = ufu;
     cs.b[B_VFAR][j] = ufv;
-    r1u = r1u + 2.0 * cs.b[B_AE1F][j] * ufu;
-    r1v = r1v + 2.0 * cs.b[B_AE1F][j] * ufv;
+    r1u = r1u + 2.0 * cs.b[B_AE1F][j] * (AF32 ? ufu - U[c] : ufu);
+    r1v = r1v + 2.0 * cs.b[B_AE1F][j] * (AF32 ? ufv - V[c] : ufv);
   }
   cs.f[F_NLX1][c] = nlx1;
   cs.f[F_NLY1][c] = nly1;
@@ -121,14 +127,30 @@ __global__ void __launch_bounds__(256) k2_explicit(const Case2D* __restrict__ ta
   cs.f[F_SV][c] = sv;
   // the old next line's coupling AN U(J+1), taken into the right-hand side before the sweep; the
   // branch-cut ghost J = NJ is BCU's periodic image J = 1
-  {
-    const double an = cs.f[G_AN1][c];
-    const long cn = j == nj - 1 ? (long)i + sj : c + sj;
+  const long cn = j == nj - 1 ? (long)i + sj : c + sj;
+  const double an = cs.f[G_AN1][c];
+  if (AF32) {
+    // the residual r = SU + AS x0(J-1) + AN x0(J+1) + AW x0(I-1) + AE x0(I+1) - AP x0 of the
+    // stage input x0 = U, with AP = 2 Re DJAC / dt + AE + AW + AN + AS (+ the folds above), as
+    // differences: AN (x0(J+1) - x0) + AS (x0(J-1) - x0) + AE (..) + AW (..) + the rest of SU
+    // (folded AW(1) = AE(NI-1) = 0; the ghost line J = 0 is BCU's periodic image J = NJ-1, as
+    // the sweep sees it)
+    const double as = cs.f[G_AN1][c - sj];  // AS(J) = AN(J-1)
+    const double ae = cs.f[G_AE1][c];
+    const double aw = cs.f[G_AE1][c - 1];  // AW(I) = AE(I-1)
+    const long cs0 = j == 1 ? (long)i + (long)sj * (nj - 1) : c - sj;
+    const double ru = r1u + an * (U[cn] - U[c]) + as * (U[cs0] - U[c]) +
+                      aw * (U[c - 1] - U[c]) + ae * (U[c + 1] - U[c]);
+    const double rv = r1v + an * (V[cn] - V[c]) + as * (V[cs0] - V[c]) +
+                      aw * (V[c - 1] - V[c]) + ae * (V[c + 1] - V[c]);
+    cs.h[H_RU][c] = (float)ru;
+    cs.h[H_RV][c] = (float)rv;
+  } else {
     r1u = r1u + an * U[cn];
     r1v = r1v + an * V[cn];
+    cs.f[F_SU1][c] = r1u;
+    cs.f[F_SV1][c] = r1v;
   }
-  cs.f[F_SU1][c] = r1u;
-  cs.f[F_SV1][c] = r1v;
 }
 
 // BCU / BCV ghosts (bcu.f90:10-51) of field F from its interior: wall row I = 0 and far-field
@@ -172,17 +194,18 @@ __device__ inline void bcu_ghosts_wave(double* F, int ni, int nj, int sj, const 
 // DIR 1: J lines (rows along J, lines I = 1..NI-1, lagged coupling AW); the line ends take the
 // ghost values (BCU, applied by k2_sweep_seed) as Dirichlet data.
 // ---------------------------------------------------------------------------------------------
-template <int R>
+template <int R, typename T>
 struct LineOps {
-  double cdn[R], q[R], iv[R], na[R], bf[R], xo[R];
-  double g0, g1;                                   // ghosts before / after the line
+  T cdn[R], q[R], iv[R], na[R], bf[R], xo[R];
+  double x0[R];                                    // the field, where the change is added
+  T g0, g1;                                        // ghosts before / after the line
 };
 
-template <int R, int DIR, bool EARLY>
-__device__ inline void sweep_load(LineOps<R>& o, const double* F, const double* Q,
-                                  const double* invd, const double* na, const double* bf,
-                                  const double* cdn, long base, long rs, long dnoff, int n, int r0,
-                                  int l) {
+template <int R, bool EARLY, typename T>
+__device__ inline void sweep_load(LineOps<R, T>& o, const T* F, const T* Q, const T* invd,
+                                  const T* na, const T* bf, const T* cdn, const double* F64,
+                                  long base, long rs, long dnoff, int n, int r0, int l) {
+  constexpr bool DELTA = std::is_same<T, float>::value;
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int r = r0 + q;
@@ -193,24 +216,46 @@ __device__ inline void sweep_load(LineOps<R>& o, const double* F, const double* 
       o.iv[q] = invd[c];
       o.na[q] = na[c];
       o.bf[q] = bf[c];
-      o.xo[q] = EARLY ? F[c] : 0.0;  // the previous pass's value of the row
+      o.xo[q] = EARLY ? F[c] : (T)0;  // the previous pass's value of the row
+      o.x0[q] = F64 ? F64[c] : 0.0;
     }
   }
-  o.g0 = l == 0 ? F[base] : 0.0;
-  o.g1 = l == 63 ? F[base + rs * (n + 1)] : 0.0;
+  // the change form's ghosts do not change
+  o.g0 = !DELTA && l == 0 ? F[base] : (T)0;
+  o.g1 = !DELTA && l == 63 ? F[base + rs * (n + 1)] : (T)0;
 }
 
-// DIR 1 runs on the J-line-major copies (rows contiguous): iterate F_TX*, right-hand side F_TQ*,
-// factors and AW G_T*, made by k2_tr_in and returned by k2_tr_out.
-template <int R, int DIR, bool EARLY>
+// T = double: the sweep of the stage on the field itself (DIR 1 on the J-line-major copies: the
+// iterate F_TX*, right-hand side F_TQ*, factors and AW G_T*, made by k2_tr_in and returned by
+// k2_tr_out).  T = float: the change form.  The sweep solves T_L d_L = r_L + AS d_(L-1) for the
+// change d = x - x0 of the stage input x0, with r the FP64 residual of x0 (k2_explicit,
+// k2_rhs2; the J-line stage's J-line-major copy by k2_tr_in32), zero ghosts and a zero ghost
+// line; the iterate is H_DU / H_DV (J-line-major for DIR 1).  The I-line stage's last pass adds
+// the change to the FP64 field, the J-line stage's k2_tr_out32 does.
+template <int R, int DIR, bool EARLY, typename T>
 __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double tol, int final) {
+  constexpr bool DELTA = std::is_same<T, float>::value;
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
-  double* F = DIR == 0 ? cs.f[comp ? F_V : F_U] : cs.f[comp ? F_TXV : F_TXU];
-  const double* Q = DIR == 0 ? cs.f[comp ? F_SV1 : F_SU1] : cs.f[comp ? F_TQV : F_TQU];
-  const double* invd = cs.f[DIR == 0 ? G_INVD1 : G_TINVD2];
-  const double* na = cs.f[DIR == 0 ? G_NA1 : G_TNA2];
-  const double* bf = cs.f[DIR == 0 ? G_B1 : G_TB2];
-  const double* cdn = cs.f[DIR == 0 ? G_AN1 : G_TAW2];
+  T* F;
+  const T *Q, *invd, *na, *bf, *cdn;
+  if constexpr (DELTA) {
+    F = cs.h[comp ? H_DV : H_DU];
+    Q = cs.h[DIR == 0 ? (comp ? H_RV : H_RU) : (comp ? H_TRV : H_TRU)];
+    invd = cs.h[DIR == 0 ? H_INVD1 : H_TINVD2];
+    na = cs.h[DIR == 0 ? H_NA1 : H_TNA2];
+    bf = cs.h[DIR == 0 ? H_B1 : H_TB2];
+    cdn = cs.h[DIR == 0 ? H_AN1 : H_TAW2];
+  } else {
+    F = DIR == 0 ? cs.f[comp ? F_V : F_U] : cs.f[comp ? F_TXV : F_TXU];
+    Q = DIR == 0 ? cs.f[comp ? F_SV1 : F_SU1] : cs.f[comp ? F_TQV : F_TQU];
+    invd = cs.f[DIR == 0 ? G_INVD1 : G_TINVD2];
+    na = cs.f[DIR == 0 ? G_NA1 : G_TNA2];
+    bf = cs.f[DIR == 0 ? G_B1 : G_TB2];
+    cdn = cs.f[DIR == 0 ? G_AN1 : G_TAW2];
+  }
+  // the change form's I-line stage adds its last pass's change to the field
+  const bool add = DELTA && DIR == 0 && final;
+  double* const F64 = add ? cs.f[comp ? F_V : F_U] : nullptr;
   const int n = DIR == 0 ? ni - 1 : nj - 1;       // rows per line
   const int nl = DIR == 0 ? nj - 1 : ni - 1;      // lines
   const long rs = 1;                              // row stride
@@ -222,45 +267,52 @@ __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double
   const long bin = ((long)(slot * 2 + comp) * nch) * n, bout = ((long)((slot ^ 1) * 2 + comp) * nch) * n;
   const int l = threadIdx.x;
   const int r0 = l * R;
-  double prev[R];
+  T prev[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int r = r0 + q;
-    // the line before the chunk: the ghost line 0, or the preceding chunk's last line
-    prev[q] = r < n ? (ca == 0 ? F[rs * (r + 1)] : bnd[bin + (long)(ca - 1) * n + r]) : 0.0;
+    // the line before the chunk: the ghost line 0 (no change in the change form), or the
+    // preceding chunk's last line
+    T v = (T)0;
+    if (r < n) {
+      if (ca > 0) v = (T)bnd[bin + (long)(ca - 1) * n + r];
+      else if (!DELTA) v = F[rs * (r + 1)];
+    }
+    prev[q] = v;
   }
-  LineOps<R> cur, nxt;
-  sweep_load<R, DIR, EARLY>(cur, F, Q, invd, na, bf, cdn, ls * a0, rs, dnoff, n, r0, l);
+  LineOps<R, T> cur, nxt;
+  sweep_load<R, EARLY>(cur, F, Q, invd, na, bf, cdn, F64, ls * a0, rs, dnoff, n, r0, l);
   bool stopped = false;
-  double xrun = 0.0;
-  for (int L = a0; L <= a1; ++L) {
+  T xrun = (T)0;
+  int L = a0;
+  for (; L <= a1; ++L) {
     const long base = ls * L;
     if (L < a1)
-      sweep_load<R, DIR, EARLY>(nxt, F, Q, invd, na, bf, cdn, base + ls, rs, dnoff, n, r0, l);
-    double al[R], be[R], bb[R];
-    double At = 0.0, Bt = 1.0;
+      sweep_load<R, EARLY>(nxt, F, Q, invd, na, bf, cdn, F64, base + ls, rs, dnoff, n, r0, l);
+    T al[R], be[R], bb[R];
+    T At = (T)0, Bt = (T)1;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       if (r0 + q < n) {
-        const double C = cur.cdn[q] * prev[q] + cur.q[q];
+        const T C = cur.cdn[q] * prev[q] + cur.q[q];
         al[q] = C * cur.iv[q];
         be[q] = cur.na[q];
         bb[q] = cur.bf[q];
       } else {
-        al[q] = 0.0; be[q] = 1.0; bb[q] = 0.0;
+        al[q] = (T)0; be[q] = (T)1; bb[q] = (T)0;
       }
       At = al[q] + be[q] * At;
       Bt = be[q] * Bt;
     }
     // lane l starts from the exclusive prefix applied to the ghost before the first row
-    double yin = wave::prefix_apply(At, Bt, wave::lane_value(cur.g0, 0));
-    double y[R];
+    T yin = wave::prefix_apply(At, Bt, wave::lane_value(cur.g0, 0));
+    T y[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       y[q] = al[q] + be[q] * yin;
       yin = y[q];
     }
-    double Ct = 1.0, Dt = 0.0;
+    T Ct = (T)1, Dt = (T)0;
 #pragma unroll
     for (int q = R - 1; q >= 0; --q) {
       if (r0 + q < n) {
@@ -269,16 +321,20 @@ __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double
       }
     }
     // and from the exclusive suffix applied to the ghost after the last row
-    double xin = wave::suffix_apply(Dt, Ct, wave::lane_value(cur.g1, 63));
-    double dl = 0.0;
+    T xin = wave::suffix_apply(Dt, Ct, wave::lane_value(cur.g1, 63));
+    T dl = (T)0;
 #pragma unroll
     for (int q = R - 1; q >= 0; --q) {
       const int r = r0 + q;
       if (r < n) {
-        const double x = y[q] - bb[q] * xin;
+        const T x = y[q] - bb[q] * xin;
         xin = x;
         prev[q] = x;
-        F[base + rs * (r + 1)] = x;
+        const long c = base + rs * (r + 1);
+        if (add)
+          F64[c] = cur.x0[q] + (double)x;
+        else
+          F[c] = x;
         if (EARLY) {
           dl = fmax(dl, fabs(x - cur.xo[q]));
           xrun = fmax(xrun, fabs(x));
@@ -291,9 +347,22 @@ __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double
         dl = fmax(dl, __shfl_xor(dl, o, 64));
         xrun = fmax(xrun, __shfl_xor(xrun, o, 64));
       }
-      if (dl <= tol * xrun) { stopped = true; break; }  // uniform over the wavefront
+      if (dl <= (T)tol * xrun) { stopped = true; break; }  // uniform over the wavefront
     }
     cur = nxt;
+  }
+  if (add && stopped) {
+    // the lines after the stop keep the previous pass's change
+    for (int M = L + 1; M <= a1; ++M) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int r = r0 + q;
+        if (r < n) {
+          const long c = ls * M + rs * (r + 1);
+          F64[c] = F64[c] + (double)F[c];
+        }
+      }
+    }
   }
   double dm = 0.0, xm = 0.0;
 #pragma unroll
@@ -301,7 +370,7 @@ __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double
     const int r = r0 + q;
     if (r < n) {
       const double old = bnd[bin + (long)ca * n + r];
-      const double x = stopped ? old : prev[q];  // a stopped chunk's last line keeps its value
+      const double x = stopped ? old : (double)prev[q];  // a stopped chunk's last line keeps its value
       dm = fmax(dm, fabs(x - old));
       xm = fmax(xm, fabs(x));
       bnd[bout + (long)ca * n + r] = x;
@@ -384,10 +453,75 @@ __global__ void __launch_bounds__(256) k2_tr_out(const Case2D* __restrict__ tab,
   }
 }
 
+// The change form's J-line-major copy of the stage-2 residual (FP32), as k2_tr_in.
+__global__ void __launch_bounds__(256) k2_tr_in32(const Case2D* __restrict__ tab,
+                                                  const int* __restrict__ bcase,
+                                                  const int* __restrict__ btile) {
+  __shared__ float tile[2][32][33];
+  const Case2D& cs = tab[bcase[blockIdx.x]];
+  const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
+  const int tx = (ni + 1 + 31) / 32, t = btile[blockIdx.x];
+  const int i0 = (t % tx) * 32, j0 = (t / tx) * 32;
+  const float* src[2] = {cs.h[H_RU], cs.h[H_RV]};
+  float* dst[2] = {cs.h[H_TRU], cs.h[H_TRV]};
+  const int l = threadIdx.x;
+  for (int r = 0; r < 4; ++r) {
+    const int li = l % 32, lj = l / 32 + 8 * r;
+    const int i = i0 + li, j = j0 + lj;
+    if (i <= ni && j <= nj) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a) tile[a][lj][li] = src[a][i + (long)sj * j];
+    }
+  }
+  __syncthreads();
+  for (int r = 0; r < 4; ++r) {
+    const int lj = l % 32, li = l / 32 + 8 * r;
+    const int i = i0 + li, j = j0 + lj;
+    if (i <= ni && j <= nj) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a) dst[a][j + (long)(nj + 1) * i] = tile[a][lj][li];
+    }
+  }
+}
+
+// The change form's J-line sweep result: the J-line-major change added to the field interior.
+__global__ void __launch_bounds__(256) k2_tr_out32(const Case2D* __restrict__ tab,
+                                                   const int* __restrict__ bcase,
+                                                   const int* __restrict__ btile) {
+  __shared__ float tile[2][32][33];
+  const Case2D& cs = tab[bcase[blockIdx.x]];
+  const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
+  const int tx = (ni + 1 + 31) / 32, t = btile[blockIdx.x];
+  const int i0 = (t % tx) * 32, j0 = (t / tx) * 32;
+  const float* src[2] = {cs.h[H_DU], cs.h[H_DV]};
+  double* dst[2] = {cs.f[F_U], cs.f[F_V]};
+  const int l = threadIdx.x;
+  for (int r = 0; r < 4; ++r) {
+    const int lj = l % 32, li = l / 32 + 8 * r;
+    const int i = i0 + li, j = j0 + lj;
+    if (i <= ni && j <= nj) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a) tile[a][lj][li] = src[a][j + (long)(nj + 1) * i];
+    }
+  }
+  __syncthreads();
+  for (int r = 0; r < 4; ++r) {
+    const int li = l % 32, lj = l / 32 + 8 * r;
+    const int i = i0 + li, j = j0 + lj;
+    if (i >= 1 && i <= ni - 1 && j >= 1 && j <= nj - 1) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const long c = i + (long)sj * j;
+        dst[a][c] = dst[a][c] + (double)tile[a][lj][li];
+      }
+    }
+  }
+}
+
 // The first pass's chunk boundary lines (the stage input's last line of every chunk) and BCU's
 // ghosts of the stage input (by each (case, component)'s chunk-0 wavefront; the ghosts and the
 // interior lines copied here do not overlap); block table as the passes'.
-template <int DIR>
+template <int DIR, bool DELTA>
 __global__ void __launch_bounds__(64) k2_sweep_seed(const Case2D* __restrict__ tab,
                                                     const int* __restrict__ bcase,
                                                     const int* __restrict__ btile) {
@@ -406,13 +540,14 @@ __global__ void __launch_bounds__(64) k2_sweep_seed(const Case2D* __restrict__ t
   const int nch = sw_nchunk(nl);
   const int a1 = min((ca + 1) * SW_C, nl);
   double* dst = cs.swb + ((long)comp * nch + ca) * n;  // slot 0
-  for (int r = threadIdx.x; r < n; r += 64) dst[r] = F[ls * a1 + rs * (r + 1)];
+  // the change form starts from no change
+  for (int r = threadIdx.x; r < n; r += 64) dst[r] = DELTA ? 0.0 : F[ls * a1 + rs * (r + 1)];
   if (DIR == 0 && comp == 0 && ca == 0 && threadIdx.x == 0) cs.stat[ST_AFCH] = 0.0;
 }
 
 // One pass over the chunks of every case (block table: case, 2 chunk + component; entries with
 // case -1 pad the table to whole XCD rounds).
-template <int DIR, bool EARLY>
+template <int DIR, bool EARLY, typename T>
 __global__ void __launch_bounds__(64) k2_sweep_pass(const Case2D* __restrict__ tab,
                                                     const int* __restrict__ bcase,
                                                     const int* __restrict__ btile, int slot,
@@ -424,19 +559,21 @@ __global__ void __launch_bounds__(64) k2_sweep_pass(const Case2D* __restrict__ t
   const int comp = t & 1, ca = t >> 1;
   const int n = DIR == 0 ? cs.ni - 1 : cs.nj - 1;
   switch ((n + 63) >> 6) {  // the case's own rows per lane
-    case 1: sweep_chunk<1, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
-    case 2: sweep_chunk<2, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
-    case 3: sweep_chunk<3, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
-    case 4: sweep_chunk<4, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
-    case 5: sweep_chunk<5, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
-    case 6: sweep_chunk<6, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
-    case 7: sweep_chunk<7, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
-    default: sweep_chunk<8, DIR, EARLY>(cs, comp, ca, slot, tol, final); break;
+    case 1: sweep_chunk<1, DIR, EARLY, T>(cs, comp, ca, slot, tol, final); break;
+    case 2: sweep_chunk<2, DIR, EARLY, T>(cs, comp, ca, slot, tol, final); break;
+    case 3: sweep_chunk<3, DIR, EARLY, T>(cs, comp, ca, slot, tol, final); break;
+    case 4: sweep_chunk<4, DIR, EARLY, T>(cs, comp, ca, slot, tol, final); break;
+    case 5: sweep_chunk<5, DIR, EARLY, T>(cs, comp, ca, slot, tol, final); break;
+    case 6: sweep_chunk<6, DIR, EARLY, T>(cs, comp, ca, slot, tol, final); break;
+    case 7: sweep_chunk<7, DIR, EARLY, T>(cs, comp, ca, slot, tol, final); break;
+    default: sweep_chunk<8, DIR, EARLY, T>(cs, comp, ca, slot, tol, final); break;
   }
 }
 
 // Stage-2 right-hand side SU = 2 Re DJAC / dt U + DD(U) with the stage-2 cross weights
-// (iter_u.f90:84-98) and BCU's folds with the stage-2 AW(1), AE(NI-1) (bcu.f90:15, 34).
+// (iter_u.f90:84-98) and BCU's folds with the stage-2 AW(1), AE(NI-1) (bcu.f90:15, 34).  AF32:
+// instead of SU, the FP64 residual of the stage input against the stage-2 line operator, in FP32.
+template <bool AF32>
 __global__ void __launch_bounds__(256) k2_rhs2(const Case2D* __restrict__ tab,
                                                const int* __restrict__ bcase,
                                                const int* __restrict__ btile) {
@@ -448,25 +585,42 @@ __global__ void __launch_bounds__(256) k2_rhs2(const Case2D* __restrict__ tab,
   const long c = (long)i + (long)sj * j;
   const double* U = cs.f[F_U];
   const double* V = cs.f[F_V];
-  const double ap0 = cs.f[G_AP0][c];
+  // the change form leaves out the 2 Re DJAC / dt U term: it cancels against the diagonal
+  const double ap0 = AF32 ? 0.0 : cs.f[G_AP0][c];
   double ru = ap0 * U[c] + cross2(cs.f[G_JXE2], U, c, sj);
   double rv = ap0 * V[c] + cross2(cs.f[G_JXE2], V, c, sj);
-  if (i == 1) {
-    ru = ru + 2.0 * cs.b[B_AW2W][j] * cs.b[B_UWALL][j];
-    rv = rv + 2.0 * cs.b[B_AW2W][j] * cs.b[B_VWALL][j];
+  if (i == 1) {  // (change form: 2 AW(1) (U_F - U))
+    ru = ru + 2.0 * cs.b[B_AW2W][j] * (AF32 ? cs.b[B_UWALL][j] - U[c] : cs.b[B_UWALL][j]);
+    rv = rv + 2.0 * cs.b[B_AW2W][j] * (AF32 ? cs.b[B_VWALL][j] - V[c] : cs.b[B_VWALL][j]);
   }
   if (i == ni - 1) {
-    ru = ru + 2.0 * cs.b[B_AE2F][j] * cs.b[B_UFAR][j];
-    rv = rv + 2.0 * cs.b[B_AE2F][j] * cs.b[B_VFAR][j];
+    ru = ru + 2.0 * cs.b[B_AE2F][j] * (AF32 ? cs.b[B_UFAR][j] - U[c] : cs.b[B_UFAR][j]);
+    rv = rv + 2.0 * cs.b[B_AE2F][j] * (AF32 ? cs.b[B_VFAR][j] - V[c] : cs.b[B_VFAR][j]);
   }
-  // the old next line's coupling AE U(I+1) (the far-field fold leaves AE(NI-1) = 0)
-  {
-    const double ae = cs.f[G_AE2][c];
+  const double ae = cs.f[G_AE2][c];
+  if (AF32) {
+    // the residual r = SU + AW x0(I-1) + AE x0(I+1) + AS x0(J-1) + AN x0(J+1) - AP x0 of the
+    // stage input x0 = U as differences (AP = 2 Re DJAC / dt + AE + AW + AN + AS + the folds
+    // above; folded AW(1) = AE(NI-1) = 0, AW(I) = AE(I-1), AS(J) = AN(J-1) below J = 1); the
+    // line ends' ghosts J = 0, NJ are BCU's periodic images J = NJ-1, 1, as the sweep sees them
+    const double aw = i == 1 ? 0.0 : cs.f[G_AE2][c - 1];
+    const double an = cs.f[G_GC2][c];
+    const double as = j == 1 ? cs.f[G_GA2][c] : cs.f[G_GC2][c - sj];
+    const long cs0 = j == 1 ? (long)i + (long)sj * (nj - 1) : c - sj;
+    const long cn0 = j == nj - 1 ? (long)i + sj : c + sj;
+    const double eu = ru + ae * (U[c + 1] - U[c]) + aw * (U[c - 1] - U[c]) +
+                      as * (U[cs0] - U[c]) + an * (U[cn0] - U[c]);
+    const double ev = rv + ae * (V[c + 1] - V[c]) + aw * (V[c - 1] - V[c]) +
+                      as * (V[cs0] - V[c]) + an * (V[cn0] - V[c]);
+    cs.h[H_RU][c] = (float)eu;
+    cs.h[H_RV][c] = (float)ev;
+  } else {
+    // the old next line's coupling AE U(I+1) (the far-field fold leaves AE(NI-1) = 0)
     ru = ru + ae * U[c + 1];
     rv = rv + ae * V[c + 1];
+    cs.f[F_SU1][c] = ru;
+    cs.f[F_SV1][c] = rv;
   }
-  cs.f[F_SU1][c] = ru;
-  cs.f[F_SV1][c] = rv;
 }
 
 // BCU ghosts written by the interior thread that owns their value after a full BCU: the wall

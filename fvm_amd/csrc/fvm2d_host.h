@@ -19,6 +19,7 @@ struct AF2D {
   // couplings to the ghosts: ga = -a (AS at the first row), gc = -c (AN at the last row)
   std::vector<double> nat[2], bt[2], ga2, gc2;
   std::vector<double> awwall[2], aefar[2];  // pre-fold AW(1, J), AE(NI-1, J) per stage
+  std::vector<double> ae1;                  // stage-1 AE after the far-field fold
 };
 
 int geo2d_build(int ni, int nj, const double* xg, const double* yg, Geo2D& g);

@@ -226,6 +226,7 @@ void geo2d_af(const Geo2D& g, double re, double dt, AF2D& a) {
     a.awwall[s].assign(nj + 1, 0.0);
     a.aefar[s].assign(nj + 1, 0.0);
   }
+  a.ae1.assign(n, 0.0);
   std::vector<double> tx(n), te(n);
   for (int s = 0; s < 2; ++s) {
     const std::vector<double>& mx = s == 0 ? g.xix : g.xiy;
@@ -255,6 +256,9 @@ void geo2d_af(const Geo2D& g, double re, double dt, AF2D& a) {
       AP[cf] = AP[cf] + AE[cf];
       AE[cf] = 0.0;
     }
+    if (s == 0)
+      for (int j = 1; j <= nj - 1; ++j)
+        for (int i = 1; i <= ni - 1; ++i) a.ae1[ix(sj, i, j)] = AE[ix(sj, i, j)];
     // transposed factors along the sweep lines (a = -AW or -AS, c = -AE or -AN)
     {
       const int nl = s == 0 ? nj - 1 : ni - 1, nr = s == 0 ? ni - 1 : nj - 1;

@@ -198,9 +198,11 @@ typedef struct fvm_case2d {
 } fvm_case2d;
 
 /* GRID, METRIC, INIT geometry and the AF / Poisson factors of every case (host set-up in the
- * reference's operations), device planes, block tables and the captured step.  mixed: the
- * Poisson correction's transforms and mode-line storage in FP32 (residual, metrics, p in FP64);
- * npass: Poisson correction passes per step (each case stops once RESIP <= resimax).
+ * reference's operations), device planes, block tables and the captured step.  mixed, bit set:
+ * 1: the Poisson correction's transforms and mode-line storage in FP32 (residual, metrics, p in
+ * FP64); 2: the AF line sweeps of both stages solve for the change of the stage in FP32 against
+ * the stage input's FP64 residual (the field stays FP64; fvm_batch2d_adjoint_init refuses such a
+ * batch).  npass: Poisson correction passes per step (each case stops once RESIP <= resimax).
  * NJ-1 must be even with prime factors 2, 3, 5, 7, 11, 13; NI-1, NJ-1 <= 512. */
 int fvm_batch2d_create(const fvm_case2d* cases, int n, int mixed, double resimax, int npass,
                        fvm_batch2d** out);
