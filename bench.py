@@ -260,14 +260,14 @@ def copy_calibration(mib, log):
 # distinct operand once, FP64 8 B, FP32 4 B; line factors shared by the u and v waves once).
 KERNEL_BYTES_2D = {
     # phase A: {u, v, p, J, xi_x, xi_y, eta_x, eta_y, D_E, D_N, J Q12, 2 Re J/dt, stage-1 cross
-    # weights, NLX, NLY} -> {U_N, V_N, NL_NM1 2, NL 2, DIF 2, S 2, SU 2}
-    "k2_explicit": 8 * 27,
+    # weights, AN, NLX, NLY} -> {U_N, V_N, NL 2, DIF 2, S 2, SU 2}
+    "k2_explicit": 8 * 26,
     # I-line sweep: factors {1/D, beta, B, AN} + per component {SU, x_old, x}
     "k2_sweep_i": 8 * 10,
     # J-line sweep: factors {1/D, beta, B, AE, AW} + per component {SU, x_old, x}
     "k2_sweep_j": 8 * 11,
-    # stage-2 right-hand side: {u, v, 2 Re J/dt, cross weights} -> SU 2
-    "k2_rhs2": 8 * 6,
+    # stage-2 right-hand side: {u, v, 2 Re J/dt, cross weights, AE} -> SU 2
+    "k2_rhs2": 8 * 7,
     # predictor correction: {u, v, S 2, J} -> {u, v}
     "k2_correct": 8 * 7,
     # Poisson source: {u, v, J, 4 metrics} -> RHS
@@ -277,11 +277,11 @@ KERNEL_BYTES_2D = {
     # projection: {p, J, 4 metrics, u, v} -> {u, v, S 2}
     "k2_project": 8 * 12,
 }
-# the FP32 change form of the AF sweeps (--ens-form mixed-af): phase A and the stage-2 right-hand
-# side also read the line operators {AP, AE} / {AP, AW, AS, AN} and store the residual in FP32;
-# the sweeps move the same operands in FP32
-KERNEL_BYTES_2D_AF32 = dict(KERNEL_BYTES_2D, k2_explicit=8 * 27 + 8, k2_sweep_i=4 * 10,
-                            k2_sweep_j=4 * 11, k2_rhs2=8 * 9)
+# the FP32 change form of the AF sweeps (--ens-form mixed): phase A and the stage-2
+# right-hand side read AE / AN instead of 2 Re J/dt and store the residual in FP32; the sweeps
+# move the same operands in FP32
+KERNEL_BYTES_2D_AF32 = dict(KERNEL_BYTES_2D, k2_explicit=8 * 24 + 4 * 2, k2_sweep_i=4 * 10,
+                            k2_sweep_j=4 * 11, k2_rhs2=8 * 5 + 4 * 2)
 
 
 def cpu_baseline_ensemble(cases, log, budget_s=20.0):
@@ -339,7 +339,7 @@ def run_ensemble(a, log):
     ncell = sum((c["ni"] - 1) * (c["nj"] - 1) for c in cases)
     log(f"ensemble: {len(cases)} cases, {ncell} cells, dt {a.ens_dt}")
     b = batch_from_cases(cases, mixed=a.ens_form != "fp64", resimax=a.resimax, npass=a.npass,
-                         dt_scale=a.startup_factor, af32=a.ens_form == "mixed-af")
+                         dt_scale=a.startup_factor, af32=a.ens_form == "mixed")
     infos = b.step(a.startup_steps)
     log(f"start-up: {a.startup_steps} steps at dt/{1 / a.startup_factor:g}, max cfl "
         f"{max(i['cfl'] for i in infos):.3f}")
@@ -367,7 +367,7 @@ def run_ensemble(a, log):
         log(f"  {k:16s} {ms:9.3f} ms {calls:6d} launches {ms / calls * 1e3:9.1f} us/launch "
             f"({100 * ms / tot:.1f}%)")
     rows = []
-    for k, nb in (KERNEL_BYTES_2D_AF32 if a.ens_form == "mixed-af" else KERNEL_BYTES_2D).items():
+    for k, nb in (KERNEL_BYTES_2D_AF32 if a.ens_form == "mixed" else KERNEL_BYTES_2D).items():
         if k in prof:
             ms, calls = prof[k]
             avg = ms / calls * 1e-3
@@ -383,10 +383,10 @@ def run_ensemble(a, log):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if a.ens_form == "fp64" else "f64+f32",
         "data": "synthetic",
-        "config": {"workload": "ensemble2d", "cases": len(cases), "cells": ncell,
+        "config": {"workload": "ensemble2d", "form": a.ens_form, "cases": len(cases), "cells": ncell,
                    "dt": a.ens_dt, "resimax": a.resimax, "poisson_passes": a.npass,
                    "poisson": "fp64" if a.ens_form == "fp64" else "mixed",
-                   "af": "fp32-change" if a.ens_form == "mixed-af" else "fp64",
+                   "af": "fp32-change" if a.ens_form == "mixed" else "fp64",
                    "tuning": {"tile_j": tune["tile_j"],
                                                   "fft_cols": tune["fft_cols"]},
                    "startup": f"{a.startup_steps} untimed steps at dt/{1 / a.startup_factor:g}"},
@@ -426,9 +426,10 @@ def main():
     ap.add_argument("--ens-dt", type=float, default=0.001,
                     help="ensemble2d: the time step of every case (the reference scheme diverges "
                          "on some of the drawn cases at 0.0015 and above)")
-    ap.add_argument("--ens-form", default="mixed", choices=("mixed", "mixed-af", "fp64"),
-                    help="ensemble2d: mixed: FP32 Poisson corrections; mixed-af: also the AF "
-                         "sweeps in the FP32 change form; fp64: neither")
+    ap.add_argument("--ens-form", default="mixed", choices=("mixed", "mixed-poisson", "fp64"),
+                    help="ensemble2d: mixed: FP32 Poisson corrections and the AF sweeps in the "
+                         "FP32 change form; mixed-poisson: the Poisson corrections only; fp64: "
+                         "neither")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB four times (HBM counter calibration) and exit")
     a = ap.parse_args()

@@ -34,7 +34,7 @@ enum F2 {
   // state
   F_U, F_V, F_P, F_NLX, F_NLY,
   // step work arrays
-  F_UN, F_VN, F_NLX1, F_NLY1, F_DIFX, F_DIFY, F_SU, F_SV, F_SU1, F_SV1, F_RHS,
+  F_UN, F_VN, F_DIFX, F_DIFY, F_SU, F_SV, F_SU1, F_SV1, F_RHS,
   F_TQU, F_TQV, F_TXU, F_TXV,  // J-line sweep: right-hand sides and iterates, J-line-major
   NF2
 };

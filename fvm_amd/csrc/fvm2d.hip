@@ -22,8 +22,9 @@
 namespace {
 
 // ---------------------------------------------------------------------------------------------
-// Phase A: U_N = U (whole plane), NL_NM1 = NL, CONTRA_VEL, NL(U_N), NL(V_N), DIF(U_N), DIF(V_N),
-// P_SOURCE, the stage-1 right-hand sides SU (iter_u.f90:43-46 / iter_v.f90) with the cross
+// Phase A: U_N = U (whole plane), CONTRA_VEL, NL(U_N), NL(V_N), DIF(U_N), DIF(V_N), P_SOURCE,
+// the stage-1 right-hand sides SU (iter_u.f90:43-46 / iter_v.f90) with the old NL (NL_NM1, held
+// in a register between the read of NL and its update, not stored) and the cross
 // terms DD of the current field, the far-field U_F of BCU/BCV (bcu.f90:22-33) from U_N and the
 // stage-1 right-hand-side folds (bcu.f90:15, 34).  AF32: instead of SU, the FP64 residual of the
 // stage input against the stage-1 line operator, in FP32, for the change form of the sweep.
@@ -117,8 +118,6 @@ __global__ void __launch_bounds__(256) k2_explicit(const Case2D* __restrict__ ta
     r1u = r1u + 2.0 * cs.b[B_AE1F][j] * (AF32 ? ufu - U[c] : ufu);
     r1v = r1v + 2.0 * cs.b[B_AE1F][j] * (AF32 ? ufv - V[c] : ufv);
   }
-  cs.f[F_NLX1][c] = nlx1;
-  cs.f[F_NLY1][c] = nly1;
   NLX[c] = nlx;
   NLY[c] = nly;
   cs.f[F_DIFX][c] = difx;

@@ -13,8 +13,9 @@ and one JSON line reports the end-of-window relative differences of u, v, w, p a
 mean C_D, rms C_L, C_L amplitude and Strouhal number (C_L zero crossings) of every form.
 
 2D (--state): a 2D case from a developed, shedding state (the trajectory fixture's inputs) on the
-batched 2D solver, FP64 Poisson correction against the FP32 one inside the FP64 defect
-correction, both to the same RESIMAX, with the same differences and statistics.
+batched 2D solver in the same three forms (mixed: the FP32 Poisson corrections and the AF sweeps
+of both stages in the FP32 change form), all to the same RESIMAX, with the same differences and
+statistics.
 """
 from __future__ import annotations
 
@@ -29,6 +30,8 @@ import numpy as np
 
 
 FORMS3D = {"fp64": (2, 0), "mixed-poisson": (3, 0), "mixed": (3, 2)}
+# 2D batch flags per form: (FP32 Poisson corrections, FP32 change form of the AF sweeps)
+FORMS2D = {"fp64": (False, False), "mixed-poisson": (True, False), "mixed": (True, True)}
 
 
 def solver3d(w, dt, poisson_mode, af_mode, resimax):
@@ -94,17 +97,9 @@ def main3d(a, log):
         w["dt"] = a.dt
     state = spinup3d(w, a.spinup, a.resimax, a.startup, log)
     res = {f: window3d(w, state, f, a.steps, a.resimax, log) for f in FORMS3D}
-    f64, s64, cl64 = res["fp64"]
     out = {"workload": a.workload, "dt": w["dt"], "spinup": a.spinup, "steps": a.steps,
-           "resimax": a.resimax, "fp64": s64}
-    for f in ("mixed-poisson", "mixed"):
-        fx, sx, clx = res[f]
-        rd = lambda k: abs(sx[k] - s64[k]) / abs(s64[k])
-        out[f] = dict(sx, field_rel_diff_end={c: float(np.abs(fx[c] - f64[c]).max() /
-                                                       np.abs(f64[c]).max()) for c in f64},
-                      cl_max_abs_diff=float(np.abs(clx - cl64).max()),
-                      cd_mean_rel_diff=rd("cd_mean"), cl_rms_rel_diff=rd("cl_rms"),
-                      cl_amp_rel_diff=rd("cl_amp"), st_rel_diff=rd("st"))
+           "resimax": a.resimax}
+    out.update(compare(res))
     print(json.dumps(out), flush=True)
 
 
@@ -117,11 +112,12 @@ def crossings_st(t, cl):
     return float(1.0 / np.mean(np.diff(tc)))
 
 
-def run2d(g, mixed, steps, resimax, log):
+def run2d(g, form, steps, resimax, log):
     from fvm_amd.batch2d import Batch2D
     ni, nj = int(g["ni"]), int(g["nj"])
+    mixed, af32 = FORMS2D[form]
     b = Batch2D([dict(ni=ni, nj=nj, re=float(g["re"]), dt=float(g["dt"]), xg=g["xg"],
-                      yg=g["yg"])], mixed=mixed, resimax=resimax, npass=6)
+                      yg=g["yg"])], mixed=mixed, resimax=resimax, npass=6, af32=af32)
     b.set_state(0, *(g["in_" + k] for k in ("u", "v", "p", "nlx", "nly")))
     t0 = time.perf_counter()
     hist = []
@@ -138,23 +134,34 @@ def run2d(g, mixed, steps, resimax, log):
     stats = {"cd_mean": float(cd.mean()), "cl_rms": float(np.sqrt(np.mean((cl - cl.mean()) ** 2))),
              "cl_amp": float(0.5 * (cl.max() - cl.min())), "st": crossings_st(t, cl),
              "resip_max": float(h[:, 6].max()), "secs": secs}
-    log(f"2D {'mixed' if mixed else 'fp64'}: {stats}")
+    log(f"2D {form}: {stats}")
     fields = {c: b.get(0, c) for c in ("u", "v", "p")}
     b.close()
     return fields, stats, cl
 
 
+def compare(res, base="fp64"):
+    """Per form other than base: its statistics and their differences to base's."""
+    f64, s64, cl64 = res[base]
+    out = {base: s64}
+    for f, (fx, sx, clx) in res.items():
+        if f == base:
+            continue
+        rd = lambda k: abs(sx[k] - s64[k]) / abs(s64[k])
+        out[f] = dict(sx, field_rel_diff_end={c: float(np.abs(fx[c] - f64[c]).max() /
+                                                       np.abs(f64[c]).max()) for c in f64},
+                      cl_max_abs_diff=float(np.abs(clx - cl64).max()),
+                      cd_mean_rel_diff=rd("cd_mean"), cl_rms_rel_diff=rd("cl_rms"),
+                      cl_amp_rel_diff=rd("cl_amp"), st_rel_diff=rd("st"))
+    return out
+
+
 def main2d(a, log):
     g = np.load(a.state)
-    f64, s64, cl64 = run2d(g, False, a.steps, a.resimax, log)
-    f32, s32, cl32 = run2d(g, True, a.steps, a.resimax, log)
-    rel = {c: float(np.abs(f32[c] - f64[c]).max() / np.abs(f64[c]).max()) for c in f64}
-    rd = lambda k: abs(s32[k] - s64[k]) / abs(s64[k])
+    res = {f: run2d(g, f, a.steps, a.resimax, log) for f in FORMS2D}
     out = {"state": os.path.basename(a.state), "steps": a.steps, "resimax": a.resimax,
-           "dt": float(g["dt"]), "re": float(g["re"]), "grid": [int(g["ni"]), int(g["nj"])],
-           "field_rel_diff_end": rel, "cl_max_abs_diff": float(np.abs(cl32 - cl64).max()),
-           "fp64": s64, "mixed": s32, "cd_mean_rel_diff": rd("cd_mean"),
-           "cl_rms_rel_diff": rd("cl_rms"), "st_rel_diff": rd("st"), "cl_amp_rel_diff": rd("cl_amp")}
+           "dt": float(g["dt"]), "re": float(g["re"]), "grid": [int(g["ni"]), int(g["nj"])]}
+    out.update(compare(res))
     print(json.dumps(out), flush=True)
 
 
