@@ -21,6 +21,17 @@
 
 namespace {
 
+// complex values of a global-address-space array, moved as their two components
+template <typename T>
+__device__ __forceinline__ Cx<T> cx_ld(gptr<const T> p, long o) {
+  return Cx<T>{p[2 * o], p[2 * o + 1]};
+}
+template <typename T>
+__device__ __forceinline__ void cx_st(gptr<T> p, long o, Cx<T> v) {
+  p[2 * o] = v.x;
+  p[2 * o + 1] = v.y;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Phase A: U_N = U (whole plane), CONTRA_VEL, NL(U_N), NL(V_N), DIF(U_N), DIF(V_N), P_SOURCE,
 // the stage-1 right-hand sides SU (iter_u.f90:43-46 / iter_v.f90) with the old NL (NL_NM1, held
@@ -39,11 +50,11 @@ __global__ void __launch_bounds__(256) k2_explicit(const Case2D* __restrict__ ta
   const int i = t.i, j = t.j;
   if (i > ni - 1 || j > nj - 1) return;
   const long c = (long)i + (long)sj * j;
-  const double* U = cs.f[F_U];
-  const double* V = cs.f[F_V];
-  const double* P = cs.f[F_P];
-  double* UN = cs.f[F_UN];
-  double* VN = cs.f[F_VN];
+  const auto U = glb(cs.f[F_U]);
+  const auto V = glb(cs.f[F_V]);
+  const auto P = glb(cs.f[F_P]);
+  const auto UN = glb(cs.f[F_UN]);
+  const auto VN = glb(cs.f[F_VN]);
   // U_N = U over the whole plane: every interior thread copies its cell and the ghosts next to
   // it (corners from the corner cells)
   {
@@ -59,43 +70,43 @@ __global__ void __launch_bounds__(256) k2_explicit(const Case2D* __restrict__ ta
       VN[q] = V[q];
     }
   }
-  const Geom g{cs.f[G_DJAC], cs.f[G_XIX], cs.f[G_XIY], cs.f[G_ETX], cs.f[G_ETY]};
+  const GeomG g = geom2(cs);
   const double j1 = ju1(g, U, V, c), j2 = ju2(g, U, V, c);
   const double ue = 0.5 * (j1 + ju1(g, U, V, c + 1));
   const double uw = i == 1 ? 0.0 : 0.5 * (j1 + ju1(g, U, V, c - 1));  // UW(1) = 0
   const double vn = 0.5 * (j2 + ju2(g, U, V, c + sj));
   const double vs = 0.5 * (j2 + ju2(g, U, V, c - sj));
-  double* NLX = cs.f[F_NLX];
-  double* NLY = cs.f[F_NLY];
+  const auto NLX = glb(cs.f[F_NLX]);
+  const auto NLY = glb(cs.f[F_NLY]);
   const double nlx1 = NLX[c], nly1 = NLY[c];
   const double nlx = nl2(U, c, sj, ue, uw, vn, vs);
   const double nly = nl2(V, c, sj, ue, uw, vn, vs);
-  const double* de = cs.f[G_DE];
-  const double* dn = cs.f[G_DN];
-  const double* jq = cs.f[G_JQ12];
+  const auto de = glb(cs.f[G_DE]);
+  const auto dn = glb(cs.f[G_DN]);
+  const auto jq = glb(cs.f[G_JQ12]);
   const double difx = dif2(de, dn, jq, U, c, sj);
   const double dify = dif2(de, dn, jq, V, c, sj);
   const double su = psrc(g.dj, g.xix, g.etx, P, c, sj);
   const double sv = psrc(g.dj, g.xiy, g.ety, P, c, sj);
   const double re = cs.re;
   // the change form leaves out the 2 Re DJAC / dt U term: it cancels against the diagonal
-  const double ap0 = AF32 ? 0.0 : cs.f[G_AP0][c];
+  const double ap0 = AF32 ? 0.0 : glb(cs.f[G_AP0])[c];
   double r1u = ap0 * U[c] - 3.0 * re * nlx + 1.0 * re * nlx1 + difx + 2.0 * re * su +
-               cross2(cs.f[G_JXE1], U, c, sj);
+               cross2(glb(cs.f[G_JXE1]), U, c, sj);
   double r1v = ap0 * V[c] - 3.0 * re * nly + 1.0 * re * nly1 + dify + 2.0 * re * sv +
-               cross2(cs.f[G_JXE1], V, c, sj);
+               cross2(glb(cs.f[G_JXE1]), V, c, sj);
   if (i == 1) {  // wall fold SU(1) += 2 AW(1) U_F (change form: 2 AW(1) (U_F - U))
-    const double uw = AF32 ? cs.b[B_UWALL][j] - U[c] : cs.b[B_UWALL][j];
-    const double vw = AF32 ? cs.b[B_VWALL][j] - V[c] : cs.b[B_VWALL][j];
-    r1u = r1u + 2.0 * cs.b[B_AW1W][j] * uw;
-    r1v = r1v + 2.0 * cs.b[B_AW1W][j] * vw;
+    const double uw = AF32 ? glb(cs.b[B_UWALL])[j] - U[c] : glb(cs.b[B_UWALL])[j];
+    const double vw = AF32 ? glb(cs.b[B_VWALL])[j] - V[c] : glb(cs.b[B_VWALL])[j];
+    r1u = r1u + 2.0 * glb(cs.b[B_AW1W])[j] * uw;
+    r1v = r1v + 2.0 * glb(cs.b[B_AW1W])[j] * vw;
   }
   if (i == ni - 1) {
     // far-field U_F: inflow Dirichlet (UIN = 1, VIN = 0) where THETA(NI, J) is in [pi/2, 3pi/2],
     // else the convective outflow value from U_N with convection speed UIN (bcu.f90:26-33)
     const double pi = 3.14159265358979323846;
     const long f = c + 1;  // (NI, J)
-    const double th = cs.f[G_THETA][f];
+    const double th = glb(cs.f[G_THETA])[f];
     double ufu, ufv;
     if (th >= 0.5 * pi && th <= 1.5 * pi) {
       ufu = 1.0;
@@ -103,60 +114,60 @@ __global__ void __launch_bounds__(256) k2_explicit(const Case2D* __restrict__ ta
     } else {
       const double ax = 0.5 * (g.xix[f] + g.xix[c]);
       const double ae = 0.5 * (g.etx[f] + g.etx[c]);
-      const double* F2[2] = {U, V};
+      const gptr<double> F2[2] = {U, V};
       double uf[2];
       for (int q = 0; q < 2; ++q) {
-        const double* W = F2[q];
+        const auto W = F2[q];
         const double s = W[f + sj] + W[c + sj] - W[f - sj] - W[c - sj];
         uf[q] = 0.5 * (W[f] + W[c]) - cs.dt * 1.0 * (ax * (W[f] - W[c]) + ae * 0.25 * s);
       }
       ufu = uf[0];
       ufv = uf[1];
     }
-    cs.b[B_UFAR][j] = ufu;
-    cs.b[B_VFAR][j] = ufv;
-    r1u = r1u + 2.0 * cs.b[B_AE1F][j] * (AF32 ? ufu - U[c] : ufu);
-    r1v = r1v + 2.0 * cs.b[B_AE1F][j] * (AF32 ? ufv - V[c] : ufv);
+    glb(cs.b[B_UFAR])[j] = ufu;
+    glb(cs.b[B_VFAR])[j] = ufv;
+    r1u = r1u + 2.0 * glb(cs.b[B_AE1F])[j] * (AF32 ? ufu - U[c] : ufu);
+    r1v = r1v + 2.0 * glb(cs.b[B_AE1F])[j] * (AF32 ? ufv - V[c] : ufv);
   }
   NLX[c] = nlx;
   NLY[c] = nly;
-  cs.f[F_DIFX][c] = difx;
-  cs.f[F_DIFY][c] = dify;
-  cs.f[F_SU][c] = su;
-  cs.f[F_SV][c] = sv;
+  glb(cs.f[F_DIFX])[c] = difx;
+  glb(cs.f[F_DIFY])[c] = dify;
+  glb(cs.f[F_SU])[c] = su;
+  glb(cs.f[F_SV])[c] = sv;
   // the old next line's coupling AN U(J+1), taken into the right-hand side before the sweep; the
   // branch-cut ghost J = NJ is BCU's periodic image J = 1
   const long cn = j == nj - 1 ? (long)i + sj : c + sj;
-  const double an = cs.f[G_AN1][c];
+  const double an = glb(cs.f[G_AN1])[c];
   if (AF32) {
     // the residual r = SU + AS x0(J-1) + AN x0(J+1) + AW x0(I-1) + AE x0(I+1) - AP x0 of the
     // stage input x0 = U, with AP = 2 Re DJAC / dt + AE + AW + AN + AS (+ the folds above), as
     // differences: AN (x0(J+1) - x0) + AS (x0(J-1) - x0) + AE (..) + AW (..) + the rest of SU
     // (folded AW(1) = AE(NI-1) = 0; the ghost line J = 0 is BCU's periodic image J = NJ-1, as
     // the sweep sees it)
-    const double as = cs.f[G_AN1][c - sj];  // AS(J) = AN(J-1)
-    const double ae = cs.f[G_AE1][c];
-    const double aw = cs.f[G_AE1][c - 1];  // AW(I) = AE(I-1)
+    const double as = glb(cs.f[G_AN1])[c - sj];  // AS(J) = AN(J-1)
+    const double ae = glb(cs.f[G_AE1])[c];
+    const double aw = glb(cs.f[G_AE1])[c - 1];  // AW(I) = AE(I-1)
     const long cs0 = j == 1 ? (long)i + (long)sj * (nj - 1) : c - sj;
     const double ru = r1u + an * (U[cn] - U[c]) + as * (U[cs0] - U[c]) +
                       aw * (U[c - 1] - U[c]) + ae * (U[c + 1] - U[c]);
     const double rv = r1v + an * (V[cn] - V[c]) + as * (V[cs0] - V[c]) +
                       aw * (V[c - 1] - V[c]) + ae * (V[c + 1] - V[c]);
-    cs.h[H_RU][c] = (float)ru;
-    cs.h[H_RV][c] = (float)rv;
+    glb(cs.h[H_RU])[c] = (float)ru;
+    glb(cs.h[H_RV])[c] = (float)rv;
   } else {
     r1u = r1u + an * U[cn];
     r1v = r1v + an * V[cn];
-    cs.f[F_SU1][c] = r1u;
-    cs.f[F_SV1][c] = r1v;
+    glb(cs.f[F_SU1])[c] = r1u;
+    glb(cs.f[F_SV1])[c] = r1v;
   }
 }
 
 // BCU / BCV ghosts (bcu.f90:10-51) of field F from its interior: wall row I = 0 and far-field
 // row I = NI for J = 0..NJ, then the branch-cut columns J = 0, NJ for I = 0..NI.  One
 // workgroup of 64 lanes.
-__device__ inline void bcu_ghosts_wave(double* F, int ni, int nj, int sj, const double* uwall,
-                                       const double* ufar) {
+template <class P, class B>
+__device__ inline void bcu_ghosts_wave(P F, int ni, int nj, int sj, B uwall, B ufar) {
   const int l = threadIdx.x;
   for (int jj = l; jj <= nj; jj += 64) {
     const bool in = jj >= 1 && jj <= nj - 1;
@@ -201,9 +212,10 @@ struct LineOps {
 };
 
 template <int R, bool EARLY, typename T>
-__device__ inline void sweep_load(LineOps<R, T>& o, const T* F, const T* Q, const T* invd,
-                                  const T* na, const T* bf, const T* cdn, const double* F64,
-                                  long base, long rs, long dnoff, int n, int r0, int l) {
+__device__ inline void sweep_load(LineOps<R, T>& o, gptr<const T> F, gptr<const T> Q,
+                                  gptr<const T> invd, gptr<const T> na, gptr<const T> bf,
+                                  gptr<const T> cdn, gptr<const double> F64, long base, long rs,
+                                  long dnoff, int n, int r0, int l) {
   constexpr bool DELTA = std::is_same<T, float>::value;
 #pragma unroll
   for (int q = 0; q < R; ++q) {
@@ -235,26 +247,30 @@ template <int R, int DIR, bool EARLY, typename T>
 __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double tol, int final) {
   constexpr bool DELTA = std::is_same<T, float>::value;
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
-  T* F;
-  const T *Q, *invd, *na, *bf, *cdn;
+  T *F0, *Q0, *iv0, *na0, *bf0, *cd0;
   if constexpr (DELTA) {
-    F = cs.h[comp ? H_DV : H_DU];
-    Q = cs.h[DIR == 0 ? (comp ? H_RV : H_RU) : (comp ? H_TRV : H_TRU)];
-    invd = cs.h[DIR == 0 ? H_INVD1 : H_TINVD2];
-    na = cs.h[DIR == 0 ? H_NA1 : H_TNA2];
-    bf = cs.h[DIR == 0 ? H_B1 : H_TB2];
-    cdn = cs.h[DIR == 0 ? H_AN1 : H_TAW2];
+    F0 = cs.h[comp ? H_DV : H_DU];
+    Q0 = cs.h[DIR == 0 ? (comp ? H_RV : H_RU) : (comp ? H_TRV : H_TRU)];
+    iv0 = cs.h[DIR == 0 ? H_INVD1 : H_TINVD2];
+    na0 = cs.h[DIR == 0 ? H_NA1 : H_TNA2];
+    bf0 = cs.h[DIR == 0 ? H_B1 : H_TB2];
+    cd0 = cs.h[DIR == 0 ? H_AN1 : H_TAW2];
   } else {
-    F = DIR == 0 ? cs.f[comp ? F_V : F_U] : cs.f[comp ? F_TXV : F_TXU];
-    Q = DIR == 0 ? cs.f[comp ? F_SV1 : F_SU1] : cs.f[comp ? F_TQV : F_TQU];
-    invd = cs.f[DIR == 0 ? G_INVD1 : G_TINVD2];
-    na = cs.f[DIR == 0 ? G_NA1 : G_TNA2];
-    bf = cs.f[DIR == 0 ? G_B1 : G_TB2];
-    cdn = cs.f[DIR == 0 ? G_AN1 : G_TAW2];
+    F0 = DIR == 0 ? cs.f[comp ? F_V : F_U] : cs.f[comp ? F_TXV : F_TXU];
+    Q0 = DIR == 0 ? cs.f[comp ? F_SV1 : F_SU1] : cs.f[comp ? F_TQV : F_TQU];
+    iv0 = cs.f[DIR == 0 ? G_INVD1 : G_TINVD2];
+    na0 = cs.f[DIR == 0 ? G_NA1 : G_TNA2];
+    bf0 = cs.f[DIR == 0 ? G_B1 : G_TB2];
+    cd0 = cs.f[DIR == 0 ? G_AN1 : G_TAW2];
   }
+  // global-address-space views: global (not flat) loads, which do not wait on LDS and scalar
+  // traffic, so the next line's loads stay in flight while this line is solved
+  const gptr<T> F = glb(F0);
+  const gptr<const T> Q = glb((const T*)Q0), invd = glb((const T*)iv0), na = glb((const T*)na0);
+  const gptr<const T> bf = glb((const T*)bf0), cdn = glb((const T*)cd0);
   // the change form's I-line stage adds its last pass's change to the field
   const bool add = DELTA && DIR == 0 && final;
-  double* const F64 = add ? cs.f[comp ? F_V : F_U] : nullptr;
+  const gptr<double> F64 = glb(add ? cs.f[comp ? F_V : F_U] : nullptr);
   const int n = DIR == 0 ? ni - 1 : nj - 1;       // rows per line
   const int nl = DIR == 0 ? nj - 1 : ni - 1;      // lines
   const long rs = 1;                              // row stride
@@ -262,7 +278,7 @@ __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double
   const long dnoff = DIR == 0 ? -(long)sj : 0;    // lower coupling of row c at cdn[c + dnoff]
   const int nch = sw_nchunk(nl);
   const int a0 = 1 + ca * SW_C, a1 = min(a0 + SW_C - 1, nl);
-  double* const bnd = cs.swb;
+  const gptr<double> bnd = glb(cs.swb);
   const long bin = ((long)(slot * 2 + comp) * nch) * n, bout = ((long)((slot ^ 1) * 2 + comp) * nch) * n;
   const int l = threadIdx.x;
   const int r0 = l * R;
@@ -399,8 +415,8 @@ __global__ void __launch_bounds__(256) k2_tr_in(const Case2D* __restrict__ tab,
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
   const int tx = (ni + 1 + 31) / 32, t = btile[blockIdx.x];
   const int i0 = (t % tx) * 32, j0 = (t / tx) * 32;
-  const double* src[4] = {cs.f[F_SU1], cs.f[F_SV1], cs.f[F_U], cs.f[F_V]};
-  double* dst[4] = {cs.f[F_TQU], cs.f[F_TQV], cs.f[F_TXU], cs.f[F_TXV]};
+  const gptr<double> src[4] = {glb(cs.f[F_SU1]), glb(cs.f[F_SV1]), glb(cs.f[F_U]), glb(cs.f[F_V])};
+  const gptr<double> dst[4] = {glb(cs.f[F_TQU]), glb(cs.f[F_TQV]), glb(cs.f[F_TXU]), glb(cs.f[F_TXV])};
   const int l = threadIdx.x;
   for (int r = 0; r < 4; ++r) {
     const int li = l % 32, lj = l / 32 + 8 * r;
@@ -430,8 +446,8 @@ __global__ void __launch_bounds__(256) k2_tr_out(const Case2D* __restrict__ tab,
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
   const int tx = (ni + 1 + 31) / 32, t = btile[blockIdx.x];
   const int i0 = (t % tx) * 32, j0 = (t / tx) * 32;
-  const double* src[2] = {cs.f[F_TXU], cs.f[F_TXV]};
-  double* dst[2] = {cs.f[F_U], cs.f[F_V]};
+  const gptr<double> src[2] = {glb(cs.f[F_TXU]), glb(cs.f[F_TXV])};
+  const gptr<double> dst[2] = {glb(cs.f[F_U]), glb(cs.f[F_V])};
   const int l = threadIdx.x;
   for (int r = 0; r < 4; ++r) {
     const int lj = l % 32, li = l / 32 + 8 * r;
@@ -461,8 +477,8 @@ __global__ void __launch_bounds__(256) k2_tr_in32(const Case2D* __restrict__ tab
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
   const int tx = (ni + 1 + 31) / 32, t = btile[blockIdx.x];
   const int i0 = (t % tx) * 32, j0 = (t / tx) * 32;
-  const float* src[2] = {cs.h[H_RU], cs.h[H_RV]};
-  float* dst[2] = {cs.h[H_TRU], cs.h[H_TRV]};
+  const gptr<float> src[2] = {glb(cs.h[H_RU]), glb(cs.h[H_RV])};
+  const gptr<float> dst[2] = {glb(cs.h[H_TRU]), glb(cs.h[H_TRV])};
   const int l = threadIdx.x;
   for (int r = 0; r < 4; ++r) {
     const int li = l % 32, lj = l / 32 + 8 * r;
@@ -492,8 +508,8 @@ __global__ void __launch_bounds__(256) k2_tr_out32(const Case2D* __restrict__ ta
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
   const int tx = (ni + 1 + 31) / 32, t = btile[blockIdx.x];
   const int i0 = (t % tx) * 32, j0 = (t / tx) * 32;
-  const float* src[2] = {cs.h[H_DU], cs.h[H_DV]};
-  double* dst[2] = {cs.f[F_U], cs.f[F_V]};
+  const gptr<float> src[2] = {glb(cs.h[H_DU]), glb(cs.h[H_DV])};
+  const gptr<double> dst[2] = {glb(cs.f[F_U]), glb(cs.f[F_V])};
   const int l = threadIdx.x;
   for (int r = 0; r < 4; ++r) {
     const int lj = l % 32, li = l / 32 + 8 * r;
@@ -529,16 +545,16 @@ __global__ void __launch_bounds__(64) k2_sweep_seed(const Case2D* __restrict__ t
   const Case2D& cs = tab[cc];
   const int t = btile[blockIdx.x];
   const int comp = t & 1, ca = t >> 1;
-  double* F = cs.f[comp ? F_V : F_U];
+  const auto F = glb(cs.f[comp ? F_V : F_U]);
   if (ca == 0)
-    bcu_ghosts_wave(F, cs.ni, cs.nj, cs.sj, cs.b[comp ? B_VWALL : B_UWALL],
-                    cs.b[comp ? B_VFAR : B_UFAR]);
+    bcu_ghosts_wave(F, cs.ni, cs.nj, cs.sj, glb(cs.b[comp ? B_VWALL : B_UWALL]),
+                    glb(cs.b[comp ? B_VFAR : B_UFAR]));
   const int n = DIR == 0 ? cs.ni - 1 : cs.nj - 1;
   const int nl = DIR == 0 ? cs.nj - 1 : cs.ni - 1;
   const long rs = DIR == 0 ? 1 : cs.sj, ls = DIR == 0 ? cs.sj : 1;
   const int nch = sw_nchunk(nl);
   const int a1 = min((ca + 1) * SW_C, nl);
-  double* dst = cs.swb + ((long)comp * nch + ca) * n;  // slot 0
+  const auto dst = glb(cs.swb) + ((long)comp * nch + ca) * n;  // slot 0
   // the change form starts from no change
   for (int r = threadIdx.x; r < n; r += 64) dst[r] = DELTA ? 0.0 : F[ls * a1 + rs * (r + 1)];
   if (DIR == 0 && comp == 0 && ca == 0 && threadIdx.x == 0) cs.stat[ST_AFCH] = 0.0;
@@ -582,50 +598,51 @@ __global__ void __launch_bounds__(256) k2_rhs2(const Case2D* __restrict__ tab,
   const int i = t.i, j = t.j;
   if (i > ni - 1 || j > nj - 1) return;
   const long c = (long)i + (long)sj * j;
-  const double* U = cs.f[F_U];
-  const double* V = cs.f[F_V];
+  const auto U = glb(cs.f[F_U]);
+  const auto V = glb(cs.f[F_V]);
   // the change form leaves out the 2 Re DJAC / dt U term: it cancels against the diagonal
-  const double ap0 = AF32 ? 0.0 : cs.f[G_AP0][c];
-  double ru = ap0 * U[c] + cross2(cs.f[G_JXE2], U, c, sj);
-  double rv = ap0 * V[c] + cross2(cs.f[G_JXE2], V, c, sj);
+  const double ap0 = AF32 ? 0.0 : glb(cs.f[G_AP0])[c];
+  double ru = ap0 * U[c] + cross2(glb(cs.f[G_JXE2]), U, c, sj);
+  double rv = ap0 * V[c] + cross2(glb(cs.f[G_JXE2]), V, c, sj);
   if (i == 1) {  // (change form: 2 AW(1) (U_F - U))
-    ru = ru + 2.0 * cs.b[B_AW2W][j] * (AF32 ? cs.b[B_UWALL][j] - U[c] : cs.b[B_UWALL][j]);
-    rv = rv + 2.0 * cs.b[B_AW2W][j] * (AF32 ? cs.b[B_VWALL][j] - V[c] : cs.b[B_VWALL][j]);
+    ru = ru + 2.0 * glb(cs.b[B_AW2W])[j] * (AF32 ? glb(cs.b[B_UWALL])[j] - U[c] : glb(cs.b[B_UWALL])[j]);
+    rv = rv + 2.0 * glb(cs.b[B_AW2W])[j] * (AF32 ? glb(cs.b[B_VWALL])[j] - V[c] : glb(cs.b[B_VWALL])[j]);
   }
   if (i == ni - 1) {
-    ru = ru + 2.0 * cs.b[B_AE2F][j] * (AF32 ? cs.b[B_UFAR][j] - U[c] : cs.b[B_UFAR][j]);
-    rv = rv + 2.0 * cs.b[B_AE2F][j] * (AF32 ? cs.b[B_VFAR][j] - V[c] : cs.b[B_VFAR][j]);
+    ru = ru + 2.0 * glb(cs.b[B_AE2F])[j] * (AF32 ? glb(cs.b[B_UFAR])[j] - U[c] : glb(cs.b[B_UFAR])[j]);
+    rv = rv + 2.0 * glb(cs.b[B_AE2F])[j] * (AF32 ? glb(cs.b[B_VFAR])[j] - V[c] : glb(cs.b[B_VFAR])[j]);
   }
-  const double ae = cs.f[G_AE2][c];
+  const double ae = glb(cs.f[G_AE2])[c];
   if (AF32) {
     // the residual r = SU + AW x0(I-1) + AE x0(I+1) + AS x0(J-1) + AN x0(J+1) - AP x0 of the
     // stage input x0 = U as differences (AP = 2 Re DJAC / dt + AE + AW + AN + AS + the folds
     // above; folded AW(1) = AE(NI-1) = 0, AW(I) = AE(I-1), AS(J) = AN(J-1) below J = 1); the
     // line ends' ghosts J = 0, NJ are BCU's periodic images J = NJ-1, 1, as the sweep sees them
-    const double aw = i == 1 ? 0.0 : cs.f[G_AE2][c - 1];
-    const double an = cs.f[G_GC2][c];
-    const double as = j == 1 ? cs.f[G_GA2][c] : cs.f[G_GC2][c - sj];
+    const double aw = i == 1 ? 0.0 : glb(cs.f[G_AE2])[c - 1];
+    const double an = glb(cs.f[G_GC2])[c];
+    const double as = j == 1 ? glb(cs.f[G_GA2])[c] : glb(cs.f[G_GC2])[c - sj];
     const long cs0 = j == 1 ? (long)i + (long)sj * (nj - 1) : c - sj;
     const long cn0 = j == nj - 1 ? (long)i + sj : c + sj;
     const double eu = ru + ae * (U[c + 1] - U[c]) + aw * (U[c - 1] - U[c]) +
                       as * (U[cs0] - U[c]) + an * (U[cn0] - U[c]);
     const double ev = rv + ae * (V[c + 1] - V[c]) + aw * (V[c - 1] - V[c]) +
                       as * (V[cs0] - V[c]) + an * (V[cn0] - V[c]);
-    cs.h[H_RU][c] = (float)eu;
-    cs.h[H_RV][c] = (float)ev;
+    glb(cs.h[H_RU])[c] = (float)eu;
+    glb(cs.h[H_RV])[c] = (float)ev;
   } else {
     // the old next line's coupling AE U(I+1) (the far-field fold leaves AE(NI-1) = 0)
     ru = ru + ae * U[c + 1];
     rv = rv + ae * V[c + 1];
-    cs.f[F_SU1][c] = ru;
-    cs.f[F_SV1][c] = rv;
+    glb(cs.f[F_SU1])[c] = ru;
+    glb(cs.f[F_SV1])[c] = rv;
   }
 }
 
 // BCU ghosts written by the interior thread that owns their value after a full BCU: the wall
 // and far-field rows, the branch-cut columns and the corners (bcu.f90:10-51)
-__device__ inline void own_ghosts(double* F, const Case2D& cs, int i, int j, long c, double v,
-                                  const double* uwall, const double* ufar) {
+template <class P, class B>
+__device__ inline void own_ghosts(P F, const Case2D& cs, int i, int j, long c, double v, B uwall,
+                                  B ufar) {
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
   double gw = 0.0, gf = 0.0;
   if (i == 1) { gw = -v + 2.0 * uwall[j]; F[c - 1] = gw; }
@@ -655,15 +672,15 @@ __global__ void __launch_bounds__(256) k2_correct(const Case2D* __restrict__ tab
   const int i = t.i, j = t.j;
   if (i > ni - 1 || j > nj - 1) return;
   const long c = (long)i + (long)sj * j;
-  const double dj = cs.f[G_DJAC][c];
-  double* U = cs.f[F_U];
-  double* V = cs.f[F_V];
-  const double u = U[c] - cs.f[F_SU][c] * cs.dt / dj;
-  const double v = V[c] - cs.f[F_SV][c] * cs.dt / dj;
+  const double dj = glb(cs.f[G_DJAC])[c];
+  const auto U = glb(cs.f[F_U]);
+  const auto V = glb(cs.f[F_V]);
+  const double u = U[c] - glb(cs.f[F_SU])[c] * cs.dt / dj;
+  const double v = V[c] - glb(cs.f[F_SV])[c] * cs.dt / dj;
   U[c] = u;
   V[c] = v;
-  own_ghosts(U, cs, i, j, c, u, cs.b[B_UWALL], cs.b[B_UFAR]);
-  own_ghosts(V, cs, i, j, c, v, cs.b[B_VWALL], cs.b[B_VFAR]);
+  own_ghosts(U, cs, i, j, c, u, glb(cs.b[B_UWALL]), glb(cs.b[B_UFAR]));
+  own_ghosts(V, cs, i, j, c, v, glb(cs.b[B_VWALL]), glb(cs.b[B_VFAR]));
 }
 
 // CONTRA_VEL of the corrected velocity and the source of ITER_P (iter_p.f90:29-32):
@@ -677,15 +694,15 @@ __global__ void __launch_bounds__(256) k2_prhs(const Case2D* __restrict__ tab,
   const int i = t.i, j = t.j;
   if (i > ni - 1 || j > nj - 1) return;
   const long c = (long)i + (long)sj * j;
-  const double* U = cs.f[F_U];
-  const double* V = cs.f[F_V];
-  const Geom g{cs.f[G_DJAC], cs.f[G_XIX], cs.f[G_XIY], cs.f[G_ETX], cs.f[G_ETY]};
+  const auto U = glb(cs.f[F_U]);
+  const auto V = glb(cs.f[F_V]);
+  const GeomG g = geom2(cs);
   const double j1 = ju1(g, U, V, c), j2 = ju2(g, U, V, c);
   const double ue = 0.5 * (j1 + ju1(g, U, V, c + 1));
   const double uw = i == 1 ? 0.0 : 0.5 * (j1 + ju1(g, U, V, c - 1));
   const double vn = 0.5 * (j2 + ju2(g, U, V, c + sj));
   const double vs = 0.5 * (j2 + ju2(g, U, V, c - sj));
-  cs.f[F_RHS][c] = 1.0 / cs.dt * (ue - uw + vn - vs);
+  glb(cs.f[F_RHS])[c] = 1.0 / cs.dt * (ue - uw + vn - vs);
 }
 
 // BCP (bcp.f90:12-43) of p: wall Neumann p(0) = p(1), outflow cubic extrapolation
@@ -695,7 +712,7 @@ __global__ void __launch_bounds__(256) k2_bcp(const Case2D* __restrict__ tab, in
   const Case2D& cs = tab[blockIdx.x];
   if (!force && C2_DONE(cs)) return;
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
-  double* P = cs.f[F_P];
+  const auto P = glb(cs.f[F_P]);
   for (int j = 1 + threadIdx.x; j <= nj - 1; j += blockDim.x) {
     const long c = (long)sj * j;
     P[c] = P[c + 1];
@@ -724,21 +741,21 @@ __global__ void __launch_bounds__(256) k2_residual(const Case2D* __restrict__ ta
   double ra = 0.0, rb = 0.0;
   if (i <= ni - 1 && j <= nj - 1) {
     const long c = (long)i + (long)sj * j;
-    const double* P = cs.f[F_P];
-    const double* de = cs.f[G_DE];
-    const double* dn = cs.f[G_DN];
+    const auto P = glb(cs.f[F_P]);
+    const auto de = glb(cs.f[G_DE]);
+    const auto dn = glb(cs.f[G_DN]);
     const double dE = de[c], dW = de[c - 1], dN = dn[c], dS = dn[c - sj];
     double ap = -dE - dW - dN - dS;
     double ae = -dE, aw = -dW;
     const double an = -dN, as = -dS;
     if (i == 1) { ap = ap - aw; aw = 0.0; }
     if (i == ni - 1) ae = 0.0;
-    const double su = cs.f[F_RHS][c] - cross2(cs.f[G_JQ12], P, c, sj);
+    const double su = glb(cs.f[F_RHS])[c] - cross2(glb(cs.f[G_JQ12]), P, c, sj);
     const double r = ae * P[c + 1] + aw * P[c - 1] + an * P[c + sj] + as * P[c - sj] + su -
                      ap * P[c];
     ra = fabs(r);
     rb = fabs(ap * P[c]);
-    ((T*)(sizeof(T) == 8 ? cs.R : cs.R32))[(i - 1) + (long)(ni - 1) * (j - 1)] = (T)r;
+    glb((T*)(sizeof(T) == 8 ? cs.R : cs.R32))[(i - 1) + (long)(ni - 1) * (j - 1)] = (T)r;
   }
   ra = wg_sum(ra, sh);
   rb = wg_sum(rb, sh);
@@ -794,14 +811,14 @@ __global__ void __launch_bounds__(256) k2_fft_fwd(const Case2D* __restrict__ tab
   const Case2D& cs = tab[bcase[blockIdx.x]];
   if (C2_DONE(cs)) return;
   const int N = cs.pj.n, MJ = cs.mj, n1 = cs.ni - 1;
-  const T* R = (const T*)(sizeof(T) == 8 ? cs.R : cs.R32);
-  Cx<T>* S = (Cx<T>*)(sizeof(T) == 8 ? cs.S : cs.S32);
-  const Cx<T>* twg = (const Cx<T>*)(sizeof(T) == 8 ? cs.twj : cs.twj32);
+  const auto R = glb((const T*)(sizeof(T) == 8 ? cs.R : cs.R32));
+  const auto S = glb((T*)(sizeof(T) == 8 ? cs.S : cs.S32));          // complex, interleaved
+  const auto twg = glb((const T*)(sizeof(T) == 8 ? cs.twj : cs.twj32));
   Cx<T>* a = (Cx<T>*)smem;
   Cx<T>* b = a + (size_t)ncol * N;
   Cx<T>* tw = b + (size_t)ncol * N;
   const int i0 = btile[blockIdx.x] * 2 * ncol;
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = cx_ld<T>(twg, k);
   for (int q = threadIdx.x; q < ncol * N; q += blockDim.x) {
     const int col = q % ncol, j = q / ncol;
     const int i = i0 + 2 * col;
@@ -821,8 +838,8 @@ __global__ void __launch_bounds__(256) k2_fft_fwd(const Case2D* __restrict__ tab
     if (i >= n1) continue;
     const Cx<T> zm = Z[m * ncol + col], zc = Z[((N - m) % N) * ncol + col];
     const long o = (long)i + (long)n1 * m;
-    S[o] = Cx<T>{(T)0.5 * (zm.x + zc.x), (T)0.5 * (zm.y - zc.y)};
-    if (i + 1 < n1) S[o + 1] = Cx<T>{(T)0.5 * (zm.y + zc.y), (T)0.5 * (zc.x - zm.x)};
+    cx_st<T>(S, o, Cx<T>{(T)0.5 * (zm.x + zc.x), (T)0.5 * (zm.y - zc.y)});
+    if (i + 1 < n1) cx_st<T>(S, o + 1, Cx<T>{(T)0.5 * (zm.y + zc.y), (T)0.5 * (zc.x - zm.x)});
   }
 }
 
@@ -835,10 +852,10 @@ __global__ void __launch_bounds__(64) k2_modes(const Case2D* __restrict__ tab,
   if (C2_DONE(cs)) return;
   const int n = cs.ni - 1;
   const long base = (long)n * btile[blockIdx.x];
-  Cx<T>* Z = (Cx<T>*)(sizeof(T) == 8 ? cs.S : cs.S32);
-  const double* dW = cs.pdw;
-  const double* invd = cs.invd;
-  const double* bf = cs.bf;
+  const auto Z = glb((T*)(sizeof(T) == 8 ? cs.S : cs.S32));  // complex, interleaved
+  const auto dW = glb((const double*)cs.pdw);
+  const auto invd = glb((const double*)cs.invd);
+  const auto bf = glb((const double*)cs.bf);
   const int l = threadIdx.x;
   const int r0 = l * R;
   double alx[R], aly[R], be[R], bb[R];
@@ -848,7 +865,7 @@ __global__ void __launch_bounds__(64) k2_modes(const Case2D* __restrict__ tab,
     const int r = r0 + q;
     if (r < n) {
       const double iv = invd[base + r];
-      const Cx<T> z = Z[base + r];
+      const Cx<T> z = cx_ld<T>(Z, base + r);
       alx[q] = (double)z.x * iv;
       aly[q] = (double)z.y * iv;
       be[q] = r == 0 ? 0.0 : -dW[r] * iv;
@@ -898,7 +915,7 @@ __global__ void __launch_bounds__(64) k2_modes(const Case2D* __restrict__ tab,
       const double xx = yx[q] - bb[q] * xx_in, xy = yy[q] - bb[q] * xy_in;
       xx_in = xx;
       xy_in = xy;
-      Z[base + r] = Cx<T>{(T)xx, (T)xy};
+      cx_st<T>(Z, base + r, Cx<T>{(T)xx, (T)xy});
     }
   }
 }
@@ -913,22 +930,22 @@ __global__ void __launch_bounds__(256) k2_fft_inv_add(const Case2D* __restrict__
   const Case2D& cs = tab[bcase[blockIdx.x]];
   if (C2_DONE(cs)) return;
   const int N = cs.pj.n, MJ = cs.mj, n1 = cs.ni - 1, sj = cs.sj;
-  const Cx<T>* S = (const Cx<T>*)(sizeof(T) == 8 ? cs.S : cs.S32);
-  const Cx<T>* twg = (const Cx<T>*)(sizeof(T) == 8 ? cs.twj : cs.twj32);
-  double* P = to_mu ? cs.a[A_MU] : cs.f[F_P];  // the adjoint solve corrects its own solution
+  const auto S = glb((const T*)(sizeof(T) == 8 ? cs.S : cs.S32));  // complex, interleaved
+  const auto twg = glb((const T*)(sizeof(T) == 8 ? cs.twj : cs.twj32));
+  const auto P = glb(to_mu ? cs.a[A_MU] : cs.f[F_P]);  // the adjoint corrects its own solution
   Cx<T>* a = (Cx<T>*)smem;
   Cx<T>* b = a + (size_t)ncol * N;
   Cx<T>* tw = b + (size_t)ncol * N;
   const int i0 = btile[blockIdx.x] * 2 * ncol;
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = cx_ld<T>(twg, k);
   for (int q = threadIdx.x; q < ncol * MJ; q += blockDim.x) {
     const int col = q % ncol, m = q / ncol;
     const int i = i0 + 2 * col;
     Cx<T> A{(T)0, (T)0}, B{(T)0, (T)0};
     if (i < n1) {
       const long o = (long)i + (long)n1 * m;
-      A = S[o];
-      if (i + 1 < n1) B = S[o + 1];
+      A = cx_ld<T>(S, o);
+      if (i + 1 < n1) B = cx_ld<T>(S, o + 1);
     }
     if (m == 0 || 2 * m == N) { A.y = (T)0; B.y = (T)0; }
     a[m * ncol + col] = Cx<T>{A.x - B.y, A.y + B.x};
@@ -961,21 +978,21 @@ __global__ void __launch_bounds__(256) k2_project(const Case2D* __restrict__ tab
   double cfl = 0.0;
   if (i <= ni - 1 && j <= nj - 1) {
     const long c = (long)i + (long)sj * j;
-    const Geom g{cs.f[G_DJAC], cs.f[G_XIX], cs.f[G_XIY], cs.f[G_ETX], cs.f[G_ETY]};
-    const double* P = cs.f[F_P];
+    const GeomG g = geom2(cs);
+    const auto P = glb(cs.f[F_P]);
     const double su = psrc(g.dj, g.xix, g.etx, P, c, sj);
     const double sv = psrc(g.dj, g.xiy, g.ety, P, c, sj);
-    double* U = cs.f[F_U];
-    double* V = cs.f[F_V];
+    const auto U = glb(cs.f[F_U]);
+    const auto V = glb(cs.f[F_V]);
     const double dj = g.dj[c];
     const double u = U[c] + su * cs.dt / dj;
     const double v = V[c] + sv * cs.dt / dj;
     U[c] = u;
     V[c] = v;
-    cs.f[F_SU][c] = su;
-    cs.f[F_SV][c] = sv;
-    own_ghosts(U, cs, i, j, c, u, cs.b[B_UWALL], cs.b[B_UFAR]);
-    own_ghosts(V, cs, i, j, c, v, cs.b[B_VWALL], cs.b[B_VFAR]);
+    glb(cs.f[F_SU])[c] = su;
+    glb(cs.f[F_SV])[c] = sv;
+    own_ghosts(U, cs, i, j, c, u, glb(cs.b[B_UWALL]), glb(cs.b[B_UFAR]));
+    own_ghosts(V, cs, i, j, c, v, glb(cs.b[B_VWALL]), glb(cs.b[B_VFAR]));
     cfl = cs.dt * (fabs(g.xix[c] * u + g.xiy[c] * v) + fabs(g.etx[c] * u + g.ety[c] * v));
   }
   cfl = wg_max(cfl, sh);

@@ -52,19 +52,30 @@ __device__ inline double wg_max(double v, double* sh) {
 
 // DJAC x contravariant velocity (contra_vel.f90:9-10: U1 = XIX U + XIY V, U2 = ETX U + ETY V;
 // the face fluxes carry DJAC U1)
-struct Geom {
-  const double *dj, *xix, *xiy, *etx, *ety;
+// (the helpers below take generic or global-address-space pointers alike)
+template <class P>
+struct GeomT {
+  P dj, xix, xiy, etx, ety;
 };
-__device__ inline double ju1(const Geom& g, const double* U, const double* V, long q) {
+using Geom = GeomT<const double*>;
+using GeomG = GeomT<gptr<const double>>;
+__device__ inline GeomG geom2(const Case2D& cs) {
+  return GeomG{glb((const double*)cs.f[G_DJAC]), glb((const double*)cs.f[G_XIX]),
+               glb((const double*)cs.f[G_XIY]), glb((const double*)cs.f[G_ETX]),
+               glb((const double*)cs.f[G_ETY])};
+}
+template <class G, class P>
+__device__ inline double ju1(const GeomT<G>& g, P U, P V, long q) {
   return g.dj[q] * (g.xix[q] * U[q] + g.xiy[q] * V[q]);
 }
-__device__ inline double ju2(const Geom& g, const double* U, const double* V, long q) {
+template <class G, class P>
+__device__ inline double ju2(const GeomT<G>& g, P U, P V, long q) {
   return g.dj[q] * (g.etx[q] * U[q] + g.ety[q] * V[q]);
 }
 
 // NL(IN, OUT) (nl.f90:21-27) with the face fluxes UE, UW, VN, VS
-__device__ inline double nl2(const double* F, long c, int sj, double ue, double uw, double vn,
-                             double vs) {
+template <class P>
+__device__ inline double nl2(P F, long c, int sj, double ue, double uw, double vn, double vs) {
   const double p = F[c];
   double t = ue * 0.5 * (p + F[c + 1]);
   t = t - uw * 0.5 * (p + F[c - 1]);
@@ -75,7 +86,8 @@ __device__ inline double nl2(const double* F, long c, int sj, double ue, double 
 
 // DIF_DIAGONAL-type cross term 0.25 (w_E d_J(F)_E - w_W d_J(F)_W + w_N d_I(F)_N - w_S d_I(F)_S)
 // (dif_diagonal.f90:13-23 with w = DJAC Q12; iter_u.f90:30-41 with w = DJAC XI_s ET_s)
-__device__ inline double cross2(const double* __restrict__ w, const double* F, long c, int sj) {
+template <class W, class P>
+__device__ inline double cross2(W __restrict__ w, P F, long c, int sj) {
   double t = w[c + 1] * (F[c + 1 + sj] - F[c + 1 - sj]);
   t = t - w[c - 1] * (F[c - 1 + sj] - F[c - 1 - sj]);
   t = t + w[c + sj] * (F[c + 1 + sj] - F[c - 1 + sj]);
@@ -84,8 +96,9 @@ __device__ inline double cross2(const double* __restrict__ w, const double* F, l
 }
 
 // DIF(IN, OUT) (dif.f90:13-23): orthogonal part with DW(i) = DE(i-1), DS(j) = DN(j-1)
-__device__ inline double dif2(const double* __restrict__ de, const double* __restrict__ dn,
-                              const double* __restrict__ jq12, const double* F, long c, int sj) {
+template <class W, class P>
+__device__ inline double dif2(W __restrict__ de, W __restrict__ dn, W __restrict__ jq12, P F,
+                              long c, int sj) {
   const double p = F[c];
   double t = de[c] * (F[c + 1] - p) + de[c - 1] * (F[c - 1] - p);
   t = t + dn[c] * (F[c + sj] - p);
@@ -96,8 +109,9 @@ __device__ inline double dif2(const double* __restrict__ de, const double* __res
 
 // P_SOURCE (p_source.f90:9-27): S = -0.5 ((DJ XI)_E p_E - (DJ XI)_W p_W + (DJ ET)_N p_N -
 // (DJ ET)_S p_S)
-__device__ inline double psrc(const double* __restrict__ dj, const double* __restrict__ mxi,
-                              const double* __restrict__ met, const double* P, long c, int sj) {
+template <class W, class Q>
+__device__ inline double psrc(W __restrict__ dj, W __restrict__ mxi, W __restrict__ met, Q P,
+                              long c, int sj) {
   double t = dj[c + 1] * mxi[c + 1] * P[c + 1];
   t = t - dj[c - 1] * mxi[c - 1] * P[c - 1];
   t = t + dj[c + sj] * met[c + sj] * P[c + sj];

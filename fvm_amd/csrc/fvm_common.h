@@ -47,6 +47,11 @@ typedef const __attribute__((address_space(1))) double* GP;
 typedef __attribute__((address_space(1))) double* GPW;
 __device__ inline GP gp(const double* p) { return (GP)p; }
 __device__ inline GPW gpw(double* p) { return (GPW)p; }
+// the same for any element type: glb(p) is p in the global address space
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> glb(T* p) { return (gptr<T>)p; }
 
 // Metric a of the table (grouped per Cartesian component: xi_s, eta_s, zeta_s for s = x, y, z)
 // at cell c.  With XZ0 the entries xi_z (a = 6) and eta_z (a = 7), which are exactly zero on
