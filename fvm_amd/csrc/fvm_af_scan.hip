@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256) k_scan_q1v(Dims d, PtrSet8 qn, double* co
   const int comp = (int)(blockIdx.z / (d.nk - 1));
   const int t = threadIdx.x;
   const T* __restrict__ src = reinterpret_cast<const T*>(qn.p[comp]);
-  T* __restrict__ dst = reinterpret_cast<T*>(q[comp]);
+  const gptr<T> dst = glb(reinterpret_cast<T*>(q[comp]));
   for (int r = 0; r < 4; ++r) {
     const int li = t % 32, lj = t / 32 + 8 * r;
     const int i = i0 + li, j = j0 + lj;
@@ -128,8 +128,8 @@ __global__ void __launch_bounds__(256) k_scan_unpack1(Dims d, const double* cons
   const int comp = (int)(blockIdx.z / (d.nk - 1));
   const int t = threadIdx.x;
   const long lsa = d.nj + 1, lsb = (long)(d.nj + 1) * (d.ni + 1);
-  const T* __restrict__ src = reinterpret_cast<const T*>(xo[comp]);
-  double* __restrict__ dst = f[comp];
+  const gptr<const T> src = glb(reinterpret_cast<const T*>(xo[comp]));
+  const gptr<double> dst = glb(f[comp]);
   for (int r = 0; r < 4; ++r) {
     const int lj = t % 32, li = t / 32 + 8 * r;
     const int i = i0 + li, j = j0 + lj;
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const T* __r
   const long lin = blockIdx.x;
   const long w = (lin % 8) * (G / 8) + lin / 8;  // XCD-aware: neighbouring items share an XCD
   const int l = threadIdx.x;
-  T* const sink = reinterpret_cast<T*>(sinkd);
+  const gptr<T> sink = glb(reinterpret_cast<T*>(sinkd));
   double dmax = 0.0, xmax = 0.0;
   if (w < (long)A.nc * nchunk * g.nb) {
     const int comp = (int)(w % A.nc);
@@ -234,9 +234,9 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const T* __r
     const int a0 = 1 + ca * SC_C;
     const int a1 = min(a0 + SC_C - 1, g.na);
     const int p0 = l * R + 1;
-    const T* __restrict__ qc = reinterpret_cast<const T*>(qt[comp]);
+    const gptr<const T> qc = glb(reinterpret_cast<const T*>(qt[comp]));
     const GP fc = gp(A.f[comp]);
-    T* __restrict__ xc = reinterpret_cast<T*>(xot[comp]);
+    const gptr<T> xc = glb(reinterpret_cast<T*>(xot[comp]));
     T xp[R];
     // line a0-1: the stage input's ghost line for the first chunk, else the preceding
     // chunk's last line of the previous pass
@@ -322,7 +322,7 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const T* __r
         dl = in ? fmax(dl, fabs(x - xo[r])) : dl;
         xrun = in ? fmax(xrun, fabs(x)) : xrun;
         // a row past n stores to this lane's slot of the sink instead of a branch
-        T* const dst = in ? xc + (lb + p) : sink + l;
+        const gptr<T> dst = in ? xc + (lb + p) : sink + l;
         *dst = x;
         xp[r] = in ? x : xp[r];
       }
