@@ -207,15 +207,14 @@ __device__ inline void bcu_ghosts_wave(P F, int ni, int nj, int sj, B uwall, B u
 template <int R, typename T>
 struct LineOps {
   T cdn[R], q[R], iv[R], na[R], bf[R], xo[R];
-  double x0[R];                                    // the field, where the change is added
   T g0, g1;                                        // ghosts before / after the line
 };
 
 template <int R, bool EARLY, typename T>
 __device__ inline void sweep_load(LineOps<R, T>& o, gptr<const T> F, gptr<const T> Q,
                                   gptr<const T> invd, gptr<const T> na, gptr<const T> bf,
-                                  gptr<const T> cdn, gptr<const double> F64, long base, long rs,
-                                  long dnoff, int n, int r0, int l) {
+                                  gptr<const T> cdn, long base, long rs, long dnoff, int n,
+                                  int r0, int l) {
   constexpr bool DELTA = std::is_same<T, float>::value;
 #pragma unroll
   for (int q = 0; q < R; ++q) {
@@ -228,7 +227,6 @@ __device__ inline void sweep_load(LineOps<R, T>& o, gptr<const T> F, gptr<const 
       o.na[q] = na[c];
       o.bf[q] = bf[c];
       o.xo[q] = EARLY ? F[c] : (T)0;  // the previous pass's value of the row
-      o.x0[q] = F64 ? F64[c] : 0.0;
     }
   }
   // the change form's ghosts do not change
@@ -296,14 +294,14 @@ __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double
     prev[q] = v;
   }
   LineOps<R, T> cur, nxt;
-  sweep_load<R, EARLY>(cur, F, Q, invd, na, bf, cdn, F64, ls * a0, rs, dnoff, n, r0, l);
+  sweep_load<R, EARLY>(cur, F, Q, invd, na, bf, cdn, ls * a0, rs, dnoff, n, r0, l);
   bool stopped = false;
   T xrun = (T)0;
   int L = a0;
   for (; L <= a1; ++L) {
     const long base = ls * L;
     if (L < a1)
-      sweep_load<R, EARLY>(nxt, F, Q, invd, na, bf, cdn, F64, base + ls, rs, dnoff, n, r0, l);
+      sweep_load<R, EARLY>(nxt, F, Q, invd, na, bf, cdn, base + ls, rs, dnoff, n, r0, l);
     T al[R], be[R], bb[R];
     T At = (T)0, Bt = (T)1;
 #pragma unroll
@@ -347,7 +345,7 @@ __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double
         prev[q] = x;
         const long c = base + rs * (r + 1);
         if (add)
-          F64[c] = cur.x0[q] + (double)x;
+          F64[c] = F64[c] + (double)x;
         else
           F[c] = x;
         if (EARLY) {
