@@ -128,11 +128,14 @@ def run_mode(w, mode, a, log, af_mode=None):
     s.sync()
     prof = s.profile_get()
     s.profile(False)
+    div = s.divergence()  # the face-flux divergence of the last step's velocity, beside RESIP
     for q, info in enumerate(infos):
         log(f"timed step {q + 1}/{a.steps} mode {mode}: poisson passes {info['niter']} "
             f"resip {info['resip']:.2e} af passes {info['af_iters']} cfl {info['cfl']:.3f} "
             f"cd {info['cd']:.4f}")
-    return dict(secs=t1 - t0, infos=infos, prof=prof)
+    log(f"mode {mode}: face-flux divergence of the last step's velocity: max {div['max']:.3e} "
+        f"mean {div['mean']:.3e} relative {div['rel']:.3e}")
+    return dict(secs=t1 - t0, infos=infos, prof=prof, div=div)
 
 
 def roofline(prof, ncell, af32=False):
@@ -491,6 +494,7 @@ def main():
         out[f"{mname}_mcell_steps_per_s"] = ncell * a.steps / rr["secs"] / 1e6
         out[f"{mname}_poisson_passes_per_step"] = float(np.mean([i["niter"] for i in rr["infos"]]))
         out[f"{mname}_resip_max"] = float(max(i["resip"] for i in rr["infos"]))
+        out[f"{mname}_div_norm"] = rr["div"]
     if "mixed" in results and "fp64" in results:
         out["mixed_over_fp64"] = results["fp64"]["secs"] / results["mixed"]["secs"]
     out["kernels"] = [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in row.items()}

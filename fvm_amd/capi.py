@@ -68,7 +68,8 @@ class StepInfo(ctypes.Structure):
 EXPORTS = ("fvm_create", "fvm_destroy", "fvm_set_field", "fvm_get_field", "fvm_step",
            "fvm_begin_step", "fvm_contra_vel", "fvm_nl", "fvm_stress_tensor_dy", "fvm_dif",
            "fvm_p_source", "fvm_iter_u", "fvm_correct", "fvm_iter_p", "fvm_pressure_reference",
-           "fvm_calccoef", "fvm_profile_enable", "fvm_profile_get", "fvm_profile_reset",
+           "fvm_calccoef", "fvm_divergence", "fvm_profile_enable", "fvm_profile_get",
+           "fvm_profile_reset",
            "fvm_sync", "fvm_set_dtime", "fvm_create2d", "fvm_set_thermal", "fvm_iter_t",
            "fvm_writedata_fields", "fvm_post_begin", "fvm_post_finalize", "fvm_ensemble_step",
            "fvm_calccoef_noalloc", "fvm_copy_field", "fvm_nl_c", "fvm_dif_c",
@@ -116,6 +117,7 @@ def load(path: str = LIB_PATH):
     L.fvm_iter_p.argtypes = [vp, pd, ctypes.POINTER(ci)]
     L.fvm_pressure_reference.argtypes = [vp, pd]
     L.fvm_calccoef.argtypes = [vp, ctypes.POINTER(StepInfo)]
+    L.fvm_divergence.argtypes = [vp, pd]
     L.fvm_calccoef_noalloc.argtypes = [vp, ctypes.POINTER(StepInfo)]
     for f in ("fvm_copy_field", "fvm_nl_c", "fvm_dif_c", "fvm_add_sgs_c", "fvm_bcu"):
         getattr(L, f).argtypes = [vp, ci, ci]

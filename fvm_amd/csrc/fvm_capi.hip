@@ -780,6 +780,11 @@ int fvm_profile_get(fvm_ctx* c, char* names, double* ms, long* calls, int cap) {
   return q;
 }
 
+int fvm_divergence(fvm_ctx* c, double* out) {
+  if (!c || !out) return FVM_ERR_ARG;
+  return fvm_k_div_norm(c, out);
+}
+
 int fvm_sync(fvm_ctx* c) {
   if (!c) return FVM_ERR_ARG;
   prof_flush(c);

@@ -223,6 +223,7 @@ int fvm_k_sgs_out(FvmCtx* c);
 int fvm_k_nut_out(FvmCtx* c);
 int fvm_k_iter_u(FvmCtx* c);
 int fvm_k_iter_p(FvmCtx* c, double* resip, int* niter);
+int fvm_k_div_norm(FvmCtx* c, double* out);
 int fvm_k_bcp(FvmCtx* c);
 int fvm_sep_setup(FvmCtx* c);
 void fvm_sep_destroy(FvmCtx* c);

@@ -39,6 +39,70 @@ __global__ void k_p_rhs(Dims d, const double* __restrict__ ju0, const double* __
   rhs[c] = 1.0 / dtime * dv;
 }
 
+// The discrete divergence of the face volume fluxes (the source ITER_P drives to zero for the
+// corrected velocity, iter_p.f90:31-34) for the interior cells of one 64 x 4 x 1 block: per block
+// max |div|, sum |div| and the flux scale sum (|UE| + |VN| + |WT|), fixed-order trees.
+__global__ void __launch_bounds__(256) k_div_part(Dims d, const double* __restrict__ ju0,
+                                                  const double* __restrict__ ju1,
+                                                  const double* __restrict__ ju2, double* part) {
+  __shared__ double sh[3][256];
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  const int t = threadIdx.x + 64 * threadIdx.y;
+  double m = 0.0, a = 0.0, f = 0.0;
+  if (i <= d.ni - 1 && j <= d.nj - 1) {
+    const long c = cidx(d, i, j, k);
+    const double ue = 0.5 * (ju0[c] + ju0[c + 1]);
+    const double uw = i == 1 ? 0.0 : 0.5 * (ju0[c] + ju0[c - 1]);
+    const double vn = 0.5 * (ju1[c] + ju1[c + d.sj]);
+    const double vs = 0.5 * (ju1[c] + ju1[c - d.sj]);
+    const double wt = 0.5 * (ju2[c] + ju2[c + d.sk]);
+    const double wb = 0.5 * (ju2[c] + ju2[c - d.sk]);
+    const double dv = ue - uw + vn - vs + wt - wb;
+    m = fabs(dv);
+    a = fabs(dv);
+    f = fabs(ue) + fabs(vn) + fabs(wt);
+  }
+  sh[0][t] = m; sh[1][t] = a; sh[2][t] = f;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      sh[0][t] = fmax(sh[0][t], sh[0][t + s]);
+      sh[1][t] = sh[1][t] + sh[1][t + s];
+      sh[2][t] = sh[2][t] + sh[2][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    part[3 * b] = sh[0][0]; part[3 * b + 1] = sh[1][0]; part[3 * b + 2] = sh[2][0];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_div_final(const double* __restrict__ part, long nb,
+                                                   double* out) {
+  __shared__ double sh[3][256];
+  const int t = threadIdx.x;
+  double m = 0.0, a = 0.0, f = 0.0;
+  for (long q = t; q < nb; q += blockDim.x) {
+    m = fmax(m, part[3 * q]);
+    a = a + part[3 * q + 1];
+    f = f + part[3 * q + 2];
+  }
+  sh[0][t] = m; sh[1][t] = a; sh[2][t] = f;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      sh[0][t] = fmax(sh[0][t], sh[0][t + s]);
+      sh[1][t] = sh[1][t] + sh[1][t + s];
+      sh[2][t] = sh[2][t] + sh[2][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) { out[0] = sh[0][0]; out[1] = sh[1][0]; out[2] = sh[2][0]; }
+}
+
 template <typename T>
 __device__ inline double cross_p(const Dims& d, long c, const T* __restrict__ f,
                                  const double* __restrict__ jq12,
@@ -266,6 +330,29 @@ static int zebra(FvmCtx* c, T* p, const T* su) {
     hipLaunchKernelGGL(k_p_sweep<T>, g, dim3(64), 0, c->stream, d, p, su, c->pdI, c->pdJ,
                        c->pdK, colour);
   HIP_CHECK(hipGetLastError());
+  return FVM_OK;
+}
+
+// out: max |div|, mean |div| over the interior cells, sum |div| / sum (|UE| + |VN| + |WT|), of
+// the face fluxes of the current velocity (CONTRA_VEL recomputed; the next step recomputes it too)
+int fvm_k_div_norm(FvmCtx* c, double* out) {
+  const Dims d = c->d;
+  int r = fvm_k_contra_vel(c);
+  if (r) return r;
+  const dim3 g((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, d.nk - 1);
+  const long nb = (long)g.x * g.y * g.z;
+  if (3 * nb + 3 > (long)c->nred) return FVM_ERR_SIZE;
+  hipLaunchKernelGGL(k_div_part, g, kBlk, 0, c->stream, d, c->ju[0], c->ju[1], c->ju[2], c->red);
+  hipLaunchKernelGGL(k_div_final, dim3(1), dim3(256), 0, c->stream, c->red, nb, c->red + 3 * nb);
+  HIP_CHECK(hipGetLastError());
+  double h[3];
+  HIP_CHECK(hipMemcpyAsync(h, c->red + 3 * nb, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  const double ncell = (double)(d.ni - 1) * (d.nj - 1) * (d.nk - 1);
+  out[0] = h[0];
+  out[1] = h[1] / ncell;
+  out[2] = h[2] > 0.0 ? h[1] / h[2] : 0.0;
   return FVM_OK;
 }
 

@@ -137,6 +137,13 @@ class Solver3D:
         capi.check(self.L.fvm_calccoef(self.h, ctypes.byref(info)), "calccoef")
         return info.as_dict()
 
+    def divergence(self):
+        """Face-flux divergence of the current velocity: max |div|, mean |div| and
+        sum |div| / sum(|UE| + |VN| + |WT|) (reported beside RESIP)."""
+        out = np.zeros(3)
+        capi.check(self.L.fvm_divergence(self.h, capi._dptr(out)), "divergence")
+        return {"max": float(out[0]), "mean": float(out[1]), "rel": float(out[2])}
+
     # kernel timing ----------------------------------------------------------------------
     def profile(self, on=True):
         capi.check(self.L.fvm_profile_enable(self.h, 1 if on else 0), "profile_enable")

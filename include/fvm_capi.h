@@ -281,6 +281,12 @@ int fvm_iter_p(fvm_ctx* ctx, double* resip, int* niter); /* ITER_P + BCP (iter_p
                                                      lisolv.f90:1-57, bcp.f90:1-38)         */
 int fvm_pressure_reference(fvm_ctx* ctx, double* pref); /* RLES/main.f90:125-135          */
 int fvm_calccoef(fvm_ctx* ctx, fvm_step_info* info);    /* CALCCOEF (calccoef.f90:1-201)  */
+/* The discrete divergence of the face volume fluxes of the current velocity (CONTRA_VEL's UE..WB,
+ * RLES/contra_vel.f90:13-56, summed as ITER_P's source, iter_p.f90:31-34), reported beside RESIP:
+ * out[0] max |div|, out[1] mean |div| over the interior cells, out[2] sum |div| / sum (|UE| + |VN|
+ * + |WT|).  No counterpart in the reference (it prints RESIP only); recomputes the context's face
+ * fluxes, which the next step recomputes anyway. */
+int fvm_divergence(fvm_ctx* ctx, double* out);
 
 /* fvm_calccoef without any device allocation: the same kernels and results, with the CFL and
  * force partial sums in a static device array of 1 MiB (131072 doubles) per GPU instead of the
