@@ -348,9 +348,15 @@ def run_ensemble(a, log):
         f"{max(i['cfl'] for i in infos):.3f}")
     for q, c in enumerate(cases):
         b.set_dtime(q, c["dt"])
-    tune = b.autotune(reps=3)
-    log(f"autotuner: tile height {tune['tile_j']}, FFT columns {tune['fft_cols']}; "
-        + ", ".join(f"{k} {v:.3f} ms" for k, v in sorted(tune["ms_per_step"].items())))
+    if a.ens_tuning:  # a fixed tile choice instead of the autotuner's timed trials
+        tj, nc = (int(x) for x in a.ens_tuning.split(","))
+        b.set_tuning(tj, nc)
+        tune = {"tile_j": tj, "fft_cols": nc, "ms_per_step": {}}
+        log(f"tuning given: tile height {tj}, FFT columns {nc}")
+    else:
+        tune = b.autotune(reps=3)
+        log(f"autotuner: tile height {tune['tile_j']}, FFT columns {tune['fft_cols']}; "
+            + ", ".join(f"{k} {v:.3f} ms" for k, v in sorted(tune["ms_per_step"].items())))
     for q in range(a.warmup):
         infos = b.step(1)
         log(f"warmup step {q + 1}/{a.warmup}: max cfl {max(i['cfl'] for i in infos):.3f} "
@@ -433,6 +439,9 @@ def main():
                     help="ensemble2d: mixed: FP32 Poisson corrections and the AF sweeps in the "
                          "FP32 change form; mixed-poisson: the Poisson corrections only; fp64: "
                          "neither")
+    ap.add_argument("--ens-tuning", default="", metavar="TJ,NCOL",
+                    help="ensemble2d: use this tile choice (stencil tile height, FFT columns) "
+                         "instead of the autotuner (profiled runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB four times (HBM counter calibration) and exit")
     a = ap.parse_args()

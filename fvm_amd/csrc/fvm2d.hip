@@ -45,6 +45,7 @@ __global__ void __launch_bounds__(256) k2_explicit(const Case2D* __restrict__ ta
                                                    const int* __restrict__ bcase,
                                                    const int* __restrict__ btile) {
   const TileIdx t = tile_index(bcase, btile, tab);
+  if (t.c < 0) return;
   const Case2D& cs = tab[t.c];
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
   const int i = t.i, j = t.j;
@@ -591,6 +592,7 @@ __global__ void __launch_bounds__(256) k2_rhs2(const Case2D* __restrict__ tab,
                                                const int* __restrict__ bcase,
                                                const int* __restrict__ btile) {
   const TileIdx t = tile_index(bcase, btile, tab);
+  if (t.c < 0) return;
   const Case2D& cs = tab[t.c];
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
   const int i = t.i, j = t.j;
@@ -665,6 +667,7 @@ __global__ void __launch_bounds__(256) k2_correct(const Case2D* __restrict__ tab
                                                   const int* __restrict__ bcase,
                                                   const int* __restrict__ btile) {
   const TileIdx t = tile_index(bcase, btile, tab);
+  if (t.c < 0) return;
   const Case2D& cs = tab[t.c];
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
   const int i = t.i, j = t.j;
@@ -687,6 +690,7 @@ __global__ void __launch_bounds__(256) k2_prhs(const Case2D* __restrict__ tab,
                                                const int* __restrict__ bcase,
                                                const int* __restrict__ btile) {
   const TileIdx t = tile_index(bcase, btile, tab);
+  if (t.c < 0) return;
   const Case2D& cs = tab[t.c];
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
   const int i = t.i, j = t.j;
@@ -732,6 +736,7 @@ __global__ void __launch_bounds__(256) k2_residual(const Case2D* __restrict__ ta
                                                    const int* __restrict__ btile) {
   __shared__ double sh[256];
   const TileIdx t = tile_index(bcase, btile, tab);
+  if (t.c < 0) return;
   const Case2D& cs = tab[t.c];
   if (C2_DONE(cs)) return;  // uniform over the workgroup
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
@@ -970,6 +975,7 @@ __global__ void __launch_bounds__(256) k2_project(const Case2D* __restrict__ tab
                                                   const int* __restrict__ btile) {
   __shared__ double sh[256];
   const TileIdx t = tile_index(bcase, btile, tab);
+  if (t.c < 0) return;
   const Case2D& cs = tab[t.c];
   const int ni = cs.ni, nj = cs.nj, sj = cs.sj;
   const int i = t.i, j = t.j;

@@ -7,7 +7,8 @@
 
 #define C2_DONE(cs) ((cs).stat[ST_DONE] != 0.0)
 
-// A 64 x TJ (I x J) tile of interior cells; blockIdx.x indexes the block table.
+// A 64 x TJ (I x J) tile of interior cells; blockIdx.x indexes the block table (case -1: a
+// padding entry, no tile).
 struct TileIdx {
   int c, i, j, tile;
 };
@@ -17,6 +18,10 @@ __device__ inline TileIdx tile_index(const int* __restrict__ bcase, const int* _
   TileIdx t;
   t.c = bcase[blockIdx.x];
   t.tile = btile[blockIdx.x];
+  if (t.c < 0) {  // padding entry of an XCD-ordered table: the caller returns
+    t.i = t.j = 0;
+    return t;
+  }
   const int tx = (tab[t.c].ni - 1 + 63) >> 6;
   t.i = 1 + (t.tile % tx) * 64 + (int)threadIdx.x;
   t.j = 1 + (t.tile / tx) * (int)blockDim.y + (int)threadIdx.y;
