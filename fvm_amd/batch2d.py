@@ -186,10 +186,10 @@ class Batch2D:
                 self.set_tuning(stored[sig]["tile_j"], stored[sig]["fft_cols"])
                 return dict(stored[sig], cached=True)
         ch = (ctypes.c_int * 2)()
-        ms = (ctypes.c_double * 15)()
+        ms = (ctypes.c_double * 20)()
         capi.check(self.L.fvm_batch2d_autotune(self.h, int(reps), ch, ms), "batch2d autotune")
         res = {"tile_j": ch[0], "fft_cols": ch[1],
-               "ms_per_step": {f"{tj}x{nc}": ms[a * 5 + b] for a, tj in enumerate((1, 2, 4))
+               "ms_per_step": {f"{tj}x{nc}": ms[a * 5 + b] for a, tj in enumerate((1, 2, 4, 8))
                                for b, nc in enumerate((1, 2, 4, 8, 16)) if ms[a * 5 + b] >= 0}}
         if cache:
             stored = {}

@@ -41,7 +41,7 @@ __device__ __forceinline__ void cx_st(gptr<T> p, long o, Cx<T> v) {
 // stage input against the stage-1 line operator, in FP32, for the change form of the sweep.
 // ---------------------------------------------------------------------------------------------
 template <bool AF32>
-__global__ void __launch_bounds__(256) k2_explicit(const Case2D* __restrict__ tab,
+__global__ void __launch_bounds__(512) k2_explicit(const Case2D* __restrict__ tab,
                                                    const int* __restrict__ bcase,
                                                    const int* __restrict__ btile) {
   const TileIdx t = tile_index(bcase, btile, tab);
@@ -588,7 +588,7 @@ __global__ void __launch_bounds__(64) k2_sweep_pass(const Case2D* __restrict__ t
 // (iter_u.f90:84-98) and BCU's folds with the stage-2 AW(1), AE(NI-1) (bcu.f90:15, 34).  AF32:
 // instead of SU, the FP64 residual of the stage input against the stage-2 line operator, in FP32.
 template <bool AF32>
-__global__ void __launch_bounds__(256) k2_rhs2(const Case2D* __restrict__ tab,
+__global__ void __launch_bounds__(512) k2_rhs2(const Case2D* __restrict__ tab,
                                                const int* __restrict__ bcase,
                                                const int* __restrict__ btile) {
   const TileIdx t = tile_index(bcase, btile, tab);
@@ -663,7 +663,7 @@ __device__ inline void own_ghosts(P F, const Case2D& cs, int i, int j, long c, d
 
 // Predictor correction U = U - S_U DTIME / DJAC (Main.f90:80-84) and the BCU ghosts
 // (Main.f90:86-87).
-__global__ void __launch_bounds__(256) k2_correct(const Case2D* __restrict__ tab,
+__global__ void __launch_bounds__(512) k2_correct(const Case2D* __restrict__ tab,
                                                   const int* __restrict__ bcase,
                                                   const int* __restrict__ btile) {
   const TileIdx t = tile_index(bcase, btile, tab);
@@ -686,7 +686,7 @@ __global__ void __launch_bounds__(256) k2_correct(const Case2D* __restrict__ tab
 
 // CONTRA_VEL of the corrected velocity and the source of ITER_P (iter_p.f90:29-32):
 // RHS = 1 / DTIME (UE - UW + VN - VS)
-__global__ void __launch_bounds__(256) k2_prhs(const Case2D* __restrict__ tab,
+__global__ void __launch_bounds__(512) k2_prhs(const Case2D* __restrict__ tab,
                                                const int* __restrict__ bcase,
                                                const int* __restrict__ btile) {
   const TileIdx t = tile_index(bcase, btile, tab);
@@ -731,10 +731,10 @@ __global__ void __launch_bounds__(256) k2_bcp(const Case2D* __restrict__ tab, in
 // AE(NI-1) = 0): r = AE p_E + AW p_W + AN p_N + AS p_S + SU - AP p, SU = RHS - DD(p), into the
 // compact array R [(i-1) + (NI-1)(j-1)], with the block's sums of |r| and |AP p|.
 template <typename T>
-__global__ void __launch_bounds__(256) k2_residual(const Case2D* __restrict__ tab,
+__global__ void __launch_bounds__(512) k2_residual(const Case2D* __restrict__ tab,
                                                    const int* __restrict__ bcase,
                                                    const int* __restrict__ btile) {
-  __shared__ double sh[256];
+  __shared__ double sh[512];  // up to 64 x 8 threads
   const TileIdx t = tile_index(bcase, btile, tab);
   if (t.c < 0) return;
   const Case2D& cs = tab[t.c];
@@ -970,10 +970,10 @@ __global__ void __launch_bounds__(256) k2_fft_inv_add(const Case2D* __restrict__
 
 // P_SOURCE(p^{n+1}), the projection U = U + S_U DTIME / DJAC (Main.f90:93-103), BCU's ghosts
 // and the block maxima of the local CFL number (calccoef.f90:9-17)
-__global__ void __launch_bounds__(256) k2_project(const Case2D* __restrict__ tab,
+__global__ void __launch_bounds__(512) k2_project(const Case2D* __restrict__ tab,
                                                   const int* __restrict__ bcase,
                                                   const int* __restrict__ btile) {
-  __shared__ double sh[256];
+  __shared__ double sh[512];  // up to 64 x 8 threads
   const TileIdx t = tile_index(bcase, btile, tab);
   if (t.c < 0) return;
   const Case2D& cs = tab[t.c];

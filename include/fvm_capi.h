@@ -222,9 +222,9 @@ int fvm_batch2d_step(fvm_batch2d* b, int nsteps, fvm_step_info* infos);
 /* The last nsteps steps' C_D, C_L, C_D pressure / friction, C_L pressure / friction, RESIP, CFL
  * per case, oldest first: out[(s n + c) 8 + q]. */
 int fvm_batch2d_history(fvm_batch2d* b, int nsteps, double* out);
-/* Tile choices {stencil tile height 1|2|4, FFT columns per workgroup 1..16}; the autotuner times
+/* Tile choices {stencil tile height 1|2|4|8, FFT columns per workgroup 1..16}; the autotuner times
  * every admissible pair with reps steps each (state saved and restored) and keeps the fastest;
- * ms_out (15 values, may be NULL) receives the timings. */
+ * ms_out (20 values, tile height major, may be NULL) receives the timings. */
 int fvm_batch2d_set_tuning(fvm_batch2d* b, const int* choice);
 int fvm_batch2d_get_tuning(fvm_batch2d* b, int* choice);
 int fvm_batch2d_autotune(fvm_batch2d* b, int reps, int* choice, double* ms_out);
