@@ -59,6 +59,14 @@ enum AB2 { AB_UFW, AB_VFW, AB_UFF, AB_VFF, NAB2 };
 enum H2 { H_RU, H_RV, H_DU, H_DV, H_TRU, H_TRV, H_INVD1, H_NA1, H_B1, H_AN1, H_TINVD2, H_TNA2,
           H_TB2, H_TAW2, NH2 };
 
+// The FP32 J-line-major planes (H_TRU, H_TRV, H_T*2 and H_DU / H_DV in the J-line stage) hold
+// cell (I, J) at (J + J32_OFF) + tls32(NJ) I: a line stride that is a multiple of four floats and
+// row J = 1 on a 16-byte boundary, so a lane's R consecutive rows (R = 2, 4, 6, 8) are read as
+// float2 / float4 vectors.
+constexpr int J32_OFF = 3;
+__host__ __device__ inline long tls32(int nj) { return ((long)nj + 1 + J32_OFF + 3) & ~3L; }
+__host__ __device__ inline long tidx32(int nj, int i, int j) { return j + J32_OFF + tls32(nj) * i; }
+
 struct Case2D {
   int ni, nj;          // node counts NI (radial), NJ (circumferential)
   int sj;              // stride of J = NI + 1

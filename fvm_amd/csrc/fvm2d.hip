@@ -211,23 +211,60 @@ struct LineOps {
   T g0, g1;                                        // ghosts before / after the line
 };
 
-template <int R, bool EARLY, typename T>
+// R consecutive floats from a 16-byte (R % 4 == 0) or 8-byte (R % 2 == 0) aligned address
+typedef float sw_f4 __attribute__((ext_vector_type(4)));
+typedef float sw_f2 __attribute__((ext_vector_type(2)));
+template <int R>
+__device__ __forceinline__ void load_rows(float (&d)[R], gptr<const float> p) {
+  if constexpr (R % 4 == 0) {
+#pragma unroll
+    for (int k = 0; k < R / 4; ++k) {
+      const sw_f4 v = ((gptr<const sw_f4>)p)[k];
+      d[4 * k] = v.x; d[4 * k + 1] = v.y; d[4 * k + 2] = v.z; d[4 * k + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) {
+      const sw_f2 v = ((gptr<const sw_f2>)p)[k];
+      d[2 * k] = v.x; d[2 * k + 1] = v.y;
+    }
+  }
+}
+
+// roff: the offset of row 0 in a line (the padded FP32 J-line-major layout)
+template <int R, int DIR, bool EARLY, typename T>
 __device__ inline void sweep_load(LineOps<R, T>& o, gptr<const T> F, gptr<const T> Q,
                                   gptr<const T> invd, gptr<const T> na, gptr<const T> bf,
-                                  gptr<const T> cdn, long base, long rs, long dnoff, int n,
-                                  int r0, int l) {
+                                  gptr<const T> cdn, long base, long rs, long roff, long dnoff,
+                                  int n, int r0, int l) {
   constexpr bool DELTA = std::is_same<T, float>::value;
+  if constexpr (DELTA && DIR == 1 && R % 2 == 0) {
+    // a lane's rows as vectors (row r at base + roff + 1 + r; dnoff = 0 in this direction).
+    // The last lane's rows past n lie inside the padded plane and are never used.
+    const long c0 = base + roff + 1 + r0;
+    load_rows<R>(o.cdn, cdn + c0);
+    load_rows<R>(o.q, Q + c0);
+    load_rows<R>(o.iv, invd + c0);
+    load_rows<R>(o.na, na + c0);
+    load_rows<R>(o.bf, bf + c0);
+    if (EARLY) load_rows<R>(o.xo, F + c0);
+    else {
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int r = r0 + q;
-    if (r < n) {
-      const long c = base + rs * (r + 1);
-      o.cdn[q] = cdn[c + dnoff];
-      o.q[q] = Q[c];
-      o.iv[q] = invd[c];
-      o.na[q] = na[c];
-      o.bf[q] = bf[c];
-      o.xo[q] = EARLY ? F[c] : (T)0;  // the previous pass's value of the row
+      for (int q = 0; q < R; ++q) o.xo[q] = (T)0;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int r = r0 + q;
+      if (r < n) {
+        const long c = base + roff + rs * (r + 1);
+        o.cdn[q] = cdn[c + dnoff];
+        o.q[q] = Q[c];
+        o.iv[q] = invd[c];
+        o.na[q] = na[c];
+        o.bf[q] = bf[c];
+        o.xo[q] = EARLY ? F[c] : (T)0;  // the previous pass's value of the row
+      }
     }
   }
   // the change form's ghosts do not change
@@ -273,7 +310,8 @@ __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double
   const int n = DIR == 0 ? ni - 1 : nj - 1;       // rows per line
   const int nl = DIR == 0 ? nj - 1 : ni - 1;      // lines
   const long rs = 1;                              // row stride
-  const long ls = DIR == 0 ? sj : nj + 1;         // line stride
+  const long ls = DIR == 0 ? sj : (DELTA ? tls32(nj) : nj + 1);  // line stride
+  const long roff = DIR == 1 && DELTA ? J32_OFF : 0;  // row 0's offset in a line
   const long dnoff = DIR == 0 ? -(long)sj : 0;    // lower coupling of row c at cdn[c + dnoff]
   const int nch = sw_nchunk(nl);
   const int a0 = 1 + ca * SW_C, a1 = min(a0 + SW_C - 1, nl);
@@ -295,14 +333,15 @@ __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double
     prev[q] = v;
   }
   LineOps<R, T> cur, nxt;
-  sweep_load<R, EARLY>(cur, F, Q, invd, na, bf, cdn, ls * a0, rs, dnoff, n, r0, l);
+  sweep_load<R, DIR, EARLY>(cur, F, Q, invd, na, bf, cdn, ls * a0, rs, roff, dnoff, n, r0, l);
   bool stopped = false;
   T xrun = (T)0;
   int L = a0;
   for (; L <= a1; ++L) {
     const long base = ls * L;
     if (L < a1)
-      sweep_load<R, EARLY>(nxt, F, Q, invd, na, bf, cdn, base + ls, rs, dnoff, n, r0, l);
+      sweep_load<R, DIR, EARLY>(nxt, F, Q, invd, na, bf, cdn, base + ls, rs, roff, dnoff, n, r0,
+                                l);
     T al[R], be[R], bb[R];
     T At = (T)0, Bt = (T)1;
 #pragma unroll
@@ -344,7 +383,7 @@ __device__ void sweep_chunk(const Case2D& cs, int comp, int ca, int slot, double
         const T x = y[q] - bb[q] * xin;
         xin = x;
         prev[q] = x;
-        const long c = base + rs * (r + 1);
+        const long c = base + roff + rs * (r + 1);
         if (add)
           F64[c] = F64[c] + (double)x;
         else
@@ -493,7 +532,7 @@ __global__ void __launch_bounds__(256) k2_tr_in32(const Case2D* __restrict__ tab
     const int i = i0 + li, j = j0 + lj;
     if (i <= ni && j <= nj) {
 #pragma unroll
-      for (int a = 0; a < 2; ++a) dst[a][j + (long)(nj + 1) * i] = tile[a][lj][li];
+      for (int a = 0; a < 2; ++a) dst[a][tidx32(nj, i, j)] = tile[a][lj][li];
     }
   }
 }
@@ -515,7 +554,7 @@ __global__ void __launch_bounds__(256) k2_tr_out32(const Case2D* __restrict__ ta
     const int i = i0 + li, j = j0 + lj;
     if (i <= ni && j <= nj) {
 #pragma unroll
-      for (int a = 0; a < 2; ++a) tile[a][lj][li] = src[a][j + (long)(nj + 1) * i];
+      for (int a = 0; a < 2; ++a) tile[a][lj][li] = src[a][tidx32(nj, i, j)];
     }
   }
   __syncthreads();
