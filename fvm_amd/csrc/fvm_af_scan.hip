@@ -43,6 +43,7 @@ constexpr int SC_R = 8;   // max rows per lane (n <= 512)
 struct ScanGeo {
   int n, na, nb;          // rows per line, lines per plane, planes
   long ls_a, ls_b;        // line-major strides of a and b (row stride 1)
+  long off;               // offset of row 0 in a line-major line
   long ns_p, ns_a, ns_b;  // natural strides of p, a, b
 };
 
@@ -50,11 +51,13 @@ __host__ __device__ inline ScanGeo scan_geo(const Dims& d, int S) {
   ScanGeo g;
   if (S == 0) {  // I-lines: p = i, a = j, b = k; line-major == natural
     g.n = d.ni - 1; g.na = d.nj - 1; g.nb = d.nk - 1;
-    g.ls_a = d.sj; g.ls_b = d.sk;
+    g.ls_a = d.sj; g.ls_b = d.sk; g.off = 0;
     g.ns_p = 1; g.ns_a = d.sj; g.ns_b = d.sk;
-  } else {       // J-lines: p = j, a = i, b = k; line-major index j + (NJ+1)(i + (NI+1) k)
+  } else {       // J-lines: p = j, a = i, b = k; line-major index 3 + j + L (i + (NI+1) k) with
+                 // L = scan_jline_stride(NJ) a multiple of four: row 1 of every line, and so a
+                 // lane's first row l R + 1 for R = 4, 8, on a 32-byte boundary (vector loads)
     g.n = d.nj - 1; g.na = d.ni - 1; g.nb = d.nk - 1;
-    g.ls_a = d.nj + 1; g.ls_b = (long)(d.nj + 1) * (d.ni + 1);
+    g.ls_a = scan_jline_stride(d.nj); g.ls_b = g.ls_a * (d.ni + 1); g.off = 3;
     g.ns_p = d.sj; g.ns_a = 1; g.ns_b = d.sk;
   }
   return g;
@@ -72,7 +75,7 @@ __global__ void k_scan_factor(Dims d, const double* const* __restrict__ fa,
   double bprev = 0.0;
   for (int p = 1; p <= g.n; ++p) {
     const long c = g.ns_p * p + g.ns_a * a + g.ns_b * b;
-    const long l = (long)p + g.ls_a * a + g.ls_b * b;
+    const long l = g.off + (long)p + g.ls_a * a + g.ls_b * b;
     const int i = S == 0 ? p : a;
     double ae = fa[0][c], aw = fa[0][c - 1], an = fa[1][c], as = fa[1][c - d.sj],
            at = fa[2][c], ab = fa[2][c - d.sk];
@@ -109,11 +112,11 @@ __global__ void __launch_bounds__(256) k_scan_q1v(Dims d, PtrSet8 qn, double* co
     if (i <= d.ni - 1 && j <= d.nj - 1) tile[lj][li] = src[cidx(d, i, j, k)];
   }
   __syncthreads();
-  const long lsa = d.nj + 1, lsb = (long)(d.nj + 1) * (d.ni + 1);
+  const ScanGeo g = scan_geo(d, 1);
   for (int r = 0; r < 4; ++r) {
     const int lj = t % 32, li = t / 32 + 8 * r;
     const int i = i0 + li, j = j0 + lj;
-    if (i <= d.ni - 1 && j <= d.nj - 1) dst[j + lsa * i + lsb * k] = tile[lj][li];
+    if (i <= d.ni - 1 && j <= d.nj - 1) dst[g.off + j + g.ls_a * i + g.ls_b * k] = tile[lj][li];
   }
 }
 
@@ -127,13 +130,13 @@ __global__ void __launch_bounds__(256) k_scan_unpack1(Dims d, const double* cons
   const int k = 1 + (int)(blockIdx.z % (d.nk - 1));
   const int comp = (int)(blockIdx.z / (d.nk - 1));
   const int t = threadIdx.x;
-  const long lsa = d.nj + 1, lsb = (long)(d.nj + 1) * (d.ni + 1);
+  const ScanGeo g = scan_geo(d, 1);
   const gptr<const T> src = glb(reinterpret_cast<const T*>(xo[comp]));
   const gptr<double> dst = glb(f[comp]);
   for (int r = 0; r < 4; ++r) {
     const int lj = t % 32, li = t / 32 + 8 * r;
     const int i = i0 + li, j = j0 + lj;
-    if (i <= d.ni - 1 && j <= d.nj - 1) tile[lj][li] = src[j + lsa * i + lsb * k];
+    if (i <= d.ni - 1 && j <= d.nj - 1) tile[lj][li] = src[g.off + j + g.ls_a * i + g.ls_b * k];
   }
   __syncthreads();
   for (int r = 0; r < 4; ++r) {
@@ -183,6 +186,35 @@ __global__ void k_scan_seed(Dims d, AFArgs A, int nchunk, T* bnd) {
     const int a = min((ca + 1) * SC_C, g.na);
     bnd[t] = std::is_same<T, float>::value ? (T)0
                                            : (T)A.f[comp][g.ns_p * p + g.ns_a * a + g.ns_b * b];
+  }
+}
+
+// R consecutive elements from an address aligned to 4 (R % 4 == 0) or 2 (R % 2 == 0) elements
+typedef float sc_f4 __attribute__((ext_vector_type(4)));
+typedef float sc_f2 __attribute__((ext_vector_type(2)));
+typedef double sc_d2 __attribute__((ext_vector_type(2)));
+template <int R>
+__device__ __forceinline__ void load_rows(float (&d)[R], gptr<const float> p) {
+  if constexpr (R % 4 == 0) {
+#pragma unroll
+    for (int k = 0; k < R / 4; ++k) {
+      const sc_f4 v = ((gptr<const sc_f4>)p)[k];
+      d[4 * k] = v.x; d[4 * k + 1] = v.y; d[4 * k + 2] = v.z; d[4 * k + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) {
+      const sc_f2 v = ((gptr<const sc_f2>)p)[k];
+      d[2 * k] = v.x; d[2 * k + 1] = v.y;
+    }
+  }
+}
+template <int R>
+__device__ __forceinline__ void load_rows(double (&d)[R], gptr<const double> p) {
+#pragma unroll
+  for (int k = 0; k < R / 2; ++k) {
+    const sc_d2 v = ((gptr<const sc_d2>)p)[k];
+    d[2 * k] = v.x; d[2 * k + 1] = v.y;
   }
 }
 
@@ -258,12 +290,29 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const T* __r
     // waits only for the ones it needs, so the next line's loads overlap this line's scans.
     T be[R], iv[R], bv[R], cm[R], qv[R], xv[R], y0n, xnn;
     auto load = [&](int a) {
-      const long lb = g.ls_a * a + g.ls_b * b;
+      const long lb = g.ls_a * a + g.ls_b * b + g.off;
+      if constexpr (S == 1 && R % 2 == 0) {
+        // J lines: a lane's rows p0 .. p0+R-1 as vectors (lb + p0 is a multiple of 4 for R = 4,
+        // 8 and of 2 for R = 6); rows past n lie in the padded line (or the next one, or the
+        // slack after the last) and are never kept
+        const long o = lb + p0;
+        load_rows<R>(be, glb(beta) + o);
+        load_rows<R>(iv, glb(invd) + o);
+        load_rows<R>(bv, glb(bb) + o);
+        load_rows<R>(cm, glb(cam) + o);
+        load_rows<R>(qv, qc + o);
+        if (EARLY) load_rows<R>(xv, (gptr<const T>)xc + o);
+        else {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const long o = lb + min(p0 + r, g.n);
-        be[r] = beta[o]; iv[r] = invd[o]; bv[r] = bb[o]; cm[r] = cam[o]; qv[r] = qc[o];
-        xv[r] = EARLY ? xc[o] : (T)0;  // the previous pass's value of the line
+          for (int r = 0; r < R; ++r) xv[r] = (T)0;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const long o = lb + min(p0 + r, g.n);
+          be[r] = beta[o]; iv[r] = invd[o]; bv[r] = bb[o]; cm[r] = cam[o]; qv[r] = qc[o];
+          xv[r] = EARLY ? xc[o] : (T)0;  // the previous pass's value of the line
+        }
       }
       if (DELTA) {
         y0n = (T)0;
@@ -279,7 +328,7 @@ __global__ void __launch_bounds__(64) k_scan_pass(Dims d, AFArgs A, const T* __r
     T xrun = (T)0;  // largest |x| of the lines this wavefront solved in this pass
     const T ttol = (T)tol;
     for (int a = a0; a <= a1; ++a) {
-      const long lb = g.ls_a * a + g.ls_b * b;
+      const long lb = g.ls_a * a + g.ls_b * b + g.off;
       const T y0 = y0n, xn = xnn;
       // forward: y(p) = al(p) + be(p) y(p-1)
       T al[R], bt[R], bw[R], xo[R];
@@ -486,6 +535,8 @@ int fvm_af_scan_setup(FvmCtx* c) {
   const Dims d = c->d;
   hipStream_t st = c->stream;
   long nbnd = 0;
+  // the factor arrays hold the natural (I) and the padded J-line-major (J) layouts
+  const long ne = std::max((long)d.n, scan_jline_elems(d.ni, d.nj, d.nk));
   for (int s = 0; s < 2; ++s) {
     const ScanGeo g = scan_geo(d, s);
     c->af_scan[s] = c->af_b_ratio[s] <= 1e-15 && g.n <= 64 * SC_R;
@@ -498,7 +549,7 @@ int fvm_af_scan_setup(FvmCtx* c) {
       const double* a0 = set ? c->ap0t : c->ap0;
       if (!L[0]) {
         for (int q = 0; q < 4; ++q) {
-          HIP_CHECK(hipMalloc((void**)&L[q], d.n * sizeof(double)));
+          HIP_CHECK(hipMalloc((void**)&L[q], ne * sizeof(double)));
           c->allocs.push_back(L[q]);
         }
       }
@@ -513,10 +564,10 @@ int fvm_af_scan_setup(FvmCtx* c) {
     if (c->prm.af_mode == 2) {  // FP32 copies of the momentum factors (correction form)
       for (int q = 0; q < 4; ++q) {
         if (!c->lf32[s][q]) {
-          HIP_CHECK(hipMalloc((void**)&c->lf32[s][q], d.n * sizeof(float)));
+          HIP_CHECK(hipMalloc((void**)&c->lf32[s][q], ne * sizeof(float)));
           c->allocs.push_back(c->lf32[s][q]);
         }
-        hipLaunchKernelGGL(k_to_float, grid1(d.n, 256), dim3(256), 0, st, d.n, c->lf[s][q],
+        hipLaunchKernelGGL(k_to_float, grid1(ne, 256), dim3(256), 0, st, ne, c->lf[s][q],
                            c->lf32[s][q]);
       }
       HIP_CHECK(hipGetLastError());

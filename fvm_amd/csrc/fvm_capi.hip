@@ -102,7 +102,9 @@ int fvm_thermal_setup(FvmCtx* c) {
     int r = 0;
     double** arr[] = {&c->t, &c->tn, &c->adv, &c->adv1, &c->dift, &c->sgst, &c->that,
                       &c->dtd[0], &c->dtd[1], &c->dtd[2], &c->kappa, &c->magh, &c->ap0t};
-    for (double** q : arr) r |= dalloc(c, q, n);
+    // T, like u, v, w, may pass through the AF work arrays (padded J-line-major size)
+    const long nt = std::max((long)n, scan_jline_elems(d.ni, d.nj, d.nk));
+    for (double** q : arr) r |= dalloc(c, q, q == &c->t ? nt : n);
     r |= dalloc(c, &c->tfar, (long)(d.nj + 1) * (d.nk + 1));
     for (int s = 0; s < 2; ++s) {
       if (hipMalloc((void**)&c->scantab_t[s], 9 * sizeof(double*)) != hipSuccess) r = FVM_ERR_HIP;
@@ -305,7 +307,11 @@ static int create_impl(const fvm_params* prm, const double* xg, const double* yg
                       &c->jq33, &c->jq12, &c->jq13, &c->jq23, &c->theta, &c->filt2, &c->ap0,
                       &c->pdI, &c->pdJ, &c->pdK, &c->u, &c->v, &c->w, &c->p, &c->un, &c->vn,
                       &c->wn, &c->rhs, &c->psu, &c->mag, &c->nut};
-  for (double** q : cells) r |= dalloc(c, q, n);
+  // the general chunked AF stages rotate u, v, w with the AF work arrays, which also hold the
+  // J-line scans' padded line-major copies: all of them get the larger size
+  const long na = std::max((long)n, scan_jline_elems(c->d.ni, c->d.nj, c->d.nk));
+  for (double** q : cells)
+    r |= dalloc(c, q, (q == &c->u || q == &c->v || q == &c->w) ? na : n);
   for (int a = 0; a < 9; ++a) r |= dalloc(c, &c->m[a], n);
   for (int s = 0; s < 3; ++s)
     for (int q = 0; q < 3; ++q) r |= dalloc(c, &c->fa[s][q], n);
@@ -318,8 +324,8 @@ static int create_impl(const fvm_params* prm, const double* xg, const double* yg
     r |= dalloc(c, &c->ju[q], n);
     r |= dalloc(c, &c->uh[q], n);
     r |= dalloc(c, &c->sgs[q], n);
-    r |= dalloc(c, &c->afb[q][0], n);
-    r |= dalloc(c, &c->afb[q][1], n);
+    r |= dalloc(c, &c->afb[q][0], na);
+    r |= dalloc(c, &c->afb[q][1], na);
   }
   for (int q = 0; q < 6; ++q) r |= dalloc(c, &c->lss[q], n);
   r |= dalloc(c, &c->ufar, 3L * (prm->nj + 1) * (prm->nk + 1));

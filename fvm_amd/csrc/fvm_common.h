@@ -47,6 +47,14 @@ typedef const __attribute__((address_space(1))) double* GP;
 typedef __attribute__((address_space(1))) double* GPW;
 __device__ inline GP gp(const double* p) { return (GP)p; }
 __device__ inline GPW gpw(double* p) { return (GPW)p; }
+// Line stride of the J-line-major copies of the line scans (a multiple of four elements, row 0
+// at offset 3) and the element count that holds them with 128 elements of slack for the last
+// line's vector loads past row NJ-1
+__host__ __device__ inline long scan_jline_stride(int nj) { return ((long)nj + 1 + 3 + 3) & ~3L; }
+__host__ __device__ inline long scan_jline_elems(int ni, int nj, int nk) {
+  return scan_jline_stride(nj) * (ni + 1) * (nk + 1) + 128;
+}
+
 // the same for any element type: glb(p) is p in the global address space
 template <typename T>
 using gptr = __attribute__((address_space(1))) T*;
