@@ -121,26 +121,29 @@ __global__ void __launch_bounds__(256) k_scan_q1v(Dims d, PtrSet8 qn, double* co
 }
 
 // converged J-line iterate (line-major) back into the field interior; T = float: the iterate is
-// the stage's change, added to the field
+// the stage's change, added to the field.  A workgroup moves a 64 (I) x 32 (J) tile of one plane
+// through LDS: 32 contiguous J rows of a line in, 64 contiguous I cells of a field row out.
 template <typename T>
 __global__ void __launch_bounds__(256) k_scan_unpack1(Dims d, const double* const* xo,
                                                       double* const* f) {
-  __shared__ T tile[32][33];
-  const int i0 = 1 + blockIdx.x * 32, j0 = 1 + blockIdx.y * 32;
+  __shared__ T tile[32][65];
+  const int i0 = 1 + blockIdx.x * 64, j0 = 1 + blockIdx.y * 32;
   const int k = 1 + (int)(blockIdx.z % (d.nk - 1));
   const int comp = (int)(blockIdx.z / (d.nk - 1));
   const int t = threadIdx.x;
   const ScanGeo g = scan_geo(d, 1);
   const gptr<const T> src = glb(reinterpret_cast<const T*>(xo[comp]));
   const gptr<double> dst = glb(f[comp]);
-  for (int r = 0; r < 4; ++r) {
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
     const int lj = t % 32, li = t / 32 + 8 * r;
     const int i = i0 + li, j = j0 + lj;
     if (i <= d.ni - 1 && j <= d.nj - 1) tile[lj][li] = src[g.off + j + g.ls_a * i + g.ls_b * k];
   }
   __syncthreads();
-  for (int r = 0; r < 4; ++r) {
-    const int li = t % 32, lj = t / 32 + 8 * r;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int li = t % 64, lj = t / 64 + 4 * r;
     const int i = i0 + li, j = j0 + lj;
     if (i <= d.ni - 1 && j <= d.nj - 1) {
       const long c = cidx(d, i, j, k);
@@ -150,16 +153,29 @@ __global__ void __launch_bounds__(256) k_scan_unpack1(Dims d, const double* cons
   }
 }
 
-// FP32 correction form of the I stage: the converged change (natural layout) added to the field
+// FP32 correction form of the I stage: the converged change (natural layout) added to the field;
+// a thread adds the change of four K planes of one cell column, loads first
+constexpr int ADD0_KP = 4;
 __global__ void __launch_bounds__(256) k_scan_add0(Dims d, PtrSet8 dl, PtrSet8 f) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + (int)(blockIdx.z % (d.nk - 1));
-  const int comp = (int)(blockIdx.z / (d.nk - 1));
+  const int nkb = (d.nk - 1 + ADD0_KP - 1) / ADD0_KP;
+  const int k0 = 1 + (int)(blockIdx.z % nkb) * ADD0_KP;
+  const int comp = (int)(blockIdx.z / nkb);
   if (i > d.ni - 1 || j > d.nj - 1) return;
-  const long c = cidx(d, i, j, k);
-  const float* __restrict__ x = reinterpret_cast<const float*>(dl.p[comp]);
-  f.p[comp][c] = f.p[comp][c] + (double)x[c];
+  const gptr<const float> x = glb(reinterpret_cast<const float*>(dl.p[comp]));
+  const gptr<double> y = glb(f.p[comp]);
+  double a[ADD0_KP];
+  float b[ADD0_KP];
+#pragma unroll
+  for (int q = 0; q < ADD0_KP; ++q) {
+    const long c = cidx(d, i, j, min(k0 + q, d.nk - 1));
+    a[q] = y[c];
+    b[q] = x[c];
+  }
+#pragma unroll
+  for (int q = 0; q < ADD0_KP; ++q)
+    if (k0 + q <= d.nk - 1) y[cidx(d, i, j, k0 + q)] = a[q] + (double)b[q];
 }
 
 __global__ void k_to_float(long n, const double* __restrict__ a, float* b) {
@@ -512,14 +528,15 @@ int scan_stage(FvmCtx* c, const AFArgs& A, int* iters, bool temp) {
   }
   if (S == 1) {
     ProfScope pk(c, "af_scan_unpack");
-    hipLaunchKernelGGL(k_scan_unpack1<T>, dim3((d.ni - 1 + 31) / 32, (d.nj - 1 + 31) / 32, nc * (d.nk - 1)),
+    hipLaunchKernelGGL(k_scan_unpack1<T>, dim3((d.ni - 1 + 63) / 64, (d.nj - 1 + 31) / 32, nc * (d.nk - 1)),
                        dim3(256), 0, st, d, (const double* const*)dxo, dtab + 6);
     HIP_CHECK(hipGetLastError());
   } else if (DELTA) {
     ProfScope pk(c, "af_scan_add");
     PtrSet8 dl{}, f{};
     for (int k = 0; k < nc; ++k) { dl.p[k] = xo[k]; f.p[k] = A.f[k]; }
-    hipLaunchKernelGGL(k_scan_add0, dim3((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, nc * (d.nk - 1)),
+    const int nkb = (d.nk - 1 + ADD0_KP - 1) / ADD0_KP;
+    hipLaunchKernelGGL(k_scan_add0, dim3((d.ni - 1 + 63) / 64, (d.nj - 1 + 3) / 4, nc * nkb),
                        dim3(64, 4, 1), 0, st, d, dl, f);
     HIP_CHECK(hipGetLastError());
   }
