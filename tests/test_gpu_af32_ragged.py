@@ -3,7 +3,7 @@
 70 x 96 x 10 interior cells: I rows not a multiple of the 64-cell tiles of the J-line unpack
 and of the I-stage add, K planes not a multiple of the add's four planes per thread, lines of
 two rows per lane (the vector row loads of the J-line pass).  From a seeded start, the predictor
-(ITER_U after the explicit terms, RLES/main.f90:66-90) of the change form equals the FP64
+(CONTRA_VEL to ITER_U, RLES/main.f90:48-88) of the change form equals the FP64
 sweeps' to the FP32 resolution of the change the stages make.
 """
 import numpy as np
